@@ -1,0 +1,73 @@
+"""ctypes binding of libdft_hip.so (declarations: include/dft_hip.h).
+
+The library is the only compute path: if it is missing or fails to load,
+every entry point raises. There is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("DFT_HIP_LIB", os.path.join(_HERE, "libdft_hip.so"))
+
+DFT_OK, DFT_EINVAL, DFT_EUNSUPPORTED, DFT_EHIP = 0, -1, -2, -3
+DFT_PLAN_UHF = 1
+DFT_FP64, DFT_MIXED = 0, 1
+MODES = {"fp64": DFT_FP64, "mixed": DFT_MIXED}
+
+_i32p = C.POINTER(C.c_int32)
+_i64p = C.POINTER(C.c_int64)
+_dp = C.POINTER(C.c_double)
+_vp = C.c_void_p
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "dft_version": (C.c_int, []),
+    "dft_last_error": (C.c_char_p, []),
+    "dft_plan_create": (C.c_int, [C.c_int, _i32p, _i32p, _i32p, _dp, _dp, _dp, C.c_int, _dp, _dp,
+                                  C.c_int, C.c_double, C.c_double, C.c_int, C.POINTER(_vp)]),
+    "dft_plan_destroy": (C.c_int, [_vp]),
+    "dft_plan_stats": (C.c_int, [_vp, _i64p, C.c_int, C.POINTER(C.c_int), _i64p, _i64p]),
+    "dft_schwarz": (C.c_int, [_vp, _dp]),
+    "dft_jk": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int, _vp]),
+    "dft_eri_quartets": (C.c_int, [_vp, _i32p, C.c_int64, _dp]),
+    "dft_oneelec": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "dft_boys": (C.c_int, [_vp, C.c_int, C.c_int, _vp, C.c_int, _vp]),
+    "dft_plan_profile": (C.c_int, [_vp, C.c_int]),
+    "dft_plan_profile_read": (C.c_int, [_vp, _dp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+}
+
+_lib = None
+
+
+class DFTError(RuntimeError):
+    pass
+
+
+def load() -> C.CDLL:
+    """Load the library once; raise if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise DFTError(f"{LIB_PATH} is not built: run `python -m dft_amd.build` "
+                       "(there is no CPU fallback)")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc: int) -> None:
+    if rc != DFT_OK:
+        msg = load().dft_last_error().decode(errors="replace")
+        raise DFTError(f"libdft_hip error {rc}: {msg}")
+
+
+def ptr(arr, ctype):
+    """ctypes pointer into a contiguous NumPy array of the matching dtype."""
+    return arr.ctypes.data_as(C.POINTER(ctype))

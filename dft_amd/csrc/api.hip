@@ -1,0 +1,632 @@
+// C ABI of libdft_hip.so: plan construction (pair pre-pass, Schwarz bounds,
+// class-sorted screened quartet lists, reduction map) and the per-iteration
+// entry points. Declarations and conventions: include/dft_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/dft_hip.h"
+#include "kernels.h"
+#include "md_device.h"
+
+using namespace dft;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                          \
+  do {                                                                                         \
+    hipError_t e_ = (expr);                                                                    \
+    if (e_ != hipSuccess) return fail(DFT_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// ---------------------------------------------------------------- Boys
+// F_m(T_k) = e^{-T} sum_i (2T)^i / ((2m+1)(2m+3)...(2m+2i+1)), a series of
+// positive terms, summed in extended precision.
+std::vector<double> boys_table_host() {
+  std::vector<double> tab((size_t)BOYS_NG * BOYS_MT);
+  for (int k = 0; k < BOYS_NG; ++k) {
+    const long double T = (long double)k / 10.0L;
+    const long double e = expl(-T);
+    for (int m = 0; m < BOYS_MT; ++m) {
+      long double term = 1.0L / (2 * m + 1), sum = term;
+      for (int i = 1; i < 2000; ++i) {
+        term *= 2.0L * T / (2 * m + 2 * i + 1);
+        sum += term;
+        if (term < sum * 1e-21L) break;
+      }
+      tab[(size_t)k * BOYS_MT + m] = (double)(e * sum);
+    }
+  }
+  return tab;
+}
+
+std::mutex g_boys_mu;
+std::map<int, double*> g_boys_dev;
+
+int boys_table_device(double** out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_boys_mu);
+  auto it = g_boys_dev.find(dev);
+  if (it != g_boys_dev.end()) {
+    *out = it->second;
+    return DFT_OK;
+  }
+  static const std::vector<double> host = boys_table_host();
+  double* d = nullptr;
+  HIP_TRY(hipMalloc(&d, host.size() * sizeof(double)));
+  HIP_TRY(hipMemcpy(d, host.data(), host.size() * sizeof(double), hipMemcpyHostToDevice));
+  g_boys_dev[dev] = d;
+  *out = d;
+  return DFT_OK;
+}
+
+// Radial part of the primitive normalisation of shelled.py:95-101:
+// (2a/pi)^{3/4} (4a)^{l/2}; the angular part 1/sqrt(prod (2l_d-1)!!) is
+// applied per axis in the pair records.
+double radial_norm(double a, int l) { return std::pow(2.0 * a / M_PI, 0.75) * std::pow(4.0 * a, 0.5 * l); }
+
+template <typename T>
+int upload(T** d, const std::vector<T>& h) {
+  *d = nullptr;
+  if (h.empty()) return DFT_OK;
+  HIP_TRY(hipMalloc(d, h.size() * sizeof(T)));
+  HIP_TRY(hipMemcpy(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  return DFT_OK;
+}
+
+struct Combo {
+  int pcb = 0, pck = 0;
+  std::vector<int2> q64, q32;  // (bra pair, ket pair) in plan pair ids
+  int2* d_q64 = nullptr;
+  int2* d_q32 = nullptr;
+  long long off64 = 0, off32 = 0;  // staging offsets (doubles)
+  long long prim64 = 0, prim32 = 0;
+  int recsz = 0;
+};
+
+}  // namespace
+
+struct dft_plan_s {
+  int nsh = 0, nbf = 0, natom = 0, nk = 1;
+  double tau = 0, tau32 = 0;
+  std::vector<ShellDev> sh;      // internal (l-sorted) order
+  std::vector<int> in2int;       // input shell -> internal shell
+  std::vector<double> exps, coefs;
+  std::vector<int4> pairs;       // (sh_a, sh_b, rec_off, nprim_pairs), class-sorted
+  std::vector<int> pair_pc;      // pair class per pair
+  std::vector<int> pair_of;      // internal (i,j), i >= j -> pair id, i*nsh + j
+  std::vector<double> Q;         // Schwarz per pair
+  std::vector<Combo> combos;
+  long long n_unscreened = 0;
+  long long stage_len = 0;       // doubles
+  int ntarget = 0;
+  // device
+  ShellDev* d_sh = nullptr;
+  double* d_exps = nullptr;
+  double* d_coefs = nullptr;
+  int4* d_pairs = nullptr;
+  int* d_shell_ao = nullptr;
+  double* d_rec64 = nullptr;
+  float* d_rec32 = nullptr;
+  double* d_stage = nullptr;
+  int4* d_tg = nullptr;
+  long long* d_tb = nullptr;
+  long long* d_contrib = nullptr;
+  double* d_half = nullptr;  // (1 + nk) * nbf * nbf
+  double* d_atoms = nullptr;  // x, y, z, Z
+  int2* d_shpairs = nullptr;
+  int nshpairs = 0;
+  double* d_boys = nullptr;
+  // optional per-launch timing: slot 2c = combo c FP64 part, 2c+1 = its weak
+  // part, then reduce, symmetrize; one begin/end event pair per slot and step
+  int prof_steps = 0, prof_used = 0;
+  std::vector<hipEvent_t> prof_ev;
+};
+
+namespace {
+
+void free_plan(dft_plan p) {
+  if (!p) return;
+  for (auto& c : p->combos) {
+    (void)hipFree(c.d_q64);
+    (void)hipFree(c.d_q32);
+  }
+  void* ptrs[] = {p->d_sh, p->d_exps, p->d_coefs, p->d_pairs, p->d_shell_ao, p->d_rec64, p->d_rec32,
+                  p->d_stage, p->d_tg, p->d_tb, p->d_contrib, p->d_half, p->d_atoms, p->d_shpairs};
+  for (void* x : ptrs) (void)hipFree(x);
+  for (hipEvent_t e : p->prof_ev) (void)hipEventDestroy(e);
+  delete p;
+}
+
+int ncart_h(int l) { return (l + 1) * (l + 2) / 2; }
+
+// Raw contracted blocks for a list of (bra pair, ket pair) in one combo,
+// synchronous, host result.
+int raw_blocks(dft_plan p, int combo, const std::vector<int2>& q, std::vector<double>& out, int blk) {
+  out.assign(q.size() * (size_t)blk, 0.0);
+  if (q.empty()) return DFT_OK;
+  int2* dq = nullptr;
+  double* dout = nullptr;
+  int rc = upload(&dq, q);
+  if (rc) return rc;
+  hipError_t e = hipMalloc(&dout, out.size() * sizeof(double));
+  if (e != hipSuccess) {
+    (void)hipFree(dq);
+    return fail(DFT_EHIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+  }
+  e = launch_eri_raw(combo, dq, (long long)q.size(), p->d_pairs, p->d_rec64, dout, p->d_boys, nullptr);
+  if (e == hipSuccess) e = hipMemcpy(out.data(), dout, out.size() * sizeof(double), hipMemcpyDeviceToHost);
+  (void)hipFree(dq);
+  (void)hipFree(dout);
+  if (e != hipSuccess) return fail(DFT_EHIP, std::string("raw ERI blocks: ") + hipGetErrorString(e));
+  return DFT_OK;
+}
+
+int build_plan(dft_plan p, int nsh, const int32_t* L, const int32_t* NP, const int32_t* AO,
+               const double* XYZ, const double* EX, const double* CO, int natom, const double* AXYZ,
+               const double* AZ) {
+  // ---- shells in l-sorted (stable) internal order
+  std::vector<int> order(nsh);
+  for (int i = 0; i < nsh; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return L[a] < L[b]; });
+  std::vector<int> prim_in(nsh + 1, 0);
+  for (int i = 0; i < nsh; ++i) prim_in[i + 1] = prim_in[i] + NP[i];
+  p->in2int.assign(nsh, 0);
+  p->sh.resize(nsh);
+  for (int k = 0; k < nsh; ++k) {
+    const int i = order[k];
+    p->in2int[i] = k;
+    ShellDev s;
+    s.l = L[i];
+    s.nprim = NP[i];
+    s.prim_off = (int)p->exps.size();
+    s.ao = AO[i];
+    s.x = XYZ[3 * i];
+    s.y = XYZ[3 * i + 1];
+    s.z = XYZ[3 * i + 2];
+    for (int j = 0; j < NP[i]; ++j) {
+      const double a = EX[prim_in[i] + j];
+      p->exps.push_back(a);
+      p->coefs.push_back(CO[prim_in[i] + j] * radial_norm(a, s.l));
+    }
+    p->sh[k] = s;
+  }
+  // ---- pairs, sorted by pair class, then (i, j)
+  std::vector<std::vector<int2>> by_pc(NPC);
+  for (int i = 0; i < nsh; ++i)
+    for (int j = 0; j <= i; ++j) by_pc[pc_index(p->sh[i].l, p->sh[j].l)].push_back(make_int2(i, j));
+  p->pair_of.assign((size_t)nsh * nsh, -1);
+  std::vector<int4> pp;       // primitive pairs: sh_a, sh_b, ia, ib
+  std::vector<int> pp_off;    // record offsets
+  long long roff = 0;
+  for (int pc = 0; pc < NPC; ++pc) {
+    int la, lb;
+    pc_moments(pc, la, lb);
+    const int rs = rec_size(la, lb);
+    for (const int2& ij : by_pc[pc]) {
+      const ShellDev &A = p->sh[ij.x], &B = p->sh[ij.y];
+      const int id = (int)p->pairs.size();
+      p->pairs.push_back(make_int4(ij.x, ij.y, (int)roff, A.nprim * B.nprim));
+      p->pair_pc.push_back(pc);
+      p->pair_of[(size_t)ij.x * nsh + ij.y] = id;
+      for (int ia = 0; ia < A.nprim; ++ia)
+        for (int ib = 0; ib < B.nprim; ++ib) {
+          pp.push_back(make_int4(ij.x, ij.y, ia, ib));
+          pp_off.push_back((int)roff);
+          roff += rs;
+        }
+    }
+  }
+  if (roff > 0x7fffffffLL) return fail(DFT_EINVAL, "pair records exceed 2^31 reals");
+  // ---- device data + pair pre-pass
+  int rc;
+  if ((rc = boys_table_device(&p->d_boys))) return rc;
+  if ((rc = upload(&p->d_sh, p->sh))) return rc;
+  if ((rc = upload(&p->d_exps, p->exps))) return rc;
+  if ((rc = upload(&p->d_coefs, p->coefs))) return rc;
+  if ((rc = upload(&p->d_pairs, p->pairs))) return rc;
+  std::vector<int> ao(nsh);
+  for (int k = 0; k < nsh; ++k) ao[k] = p->sh[k].ao;
+  if ((rc = upload(&p->d_shell_ao, ao))) return rc;
+  {
+    int4* dpp = nullptr;
+    int* doff = nullptr;
+    if ((rc = upload(&dpp, pp))) return rc;
+    if ((rc = upload(&doff, pp_off))) return rc;
+    HIP_TRY(hipMalloc(&p->d_rec64, std::max<long long>(roff, 1) * sizeof(double)));
+    HIP_TRY(hipMalloc(&p->d_rec32, std::max<long long>(roff, 1) * sizeof(float)));
+    hipError_t e = launch_pair_records(dpp, (int)pp.size(), doff, p->d_sh, p->d_exps, p->d_coefs, p->d_rec64,
+                                       p->d_rec32, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    (void)hipFree(dpp);
+    (void)hipFree(doff);
+    if (e != hipSuccess) return fail(DFT_EHIP, std::string("pair pre-pass: ") + hipGetErrorString(e));
+  }
+  // ---- atoms and shell pairs for the 1-e kernel
+  std::vector<double> at(4 * (size_t)natom);
+  for (int a = 0; a < natom; ++a) {
+    at[4 * a] = AXYZ[3 * a];
+    at[4 * a + 1] = AXYZ[3 * a + 1];
+    at[4 * a + 2] = AXYZ[3 * a + 2];
+    at[4 * a + 3] = AZ[a];
+  }
+  if ((rc = upload(&p->d_atoms, at))) return rc;
+  std::vector<int2> sp;
+  for (int i = 0; i < nsh; ++i)
+    for (int j = 0; j <= i; ++j) sp.push_back(make_int2(i, j));
+  p->nshpairs = (int)sp.size();
+  if ((rc = upload(&p->d_shpairs, sp))) return rc;
+  // ---- Schwarz: diagonal quartets (ab|ab)
+  const int npair = (int)p->pairs.size();
+  p->Q.assign(npair, 0.0);
+  for (int pc = 0; pc < NPC; ++pc) {
+    int la, lb;
+    pc_moments(pc, la, lb);
+    const int nab = ncart_h(la) * ncart_h(lb);
+    std::vector<int2> q;
+    std::vector<int> ids;
+    for (int id = 0; id < npair; ++id)
+      if (p->pair_pc[id] == pc) {
+        q.push_back(make_int2(id, id));
+        ids.push_back(id);
+      }
+    std::vector<double> blk;
+    if ((rc = raw_blocks(p, combo_index(pc, pc), q, blk, nab * nab))) return rc;
+    for (size_t k = 0; k < ids.size(); ++k) {
+      double m = 0.0;
+      for (int x = 0; x < nab; ++x) m = std::max(m, std::fabs(blk[k * nab * nab + (size_t)x * nab + x]));
+      p->Q[ids[k]] = std::sqrt(m);
+    }
+  }
+  // ---- screened, class-sorted quartet lists
+  p->combos.assign(NCOMBO, Combo());
+  std::vector<std::vector<int>> class_pairs(NPC);
+  for (int id = 0; id < npair; ++id) class_pairs[p->pair_pc[id]].push_back(id);
+  long long cur = 0;
+  for (int pcb = 0; pcb < NPC; ++pcb)
+    for (int pck = 0; pck <= pcb; ++pck) {
+      Combo& c = p->combos[combo_index(pcb, pck)];
+      c.pcb = pcb;
+      c.pck = pck;
+      int la, lb, lc, ld;
+      pc_moments(pcb, la, lb);
+      pc_moments(pck, lc, ld);
+      const int na = ncart_h(la), nb = ncart_h(lb), ncc = ncart_h(lc), nd = ncart_h(ld);
+      c.recsz = na * nb + ncc * nd + p->nk * (na + nb) * (ncc + nd);
+      for (int ib : class_pairs[pcb]) {
+        for (int ik : class_pairs[pck]) {
+          if (pcb == pck && ik > ib) break;
+          ++p->n_unscreened;
+          const double b = p->Q[ib] * p->Q[ik];
+          if (b < p->tau) continue;
+          const long long np = (long long)p->pairs[ib].w * p->pairs[ik].w;
+          if (b < p->tau32) {
+            c.q32.push_back(make_int2(ib, ik));
+            c.prim32 += np;
+          } else {
+            c.q64.push_back(make_int2(ib, ik));
+            c.prim64 += np;
+          }
+        }
+      }
+      c.off64 = cur;
+      cur += (long long)c.q64.size() * c.recsz;
+      c.off32 = cur;
+      cur += (long long)c.q32.size() * c.recsz;
+      if ((rc = upload(&c.d_q64, c.q64))) return rc;
+      if ((rc = upload(&c.d_q32, c.q32))) return rc;
+    }
+  p->stage_len = cur;
+  // ---- reduction map: ordered shell pairs (X, Y) of each half matrix
+  const int nmat = 1 + p->nk;
+  p->ntarget = nmat * nsh * nsh;
+  std::vector<long long> cnt((size_t)p->ntarget + 1, 0);
+  auto tid = [&](int m, int x, int y) { return ((size_t)m * nsh + x) * nsh + y; };
+  auto visit = [&](auto&& emit) {
+    for (const Combo& c : p->combos) {
+      int la, lb, lc, ld;
+      pc_moments(c.pcb, la, lb);
+      pc_moments(c.pck, lc, ld);
+      const int na = ncart_h(la), nb = ncart_h(lb), ncc = ncart_h(lc), nd = ncart_h(ld);
+      for (int part = 0; part < 2; ++part) {
+        const std::vector<int2>& ql = part ? c.q32 : c.q64;
+        const long long base0 = part ? c.off32 : c.off64;
+        for (size_t q = 0; q < ql.size(); ++q) {
+          const int4 pb = p->pairs[ql[q].x], pk = p->pairs[ql[q].y];
+          const int A = pb.x, B = pb.y, C = pk.x, D = pk.y;
+          long long o = base0 + (long long)q * c.recsz;
+          emit(tid(0, A, B), o);
+          o += na * nb;
+          emit(tid(0, C, D), o);
+          o += ncc * nd;
+          for (int k = 0; k < p->nk; ++k) {
+            emit(tid(1 + k, A, C), o);
+            o += na * ncc;
+            emit(tid(1 + k, A, D), o);
+            o += na * nd;
+            emit(tid(1 + k, B, C), o);
+            o += nb * ncc;
+            emit(tid(1 + k, B, D), o);
+            o += nb * nd;
+          }
+        }
+      }
+    }
+  };
+  visit([&](size_t t, long long) { ++cnt[t + 1]; });
+  for (int t = 0; t < p->ntarget; ++t) cnt[t + 1] += cnt[t];
+  std::vector<long long> contrib((size_t)cnt[p->ntarget]);
+  std::vector<long long> fillp(cnt.begin(), cnt.end() - 1);
+  visit([&](size_t t, long long o) { contrib[fillp[t]++] = o; });
+  std::vector<int4> tg(p->ntarget);
+  for (int m = 0; m < nmat; ++m)
+    for (int x = 0; x < nsh; ++x)
+      for (int y = 0; y < nsh; ++y)
+        tg[tid(m, x, y)] = make_int4(m * p->nbf + p->sh[x].ao, p->sh[y].ao, ncart_h(p->sh[x].l),
+                                     ncart_h(p->sh[y].l));
+  if ((rc = upload(&p->d_tg, tg))) return rc;
+  if ((rc = upload(&p->d_tb, cnt))) return rc;
+  if ((rc = upload(&p->d_contrib, contrib))) return rc;
+  HIP_TRY(hipMalloc(&p->d_stage, std::max<long long>(p->stage_len, 1) * sizeof(double)));
+  HIP_TRY(hipMalloc(&p->d_half, (size_t)nmat * p->nbf * p->nbf * sizeof(double)));
+  return DFT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dft_version(void) { return 100; }
+
+const char* dft_last_error(void) { return g_err.c_str(); }
+
+int dft_plan_create(int nsh, const int32_t* shell_l, const int32_t* shell_nprim, const int32_t* shell_ao,
+                    const double* shell_xyz, const double* prim_exp, const double* prim_coef, int natom,
+                    const double* atom_xyz, const double* atom_charge, int nbf, double schwarz_tau,
+                    double fp32_tau, int flags, dft_plan* out) {
+  if (!out) return fail(DFT_EINVAL, "out is NULL");
+  *out = nullptr;
+  if (nsh <= 0 || nbf <= 0 || natom < 0) return fail(DFT_EINVAL, "nsh, nbf must be positive, natom >= 0");
+  if (!shell_l || !shell_nprim || !shell_ao || !shell_xyz || !prim_exp || !prim_coef)
+    return fail(DFT_EINVAL, "NULL shell array");
+  if (natom > 0 && (!atom_xyz || !atom_charge)) return fail(DFT_EINVAL, "NULL atom array");
+  if (!(schwarz_tau >= 0.0) || !(fp32_tau >= 0.0)) return fail(DFT_EINVAL, "thresholds must be >= 0");
+  for (int i = 0; i < nsh; ++i) {
+    if (shell_l[i] < 0 || shell_l[i] > LMAX)
+      return fail(DFT_EUNSUPPORTED, "shell " + std::to_string(i) + ": angular momentum " +
+                                        std::to_string(shell_l[i]) + " not built (max " + std::to_string(LMAX) + ")");
+    if (shell_nprim[i] <= 0) return fail(DFT_EINVAL, "shell " + std::to_string(i) + ": nprim <= 0");
+    if (shell_ao[i] < 0 || shell_ao[i] + ncart_h(shell_l[i]) > nbf)
+      return fail(DFT_EINVAL, "shell " + std::to_string(i) + ": AO range outside [0, nbf)");
+  }
+  dft_plan p = new dft_plan_s();
+  p->nsh = nsh;
+  p->nbf = nbf;
+  p->natom = natom;
+  p->nk = (flags & DFT_PLAN_UHF) ? 2 : 1;
+  p->tau = schwarz_tau;
+  p->tau32 = fp32_tau;
+  int rc = build_plan(p, nsh, shell_l, shell_nprim, shell_ao, shell_xyz, prim_exp, prim_coef, natom, atom_xyz,
+                      atom_charge);
+  if (rc) {
+    free_plan(p);
+    return rc;
+  }
+  *out = p;
+  return DFT_OK;
+}
+
+int dft_plan_destroy(dft_plan plan) {
+  free_plan(plan);
+  return DFT_OK;
+}
+
+int dft_plan_stats(dft_plan p, int64_t* counts, int ncombo, int* ncombo_out, int64_t* n_unscreened,
+                   int64_t* staging_bytes) {
+  if (!p) return fail(DFT_EINVAL, "plan is NULL");
+  if (ncombo_out) *ncombo_out = (int)p->combos.size();
+  if (counts)
+    for (int c = 0; c < std::min<int>(ncombo, (int)p->combos.size()); ++c) {
+      counts[4 * c] = (int64_t)p->combos[c].q64.size();
+      counts[4 * c + 1] = (int64_t)p->combos[c].q32.size();
+      counts[4 * c + 2] = p->combos[c].prim64;
+      counts[4 * c + 3] = p->combos[c].prim32;
+    }
+  if (n_unscreened) *n_unscreened = p->n_unscreened;
+  if (staging_bytes) *staging_bytes = p->stage_len * (int64_t)sizeof(double);
+  return DFT_OK;
+}
+
+int dft_schwarz(dft_plan p, double* Q) {
+  if (!p || !Q) return fail(DFT_EINVAL, "NULL argument");
+  const int n = p->nsh;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) {
+      int a = p->in2int[i], b = p->in2int[j];
+      if (a < b) std::swap(a, b);
+      Q[(size_t)i * n + j] = p->Q[p->pair_of[(size_t)a * n + b]];
+    }
+  return DFT_OK;
+}
+
+int dft_jk(dft_plan p, const double* P, const double* Pa, const double* Pb, double* J, double* Ka, double* Kb,
+           int mode, void* stream) {
+  if (!p) return fail(DFT_EINVAL, "plan is NULL");
+  if (!P || !Pa || !J || !Ka) return fail(DFT_EINVAL, "P, Pa, J, Ka must be non-NULL");
+  if ((p->nk == 2) != (Pb != nullptr) || (p->nk == 2) != (Kb != nullptr))
+    return fail(DFT_EINVAL, "Pb/Kb must be given exactly for a UHF plan");
+  if (mode != DFT_FP64 && mode != DFT_MIXED) return fail(DFT_EINVAL, "mode must be DFT_FP64 or DFT_MIXED");
+  hipStream_t s = (hipStream_t)stream;
+  const int cap = 256 * 16;
+  const int nslot = 2 * (int)p->combos.size() + 2;
+  hipEvent_t* ev = nullptr;
+  if (p->prof_used < p->prof_steps) ev = &p->prof_ev[(size_t)p->prof_used++ * nslot * 2];
+  auto mark = [&](int slot, int end) {
+    if (ev) (void)hipEventRecord(ev[2 * slot + end], s);
+  };
+  for (size_t ci = 0; ci < p->combos.size(); ++ci) {
+    const Combo& c = p->combos[ci];
+    mark(2 * (int)ci, 0);
+    hipError_t e = launch_eri_jk((int)ci, false, c.d_q64, (long long)c.q64.size(), p->d_pairs, p->d_rec64,
+                                 p->d_shell_ao, P, Pa, Pb, p->nk, p->nbf, p->d_stage, c.off64, p->d_boys, cap, s);
+    if (e != hipSuccess) return fail(DFT_EHIP, std::string("eri_jk fp64: ") + hipGetErrorString(e));
+    mark(2 * (int)ci, 1);
+    const bool f32 = mode == DFT_MIXED;
+    mark(2 * (int)ci + 1, 0);
+    e = launch_eri_jk((int)ci, f32, c.d_q32, (long long)c.q32.size(), p->d_pairs,
+                      f32 ? (const void*)p->d_rec32 : (const void*)p->d_rec64, p->d_shell_ao, P, Pa, Pb, p->nk,
+                      p->nbf, p->d_stage, c.off32, p->d_boys, cap, s);
+    if (e != hipSuccess) return fail(DFT_EHIP, std::string("eri_jk weak part: ") + hipGetErrorString(e));
+    mark(2 * (int)ci + 1, 1);
+  }
+  mark(nslot - 2, 0);
+  hipError_t e = launch_reduce(p->ntarget, p->d_tg, p->d_tb, p->d_contrib, p->d_stage, p->d_half, p->nbf, s);
+  if (e != hipSuccess) return fail(DFT_EHIP, std::string("reduce: ") + hipGetErrorString(e));
+  mark(nslot - 2, 1);
+  mark(nslot - 1, 0);
+  const size_t nn = (size_t)p->nbf * p->nbf;
+  double* outs[3] = {J, Ka, Kb};
+  for (int m = 0; m < 1 + p->nk; ++m) {
+    e = launch_symmetrize(p->d_half + m * nn, outs[m], p->nbf, s);
+    if (e != hipSuccess) return fail(DFT_EHIP, std::string("symmetrize: ") + hipGetErrorString(e));
+  }
+  mark(nslot - 1, 1);
+  return DFT_OK;
+}
+
+int dft_plan_profile(dft_plan p, int nsteps) {
+  if (!p || nsteps < 0) return fail(DFT_EINVAL, "bad arguments");
+  for (hipEvent_t e : p->prof_ev) (void)hipEventDestroy(e);
+  p->prof_ev.clear();
+  p->prof_steps = p->prof_used = 0;
+  if (nsteps == 0) return DFT_OK;
+  const int nslot = 2 * (int)p->combos.size() + 2;
+  p->prof_ev.resize((size_t)nsteps * nslot * 2);
+  for (auto& e : p->prof_ev) HIP_TRY(hipEventCreate(&e));
+  p->prof_steps = nsteps;
+  return DFT_OK;
+}
+
+int dft_plan_profile_read(dft_plan p, double* ms_sum, int cap, int* nslot_out, int* nsteps_out) {
+  if (!p) return fail(DFT_EINVAL, "plan is NULL");
+  const int nslot = 2 * (int)p->combos.size() + 2;
+  if (nslot_out) *nslot_out = nslot;
+  if (nsteps_out) *nsteps_out = p->prof_used;
+  if (!ms_sum) return DFT_OK;
+  for (int k = 0; k < std::min(cap, nslot); ++k) ms_sum[k] = 0.0;
+  for (int st = 0; st < p->prof_used; ++st)
+    for (int k = 0; k < std::min(cap, nslot); ++k) {
+      hipEvent_t* ev = &p->prof_ev[((size_t)st * nslot + k) * 2];
+      HIP_TRY(hipEventSynchronize(ev[1]));
+      float ms = 0.f;
+      HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1]));
+      ms_sum[k] += ms;
+    }
+  return DFT_OK;
+}
+
+int dft_eri_quartets(dft_plan p, const int32_t* quartets, int64_t nq, double* out) {
+  if (!p) return fail(DFT_EINVAL, "plan is NULL");
+  if (nq < 0 || (nq > 0 && (!quartets || !out))) return fail(DFT_EINVAL, "bad quartet list");
+  const int n = p->nsh;
+  struct Req {
+    int2 q;
+    int perm[4];  // internal position -> requested position
+    long long out_off;
+  };
+  std::vector<std::vector<Req>> by_combo(p->combos.size());
+  long long off = 0;
+  std::vector<long long> offs(nq);
+  for (int64_t r = 0; r < nq; ++r) {
+    int s[4], ii[4];
+    for (int k = 0; k < 4; ++k) {
+      s[k] = quartets[4 * r + k];
+      if (s[k] < 0 || s[k] >= n) return fail(DFT_EINVAL, "quartet shell index out of range");
+      ii[k] = p->in2int[s[k]];
+    }
+    // positions of the internal bra (A >= B) and ket (C >= D)
+    int pos[4] = {0, 1, 2, 3};
+    if (ii[0] < ii[1]) std::swap(pos[0], pos[1]);
+    if (ii[2] < ii[3]) std::swap(pos[2], pos[3]);
+    int pb = p->pair_of[(size_t)ii[pos[0]] * n + ii[pos[1]]];
+    int pk = p->pair_of[(size_t)ii[pos[2]] * n + ii[pos[3]]];
+    if (p->pair_pc[pb] < p->pair_pc[pk]) {
+      std::swap(pb, pk);
+      std::swap(pos[0], pos[2]);
+      std::swap(pos[1], pos[3]);
+    }
+    Req q;
+    q.q = make_int2(pb, pk);
+    for (int k = 0; k < 4; ++k) q.perm[k] = pos[k];
+    q.out_off = off;
+    int sz = 1;
+    for (int k = 0; k < 4; ++k) sz *= ncart_h(p->sh[ii[k]].l);
+    off += sz;
+    by_combo[combo_index(p->pair_pc[pb], p->pair_pc[pk])].push_back(q);
+  }
+  for (size_t ci = 0; ci < by_combo.size(); ++ci) {
+    const auto& reqs = by_combo[ci];
+    if (reqs.empty()) continue;
+    int la, lb, lc, ld;
+    pc_moments(p->combos[ci].pcb, la, lb);
+    pc_moments(p->combos[ci].pck, lc, ld);
+    const int nI[4] = {ncart_h(la), ncart_h(lb), ncart_h(lc), ncart_h(ld)};
+    const int blk = nI[0] * nI[1] * nI[2] * nI[3];
+    std::vector<int2> q(reqs.size());
+    for (size_t k = 0; k < reqs.size(); ++k) q[k] = reqs[k].q;
+    std::vector<double> raw;
+    int rc = raw_blocks(p, (int)ci, q, raw, blk);
+    if (rc) return rc;
+    for (size_t k = 0; k < reqs.size(); ++k) {
+      const Req& R = reqs[k];
+      int nreq[4];
+      for (int i = 0; i < 4; ++i) nreq[R.perm[i]] = nI[i];
+      int x[4];
+      for (x[0] = 0; x[0] < nI[0]; ++x[0])
+        for (x[1] = 0; x[1] < nI[1]; ++x[1])
+          for (x[2] = 0; x[2] < nI[2]; ++x[2])
+            for (x[3] = 0; x[3] < nI[3]; ++x[3]) {
+              int y[4];
+              for (int i = 0; i < 4; ++i) y[R.perm[i]] = x[i];
+              const long long o = ((long long)(y[0] * nreq[1] + y[1]) * nreq[2] + y[2]) * nreq[3] + y[3];
+              out[R.out_off + o] = raw[k * blk + ((x[0] * nI[1] + x[1]) * nI[2] + x[2]) * nI[3] + x[3]];
+            }
+    }
+  }
+  return DFT_OK;
+}
+
+int dft_oneelec(dft_plan p, double* S, double* T, double* V, void* stream) {
+  if (!p || !S || !T || !V) return fail(DFT_EINVAL, "NULL argument");
+  hipError_t e = launch_one_electron(p->d_shpairs, p->nshpairs, p->d_sh, p->d_exps, p->d_coefs, p->natom,
+                                     p->d_atoms, S, T, V, p->nbf, p->d_boys, (hipStream_t)stream);
+  if (e != hipSuccess) return fail(DFT_EHIP, std::string("one-electron: ") + hipGetErrorString(e));
+  return DFT_OK;
+}
+
+int dft_boys(const double* T, int n, int mmax, double* out, int fp32, void* stream) {
+  if (n < 0 || (n > 0 && (!T || !out))) return fail(DFT_EINVAL, "bad arguments");
+  if (mmax < 0 || mmax > 16) return fail(DFT_EUNSUPPORTED, "mmax must be in [0, 16]");
+  double* tab = nullptr;
+  int rc = boys_table_device(&tab);
+  if (rc) return rc;
+  hipError_t e = launch_boys(T, n, mmax, out, fp32, tab, (hipStream_t)stream);
+  if (e != hipSuccess) return fail(DFT_EHIP, std::string("boys: ") + hipGetErrorString(e));
+  return DFT_OK;
+}
+
+}  // extern "C"

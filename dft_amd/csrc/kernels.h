@@ -1,0 +1,63 @@
+// Host-visible launch entry points of the device kernels (implemented in the
+// kernels_*.hip translation units). All pointers are device pointers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace dft {
+
+// number of pair classes / class combinations for shells up to l = 1 (s, p)
+constexpr int LMAX = 1;
+constexpr int NPC = (LMAX + 1) * (LMAX + 2) / 2;   // pair classes: ss, ps, pp
+constexpr int NCOMBO = NPC * (NPC + 1) / 2;        // (bra class >= ket class)
+
+inline int pc_index(int la, int lb) { return la * (la + 1) / 2 + lb; }
+inline void pc_moments(int pc, int& la, int& lb) {
+  la = 0;
+  while ((la + 1) * (la + 2) / 2 <= pc) ++la;
+  lb = pc - la * (la + 1) / 2;
+}
+inline int combo_index(int pcb, int pck) { return pcb * (pcb + 1) / 2 + pck; }
+
+struct ShellDev {      // one contracted (segmented) shell, internal order
+  int l, nprim, prim_off, ao;
+  double x, y, z;
+};
+
+// pair pre-pass: one lane per primitive pair; writes FP64 and FP32 records
+hipError_t launch_pair_records(const int4* pairs_prim /* sh_a, sh_b, i_a, i_b */, int nprim_pairs,
+                               const int* rec_off /* per primitive pair, in reals */,
+                               const ShellDev* shells, const double* exps, const double* coefs,
+                               double* rec64, float* rec32, hipStream_t s);
+
+// fused ERI + J/K digest for one class combination.
+hipError_t launch_eri_jk(int combo, bool fp32, const int2* quartets, long long nq,
+                         const int4* pairs, const void* rec, const int* shell_ao,
+                         const double* DJ, const double* DK0, const double* DK1, int nk, int nbf,
+                         double* stage, long long stage_off, const double* boys_tab,
+                         int grid_cap, hipStream_t s);
+
+// raw contracted blocks (internal orientation) for one class combination:
+// out + q * NA*NB*NC*ND
+hipError_t launch_eri_raw(int combo, const int2* quartets, long long nq, const int4* pairs,
+                          const double* rec, double* out, const double* boys_tab, hipStream_t s);
+
+// deterministic target reduction: one wave per target block
+hipError_t launch_reduce(int ntarget, const int4* targets /* row0, col0, nrow, ncol */,
+                         const long long* tgt_begin, const long long* contrib, const double* stage,
+                         double* half, int nbf, hipStream_t s);
+
+// M = H + H^T for nmat matrices of nbf x nbf
+hipError_t launch_symmetrize(const double* half, double* out, int nbf, hipStream_t s);
+
+// S, T, V over shell pairs (one lane per component pair of a shell pair)
+hipError_t launch_one_electron(const int2* shell_pairs, int npairs, const ShellDev* shells,
+                               const double* exps, const double* coefs, int natom,
+                               const double* atom_xyzq, double* S, double* T, double* V, int nbf,
+                               const double* boys_tab, hipStream_t s);
+
+// F_m(T) test hook
+hipError_t launch_boys(const double* T, int n, int mmax, double* out, int fp32,
+                       const double* boys_tab, hipStream_t s);
+
+}  // namespace dft

@@ -1,0 +1,224 @@
+// Fused ERI + J/K digest kernels and raw-block kernels for s/p class
+// combinations, instantiated per (LA LB | LC LD) and per precision.
+#include "eri_quartet.h"
+#include "kernels.h"
+
+namespace dft {
+
+constexpr int ERI_BLOCK = 256;
+
+// Each lane evaluates one canonical quartet (bra pair >= ket pair in the
+// class-sorted pair order) and writes its contribution record
+//   [Jab | Jcd | (Kac | Kad | Kbc | Kbd) x nk]
+// to stage + stage_off + q * RECSZ. With w = 2^-([a=b]+[c=d]+[ab=cd]):
+//   Jab = 2w sum_cd I P_cd        Jcd = 2w sum_ab I P_ab
+//   Kac = w sum_bd I P_bd  Kad = w sum_bc I P_bc  Kbc = w sum_ad I P_ad  Kbd = w sum_ac I P_ac
+// and the half matrices Jh, Kh summed over all records give J = Jh + Jh^T and
+// K = Kh + Kh^T, i.e. the eight orientations and degeneracy weights of
+// fill_matrices (shelled.py:791-859). No atomics: every record has its own
+// slot and the reduction over slots is a separate, fixed-order pass.
+template <int LA, int LB, int LC, int LD, typename real, int NK>
+__global__ __launch_bounds__(ERI_BLOCK) void eri_jk_kernel(
+    const int2* __restrict__ quartets, long long nq, const int4* __restrict__ pairs,
+    const real* __restrict__ rec, const int* __restrict__ shell_ao, const double* __restrict__ DJ,
+    const double* __restrict__ DK0, const double* __restrict__ DK1, int nbf,
+    double* __restrict__ stage, long long stage_off, const double* __restrict__ boys_gtab) {
+  using S = QuartetShape<LA, LB, LC, LD, real>;
+  constexpr int NA = S::NA, NB = S::NB, NC = S::NC, ND = S::ND;
+  constexpr int RECSZ = S::NAB + S::NCD + NK * (NA + NB) * (NC + ND);
+  __shared__ double btab[BOYS_NG * BOYS_TAYLOR];
+  boys_stage<S::L>(boys_gtab, btab);
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += stride) {
+    const int2 qd = quartets[q];
+    const int4 pb = pairs[qd.x];
+    const int4 pk = pairs[qd.y];
+    real I[S::NI];
+    quartet_integrals<LA, LB, LC, LD, real>(rec + pb.z, pb.w, rec + pk.z, pk.w, btab, I);
+    double w = 1.0;
+    if (pb.x == pb.y) w *= 0.5;
+    if (pk.x == pk.y) w *= 0.5;
+    if (qd.x == qd.y) w *= 0.5;
+    const int a0 = shell_ao[pb.x], b0 = shell_ao[pb.y];
+    const int c0 = shell_ao[pk.x], d0 = shell_ao[pk.y];
+    double* out = stage + stage_off + q * RECSZ;
+    // Coulomb
+    {
+      double Pcd[S::NCD], Pab[S::NAB];
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int d = 0; d < ND; ++d) Pcd[c * ND + d] = DJ[(long long)(c0 + c) * nbf + d0 + d];
+#pragma unroll
+      for (int a = 0; a < NA; ++a)
+#pragma unroll
+        for (int b = 0; b < NB; ++b) Pab[a * NB + b] = DJ[(long long)(a0 + a) * nbf + b0 + b];
+      double jcd[S::NCD];
+#pragma unroll
+      for (int cd = 0; cd < S::NCD; ++cd) jcd[cd] = 0.0;
+#pragma unroll
+      for (int ab = 0; ab < S::NAB; ++ab) {
+        double s = 0.0;
+#pragma unroll
+        for (int cd = 0; cd < S::NCD; ++cd) {
+          const double v = (double)I[ab * S::NCD + cd];
+          s = fma(v, Pcd[cd], s);
+          jcd[cd] = fma(v, Pab[ab], jcd[cd]);
+        }
+        out[ab] = 2.0 * w * s;
+      }
+#pragma unroll
+      for (int cd = 0; cd < S::NCD; ++cd) out[S::NAB + cd] = 2.0 * w * jcd[cd];
+    }
+    // Exchange, one density at a time
+#pragma unroll
+    for (int k = 0; k < NK; ++k) {
+      const double* D = k == 0 ? DK0 : DK1;
+      double* o = out + S::NAB + S::NCD + k * (NA + NB) * (NC + ND);
+      double Pbd[NB * ND], Pbc[NB * NC], Pad[NA * ND], Pac[NA * NC];
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+#pragma unroll
+        for (int d = 0; d < ND; ++d) Pbd[b * ND + d] = D[(long long)(b0 + b) * nbf + d0 + d];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) Pbc[b * NC + c] = D[(long long)(b0 + b) * nbf + c0 + c];
+      }
+#pragma unroll
+      for (int a = 0; a < NA; ++a) {
+#pragma unroll
+        for (int d = 0; d < ND; ++d) Pad[a * ND + d] = D[(long long)(a0 + a) * nbf + d0 + d];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) Pac[a * NC + c] = D[(long long)(a0 + a) * nbf + c0 + c];
+      }
+      double kac[NA * NC], kad[NA * ND], kbc[NB * NC], kbd[NB * ND];
+#pragma unroll
+      for (int x = 0; x < NA * NC; ++x) kac[x] = 0.0;
+#pragma unroll
+      for (int x = 0; x < NA * ND; ++x) kad[x] = 0.0;
+#pragma unroll
+      for (int x = 0; x < NB * NC; ++x) kbc[x] = 0.0;
+#pragma unroll
+      for (int x = 0; x < NB * ND; ++x) kbd[x] = 0.0;
+#pragma unroll
+      for (int a = 0; a < NA; ++a)
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+          for (int c = 0; c < NC; ++c)
+#pragma unroll
+            for (int d = 0; d < ND; ++d) {
+              const double v = (double)I[((a * NB + b) * NC + c) * ND + d];
+              kac[a * NC + c] = fma(v, Pbd[b * ND + d], kac[a * NC + c]);
+              kad[a * ND + d] = fma(v, Pbc[b * NC + c], kad[a * ND + d]);
+              kbc[b * NC + c] = fma(v, Pad[a * ND + d], kbc[b * NC + c]);
+              kbd[b * ND + d] = fma(v, Pac[a * NC + c], kbd[b * ND + d]);
+            }
+#pragma unroll
+      for (int x = 0; x < NA * NC; ++x) o[x] = w * kac[x];
+      o += NA * NC;
+#pragma unroll
+      for (int x = 0; x < NA * ND; ++x) o[x] = w * kad[x];
+      o += NA * ND;
+#pragma unroll
+      for (int x = 0; x < NB * NC; ++x) o[x] = w * kbc[x];
+      o += NB * NC;
+#pragma unroll
+      for (int x = 0; x < NB * ND; ++x) o[x] = w * kbd[x];
+    }
+  }
+}
+
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(ERI_BLOCK) void eri_raw_kernel(const int2* __restrict__ quartets, long long nq,
+                                                            const int4* __restrict__ pairs,
+                                                            const double* __restrict__ rec,
+                                                            double* __restrict__ out,
+                                                            const double* __restrict__ boys_gtab) {
+  using S = QuartetShape<LA, LB, LC, LD, double>;
+  __shared__ double btab[BOYS_NG * BOYS_TAYLOR];
+  boys_stage<S::L>(boys_gtab, btab);
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += stride) {
+    const int2 qd = quartets[q];
+    const int4 pb = pairs[qd.x];
+    const int4 pk = pairs[qd.y];
+    double I[S::NI];
+    quartet_integrals<LA, LB, LC, LD, double>(rec + pb.z, pb.w, rec + pk.z, pk.w, btab, I);
+    double* o = out + q * S::NI;
+#pragma unroll
+    for (int x = 0; x < S::NI; ++x) o[x] = I[x];
+  }
+}
+
+static int grid_for(long long nq, int cap) {
+  long long g = (nq + ERI_BLOCK - 1) / ERI_BLOCK;
+  if (g > cap) g = cap;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+template <int LA, int LB, int LC, int LD, typename real>
+static hipError_t jk_one(const int2* q, long long nq, const int4* pairs, const void* rec, const int* ao,
+                         const double* DJ, const double* DK0, const double* DK1, int nk, int nbf,
+                         double* stage, long long off, const double* bt, int cap, hipStream_t s) {
+  if (nq <= 0) return hipSuccess;
+  dim3 grid(grid_for(nq, cap)), block(ERI_BLOCK);
+  if (nk == 1)
+    hipLaunchKernelGGL((eri_jk_kernel<LA, LB, LC, LD, real, 1>), grid, block, 0, s, q, nq, pairs,
+                       (const real*)rec, ao, DJ, DK0, DK1, nbf, stage, off, bt);
+  else
+    hipLaunchKernelGGL((eri_jk_kernel<LA, LB, LC, LD, real, 2>), grid, block, 0, s, q, nq, pairs,
+                       (const real*)rec, ao, DJ, DK0, DK1, nbf, stage, off, bt);
+  return hipGetLastError();
+}
+
+template <int LA, int LB, int LC, int LD>
+static hipError_t raw_one(const int2* q, long long nq, const int4* pairs, const double* rec, double* out,
+                          const double* bt, hipStream_t s) {
+  if (nq <= 0) return hipSuccess;
+  dim3 grid(grid_for(nq, 1 << 20)), block(ERI_BLOCK);
+  hipLaunchKernelGGL((eri_raw_kernel<LA, LB, LC, LD>), grid, block, 0, s, q, nq, pairs, rec, out, bt);
+  return hipGetLastError();
+}
+
+// combo index = pcb (pcb+1)/2 + pck over pair classes ss=0, ps=1, pp=2
+#define DFT_FOR_COMBOS(X) \
+  X(0, 0, 0, 0, 0)        \
+  X(1, 1, 0, 0, 0)        \
+  X(2, 1, 0, 1, 0)        \
+  X(3, 1, 1, 0, 0)        \
+  X(4, 1, 1, 1, 0)        \
+  X(5, 1, 1, 1, 1)
+
+hipError_t launch_eri_jk(int combo, bool fp32, const int2* q, long long nq, const int4* pairs,
+                         const void* rec, const int* ao, const double* DJ, const double* DK0,
+                         const double* DK1, int nk, int nbf, double* stage, long long off,
+                         const double* bt, int cap, hipStream_t s) {
+  switch (combo) {
+#define X(id, la, lb, lc, ld)                                                                          \
+  case id:                                                                                             \
+    return fp32 ? jk_one<la, lb, lc, ld, float>(q, nq, pairs, rec, ao, DJ, DK0, DK1, nk, nbf, stage,  \
+                                                 off, bt, cap, s)                                      \
+                : jk_one<la, lb, lc, ld, double>(q, nq, pairs, rec, ao, DJ, DK0, DK1, nk, nbf, stage, \
+                                                  off, bt, cap, s);
+    DFT_FOR_COMBOS(X)
+#undef X
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_eri_raw(int combo, const int2* q, long long nq, const int4* pairs, const double* rec,
+                          double* out, const double* bt, hipStream_t s) {
+  switch (combo) {
+#define X(id, la, lb, lc, ld) \
+  case id:                    \
+    return raw_one<la, lb, lc, ld>(q, nq, pairs, rec, out, bt, s);
+    DFT_FOR_COMBOS(X)
+#undef X
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace dft

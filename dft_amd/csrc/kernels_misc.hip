@@ -1,0 +1,324 @@
+// Pair pre-pass, fixed-order reduction, symmetrisation, one-electron
+// integrals and the Boys test hook.
+#include "kernels.h"
+#include "md_device.h"
+
+namespace dft {
+
+// ------------------------------------------------------- pair pre-pass
+// One lane per primitive pair (i_a of shell a, i_b of shell b). Record:
+//   p, Px, Py, Pz, K_ab = c_a c_b exp(-ab/p |A-B|^2), then per axis the
+//   table E_t^{ia,ib} (ia <= la, ib <= lb) of calc_E (shelled.py:228-294),
+//   with E_0^{00} = 1 and the angular factor 1/sqrt((2ia-1)!!(2ib-1)!!).
+// a is raised first (X_PA = -b Q/p), then b (X_PB = a Q/p), as in calc_E.
+__global__ void pair_records_kernel(const int4* __restrict__ pp, int n, const int* __restrict__ rec_off,
+                                    const ShellDev* __restrict__ sh, const double* __restrict__ exps,
+                                    const double* __restrict__ coefs, double* __restrict__ r64,
+                                    float* __restrict__ r32) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  const int4 q = pp[idx];
+  const ShellDev A = sh[q.x], B = sh[q.y];
+  const double a = exps[A.prim_off + q.z], b = exps[B.prim_off + q.w];
+  const double ca = coefs[A.prim_off + q.z], cb = coefs[B.prim_off + q.w];
+  const int la = A.l, lb = B.l;
+  const double p = a + b, mu = a * b / p;
+  const double AB[3] = {A.x - B.x, A.y - B.y, A.z - B.z};
+  const double Ac[3] = {A.x, A.y, A.z}, Bc[3] = {B.x, B.y, B.z};
+  const int NE = ne_axis(la, lb);
+  double* o64 = r64 + rec_off[idx];
+  float* o32 = r32 + rec_off[idx];
+  double head[5];
+  for (int d = 0; d < 3; ++d) head[1 + d] = (a * Ac[d] + b * Bc[d]) / p;
+  head[0] = p;
+  head[4] = ca * cb * exp(-mu * (AB[0] * AB[0] + AB[1] * AB[1] + AB[2] * AB[2]));
+  for (int k = 0; k < 5; ++k) { o64[k] = head[k]; o32[k] = (float)head[k]; }
+  // (2l-1)!! for l <= 4: 1, 1, 3, 15, 105
+  const double dfac[5] = {1.0, 1.0, 3.0, 15.0, 105.0};
+  for (int d = 0; d < 3; ++d) {
+    const double Q = AB[d];
+    const double xpa = -b * Q / p, xpb = a * Q / p, i2p = 0.5 / p;
+    double Ea[5][9];  // E^{ia,0}_t
+    for (int ia = 0; ia <= la; ++ia)
+      for (int t = 0; t < 9; ++t) Ea[ia][t] = 0.0;
+    Ea[0][0] = 1.0;
+    for (int ia = 1; ia <= la; ++ia)
+      for (int t = 0; t <= ia; ++t) {
+        double v = 0.0;
+        if (t <= ia - 1) v += xpa * Ea[ia - 1][t];
+        if (t > 0) v += Ea[ia - 1][t - 1] * i2p;
+        if (t + 1 <= ia - 1) v += (t + 1) * Ea[ia - 1][t + 1];
+        Ea[ia][t] = v;
+      }
+    for (int ia = 0; ia <= la; ++ia) {
+      double cur[9], nxt[9];
+      for (int t = 0; t < 9; ++t) cur[t] = Ea[ia][t];
+      for (int ib = 0; ib <= lb; ++ib) {
+        const int ord = ia + ib;
+        const double nrm = 1.0 / sqrt(dfac[ia] * dfac[ib]);
+        const int o = 5 + d * NE + eoff(lb, ia, ib);
+        for (int t = 0; t <= ord; ++t) {
+          o64[o + t] = cur[t] * nrm;
+          o32[o + t] = (float)(cur[t] * nrm);
+        }
+        if (ib == lb) break;
+        for (int t = 0; t <= ord + 1; ++t) {
+          double v = 0.0;
+          if (t <= ord) v += xpb * cur[t];
+          if (t > 0) v += cur[t - 1] * i2p;
+          if (t + 1 <= ord) v += (t + 1) * cur[t + 1];
+          nxt[t] = v;
+        }
+        for (int t = 0; t <= ord + 1; ++t) cur[t] = nxt[t];
+      }
+    }
+  }
+}
+
+hipError_t launch_pair_records(const int4* pp, int n, const int* rec_off, const ShellDev* sh,
+                               const double* exps, const double* coefs, double* r64, float* r32,
+                               hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(pair_records_kernel, dim3((n + 127) / 128), dim3(128), 0, s, pp, n, rec_off, sh, exps,
+                     coefs, r64, r32);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------- fixed-order reduction
+// One 64-lane wave per target block (an ordered shell pair of one half
+// matrix). Lane l sums contributions l, l+64, ... in order, then a xor
+// butterfly combines the lanes; the summation tree is fixed, so J and K are
+// bitwise reproducible run to run (the reference's add.at scatter is not:
+// shelled.py:857-859).
+__global__ void reduce_kernel(int ntarget, const int4* __restrict__ tg, const long long* __restrict__ tb,
+                              const long long* __restrict__ contrib, const double* __restrict__ stage,
+                              double* __restrict__ half, int nbf) {
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (wave >= ntarget) return;
+  const int4 t = tg[wave];
+  const long long b = tb[wave], e = tb[wave + 1];
+  const int ne = t.z * t.w;
+  for (int el = 0; el < ne; ++el) {
+    double s = 0.0;
+    for (long long k = b + lane; k < e; k += 64) s += stage[contrib[k] + el];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) half[(long long)(t.x + el / t.w) * nbf + t.y + el % t.w] = s;
+  }
+}
+
+hipError_t launch_reduce(int ntarget, const int4* tg, const long long* tb, const long long* contrib,
+                         const double* stage, double* half, int nbf, hipStream_t s) {
+  if (ntarget <= 0) return hipSuccess;
+  const int waves_per_block = 4;
+  hipLaunchKernelGGL(reduce_kernel, dim3((ntarget + waves_per_block - 1) / waves_per_block),
+                     dim3(64 * waves_per_block), 0, s, ntarget, tg, tb, contrib, stage, half, nbf);
+  return hipGetLastError();
+}
+
+__global__ void symmetrize_kernel(const double* __restrict__ h, double* __restrict__ o, int n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)n * n) return;
+  const int r = (int)(i / n), c = (int)(i % n);
+  o[i] = h[i] + h[(long long)c * n + r];
+}
+
+hipError_t launch_symmetrize(const double* half, double* out, int n, hipStream_t s) {
+  const long long nn = (long long)n * n;
+  hipLaunchKernelGGL(symmetrize_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, half, out, n);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------ run-time-L helpers
+// Boys from the global table (no LDS), for the one-time 1-e kernel.
+__device__ void boys_rt(int L, double T, const double* __restrict__ g, double* F) {
+  if (T < BOYS_TMAX) {
+    const int k = __double2int_rn(T * BOYS_INV_DT);
+    const double d = -(T - k * BOYS_DT);
+    const double inv_fact[BOYS_TAYLOR] = {1.0, 1.0, 0.5, 1.0 / 6, 1.0 / 24, 1.0 / 120, 1.0 / 720, 1.0 / 5040};
+    double f = 0.0;
+    for (int j = BOYS_TAYLOR - 1; j >= 0; --j) f = fma(f, d, g[k * BOYS_MT + L + j] * inv_fact[j]);
+    F[L] = f;
+    const double e = exp(-T);
+    for (int m = L - 1; m >= 0; --m) F[m] = (2.0 * T * F[m + 1] + e) / (2.0 * m + 1.0);
+  } else {
+    F[0] = 0.88622692545275801365 / sqrt(T);
+    const double e = exp(-T);
+    for (int m = 0; m < L; ++m) F[m + 1] = ((2 * m + 1) * F[m] - e) / (2.0 * T);
+  }
+}
+
+// R^0_tuv for t+u+v <= L (L <= 8) from F, run-time L.
+__device__ void hermite_R_rt(int L, const double* F, double m2a, double X, double Y, double Z, double* R0) {
+  double lev[2][nherm(8)];
+  double pw = 1.0, s[9];
+  for (int n = 0; n <= L; ++n) { s[n] = F[n] * pw; pw *= m2a; }
+  int cur = 0;
+  lev[cur][0] = s[L];
+  for (int n = L - 1; n >= 0; --n) {
+    const int S = L - n, nxt = cur ^ 1;
+    lev[nxt][0] = s[n];
+    for (int h = 1; h < nherm(S); ++h) {
+      const int t = h_t(h), u = h_u(h), v = h_v(h);
+      double r;
+      if (t > 0) {
+        r = X * lev[cur][hidx(t - 1, u, v)];
+        if (t > 1) r += (t - 1) * lev[cur][hidx(t - 2, u, v)];
+      } else if (u > 0) {
+        r = Y * lev[cur][hidx(t, u - 1, v)];
+        if (u > 1) r += (u - 1) * lev[cur][hidx(t, u - 2, v)];
+      } else {
+        r = Z * lev[cur][hidx(t, u, v - 1)];
+        if (v > 1) r += (v - 1) * lev[cur][hidx(t, u, v - 2)];
+      }
+      lev[nxt][h] = r;
+    }
+    cur = nxt;
+  }
+  for (int h = 0; h < nherm(L); ++h) R0[h] = lev[cur][h];
+}
+
+// 1-D E_t^{ia,ib} with E_0^{00} = exp(-mu Q^2), as calc_E (shelled.py:228-294)
+__device__ void e1d(double a, double b, double Q, int ia, int ib, double* E /* >= ia+ib+1 */) {
+  const double p = a + b;
+  double prev[12], next[12];
+  for (int t = 0; t < 12; ++t) prev[t] = 0.0;
+  prev[0] = exp(-(a * b / p) * Q * Q);
+  int order = 0;
+  for (int step = 0; step < ia + ib; ++step) {
+    const double coef = step < ia ? -(b * Q / p) : (a * Q / p);
+    const int no = order + 1;
+    for (int t = 0; t <= no; ++t) {
+      double v = 0.0;
+      if (t <= order) v += coef * prev[t];
+      if (t > 0) v += prev[t - 1] / (2.0 * p);
+      if (t + 1 <= order) v += (t + 1) * prev[t + 1];
+      next[t] = v;
+    }
+    order = no;
+    for (int t = 0; t <= order; ++t) prev[t] = next[t];
+  }
+  for (int t = 0; t <= order; ++t) E[t] = prev[t];
+}
+
+// ------------------------------------------------------- S, T, V
+// One wave per shell pair, lane = component pair. S from E_0 sqrt(pi/p) per
+// axis (equal to the binomial closed form of shelled.py:104-146), T from the
+// per-axis kinetic piece of T_raw (shelled.py:149-155) and V from the
+// nuclear_kernel sum (shelled.py:413-436). Normalisation and contraction as
+// shelled.py:976-1045; the caller's coefs already carry the radial part.
+__global__ void one_electron_kernel(const int2* __restrict__ sp, int npairs, const ShellDev* __restrict__ sh,
+                                    const double* __restrict__ exps, const double* __restrict__ coefs,
+                                    int natom, const double* __restrict__ atoms, double* __restrict__ S,
+                                    double* __restrict__ T, double* __restrict__ V, int nbf,
+                                    const double* __restrict__ gtab) {
+  const int pidx = blockIdx.x;
+  const int lane = threadIdx.x;
+  if (pidx >= npairs) return;
+  const int2 pr = sp[pidx];
+  const ShellDev A = sh[pr.x], B = sh[pr.y];
+  const int na = ncart(A.l), nb = ncart(B.l);
+  if (lane >= na * nb) return;
+  const int ca = lane / nb, cb = lane % nb;
+  int pa[3], pb[3];
+  for (int d = 0; d < 3; ++d) { pa[d] = comp_of(A.l, ca, d); pb[d] = comp_of(B.l, cb, d); }
+  const double Ac[3] = {A.x, A.y, A.z}, Bc[3] = {B.x, B.y, B.z};
+  const double dfac[5] = {1.0, 1.0, 3.0, 15.0, 105.0};
+  double nang = 1.0;
+  for (int d = 0; d < 3; ++d) nang /= sqrt(dfac[pa[d]] * dfac[pb[d]]);
+  double s_acc = 0.0, t_acc = 0.0, v_acc = 0.0;
+  for (int i = 0; i < A.nprim; ++i) {
+    for (int j = 0; j < B.nprim; ++j) {
+      const double a = exps[A.prim_off + i], b = exps[B.prim_off + j];
+      const double cc = coefs[A.prim_off + i] * coefs[B.prim_off + j] * nang;
+      const double p = a + b;
+      const double sq = sqrt(M_PI / p);
+      double E[3][12], S1[3], T1[3];
+      for (int d = 0; d < 3; ++d) {
+        const double Q = Ac[d] - Bc[d];
+        double Et[12];
+        e1d(a, b, Q, pa[d], pb[d], E[d]);
+        S1[d] = E[d][0] * sq;
+        e1d(a, b, Q, pa[d], pb[d] + 2, Et);
+        double tk = -2.0 * b * (2 * pb[d] + 1) * S1[d] + 4.0 * b * b * Et[0] * sq;
+        if (pb[d] >= 2) {
+          e1d(a, b, Q, pa[d], pb[d] - 2, Et);
+          tk += pb[d] * (pb[d] - 1) * Et[0] * sq;
+        }
+        T1[d] = tk;
+      }
+      s_acc += cc * S1[0] * S1[1] * S1[2];
+      t_acc += cc * -0.5 * (T1[0] * S1[1] * S1[2] + S1[0] * T1[1] * S1[2] + S1[0] * S1[1] * T1[2]);
+      // nuclear attraction
+      const int L = A.l + B.l;
+      const double P[3] = {(a * A.x + b * B.x) / p, (a * A.y + b * B.y) / p, (a * A.z + b * B.z) / p};
+      double vsum = 0.0;
+      for (int c = 0; c < natom; ++c) {
+        const double X = P[0] - atoms[4 * c], Y = P[1] - atoms[4 * c + 1], Z = P[2] - atoms[4 * c + 2];
+        double F[9], R0[nherm(8)];
+        boys_rt(L, p * (X * X + Y * Y + Z * Z), gtab, F);
+        hermite_R_rt(L, F, -2.0 * p, X, Y, Z, R0);
+        double cs = 0.0;
+        for (int t = 0; t <= pa[0] + pb[0]; ++t)
+          for (int u = 0; u <= pa[1] + pb[1]; ++u)
+            for (int v = 0; v <= pa[2] + pb[2]; ++v) cs += E[0][t] * E[1][u] * E[2][v] * R0[hidx(t, u, v)];
+        vsum += atoms[4 * c + 3] * cs;
+      }
+      v_acc += cc * vsum * (-2.0 * M_PI / p);
+    }
+  }
+  const long long r = A.ao + ca, c = B.ao + cb;
+  S[r * nbf + c] = s_acc;
+  S[c * nbf + r] = s_acc;
+  T[r * nbf + c] = t_acc;
+  T[c * nbf + r] = t_acc;
+  V[r * nbf + c] = v_acc;
+  V[c * nbf + r] = v_acc;
+}
+
+hipError_t launch_one_electron(const int2* sp, int npairs, const ShellDev* sh, const double* exps,
+                               const double* coefs, int natom, const double* atoms, double* S, double* T,
+                               double* V, int nbf, const double* gtab, hipStream_t s) {
+  if (npairs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(one_electron_kernel, dim3(npairs), dim3(64), 0, s, sp, npairs, sh, exps, coefs, natom,
+                     atoms, S, T, V, nbf, gtab);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------- Boys hook
+template <int L>
+__global__ void boys_kernel(const double* __restrict__ T, int n, double* __restrict__ out, int fp32,
+                            const double* __restrict__ gtab) {
+  __shared__ double btab[BOYS_NG * BOYS_TAYLOR];
+  boys_stage<L>(gtab, btab);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (fp32) {
+    float F[L + 1];
+    boys_fp32<L>((float)T[i], btab, F);
+    for (int m = 0; m <= L; ++m) out[(long long)i * (L + 1) + m] = F[m];
+  } else {
+    double F[L + 1];
+    boys_fp64<L>(T[i], btab, F);
+    for (int m = 0; m <= L; ++m) out[(long long)i * (L + 1) + m] = F[m];
+  }
+}
+
+hipError_t launch_boys(const double* T, int n, int mmax, double* out, int fp32, const double* gtab,
+                       hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  dim3 g((n + 255) / 256), b(256);
+  switch (mmax) {
+#define B(LL) \
+  case LL:    \
+    hipLaunchKernelGGL(boys_kernel<LL>, g, b, 0, s, T, n, out, fp32, gtab); break;
+    B(0) B(1) B(2) B(3) B(4) B(5) B(6) B(7) B(8) B(9) B(10) B(11) B(12) B(13) B(14) B(15) B(16)
+#undef B
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace dft

@@ -1,0 +1,209 @@
+// McMurchie-Davidson building blocks for gfx950: compile-time index tables,
+// the tabulated Boys function and the Hermite R_tuv recursion.
+//
+// Everything here is templated on the angular momenta so that every index is
+// a compile-time constant after unrolling and all intermediates stay in
+// VGPRs. The reference evaluates the same quantities per primitive Cartesian
+// quartet with run-time loops over a dense 16 KB local box
+// (shelled.py:174-348); here they are evaluated once per primitive shell
+// quartet and shared by all Cartesian components.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <type_traits>
+#include <utility>
+
+namespace dft {
+
+// ------------------------------------------------------------ static_for
+template <int Begin, int End, typename F>
+__host__ __device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (Begin < End) {
+    f(std::integral_constant<int, Begin>{});
+    static_for<Begin + 1, End>(static_cast<F&&>(f));
+  }
+}
+
+// -------------------------------------------------------- index algebra
+__host__ __device__ constexpr int ncart(int l) { return (l + 1) * (l + 2) / 2; }
+// number of Hermite functions with t+u+v <= L
+__host__ __device__ constexpr int nherm(int L) { return (L + 1) * (L + 2) * (L + 3) / 6; }
+// tetrahedral numbering of (t,u,v): by s=t+u+v, then r=u+v, then v
+__host__ __device__ constexpr int hidx(int t, int u, int v) {
+  int s = t + u + v, r = u + v;
+  return s * (s + 1) * (s + 2) / 6 + r * (r + 1) / 2 + v;
+}
+__host__ __device__ constexpr int h_s(int h) { int s = 0; while (nherm(s) <= h) ++s; return s; }
+__host__ __device__ constexpr int h_r(int h) {
+  int s = h_s(h); int rem = h - (s == 0 ? 0 : nherm(s - 1)); int r = 0;
+  while ((r + 1) * (r + 2) / 2 <= rem) ++r; return r;
+}
+__host__ __device__ constexpr int h_v(int h) {
+  int s = h_s(h); int rem = h - (s == 0 ? 0 : nherm(s - 1)); int r = h_r(h);
+  return rem - r * (r + 1) / 2;
+}
+__host__ __device__ constexpr int h_u(int h) { return h_r(h) - h_v(h); }
+__host__ __device__ constexpr int h_t(int h) { return h_s(h) - h_r(h); }
+
+// Cartesian component k of a shell of momentum l, in the order
+// z-outer / y-middle / x-inner of shelled.py:15-23 (p: x,y,z; d: xx,xy,yy,xz,yz,zz)
+__host__ __device__ constexpr int comp_z(int l, int k) {
+  int z = 0; while (k >= l - z + 1) { k -= l - z + 1; ++z; } return z;
+}
+__host__ __device__ constexpr int comp_y(int l, int k) {
+  int z = 0; while (k >= l - z + 1) { k -= l - z + 1; ++z; } return k;
+}
+__host__ __device__ constexpr int comp_x(int l, int k) { return l - comp_y(l, k) - comp_z(l, k); }
+__host__ __device__ constexpr int comp_of(int l, int k, int axis) {
+  return axis == 0 ? comp_x(l, k) : (axis == 1 ? comp_y(l, k) : comp_z(l, k));
+}
+
+// 1-D Hermite table of a primitive pair of class (la, lb): for every
+// ia <= la, ib <= lb the coefficients E_t^{ia,ib}, t = 0..ia+ib, packed
+// ia-major. NE = (la+1)(lb+1)(la+lb+2)/2 per axis.
+__host__ __device__ constexpr int ne_axis(int la, int lb) { return (la + 1) * (lb + 1) * (la + lb + 2) / 2; }
+__host__ __device__ constexpr int eoff(int lb, int ia, int ib) {
+  int o = 0;
+  for (int a = 0; a < ia; ++a)
+    for (int b = 0; b <= lb; ++b) o += a + b + 1;
+  for (int b = 0; b < ib; ++b) o += ia + b + 1;
+  return o;
+}
+// primitive-pair record: p, Px, Py, Pz, Kab, then Ex[NE], Ey[NE], Ez[NE]
+__host__ __device__ constexpr int rec_size(int la, int lb) { return 5 + 3 * ne_axis(la, lb); }
+
+// ------------------------------------------------------------- Boys F_m
+// Table of F_m(T_k), T_k = k * BOYS_DT, k < BOYS_NG, m < BOYS_MT, generated on
+// the host in extended precision. A kernel of total momentum L stages
+// F_{L+j}(T_k)/j!, j < BOYS_TAYLOR, into LDS and evaluates
+//   F_L(T) = sum_j F_{L+j}(T_k)/j! (-d)^j,  d = T - T_k,  |d| <= DT/2
+// (dF_m/dT = -F_{m+1}), then recurses downward
+//   F_m = (2T F_{m+1} + e^{-T}) / (2m+1).
+// For T >= BOYS_TMAX: F_0 = sqrt(pi/T)/2 (erfc(sqrt T) < 1e-18) and the upward
+// recursion F_{m+1} = ((2m+1) F_m - e^{-T}) / (2T), stable for m < T.
+constexpr int BOYS_NG = 401;
+constexpr double BOYS_DT = 0.1;
+constexpr double BOYS_INV_DT = 10.0;
+constexpr double BOYS_TMAX = 40.0;
+constexpr int BOYS_TAYLOR = 8;
+constexpr int BOYS_MT = 32;  // m range held in the global table
+
+__host__ __device__ constexpr double inv_odd(int m) { return 1.0 / (2.0 * m + 1.0); }
+
+// FP64 Boys function F_0..F_L from the LDS slice tab[k * BOYS_TAYLOR + j].
+template <int L>
+__device__ __forceinline__ void boys_fp64(double T, const double* __restrict__ tab, double (&F)[L + 1]) {
+  if (T < BOYS_TMAX) {
+    int k = __double2int_rn(T * BOYS_INV_DT);
+    double d = -(T - k * BOYS_DT);
+    const double* c = tab + k * BOYS_TAYLOR;
+    double f = c[BOYS_TAYLOR - 1];
+#pragma unroll
+    for (int j = BOYS_TAYLOR - 2; j >= 0; --j) f = fma(f, d, c[j]);
+    F[L] = f;
+    if constexpr (L > 0) {
+      double e = exp(-T);
+      double t2 = T + T;
+#pragma unroll
+      for (int m = L - 1; m >= 0; --m) F[m] = fma(t2, F[m + 1], e) * inv_odd(m);
+    }
+  } else {
+    double rs = rsqrt(T);
+    F[0] = 0.88622692545275801365 * rs;  // sqrt(pi)/2 / sqrt(T)
+    if constexpr (L > 0) {
+      double e = exp(-T);
+      double h = 0.5 / T;
+#pragma unroll
+      for (int m = 0; m < L; ++m) F[m + 1] = ((2 * m + 1) * F[m] - e) * h;
+    }
+  }
+}
+
+// FP32 Boys: the table seed F_L(T_k) and its Taylor step stay FP64; the
+// downward/upward recursion runs in FP32 with the native exp.
+template <int L>
+__device__ __forceinline__ void boys_fp32(float Tf, const double* __restrict__ tab, float (&F)[L + 1]) {
+  if (Tf < (float)BOYS_TMAX) {
+    double T = (double)Tf;
+    int k = __double2int_rn(T * BOYS_INV_DT);
+    double d = -(T - k * BOYS_DT);
+    const double* c = tab + k * BOYS_TAYLOR;
+    double f = c[BOYS_TAYLOR - 1];
+#pragma unroll
+    for (int j = BOYS_TAYLOR - 2; j >= 0; --j) f = fma(f, d, c[j]);
+    F[L] = (float)f;
+    if constexpr (L > 0) {
+      float e = __expf(-Tf);
+      float t2 = Tf + Tf;
+#pragma unroll
+      for (int m = L - 1; m >= 0; --m) F[m] = fmaf(t2, F[m + 1], e) * (float)inv_odd(m);
+    }
+  } else {
+    float rs = rsqrtf(Tf);
+    F[0] = 0.886226925f * rs;
+    if constexpr (L > 0) {
+      float e = __expf(-Tf);
+      float h = 0.5f / Tf;
+#pragma unroll
+      for (int m = 0; m < L; ++m) F[m + 1] = ((2 * m + 1) * F[m] - e) * h;
+    }
+  }
+}
+
+template <int L, typename real>
+__device__ __forceinline__ void boys_eval(real T, const double* __restrict__ tab, real (&F)[L + 1]) {
+  if constexpr (std::is_same<real, double>::value) boys_fp64<L>(T, tab, F);
+  else boys_fp32<L>(T, tab, F);
+}
+
+// Stage F_{L+j}(T_k)/j! for all k into LDS (cooperative, whole block).
+template <int L>
+__device__ __forceinline__ void boys_stage(const double* __restrict__ gtab, double* __restrict__ lds) {
+  constexpr double inv_fact[BOYS_TAYLOR] = {1.0, 1.0, 0.5, 1.0 / 6, 1.0 / 24, 1.0 / 120, 1.0 / 720, 1.0 / 5040};
+  for (int i = threadIdx.x; i < BOYS_NG * BOYS_TAYLOR; i += blockDim.x) {
+    int k = i / BOYS_TAYLOR, j = i - k * BOYS_TAYLOR;
+    lds[i] = gtab[k * BOYS_MT + L + j] * inv_fact[j];
+  }
+  __syncthreads();
+}
+
+// --------------------------------------------------- Hermite R^0_{tuv}
+// R^n_000 = (-2a)^n F_n; R^n_{t+1,u,v} = t R^{n+1}_{t-1,u,v} + X R^{n+1}_{tuv}
+// (and the same in u/Y, v/Z). Evaluated level by level from n = L down to 0
+// entirely in registers; returns R^0 for t+u+v <= L in hidx order.
+template <int L, typename real>
+__device__ __forceinline__ void hermite_R(const real (&F)[L + 1], real m2a, real X, real Y, real Z,
+                                          real (&R)[nherm(L)]) {
+  real s[L + 1];
+  real pw = real(1);
+#pragma unroll
+  for (int n = 0; n <= L; ++n) { s[n] = F[n] * pw; pw *= m2a; }
+  real prev[nherm(L)];
+  prev[0] = s[L];
+  static_for<0, L>([&](auto it) {
+    constexpr int n = L - 1 - decltype(it)::value;
+    constexpr int S = L - n;
+    real cur[nherm(S)];
+    cur[0] = s[n];
+    static_for<1, nherm(S)>([&](auto hI) {
+      constexpr int h = decltype(hI)::value;
+      constexpr int t = h_t(h), u = h_u(h), v = h_v(h);
+      if constexpr (t > 0) {
+        real r = X * prev[hidx(t - 1, u, v)];
+        if constexpr (t > 1) r = fma(real(t - 1), prev[hidx(t - 2, u, v)], r);
+        cur[h] = r;
+      } else if constexpr (u > 0) {
+        real r = Y * prev[hidx(t, u - 1, v)];
+        if constexpr (u > 1) r = fma(real(u - 1), prev[hidx(t, u - 2, v)], r);
+        cur[h] = r;
+      } else {
+        real r = Z * prev[hidx(t, u, v - 1)];
+        if constexpr (v > 1) r = fma(real(v - 1), prev[hidx(t, u, v - 2)], r);
+        cur[h] = r;
+      }
+    });
+    static_for<0, nherm(S)>([&](auto hI) { prev[decltype(hI)::value] = cur[decltype(hI)::value]; });
+  });
+  static_for<0, nherm(L)>([&](auto hI) { R[decltype(hI)::value] = prev[decltype(hI)::value]; });
+}
+
+}  // namespace dft

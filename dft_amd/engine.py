@@ -1,0 +1,226 @@
+"""Python front end of the HIP integral engine.
+
+``Plan`` wraps one ``dft_plan`` of libdft_hip.so: it flattens the reference's
+shell dicts (``shelled.py:58-77,938-952``) into the C-ABI arrays, splits
+general contractions into segmented shells (a shell with k coefficient rows
+becomes k shells whose AO blocks follow each other, which is exactly the
+reference's AO = contraction * ncomp + component order), and exposes the
+device entry points on PyTorch-ROCm tensors.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+
+
+def _ncart(l: int) -> int:
+    return (l + 1) * (l + 2) // 2
+
+
+class Plan:
+    """Integral plan of one molecule (pair data, Schwarz bounds, quartet lists).
+
+    Parameters follow ``dft_plan_create``: ``schwarz_tau`` drops quartets with
+    Q_ab Q_cd below it; ``fp32_tau`` marks quartets below it for FP32 in the
+    mixed mode; ``uhf`` builds K for two densities.
+    """
+
+    def __init__(self, shells, Z, centers, schwarz_tau=1e-10, fp32_tau=1e-4, uhf=False,
+                 device=None):
+        self.lib = _lib.load()
+        if not torch.cuda.is_available():
+            raise _lib.DFTError("no HIP device visible: the integral engine runs on the GPU only")
+        self.device = torch.device("cuda", torch.cuda.current_device() if device is None else device)
+        self.shells = shells
+        self.nbf = int(shells[-1]["ao_stop"])
+        self.uhf = bool(uhf)
+        seg_l, seg_np, seg_ao, seg_xyz, exps, coefs = [], [], [], [], [], []
+        self.segments = []  # per reference shell: list of segment ids
+        for sh in shells:
+            ids = []
+            ncomp = sh["n_components"]
+            for r in range(sh["n_contractions"]):
+                ids.append(len(seg_l))
+                seg_l.append(sh["angular_momentum"])
+                seg_np.append(sh["n_exponents"])
+                seg_ao.append(sh["ao_start"] + r * ncomp)
+                seg_xyz.append(np.asarray(sh["atom_pos"], dtype=np.float64))
+                exps.append(np.asarray(sh["exponents"], dtype=np.float64))
+                coefs.append(np.asarray(sh["coefficients"], dtype=np.float64)[r])
+            self.segments.append(ids)
+        self.seg_l = np.asarray(seg_l, dtype=np.int32)
+        self.nseg = len(seg_l)
+        seg_np = np.asarray(seg_np, dtype=np.int32)
+        seg_ao = np.asarray(seg_ao, dtype=np.int32)
+        seg_xyz = np.ascontiguousarray(np.stack(seg_xyz), dtype=np.float64)
+        exps = np.ascontiguousarray(np.concatenate(exps))
+        coefs = np.ascontiguousarray(np.concatenate(coefs))
+        Zc = np.ascontiguousarray(np.asarray(Z, dtype=np.float64))
+        xyz = np.ascontiguousarray(np.asarray(centers, dtype=np.float64).reshape(-1, 3))
+        self.Z, self.centers = Zc, xyz
+        handle = C.c_void_p()
+        with torch.cuda.device(self.device):
+            _lib.check(self.lib.dft_plan_create(
+                self.nseg, _lib.ptr(self.seg_l, C.c_int32), _lib.ptr(seg_np, C.c_int32),
+                _lib.ptr(seg_ao, C.c_int32), _lib.ptr(seg_xyz, C.c_double),
+                _lib.ptr(exps, C.c_double), _lib.ptr(coefs, C.c_double), len(Zc),
+                _lib.ptr(xyz, C.c_double), _lib.ptr(Zc, C.c_double), self.nbf,
+                float(schwarz_tau), float(fp32_tau), _lib.DFT_PLAN_UHF if uhf else 0,
+                C.byref(handle)))
+        self.handle = handle
+        self.schwarz_tau, self.fp32_tau = schwarz_tau, fp32_tau
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            self.lib.dft_plan_destroy(h)
+            self.handle = None
+
+    # ------------------------------------------------------------ stats
+    def stats(self) -> dict:
+        counts = np.zeros(4 * 64, dtype=np.int64)
+        nc = C.c_int()
+        nun = C.c_int64()
+        sb = C.c_int64()
+        _lib.check(self.lib.dft_plan_stats(self.handle, _lib.ptr(counts, C.c_int64), 64, C.byref(nc),
+                                           C.byref(nun), C.byref(sb)))
+        c = counts[:4 * nc.value].reshape(-1, 4)
+        return {"per_combo": c, "n_quartets_fp64": int(c[:, 0].sum()), "n_quartets_fp32": int(c[:, 1].sum()),
+                "n_quartets": int(c[:, :2].sum()), "n_prim_quartets": int(c[:, 2:].sum()),
+                "n_unscreened": int(nun.value), "staging_bytes": int(sb.value)}
+
+    def schwarz(self) -> np.ndarray:
+        Q = np.zeros(self.nseg * self.nseg)
+        _lib.check(self.lib.dft_schwarz(self.handle, _lib.ptr(Q, C.c_double)))
+        return Q.reshape(self.nseg, self.nseg)
+
+    # -------------------------------------------------------------- J/K
+    def _dev(self, M):
+        if isinstance(M, torch.Tensor):
+            t = M.to(device=self.device, dtype=torch.float64)
+        else:
+            t = torch.as_tensor(np.asarray(M, dtype=np.float64), device=self.device)
+        t = t.contiguous()
+        if t.shape != (self.nbf, self.nbf):
+            raise ValueError(f"density must be {self.nbf}x{self.nbf}, got {tuple(t.shape)}")
+        return t
+
+    def jk_into(self, P, Pa, Pb, J, Ka, Kb=None, mode="fp64", stream=None):
+        """Enqueue J/K on ``stream`` into preallocated device tensors (no sync)."""
+        s = torch.cuda.current_stream(self.device) if stream is None else stream
+        _lib.check(self.lib.dft_jk(
+            self.handle, P.data_ptr(), Pa.data_ptr(), Pb.data_ptr() if Pb is not None else None,
+            J.data_ptr(), Ka.data_ptr(), Kb.data_ptr() if Kb is not None else None,
+            _lib.MODES[mode], C.c_void_p(s.cuda_stream)))
+
+    def jk(self, P, Pa, Pb=None, mode="fp64"):
+        """J[P], K[Pa] (and K[Pb] for a UHF plan) as device tensors."""
+        P, Pa = self._dev(P), self._dev(Pa)
+        if self.uhf:
+            if Pb is None:
+                raise ValueError("UHF plan needs Pb")
+            Pb = self._dev(Pb)
+        else:
+            Pb = None
+        J = torch.empty_like(P)
+        Ka = torch.empty_like(P)
+        Kb = torch.empty_like(P) if self.uhf else None
+        self.jk_into(P, Pa, Pb, J, Ka, Kb, mode)
+        return J, Ka, Kb
+
+    # --------------------------------------------------- 1-e integrals
+    def one_electron(self):
+        n = self.nbf
+        S = torch.empty((n, n), dtype=torch.float64, device=self.device)
+        T = torch.empty_like(S)
+        V = torch.empty_like(S)
+        s = torch.cuda.current_stream(self.device)
+        _lib.check(self.lib.dft_oneelec(self.handle, S.data_ptr(), T.data_ptr(), V.data_ptr(),
+                                        C.c_void_p(s.cuda_stream)))
+        return S, T, V
+
+    # ----------------------------------------------------- raw ERIs
+    def eri_segment_blocks(self, quartets) -> np.ndarray:
+        """Concatenated blocks for segment quartets (int array [nq, 4])."""
+        q = np.ascontiguousarray(np.asarray(quartets, dtype=np.int32).reshape(-1, 4))
+        sizes = np.prod(np.array([[_ncart(int(self.seg_l[s])) for s in row] for row in q]), axis=1) \
+            if len(q) else np.zeros(0, dtype=np.int64)
+        out = np.zeros(int(sizes.sum()))
+        torch.cuda.synchronize(self.device)
+        _lib.check(self.lib.dft_eri_quartets(self.handle, _lib.ptr(q, C.c_int32), len(q),
+                                             _lib.ptr(out, C.c_double)))
+        return out
+
+    def eri_blocks(self, quartets):
+        """Blocks [n_ao_a, n_ao_b, n_ao_c, n_ao_d] for reference shell quartets."""
+        seg_q, plan = [], []
+        for (a, b, c, d) in quartets:
+            sa, sb, sc, sd = (self.segments[k] for k in (a, b, c, d))
+            plan.append((a, b, c, d))
+            for i in sa:
+                for j in sb:
+                    for k in sc:
+                        for m in sd:
+                            seg_q.append((i, j, k, m))
+        flat = self.eri_segment_blocks(seg_q)
+        out, pos = [], 0
+        for (a, b, c, d) in plan:
+            sh = [self.shells[k] for k in (a, b, c, d)]
+            nc = [s["n_components"] for s in sh]
+            blk = np.zeros([s["n_ao"] for s in sh])
+            for ri in range(sh[0]["n_contractions"]):
+                for rj in range(sh[1]["n_contractions"]):
+                    for rk in range(sh[2]["n_contractions"]):
+                        for rm in range(sh[3]["n_contractions"]):
+                            size = nc[0] * nc[1] * nc[2] * nc[3]
+                            sub = flat[pos:pos + size].reshape(nc)
+                            pos += size
+                            blk[ri * nc[0]:(ri + 1) * nc[0], rj * nc[1]:(rj + 1) * nc[1],
+                                rk * nc[2]:(rk + 1) * nc[2], rm * nc[3]:(rm + 1) * nc[3]] = sub
+            out.append(blk)
+        return out
+
+    def eri_tensor(self) -> np.ndarray:
+        """Full (nbf^4) ERI tensor, from the unique shell quartets."""
+        n = self.nbf
+        nsh = len(self.shells)
+        pairs = [(i, j) for i in range(nsh) for j in range(i + 1)]
+        quart = [(a, b, c, d) for ab, (a, b) in enumerate(pairs) for (c, d) in pairs[:ab + 1]]
+        blocks = self.eri_blocks(quart)
+        T = np.zeros((n, n, n, n))
+        for (a, b, c, d), blk in zip(quart, blocks):
+            s = [slice(self.shells[k]["ao_start"], self.shells[k]["ao_stop"]) for k in (a, b, c, d)]
+            for perm in ((0, 1, 2, 3), (1, 0, 2, 3), (0, 1, 3, 2), (1, 0, 3, 2),
+                         (2, 3, 0, 1), (3, 2, 0, 1), (2, 3, 1, 0), (3, 2, 1, 0)):
+                T[s[perm[0]], s[perm[1]], s[perm[2]], s[perm[3]]] = blk.transpose(perm)
+        return T
+
+
+def boys(T, mmax: int, fp32: bool = False) -> torch.Tensor:
+    """F_m(T), m = 0..mmax, from the device evaluator (test hook)."""
+    lib = _lib.load()
+    T = torch.as_tensor(T, dtype=torch.float64).to("cuda").contiguous()
+    out = torch.empty((T.numel(), mmax + 1), dtype=torch.float64, device=T.device)
+    s = torch.cuda.current_stream(T.device)
+    _lib.check(lib.dft_boys(T.data_ptr(), T.numel(), mmax, out.data_ptr(), int(fp32),
+                            C.c_void_p(s.cuda_stream)))
+    return out
+
+
+def profile_start(plan: Plan, nsteps: int) -> None:
+    """Record per-launch HIP-event timings for the next ``nsteps`` J/K builds."""
+    _lib.check(plan.lib.dft_plan_profile(plan.handle, int(nsteps)))
+
+
+def profile_read(plan: Plan):
+    """(ms summed over recorded steps per slot, number of steps)."""
+    ns, st = C.c_int(), C.c_int()
+    _lib.check(plan.lib.dft_plan_profile_read(plan.handle, None, 0, C.byref(ns), C.byref(st)))
+    ms = np.zeros(ns.value)
+    _lib.check(plan.lib.dft_plan_profile_read(plan.handle, _lib.ptr(ms, C.c_double), ns.value, C.byref(ns),
+                                              C.byref(st)))
+    return ms, st.value

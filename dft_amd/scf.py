@@ -1,0 +1,229 @@
+"""SCF drivers on top of the HIP Fock build.
+
+``scf_plain`` is the plain UHF Roothaan loop of ``main.py:753-828`` (core
+Hamiltonian guess, symmetric orthogonalisation without cutoff, aufbau
+occupation, convergence on |dE| < 1e-8 and max|dP| < 1e-6). ``scf_diis`` is
+the damped + DIIS loop of ``shelled.py:1103-1292``. J/K come from
+``Plan.jk`` (fused direct build on the GPU); orthogonalisation,
+diagonalisation, density build and energy stay FP64 on the GPU through
+torch.linalg. RHF (N_alpha == N_beta) uses one exchange matrix, since then
+P_a = P_b and K_a = K_b exactly.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from .engine import Plan
+
+
+def nuclear_repulsion(Z, centers) -> float:
+    """sum_{i<j} Z_i Z_j / |R_i - R_j| (shelled.py:157-163)."""
+    Z = np.asarray(Z, dtype=np.float64)
+    R = np.asarray(centers, dtype=np.float64).reshape(-1, 3)
+    d = np.linalg.norm(R[:, None, :] - R[None, :, :], axis=-1)
+    iu = np.triu_indices(len(Z), 1)
+    return float(np.sum(Z[iu[0]] * Z[iu[1]] / d[iu]))
+
+
+def init_UHF_density(C_a, C_b, N_a, N_b):
+    """P_sigma = C_occ C_occ^T (shelled.py:549-559)."""
+    Ca = C_a[:, :int(N_a)]
+    Cb = C_b[:, :int(N_b)]
+    return Ca @ Ca.T, Cb @ Cb.T
+
+
+def UHF_density(C_a, C_b, N_a, N_b, old_P_a, old_P_b, rate):
+    """Damped density, rate * new + (1 - rate) * old (shelled.py:561-574)."""
+    P_a, P_b = init_UHF_density(C_a, C_b, N_a, N_b)
+    return rate * P_a + (1 - rate) * old_P_a, rate * P_b + (1 - rate) * old_P_b
+
+
+@dataclass
+class SCFResult:
+    energy: float
+    converged: bool
+    iterations: int
+    P_a: torch.Tensor
+    P_b: torch.Tensor
+    F_a: torch.Tensor
+    F_b: torch.Tensor
+    C_a: torch.Tensor
+    C_b: torch.Tensor
+    eps_a: torch.Tensor
+    eps_b: torch.Tensor
+    fock_times: list = field(default_factory=list)   # seconds per Fock build (device events)
+    iter_times: list = field(default_factory=list)   # seconds per full iteration (wall)
+    energies: list = field(default_factory=list)
+
+
+def electron_counts(Z, charge=0, mult=1):
+    n = int(round(float(np.sum(Z)))) - charge
+    N_a = (n + mult - 1) // 2
+    N_b = (n - mult + 1) // 2
+    return N_a, N_b
+
+
+class _Fock:
+    """J/K on preallocated device buffers, timed with HIP events."""
+
+    def __init__(self, plan: Plan, mode: str, rhf: bool):
+        self.plan, self.mode, self.rhf = plan, mode, rhf
+        n = plan.nbf
+        dev = plan.device
+        self.J = torch.empty((n, n), dtype=torch.float64, device=dev)
+        self.Ka = torch.empty_like(self.J)
+        self.Kb = None if rhf else torch.empty_like(self.J)
+        self.times = []
+
+    def __call__(self, P, P_a, P_b):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        if self.rhf:
+            self.plan.jk_into(P.contiguous(), P_a.contiguous(), None, self.J, self.Ka, None, self.mode)
+        else:
+            self.plan.jk_into(P.contiguous(), P_a.contiguous(), P_b.contiguous(), self.J, self.Ka, self.Kb,
+                              self.mode)
+        e1.record()
+        e1.synchronize()
+        self.times.append(e0.elapsed_time(e1) * 1e-3)
+        return self.J, self.Ka, (self.Ka if self.rhf else self.Kb)
+
+
+def _setup(plan: Plan, Z, centers):
+    S, T, V = plan.one_electron()
+    H = T + V
+    w, U = torch.linalg.eigh(S)
+    X = U @ torch.diag(w.rsqrt()) @ U.T
+    return S, H, X, nuclear_repulsion(Z, centers)
+
+
+def scf_plain(plan: Plan, Z, centers, N_a, N_b, mode="fp64", e_tol=1e-8, p_tol=1e-6, max_count=100,
+              p_rms_tol=None):
+    """Plain UHF loop of main.py:753-828; RHF fast path when N_a == N_b.
+
+    Exits when |dE| < e_tol and max|dP| < p_tol (and RMS dP < p_rms_tol if
+    given), or after count > max_count with ``converged`` False (the
+    reference silently exits there, main.py:806).
+    """
+    rhf = N_a == N_b and not plan.uhf
+    S, H, X, E_NN = _setup(plan, Z, centers)
+    _, Cp = torch.linalg.eigh(X.T @ H @ X)
+    C = X @ Cp
+    P_a, P_b = init_UHF_density(C, C, N_a, N_b)
+    fock = _Fock(plan, mode, rhf)
+    E_total = -1000.0
+    count = 0
+    energies, iter_times = [], []
+    while True:
+        t0 = time.perf_counter()
+        P = P_a + P_b
+        J, Ka, Kb = fock(P, P_a, P_b)
+        Fa = H + J - Ka
+        Fb = H + J - Kb
+        if not bool(torch.isfinite(Fa).all()):
+            raise FloatingPointError(f"non-finite Fock matrix at iteration {count}")
+        E_new = float(0.5 * torch.sum(P * H + P_a * Fa + P_b * Fb)) + E_NN
+        dE = abs(E_new - E_total)
+        ea, Ca = torch.linalg.eigh(X.T @ Fa @ X)
+        if rhf:
+            eb, Cb = ea, Ca
+        else:
+            eb, Cb = torch.linalg.eigh(X.T @ Fb @ X)
+        Ca, Cb = X @ Ca, X @ Cb
+        Pa_new, Pb_new = init_UHF_density(Ca, Cb, N_a, N_b)
+        dPa, dPb = Pa_new - P_a, Pb_new - P_b
+        dP = float(torch.maximum(dPa.abs().max(), dPb.abs().max()))
+        dP_rms = float(torch.sqrt(0.5 * (dPa.square().mean() + dPb.square().mean())))
+        energies.append(E_new)
+        iter_times.append(time.perf_counter() - t0)
+        ok = dE < e_tol and dP < p_tol and (p_rms_tol is None or dP_rms < p_rms_tol)
+        if ok or count > max_count:
+            return SCFResult(E_new, ok, count, P_a, P_b, Fa, Fb, Ca, Cb, ea, eb, fock.times, iter_times,
+                             energies)
+        E_total = E_new
+        P_a, P_b = Pa_new, Pb_new
+        count += 1
+
+
+def scf_fixed(plan: Plan, Z, centers, N_a, N_b, n_iter, mode="fp64"):
+    """Exactly ``n_iter`` plain Roothaan iterations (benchmark loop)."""
+    res = scf_plain(plan, Z, centers, N_a, N_b, mode=mode, e_tol=-1.0, p_tol=-1.0, max_count=n_iter - 2)
+    return res
+
+
+def scf_diis(plan: Plan, Z, centers, N_a, N_b, mode="fp64", rate=0.4, diis_threshold=1e-3, history=10,
+             e_tol=1e-8, p_tol=1e-6, max_count=500):
+    """Damped + DIIS UHF loop of shelled.py:1103-1292.
+
+    Damping rate 0.4, divided by 1.1 every fourth strike (an iteration whose
+    dE or dP grew), DIIS on the commutator residuals FPS - SPF from the
+    first iteration with dE < 1e-3, history 10, no Lagrange row and the
+    coefficients normalised to sum 1 (shelled.py:1196-1201). Unlike the
+    reference it stops after ``max_count`` iterations with converged False.
+    """
+    S, H, X, E_NN = _setup(plan, Z, centers)
+    _, Cp = torch.linalg.eigh(X.T @ H @ X)
+    C = X @ Cp
+    P_a, P_b = init_UHF_density(C, C, N_a, N_b)
+    rhf = N_a == N_b and not plan.uhf
+    fock = _Fock(plan, mode, rhf)
+    hist = {k: [] for k in ("Pa", "Pb", "Fa", "Fb", "Ra", "Rb")}
+    E_total = None
+    prev_dE = prev_dP = float("inf")
+    strikes = 0
+    use_diis = False
+    count = 0
+    energies = []
+    while True:
+        P = P_a + P_b
+        J, Ka, Kb = fock(P, P_a, P_b)
+        Fa = H + J - Ka
+        Fb = H + J - Kb
+        for k, v in (("Pa", P_a), ("Pb", P_b), ("Fa", Fa), ("Fb", Fb), ("Ra", Fa @ P_a @ S - S @ P_a @ Fa),
+                     ("Rb", Fb @ P_b @ S - S @ P_b @ Fb)):
+            hist[k].insert(0, v)
+            del hist[k][history:]
+        if use_diis:
+            Ra = torch.stack(hist["Ra"])
+            Rb = torch.stack(hist["Rb"])
+            B = torch.einsum("iab,jab->ij", Ra, Ra) + torch.einsum("iab,jab->ij", Rb, Rb)
+            c = torch.linalg.solve(B, torch.ones(B.shape[0], dtype=B.dtype, device=B.device))
+            c = c / c.sum()
+            Fa = torch.einsum("i,iab->ab", c, torch.stack(hist["Fa"]))
+            Fb = torch.einsum("i,iab->ab", c, torch.stack(hist["Fb"]))
+        E_new = float(0.5 * torch.sum(P * H + P_a * Fa + P_b * Fb)) + E_NN
+        dE = float("inf") if E_total is None else abs(E_new - E_total)
+        _, Ca = torch.linalg.eigh(X.T @ Fa @ X)
+        _, Cb = torch.linalg.eigh(X.T @ Fb @ X)
+        Ca, Cb = X @ Ca, X @ Cb
+        Pa_new, Pb_new = UHF_density(Ca, Cb, N_a, N_b, P_a, P_b, rate)
+        dP = float(torch.maximum((Pa_new - P_a).abs().max(), (Pb_new - P_b).abs().max()))
+        energies.append(E_new)
+        if (dE > prev_dE or dP > prev_dP) and not use_diis:
+            if strikes == 3:
+                rate /= 1.1
+                strikes = 0
+            else:
+                strikes += 1
+        if dE < diis_threshold:
+            use_diis = True
+        if (dE < e_tol and dP < p_tol) or count >= max_count:
+            # final rebuild from the last density, as shelled.py:1261-1285
+            P = P_a + P_b
+            J, Ka, Kb = fock(P, P_a, P_b)
+            Fa = H + J - Ka
+            Fb = H + J - Kb
+            E_fin = float(0.5 * torch.sum(P * H + P_a * Fa + P_b * Fb)) + E_NN
+            ea, Ca = torch.linalg.eigh(X.T @ Fa @ X)
+            eb, Cb = torch.linalg.eigh(X.T @ Fb @ X)
+            return SCFResult(E_fin, dE < e_tol and dP < p_tol, count, P_a, P_b, Fa, Fb, X @ Ca, X @ Cb, ea, eb,
+                             fock.times, [], energies)
+        E_total = E_new
+        prev_dE, prev_dP = dE, dP
+        P_a, P_b = Pa_new, Pb_new
+        count += 1

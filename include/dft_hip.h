@@ -1,0 +1,162 @@
+/*
+ * dft_hip.h - C ABI of libdft_hip.so, the MI355X (gfx950) Gaussian-integral
+ * and direct Fock-build library.
+ *
+ * Conventions
+ *  - Every function returns int: 0 on success, negative on error
+ *    (DFT_EINVAL bad argument, DFT_EUNSUPPORTED angular momentum or mode not
+ *    built, DFT_EHIP a HIP runtime error). The message of the last error on
+ *    the calling thread is returned by dft_last_error(). No C++ exception
+ *    crosses this boundary.
+ *  - Plan-creation inputs are host arrays. Per-iteration inputs and outputs
+ *    (densities, J, K, S, T, V, gradients) are caller-owned device pointers,
+ *    row-major nbf x nbf FP64 in the caller's Cartesian AO order.
+ *  - stream is a hipStream_t (NULL = the null stream). Per-iteration calls
+ *    only enqueue work: they neither allocate nor synchronise, so a sequence
+ *    of them can be captured into a hipGraph.
+ *  - J/K are deterministic: no atomics, a fixed-order reduction.
+ *  - Distinct plans may be used concurrently from different threads/streams;
+ *    one plan must not be used by two streams at once (it owns its staging).
+ */
+#ifndef DFT_HIP_H
+#define DFT_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DFT_OK 0
+#define DFT_EINVAL -1
+#define DFT_EUNSUPPORTED -2
+#define DFT_EHIP -3
+
+/* plan flags */
+#define DFT_PLAN_UHF 1 /* K for two densities (alpha, beta) */
+
+/* precision modes of dft_jk */
+#define DFT_FP64 0  /* every quartet in FP64 */
+#define DFT_MIXED 1 /* quartets with Schwarz bound Q_ab Q_cd < fp32_tau in FP32 */
+
+typedef struct dft_plan_s* dft_plan;
+
+/* Version of the library, as major*10000 + minor*100 + patch. */
+int dft_version(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* dft_last_error(void);
+
+/*
+ * Build the integral plan of one molecule: primitive-pair records (pair
+ * pre-pass), Schwarz bounds Q_ab = sqrt(max |(ab|ab)|), the screened,
+ * angular-momentum-class-sorted quartet lists (FP64 and FP32 parts) and the
+ * fixed-order reduction map.
+ *
+ * Replaces the one-time set-up of the reference: shell construction and
+ * normalisation (shelled.py:934-952), nuclear_kernel pair data
+ * (shelled.py:350-450 launched from :1000-1058) and the quartet enumeration
+ * of calc_eri (shelled.py:634-640).
+ *
+ *  nsh                 number of segmented shells (one coefficient row each)
+ *  shell_l[nsh]        angular momentum (0 = s, 1 = p)
+ *  shell_nprim[nsh]    primitives per shell
+ *  shell_ao[nsh]       first Cartesian AO of the shell (components follow in
+ *                      the order x, y, z for p)
+ *  shell_xyz[3*nsh]    shell centre (bohr)
+ *  prim_exp[], prim_coef[]  concatenated per shell: exponents and raw
+ *                      contraction coefficients; primitive normalisation
+ *                      (shelled.py:95-101) is applied inside
+ *  natom, atom_xyz[3*natom], atom_charge[natom]  nuclei for V
+ *  nbf                 number of Cartesian AOs
+ *  schwarz_tau         quartets with Q_ab Q_cd < schwarz_tau are skipped
+ *  fp32_tau            quartets with Q_ab Q_cd < fp32_tau form the FP32 part
+ *  flags               DFT_PLAN_UHF or 0
+ */
+int dft_plan_create(int nsh, const int32_t* shell_l, const int32_t* shell_nprim, const int32_t* shell_ao,
+                    const double* shell_xyz, const double* prim_exp, const double* prim_coef, int natom,
+                    const double* atom_xyz, const double* atom_charge, int nbf, double schwarz_tau,
+                    double fp32_tau, int flags, dft_plan* out);
+
+int dft_plan_destroy(dft_plan plan);
+
+/*
+ * Plan statistics. counts receives, for each class combination c (pair
+ * classes ss=0, ps=1, pp=2; combination index pcb*(pcb+1)/2 + pck):
+ *   counts[4*c+0] FP64 quartets, [4*c+1] FP32 quartets,
+ *   [4*c+2] primitive shell quartets of the FP64 part, [4*c+3] of the FP32 part.
+ * ncombo is the capacity of counts in combinations; the number of
+ * combinations is returned in *ncombo_out. Also reports the quartets before
+ * screening and the staging size in bytes.
+ */
+int dft_plan_stats(dft_plan plan, int64_t* counts, int ncombo, int* ncombo_out, int64_t* n_unscreened,
+                   int64_t* staging_bytes);
+
+/*
+ * Schwarz bound per input shell pair: Q[i*nsh + j] = Q[j*nsh + i] =
+ * sqrt(max_{mu in i, nu in j} |(mu nu|mu nu)|). Q is a host array nsh*nsh.
+ */
+int dft_schwarz(dft_plan plan, double* Q);
+
+/*
+ * Direct J/K build: J = sum_cd (ab|cd) P_cd, K_a = sum_bd (ab|cd) Pa_bd and,
+ * for a UHF plan, K_b from Pb. Fused ERI evaluation + digestion, then a
+ * deterministic reduction. Overwrites J, Ka, Kb (no accumulation).
+ *
+ * Replaces fill_matrices(shells, P, P_a, P_b, J, K_a, K_b)
+ * (shelled.py:770-877) together with the ERI store it reads
+ * (calc_eri/process_eri, shelled.py:596-768), and the dense J/K of
+ * main.py:787-789.
+ *
+ *  P, Pa, Pb   device nbf*nbf; Pb and Kb must be NULL for a non-UHF plan
+ *  mode        DFT_FP64 or DFT_MIXED
+ */
+int dft_jk(dft_plan plan, const double* P, const double* Pa, const double* Pb, double* J, double* Ka,
+           double* Kb, int mode, void* stream);
+
+/*
+ * Per-launch timing of the next nsteps dft_jk calls (0 disables). Creates
+ * 2*nslot HIP events per step; no device memory. Slots: 2c = class
+ * combination c, FP64 part; 2c+1 = its weak (FP32-eligible) part; then the
+ * reduction and the symmetrisation. No counterpart in the reference (it has
+ * no timing code, main.py:6).
+ */
+int dft_plan_profile(dft_plan plan, int nsteps);
+
+/*
+ * Sum over the recorded steps of each slot's duration in ms (synchronises
+ * on the recorded events). ms_sum has capacity cap; *nslot and *nsteps
+ * receive the slot count and the number of recorded steps.
+ */
+int dft_plan_profile_read(dft_plan plan, double* ms_sum, int cap, int* nslot, int* nsteps);
+
+/*
+ * Contracted, normalised ERI blocks for arbitrary shell quartets (input shell
+ * indices). quartets is a host int32 array [4*nq]; out is a host array that
+ * receives the blocks one after another, each laid out [n_a][n_b][n_c][n_d]
+ * in the requested orientation. Synchronous.
+ *
+ * Replaces eri_kernel + the normalise/contract step of calc_eri
+ * (shelled.py:452-532, 642-689).
+ */
+int dft_eri_quartets(dft_plan plan, const int32_t* quartets, int64_t nq, double* out);
+
+/*
+ * Overlap, kinetic and nuclear-attraction matrices (device nbf*nbf each).
+ * Replaces overlap/T_raw/nuclear_kernel + contraction (shelled.py:962-1069).
+ */
+int dft_oneelec(dft_plan plan, double* S, double* T, double* V, void* stream);
+
+/*
+ * Boys function test hook: out[i*(mmax+1)+m] = F_m(T[i]), m = 0..mmax, with
+ * the tabulated evaluator the ERI kernels use. T and out are device arrays.
+ * fp32 selects the FP32 recursion (FP64 seed). Replaces boys/taylor
+ * (shelled.py:174-211).
+ */
+int dft_boys(const double* T, int n, int mmax, double* out, int fp32, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DFT_HIP_H */

@@ -1,0 +1,54 @@
+"""The C-ABI library loads and exports every symbol include/dft_hip.h declares
+(no compute calls: this runs without a GPU)."""
+import ctypes
+import os
+import re
+
+from conftest import ROOT
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "dft_hip.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(dft_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_header_declares_entry_points():
+    syms = declared_symbols()
+    for name in ("dft_plan_create", "dft_jk", "dft_eri_quartets", "dft_oneelec", "dft_boys", "dft_last_error"):
+        assert name in syms
+
+
+def test_library_exports_all_declared_symbols():
+    from dft_amd import _lib
+    lib = _lib.load()
+    for name in declared_symbols():
+        assert hasattr(lib, name), name
+        assert name in _lib.SIGNATURES, f"{name} has no ctypes signature"
+
+
+def test_version_and_error_string():
+    from dft_amd import _lib
+    lib = _lib.load()
+    assert lib.dft_version() >= 100
+    # argument validation happens before any HIP call
+    rc = lib.dft_plan_create(0, None, None, None, None, None, None, 0, None, None, 0, 0.0, 0.0, 0,
+                             ctypes.byref(ctypes.c_void_p()))
+    assert rc == _lib.DFT_EINVAL
+    assert b"nsh" in lib.dft_last_error()
+
+
+def test_unsupported_angular_momentum_is_rejected():
+    import numpy as np
+    from dft_amd import _lib
+    lib = _lib.load()
+    i32 = lambda *v: np.array(v, dtype=np.int32)  # noqa: E731
+    f64 = lambda *v: np.array(v, dtype=np.float64)  # noqa: E731
+    l, npr, ao = i32(9), i32(1), i32(0)
+    xyz, ex, co = f64(0, 0, 0), f64(1.0), f64(1.0)
+    rc = lib.dft_plan_create(1, _lib.ptr(l, ctypes.c_int32), _lib.ptr(npr, ctypes.c_int32),
+                             _lib.ptr(ao, ctypes.c_int32), _lib.ptr(xyz, ctypes.c_double),
+                             _lib.ptr(ex, ctypes.c_double), _lib.ptr(co, ctypes.c_double), 0, None, None, 55,
+                             0.0, 0.0, 0, ctypes.byref(ctypes.c_void_p()))
+    assert rc == _lib.DFT_EUNSUPPORTED
+    assert b"angular momentum" in lib.dft_last_error()
