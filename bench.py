@@ -78,6 +78,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--fp64-only", action="store_true",
+                    help="only the FP64 timed region (for profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the Fock build shards by quartet blocks with one "
@@ -141,9 +143,10 @@ def main():
     ncombo = len(per)
     eri_ms = ms_slots[:2 * ncombo]
     dom = int(np.argmax(eri_ms))
-    combo, part = dom // 2, dom % 2
-    nq_dom = int(per[combo][part])
-    npq_dom = int(per[combo][2 + part])
+    combo = dom // 2
+    # FP64 mode: one launch per class combination over its whole list
+    nq_dom = int(per[combo][0] + per[combo][1])
+    npq_dom = int(per[combo][2] + per[combo][3])
     f_dom = combo_flops(combo, nq_dom, npq_dom)
     t_dom = eri_ms[dom] / nrec * 1e-3
     achieved = f_dom / t_dom / 1e12
@@ -159,6 +162,24 @@ def main():
             "whole_build_frac": f64_all / (ms_per_step * 1e-3) / FP64_PEAK,
             "slot_ms_per_step": [float(x) / nrec for x in ms_slots]}
 
+    out = {
+        "metric": "Fock-build s/iter + ERI quartets/s, FP64 vs mixed at 1 µEh; gradient s/call",
+        "value": value, "unit": "quartets/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "(H2O)16 STO-3G direct J/K (config 1): 4x2x2 lattice 3.1 A, seed-0 rotations, "
+                               "112 bf, Schwarz 1e-10", "n_quartets": quartets,
+                   "n_unscreened": st["n_unscreened"], "n_prim_quartets": st["n_prim_quartets"],
+                   "parallelism": "single GPU"},
+        "s_per_iter_fock_fp64": ms_per_step * 1e-3,
+        "plan_seconds": t_plan,
+        "roofline": roof,
+        "cpu_baseline": None,
+    }
+    if args.fp64_only:
+        print(json.dumps(out), flush=True)
+        return
+
     # mixed precision
     run(args.warmup, "mixed")
     torch.cuda.synchronize()
@@ -171,16 +192,7 @@ def main():
     scf32 = scf_plain(plan, Z, xyz, nocc, nocc, mode="mixed")
     log(f"mixed: {ms_mixed:.3f} ms/build; E64 {scf64.energy:.10f} E_mixed {scf32.energy:.10f}")
 
-    out = {
-        "metric": "Fock-build s/iter + ERI quartets/s, FP64 vs mixed at 1 µEh; gradient s/call",
-        "value": value, "unit": "quartets/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "(H2O)16 STO-3G direct J/K (config 1): 4x2x2 lattice 3.1 A, seed-0 rotations, "
-                               "112 bf, Schwarz 1e-10", "n_quartets": quartets,
-                   "n_unscreened": st["n_unscreened"], "n_prim_quartets": st["n_prim_quartets"],
-                   "parallelism": "single GPU"},
-        "s_per_iter_fock_fp64": ms_per_step * 1e-3,
+    out.update({
         "s_per_iter_fock_mixed": ms_mixed * 1e-3,
         "mixed_speedup": ms_per_step / ms_mixed,
         "mixed_fp32_quartets": st["n_quartets_fp32"],
@@ -188,10 +200,7 @@ def main():
         "abs_dE_mixed_Eh": abs(scf64.energy - scf32.energy),
         "scf_iterations_fp64": scf64.iterations,
         "scf_s_per_iter_wall": float(np.median(scf64.iter_times[1:])) if len(scf64.iter_times) > 1 else None,
-        "plan_seconds": t_plan,
-        "roofline": roof,
-        "cpu_baseline": None,
-    }
+    })
     if not args.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(shells, plan.schwarz(), tau, P.cpu().numpy(), args.cpu_seconds,

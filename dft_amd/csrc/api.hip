@@ -90,12 +90,11 @@ int upload(T** d, const std::vector<T>& h) {
 
 struct Combo {
   int pcb = 0, pck = 0;
-  std::vector<int2> q64, q32;  // (bra pair, ket pair) in plan pair ids
-  int2* d_q64 = nullptr;
-  int2* d_q32 = nullptr;
-  long long off64 = 0, off32 = 0;  // staging offsets (doubles)
+  std::vector<int2> q;         // (bra pair, ket pair) plan pair ids: FP64 part, then weak part
+  long long n64 = 0;           // size of the FP64 part
+  int2* d_q = nullptr;
+  long long* d_slots = nullptr;  // per quartet 2 + 4 nk staging offsets
   long long prim64 = 0, prim32 = 0;
-  int recsz = 0;
 };
 
 }  // namespace
@@ -124,8 +123,8 @@ struct dft_plan_s {
   float* d_rec32 = nullptr;
   double* d_stage = nullptr;
   int4* d_tg = nullptr;
-  long long* d_tb = nullptr;
-  long long* d_contrib = nullptr;
+  long long* d_tbase = nullptr;
+  int* d_tcount = nullptr;
   double* d_half = nullptr;  // (1 + nk) * nbf * nbf
   double* d_atoms = nullptr;  // x, y, z, Z
   int2* d_shpairs = nullptr;
@@ -142,11 +141,11 @@ namespace {
 void free_plan(dft_plan p) {
   if (!p) return;
   for (auto& c : p->combos) {
-    (void)hipFree(c.d_q64);
-    (void)hipFree(c.d_q32);
+    (void)hipFree(c.d_q);
+    (void)hipFree(c.d_slots);
   }
   void* ptrs[] = {p->d_sh, p->d_exps, p->d_coefs, p->d_pairs, p->d_shell_ao, p->d_rec64, p->d_rec32,
-                  p->d_stage, p->d_tg, p->d_tb, p->d_contrib, p->d_half, p->d_atoms, p->d_shpairs};
+                  p->d_stage, p->d_tg, p->d_tbase, p->d_tcount, p->d_half, p->d_atoms, p->d_shpairs};
   for (void* x : ptrs) (void)hipFree(x);
   for (hipEvent_t e : p->prof_ev) (void)hipEventDestroy(e);
   delete p;
@@ -168,7 +167,15 @@ int raw_blocks(dft_plan p, int combo, const std::vector<int2>& q, std::vector<do
     (void)hipFree(dq);
     return fail(DFT_EHIP, std::string("hipMalloc: ") + hipGetErrorString(e));
   }
-  e = launch_eri_raw(combo, dq, (long long)q.size(), p->d_pairs, p->d_rec64, dout, p->d_boys, nullptr);
+  if (lane_kernel_combo(combo)) {
+    e = launch_eri_raw(combo, dq, (long long)q.size(), p->d_pairs, p->d_rec64, dout, p->d_boys, nullptr);
+  } else {
+    int la, lb, lc, ld;
+    pc_moments(combo_pcb(combo), la, lb);
+    pc_moments(combo_pck(combo), lc, ld);
+    e = launch_eri_wave(la, lb, lc, ld, false, true, dq, nullptr, (long long)q.size(), p->d_pairs, p->d_rec64,
+                        nullptr, nullptr, nullptr, nullptr, 1, p->nbf, dout, p->d_boys, 1 << 20, nullptr);
+  }
   if (e == hipSuccess) e = hipMemcpy(out.data(), dout, out.size() * sizeof(double), hipMemcpyDeviceToHost);
   (void)hipFree(dq);
   (void)hipFree(dout);
@@ -292,21 +299,16 @@ int build_plan(dft_plan p, int nsh, const int32_t* L, const int32_t* NP, const i
       p->Q[ids[k]] = std::sqrt(m);
     }
   }
-  // ---- screened, class-sorted quartet lists
+  // ---- screened, class-sorted quartet lists (FP64 part, then weak part)
   p->combos.assign(NCOMBO, Combo());
   std::vector<std::vector<int>> class_pairs(NPC);
   for (int id = 0; id < npair; ++id) class_pairs[p->pair_pc[id]].push_back(id);
-  long long cur = 0;
   for (int pcb = 0; pcb < NPC; ++pcb)
     for (int pck = 0; pck <= pcb; ++pck) {
       Combo& c = p->combos[combo_index(pcb, pck)];
       c.pcb = pcb;
       c.pck = pck;
-      int la, lb, lc, ld;
-      pc_moments(pcb, la, lb);
-      pc_moments(pck, lc, ld);
-      const int na = ncart_h(la), nb = ncart_h(lb), ncc = ncart_h(lc), nd = ncart_h(ld);
-      c.recsz = na * nb + ncc * nd + p->nk * (na + nb) * (ncc + nd);
+      std::vector<int2> weak;
       for (int ib : class_pairs[pcb]) {
         for (int ik : class_pairs[pck]) {
           if (pcb == pck && ik > ib) break;
@@ -315,72 +317,72 @@ int build_plan(dft_plan p, int nsh, const int32_t* L, const int32_t* NP, const i
           if (b < p->tau) continue;
           const long long np = (long long)p->pairs[ib].w * p->pairs[ik].w;
           if (b < p->tau32) {
-            c.q32.push_back(make_int2(ib, ik));
+            weak.push_back(make_int2(ib, ik));
             c.prim32 += np;
           } else {
-            c.q64.push_back(make_int2(ib, ik));
+            c.q.push_back(make_int2(ib, ik));
             c.prim64 += np;
           }
         }
       }
-      c.off64 = cur;
-      cur += (long long)c.q64.size() * c.recsz;
-      c.off32 = cur;
-      cur += (long long)c.q32.size() * c.recsz;
-      if ((rc = upload(&c.d_q64, c.q64))) return rc;
-      if ((rc = upload(&c.d_q32, c.q32))) return rc;
+      c.n64 = (long long)c.q.size();
+      c.q.insert(c.q.end(), weak.begin(), weak.end());
+      if ((rc = upload(&c.d_q, c.q))) return rc;
     }
-  p->stage_len = cur;
-  // ---- reduction map: ordered shell pairs (X, Y) of each half matrix
+  // ---- target-major staging: target t = ordered shell pair (X, Y) of half
+  // matrix m (0 = J, 1.. = K per density); its blocks are contiguous and
+  // ordered by (combo, quartet), so every sum has a fixed order.
   const int nmat = 1 + p->nk;
+  const int nblk = 2 + 4 * p->nk;
   p->ntarget = nmat * nsh * nsh;
-  std::vector<long long> cnt((size_t)p->ntarget + 1, 0);
   auto tid = [&](int m, int x, int y) { return ((size_t)m * nsh + x) * nsh + y; };
-  auto visit = [&](auto&& emit) {
-    for (const Combo& c : p->combos) {
-      int la, lb, lc, ld;
-      pc_moments(c.pcb, la, lb);
-      pc_moments(c.pck, lc, ld);
-      const int na = ncart_h(la), nb = ncart_h(lb), ncc = ncart_h(lc), nd = ncart_h(ld);
-      for (int part = 0; part < 2; ++part) {
-        const std::vector<int2>& ql = part ? c.q32 : c.q64;
-        const long long base0 = part ? c.off32 : c.off64;
-        for (size_t q = 0; q < ql.size(); ++q) {
-          const int4 pb = p->pairs[ql[q].x], pk = p->pairs[ql[q].y];
-          const int A = pb.x, B = pb.y, C = pk.x, D = pk.y;
-          long long o = base0 + (long long)q * c.recsz;
-          emit(tid(0, A, B), o);
-          o += na * nb;
-          emit(tid(0, C, D), o);
-          o += ncc * nd;
-          for (int k = 0; k < p->nk; ++k) {
-            emit(tid(1 + k, A, C), o);
-            o += na * ncc;
-            emit(tid(1 + k, A, D), o);
-            o += na * nd;
-            emit(tid(1 + k, B, C), o);
-            o += nb * ncc;
-            emit(tid(1 + k, B, D), o);
-            o += nb * nd;
-          }
-        }
-      }
+  auto quartet_targets = [&](const int2& qq, size_t* t) {
+    const int4 pb = p->pairs[qq.x], pk = p->pairs[qq.y];
+    const int A = pb.x, B = pb.y, Cc = pk.x, D = pk.y;
+    t[0] = tid(0, A, B);
+    t[1] = tid(0, Cc, D);
+    for (int k = 0; k < p->nk; ++k) {
+      t[2 + 4 * k] = tid(1 + k, A, Cc);
+      t[3 + 4 * k] = tid(1 + k, A, D);
+      t[4 + 4 * k] = tid(1 + k, B, Cc);
+      t[5 + 4 * k] = tid(1 + k, B, D);
     }
   };
-  visit([&](size_t t, long long) { ++cnt[t + 1]; });
-  for (int t = 0; t < p->ntarget; ++t) cnt[t + 1] += cnt[t];
-  std::vector<long long> contrib((size_t)cnt[p->ntarget]);
-  std::vector<long long> fillp(cnt.begin(), cnt.end() - 1);
-  visit([&](size_t t, long long o) { contrib[fillp[t]++] = o; });
+  std::vector<int> tcount(p->ntarget, 0);
+  size_t tq[10];
+  for (const Combo& c : p->combos)
+    for (const int2& qq : c.q) {
+      quartet_targets(qq, tq);
+      for (int k = 0; k < nblk; ++k) ++tcount[tq[k]];
+    }
+  std::vector<long long> tbase(p->ntarget, 0), tnext(p->ntarget, 0);
   std::vector<int4> tg(p->ntarget);
+  long long cur = 0;
   for (int m = 0; m < nmat; ++m)
     for (int x = 0; x < nsh; ++x)
-      for (int y = 0; y < nsh; ++y)
-        tg[tid(m, x, y)] = make_int4(m * p->nbf + p->sh[x].ao, p->sh[y].ao, ncart_h(p->sh[x].l),
-                                     ncart_h(p->sh[y].l));
+      for (int y = 0; y < nsh; ++y) {
+        const size_t t = tid(m, x, y);
+        const int nr = ncart_h(p->sh[x].l), nc = ncart_h(p->sh[y].l);
+        tg[t] = make_int4(m * p->nbf + p->sh[x].ao, p->sh[y].ao, nr, nc);
+        tbase[t] = tnext[t] = cur;
+        cur += (long long)tcount[t] * nr * nc;
+      }
+  p->stage_len = cur;
+  for (Combo& c : p->combos) {
+    std::vector<long long> slots(c.q.size() * nblk);
+    for (size_t i = 0; i < c.q.size(); ++i) {
+      quartet_targets(c.q[i], tq);
+      for (int k = 0; k < nblk; ++k) {
+        const int4& T = tg[tq[k]];
+        slots[i * nblk + k] = tnext[tq[k]];
+        tnext[tq[k]] += (long long)T.z * T.w;
+      }
+    }
+    if ((rc = upload(&c.d_slots, slots))) return rc;
+  }
   if ((rc = upload(&p->d_tg, tg))) return rc;
-  if ((rc = upload(&p->d_tb, cnt))) return rc;
-  if ((rc = upload(&p->d_contrib, contrib))) return rc;
+  if ((rc = upload(&p->d_tbase, tbase))) return rc;
+  if ((rc = upload(&p->d_tcount, tcount))) return rc;
   HIP_TRY(hipMalloc(&p->d_stage, std::max<long long>(p->stage_len, 1) * sizeof(double)));
   HIP_TRY(hipMalloc(&p->d_half, (size_t)nmat * p->nbf * p->nbf * sizeof(double)));
   return DFT_OK;
@@ -441,8 +443,8 @@ int dft_plan_stats(dft_plan p, int64_t* counts, int ncombo, int* ncombo_out, int
   if (ncombo_out) *ncombo_out = (int)p->combos.size();
   if (counts)
     for (int c = 0; c < std::min<int>(ncombo, (int)p->combos.size()); ++c) {
-      counts[4 * c] = (int64_t)p->combos[c].q64.size();
-      counts[4 * c + 1] = (int64_t)p->combos[c].q32.size();
+      counts[4 * c] = (int64_t)p->combos[c].n64;
+      counts[4 * c + 1] = (int64_t)p->combos[c].q.size() - p->combos[c].n64;
       counts[4 * c + 2] = p->combos[c].prim64;
       counts[4 * c + 3] = p->combos[c].prim32;
     }
@@ -478,23 +480,38 @@ int dft_jk(dft_plan p, const double* P, const double* Pa, const double* Pb, doub
   auto mark = [&](int slot, int end) {
     if (ev) (void)hipEventRecord(ev[2 * slot + end], s);
   };
+  const int nblk = 2 + 4 * p->nk;
   for (size_t ci = 0; ci < p->combos.size(); ++ci) {
     const Combo& c = p->combos[ci];
+    const long long ntot = (long long)c.q.size();
+    // FP64 mode: one launch over the whole list; mixed: FP64 part, then the
+    // weak part through the FP32 kernel
+    const long long n1 = mode == DFT_MIXED ? c.n64 : ntot;
+    const bool lane = lane_kernel_combo((int)ci);
+    int la, lb, lc, ld;
+    pc_moments(c.pcb, la, lb);
+    pc_moments(c.pck, lc, ld);
+    auto launch = [&](bool f32, long long first, long long n) {
+      const void* rec = f32 ? (const void*)p->d_rec32 : (const void*)p->d_rec64;
+      if (lane)
+        return launch_eri_jk((int)ci, f32, c.d_q + first, c.d_slots + first * nblk, n, p->d_pairs, rec,
+                             p->d_shell_ao, P, Pa, Pb, p->nk, p->nbf, p->d_stage, p->d_boys, cap, s);
+      return launch_eri_wave(la, lb, lc, ld, f32, false, c.d_q + first, c.d_slots + first * nblk, n, p->d_pairs,
+                             rec, p->d_shell_ao, P, Pa, Pb, p->nk, p->nbf, p->d_stage, p->d_boys, 256 * 64, s);
+    };
     mark(2 * (int)ci, 0);
-    hipError_t e = launch_eri_jk((int)ci, false, c.d_q64, (long long)c.q64.size(), p->d_pairs, p->d_rec64,
-                                 p->d_shell_ao, P, Pa, Pb, p->nk, p->nbf, p->d_stage, c.off64, p->d_boys, cap, s);
+    hipError_t e = launch(false, 0, n1);
     if (e != hipSuccess) return fail(DFT_EHIP, std::string("eri_jk fp64: ") + hipGetErrorString(e));
     mark(2 * (int)ci, 1);
-    const bool f32 = mode == DFT_MIXED;
     mark(2 * (int)ci + 1, 0);
-    e = launch_eri_jk((int)ci, f32, c.d_q32, (long long)c.q32.size(), p->d_pairs,
-                      f32 ? (const void*)p->d_rec32 : (const void*)p->d_rec64, p->d_shell_ao, P, Pa, Pb, p->nk,
-                      p->nbf, p->d_stage, c.off32, p->d_boys, cap, s);
-    if (e != hipSuccess) return fail(DFT_EHIP, std::string("eri_jk weak part: ") + hipGetErrorString(e));
+    if (mode == DFT_MIXED) {
+      e = launch(true, c.n64, ntot - c.n64);
+      if (e != hipSuccess) return fail(DFT_EHIP, std::string("eri_jk fp32: ") + hipGetErrorString(e));
+    }
     mark(2 * (int)ci + 1, 1);
   }
   mark(nslot - 2, 0);
-  hipError_t e = launch_reduce(p->ntarget, p->d_tg, p->d_tb, p->d_contrib, p->d_stage, p->d_half, p->nbf, s);
+  hipError_t e = launch_reduce(p->ntarget, p->d_tg, p->d_tbase, p->d_tcount, p->d_stage, p->d_half, p->nbf, s);
   if (e != hipSuccess) return fail(DFT_EHIP, std::string("reduce: ") + hipGetErrorString(e));
   mark(nslot - 2, 1);
   mark(nslot - 1, 0);

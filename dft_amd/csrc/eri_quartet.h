@@ -21,6 +21,14 @@
 
 namespace dft {
 
+// E_t^{ia,ib} of one axis from a pair record; E_0^{00} = 1 by construction
+// (the exp(-mu X_AB^2) factor lives in K_ab), so that entry is never loaded.
+template <int LB, int IA, int IB, int T, typename real>
+__device__ __forceinline__ real e_at(const real* __restrict__ E) {
+  if constexpr (IA == 0 && IB == 0) return real(1);
+  else return E[eoff(LB, IA, IB) + T];
+}
+
 constexpr double TWO_PI_2P5 = 34.986836655249725693;  // 2 pi^{5/2}
 
 template <int LA, int LB, int LC, int LD, typename real>
@@ -70,7 +78,6 @@ __device__ __forceinline__ void quartet_integrals(const real* __restrict__ recb,
         constexpr int c = cd / S::ND, d = cd % S::ND;
         constexpr int cx = comp_x(LC, c), cy = comp_y(LC, c), cz = comp_z(LC, c);
         constexpr int dx = comp_x(LD, d), dy = comp_y(LD, d), dz = comp_z(LD, d);
-        constexpr int ox = eoff(LD, cx, dx), oy = eoff(LD, cy, dy), oz = eoff(LD, cz, dz);
         // E^cd products with the (-1)^{tau+nu+phi} sign and pref folded in
         real Ek[(cx + dx + 1) * (cy + dy + 1) * (cz + dz + 1)];
         static_for<0, cx + dx + 1>([&](auto tI) {
@@ -78,7 +85,7 @@ __device__ __forceinline__ void quartet_integrals(const real* __restrict__ recb,
             static_for<0, cz + dz + 1>([&](auto vI) {
               constexpr int ta = decltype(tI)::value, nu = decltype(uI)::value, ph = decltype(vI)::value;
               constexpr int k = (ta * (cy + dy + 1) + nu) * (cz + dz + 1) + ph;
-              real e = Exk[ox + ta] * Eyk[oy + nu] * Ezk[oz + ph] * pref;
+              real e = e_at<LD, cx, dx, ta>(Exk) * e_at<LD, cy, dy, nu>(Eyk) * e_at<LD, cz, dz, ph>(Ezk) * pref;
               Ek[k] = ((ta + nu + ph) & 1) ? -e : e;
             });
           });
@@ -108,13 +115,12 @@ __device__ __forceinline__ void quartet_integrals(const real* __restrict__ recb,
       constexpr int a = ab / S::NB, b = ab % S::NB;
       constexpr int ax = comp_x(LA, a), ay = comp_y(LA, a), az = comp_z(LA, a);
       constexpr int bx = comp_x(LB, b), by = comp_y(LB, b), bz = comp_z(LB, b);
-      constexpr int ox = eoff(LB, ax, bx), oy = eoff(LB, ay, by), oz = eoff(LB, az, bz);
       static_for<0, ax + bx + 1>([&](auto tI) {
         static_for<0, ay + by + 1>([&](auto uI) {
           static_for<0, az + bz + 1>([&](auto vI) {
             constexpr int t = decltype(tI)::value, u = decltype(uI)::value, v = decltype(vI)::value;
             constexpr int h = hidx(t, u, v);
-            const real e = Exb[ox + t] * Eyb[oy + u] * Ezb[oz + v];
+            const real e = e_at<LB, ax, bx, t>(Exb) * e_at<LB, ay, by, u>(Eyb) * e_at<LB, az, bz, v>(Ezb);
             static_for<0, S::NCD>([&](auto cdI) {
               constexpr int cd = decltype(cdI)::value;
               I[ab * S::NCD + cd] = fma(e, G[h * S::NCD + cd], I[ab * S::NCD + cd]);

@@ -6,8 +6,8 @@
 
 namespace dft {
 
-// number of pair classes / class combinations for shells up to l = 1 (s, p)
-constexpr int LMAX = 1;
+// number of pair classes / class combinations for shells up to l = 2 (s, p, d)
+constexpr int LMAX = 2;
 constexpr int NPC = (LMAX + 1) * (LMAX + 2) / 2;   // pair classes: ss, ps, pp
 constexpr int NCOMBO = NPC * (NPC + 1) / 2;        // (bra class >= ket class)
 
@@ -18,6 +18,12 @@ inline void pc_moments(int pc, int& la, int& lb) {
   lb = pc - la * (la + 1) / 2;
 }
 inline int combo_index(int pcb, int pck) { return pcb * (pcb + 1) / 2 + pck; }
+inline int combo_pcb(int combo) {
+  int b = 0;
+  while ((b + 1) * (b + 2) / 2 <= combo) ++b;
+  return b;
+}
+inline int combo_pck(int combo) { return combo - combo_pcb(combo) * (combo_pcb(combo) + 1) / 2; }
 
 struct ShellDev {      // one contracted (segmented) shell, internal order
   int l, nprim, prim_off, ao;
@@ -30,21 +36,33 @@ hipError_t launch_pair_records(const int4* pairs_prim /* sh_a, sh_b, i_a, i_b */
                                const ShellDev* shells, const double* exps, const double* coefs,
                                double* rec64, float* rec32, hipStream_t s);
 
-// fused ERI + J/K digest for one class combination.
-hipError_t launch_eri_jk(int combo, bool fp32, const int2* quartets, long long nq,
+// fused ERI + J/K digest for one class combination; slots holds 2 + 4*nk
+// staging offsets (doubles) per quartet.
+hipError_t launch_eri_jk(int combo, bool fp32, const int2* quartets, const long long* slots, long long nq,
                          const int4* pairs, const void* rec, const int* shell_ao,
                          const double* DJ, const double* DK0, const double* DK1, int nk, int nbf,
-                         double* stage, long long stage_off, const double* boys_tab,
-                         int grid_cap, hipStream_t s);
+                         double* stage, const double* boys_tab, int grid_cap, hipStream_t s);
+
+// true if the lane-per-quartet kernel is instantiated for this combination
+// (s/p classes and the light d classes); the others use the wave kernel
+bool lane_kernel_combo(int combo);
+
+// one-wave-per-quartet kernel (any class up to (dd|dd)); raw = write the
+// integral blocks to out + q * NA*NB*NC*ND instead of J/K contributions
+hipError_t launch_eri_wave(int la, int lb, int lc, int ld, bool fp32, bool raw, const int2* quartets,
+                           const long long* slots, long long nq, const int4* pairs, const void* rec,
+                           const int* shell_ao, const double* DJ, const double* DK0, const double* DK1, int nk,
+                           int nbf, double* out, const double* boys_tab, int grid_cap, hipStream_t s);
 
 // raw contracted blocks (internal orientation) for one class combination:
 // out + q * NA*NB*NC*ND
 hipError_t launch_eri_raw(int combo, const int2* quartets, long long nq, const int4* pairs,
                           const double* rec, double* out, const double* boys_tab, hipStream_t s);
 
-// deterministic target reduction: one wave per target block
+// deterministic target reduction: one wave per target block; the blocks of
+// target t are contiguous: stage[tgt_base[t] + k * nrow * ncol + e], k < tgt_count[t]
 hipError_t launch_reduce(int ntarget, const int4* targets /* row0, col0, nrow, ncol */,
-                         const long long* tgt_begin, const long long* contrib, const double* stage,
+                         const long long* tgt_base, const int* tgt_count, const double* stage,
                          double* half, int nbf, hipStream_t s);
 
 // M = H + H^T for nmat matrices of nbf x nbf

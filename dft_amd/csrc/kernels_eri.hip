@@ -8,24 +8,25 @@ namespace dft {
 constexpr int ERI_BLOCK = 256;
 
 // Each lane evaluates one canonical quartet (bra pair >= ket pair in the
-// class-sorted pair order) and writes its contribution record
-//   [Jab | Jcd | (Kac | Kad | Kbc | Kbd) x nk]
-// to stage + stage_off + q * RECSZ. With w = 2^-([a=b]+[c=d]+[ab=cd]):
+// class-sorted pair order) and writes its contribution blocks
+//   Jab, Jcd, then per exchange density Kac, Kad, Kbc, Kbd
+// to stage + slots[q * NBLK + k]. With w = 2^-([a=b]+[c=d]+[ab=cd]):
 //   Jab = 2w sum_cd I P_cd        Jcd = 2w sum_ab I P_ab
 //   Kac = w sum_bd I P_bd  Kad = w sum_bc I P_bc  Kbc = w sum_ad I P_ad  Kbd = w sum_ac I P_ac
-// and the half matrices Jh, Kh summed over all records give J = Jh + Jh^T and
+// and the half matrices Jh, Kh summed over all blocks give J = Jh + Jh^T and
 // K = Kh + Kh^T, i.e. the eight orientations and degeneracy weights of
-// fill_matrices (shelled.py:791-859). No atomics: every record has its own
-// slot and the reduction over slots is a separate, fixed-order pass.
+// fill_matrices (shelled.py:791-859). No atomics: the slots of one target
+// (ordered shell pair of a half matrix) are contiguous, assigned in quartet
+// order at plan time, and summed by a separate fixed-order pass.
 template <int LA, int LB, int LC, int LD, typename real, int NK>
 __global__ __launch_bounds__(ERI_BLOCK) void eri_jk_kernel(
-    const int2* __restrict__ quartets, long long nq, const int4* __restrict__ pairs,
-    const real* __restrict__ rec, const int* __restrict__ shell_ao, const double* __restrict__ DJ,
-    const double* __restrict__ DK0, const double* __restrict__ DK1, int nbf,
-    double* __restrict__ stage, long long stage_off, const double* __restrict__ boys_gtab) {
+    const int2* __restrict__ quartets, const long long* __restrict__ slots, long long nq,
+    const int4* __restrict__ pairs, const real* __restrict__ rec, const int* __restrict__ shell_ao,
+    const double* __restrict__ DJ, const double* __restrict__ DK0, const double* __restrict__ DK1, int nbf,
+    double* __restrict__ stage, const double* __restrict__ boys_gtab) {
   using S = QuartetShape<LA, LB, LC, LD, real>;
   constexpr int NA = S::NA, NB = S::NB, NC = S::NC, ND = S::ND;
-  constexpr int RECSZ = S::NAB + S::NCD + NK * (NA + NB) * (NC + ND);
+  constexpr int NBLK = 2 + 4 * NK;
   __shared__ double btab[BOYS_NG * BOYS_TAYLOR];
   boys_stage<S::L>(boys_gtab, btab);
   const long long stride = (long long)gridDim.x * blockDim.x;
@@ -41,7 +42,7 @@ __global__ __launch_bounds__(ERI_BLOCK) void eri_jk_kernel(
     if (qd.x == qd.y) w *= 0.5;
     const int a0 = shell_ao[pb.x], b0 = shell_ao[pb.y];
     const int c0 = shell_ao[pk.x], d0 = shell_ao[pk.y];
-    double* out = stage + stage_off + q * RECSZ;
+    const long long* sl = slots + q * NBLK;
     // Coulomb
     {
       double Pcd[S::NCD], Pab[S::NAB];
@@ -65,16 +66,15 @@ __global__ __launch_bounds__(ERI_BLOCK) void eri_jk_kernel(
           s = fma(v, Pcd[cd], s);
           jcd[cd] = fma(v, Pab[ab], jcd[cd]);
         }
-        out[ab] = 2.0 * w * s;
+        stage[sl[0] + ab] = 2.0 * w * s;
       }
 #pragma unroll
-      for (int cd = 0; cd < S::NCD; ++cd) out[S::NAB + cd] = 2.0 * w * jcd[cd];
+      for (int cd = 0; cd < S::NCD; ++cd) stage[sl[1] + cd] = 2.0 * w * jcd[cd];
     }
     // Exchange, one density at a time
 #pragma unroll
     for (int k = 0; k < NK; ++k) {
       const double* D = k == 0 ? DK0 : DK1;
-      double* o = out + S::NAB + S::NCD + k * (NA + NB) * (NC + ND);
       double Pbd[NB * ND], Pbc[NB * NC], Pad[NA * ND], Pac[NA * NC];
 #pragma unroll
       for (int b = 0; b < NB; ++b) {
@@ -113,17 +113,15 @@ __global__ __launch_bounds__(ERI_BLOCK) void eri_jk_kernel(
               kbc[b * NC + c] = fma(v, Pad[a * ND + d], kbc[b * NC + c]);
               kbd[b * ND + d] = fma(v, Pac[a * NC + c], kbd[b * ND + d]);
             }
+      const long long* ks = sl + 2 + 4 * k;
 #pragma unroll
-      for (int x = 0; x < NA * NC; ++x) o[x] = w * kac[x];
-      o += NA * NC;
+      for (int x = 0; x < NA * NC; ++x) stage[ks[0] + x] = w * kac[x];
 #pragma unroll
-      for (int x = 0; x < NA * ND; ++x) o[x] = w * kad[x];
-      o += NA * ND;
+      for (int x = 0; x < NA * ND; ++x) stage[ks[1] + x] = w * kad[x];
 #pragma unroll
-      for (int x = 0; x < NB * NC; ++x) o[x] = w * kbc[x];
-      o += NB * NC;
+      for (int x = 0; x < NB * NC; ++x) stage[ks[2] + x] = w * kbc[x];
 #pragma unroll
-      for (int x = 0; x < NB * ND; ++x) o[x] = w * kbd[x];
+      for (int x = 0; x < NB * ND; ++x) stage[ks[3] + x] = w * kbd[x];
     }
   }
 }
@@ -158,17 +156,17 @@ static int grid_for(long long nq, int cap) {
 }
 
 template <int LA, int LB, int LC, int LD, typename real>
-static hipError_t jk_one(const int2* q, long long nq, const int4* pairs, const void* rec, const int* ao,
-                         const double* DJ, const double* DK0, const double* DK1, int nk, int nbf,
-                         double* stage, long long off, const double* bt, int cap, hipStream_t s) {
+static hipError_t jk_one(const int2* q, const long long* sl, long long nq, const int4* pairs, const void* rec,
+                         const int* ao, const double* DJ, const double* DK0, const double* DK1, int nk, int nbf,
+                         double* stage, const double* bt, int cap, hipStream_t s) {
   if (nq <= 0) return hipSuccess;
   dim3 grid(grid_for(nq, cap)), block(ERI_BLOCK);
   if (nk == 1)
-    hipLaunchKernelGGL((eri_jk_kernel<LA, LB, LC, LD, real, 1>), grid, block, 0, s, q, nq, pairs,
-                       (const real*)rec, ao, DJ, DK0, DK1, nbf, stage, off, bt);
+    hipLaunchKernelGGL((eri_jk_kernel<LA, LB, LC, LD, real, 1>), grid, block, 0, s, q, sl, nq, pairs,
+                       (const real*)rec, ao, DJ, DK0, DK1, nbf, stage, bt);
   else
-    hipLaunchKernelGGL((eri_jk_kernel<LA, LB, LC, LD, real, 2>), grid, block, 0, s, q, nq, pairs,
-                       (const real*)rec, ao, DJ, DK0, DK1, nbf, stage, off, bt);
+    hipLaunchKernelGGL((eri_jk_kernel<LA, LB, LC, LD, real, 2>), grid, block, 0, s, q, sl, nq, pairs,
+                       (const real*)rec, ao, DJ, DK0, DK1, nbf, stage, bt);
   return hipGetLastError();
 }
 
@@ -181,26 +179,45 @@ static hipError_t raw_one(const int2* q, long long nq, const int4* pairs, const 
   return hipGetLastError();
 }
 
-// combo index = pcb (pcb+1)/2 + pck over pair classes ss=0, ps=1, pp=2
+// combo index = pcb (pcb+1)/2 + pck over pair classes ss=0, ps=1, pp=2,
+// ds=3, dp=4, dd=5; the lane kernel is built for the s/p combinations and
+// the d combinations whose integrals, G and R fit a lane's registers
 #define DFT_FOR_COMBOS(X) \
   X(0, 0, 0, 0, 0)        \
   X(1, 1, 0, 0, 0)        \
   X(2, 1, 0, 1, 0)        \
   X(3, 1, 1, 0, 0)        \
   X(4, 1, 1, 1, 0)        \
-  X(5, 1, 1, 1, 1)
+  X(5, 1, 1, 1, 1)        \
+  X(6, 2, 0, 0, 0)        \
+  X(7, 2, 0, 1, 0)        \
+  X(9, 2, 0, 2, 0)        \
+  X(10, 2, 1, 0, 0)       \
+  X(15, 2, 2, 0, 0)
 
-hipError_t launch_eri_jk(int combo, bool fp32, const int2* q, long long nq, const int4* pairs,
-                         const void* rec, const int* ao, const double* DJ, const double* DK0,
-                         const double* DK1, int nk, int nbf, double* stage, long long off,
-                         const double* bt, int cap, hipStream_t s) {
+bool lane_kernel_combo(int combo) {
+  switch (combo) {
+#define X(id, la, lb, lc, ld) \
+  case id:                    \
+    return true;
+    DFT_FOR_COMBOS(X)
+#undef X
+    default:
+      return false;
+  }
+}
+
+hipError_t launch_eri_jk(int combo, bool fp32, const int2* q, const long long* sl, long long nq,
+                         const int4* pairs, const void* rec, const int* ao, const double* DJ, const double* DK0,
+                         const double* DK1, int nk, int nbf, double* stage, const double* bt, int cap,
+                         hipStream_t s) {
   switch (combo) {
 #define X(id, la, lb, lc, ld)                                                                          \
   case id:                                                                                             \
-    return fp32 ? jk_one<la, lb, lc, ld, float>(q, nq, pairs, rec, ao, DJ, DK0, DK1, nk, nbf, stage,  \
-                                                 off, bt, cap, s)                                      \
-                : jk_one<la, lb, lc, ld, double>(q, nq, pairs, rec, ao, DJ, DK0, DK1, nk, nbf, stage, \
-                                                  off, bt, cap, s);
+    return fp32 ? jk_one<la, lb, lc, ld, float>(q, sl, nq, pairs, rec, ao, DJ, DK0, DK1, nk, nbf, stage, \
+                                                 bt, cap, s)                                           \
+                : jk_one<la, lb, lc, ld, double>(q, sl, nq, pairs, rec, ao, DJ, DK0, DK1, nk, nbf, stage, \
+                                                  bt, cap, s);
     DFT_FOR_COMBOS(X)
 #undef X
     default:

@@ -86,34 +86,36 @@ hipError_t launch_pair_records(const int4* pp, int n, const int* rec_off, const 
 
 // ------------------------------------------------- fixed-order reduction
 // One 64-lane wave per target block (an ordered shell pair of one half
-// matrix). Lane l sums contributions l, l+64, ... in order, then a xor
-// butterfly combines the lanes; the summation tree is fixed, so J and K are
-// bitwise reproducible run to run (the reference's add.at scatter is not:
+// matrix). The target's contribution blocks are contiguous in the staging
+// buffer; lane l sums blocks l, l+64, ... in order, then a xor butterfly
+// combines the lanes. The summation tree is fixed, so J and K are bitwise
+// reproducible run to run (the reference's add.at scatter is not:
 // shelled.py:857-859).
-__global__ void reduce_kernel(int ntarget, const int4* __restrict__ tg, const long long* __restrict__ tb,
-                              const long long* __restrict__ contrib, const double* __restrict__ stage,
+__global__ void reduce_kernel(int ntarget, const int4* __restrict__ tg, const long long* __restrict__ tbase,
+                              const int* __restrict__ tcount, const double* __restrict__ stage,
                               double* __restrict__ half, int nbf) {
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   if (wave >= ntarget) return;
   const int4 t = tg[wave];
-  const long long b = tb[wave], e = tb[wave + 1];
   const int ne = t.z * t.w;
+  const double* src = stage + tbase[wave];
+  const int cnt = tcount[wave];
   for (int el = 0; el < ne; ++el) {
     double s = 0.0;
-    for (long long k = b + lane; k < e; k += 64) s += stage[contrib[k] + el];
+    for (int k = lane; k < cnt; k += 64) s += src[(long long)k * ne + el];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
     if (lane == 0) half[(long long)(t.x + el / t.w) * nbf + t.y + el % t.w] = s;
   }
 }
 
-hipError_t launch_reduce(int ntarget, const int4* tg, const long long* tb, const long long* contrib,
+hipError_t launch_reduce(int ntarget, const int4* tg, const long long* tbase, const int* tcount,
                          const double* stage, double* half, int nbf, hipStream_t s) {
   if (ntarget <= 0) return hipSuccess;
   const int waves_per_block = 4;
   hipLaunchKernelGGL(reduce_kernel, dim3((ntarget + waves_per_block - 1) / waves_per_block),
-                     dim3(64 * waves_per_block), 0, s, ntarget, tg, tb, contrib, stage, half, nbf);
+                     dim3(64 * waves_per_block), 0, s, ntarget, tg, tbase, tcount, stage, half, nbf);
   return hipGetLastError();
 }
 

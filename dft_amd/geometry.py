@@ -149,3 +149,10 @@ def batch_of_clusters(n_mol=64, seed=2, lattice=(3, 3, 3), spacing=3.1, nmax=24)
         sites = rng.choice(nsite, size=n, replace=False)
         mols.append(water_cluster(lattice, spacing, sites=sites, rng=rng))
     return mols
+
+
+def methane():
+    """CH4, tetrahedral, C-H 1.09 A (Z list, bohr)."""
+    d = 1.09 / np.sqrt(3.0)
+    xyz = np.array([[0, 0, 0], [d, d, d], [-d, -d, d], [-d, d, -d], [d, -d, -d]])
+    return [6, 1, 1, 1, 1], xyz * BOHR_PER_ANG

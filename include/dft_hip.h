@@ -59,10 +59,11 @@ const char* dft_last_error(void);
  * of calc_eri (shelled.py:634-640).
  *
  *  nsh                 number of segmented shells (one coefficient row each)
- *  shell_l[nsh]        angular momentum (0 = s, 1 = p)
+ *  shell_l[nsh]        angular momentum (0 = s, 1 = p, 2 = d)
  *  shell_nprim[nsh]    primitives per shell
  *  shell_ao[nsh]       first Cartesian AO of the shell (components follow in
- *                      the order x, y, z for p)
+ *                      the order x, y, z for p; xx, xy, yy, xz, yz, zz for d,
+ *                      as combinations() of shelled.py:15-23)
  *  shell_xyz[3*nsh]    shell centre (bohr)
  *  prim_exp[], prim_coef[]  concatenated per shell: exponents and raw
  *                      contraction coefficients; primitive normalisation
@@ -82,7 +83,8 @@ int dft_plan_destroy(dft_plan plan);
 
 /*
  * Plan statistics. counts receives, for each class combination c (pair
- * classes ss=0, ps=1, pp=2; combination index pcb*(pcb+1)/2 + pck):
+ * classes ss=0, ps=1, pp=2, ds=3, dp=4, dd=5; combination index
+ * pcb*(pcb+1)/2 + pck):
  *   counts[4*c+0] FP64 quartets, [4*c+1] FP32 quartets,
  *   [4*c+2] primitive shell quartets of the FP64 part, [4*c+3] of the FP32 part.
  * ncombo is the capacity of counts in combinations; the number of

@@ -60,8 +60,42 @@ def molecule_fixture(name, Z, xyz):
     print(name, res["E"], res["count"])
 
 
-if __name__ == "__main__":
+if __name__ == "__main__" and len(sys.argv) == 1:
     boys_fixture()
     molecule_fixture("water_sto3g.npz", *water())
     Z, xyz = water_cluster()
     molecule_fixture("dimer_sto3g.npz", Z[:6], xyz[:6])
+
+
+def methane_spd_fixture():
+    """CH4 with the synthetic s/p/d basis rules of the C12H26 workload (seed 1).
+
+    Reference values from the C restatement (oracle/md_ref.c): full ERI tensor
+    (Cartesian), and J/K for two random symmetric densities.
+    """
+    import oracle.ref as R
+    from dft_amd.basis import build_shells_from_entries
+    from dft_amd.geometry import methane, synthetic_spd_basis
+    Z, xyz = methane()
+    shells = build_shells_from_entries(synthetic_spd_basis(Z, seed=1), xyz)
+    seg = R.Segments(shells)
+    q = R.unique_quartets(seg.n)
+    flat, sizes = R.eri_blocks(seg, q)
+    n = seg.nbf
+    ERI = np.zeros((n, n, n, n))
+    pos = 0
+    for (a, b, c, d), sz in zip(q, sizes):
+        nc = [(l + 1) * (l + 2) // 2 for l in seg.l[[a, b, c, d]]]
+        blk = flat[pos:pos + sz].reshape(nc)
+        pos += sz
+        s = [slice(seg.ao[k], seg.ao[k] + m) for k, m in zip((a, b, c, d), nc)]
+        for perm in ((0, 1, 2, 3), (1, 0, 2, 3), (0, 1, 3, 2), (1, 0, 3, 2),
+                     (2, 3, 0, 1), (3, 2, 0, 1), (2, 3, 1, 0), (3, 2, 1, 0)):
+            ERI[s[perm[0]], s[perm[1]], s[perm[2]], s[perm[3]]] = blk.transpose(perm)
+    S, T, V = O.one_electron(shells, Z, xyz)
+    np.savez_compressed(os.path.join(OUT, "methane_spd.npz"), Z=np.array(Z), xyz=xyz, ERI=ERI, S=S, T=T, V=V)
+    print("methane_spd", n, "bf", len(q), "quartets")
+
+
+if __name__ == "__main__" and "methane" in sys.argv:
+    methane_spd_fixture()

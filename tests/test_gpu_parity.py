@@ -134,3 +134,62 @@ def test_schwarz_bound_holds(torch_cuda):
             sb = slice(shells[b]["ao_start"], shells[b]["ao_stop"])
             diag = np.abs(np.einsum("ijij->ij", ERI[sa, sb, sa, sb])).max()
             assert abs(Q[a, b] - np.sqrt(diag)) < 1e-12
+
+
+def _methane_plan(uhf=False, tau=0.0, fp32_tau=0.0):
+    from dft_amd.basis import build_shells_from_entries
+    from dft_amd.engine import Plan
+    from dft_amd.geometry import methane, synthetic_spd_basis
+    Z, xyz = methane()
+    shells = build_shells_from_entries(synthetic_spd_basis(Z, seed=1), xyz)
+    return shells, Plan(shells, Z, xyz, schwarz_tau=tau, fp32_tau=fp32_tau, uhf=uhf)
+
+
+def test_eri_tensor_spd_vs_golden(torch_cuda):
+    g = golden("methane_spd.npz")
+    _, plan = _methane_plan()
+    T = plan.eri_tensor()
+    ref = g["ERI"]
+    err = np.abs(T - ref) / (1e-10 + 1e-11 * np.abs(ref))
+    assert err.max() < 1.0, (np.abs(T - ref).max(), np.abs(ref).max())
+
+
+def test_one_electron_spd_vs_golden(torch_cuda):
+    g = golden("methane_spd.npz")
+    _, plan = _methane_plan()
+    S, T, V = (x.cpu().numpy() for x in plan.one_electron())
+    for got, ref in ((S, g["S"]), (T, g["T"]), (V, g["V"])):
+        assert np.abs(got - ref).max() < 1e-10 + 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("uhf", [False, True])
+def test_jk_spd_vs_dense(torch_cuda, uhf):
+    import oracle.md_oracle as O
+    g = golden("methane_spd.npz")
+    ERI = g["ERI"]
+    n = ERI.shape[0]
+    rng = np.random.default_rng(13)
+    Pa, Pb = _rand_density(n, rng) * 0.1, _rand_density(n, rng) * 0.1
+    _, plan = _methane_plan(uhf=uhf)
+    J, Ka, Kb = plan.jk(Pa + Pb, Pa, Pb if uhf else None)
+    Jr, Kar, Kbr = O.jk_dense(ERI, Pa + Pb, Pa, Pb)
+    scale = 1e-10 * max(1.0, np.abs(Jr).max())
+    assert np.abs(J.cpu().numpy() - Jr).max() < scale
+    assert np.abs(Ka.cpu().numpy() - Kar).max() < scale
+    if uhf:
+        assert np.abs(Kb.cpu().numpy() - Kbr).max() < scale
+
+
+def test_mixed_precision_spd_error_small(torch_cuda):
+    g = golden("methane_spd.npz")
+    n = g["ERI"].shape[0]
+    rng = np.random.default_rng(17)
+    Pa = _rand_density(n, rng) * 0.05
+    _, plan = _methane_plan(tau=1e-12, fp32_tau=1e-4)
+    st = plan.stats()
+    assert st["n_quartets_fp32"] > 0
+    J64, K64, _ = plan.jk(2 * Pa, Pa, mode="fp64")
+    J32, K32, _ = plan.jk(2 * Pa, Pa, mode="mixed")
+    # FP32 only on quartets with Q_ab Q_cd < 1e-4: error far below the largest element
+    assert (J64 - J32).abs().max().item() < 1e-6 * max(1.0, J64.abs().max().item())
+    assert (K64 - K32).abs().max().item() < 1e-6 * max(1.0, K64.abs().max().item())

@@ -202,3 +202,36 @@ def test_spherical_transform_orthonormal_f():
     S, _, _ = O.one_electron(sh, [1], np.zeros((1, 3)))
     A = O.independent(sh, 7, np.eye(10))
     assert np.allclose(A @ S @ A.T, np.eye(7), atol=1e-12)
+
+
+def test_centre_derivative_generates_d():
+    # d/dA_x (px_A s_B|s_C s_D) = 2a (dxx_A s_B|..) - (s_A s_B|..), unnormalised primitives
+    a, b, c, d = 0.7, 1.2, 0.9, 0.5
+    A, B, Cc, D = np.array([0.2, -0.1, 0.4]), np.array([0.6, 0.3, -0.2]), np.array([-0.4, 0.5, 0.7]), \
+        np.array([0.1, -0.6, 0.2])
+    h = 1e-5
+
+    def raw(l, Ax):
+        s = [_prim_shell(l, a, Ax)] + [_prim_shell(0, e, r) for e, r in ((b, B), (c, Cc), (d, D))]
+        return O.eri_prim(O.pair_data(s[0], s[1]), O.pair_data(s[2], s[3]))[:, 0]
+    ex = np.array([1.0, 0, 0])
+    fd = (raw(1, A + h * ex)[0] - raw(1, A - h * ex)[0]) / (2 * h)   # component px
+    dxx = raw(2, A)[0]                                             # component xx
+    ss = raw(0, A)[0]
+    assert np.isclose(fd, 2 * a * dxx - ss, rtol=1e-7, atol=1e-12)
+
+
+def test_c_oracle_matches_numpy_oracle_d():
+    import oracle.ref as R
+    from dft_amd.basis import build_shells_from_entries
+    ent = [[[[2], [0.9, 3.0], [[0.6, 0.8]]], [[1], [1.3], [[1.0]]]], [[[0], [0.5, 2.0], [[0.7, 0.7]]]]]
+    sh = build_shells_from_entries(ent, [[0, 0, 0], [0.5, 0.9, -1.1]])
+    seg = R.Segments(sh)
+    q = R.unique_quartets(seg.n)
+    flat, sizes = R.eri_blocks(seg, q)
+    pairs = O.all_pairs(sh)
+    pos = 0
+    for (a, b, c, d), sz in zip(q, sizes):
+        ref = O.shell_quartet(sh, a, b, c, d, pairs).ravel()
+        assert np.allclose(flat[pos:pos + sz], ref, rtol=1e-12, atol=1e-14)
+        pos += sz
