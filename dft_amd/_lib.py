@@ -34,6 +34,10 @@ SIGNATURES = {
     "dft_eri_quartets": (C.c_int, [_vp, _i32p, C.c_int64, _dp]),
     "dft_oneelec": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "dft_boys": (C.c_int, [_vp, C.c_int, C.c_int, _vp, C.c_int, _vp]),
+    "dft_nuclear_prim": (C.c_int, [C.c_int, C.c_int, _vp, _vp, _vp, C.c_int, _vp, C.c_int, _vp, _vp, C.c_int,
+                                   _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "dft_eri_prim": (C.c_int, [C.c_int64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int64, _vp, _vp, _vp, _vp,
+                               _vp, _vp, _vp, _vp, C.c_int, _vp, _vp]),
     "dft_plan_profile": (C.c_int, [_vp, C.c_int]),
     "dft_plan_profile_read": (C.c_int, [_vp, _dp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
 }

@@ -635,6 +635,38 @@ int dft_oneelec(dft_plan p, double* S, double* T, double* V, void* stream) {
   return DFT_OK;
 }
 
+int dft_nuclear_prim(int nfa, int nfb, const double* exp_a, const double* exp_b, const double* cen_a,
+                     int cen_a_stride, const double* cen_b, int cen_b_stride, const int32_t* pow_a,
+                     const int32_t* pow_b, int natom, const int32_t* atoms, const double* atom_xyz, double* V,
+                     double* Ex, double* Ey, double* Ez, double* p, double* P, int32_t* nx, int32_t* ny, int32_t* nz,
+                     void* stream) {
+  if (nfa < 0 || nfb < 0 || natom < 0) return fail(DFT_EINVAL, "negative size");
+  if ((cen_a_stride != 0 && cen_a_stride != 3) || (cen_b_stride != 0 && cen_b_stride != 3))
+    return fail(DFT_EINVAL, "centre stride must be 0 (one centre) or 3 (per function)");
+  double* tab = nullptr;
+  int rc = boys_table_device(&tab);
+  if (rc) return rc;
+  hipError_t e = launch_nuclear_prim(nfa, nfb, exp_a, exp_b, cen_a, cen_a_stride, cen_b, cen_b_stride, pow_a, pow_b,
+                                     natom, atoms, atom_xyz, V, Ex, Ey, Ez, p, P, nx, ny, nz, tab, (hipStream_t)stream);
+  if (e != hipSuccess) return fail(DFT_EHIP, std::string("nuclear_prim: ") + hipGetErrorString(e));
+  return DFT_OK;
+}
+
+int dft_eri_prim(int64_t n_ab, const double* p_ab, const double* P_ab, const double* Ex_ab, const double* Ey_ab,
+                 const double* Ez_ab, const int32_t* nx_ab, const int32_t* ny_ab, const int32_t* nz_ab, int64_t n_cd,
+                 const double* p_cd, const double* P_cd, const double* Ex_cd, const double* Ey_cd,
+                 const double* Ez_cd, const int32_t* nx_cd, const int32_t* ny_cd, const int32_t* nz_cd, int e_stride,
+                 double* out, void* stream) {
+  if (n_ab < 0 || n_cd < 0 || e_stride < 1) return fail(DFT_EINVAL, "bad sizes");
+  double* tab = nullptr;
+  int rc = boys_table_device(&tab);
+  if (rc) return rc;
+  hipError_t e = launch_eri_prim(n_ab, p_ab, P_ab, Ex_ab, Ey_ab, Ez_ab, nx_ab, ny_ab, nz_ab, n_cd, p_cd, P_cd, Ex_cd,
+                                 Ey_cd, Ez_cd, nx_cd, ny_cd, nz_cd, e_stride, out, tab, (hipStream_t)stream);
+  if (e != hipSuccess) return fail(DFT_EHIP, std::string("eri_prim: ") + hipGetErrorString(e));
+  return DFT_OK;
+}
+
 int dft_boys(const double* T, int n, int mmax, double* out, int fp32, void* stream) {
   if (n < 0 || (n > 0 && (!T || !out))) return fail(DFT_EINVAL, "bad arguments");
   if (mmax < 0 || mmax > 16) return fail(DFT_EUNSUPPORTED, "mmax must be in [0, 16]");

@@ -74,6 +74,17 @@ hipError_t launch_one_electron(const int2* shell_pairs, int npairs, const ShellD
                                const double* atom_xyzq, double* S, double* T, double* V, int nbf,
                                const double* boys_tab, hipStream_t s);
 
+// reference-layout primitive kernels (nuclear_kernel / eri_kernel buffers)
+hipError_t launch_nuclear_prim(int nfa, int nfb, const double* ea, const double* eb, const double* ca, int sa,
+                               const double* cb, int sb, const int* pa, const int* pb, int natom, const int* Z,
+                               const double* xyz, double* V, double* Ex, double* Ey, double* Ez, double* p,
+                               double* P, int* nx, int* ny, int* nz, const double* boys_tab, hipStream_t s);
+hipError_t launch_eri_prim(long long nab, const double* pab, const double* Pab, const double* Exab,
+                           const double* Eyab, const double* Ezab, const int* nxab, const int* nyab,
+                           const int* nzab, long long ncd, const double* pcd, const double* Pcd, const double* Excd,
+                           const double* Eycd, const double* Ezcd, const int* nxcd, const int* nycd,
+                           const int* nzcd, int e_stride, double* out, const double* boys_tab, hipStream_t s);
+
 // F_m(T) test hook
 hipError_t launch_boys(const double* T, int n, int mmax, double* out, int fp32,
                        const double* boys_tab, hipStream_t s);

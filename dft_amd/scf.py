@@ -68,10 +68,15 @@ def electron_counts(Z, charge=0, mult=1):
 
 
 class _Fock:
-    """J/K on preallocated device buffers, timed with HIP events."""
+    """J/K on preallocated device buffers, timed with HIP events.
 
-    def __init__(self, plan: Plan, mode: str, rhf: bool):
-        self.plan, self.mode, self.rhf = plan, mode, rhf
+    With ``A`` (spherical x Cartesian, from shelled.spherical_matrix) the
+    densities arrive in the spherical basis, are taken to Cartesian as
+    A^T P A for the build and J/K return as A J A^T (shelled.py:1166-1177).
+    """
+
+    def __init__(self, plan: Plan, mode: str, rhf: bool, A=None):
+        self.plan, self.mode, self.rhf, self.A = plan, mode, rhf, A
         n = plan.nbf
         dev = plan.device
         self.J = torch.empty((n, n), dtype=torch.float64, device=dev)
@@ -80,6 +85,9 @@ class _Fock:
         self.times = []
 
     def __call__(self, P, P_a, P_b):
+        A = self.A
+        if A is not None:
+            P, P_a, P_b = (A.T @ x @ A for x in (P, P_a, P_b))
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
         e0.record()
@@ -91,19 +99,24 @@ class _Fock:
         e1.record()
         e1.synchronize()
         self.times.append(e0.elapsed_time(e1) * 1e-3)
-        return self.J, self.Ka, (self.Ka if self.rhf else self.Kb)
+        J, Ka, Kb = self.J, self.Ka, (self.Ka if self.rhf else self.Kb)
+        if A is not None:
+            J, Ka, Kb = (A @ x @ A.T for x in (J, Ka, Kb))
+        return J, Ka, Kb
 
 
-def _setup(plan: Plan, Z, centers):
+def _setup(plan: Plan, Z, centers, A=None):
     S, T, V = plan.one_electron()
     H = T + V
+    if A is not None:
+        S, H = A @ S @ A.T, A @ H @ A.T
     w, U = torch.linalg.eigh(S)
     X = U @ torch.diag(w.rsqrt()) @ U.T
     return S, H, X, nuclear_repulsion(Z, centers)
 
 
 def scf_plain(plan: Plan, Z, centers, N_a, N_b, mode="fp64", e_tol=1e-8, p_tol=1e-6, max_count=100,
-              p_rms_tol=None):
+              p_rms_tol=None, A=None):
     """Plain UHF loop of main.py:753-828; RHF fast path when N_a == N_b.
 
     Exits when |dE| < e_tol and max|dP| < p_tol (and RMS dP < p_rms_tol if
@@ -111,11 +124,11 @@ def scf_plain(plan: Plan, Z, centers, N_a, N_b, mode="fp64", e_tol=1e-8, p_tol=1
     reference silently exits there, main.py:806).
     """
     rhf = N_a == N_b and not plan.uhf
-    S, H, X, E_NN = _setup(plan, Z, centers)
+    S, H, X, E_NN = _setup(plan, Z, centers, A)
     _, Cp = torch.linalg.eigh(X.T @ H @ X)
     C = X @ Cp
     P_a, P_b = init_UHF_density(C, C, N_a, N_b)
-    fock = _Fock(plan, mode, rhf)
+    fock = _Fock(plan, mode, rhf, A)
     E_total = -1000.0
     count = 0
     energies, iter_times = [], []
@@ -150,14 +163,8 @@ def scf_plain(plan: Plan, Z, centers, N_a, N_b, mode="fp64", e_tol=1e-8, p_tol=1
         count += 1
 
 
-def scf_fixed(plan: Plan, Z, centers, N_a, N_b, n_iter, mode="fp64"):
-    """Exactly ``n_iter`` plain Roothaan iterations (benchmark loop)."""
-    res = scf_plain(plan, Z, centers, N_a, N_b, mode=mode, e_tol=-1.0, p_tol=-1.0, max_count=n_iter - 2)
-    return res
-
-
 def scf_diis(plan: Plan, Z, centers, N_a, N_b, mode="fp64", rate=0.4, diis_threshold=1e-3, history=10,
-             e_tol=1e-8, p_tol=1e-6, max_count=500):
+             e_tol=1e-8, p_tol=1e-6, max_count=500, A=None):
     """Damped + DIIS UHF loop of shelled.py:1103-1292.
 
     Damping rate 0.4, divided by 1.1 every fourth strike (an iteration whose
@@ -166,12 +173,12 @@ def scf_diis(plan: Plan, Z, centers, N_a, N_b, mode="fp64", rate=0.4, diis_thres
     coefficients normalised to sum 1 (shelled.py:1196-1201). Unlike the
     reference it stops after ``max_count`` iterations with converged False.
     """
-    S, H, X, E_NN = _setup(plan, Z, centers)
+    S, H, X, E_NN = _setup(plan, Z, centers, A)
     _, Cp = torch.linalg.eigh(X.T @ H @ X)
     C = X @ Cp
     P_a, P_b = init_UHF_density(C, C, N_a, N_b)
     rhf = N_a == N_b and not plan.uhf
-    fock = _Fock(plan, mode, rhf)
+    fock = _Fock(plan, mode, rhf, A)
     hist = {k: [] for k in ("Pa", "Pb", "Fa", "Fb", "Ra", "Rb")}
     E_total = None
     prev_dE = prev_dP = float("inf")

@@ -150,6 +150,39 @@ int dft_eri_quartets(dft_plan plan, const int32_t* quartets, int64_t nq, double*
 int dft_oneelec(dft_plan plan, double* S, double* T, double* V, void* stream);
 
 /*
+ * Primitive one-electron pass in the buffer layout of the reference's
+ * nuclear_kernel: one entry q = a * nfb + b per primitive Cartesian pair.
+ * Outputs V[q] = -(2 pi/p) sum_C Z_C sum_tuv E_t E_u E_v R_tuv(p, P - C),
+ * the Hermite coefficients Ex/Ey/Ez[q*7 + t] (zero padded), p[q], P[3q+k]
+ * and the lengths nx/ny/nz[q] = l_a + l_b + 1 per axis. cen_*_stride is 0
+ * for one centre per side (shelled.py:350-371, per shell pair) or 3 for a
+ * centre per function (main.py:392-409, flat variant). Device pointers;
+ * l_a + l_b up to 12 (the reference's capacity).
+ *
+ * Replaces nuclear_kernel (shelled.py:350-450, main.py:392-488).
+ */
+int dft_nuclear_prim(int nfa, int nfb, const double* exp_a, const double* exp_b, const double* cen_a,
+                     int cen_a_stride, const double* cen_b, int cen_b_stride, const int32_t* pow_a,
+                     const int32_t* pow_b, int natom, const int32_t* atoms, const double* atom_xyz, double* V,
+                     double* Ex, double* Ey, double* Ez, double* p, double* P, int32_t* nx, int32_t* ny, int32_t* nz,
+                     void* stream);
+
+/*
+ * Unnormalised, uncontracted primitive Cartesian ERIs from two pair-record
+ * sets in the reference layout: out[i * n_cd + j] = (i | j) for every bra
+ * record i < n_ab and ket record j < n_cd. E arrays have stride e_stride (7
+ * in the reference). Total momentum up to 12. Device pointers.
+ *
+ * Replaces eri_kernel (shelled.py:452-532; the flat main.py:490-563 variant
+ * is the same call with the bra pointers offset by row_start).
+ */
+int dft_eri_prim(int64_t n_ab, const double* p_ab, const double* P_ab, const double* Ex_ab, const double* Ey_ab,
+                 const double* Ez_ab, const int32_t* nx_ab, const int32_t* ny_ab, const int32_t* nz_ab, int64_t n_cd,
+                 const double* p_cd, const double* P_cd, const double* Ex_cd, const double* Ey_cd,
+                 const double* Ez_cd, const int32_t* nx_cd, const int32_t* ny_cd, const int32_t* nz_cd, int e_stride,
+                 double* out, void* stream);
+
+/*
  * Boys function test hook: out[i*(mmax+1)+m] = F_m(T[i]), m = 0..mmax, with
  * the tabulated evaluator the ERI kernels use. T and out are device arrays.
  * fp32 selects the FP32 recursion (FP64 seed). Replaces boys/taylor
