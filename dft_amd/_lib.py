@@ -38,6 +38,7 @@ SIGNATURES = {
                                    _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "dft_eri_prim": (C.c_int, [C.c_int64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int64, _vp, _vp, _vp, _vp,
                                _vp, _vp, _vp, _vp, C.c_int, _vp, _vp]),
+    "dft_grad": (C.c_int, [_vp, _i32p, _vp, _vp, _vp, _vp, _vp, _vp]),
     "dft_plan_profile": (C.c_int, [_vp, C.c_int]),
     "dft_plan_profile_read": (C.c_int, [_vp, _dp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
 }

@@ -134,6 +134,17 @@ struct dft_plan_s {
   // part, then reduce, symmetrize; one begin/end event pair per slot and step
   int prof_steps = 0, prof_used = 0;
   std::vector<hipEvent_t> prof_ev;
+  // gradient data, built by the first dft_grad call
+  bool grad_ready = false;
+  std::vector<int> grad_atom;     // internal shell -> atom
+  double* d_grec = nullptr;       // extended pair records
+  int4* d_gpairs = nullptr;       // (sh_a, sh_b, grec_off, nprim_pairs) per pair
+  std::vector<int2*> d_gq_swap;   // per combo: quartets with bra and ket exchanged
+  std::vector<long long*> d_gslots_ab, d_gslots_cd;
+  long long* d_g1slots = nullptr;
+  long long* d_gbase = nullptr;
+  int* d_gcount = nullptr;
+  double* d_gstage = nullptr;
 };
 
 namespace {
@@ -148,6 +159,11 @@ void free_plan(dft_plan p) {
                   p->d_stage, p->d_tg, p->d_tbase, p->d_tcount, p->d_half, p->d_atoms, p->d_shpairs};
   for (void* x : ptrs) (void)hipFree(x);
   for (hipEvent_t e : p->prof_ev) (void)hipEventDestroy(e);
+  for (auto* x : p->d_gq_swap) (void)hipFree(x);
+  for (auto* x : p->d_gslots_ab) (void)hipFree(x);
+  for (auto* x : p->d_gslots_cd) (void)hipFree(x);
+  void* gptrs[] = {p->d_grec, p->d_gpairs, p->d_g1slots, p->d_gbase, p->d_gcount, p->d_gstage};
+  for (void* x : gptrs) (void)hipFree(x);
   delete p;
 }
 
@@ -388,9 +404,155 @@ int build_plan(dft_plan p, int nsh, const int32_t* L, const int32_t* NP, const i
   return DFT_OK;
 }
 
+// Gradient set-up: extended pair records, swapped quartet lists and the
+// per-atom slot map (2-e: four centres per quartet; 1-e: three per
+// (shell pair, nucleus) block). Slots of one atom are contiguous and ordered
+// (2-e AB, 2-e CD per combination, then 1-e), so the sum order is fixed.
+int grad_prepare(dft_plan p, const int32_t* shell_atom) {
+  const int nsh = p->nsh;
+  p->grad_atom.assign(nsh, 0);
+  for (int i = 0; i < nsh; ++i) {
+    const int a = shell_atom[i];
+    if (a < 0 || a >= p->natom) return fail(DFT_EINVAL, "shell_atom out of range");
+    p->grad_atom[p->in2int[i]] = a;
+  }
+  for (const auto& s : p->sh)
+    if (s.l > 1) return fail(DFT_EUNSUPPORTED, "analytic gradients are built for s and p shells only");
+  // extended records
+  std::vector<int4> pp, gp(p->pairs.size());
+  std::vector<int> off;
+  long long roff = 0;
+  for (size_t id = 0; id < p->pairs.size(); ++id) {
+    const int4 pr = p->pairs[id];
+    const ShellDev &A = p->sh[pr.x], &B = p->sh[pr.y];
+    gp[id] = make_int4(pr.x, pr.y, (int)roff, pr.w);
+    const int rs = 6 + 3 * ne_axis(A.l + 1, B.l + 1);
+    for (int ia = 0; ia < A.nprim; ++ia)
+      for (int ib = 0; ib < B.nprim; ++ib) {
+        pp.push_back(make_int4(pr.x, pr.y, ia, ib));
+        off.push_back((int)roff);
+        roff += rs;
+      }
+  }
+  int rc;
+  if ((rc = upload(&p->d_gpairs, gp))) return rc;
+  HIP_TRY(hipMalloc(&p->d_grec, std::max<long long>(roff, 1) * sizeof(double)));
+  {
+    int4* dpp = nullptr;
+    int* doff = nullptr;
+    if ((rc = upload(&dpp, pp))) return rc;
+    if ((rc = upload(&doff, off))) return rc;
+    hipError_t e = launch_grad_records(dpp, (int)pp.size(), doff, p->d_sh, p->d_exps, p->d_coefs, p->d_grec, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    (void)hipFree(dpp);
+    (void)hipFree(doff);
+    if (e != hipSuccess) return fail(DFT_EHIP, std::string("gradient records: ") + hipGetErrorString(e));
+  }
+  // per-atom contribution counts
+  const int natom = p->natom;
+  std::vector<int> cnt(natom, 0);
+  auto atom_of_pair_shell = [&](int pid, int which) {
+    return p->grad_atom[which == 0 ? p->pairs[pid].x : p->pairs[pid].y];
+  };
+  for (const Combo& c : p->combos)
+    for (const int2& q : c.q) {
+      ++cnt[atom_of_pair_shell(q.x, 0)];
+      ++cnt[atom_of_pair_shell(q.x, 1)];
+      ++cnt[atom_of_pair_shell(q.y, 0)];
+      ++cnt[atom_of_pair_shell(q.y, 1)];
+    }
+  const int nsp = p->nshpairs;
+  std::vector<int2> sp;
+  for (int i = 0; i < nsh; ++i)
+    for (int j = 0; j <= i; ++j) sp.push_back(make_int2(i, j));
+  for (int k = 0; k < nsp; ++k)
+    for (int cn = 0; cn < natom; ++cn) {
+      ++cnt[p->grad_atom[sp[k].x]];
+      ++cnt[p->grad_atom[sp[k].y]];
+      ++cnt[cn];
+    }
+  std::vector<long long> base(natom), next(natom);
+  long long cur = 0;
+  for (int a = 0; a < natom; ++a) {
+    base[a] = next[a] = cur;
+    cur += 3LL * cnt[a];
+  }
+  auto slot = [&](int atom) {
+    const long long s = next[atom];
+    next[atom] += 3;
+    return s;
+  };
+  for (Combo& c : p->combos) {
+    std::vector<long long> sab(2 * c.q.size()), scd(2 * c.q.size());
+    std::vector<int2> sw(c.q.size());
+    for (size_t i = 0; i < c.q.size(); ++i) {
+      sab[2 * i] = slot(atom_of_pair_shell(c.q[i].x, 0));
+      sab[2 * i + 1] = slot(atom_of_pair_shell(c.q[i].x, 1));
+    }
+    for (size_t i = 0; i < c.q.size(); ++i) {
+      scd[2 * i] = slot(atom_of_pair_shell(c.q[i].y, 0));
+      scd[2 * i + 1] = slot(atom_of_pair_shell(c.q[i].y, 1));
+      sw[i] = make_int2(c.q[i].y, c.q[i].x);
+    }
+    int2* dsw = nullptr;
+    long long *dab = nullptr, *dcd = nullptr;
+    if ((rc = upload(&dsw, sw))) return rc;
+    if ((rc = upload(&dab, sab))) return rc;
+    if ((rc = upload(&dcd, scd))) return rc;
+    p->d_gq_swap.push_back(dsw);
+    p->d_gslots_ab.push_back(dab);
+    p->d_gslots_cd.push_back(dcd);
+  }
+  std::vector<long long> s1(3 * (size_t)nsp * natom);
+  for (int k = 0; k < nsp; ++k)
+    for (int cn = 0; cn < natom; ++cn) {
+      const size_t b = 3 * ((size_t)k * natom + cn);
+      s1[b] = slot(p->grad_atom[sp[k].x]);
+      s1[b + 1] = slot(p->grad_atom[sp[k].y]);
+      s1[b + 2] = slot(cn);
+    }
+  if ((rc = upload(&p->d_g1slots, s1))) return rc;
+  if ((rc = upload(&p->d_gbase, base))) return rc;
+  if ((rc = upload(&p->d_gcount, cnt))) return rc;
+  HIP_TRY(hipMalloc(&p->d_gstage, std::max<long long>(cur, 1) * sizeof(double)));
+  p->grad_ready = true;
+  return DFT_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int dft_grad(dft_plan p, const int32_t* shell_atom, const double* P, const double* Pa, const double* Pb,
+             const double* W, double* grad, void* stream) {
+  if (!p) return fail(DFT_EINVAL, "plan is NULL");
+  if (!shell_atom || !P || !Pa || !W || !grad) return fail(DFT_EINVAL, "NULL argument");
+  if (p->natom <= 0) return fail(DFT_EINVAL, "plan has no nuclei");
+  if (!p->grad_ready) {
+    int rc = grad_prepare(p, shell_atom);
+    if (rc) return rc;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  for (size_t ci = 0; ci < p->combos.size(); ++ci) {
+    const Combo& c = p->combos[ci];
+    if (c.q.empty()) continue;
+    int la, lb, lc, ld;
+    pc_moments(c.pcb, la, lb);
+    pc_moments(c.pck, lc, ld);
+    const long long nq = (long long)c.q.size();
+    hipError_t e = launch_eri_grad(la, lb, lc, ld, c.d_q, p->d_gslots_ab[ci], nq, p->d_gpairs, p->d_grec,
+                                   p->d_shell_ao, P, Pa, Pb, p->nbf, p->d_gstage, p->d_boys, s);
+    if (e == hipSuccess)
+      e = launch_eri_grad(lc, ld, la, lb, p->d_gq_swap[ci], p->d_gslots_cd[ci], nq, p->d_gpairs, p->d_grec,
+                          p->d_shell_ao, P, Pa, Pb, p->nbf, p->d_gstage, p->d_boys, s);
+    if (e != hipSuccess) return fail(DFT_EHIP, std::string("eri_grad: ") + hipGetErrorString(e));
+  }
+  hipError_t e = launch_grad1e(p->d_shpairs, p->nshpairs, p->d_sh, p->d_exps, p->d_coefs, p->natom, p->d_atoms, P, W,
+                               p->nbf, p->d_g1slots, p->d_gstage, p->d_boys, s);
+  if (e == hipSuccess) e = launch_grad_reduce(p->natom, p->d_gbase, p->d_gcount, p->d_gstage, grad, s);
+  if (e != hipSuccess) return fail(DFT_EHIP, std::string("grad 1e/reduce: ") + hipGetErrorString(e));
+  return DFT_OK;
+}
 
 int dft_version(void) { return 100; }
 

@@ -74,6 +74,22 @@ hipError_t launch_one_electron(const int2* shell_pairs, int npairs, const ShellD
                                const double* atom_xyzq, double* S, double* T, double* V, int nbf,
                                const double* boys_tab, hipStream_t s);
 
+// analytic gradients (s/p): extended pair records, derivative ERIs
+// contracted with the density pair (bra-centre derivatives; call again with
+// bra and ket exchanged for the ket centres), 1-e derivatives and the
+// per-atom reduction. Every contribution is 3 doubles at its own slot.
+hipError_t launch_grad_records(const int4* pairs_prim, int n, const int* rec_off, const ShellDev* shells,
+                               const double* exps, const double* coefs, double* out, hipStream_t s);
+hipError_t launch_eri_grad(int la, int lb, int lc, int ld, const int2* quartets, const long long* slots, long long nq,
+                           const int4* gpairs, const double* grec, const int* shell_ao, const double* DJ,
+                           const double* DK0, const double* DK1, int nbf, double* stage, const double* boys_tab,
+                           hipStream_t s);
+hipError_t launch_grad1e(const int2* shell_pairs, int npairs, const ShellDev* shells, const double* exps,
+                         const double* coefs, int natom, const double* atom_xyzq, const double* P, const double* W,
+                         int nbf, const long long* slots, double* stage, const double* boys_tab, hipStream_t s);
+hipError_t launch_grad_reduce(int natom, const long long* base, const int* count, const double* stage, double* grad,
+                              hipStream_t s);
+
 // reference-layout primitive kernels (nuclear_kernel / eri_kernel buffers)
 hipError_t launch_nuclear_prim(int nfa, int nfb, const double* ea, const double* eb, const double* ca, int sa,
                                const double* cb, int sb, const int* pa, const int* pb, int natom, const int* Z,

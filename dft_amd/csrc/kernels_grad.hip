@@ -1,0 +1,488 @@
+// Analytic nuclear-gradient kernels (s/p shells).
+//
+// Two-electron part. For a canonical quartet with degeneracy weight w the
+// energy is sum_{abcd} Gamma_abcd (ab|cd) with the density pair
+//   Gamma_abcd = w [4 P_ab P_cd - 2 sum_s (P^s_ac P^s_bd + P^s_ad P^s_bc)],
+// so dE/dA_x = sum Gamma_abcd d(ab|cd)/dA_x. The derivative of a Cartesian
+// Gaussian, d/dA_x x_A^i e^{-a x_A^2} = 2a x_A^{i+1} e - i x_A^{i-1} e, gives
+// the Hermite E derivative recurrence
+//   dE^{ij}_t / dA_x = 2a E^{i+1,j}_t - i E^{i-1,j}_t   (dB: 2b E^{i,j+1}_t - j E^{i,j-1}_t)
+// used here with E tables extended by one in each index. One launch gives
+// the derivatives with respect to the two bra centres; the same quartets
+// with bra and ket exchanged give the ket centres (Gamma is symmetric under
+// the exchange).
+//
+// One-electron part: per (shell pair, nucleus C) block, the derivatives of
+// S (weighted by W), T (by P) and of the nuclear attraction of C (by P) with
+// respect to A and B, and the operator derivative with respect to C
+// (dV^C/dC = -(dV^C/dA + dV^C/dB), translational invariance).
+//
+// Every contribution is written to its own staging slot; a fixed-order
+// per-atom reduction sums them (no atomics).
+#include "eri_quartet.h"
+#include "kernels.h"
+
+namespace dft {
+
+// gradient pair record: p, Px, Py, Pz, Kab, alpha, then E tables of class
+// (la+1, lb+1) without the angular normalisation (s/p functions have none)
+__host__ __device__ constexpr int grec_size(int la, int lb) { return 6 + 3 * ne_axis(la + 1, lb + 1); }
+
+template <int LB1, int I, int J, int T>
+__device__ __forceinline__ double eg(const double* __restrict__ E) {
+  if constexpr (I < 0 || J < 0 || T < 0 || T > I + J) return 0.0;
+  else if constexpr (I == 0 && J == 0) return 1.0;
+  else return E[eoff(LB1, I, J) + T];
+}
+
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(256) void eri_grad_kernel(const int2* __restrict__ quartets,
+                                                       const long long* __restrict__ slots, long long nq,
+                                                       const int4* __restrict__ gpairs, const double* __restrict__ grec,
+                                                       const int* __restrict__ shell_ao, const double* __restrict__ DJ,
+                                                       const double* __restrict__ DK0, const double* __restrict__ DK1,
+                                                       int nbf, double* __restrict__ stage,
+                                                       const double* __restrict__ boys_gtab) {
+  constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+  constexpr int NAB = NA * NB, NCD = NC * ND;
+  constexpr int L1 = LA + LB + LC + LD + 1;
+  constexpr int NH1 = nherm(LA + LB + 1);
+  constexpr int RSB = grec_size(LA, LB), RSK = grec_size(LC, LD);
+  constexpr int NEB = ne_axis(LA + 1, LB + 1), NEK = ne_axis(LC + 1, LD + 1);
+  __shared__ double btab[BOYS_NG * BOYS_TAYLOR];
+  boys_stage<L1>(boys_gtab, btab);
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += stride) {
+    const int2 qd = quartets[q];
+    const int4 pb = gpairs[qd.x];
+    const int4 pk = gpairs[qd.y];
+    // the canonical weight is symmetric in bra/ket, so it is the same in both launches
+    double w = 1.0;
+    if (pb.x == pb.y) w *= 0.5;
+    if (pk.x == pk.y) w *= 0.5;
+    if (qd.x == qd.y) w *= 0.5;
+    const long long a0 = shell_ao[pb.x], b0 = shell_ao[pb.y], c0 = shell_ao[pk.x], d0 = shell_ao[pk.y];
+    const double* DB = DK1 ? DK1 : DK0;
+    double Gam[NAB * NCD];
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+          for (int d = 0; d < ND; ++d) {
+            const long long A = a0 + a, B = b0 + b, Cc = c0 + c, D = d0 + d;
+            const double x = DK0[A * nbf + Cc] * DK0[B * nbf + D] + DK0[A * nbf + D] * DK0[B * nbf + Cc] +
+                             DB[A * nbf + Cc] * DB[B * nbf + D] + DB[A * nbf + D] * DB[B * nbf + Cc];
+            Gam[((a * NB + b) * NC + c) * ND + d] = w * (4.0 * DJ[A * nbf + B] * DJ[Cc * nbf + D] - 2.0 * x);
+          }
+    double gA[3] = {0.0, 0.0, 0.0}, gB[3] = {0.0, 0.0, 0.0};
+    for (int i = 0; i < pb.w; ++i) {
+      const double* rb = grec + pb.z + (long long)i * RSB;
+      const double p = rb[0], Px = rb[1], Py = rb[2], Pz = rb[3], Kab = rb[4], al = rb[5], be = p - al;
+      double G[NH1 * NCD];
+#pragma unroll
+      for (int x = 0; x < NH1 * NCD; ++x) G[x] = 0.0;
+      for (int j = 0; j < pk.w; ++j) {
+        const double* rk = grec + pk.z + (long long)j * RSK;
+        const double qq = rk[0];
+        const double X = Px - rk[1], Y = Py - rk[2], Z = Pz - rk[3];
+        const double pq = p + qq;
+        const double alpha = p * qq / pq;
+        const double pref = TWO_PI_2P5 * Kab * rk[4] / (p * qq) * sqrt(1.0 / pq);
+        double F[L1 + 1];
+        boys_fp64<L1>(alpha * (X * X + Y * Y + Z * Z), btab, F);
+        double R[nherm(L1)];
+        hermite_R<L1, double>(F, -2.0 * alpha, X, Y, Z, R);
+        const double* Exk = rk + 6;
+        const double* Eyk = Exk + NEK;
+        const double* Ezk = Eyk + NEK;
+        static_for<0, NCD>([&](auto cdI) {
+          constexpr int cd = decltype(cdI)::value;
+          constexpr int c = cd / ND, d = cd % ND;
+          constexpr int cx = comp_x(LC, c), cy = comp_y(LC, c), cz = comp_z(LC, c);
+          constexpr int dx = comp_x(LD, d), dy = comp_y(LD, d), dz = comp_z(LD, d);
+          static_for<0, NH1>([&](auto hI) {
+            constexpr int h = decltype(hI)::value;
+            constexpr int t = h_t(h), u = h_u(h), v = h_v(h);
+            double s = 0.0;
+            static_for<0, cx + dx + 1>([&](auto tI) {
+              static_for<0, cy + dy + 1>([&](auto uI) {
+                static_for<0, cz + dz + 1>([&](auto vI) {
+                  constexpr int ta = decltype(tI)::value, nu = decltype(uI)::value, ph = decltype(vI)::value;
+                  const double e = eg<LD + 1, cx, dx, ta>(Exk) * eg<LD + 1, cy, dy, nu>(Eyk) *
+                                   eg<LD + 1, cz, dz, ph>(Ezk);
+                  const double r = R[hidx(t + ta, u + nu, v + ph)];
+                  s = ((ta + nu + ph) & 1) ? fma(-e, r, s) : fma(e, r, s);
+                });
+              });
+            });
+            G[h * NCD + cd] = fma(pref, s, G[h * NCD + cd]);
+          });
+        });
+      }
+      const double* Exb = rb + 6;
+      const double* Eyb = Exb + NEB;
+      const double* Ezb = Eyb + NEB;
+      static_for<0, NAB>([&](auto abI) {
+        constexpr int ab = decltype(abI)::value;
+        constexpr int a = ab / NB, b = ab % NB;
+        constexpr int ai[3] = {comp_x(LA, a), comp_y(LA, a), comp_z(LA, a)};
+        constexpr int bi[3] = {comp_x(LB, b), comp_y(LB, b), comp_z(LB, b)};
+        // Gt[h] = sum_cd Gamma_abcd G[h][cd]
+        double Gt[NH1];
+#pragma unroll
+        for (int h = 0; h < NH1; ++h) {
+          double s = 0.0;
+#pragma unroll
+          for (int cd = 0; cd < NCD; ++cd) s = fma(Gam[ab * NCD + cd], G[h * NCD + cd], s);
+          Gt[h] = s;
+        }
+        const double* Eax[3] = {Exb, Eyb, Ezb};
+        static_for<0, 3>([&](auto axI) {
+          constexpr int ax = decltype(axI)::value;
+          constexpr int o1 = (ax + 1) % 3, o2 = (ax + 2) % 3;
+          // undifferentiated E along the other two axes times the derivative along ax
+          static_for<0, ai[ax] + bi[ax] + 2>([&](auto tI) {
+            constexpr int t = decltype(tI)::value;
+            const double dA = 2.0 * al * eg<LB + 1, ai[ax] + 1, bi[ax], t>(Eax[ax]) -
+                              (ai[ax] > 0 ? ai[ax] * eg<LB + 1, ai[ax] - 1, bi[ax], t>(Eax[ax]) : 0.0);
+            const double dB = 2.0 * be * eg<LB + 1, ai[ax], bi[ax] + 1, t>(Eax[ax]) -
+                              (bi[ax] > 0 ? bi[ax] * eg<LB + 1, ai[ax], bi[ax] - 1, t>(Eax[ax]) : 0.0);
+            static_for<0, ai[o1] + bi[o1] + 1>([&](auto uI) {
+              static_for<0, ai[o2] + bi[o2] + 1>([&](auto vI) {
+                constexpr int u = decltype(uI)::value, v = decltype(vI)::value;
+                const double e = eg<LB + 1, ai[o1], bi[o1], u>(Eax[o1]) * eg<LB + 1, ai[o2], bi[o2], v>(Eax[o2]);
+                // (t,u,v) along (ax, o1, o2) -> (x, y, z) indices
+                constexpr int hx = ax == 0 ? t : (o1 == 0 ? u : v);
+                constexpr int hy = ax == 1 ? t : (o1 == 1 ? u : v);
+                constexpr int hz = ax == 2 ? t : (o1 == 2 ? u : v);
+                const double g = Gt[hidx(hx, hy, hz)] * e;
+                gA[ax] = fma(dA, g, gA[ax]);
+                gB[ax] = fma(dB, g, gB[ax]);
+              });
+            });
+          });
+        });
+      });
+    }
+    const long long* sl = slots + 2 * q;
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+      stage[sl[0] + x] = gA[x];
+      stage[sl[1] + x] = gB[x];
+    }
+  }
+}
+
+// gradient pair records: one lane per primitive pair, E up to (la+1, lb+1)
+__global__ void grad_records_kernel(const int4* __restrict__ pp, int n, const int* __restrict__ rec_off,
+                                    const ShellDev* __restrict__ sh, const double* __restrict__ exps,
+                                    const double* __restrict__ coefs, double* __restrict__ out) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  const int4 q = pp[idx];
+  const ShellDev A = sh[q.x], B = sh[q.y];
+  const double a = exps[A.prim_off + q.z], b = exps[B.prim_off + q.w];
+  const double ca = coefs[A.prim_off + q.z], cb = coefs[B.prim_off + q.w];
+  const int la = A.l + 1, lb = B.l + 1;
+  const double p = a + b, mu = a * b / p;
+  const double AB[3] = {A.x - B.x, A.y - B.y, A.z - B.z};
+  const double Ac[3] = {A.x, A.y, A.z}, Bc[3] = {B.x, B.y, B.z};
+  const int NE = ne_axis(la, lb);
+  double* o = out + rec_off[idx];
+  o[0] = p;
+  for (int d = 0; d < 3; ++d) o[1 + d] = (a * Ac[d] + b * Bc[d]) / p;
+  o[4] = ca * cb * exp(-mu * (AB[0] * AB[0] + AB[1] * AB[1] + AB[2] * AB[2]));
+  o[5] = a;
+  for (int d = 0; d < 3; ++d) {
+    const double Q = AB[d];
+    const double xpa = -b * Q / p, xpb = a * Q / p, i2p = 0.5 / p;
+    double Ea[4][8];
+    for (int ia = 0; ia <= la; ++ia)
+      for (int t = 0; t < 8; ++t) Ea[ia][t] = 0.0;
+    Ea[0][0] = 1.0;
+    for (int ia = 1; ia <= la; ++ia)
+      for (int t = 0; t <= ia; ++t) {
+        double v = 0.0;
+        if (t <= ia - 1) v += xpa * Ea[ia - 1][t];
+        if (t > 0) v += Ea[ia - 1][t - 1] * i2p;
+        if (t + 1 <= ia - 1) v += (t + 1) * Ea[ia - 1][t + 1];
+        Ea[ia][t] = v;
+      }
+    for (int ia = 0; ia <= la; ++ia) {
+      double cur[8], nxt[8];
+      for (int t = 0; t < 8; ++t) cur[t] = Ea[ia][t];
+      for (int ib = 0; ib <= lb; ++ib) {
+        const int ord = ia + ib;
+        const int off = 6 + d * NE + eoff(lb, ia, ib);
+        for (int t = 0; t <= ord; ++t) o[off + t] = cur[t];
+        if (ib == lb) break;
+        for (int t = 0; t <= ord + 1; ++t) {
+          double v = 0.0;
+          if (t <= ord) v += xpb * cur[t];
+          if (t > 0) v += cur[t - 1] * i2p;
+          if (t + 1 <= ord) v += (t + 1) * cur[t + 1];
+          nxt[t] = v;
+        }
+        for (int t = 0; t <= ord + 1; ++t) cur[t] = nxt[t];
+      }
+    }
+  }
+}
+
+hipError_t launch_grad_records(const int4* pp, int n, const int* rec_off, const ShellDev* sh, const double* exps,
+                               const double* coefs, double* out, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(grad_records_kernel, dim3((n + 127) / 128), dim3(128), 0, s, pp, n, rec_off, sh, exps, coefs,
+                     out);
+  return hipGetLastError();
+}
+
+// ----------------------------------------------------- one-electron part
+__device__ void e1d_g(double a, double b, double Q, int ia, int ib, double* E) {
+  const double p = a + b;
+  double prev[12], next[12];
+  for (int t = 0; t < 12; ++t) prev[t] = 0.0;
+  prev[0] = exp(-(a * b / p) * Q * Q);
+  int order = 0;
+  for (int step = 0; step < ia + ib; ++step) {
+    const double coef = step < ia ? -(b * Q / p) : (a * Q / p);
+    const int no = order + 1;
+    for (int t = 0; t <= no; ++t) {
+      double v = 0.0;
+      if (t <= order) v += coef * prev[t];
+      if (t > 0) v += prev[t - 1] / (2.0 * p);
+      if (t + 1 <= order) v += (t + 1) * prev[t + 1];
+      next[t] = v;
+    }
+    order = no;
+    for (int t = 0; t <= order; ++t) prev[t] = next[t];
+  }
+  for (int t = 0; t < 12; ++t) E[t] = t <= order ? prev[t] : 0.0;
+}
+
+__device__ void boys_g(int L, double T, const double* __restrict__ g, double* F) {
+  if (T < BOYS_TMAX) {
+    const int k = __double2int_rn(T * BOYS_INV_DT);
+    const double d = -(T - k * BOYS_DT);
+    const double inv_fact[BOYS_TAYLOR] = {1.0, 1.0, 0.5, 1.0 / 6, 1.0 / 24, 1.0 / 120, 1.0 / 720, 1.0 / 5040};
+    double f = 0.0;
+    for (int j = BOYS_TAYLOR - 1; j >= 0; --j) f = fma(f, d, g[k * BOYS_MT + L + j] * inv_fact[j]);
+    F[L] = f;
+    const double e = exp(-T);
+    for (int m = L - 1; m >= 0; --m) F[m] = (2.0 * T * F[m + 1] + e) / (2.0 * m + 1.0);
+  } else {
+    F[0] = 0.88622692545275801365 / sqrt(T);
+    const double e = exp(-T);
+    for (int m = 0; m < L; ++m) F[m + 1] = ((2 * m + 1) * F[m] - e) / (2.0 * T);
+  }
+}
+
+__device__ void herm_g(int L, const double* F, double m2a, double X, double Y, double Z, double* R0) {
+  double lev[2][nherm(6)];
+  double s[7], pw = 1.0;
+  for (int n = 0; n <= L; ++n) { s[n] = F[n] * pw; pw *= m2a; }
+  int cur = 0;
+  lev[0][0] = s[L];
+  for (int n = L - 1; n >= 0; --n) {
+    const int S = L - n, nx = cur ^ 1;
+    lev[nx][0] = s[n];
+    for (int h = 1; h < nherm(S); ++h) {
+      const int t = h_t(h), u = h_u(h), v = h_v(h);
+      double r;
+      if (t > 0) {
+        r = X * lev[cur][hidx(t - 1, u, v)];
+        if (t > 1) r += (t - 1) * lev[cur][hidx(t - 2, u, v)];
+      } else if (u > 0) {
+        r = Y * lev[cur][hidx(t, u - 1, v)];
+        if (u > 1) r += (u - 1) * lev[cur][hidx(t, u - 2, v)];
+      } else {
+        r = Z * lev[cur][hidx(t, u, v - 1)];
+        if (v > 1) r += (v - 1) * lev[cur][hidx(t, u, v - 2)];
+      }
+      lev[nx][h] = r;
+    }
+    cur = nx;
+  }
+  for (int h = 0; h < nherm(L); ++h) R0[h] = lev[cur][h];
+}
+
+// One block (64 lanes, lane = component pair) per (shell pair, nucleus).
+// Block (pair, c) writes three xyz contributions: to the atom of A, of B and
+// to nucleus c. For c == 0 the S (with W) and T (with P) derivatives are
+// added to the A and B parts.
+__global__ void grad1e_kernel(const int2* __restrict__ sp, int npairs, const ShellDev* __restrict__ sh,
+                              const double* __restrict__ exps, const double* __restrict__ coefs, int natom,
+                              const double* __restrict__ atoms, const double* __restrict__ P,
+                              const double* __restrict__ W, int nbf, const long long* __restrict__ slots,
+                              double* __restrict__ stage, const double* __restrict__ gtab) {
+  const int pid = blockIdx.x / natom, cn = blockIdx.x % natom;
+  const int lane = threadIdx.x;
+  if (pid >= npairs) return;
+  const int2 pr = sp[pid];
+  const ShellDev A = sh[pr.x], B = sh[pr.y];
+  const int na = ncart(A.l), nb = ncart(B.l);
+  double acc[9];
+  for (int k = 0; k < 9; ++k) acc[k] = 0.0;  // gA xyz, gB xyz, gC xyz
+  if (lane < na * nb) {
+    const int ca = lane / nb, cb = lane % nb;
+    int pa[3], pb[3];
+    for (int d = 0; d < 3; ++d) {
+      pa[d] = comp_of(A.l, ca, d);
+      pb[d] = comp_of(B.l, cb, d);
+    }
+    const long long r = A.ao + ca, c = B.ao + cb;
+    // off-diagonal shell pairs stand for both (ab) and (ba)
+    const double sym = pr.x == pr.y ? 1.0 : 2.0;
+    const double Pab = sym * P[r * nbf + c], Wab = sym * W[r * nbf + c];
+    const double Ac[3] = {A.x, A.y, A.z}, Bc[3] = {B.x, B.y, B.z};
+    const double Cx = atoms[4 * cn], Cy = atoms[4 * cn + 1], Cz = atoms[4 * cn + 2], Zc = atoms[4 * cn + 3];
+    for (int i = 0; i < A.nprim; ++i)
+      for (int j = 0; j < B.nprim; ++j) {
+        const double al = exps[A.prim_off + i], be = exps[B.prim_off + j];
+        const double cc = coefs[A.prim_off + i] * coefs[B.prim_off + j];
+        const double p = al + be;
+        const double sq = sqrt(M_PI / p);
+        // E[d][k][m][t]: E_t^{pa+k-1, pb+m-2} along axis d (k: -1..+1, m: -2..+3)
+        double E[3][3][6][12];
+        for (int d = 0; d < 3; ++d)
+          for (int k = 0; k < 3; ++k)
+            for (int m = 0; m < 6; ++m) {
+              const int ia = pa[d] + k - 1, ib = pb[d] + m - 2;
+              if (ia < 0 || ib < 0)
+                for (int t = 0; t < 12; ++t) E[d][k][m][t] = 0.0;
+              else
+                e1d_g(al, be, Ac[d] - Bc[d], ia, ib, E[d][k][m]);
+            }
+        // 1-D overlap S1(k, m) and kinetic piece T1 (shelled.py:149-155) at offsets
+        auto S1 = [&](int d, int k, int m) { return E[d][k][m][0] * sq; };
+        auto T1 = [&](int d, int k) {  // k: a offset, b at its own value (m = 2)
+          const double bj = pb[d];
+          double t = -2.0 * be * (2 * bj + 1) * S1(d, k, 2) + 4.0 * be * be * S1(d, k, 4);
+          if (pb[d] >= 2) t += bj * (bj - 1) * S1(d, k, 0);
+          return t;
+        };
+        if (cn == 0) {
+          for (int d = 0; d < 3; ++d) {
+            const int o1 = (d + 1) % 3, o2 = (d + 2) % 3;
+            const double dS = 2.0 * al * S1(d, 2, 2) - (pa[d] > 0 ? pa[d] * S1(d, 0, 2) : 0.0);
+            const double dT = 2.0 * al * T1(d, 2) - (pa[d] > 0 ? pa[d] * T1(d, 0) : 0.0);
+            const double s1 = S1(o1, 1, 2), s2 = S1(o2, 1, 2);
+            const double gS = dS * s1 * s2;
+            const double gT = -0.5 * (dT * s1 * s2 + dS * (T1(o1, 1) * s2 + s1 * T1(o2, 1)));
+            const double v = cc * (Pab * gT - Wab * gS);
+            acc[d] += v;      // d/dA
+            acc[3 + d] -= v;  // d/dB = -d/dA for two-centre integrals
+          }
+        }
+        // nuclear attraction of nucleus cn: basis-centre derivatives
+        const double Pp[3] = {(al * A.x + be * B.x) / p, (al * A.y + be * B.y) / p, (al * A.z + be * B.z) / p};
+        const double X = Pp[0] - Cx, Y = Pp[1] - Cy, Z = Pp[2] - Cz;
+        const int L = A.l + B.l + 1;
+        double F[8], R0[nherm(6)];
+        boys_g(L, p * (X * X + Y * Y + Z * Z), gtab, F);
+        herm_g(L, F, -2.0 * p, X, Y, Z, R0);
+        const double pre = -2.0 * M_PI / p * Zc * cc * Pab;
+        for (int d = 0; d < 3; ++d) {
+          const int o1 = (d + 1) % 3, o2 = (d + 2) % 3;
+          double sA = 0.0, sB = 0.0;
+          for (int t = 0; t <= pa[d] + pb[d] + 1; ++t) {
+            const double dA = 2.0 * al * E[d][2][2][t] - (pa[d] > 0 ? pa[d] * E[d][0][2][t] : 0.0);
+            const double dB = 2.0 * be * E[d][1][3][t] - (pb[d] > 0 ? pb[d] * E[d][1][1][t] : 0.0);
+            for (int u = 0; u <= pa[o1] + pb[o1]; ++u)
+              for (int v = 0; v <= pa[o2] + pb[o2]; ++v) {
+                int hh[3];
+                hh[d] = t;
+                hh[o1] = u;
+                hh[o2] = v;
+                const double rv = R0[hidx(hh[0], hh[1], hh[2])] * E[o1][1][2][u] * E[o2][1][2][v];
+                sA = fma(dA, rv, sA);
+                sB = fma(dB, rv, sB);
+              }
+          }
+          acc[d] += pre * sA;
+          acc[3 + d] += pre * sB;
+          acc[6 + d] -= pre * (sA + sB);
+        }
+      }
+  }
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    double v = acc[k];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    acc[k] = v;
+  }
+  if (lane == 0) {
+    const long long* sl = slots + 3 * (long long)blockIdx.x;
+    for (int x = 0; x < 3; ++x) {
+      stage[sl[0] + x] = acc[x];
+      stage[sl[1] + x] = acc[3 + x];
+      stage[sl[2] + x] = acc[6 + x];
+    }
+  }
+}
+
+hipError_t launch_grad1e(const int2* sp, int npairs, const ShellDev* sh, const double* exps, const double* coefs,
+                         int natom, const double* atoms, const double* P, const double* W, int nbf,
+                         const long long* slots, double* stage, const double* gtab, hipStream_t s) {
+  if (npairs <= 0 || natom <= 0) return hipSuccess;
+  hipLaunchKernelGGL(grad1e_kernel, dim3(npairs * natom), dim3(64), 0, s, sp, npairs, sh, exps, coefs, natom, atoms,
+                     P, W, nbf, slots, stage, gtab);
+  return hipGetLastError();
+}
+
+// per-atom fixed-order sum of 3-vectors: wave per atom
+__global__ void grad_reduce_kernel(int natom, const long long* __restrict__ base, const int* __restrict__ count,
+                                   const double* __restrict__ stage, double* __restrict__ grad) {
+  const int atom = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (atom >= natom) return;
+  const double* src = stage + base[atom];
+  for (int x = 0; x < 3; ++x) {
+    double s = 0.0;
+    for (int k = lane; k < count[atom]; k += 64) s += src[3 * (long long)k + x];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) grad[3 * atom + x] = s;
+  }
+}
+
+hipError_t launch_grad_reduce(int natom, const long long* base, const int* count, const double* stage, double* grad,
+                              hipStream_t s) {
+  if (natom <= 0) return hipSuccess;
+  hipLaunchKernelGGL(grad_reduce_kernel, dim3((natom + 3) / 4), dim3(256), 0, s, natom, base, count, stage, grad);
+  return hipGetLastError();
+}
+
+// all ordered s/p class combinations (bra class, ket class), either order
+#define DFT_FOR_GRAD_COMBOS(X) \
+  X(0, 0, 0, 0)                \
+  X(0, 0, 1, 0)                \
+  X(0, 0, 1, 1)                \
+  X(1, 0, 0, 0)                \
+  X(1, 0, 1, 0)                \
+  X(1, 0, 1, 1)                \
+  X(1, 1, 0, 0)                \
+  X(1, 1, 1, 0)                \
+  X(1, 1, 1, 1)
+
+hipError_t launch_eri_grad(int la, int lb, int lc, int ld, const int2* q, const long long* slots, long long nq,
+                           const int4* gpairs, const double* grec, const int* ao, const double* DJ, const double* DK0,
+                           const double* DK1, int nbf, double* stage, const double* bt, hipStream_t s) {
+  if (nq <= 0) return hipSuccess;
+  long long g = (nq + 255) / 256;
+  if (g > 256 * 16) g = 256 * 16;
+  dim3 grid((unsigned)g), block(256);
+#define X(A, B, C, D)                                                                                        \
+  if (la == A && lb == B && lc == C && ld == D) {                                                            \
+    hipLaunchKernelGGL((eri_grad_kernel<A, B, C, D>), grid, block, 0, s, q, slots, nq, gpairs, grec, ao, DJ, \
+                       DK0, DK1, nbf, stage, bt);                                                            \
+    return hipGetLastError();                                                                                \
+  }
+  DFT_FOR_GRAD_COMBOS(X)
+#undef X
+  return hipErrorInvalidValue;
+}
+
+}  // namespace dft

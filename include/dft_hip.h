@@ -117,6 +117,23 @@ int dft_jk(dft_plan plan, const double* P, const double* Pa, const double* Pb, d
            double* Kb, int mode, void* stream);
 
 /*
+ * Analytic nuclear gradient of the (U)HF energy without the nuclear
+ * repulsion term:
+ *   grad = sum P dH/dR + sum Gamma d(ab|cd)/dR - sum W dS/dR
+ * with Gamma the density pair of P (total), Pa, Pb (Pb NULL: Pb = Pa) and W
+ * the energy-weighted density sum_s C_s,occ diag(eps_s) C_s,occ^T. grad is a
+ * device array [3*natom]. shell_atom[i] is the nucleus index of input shell
+ * i; the first call builds the gradient data of the plan (extended pair
+ * records, slot map) and allocates, later calls only enqueue work and
+ * reuse that mapping. s and p shells. Deterministic.
+ *
+ * No counterpart in the reference (it has no gradients); it is the backward
+ * of the energy for torch.autograd.
+ */
+int dft_grad(dft_plan plan, const int32_t* shell_atom, const double* P, const double* Pa, const double* Pb,
+             const double* W, double* grad, void* stream);
+
+/*
  * Per-launch timing of the next nsteps dft_jk calls (0 disables). Creates
  * 2*nslot HIP events per step; no device memory. Slots: 2c = class
  * combination c, FP64 part; 2c+1 = its weak (FP32-eligible) part; then the

@@ -1,0 +1,188 @@
+"""GPU tests of the drop-in call surface (dft_amd.shelled): reference-layout
+nuclear_kernel / eri_kernel launches, the ERI store, fill_matrices, 1-D
+overlap / kinetic helpers, the spherical transform and the spherical SCF,
+each against the oracle."""
+import numpy as np
+import pytest
+
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    return torch
+
+
+def _water_shells():
+    from dft_amd.basis import build_shells, load_basis
+    g = golden("water_sto3g.npz")
+    Z = list(g["Z"])
+    return Z, g["xyz"], build_shells(load_basis(), Z, g["xyz"])
+
+
+def _pair_buffers(torch, K):
+    f = lambda n: torch.empty(n, dtype=torch.float64, device="cuda")  # noqa: E731
+    i = lambda n: torch.empty(n, dtype=torch.int32, device="cuda")  # noqa: E731
+    return f(K), f(7 * K), f(7 * K), f(7 * K), f(K), f(3 * K), i(K), i(K), i(K)
+
+
+def test_nuclear_kernel_matches_oracle(torch_cuda):
+    import oracle.md_oracle as O
+    from dft_amd import shelled as S
+    torch = torch_cuda
+    Z, xyz, shells = _water_shells()
+    for i, j in [(0, 0), (2, 1), (2, 2), (4, 2), (3, 0)]:
+        a, b = shells[i], shells[j]
+        ea = np.tile(a["exponents"], a["n_components"])
+        eb = np.tile(b["exponents"], b["n_components"])
+        pa = np.repeat(a["components"], a["n_exponents"], axis=0)
+        pb = np.repeat(b["components"], b["n_exponents"], axis=0)
+        K = ea.size * eb.size
+        V, Ex, Ey, Ez, p, P, nx, ny, nz = _pair_buffers(torch, K)
+        S.nuclear_kernel((1,), (128,), (ea, eb, a["atom_pos"], b["atom_pos"], pa, pb, np.array(Z), xyz, len(Z),
+                                        ea.size, eb.size, V, Ex, Ey, Ez, p, P, nx, ny, nz))
+        ref = O.pair_data(a, b, Z, xyz)
+        assert np.allclose(V.cpu().numpy(), ref["V"], rtol=1e-12, atol=1e-13)
+        for got, key in ((Ex, "Ex"), (Ey, "Ey"), (Ez, "Ez")):
+            assert np.allclose(got.cpu().numpy().reshape(K, 7), ref[key], rtol=1e-13, atol=1e-15)
+        assert np.allclose(P.cpu().numpy().reshape(K, 3), ref["P"], rtol=1e-15)
+        assert (nx.cpu().numpy() == ref["nx"]).all() and (nz.cpu().numpy() == ref["nz"]).all()
+
+
+def test_eri_kernel_matches_oracle(torch_cuda):
+    import oracle.md_oracle as O
+    from dft_amd import shelled as S
+    torch = torch_cuda
+    Z, xyz, shells = _water_shells()
+    pairs = {k: O.pair_data(shells[k[0]], shells[k[1]]) for k in [(2, 1), (2, 2), (4, 3), (1, 0)]}
+    dev = lambda x, t=torch.float64: torch.as_tensor(np.ascontiguousarray(x), dtype=t, device="cuda")  # noqa
+    for ab, cd in [((2, 2), (2, 1)), ((4, 3), (1, 0)), ((2, 1), (4, 3))]:
+        A, B = pairs[ab], pairs[cd]
+        out = torch.empty(A["K"] * B["K"], dtype=torch.float64, device="cuda")
+        args = [A["K"], dev(A["p"]), dev(A["P"]), dev(A["Ex"]), dev(A["Ey"]), dev(A["Ez"]),
+                dev(A["nx"], torch.int32), dev(A["ny"], torch.int32), dev(A["nz"], torch.int32),
+                B["K"], dev(B["p"]), dev(B["P"]), dev(B["Ex"]), dev(B["Ey"]), dev(B["Ez"]),
+                dev(B["nx"], torch.int32), dev(B["ny"], torch.int32), dev(B["nz"], torch.int32), out]
+        S.eri_kernel((1,), (128,), tuple(args))
+        ref = O.eri_prim(A, B)
+        assert np.allclose(out.cpu().numpy().reshape(ref.shape), ref, rtol=1e-12, atol=1e-14)
+
+
+def test_flat_main_variant_matches_oracle(torch_cuda):
+    """main.py flat layout: all primitive Cartesian functions, rows in chunks."""
+    import oracle.md_oracle as O
+    from dft_amd import shelled as S
+    torch = torch_cuda
+    Z, xyz, shells = _water_shells()
+    exp = np.concatenate([np.tile(s["exponents"], s["n_components"]) for s in shells])
+    pw = np.concatenate([np.repeat(s["components"], s["n_exponents"], axis=0) for s in shells])
+    cen = np.concatenate([np.repeat(s["atom_pos"][None], s["n_components"] * s["n_exponents"], 0) for s in shells])
+    N = exp.size
+    K = N * N
+    V, Ex, Ey, Ez, p, P, nx, ny, nz = _pair_buffers(torch, K)
+    S.nuclear_kernel((1,), (128,), (exp, cen, pw, np.array(Z), xyz, len(Z), N, V, Ex, Ey, Ez, p, P, nx, ny, nz))
+    flat = {"K": K, "p": p.cpu().numpy(), "P": P.cpu().numpy().reshape(K, 3), "Ex": Ex.cpu().numpy().reshape(K, 7),
+            "Ey": Ey.cpu().numpy().reshape(K, 7), "Ez": Ez.cpu().numpy().reshape(K, 7), "nx": nx.cpu().numpy(),
+            "ny": ny.cpu().numpy(), "nz": nz.cpu().numpy()}
+    row0, rows = 37, 20
+    out = torch.empty(rows * K, dtype=torch.float64, device="cuda")
+    S.eri_kernel((1,), (128,), (row0, K, rows * K, p, P, Ex, Ey, Ez, nx, ny, nz, 7, 7, 7, out))
+    sub = {k: (v[row0:row0 + rows] if k != "K" else rows) for k, v in flat.items()}
+    ref = O.eri_prim(sub, flat)
+    assert np.allclose(out.cpu().numpy().reshape(rows, K), ref, rtol=1e-12, atol=1e-14)
+
+
+@pytest.mark.parametrize("same", [True, False])
+def test_fill_matrices_dropin(torch_cuda, same):
+    import oracle.md_oracle as O
+    from dft_amd import shelled as S
+    g = golden("dimer_sto3g.npz")
+    from dft_amd.basis import build_shells, load_basis
+    shells = build_shells(load_basis(), list(g["Z"]), g["xyz"])
+    n = g["ERI"].shape[0]
+    rng = np.random.default_rng(4)
+    A = rng.standard_normal((n, n))
+    Pa = A + A.T
+    Pb = Pa.copy() if same else (lambda B: B + B.T)(rng.standard_normal((n, n)))
+    J, Ka, Kb = np.ones((n, n)), np.zeros((n, n)), np.zeros((n, n))
+    S.fill_matrices(shells, Pa + Pb, Pa, Pb, J, Ka, Kb)
+    Jr, Kar, Kbr = O.jk_dense(g["ERI"], Pa + Pb, Pa, Pb)
+    assert np.abs(J - 1.0 - Jr).max() < 1e-10
+    assert np.abs(Ka - Kar).max() < 1e-10 and np.abs(Kb - Kbr).max() < 1e-10
+
+
+def test_calc_eri_store_and_process(torch_cuda, tmp_path):
+    from dft_amd import shelled as S
+    Z, xyz, shells = _water_shells()
+    g = golden("water_sto3g.npz")
+    pl = [(i, j) for i in range(len(shells)) for j in range(i + 1)]
+    S.calc_eri(pl, None, shells, out_dir=str(tmp_path))
+    S.process_eri(out_dir=str(tmp_path))
+    vals = np.load(tmp_path / "processed_eri_values.npy")
+    idx = np.load(tmp_path / "processed_eri_indices.npy")
+    quads = np.load(tmp_path / "processed_eri_quads.npy")
+    blocks = np.load(tmp_path / "block_idxs.npy")
+    shapes = np.load(tmp_path / "chunked_quad_shape.npy")
+    assert len(quads) == len(pl) * (len(pl) + 1) // 2
+    assert blocks[-1, 1] == len(quads) and (shapes[:, 0] >= 1).all()
+    for k in range(len(quads)):
+        a, b, c, d = quads[k]
+        s = [slice(shells[m]["ao_start"], shells[m]["ao_stop"]) for m in (a, b, c, d)]
+        ref = g["ERI"][s[0], s[1], s[2], s[3]].ravel()
+        assert np.abs(vals[idx[k, 0]:idx[k, 1]] - ref).max() < 1e-10
+
+
+def test_overlap_and_T_raw_match_oracle(torch_cuda):
+    import oracle.md_oracle as O
+    from dft_amd import shelled as S
+    a = np.array([0.3, 1.7, 5.0])
+    b = np.array([0.8, 2.2])
+    p1 = np.array([0, 1, 2])
+    p2 = np.array([1, 3])
+    got = S.overlap(a, b, 0.25, -0.6, p1, p2).cpu().numpy()
+    ref = O.overlap_1d(a, b, 0.25, -0.6, p1, p2)
+    assert np.allclose(got, ref, rtol=1e-12, atol=1e-15)
+    gt = S.T_raw(a, b, 0.25, -0.6, p1, p2, ref).cpu().numpy()
+    rt = O.T_raw(a, b, 0.25, -0.6, p1, p2, ref)
+    assert np.allclose(gt, rt, rtol=1e-11, atol=1e-14)
+
+
+def test_independent_matches_oracle():
+    import oracle.md_oracle as O
+    from dft_amd import shelled as S
+    from dft_amd.basis import build_shells_from_entries
+    ent = [[[[2], [0.9, 3.0], [[0.6, 0.8], [0.3, 0.1]]], [[3], [1.3], [[1.0]]], [[1], [0.5], [[1.0]]]]]
+    sh = build_shells_from_entries(ent, [[0, 0, 0]])
+    n_sph = sh[-1]["sphere_ao_stop"]
+    C = np.random.default_rng(0).standard_normal((sh[-1]["ao_stop"], 4))
+    assert np.allclose(S.independent(sh, n_sph, C), O.independent(sh, n_sph, C), atol=1e-14)
+
+
+def test_spherical_scf_methane_spd(torch_cuda):
+    import oracle.md_oracle as O
+    from dft_amd import shelled as S
+    from dft_amd.basis import build_shells_from_entries
+    from dft_amd.engine import Plan
+    from dft_amd.geometry import methane, synthetic_spd_basis
+    from dft_amd.scf import scf_plain
+    torch = torch_cuda
+    g = golden("methane_spd.npz")
+    Z, xyz = methane()
+    shells = build_shells_from_entries(synthetic_spd_basis(Z, seed=1), xyz)
+    n_sph = shells[-1]["sphere_ao_stop"]
+    A = S.spherical_matrix(shells, n_sph)
+    ERI = g["ERI"]
+
+    def jk(P, Pa, Pb):  # spherical densities -> Cartesian build -> spherical J/K
+        J, Ka, Kb = O.jk_dense(ERI, A.T @ P @ A, A.T @ Pa @ A, A.T @ Pb @ A)
+        return A @ J @ A.T, A @ Ka @ A.T, A @ Kb @ A.T
+    ref = O.scf_plain(A @ (g["T"] + g["V"]) @ A.T, A @ g["S"] @ A.T, jk, O.nuclear_repulsion(Z, xyz), 5, 5,
+                      max_count=300)
+    plan = Plan(shells, Z, xyz, schwarz_tau=1e-13)
+    res = scf_plain(plan, Z, xyz, 5, 5, A=torch.as_tensor(A, device="cuda"), max_count=300)
+    assert ref["converged"] and res.converged
+    assert abs(res.energy - ref["E"]) < 1e-8, (res.energy, ref["E"])
