@@ -27,7 +27,14 @@ SIGNATURES = {
     "dft_last_error": (C.c_char_p, []),
     "dft_plan_create": (C.c_int, [C.c_int, _i32p, _i32p, _i32p, _dp, _dp, _dp, C.c_int, _dp, _dp,
                                   C.c_int, C.c_double, C.c_double, C.c_int, C.POINTER(_vp)]),
+    "dft_plan_create_batch": (C.c_int, [C.c_int, _i32p, _i32p, _i32p, _i32p, _i32p, _i32p, _dp, _dp, _dp, _dp,
+                                        _dp, C.c_double, C.c_double, C.c_int, C.POINTER(_vp)]),
     "dft_plan_destroy": (C.c_int, [_vp]),
+    "dft_jk_graph_create": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int, C.POINTER(_vp)]),
+    "dft_graph_launch": (C.c_int, [_vp, _vp]),
+    "dft_graph_destroy": (C.c_int, [_vp]),
+    "dft_plan_tune": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.POINTER(C.c_int), C.c_int, _vp]),
+    "dft_plan_set_shapes": (C.c_int, [_vp, C.POINTER(C.c_int), C.c_int]),
     "dft_plan_stats": (C.c_int, [_vp, _i64p, C.c_int, C.POINTER(C.c_int), _i64p, _i64p]),
     "dft_schwarz": (C.c_int, [_vp, _dp]),
     "dft_jk": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int, _vp]),

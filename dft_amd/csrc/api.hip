@@ -1,6 +1,8 @@
 // C ABI of libdft_hip.so: plan construction (pair pre-pass, Schwarz bounds,
-// class-sorted screened quartet lists, reduction map) and the per-iteration
-// entry points. Declarations and conventions: include/dft_hip.h.
+// class-sorted screened quartet lists, reduction map) for one molecule or a
+// packed batch of molecules, the per-iteration entry points, hipGraph
+// capture of the Fock build and the launch-shape autotuner. Declarations and
+// conventions: include/dft_hip.h.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -26,9 +28,9 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
-#define HIP_TRY(expr)                                                                          \
-  do {                                                                                         \
-    hipError_t e_ = (expr);                                                                    \
+#define HIP_TRY(expr)                                                                                  \
+  do {                                                                                                 \
+    hipError_t e_ = (expr);                                                                            \
     if (e_ != hipSuccess) return fail(DFT_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
@@ -88,6 +90,15 @@ int upload(T** d, const std::vector<T>& h) {
   return DFT_OK;
 }
 
+int ncart_h(int l) { return (l + 1) * (l + 2) / 2; }
+
+// launch shape of one class-combination launch (chosen by dft_plan_tune)
+struct Shape {
+  int wave = 0;         // 0: lane-per-quartet kernel, 1: wave-per-quartet kernel
+  int block = 256;      // lane kernel workgroup size
+  int cap = 256 * 16;   // maximum workgroups (persistent grid-stride loop)
+};
+
 struct Combo {
   int pcb = 0, pck = 0;
   std::vector<int2> q;         // (bra pair, ket pair) plan pair ids: FP64 part, then weak part
@@ -95,19 +106,24 @@ struct Combo {
   int2* d_q = nullptr;
   long long* d_slots = nullptr;  // per quartet 2 + 4 nk staging offsets
   long long prim64 = 0, prim32 = 0;
+  Shape shape64, shape32;        // for the FP64 launch and the FP32 (weak part) launch
 };
 
 }  // namespace
 
 struct dft_plan_s {
-  int nsh = 0, nbf = 0, natom = 0, nk = 1;
+  int nsh = 0, natom = 0, nk = 1, nmol = 1;
+  long long nelem = 0;           // packed matrix elements (sum of nbf_m^2)
+  int max_nbf = 0;
   double tau = 0, tau32 = 0;
+  std::vector<int> mol_nbf;
+  std::vector<long long> mol_base;
   std::vector<ShellDev> sh;      // internal (l-sorted) order
   std::vector<int> in2int;       // input shell -> internal shell
   std::vector<double> exps, coefs;
   std::vector<int4> pairs;       // (sh_a, sh_b, rec_off, nprim_pairs), class-sorted
   std::vector<int> pair_pc;      // pair class per pair
-  std::vector<int> pair_of;      // internal (i,j), i >= j -> pair id, i*nsh + j
+  std::map<long long, int> pair_of;  // internal (i,j), i >= j -> pair id
   std::vector<double> Q;         // Schwarz per pair
   std::vector<Combo> combos;
   long long n_unscreened = 0;
@@ -118,33 +134,37 @@ struct dft_plan_s {
   double* d_exps = nullptr;
   double* d_coefs = nullptr;
   int4* d_pairs = nullptr;
+  int4* d_shell_dm = nullptr;
   int* d_shell_ao = nullptr;
   double* d_rec64 = nullptr;
   float* d_rec32 = nullptr;
   double* d_stage = nullptr;
-  int4* d_tg = nullptr;
+  Target* d_tg = nullptr;
   long long* d_tbase = nullptr;
   int* d_tcount = nullptr;
-  double* d_half = nullptr;  // (1 + nk) * nbf * nbf
-  double* d_atoms = nullptr;  // x, y, z, Z
+  double* d_half = nullptr;      // (1 + nk) * nelem
+  long long* d_mol_base = nullptr;
+  int* d_mol_nbf = nullptr;
+  double* d_atoms = nullptr;     // x, y, z, Z
   int2* d_shpairs = nullptr;
   int nshpairs = 0;
   double* d_boys = nullptr;
-  // optional per-launch timing: slot 2c = combo c FP64 part, 2c+1 = its weak
-  // part, then reduce, symmetrize; one begin/end event pair per slot and step
+  // optional per-launch timing: slot 2c = combo c FP64 launch, 2c+1 = its
+  // FP32 launch, then reduce, symmetrize; one begin/end event pair per slot and step
   int prof_steps = 0, prof_used = 0;
   std::vector<hipEvent_t> prof_ev;
-  // gradient data, built by the first dft_grad call
+  // gradient data, built by the first dft_grad call (single molecule)
   bool grad_ready = false;
   std::vector<int> grad_atom;     // internal shell -> atom
-  double* d_grec = nullptr;       // extended pair records
-  int4* d_gpairs = nullptr;       // (sh_a, sh_b, grec_off, nprim_pairs) per pair
-  std::vector<int2*> d_gq_swap;   // per combo: quartets with bra and ket exchanged
+  double* d_grec = nullptr;
+  int4* d_gpairs = nullptr;
+  std::vector<int2*> d_gq_swap;
   std::vector<long long*> d_gslots_ab, d_gslots_cd;
   long long* d_g1slots = nullptr;
   long long* d_gbase = nullptr;
   int* d_gcount = nullptr;
   double* d_gstage = nullptr;
+  bool tuned = false;
 };
 
 namespace {
@@ -155,19 +175,19 @@ void free_plan(dft_plan p) {
     (void)hipFree(c.d_q);
     (void)hipFree(c.d_slots);
   }
-  void* ptrs[] = {p->d_sh, p->d_exps, p->d_coefs, p->d_pairs, p->d_shell_ao, p->d_rec64, p->d_rec32,
-                  p->d_stage, p->d_tg, p->d_tbase, p->d_tcount, p->d_half, p->d_atoms, p->d_shpairs};
+  void* ptrs[] = {p->d_sh,    p->d_exps,  p->d_coefs,   p->d_pairs,    p->d_shell_dm,  p->d_shell_ao,
+                  p->d_rec64, p->d_rec32, p->d_stage,   p->d_tg,       p->d_tbase,     p->d_tcount,
+                  p->d_half,  p->d_atoms, p->d_shpairs, p->d_mol_base, p->d_mol_nbf,   p->d_grec,
+                  p->d_gpairs, p->d_g1slots, p->d_gbase, p->d_gcount, p->d_gstage};
   for (void* x : ptrs) (void)hipFree(x);
   for (hipEvent_t e : p->prof_ev) (void)hipEventDestroy(e);
   for (auto* x : p->d_gq_swap) (void)hipFree(x);
   for (auto* x : p->d_gslots_ab) (void)hipFree(x);
   for (auto* x : p->d_gslots_cd) (void)hipFree(x);
-  void* gptrs[] = {p->d_grec, p->d_gpairs, p->d_g1slots, p->d_gbase, p->d_gcount, p->d_gstage};
-  for (void* x : gptrs) (void)hipFree(x);
   delete p;
 }
 
-int ncart_h(int l) { return (l + 1) * (l + 2) / 2; }
+long long pair_key(int i, int j) { return ((long long)i << 32) | (unsigned)j; }
 
 // Raw contracted blocks for a list of (bra pair, ket pair) in one combo,
 // synchronous, host result.
@@ -190,7 +210,7 @@ int raw_blocks(dft_plan p, int combo, const std::vector<int2>& q, std::vector<do
     pc_moments(combo_pcb(combo), la, lb);
     pc_moments(combo_pck(combo), lc, ld);
     e = launch_eri_wave(la, lb, lc, ld, false, true, dq, nullptr, (long long)q.size(), p->d_pairs, p->d_rec64,
-                        nullptr, nullptr, nullptr, nullptr, 1, p->nbf, dout, p->d_boys, 1 << 20, nullptr);
+                        nullptr, nullptr, nullptr, nullptr, 1, dout, p->d_boys, 1 << 20, nullptr);
   }
   if (e == hipSuccess) e = hipMemcpy(out.data(), dout, out.size() * sizeof(double), hipMemcpyDeviceToHost);
   (void)hipFree(dq);
@@ -199,43 +219,79 @@ int raw_blocks(dft_plan p, int combo, const std::vector<int2>& q, std::vector<do
   return DFT_OK;
 }
 
-int build_plan(dft_plan p, int nsh, const int32_t* L, const int32_t* NP, const int32_t* AO,
-               const double* XYZ, const double* EX, const double* CO, int natom, const double* AXYZ,
-               const double* AZ) {
+// Input description of a (batch) plan: molecule m owns shells
+// [sh_begin[m], sh_begin[m+1]) and atoms [at_begin[m], at_begin[m+1]).
+struct Input {
+  int nmol;
+  std::vector<int> sh_begin, at_begin, nbf;
+  const int32_t *L, *NP, *AO;
+  const double *XYZ, *EX, *CO, *AXYZ, *AZ;
+};
+
+int build_plan(dft_plan p, const Input& in) {
+  const int nsh = p->nsh;
+  // ---- molecules: packed matrix bases
+  p->nmol = in.nmol;
+  p->mol_nbf = in.nbf;
+  p->mol_base.assign(in.nmol, 0);
+  long long nel = 0;
+  for (int m = 0; m < in.nmol; ++m) {
+    p->mol_base[m] = nel;
+    nel += (long long)in.nbf[m] * in.nbf[m];
+    p->max_nbf = std::max(p->max_nbf, in.nbf[m]);
+  }
+  p->nelem = nel;
+  std::vector<int> shell_mol(nsh);
+  for (int m = 0; m < in.nmol; ++m)
+    for (int i = in.sh_begin[m]; i < in.sh_begin[m + 1]; ++i) shell_mol[i] = m;
   // ---- shells in l-sorted (stable) internal order
   std::vector<int> order(nsh);
   for (int i = 0; i < nsh; ++i) order[i] = i;
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return L[a] < L[b]; });
-  std::vector<int> prim_in(nsh + 1, 0);
-  for (int i = 0; i < nsh; ++i) prim_in[i + 1] = prim_in[i] + NP[i];
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return in.L[a] < in.L[b]; });
+  std::vector<long long> prim_in(nsh + 1, 0);
+  for (int i = 0; i < nsh; ++i) prim_in[i + 1] = prim_in[i] + in.NP[i];
   p->in2int.assign(nsh, 0);
   p->sh.resize(nsh);
   for (int k = 0; k < nsh; ++k) {
     const int i = order[k];
+    const int m = shell_mol[i];
     p->in2int[i] = k;
     ShellDev s;
-    s.l = L[i];
-    s.nprim = NP[i];
+    s.l = in.L[i];
+    s.nprim = in.NP[i];
     s.prim_off = (int)p->exps.size();
-    s.ao = AO[i];
-    s.x = XYZ[3 * i];
-    s.y = XYZ[3 * i + 1];
-    s.z = XYZ[3 * i + 2];
-    for (int j = 0; j < NP[i]; ++j) {
-      const double a = EX[prim_in[i] + j];
+    s.ao = in.AO[i];
+    s.x = in.XYZ[3 * i];
+    s.y = in.XYZ[3 * i + 1];
+    s.z = in.XYZ[3 * i + 2];
+    s.atom_begin = in.at_begin[m];
+    s.atom_end = in.at_begin[m + 1];
+    s.nbf = in.nbf[m];
+    s.mol = m;
+    s.mbase = p->mol_base[m];
+    for (int j = 0; j < in.NP[i]; ++j) {
+      const double a = in.EX[prim_in[i] + j];
       p->exps.push_back(a);
-      p->coefs.push_back(CO[prim_in[i] + j] * radial_norm(a, s.l));
+      p->coefs.push_back(in.CO[prim_in[i] + j] * radial_norm(a, s.l));
     }
     p->sh[k] = s;
   }
-  // ---- pairs, sorted by pair class, then (i, j)
+  // ---- pairs (within one molecule), sorted by pair class, then (i, j)
+  std::vector<std::vector<int>> mol_shells(in.nmol);
+  for (int k = 0; k < nsh; ++k) mol_shells[p->sh[k].mol].push_back(k);
   std::vector<std::vector<int2>> by_pc(NPC);
-  for (int i = 0; i < nsh; ++i)
-    for (int j = 0; j <= i; ++j) by_pc[pc_index(p->sh[i].l, p->sh[j].l)].push_back(make_int2(i, j));
-  p->pair_of.assign((size_t)nsh * nsh, -1);
+  for (int m = 0; m < in.nmol; ++m) {
+    const auto& ms = mol_shells[m];
+    for (size_t a = 0; a < ms.size(); ++a)
+      for (size_t b = 0; b <= a; ++b) {
+        const int i = ms[a], j = ms[b];
+        by_pc[pc_index(p->sh[i].l, p->sh[j].l)].push_back(make_int2(i, j));
+      }
+  }
   std::vector<int4> pp;       // primitive pairs: sh_a, sh_b, ia, ib
   std::vector<int> pp_off;    // record offsets
   long long roff = 0;
+  std::vector<std::vector<std::vector<int>>> class_pairs(in.nmol, std::vector<std::vector<int>>(NPC));
   for (int pc = 0; pc < NPC; ++pc) {
     int la, lb;
     pc_moments(pc, la, lb);
@@ -245,16 +301,17 @@ int build_plan(dft_plan p, int nsh, const int32_t* L, const int32_t* NP, const i
       const int id = (int)p->pairs.size();
       p->pairs.push_back(make_int4(ij.x, ij.y, (int)roff, A.nprim * B.nprim));
       p->pair_pc.push_back(pc);
-      p->pair_of[(size_t)ij.x * nsh + ij.y] = id;
+      p->pair_of[pair_key(ij.x, ij.y)] = id;
+      class_pairs[A.mol][pc].push_back(id);
       for (int ia = 0; ia < A.nprim; ++ia)
         for (int ib = 0; ib < B.nprim; ++ib) {
           pp.push_back(make_int4(ij.x, ij.y, ia, ib));
           pp_off.push_back((int)roff);
           roff += rs;
         }
+      if (roff > 0x7fffffffLL) return fail(DFT_EINVAL, "pair records exceed 2^31 reals");
     }
   }
-  if (roff > 0x7fffffffLL) return fail(DFT_EINVAL, "pair records exceed 2^31 reals");
   // ---- device data + pair pre-pass
   int rc;
   if ((rc = boys_table_device(&p->d_boys))) return rc;
@@ -262,9 +319,17 @@ int build_plan(dft_plan p, int nsh, const int32_t* L, const int32_t* NP, const i
   if ((rc = upload(&p->d_exps, p->exps))) return rc;
   if ((rc = upload(&p->d_coefs, p->coefs))) return rc;
   if ((rc = upload(&p->d_pairs, p->pairs))) return rc;
+  std::vector<int4> dm(nsh);
   std::vector<int> ao(nsh);
-  for (int k = 0; k < nsh; ++k) ao[k] = p->sh[k].ao;
+  for (int k = 0; k < nsh; ++k) {
+    const long long mb = p->sh[k].mbase;
+    dm[k] = make_int4(p->sh[k].ao, p->sh[k].nbf, (int)(unsigned)(mb & 0xffffffffLL), (int)(mb >> 32));
+    ao[k] = p->sh[k].ao;
+  }
+  if ((rc = upload(&p->d_shell_dm, dm))) return rc;
   if ((rc = upload(&p->d_shell_ao, ao))) return rc;
+  if ((rc = upload(&p->d_mol_base, p->mol_base))) return rc;
+  if ((rc = upload(&p->d_mol_nbf, p->mol_nbf))) return rc;
   {
     int4* dpp = nullptr;
     int* doff = nullptr;
@@ -280,17 +345,20 @@ int build_plan(dft_plan p, int nsh, const int32_t* L, const int32_t* NP, const i
     if (e != hipSuccess) return fail(DFT_EHIP, std::string("pair pre-pass: ") + hipGetErrorString(e));
   }
   // ---- atoms and shell pairs for the 1-e kernel
-  std::vector<double> at(4 * (size_t)natom);
-  for (int a = 0; a < natom; ++a) {
-    at[4 * a] = AXYZ[3 * a];
-    at[4 * a + 1] = AXYZ[3 * a + 1];
-    at[4 * a + 2] = AXYZ[3 * a + 2];
-    at[4 * a + 3] = AZ[a];
+  std::vector<double> at(4 * (size_t)p->natom);
+  for (int a = 0; a < p->natom; ++a) {
+    at[4 * a] = in.AXYZ[3 * a];
+    at[4 * a + 1] = in.AXYZ[3 * a + 1];
+    at[4 * a + 2] = in.AXYZ[3 * a + 2];
+    at[4 * a + 3] = in.AZ[a];
   }
   if ((rc = upload(&p->d_atoms, at))) return rc;
   std::vector<int2> sp;
-  for (int i = 0; i < nsh; ++i)
-    for (int j = 0; j <= i; ++j) sp.push_back(make_int2(i, j));
+  for (int m = 0; m < in.nmol; ++m) {
+    const auto& ms = mol_shells[m];
+    for (size_t a = 0; a < ms.size(); ++a)
+      for (size_t b = 0; b <= a; ++b) sp.push_back(make_int2(ms[a], ms[b]));
+  }
   p->nshpairs = (int)sp.size();
   if ((rc = upload(&p->d_shpairs, sp))) return rc;
   // ---- Schwarz: diagonal quartets (ab|ab)
@@ -310,48 +378,64 @@ int build_plan(dft_plan p, int nsh, const int32_t* L, const int32_t* NP, const i
     std::vector<double> blk;
     if ((rc = raw_blocks(p, combo_index(pc, pc), q, blk, nab * nab))) return rc;
     for (size_t k = 0; k < ids.size(); ++k) {
-      double m = 0.0;
-      for (int x = 0; x < nab; ++x) m = std::max(m, std::fabs(blk[k * nab * nab + (size_t)x * nab + x]));
-      p->Q[ids[k]] = std::sqrt(m);
+      double mx = 0.0;
+      for (int x = 0; x < nab; ++x) mx = std::max(mx, std::fabs(blk[k * nab * nab + (size_t)x * nab + x]));
+      p->Q[ids[k]] = std::sqrt(mx);
     }
   }
-  // ---- screened, class-sorted quartet lists (FP64 part, then weak part)
+  // ---- screened, class-sorted quartet lists (FP64 part, then weak part),
+  // molecule-major within each class combination; bra and ket always come
+  // from the same molecule
   p->combos.assign(NCOMBO, Combo());
-  std::vector<std::vector<int>> class_pairs(NPC);
-  for (int id = 0; id < npair; ++id) class_pairs[p->pair_pc[id]].push_back(id);
   for (int pcb = 0; pcb < NPC; ++pcb)
     for (int pck = 0; pck <= pcb; ++pck) {
       Combo& c = p->combos[combo_index(pcb, pck)];
       c.pcb = pcb;
       c.pck = pck;
+      if (!lane_kernel_combo(combo_index(pcb, pck))) c.shape64.wave = c.shape32.wave = 1;
       std::vector<int2> weak;
-      for (int ib : class_pairs[pcb]) {
-        for (int ik : class_pairs[pck]) {
-          if (pcb == pck && ik > ib) break;
-          ++p->n_unscreened;
-          const double b = p->Q[ib] * p->Q[ik];
-          if (b < p->tau) continue;
-          const long long np = (long long)p->pairs[ib].w * p->pairs[ik].w;
-          if (b < p->tau32) {
-            weak.push_back(make_int2(ib, ik));
-            c.prim32 += np;
-          } else {
-            c.q.push_back(make_int2(ib, ik));
-            c.prim64 += np;
+      for (int m = 0; m < in.nmol; ++m)
+        for (int ib : class_pairs[m][pcb]) {
+          for (int ik : class_pairs[m][pck]) {
+            if (pcb == pck && ik > ib) break;
+            ++p->n_unscreened;
+            const double b = p->Q[ib] * p->Q[ik];
+            if (b < p->tau) continue;
+            const long long np = (long long)p->pairs[ib].w * p->pairs[ik].w;
+            if (b < p->tau32) {
+              weak.push_back(make_int2(ib, ik));
+              c.prim32 += np;
+            } else {
+              c.q.push_back(make_int2(ib, ik));
+              c.prim64 += np;
+            }
           }
         }
-      }
       c.n64 = (long long)c.q.size();
       c.q.insert(c.q.end(), weak.begin(), weak.end());
       if ((rc = upload(&c.d_q, c.q))) return rc;
     }
-  // ---- target-major staging: target t = ordered shell pair (X, Y) of half
-  // matrix m (0 = J, 1.. = K per density); its blocks are contiguous and
-  // ordered by (combo, quartet), so every sum has a fixed order.
+  // ---- target-major staging: a target is an ordered shell pair (X, Y) of
+  // one molecule in half matrix m (0 = J, 1.. = K per density); its blocks
+  // are contiguous and ordered by (combo, quartet), so every sum has a
+  // fixed order.
   const int nmat = 1 + p->nk;
   const int nblk = 2 + 4 * p->nk;
-  p->ntarget = nmat * nsh * nsh;
-  auto tid = [&](int m, int x, int y) { return ((size_t)m * nsh + x) * nsh + y; };
+  std::vector<long long> mol_toff(in.nmol + 1, 0);  // first target of molecule m (matrix 0)
+  std::vector<int> local(nsh);                       // index of the shell within its molecule
+  for (int m = 0; m < in.nmol; ++m) {
+    const long long ns = (long long)mol_shells[m].size();
+    mol_toff[m + 1] = mol_toff[m] + ns * ns;
+    for (size_t a = 0; a < mol_shells[m].size(); ++a) local[mol_shells[m][a]] = (int)a;
+  }
+  const long long tpm = mol_toff[in.nmol];  // targets per matrix kind
+  if (nmat * tpm > 0x7fffffffLL) return fail(DFT_EINVAL, "too many reduction targets");
+  p->ntarget = (int)(nmat * tpm);
+  auto tid = [&](int m, int x, int y) {
+    const int mol = p->sh[x].mol;
+    const long long ns = (long long)mol_shells[mol].size();
+    return (size_t)(m * tpm + mol_toff[mol] + local[x] * ns + local[y]);
+  };
   auto quartet_targets = [&](const int2& qq, size_t* t) {
     const int4 pb = p->pairs[qq.x], pk = p->pairs[qq.y];
     const int A = pb.x, B = pb.y, Cc = pk.x, D = pk.y;
@@ -372,26 +456,34 @@ int build_plan(dft_plan p, int nsh, const int32_t* L, const int32_t* NP, const i
       for (int k = 0; k < nblk; ++k) ++tcount[tq[k]];
     }
   std::vector<long long> tbase(p->ntarget, 0), tnext(p->ntarget, 0);
-  std::vector<int4> tg(p->ntarget);
+  std::vector<Target> tg(p->ntarget);
   long long cur = 0;
   for (int m = 0; m < nmat; ++m)
-    for (int x = 0; x < nsh; ++x)
-      for (int y = 0; y < nsh; ++y) {
-        const size_t t = tid(m, x, y);
-        const int nr = ncart_h(p->sh[x].l), nc = ncart_h(p->sh[y].l);
-        tg[t] = make_int4(m * p->nbf + p->sh[x].ao, p->sh[y].ao, nr, nc);
-        tbase[t] = tnext[t] = cur;
-        cur += (long long)tcount[t] * nr * nc;
-      }
+    for (int mol = 0; mol < in.nmol; ++mol)
+      for (int x : mol_shells[mol])
+        for (int y : mol_shells[mol]) {
+          const size_t t = tid(m, x, y);
+          const int nr = ncart_h(p->sh[x].l), nc = ncart_h(p->sh[y].l);
+          const int nbf = in.nbf[mol];
+          Target T;
+          T.base = m * p->nelem + p->mol_base[mol] + (long long)p->sh[x].ao * nbf + p->sh[y].ao;
+          T.nrow = nr;
+          T.ncol = nc;
+          T.stride = nbf;
+          T.pad = 0;
+          tg[t] = T;
+          tbase[t] = tnext[t] = cur;
+          cur += (long long)tcount[t] * nr * nc;
+        }
   p->stage_len = cur;
   for (Combo& c : p->combos) {
     std::vector<long long> slots(c.q.size() * nblk);
     for (size_t i = 0; i < c.q.size(); ++i) {
       quartet_targets(c.q[i], tq);
       for (int k = 0; k < nblk; ++k) {
-        const int4& T = tg[tq[k]];
+        const Target& T = tg[tq[k]];
         slots[i * nblk + k] = tnext[tq[k]];
-        tnext[tq[k]] += (long long)T.z * T.w;
+        tnext[tq[k]] += (long long)T.nrow * T.ncol;
       }
     }
     if ((rc = upload(&c.d_slots, slots))) return rc;
@@ -400,7 +492,74 @@ int build_plan(dft_plan p, int nsh, const int32_t* L, const int32_t* NP, const i
   if ((rc = upload(&p->d_tbase, tbase))) return rc;
   if ((rc = upload(&p->d_tcount, tcount))) return rc;
   HIP_TRY(hipMalloc(&p->d_stage, std::max<long long>(p->stage_len, 1) * sizeof(double)));
-  HIP_TRY(hipMalloc(&p->d_half, (size_t)nmat * p->nbf * p->nbf * sizeof(double)));
+  HIP_TRY(hipMalloc(&p->d_half, std::max<long long>(nmat * p->nelem, 1) * sizeof(double)));
+  return DFT_OK;
+}
+
+// One class-combination launch of the FP64 or FP32 kernel on quartets
+// [first, first + n) with the given launch shape.
+hipError_t launch_combo(dft_plan p, size_t ci, bool f32, long long first, long long n, const Shape& sh,
+                        const double* P, const double* Pa, const double* Pb, hipStream_t s) {
+  const Combo& c = p->combos[ci];
+  const int nblk = 2 + 4 * p->nk;
+  const void* rec = f32 ? (const void*)p->d_rec32 : (const void*)p->d_rec64;
+  if (!sh.wave && lane_kernel_combo((int)ci))
+    return launch_eri_jk((int)ci, f32, c.d_q + first, c.d_slots + first * nblk, n, p->d_pairs, rec, p->d_shell_dm, P,
+                         Pa, Pb, p->nk, p->d_stage, p->d_boys, sh.block, sh.cap, s);
+  int la, lb, lc, ld;
+  pc_moments(c.pcb, la, lb);
+  pc_moments(c.pck, lc, ld);
+  return launch_eri_wave(la, lb, lc, ld, f32, false, c.d_q + first, c.d_slots + first * nblk, n, p->d_pairs, rec,
+                         p->d_shell_dm, P, Pa, Pb, p->nk, p->d_stage, p->d_boys, sh.cap, s);
+}
+
+// The whole Fock build: class launches, reduction, symmetrisation.
+int jk_enqueue(dft_plan p, const double* P, const double* Pa, const double* Pb, double* J, double* Ka, double* Kb,
+               int mode, hipStream_t s, bool profile) {
+  const int nslot = 2 * (int)p->combos.size() + 2;
+  hipEvent_t* ev = nullptr;
+  if (profile && p->prof_used < p->prof_steps) ev = &p->prof_ev[(size_t)p->prof_used++ * nslot * 2];
+  auto mark = [&](int slot, int end) {
+    if (ev) (void)hipEventRecord(ev[2 * slot + end], s);
+  };
+  for (size_t ci = 0; ci < p->combos.size(); ++ci) {
+    const Combo& c = p->combos[ci];
+    const long long ntot = (long long)c.q.size();
+    // FP64 mode: one launch over the whole list; mixed: FP64 part, then the
+    // weak part through the FP32 kernel
+    const long long n1 = mode == DFT_MIXED ? c.n64 : ntot;
+    mark(2 * (int)ci, 0);
+    hipError_t e = launch_combo(p, ci, false, 0, n1, c.shape64, P, Pa, Pb, s);
+    if (e != hipSuccess) return fail(DFT_EHIP, std::string("eri_jk fp64: ") + hipGetErrorString(e));
+    mark(2 * (int)ci, 1);
+    mark(2 * (int)ci + 1, 0);
+    if (mode == DFT_MIXED) {
+      e = launch_combo(p, ci, true, c.n64, ntot - c.n64, c.shape32, P, Pa, Pb, s);
+      if (e != hipSuccess) return fail(DFT_EHIP, std::string("eri_jk fp32: ") + hipGetErrorString(e));
+    }
+    mark(2 * (int)ci + 1, 1);
+  }
+  mark(nslot - 2, 0);
+  hipError_t e = launch_reduce(p->ntarget, p->d_tg, p->d_tbase, p->d_tcount, p->d_stage, p->d_half, s);
+  if (e != hipSuccess) return fail(DFT_EHIP, std::string("reduce: ") + hipGetErrorString(e));
+  mark(nslot - 2, 1);
+  mark(nslot - 1, 0);
+  double* outs[3] = {J, Ka, Kb};
+  for (int m = 0; m < 1 + p->nk; ++m) {
+    e = launch_symmetrize(p->nmol, p->d_mol_base, p->d_mol_nbf, p->max_nbf, p->d_half + m * p->nelem, outs[m], s);
+    if (e != hipSuccess) return fail(DFT_EHIP, std::string("symmetrize: ") + hipGetErrorString(e));
+  }
+  mark(nslot - 1, 1);
+  return DFT_OK;
+}
+
+int check_jk_args(dft_plan p, const double* P, const double* Pa, const double* Pb, double* J, double* Ka, double* Kb,
+                  int mode) {
+  if (!p) return fail(DFT_EINVAL, "plan is NULL");
+  if (!P || !Pa || !J || !Ka) return fail(DFT_EINVAL, "P, Pa, J, Ka must be non-NULL");
+  if ((p->nk == 2) != (Pb != nullptr) || (p->nk == 2) != (Kb != nullptr))
+    return fail(DFT_EINVAL, "Pb/Kb must be given exactly for a UHF plan");
+  if (mode != DFT_FP64 && mode != DFT_MIXED) return fail(DFT_EINVAL, "mode must be DFT_FP64 or DFT_MIXED");
   return DFT_OK;
 }
 
@@ -418,7 +577,6 @@ int grad_prepare(dft_plan p, const int32_t* shell_atom) {
   }
   for (const auto& s : p->sh)
     if (s.l > 1) return fail(DFT_EUNSUPPORTED, "analytic gradients are built for s and p shells only");
-  // extended records
   std::vector<int4> pp, gp(p->pairs.size());
   std::vector<int> off;
   long long roff = 0;
@@ -448,18 +606,15 @@ int grad_prepare(dft_plan p, const int32_t* shell_atom) {
     (void)hipFree(doff);
     if (e != hipSuccess) return fail(DFT_EHIP, std::string("gradient records: ") + hipGetErrorString(e));
   }
-  // per-atom contribution counts
   const int natom = p->natom;
   std::vector<int> cnt(natom, 0);
-  auto atom_of_pair_shell = [&](int pid, int which) {
-    return p->grad_atom[which == 0 ? p->pairs[pid].x : p->pairs[pid].y];
-  };
+  auto atom_of = [&](int pid, int which) { return p->grad_atom[which == 0 ? p->pairs[pid].x : p->pairs[pid].y]; };
   for (const Combo& c : p->combos)
     for (const int2& q : c.q) {
-      ++cnt[atom_of_pair_shell(q.x, 0)];
-      ++cnt[atom_of_pair_shell(q.x, 1)];
-      ++cnt[atom_of_pair_shell(q.y, 0)];
-      ++cnt[atom_of_pair_shell(q.y, 1)];
+      ++cnt[atom_of(q.x, 0)];
+      ++cnt[atom_of(q.x, 1)];
+      ++cnt[atom_of(q.y, 0)];
+      ++cnt[atom_of(q.y, 1)];
     }
   const int nsp = p->nshpairs;
   std::vector<int2> sp;
@@ -486,12 +641,12 @@ int grad_prepare(dft_plan p, const int32_t* shell_atom) {
     std::vector<long long> sab(2 * c.q.size()), scd(2 * c.q.size());
     std::vector<int2> sw(c.q.size());
     for (size_t i = 0; i < c.q.size(); ++i) {
-      sab[2 * i] = slot(atom_of_pair_shell(c.q[i].x, 0));
-      sab[2 * i + 1] = slot(atom_of_pair_shell(c.q[i].x, 1));
+      sab[2 * i] = slot(atom_of(c.q[i].x, 0));
+      sab[2 * i + 1] = slot(atom_of(c.q[i].x, 1));
     }
     for (size_t i = 0; i < c.q.size(); ++i) {
-      scd[2 * i] = slot(atom_of_pair_shell(c.q[i].y, 0));
-      scd[2 * i + 1] = slot(atom_of_pair_shell(c.q[i].y, 1));
+      scd[2 * i] = slot(atom_of(c.q[i].y, 0));
+      scd[2 * i + 1] = slot(atom_of(c.q[i].y, 1));
       sw[i] = make_int2(c.q[i].y, c.q[i].x);
     }
     int2* dsw = nullptr;
@@ -519,42 +674,68 @@ int grad_prepare(dft_plan p, const int32_t* shell_atom) {
   return DFT_OK;
 }
 
+int validate_and_build(int nmol, const int32_t* mol_nsh, const int32_t* mol_natom, const int32_t* mol_nbf,
+                       const int32_t* shell_l, const int32_t* shell_nprim, const int32_t* shell_ao,
+                       const double* shell_xyz, const double* prim_exp, const double* prim_coef, const double* atom_xyz,
+                       const double* atom_charge, double schwarz_tau, double fp32_tau, int flags, dft_plan* out) {
+  if (!out) return fail(DFT_EINVAL, "out is NULL");
+  *out = nullptr;
+  if (nmol <= 0 || !mol_nsh || !mol_natom || !mol_nbf) return fail(DFT_EINVAL, "bad molecule arrays");
+  if (!shell_l || !shell_nprim || !shell_ao || !shell_xyz || !prim_exp || !prim_coef)
+    return fail(DFT_EINVAL, "NULL shell array");
+  if (!(schwarz_tau >= 0.0) || !(fp32_tau >= 0.0)) return fail(DFT_EINVAL, "thresholds must be >= 0");
+  Input in;
+  in.nmol = nmol;
+  in.sh_begin.assign(nmol + 1, 0);
+  in.at_begin.assign(nmol + 1, 0);
+  in.nbf.assign(nmol, 0);
+  for (int m = 0; m < nmol; ++m) {
+    if (mol_nsh[m] <= 0 || mol_nbf[m] <= 0 || mol_natom[m] < 0)
+      return fail(DFT_EINVAL, "molecule " + std::to_string(m) + ": nsh, nbf must be positive, natom >= 0");
+    in.sh_begin[m + 1] = in.sh_begin[m] + mol_nsh[m];
+    in.at_begin[m + 1] = in.at_begin[m] + mol_natom[m];
+    in.nbf[m] = mol_nbf[m];
+  }
+  const int nsh = in.sh_begin[nmol], natom = in.at_begin[nmol];
+  if (natom > 0 && (!atom_xyz || !atom_charge)) return fail(DFT_EINVAL, "NULL atom array");
+  for (int m = 0; m < nmol; ++m)
+    for (int i = in.sh_begin[m]; i < in.sh_begin[m + 1]; ++i) {
+      if (shell_l[i] < 0 || shell_l[i] > LMAX)
+        return fail(DFT_EUNSUPPORTED, "shell " + std::to_string(i) + ": angular momentum " +
+                                          std::to_string(shell_l[i]) + " not built (max " + std::to_string(LMAX) +
+                                          ")");
+      if (shell_nprim[i] <= 0) return fail(DFT_EINVAL, "shell " + std::to_string(i) + ": nprim <= 0");
+      if (shell_ao[i] < 0 || shell_ao[i] + ncart_h(shell_l[i]) > mol_nbf[m])
+        return fail(DFT_EINVAL, "shell " + std::to_string(i) + ": AO range outside [0, nbf)");
+    }
+  in.L = shell_l;
+  in.NP = shell_nprim;
+  in.AO = shell_ao;
+  in.XYZ = shell_xyz;
+  in.EX = prim_exp;
+  in.CO = prim_coef;
+  in.AXYZ = atom_xyz;
+  in.AZ = atom_charge;
+  dft_plan p = new dft_plan_s();
+  p->nsh = nsh;
+  p->natom = natom;
+  p->nk = (flags & DFT_PLAN_UHF) ? 2 : 1;
+  p->tau = schwarz_tau;
+  p->tau32 = fp32_tau;
+  int rc = build_plan(p, in);
+  if (rc) {
+    free_plan(p);
+    return rc;
+  }
+  *out = p;
+  return DFT_OK;
+}
+
 }  // namespace
 
 extern "C" {
 
-int dft_grad(dft_plan p, const int32_t* shell_atom, const double* P, const double* Pa, const double* Pb,
-             const double* W, double* grad, void* stream) {
-  if (!p) return fail(DFT_EINVAL, "plan is NULL");
-  if (!shell_atom || !P || !Pa || !W || !grad) return fail(DFT_EINVAL, "NULL argument");
-  if (p->natom <= 0) return fail(DFT_EINVAL, "plan has no nuclei");
-  if (!p->grad_ready) {
-    int rc = grad_prepare(p, shell_atom);
-    if (rc) return rc;
-  }
-  hipStream_t s = (hipStream_t)stream;
-  for (size_t ci = 0; ci < p->combos.size(); ++ci) {
-    const Combo& c = p->combos[ci];
-    if (c.q.empty()) continue;
-    int la, lb, lc, ld;
-    pc_moments(c.pcb, la, lb);
-    pc_moments(c.pck, lc, ld);
-    const long long nq = (long long)c.q.size();
-    hipError_t e = launch_eri_grad(la, lb, lc, ld, c.d_q, p->d_gslots_ab[ci], nq, p->d_gpairs, p->d_grec,
-                                   p->d_shell_ao, P, Pa, Pb, p->nbf, p->d_gstage, p->d_boys, s);
-    if (e == hipSuccess)
-      e = launch_eri_grad(lc, ld, la, lb, p->d_gq_swap[ci], p->d_gslots_cd[ci], nq, p->d_gpairs, p->d_grec,
-                          p->d_shell_ao, P, Pa, Pb, p->nbf, p->d_gstage, p->d_boys, s);
-    if (e != hipSuccess) return fail(DFT_EHIP, std::string("eri_grad: ") + hipGetErrorString(e));
-  }
-  hipError_t e = launch_grad1e(p->d_shpairs, p->nshpairs, p->d_sh, p->d_exps, p->d_coefs, p->natom, p->d_atoms, P, W,
-                               p->nbf, p->d_g1slots, p->d_gstage, p->d_boys, s);
-  if (e == hipSuccess) e = launch_grad_reduce(p->natom, p->d_gbase, p->d_gcount, p->d_gstage, grad, s);
-  if (e != hipSuccess) return fail(DFT_EHIP, std::string("grad 1e/reduce: ") + hipGetErrorString(e));
-  return DFT_OK;
-}
-
-int dft_version(void) { return 100; }
+int dft_version(void) { return 200; }
 
 const char* dft_last_error(void) { return g_err.c_str(); }
 
@@ -562,36 +743,22 @@ int dft_plan_create(int nsh, const int32_t* shell_l, const int32_t* shell_nprim,
                     const double* shell_xyz, const double* prim_exp, const double* prim_coef, int natom,
                     const double* atom_xyz, const double* atom_charge, int nbf, double schwarz_tau,
                     double fp32_tau, int flags, dft_plan* out) {
-  if (!out) return fail(DFT_EINVAL, "out is NULL");
-  *out = nullptr;
-  if (nsh <= 0 || nbf <= 0 || natom < 0) return fail(DFT_EINVAL, "nsh, nbf must be positive, natom >= 0");
-  if (!shell_l || !shell_nprim || !shell_ao || !shell_xyz || !prim_exp || !prim_coef)
-    return fail(DFT_EINVAL, "NULL shell array");
-  if (natom > 0 && (!atom_xyz || !atom_charge)) return fail(DFT_EINVAL, "NULL atom array");
-  if (!(schwarz_tau >= 0.0) || !(fp32_tau >= 0.0)) return fail(DFT_EINVAL, "thresholds must be >= 0");
-  for (int i = 0; i < nsh; ++i) {
-    if (shell_l[i] < 0 || shell_l[i] > LMAX)
-      return fail(DFT_EUNSUPPORTED, "shell " + std::to_string(i) + ": angular momentum " +
-                                        std::to_string(shell_l[i]) + " not built (max " + std::to_string(LMAX) + ")");
-    if (shell_nprim[i] <= 0) return fail(DFT_EINVAL, "shell " + std::to_string(i) + ": nprim <= 0");
-    if (shell_ao[i] < 0 || shell_ao[i] + ncart_h(shell_l[i]) > nbf)
-      return fail(DFT_EINVAL, "shell " + std::to_string(i) + ": AO range outside [0, nbf)");
+  if (nsh <= 0 || nbf <= 0 || natom < 0) {
+    if (out) *out = nullptr;
+    return fail(DFT_EINVAL, "nsh, nbf must be positive, natom >= 0");
   }
-  dft_plan p = new dft_plan_s();
-  p->nsh = nsh;
-  p->nbf = nbf;
-  p->natom = natom;
-  p->nk = (flags & DFT_PLAN_UHF) ? 2 : 1;
-  p->tau = schwarz_tau;
-  p->tau32 = fp32_tau;
-  int rc = build_plan(p, nsh, shell_l, shell_nprim, shell_ao, shell_xyz, prim_exp, prim_coef, natom, atom_xyz,
-                      atom_charge);
-  if (rc) {
-    free_plan(p);
-    return rc;
-  }
-  *out = p;
-  return DFT_OK;
+  const int32_t ns = nsh, na = natom, nb = nbf;
+  return validate_and_build(1, &ns, &na, &nb, shell_l, shell_nprim, shell_ao, shell_xyz, prim_exp, prim_coef,
+                            atom_xyz, atom_charge, schwarz_tau, fp32_tau, flags, out);
+}
+
+int dft_plan_create_batch(int nmol, const int32_t* mol_nsh, const int32_t* mol_natom, const int32_t* mol_nbf,
+                          const int32_t* shell_l, const int32_t* shell_nprim, const int32_t* shell_ao,
+                          const double* shell_xyz, const double* prim_exp, const double* prim_coef,
+                          const double* atom_xyz, const double* atom_charge, double schwarz_tau, double fp32_tau,
+                          int flags, dft_plan* out) {
+  return validate_and_build(nmol, mol_nsh, mol_natom, mol_nbf, shell_l, shell_nprim, shell_ao, shell_xyz, prim_exp,
+                            prim_coef, atom_xyz, atom_charge, schwarz_tau, fp32_tau, flags, out);
 }
 
 int dft_plan_destroy(dft_plan plan) {
@@ -617,73 +784,162 @@ int dft_plan_stats(dft_plan p, int64_t* counts, int ncombo, int* ncombo_out, int
 
 int dft_schwarz(dft_plan p, double* Q) {
   if (!p || !Q) return fail(DFT_EINVAL, "NULL argument");
+  if (p->nmol != 1) return fail(DFT_EINVAL, "dft_schwarz takes a single-molecule plan");
   const int n = p->nsh;
   for (int i = 0; i < n; ++i)
     for (int j = 0; j < n; ++j) {
       int a = p->in2int[i], b = p->in2int[j];
       if (a < b) std::swap(a, b);
-      Q[(size_t)i * n + j] = p->Q[p->pair_of[(size_t)a * n + b]];
+      Q[(size_t)i * n + j] = p->Q[p->pair_of.at(pair_key(a, b))];
     }
   return DFT_OK;
 }
 
 int dft_jk(dft_plan p, const double* P, const double* Pa, const double* Pb, double* J, double* Ka, double* Kb,
            int mode, void* stream) {
-  if (!p) return fail(DFT_EINVAL, "plan is NULL");
-  if (!P || !Pa || !J || !Ka) return fail(DFT_EINVAL, "P, Pa, J, Ka must be non-NULL");
-  if ((p->nk == 2) != (Pb != nullptr) || (p->nk == 2) != (Kb != nullptr))
-    return fail(DFT_EINVAL, "Pb/Kb must be given exactly for a UHF plan");
-  if (mode != DFT_FP64 && mode != DFT_MIXED) return fail(DFT_EINVAL, "mode must be DFT_FP64 or DFT_MIXED");
+  int rc = check_jk_args(p, P, Pa, Pb, J, Ka, Kb, mode);
+  if (rc) return rc;
+  return jk_enqueue(p, P, Pa, Pb, J, Ka, Kb, mode, (hipStream_t)stream, true);
+}
+
+int dft_jk_graph_create(dft_plan p, const double* P, const double* Pa, const double* Pb, double* J, double* Ka,
+                        double* Kb, int mode, dft_graph* out) {
+  if (!out) return fail(DFT_EINVAL, "out is NULL");
+  *out = nullptr;
+  int rc = check_jk_args(p, P, Pa, Pb, J, Ka, Kb, mode);
+  if (rc) return rc;
+  hipStream_t cs = nullptr;
+  HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    rc = jk_enqueue(p, P, Pa, Pb, J, Ka, Kb, mode, cs, false);
+    hipError_t e2 = hipStreamEndCapture(cs, &graph);
+    if (rc == DFT_OK && e2 != hipSuccess) e = e2;
+  }
+  hipGraphExec_t exec = nullptr;
+  if (e == hipSuccess && rc == DFT_OK) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (graph) (void)hipGraphDestroy(graph);
+  (void)hipStreamDestroy(cs);
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(DFT_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
+  *out = (dft_graph)exec;
+  return DFT_OK;
+}
+
+int dft_graph_launch(dft_graph g, void* stream) {
+  if (!g) return fail(DFT_EINVAL, "graph is NULL");
+  HIP_TRY(hipGraphLaunch((hipGraphExec_t)g, (hipStream_t)stream));
+  return DFT_OK;
+}
+
+int dft_graph_destroy(dft_graph g) {
+  if (g) (void)hipGraphExecDestroy((hipGraphExec_t)g);
+  return DFT_OK;
+}
+
+int dft_plan_tune(dft_plan p, const double* P, const double* Pa, const double* Pb, int reps, int* report,
+                  int report_cap, void* stream) {
+  if (!p || !P || !Pa) return fail(DFT_EINVAL, "NULL argument");
+  if ((p->nk == 2) != (Pb != nullptr)) return fail(DFT_EINVAL, "Pb must be given exactly for a UHF plan");
+  if (reps < 1) reps = 1;
   hipStream_t s = (hipStream_t)stream;
-  const int cap = 256 * 16;
-  const int nslot = 2 * (int)p->combos.size() + 2;
-  hipEvent_t* ev = nullptr;
-  if (p->prof_used < p->prof_steps) ev = &p->prof_ev[(size_t)p->prof_used++ * nslot * 2];
-  auto mark = [&](int slot, int end) {
-    if (ev) (void)hipEventRecord(ev[2 * slot + end], s);
-  };
-  const int nblk = 2 + 4 * p->nk;
-  for (size_t ci = 0; ci < p->combos.size(); ++ci) {
-    const Combo& c = p->combos[ci];
-    const long long ntot = (long long)c.q.size();
-    // FP64 mode: one launch over the whole list; mixed: FP64 part, then the
-    // weak part through the FP32 kernel
-    const long long n1 = mode == DFT_MIXED ? c.n64 : ntot;
-    const bool lane = lane_kernel_combo((int)ci);
-    int la, lb, lc, ld;
-    pc_moments(c.pcb, la, lb);
-    pc_moments(c.pck, lc, ld);
-    auto launch = [&](bool f32, long long first, long long n) {
-      const void* rec = f32 ? (const void*)p->d_rec32 : (const void*)p->d_rec64;
-      if (lane)
-        return launch_eri_jk((int)ci, f32, c.d_q + first, c.d_slots + first * nblk, n, p->d_pairs, rec,
-                             p->d_shell_ao, P, Pa, Pb, p->nk, p->nbf, p->d_stage, p->d_boys, cap, s);
-      return launch_eri_wave(la, lb, lc, ld, f32, false, c.d_q + first, c.d_slots + first * nblk, n, p->d_pairs,
-                             rec, p->d_shell_ao, P, Pa, Pb, p->nk, p->nbf, p->d_stage, p->d_boys, 256 * 64, s);
-    };
-    mark(2 * (int)ci, 0);
-    hipError_t e = launch(false, 0, n1);
-    if (e != hipSuccess) return fail(DFT_EHIP, std::string("eri_jk fp64: ") + hipGetErrorString(e));
-    mark(2 * (int)ci, 1);
-    mark(2 * (int)ci + 1, 0);
-    if (mode == DFT_MIXED) {
-      e = launch(true, c.n64, ntot - c.n64);
-      if (e != hipSuccess) return fail(DFT_EHIP, std::string("eri_jk fp32: ") + hipGetErrorString(e));
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  // candidate launch shapes: lane kernel (block x grid cap) and wave kernel
+  std::vector<Shape> lane_c, wave_c;
+  for (int b : {64, 128, 256})
+    for (int cap : {256 * 2, 256 * 4, 256 * 8, 256 * 16, 256 * 32}) {
+      Shape sh;
+      sh.wave = 0;
+      sh.block = b;
+      sh.cap = cap;
+      lane_c.push_back(sh);
     }
-    mark(2 * (int)ci + 1, 1);
+  for (int cap : {256 * 4, 256 * 16, 256 * 64}) {
+    Shape sh;
+    sh.wave = 1;
+    sh.cap = cap;
+    wave_c.push_back(sh);
   }
-  mark(nslot - 2, 0);
-  hipError_t e = launch_reduce(p->ntarget, p->d_tg, p->d_tbase, p->d_tcount, p->d_stage, p->d_half, p->nbf, s);
-  if (e != hipSuccess) return fail(DFT_EHIP, std::string("reduce: ") + hipGetErrorString(e));
-  mark(nslot - 2, 1);
-  mark(nslot - 1, 0);
-  const size_t nn = (size_t)p->nbf * p->nbf;
-  double* outs[3] = {J, Ka, Kb};
-  for (int m = 0; m < 1 + p->nk; ++m) {
-    e = launch_symmetrize(p->d_half + m * nn, outs[m], p->nbf, s);
-    if (e != hipSuccess) return fail(DFT_EHIP, std::string("symmetrize: ") + hipGetErrorString(e));
+  int rc = DFT_OK;
+  for (size_t ci = 0; ci < p->combos.size() && rc == DFT_OK; ++ci) {
+    Combo& c = p->combos[ci];
+    const long long ntot = (long long)c.q.size();
+    for (int part = 0; part < 2 && rc == DFT_OK; ++part) {
+      // part 0: whole list in FP64 (used by the FP64 mode and, for its FP64
+      // part, by the mixed mode); part 1: weak part in FP32
+      const long long first = part ? c.n64 : 0, n = part ? ntot - c.n64 : ntot;
+      if (n <= 0) continue;
+      std::vector<Shape> cand = wave_c;
+      if (lane_kernel_combo((int)ci)) cand.insert(cand.begin(), lane_c.begin(), lane_c.end());
+      float best = 1e30f;
+      Shape best_sh = part ? c.shape32 : c.shape64;
+      for (const Shape& sh : cand) {
+        hipError_t e = launch_combo(p, ci, part == 1, first, n, sh, P, Pa, Pb, s);  // warm-up
+        std::vector<float> t;
+        for (int r = 0; r < reps && e == hipSuccess; ++r) {
+          (void)hipEventRecord(e0, s);
+          e = launch_combo(p, ci, part == 1, first, n, sh, P, Pa, Pb, s);
+          (void)hipEventRecord(e1, s);
+          if (e == hipSuccess) e = hipEventSynchronize(e1);
+          float ms = 0.f;
+          if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+          t.push_back(ms);
+        }
+        if (e != hipSuccess) {
+          rc = fail(DFT_EHIP, std::string("tune: ") + hipGetErrorString(e));
+          break;
+        }
+        std::sort(t.begin(), t.end());
+        const float med = t[t.size() / 2];
+        if (med < best) {
+          best = med;
+          best_sh = sh;
+        }
+      }
+      if (part == 0) {
+        c.shape64 = best_sh;
+        c.shape32 = best_sh;  // default for the FP32 launch until tuned below
+      } else {
+        c.shape32 = best_sh;
+      }
+    }
   }
-  mark(nslot - 1, 1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  p->tuned = true;
+  if (report)
+    for (size_t ci = 0; ci < p->combos.size() && (int)(6 * ci + 5) < report_cap; ++ci) {
+      const Combo& c = p->combos[ci];
+      const int v[6] = {c.shape64.wave, c.shape64.block, c.shape64.cap, c.shape32.wave, c.shape32.block, c.shape32.cap};
+      for (int k = 0; k < 6; ++k) report[6 * ci + k] = v[k];
+    }
+  return DFT_OK;
+}
+
+int dft_plan_set_shapes(dft_plan p, const int* shapes, int n) {
+  if (!p || !shapes) return fail(DFT_EINVAL, "NULL argument");
+  for (size_t ci = 0; ci < p->combos.size() && (int)(6 * ci + 5) < n; ++ci) {
+    Combo& c = p->combos[ci];
+    Shape s64, s32;
+    s64.wave = shapes[6 * ci];
+    s64.block = shapes[6 * ci + 1];
+    s64.cap = shapes[6 * ci + 2];
+    s32.wave = shapes[6 * ci + 3];
+    s32.block = shapes[6 * ci + 4];
+    s32.cap = shapes[6 * ci + 5];
+    for (const Shape* s : {&s64, &s32}) {
+      if (!s->wave && !lane_kernel_combo((int)ci)) return fail(DFT_EINVAL, "combination has no lane kernel");
+      if (!s->wave && s->block != 64 && s->block != 128 && s->block != 256)
+        return fail(DFT_EINVAL, "block must be 64, 128 or 256");
+      if (s->cap < 1) return fail(DFT_EINVAL, "grid cap must be positive");
+    }
+    c.shape64 = s64;
+    c.shape32 = s32;
+  }
   return DFT_OK;
 }
 
@@ -729,7 +985,6 @@ int dft_eri_quartets(dft_plan p, const int32_t* quartets, int64_t nq, double* ou
   };
   std::vector<std::vector<Req>> by_combo(p->combos.size());
   long long off = 0;
-  std::vector<long long> offs(nq);
   for (int64_t r = 0; r < nq; ++r) {
     int s[4], ii[4];
     for (int k = 0; k < 4; ++k) {
@@ -737,12 +992,15 @@ int dft_eri_quartets(dft_plan p, const int32_t* quartets, int64_t nq, double* ou
       if (s[k] < 0 || s[k] >= n) return fail(DFT_EINVAL, "quartet shell index out of range");
       ii[k] = p->in2int[s[k]];
     }
+    if (p->sh[ii[0]].mol != p->sh[ii[1]].mol || p->sh[ii[0]].mol != p->sh[ii[2]].mol ||
+        p->sh[ii[0]].mol != p->sh[ii[3]].mol)
+      return fail(DFT_EINVAL, "quartet shells belong to different molecules");
     // positions of the internal bra (A >= B) and ket (C >= D)
     int pos[4] = {0, 1, 2, 3};
     if (ii[0] < ii[1]) std::swap(pos[0], pos[1]);
     if (ii[2] < ii[3]) std::swap(pos[2], pos[3]);
-    int pb = p->pair_of[(size_t)ii[pos[0]] * n + ii[pos[1]]];
-    int pk = p->pair_of[(size_t)ii[pos[2]] * n + ii[pos[3]]];
+    int pb = p->pair_of.at(pair_key(ii[pos[0]], ii[pos[1]]));
+    int pk = p->pair_of.at(pair_key(ii[pos[2]], ii[pos[3]]));
     if (p->pair_pc[pb] < p->pair_pc[pk]) {
       std::swap(pb, pk);
       std::swap(pos[0], pos[2]);
@@ -791,9 +1049,42 @@ int dft_eri_quartets(dft_plan p, const int32_t* quartets, int64_t nq, double* ou
 
 int dft_oneelec(dft_plan p, double* S, double* T, double* V, void* stream) {
   if (!p || !S || !T || !V) return fail(DFT_EINVAL, "NULL argument");
-  hipError_t e = launch_one_electron(p->d_shpairs, p->nshpairs, p->d_sh, p->d_exps, p->d_coefs, p->natom,
-                                     p->d_atoms, S, T, V, p->nbf, p->d_boys, (hipStream_t)stream);
+  hipError_t e = launch_one_electron(p->d_shpairs, p->nshpairs, p->d_sh, p->d_exps, p->d_coefs, p->d_atoms, S, T, V,
+                                     p->d_boys, (hipStream_t)stream);
   if (e != hipSuccess) return fail(DFT_EHIP, std::string("one-electron: ") + hipGetErrorString(e));
+  return DFT_OK;
+}
+
+int dft_grad(dft_plan p, const int32_t* shell_atom, const double* P, const double* Pa, const double* Pb,
+             const double* W, double* grad, void* stream) {
+  if (!p) return fail(DFT_EINVAL, "plan is NULL");
+  if (!shell_atom || !P || !Pa || !W || !grad) return fail(DFT_EINVAL, "NULL argument");
+  if (p->natom <= 0) return fail(DFT_EINVAL, "plan has no nuclei");
+  if (p->nmol != 1) return fail(DFT_EUNSUPPORTED, "gradients take a single-molecule plan");
+  if (!p->grad_ready) {
+    int rc = grad_prepare(p, shell_atom);
+    if (rc) return rc;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const int nbf = p->mol_nbf[0];
+  for (size_t ci = 0; ci < p->combos.size(); ++ci) {
+    const Combo& c = p->combos[ci];
+    if (c.q.empty()) continue;
+    int la, lb, lc, ld;
+    pc_moments(c.pcb, la, lb);
+    pc_moments(c.pck, lc, ld);
+    const long long nq = (long long)c.q.size();
+    hipError_t e = launch_eri_grad(la, lb, lc, ld, c.d_q, p->d_gslots_ab[ci], nq, p->d_gpairs, p->d_grec,
+                                   p->d_shell_ao, P, Pa, Pb, nbf, p->d_gstage, p->d_boys, s);
+    if (e == hipSuccess)
+      e = launch_eri_grad(lc, ld, la, lb, p->d_gq_swap[ci], p->d_gslots_cd[ci], nq, p->d_gpairs, p->d_grec,
+                          p->d_shell_ao, P, Pa, Pb, nbf, p->d_gstage, p->d_boys, s);
+    if (e != hipSuccess) return fail(DFT_EHIP, std::string("eri_grad: ") + hipGetErrorString(e));
+  }
+  hipError_t e = launch_grad1e(p->d_shpairs, p->nshpairs, p->d_sh, p->d_exps, p->d_coefs, p->natom, p->d_atoms, P, W,
+                               nbf, p->d_g1slots, p->d_gstage, p->d_boys, s);
+  if (e == hipSuccess) e = launch_grad_reduce(p->natom, p->d_gbase, p->d_gcount, p->d_gstage, grad, s);
+  if (e != hipSuccess) return fail(DFT_EHIP, std::string("grad 1e/reduce: ") + hipGetErrorString(e));
   return DFT_OK;
 }
 

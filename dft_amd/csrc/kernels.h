@@ -1,5 +1,11 @@
 // Host-visible launch entry points of the device kernels (implemented in the
 // kernels_*.hip translation units). All pointers are device pointers.
+//
+// Matrices (densities, J, K, S, T, V) are packed per molecule: molecule m's
+// nbf_m x nbf_m row-major block starts at element mbase_m. A single-molecule
+// plan is the case of one block at offset 0. Every shell carries its
+// molecule's (mbase, nbf) in a packed int4 "shell_dm" = {ao, nbf, mbase lo,
+// mbase hi}.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
@@ -8,7 +14,7 @@ namespace dft {
 
 // number of pair classes / class combinations for shells up to l = 2 (s, p, d)
 constexpr int LMAX = 2;
-constexpr int NPC = (LMAX + 1) * (LMAX + 2) / 2;   // pair classes: ss, ps, pp
+constexpr int NPC = (LMAX + 1) * (LMAX + 2) / 2;   // pair classes: ss, ps, pp, ds, dp, dd
 constexpr int NCOMBO = NPC * (NPC + 1) / 2;        // (bra class >= ket class)
 
 inline int pc_index(int la, int lb) { return la * (la + 1) / 2 + lb; }
@@ -28,6 +34,14 @@ inline int combo_pck(int combo) { return combo - combo_pcb(combo) * (combo_pcb(c
 struct ShellDev {      // one contracted (segmented) shell, internal order
   int l, nprim, prim_off, ao;
   double x, y, z;
+  int atom_begin, atom_end;  // nuclei of the shell's molecule (for V)
+  int nbf, mol;
+  long long mbase;           // packed-matrix offset of the molecule
+};
+
+struct Target {        // one ordered shell pair (X, Y) of a half matrix
+  long long base;      // element offset of (X row 0, Y col 0) in the half array
+  int nrow, ncol, stride, pad;
 };
 
 // pair pre-pass: one lane per primitive pair; writes FP64 and FP32 records
@@ -36,47 +50,47 @@ hipError_t launch_pair_records(const int4* pairs_prim /* sh_a, sh_b, i_a, i_b */
                                const ShellDev* shells, const double* exps, const double* coefs,
                                double* rec64, float* rec32, hipStream_t s);
 
-// fused ERI + J/K digest for one class combination; slots holds 2 + 4*nk
-// staging offsets (doubles) per quartet.
-hipError_t launch_eri_jk(int combo, bool fp32, const int2* quartets, const long long* slots, long long nq,
-                         const int4* pairs, const void* rec, const int* shell_ao,
-                         const double* DJ, const double* DK0, const double* DK1, int nk, int nbf,
-                         double* stage, const double* boys_tab, int grid_cap, hipStream_t s);
-
 // true if the lane-per-quartet kernel is instantiated for this combination
 // (s/p classes and the light d classes); the others use the wave kernel
 bool lane_kernel_combo(int combo);
+
+// fused ERI + J/K digest for one class combination (lane per quartet);
+// slots holds 2 + 4*nk staging offsets (doubles) per quartet; block is the
+// workgroup size (64, 128 or 256), grid_cap the maximum number of workgroups
+hipError_t launch_eri_jk(int combo, bool fp32, const int2* quartets, const long long* slots, long long nq,
+                         const int4* pairs, const void* rec, const int4* shell_dm,
+                         const double* DJ, const double* DK0, const double* DK1, int nk,
+                         double* stage, const double* boys_tab, int block, int grid_cap, hipStream_t s);
 
 // one-wave-per-quartet kernel (any class up to (dd|dd)); raw = write the
 // integral blocks to out + q * NA*NB*NC*ND instead of J/K contributions
 hipError_t launch_eri_wave(int la, int lb, int lc, int ld, bool fp32, bool raw, const int2* quartets,
                            const long long* slots, long long nq, const int4* pairs, const void* rec,
-                           const int* shell_ao, const double* DJ, const double* DK0, const double* DK1, int nk,
-                           int nbf, double* out, const double* boys_tab, int grid_cap, hipStream_t s);
+                           const int4* shell_dm, const double* DJ, const double* DK0, const double* DK1, int nk,
+                           double* out, const double* boys_tab, int grid_cap, hipStream_t s);
 
 // raw contracted blocks (internal orientation) for one class combination:
 // out + q * NA*NB*NC*ND
 hipError_t launch_eri_raw(int combo, const int2* quartets, long long nq, const int4* pairs,
                           const double* rec, double* out, const double* boys_tab, hipStream_t s);
 
-// deterministic target reduction: one wave per target block; the blocks of
-// target t are contiguous: stage[tgt_base[t] + k * nrow * ncol + e], k < tgt_count[t]
-hipError_t launch_reduce(int ntarget, const int4* targets /* row0, col0, nrow, ncol */,
-                         const long long* tgt_base, const int* tgt_count, const double* stage,
-                         double* half, int nbf, hipStream_t s);
+// deterministic target reduction: one wave per target; the blocks of target
+// t are contiguous: stage[tgt_base[t] + k * nrow * ncol + e], k < tgt_count[t]
+hipError_t launch_reduce(int ntarget, const Target* targets, const long long* tgt_base, const int* tgt_count,
+                         const double* stage, double* half, hipStream_t s);
 
-// M = H + H^T for nmat matrices of nbf x nbf
-hipError_t launch_symmetrize(const double* half, double* out, int nbf, hipStream_t s);
+// M = H + H^T per molecule block (packed layout), for one half matrix
+hipError_t launch_symmetrize(int nmol, const long long* mol_base, const int* mol_nbf, int max_nbf,
+                             const double* half, double* out, hipStream_t s);
 
-// S, T, V over shell pairs (one lane per component pair of a shell pair)
+// S, T, V over shell pairs (one wave per shell pair), packed outputs
 hipError_t launch_one_electron(const int2* shell_pairs, int npairs, const ShellDev* shells,
-                               const double* exps, const double* coefs, int natom,
-                               const double* atom_xyzq, double* S, double* T, double* V, int nbf,
-                               const double* boys_tab, hipStream_t s);
+                               const double* exps, const double* coefs, const double* atom_xyzq, double* S,
+                               double* T, double* V, const double* boys_tab, hipStream_t s);
 
-// analytic gradients (s/p): extended pair records, derivative ERIs
-// contracted with the density pair (bra-centre derivatives; call again with
-// bra and ket exchanged for the ket centres), 1-e derivatives and the
+// analytic gradients (s/p, one molecule): extended pair records, derivative
+// ERIs contracted with the density pair (bra-centre derivatives; call again
+// with bra and ket exchanged for the ket centres), 1-e derivatives and the
 // per-atom reduction. Every contribution is 3 doubles at its own slot.
 hipError_t launch_grad_records(const int4* pairs_prim, int n, const int* rec_off, const ShellDev* shells,
                                const double* exps, const double* coefs, double* out, hipStream_t s);

@@ -21,8 +21,8 @@ constexpr int ERI_BLOCK = 256;
 template <int LA, int LB, int LC, int LD, typename real, int NK>
 __global__ __launch_bounds__(ERI_BLOCK) void eri_jk_kernel(
     const int2* __restrict__ quartets, const long long* __restrict__ slots, long long nq,
-    const int4* __restrict__ pairs, const real* __restrict__ rec, const int* __restrict__ shell_ao,
-    const double* __restrict__ DJ, const double* __restrict__ DK0, const double* __restrict__ DK1, int nbf,
+    const int4* __restrict__ pairs, const real* __restrict__ rec, const int4* __restrict__ shell_dm,
+    const double* __restrict__ DJ, const double* __restrict__ DK0, const double* __restrict__ DK1,
     double* __restrict__ stage, const double* __restrict__ boys_gtab) {
   using S = QuartetShape<LA, LB, LC, LD, real>;
   constexpr int NA = S::NA, NB = S::NB, NC = S::NC, ND = S::ND;
@@ -40,8 +40,13 @@ __global__ __launch_bounds__(ERI_BLOCK) void eri_jk_kernel(
     if (pb.x == pb.y) w *= 0.5;
     if (pk.x == pk.y) w *= 0.5;
     if (qd.x == qd.y) w *= 0.5;
-    const int a0 = shell_ao[pb.x], b0 = shell_ao[pb.y];
-    const int c0 = shell_ao[pk.x], d0 = shell_ao[pk.y];
+    // density / J / K of the quartet's molecule: packed matrix at mb, stride nbf
+    const int4 sa = shell_dm[pb.x];
+    const int a0 = sa.x, b0 = shell_dm[pb.y].x;
+    const int c0 = shell_dm[pk.x].x, d0 = shell_dm[pk.y].x;
+    const int nbf = sa.y;
+    const long long mb = (long long)(unsigned)sa.z | ((long long)sa.w << 32);
+    const double* DJm = DJ + mb;
     const long long* sl = slots + q * NBLK;
     // Coulomb
     {
@@ -49,11 +54,11 @@ __global__ __launch_bounds__(ERI_BLOCK) void eri_jk_kernel(
 #pragma unroll
       for (int c = 0; c < NC; ++c)
 #pragma unroll
-        for (int d = 0; d < ND; ++d) Pcd[c * ND + d] = DJ[(long long)(c0 + c) * nbf + d0 + d];
+        for (int d = 0; d < ND; ++d) Pcd[c * ND + d] = DJm[(long long)(c0 + c) * nbf + d0 + d];
 #pragma unroll
       for (int a = 0; a < NA; ++a)
 #pragma unroll
-        for (int b = 0; b < NB; ++b) Pab[a * NB + b] = DJ[(long long)(a0 + a) * nbf + b0 + b];
+        for (int b = 0; b < NB; ++b) Pab[a * NB + b] = DJm[(long long)(a0 + a) * nbf + b0 + b];
       double jcd[S::NCD];
 #pragma unroll
       for (int cd = 0; cd < S::NCD; ++cd) jcd[cd] = 0.0;
@@ -74,7 +79,7 @@ __global__ __launch_bounds__(ERI_BLOCK) void eri_jk_kernel(
     // Exchange, one density at a time
 #pragma unroll
     for (int k = 0; k < NK; ++k) {
-      const double* D = k == 0 ? DK0 : DK1;
+      const double* D = (k == 0 ? DK0 : DK1) + mb;
       double Pbd[NB * ND], Pbc[NB * NC], Pad[NA * ND], Pac[NA * NC];
 #pragma unroll
       for (int b = 0; b < NB; ++b) {
@@ -148,8 +153,8 @@ __global__ __launch_bounds__(ERI_BLOCK) void eri_raw_kernel(const int2* __restri
   }
 }
 
-static int grid_for(long long nq, int cap) {
-  long long g = (nq + ERI_BLOCK - 1) / ERI_BLOCK;
+static int grid_for(long long nq, int cap, int block = ERI_BLOCK) {
+  long long g = (nq + block - 1) / block;
   if (g > cap) g = cap;
   if (g < 1) g = 1;
   return (int)g;
@@ -157,16 +162,17 @@ static int grid_for(long long nq, int cap) {
 
 template <int LA, int LB, int LC, int LD, typename real>
 static hipError_t jk_one(const int2* q, const long long* sl, long long nq, const int4* pairs, const void* rec,
-                         const int* ao, const double* DJ, const double* DK0, const double* DK1, int nk, int nbf,
-                         double* stage, const double* bt, int cap, hipStream_t s) {
+                         const int4* ao, const double* DJ, const double* DK0, const double* DK1, int nk,
+                         double* stage, const double* bt, int blk, int cap, hipStream_t s) {
   if (nq <= 0) return hipSuccess;
-  dim3 grid(grid_for(nq, cap)), block(ERI_BLOCK);
+  if (blk != 64 && blk != 128 && blk != 256) return hipErrorInvalidValue;
+  dim3 grid(grid_for(nq, cap, blk)), block(blk);
   if (nk == 1)
     hipLaunchKernelGGL((eri_jk_kernel<LA, LB, LC, LD, real, 1>), grid, block, 0, s, q, sl, nq, pairs,
-                       (const real*)rec, ao, DJ, DK0, DK1, nbf, stage, bt);
+                       (const real*)rec, ao, DJ, DK0, DK1, stage, bt);
   else
     hipLaunchKernelGGL((eri_jk_kernel<LA, LB, LC, LD, real, 2>), grid, block, 0, s, q, sl, nq, pairs,
-                       (const real*)rec, ao, DJ, DK0, DK1, nbf, stage, bt);
+                       (const real*)rec, ao, DJ, DK0, DK1, stage, bt);
   return hipGetLastError();
 }
 
@@ -208,16 +214,16 @@ bool lane_kernel_combo(int combo) {
 }
 
 hipError_t launch_eri_jk(int combo, bool fp32, const int2* q, const long long* sl, long long nq,
-                         const int4* pairs, const void* rec, const int* ao, const double* DJ, const double* DK0,
-                         const double* DK1, int nk, int nbf, double* stage, const double* bt, int cap,
+                         const int4* pairs, const void* rec, const int4* ao, const double* DJ, const double* DK0,
+                         const double* DK1, int nk, double* stage, const double* bt, int block, int cap,
                          hipStream_t s) {
   switch (combo) {
 #define X(id, la, lb, lc, ld)                                                                          \
   case id:                                                                                             \
-    return fp32 ? jk_one<la, lb, lc, ld, float>(q, sl, nq, pairs, rec, ao, DJ, DK0, DK1, nk, nbf, stage, \
-                                                 bt, cap, s)                                           \
-                : jk_one<la, lb, lc, ld, double>(q, sl, nq, pairs, rec, ao, DJ, DK0, DK1, nk, nbf, stage, \
-                                                  bt, cap, s);
+    return fp32 ? jk_one<la, lb, lc, ld, float>(q, sl, nq, pairs, rec, ao, DJ, DK0, DK1, nk, stage, \
+                                                 bt, block, cap, s)                                    \
+                : jk_one<la, lb, lc, ld, double>(q, sl, nq, pairs, rec, ao, DJ, DK0, DK1, nk, stage, \
+                                                  bt, block, cap, s);
     DFT_FOR_COMBOS(X)
 #undef X
     default:

@@ -91,14 +91,14 @@ hipError_t launch_pair_records(const int4* pp, int n, const int* rec_off, const 
 // combines the lanes. The summation tree is fixed, so J and K are bitwise
 // reproducible run to run (the reference's add.at scatter is not:
 // shelled.py:857-859).
-__global__ void reduce_kernel(int ntarget, const int4* __restrict__ tg, const long long* __restrict__ tbase,
+__global__ void reduce_kernel(int ntarget, const Target* __restrict__ tg, const long long* __restrict__ tbase,
                               const int* __restrict__ tcount, const double* __restrict__ stage,
-                              double* __restrict__ half, int nbf) {
+                              double* __restrict__ half) {
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   if (wave >= ntarget) return;
-  const int4 t = tg[wave];
-  const int ne = t.z * t.w;
+  const Target t = tg[wave];
+  const int ne = t.nrow * t.ncol;
   const double* src = stage + tbase[wave];
   const int cnt = tcount[wave];
   for (int el = 0; el < ne; ++el) {
@@ -106,29 +106,37 @@ __global__ void reduce_kernel(int ntarget, const int4* __restrict__ tg, const lo
     for (int k = lane; k < cnt; k += 64) s += src[(long long)k * ne + el];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    if (lane == 0) half[(long long)(t.x + el / t.w) * nbf + t.y + el % t.w] = s;
+    if (lane == 0) half[t.base + (long long)(el / t.ncol) * t.stride + el % t.ncol] = s;
   }
 }
 
-hipError_t launch_reduce(int ntarget, const int4* tg, const long long* tbase, const int* tcount,
-                         const double* stage, double* half, int nbf, hipStream_t s) {
+hipError_t launch_reduce(int ntarget, const Target* tg, const long long* tbase, const int* tcount,
+                         const double* stage, double* half, hipStream_t s) {
   if (ntarget <= 0) return hipSuccess;
   const int waves_per_block = 4;
   hipLaunchKernelGGL(reduce_kernel, dim3((ntarget + waves_per_block - 1) / waves_per_block),
-                     dim3(64 * waves_per_block), 0, s, ntarget, tg, tbase, tcount, stage, half, nbf);
+                     dim3(64 * waves_per_block), 0, s, ntarget, tg, tbase, tcount, stage, half);
   return hipGetLastError();
 }
 
-__global__ void symmetrize_kernel(const double* __restrict__ h, double* __restrict__ o, int n) {
+// out = H + H^T per molecule block; blockIdx.y = molecule
+__global__ void symmetrize_kernel(const long long* __restrict__ mbase, const int* __restrict__ mnbf,
+                                  const double* __restrict__ h, double* __restrict__ o) {
+  const int m = blockIdx.y;
+  const int n = mnbf[m];
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (long long)n * n) return;
   const int r = (int)(i / n), c = (int)(i % n);
-  o[i] = h[i] + h[(long long)c * n + r];
+  const long long b = mbase[m];
+  o[b + i] = h[b + i] + h[b + (long long)c * n + r];
 }
 
-hipError_t launch_symmetrize(const double* half, double* out, int n, hipStream_t s) {
-  const long long nn = (long long)n * n;
-  hipLaunchKernelGGL(symmetrize_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, half, out, n);
+hipError_t launch_symmetrize(int nmol, const long long* mbase, const int* mnbf, int max_nbf, const double* half,
+                             double* out, hipStream_t s) {
+  if (nmol <= 0) return hipSuccess;
+  const long long nn = (long long)max_nbf * max_nbf;
+  hipLaunchKernelGGL(symmetrize_kernel, dim3((unsigned)((nn + 255) / 256), (unsigned)nmol), dim3(256), 0, s, mbase,
+                     mnbf, half, out);
   return hipGetLastError();
 }
 
@@ -212,8 +220,8 @@ __device__ void e1d(double a, double b, double Q, int ia, int ib, double* E /* >
 // shelled.py:976-1045; the caller's coefs already carry the radial part.
 __global__ void one_electron_kernel(const int2* __restrict__ sp, int npairs, const ShellDev* __restrict__ sh,
                                     const double* __restrict__ exps, const double* __restrict__ coefs,
-                                    int natom, const double* __restrict__ atoms, double* __restrict__ S,
-                                    double* __restrict__ T, double* __restrict__ V, int nbf,
+                                    const double* __restrict__ atoms, double* __restrict__ S,
+                                    double* __restrict__ T, double* __restrict__ V,
                                     const double* __restrict__ gtab) {
   const int pidx = blockIdx.x;
   const int lane = threadIdx.x;
@@ -256,7 +264,7 @@ __global__ void one_electron_kernel(const int2* __restrict__ sp, int npairs, con
       const int L = A.l + B.l;
       const double P[3] = {(a * A.x + b * B.x) / p, (a * A.y + b * B.y) / p, (a * A.z + b * B.z) / p};
       double vsum = 0.0;
-      for (int c = 0; c < natom; ++c) {
+      for (int c = A.atom_begin; c < A.atom_end; ++c) {
         const double X = P[0] - atoms[4 * c], Y = P[1] - atoms[4 * c + 1], Z = P[2] - atoms[4 * c + 2];
         double F[9], R0[nherm(8)];
         boys_rt(L, p * (X * X + Y * Y + Z * Z), gtab, F);
@@ -271,20 +279,21 @@ __global__ void one_electron_kernel(const int2* __restrict__ sp, int npairs, con
     }
   }
   const long long r = A.ao + ca, c = B.ao + cb;
-  S[r * nbf + c] = s_acc;
-  S[c * nbf + r] = s_acc;
-  T[r * nbf + c] = t_acc;
-  T[c * nbf + r] = t_acc;
-  V[r * nbf + c] = v_acc;
-  V[c * nbf + r] = v_acc;
+  const long long nbf = A.nbf, rc = A.mbase + r * nbf + c, cr = A.mbase + c * nbf + r;
+  S[rc] = s_acc;
+  S[cr] = s_acc;
+  T[rc] = t_acc;
+  T[cr] = t_acc;
+  V[rc] = v_acc;
+  V[cr] = v_acc;
 }
 
 hipError_t launch_one_electron(const int2* sp, int npairs, const ShellDev* sh, const double* exps,
-                               const double* coefs, int natom, const double* atoms, double* S, double* T,
-                               double* V, int nbf, const double* gtab, hipStream_t s) {
+                               const double* coefs, const double* atoms, double* S, double* T, double* V,
+                               const double* gtab, hipStream_t s) {
   if (npairs <= 0) return hipSuccess;
-  hipLaunchKernelGGL(one_electron_kernel, dim3(npairs), dim3(64), 0, s, sp, npairs, sh, exps, coefs, natom,
-                     atoms, S, T, V, nbf, gtab);
+  hipLaunchKernelGGL(one_electron_kernel, dim3(npairs), dim3(64), 0, s, sp, npairs, sh, exps, coefs, atoms, S, T, V,
+                     gtab);
   return hipGetLastError();
 }
 

@@ -65,9 +65,9 @@ template <typename real, int NK, bool RAW>
 __global__ __launch_bounds__(64) void eri_wave_kernel(WaveClass C, const int2* __restrict__ quartets,
                                                       const long long* __restrict__ slots, long long nq,
                                                       const int4* __restrict__ pairs, const real* __restrict__ rec,
-                                                      const int* __restrict__ shell_ao,
+                                                      const int4* __restrict__ shell_dm,
                                                       const double* __restrict__ DJ, const double* __restrict__ DK0,
-                                                      const double* __restrict__ DK1, int nbf,
+                                                      const double* __restrict__ DK1,
                                                       double* __restrict__ out,
                                                       const double* __restrict__ boys_gtab) {
   __shared__ real sR[2][WMAX_NH];
@@ -210,7 +210,11 @@ __global__ __launch_bounds__(64) void eri_wave_kernel(WaveClass C, const int2* _
       if (pb.x == pb.y) w *= 0.5;
       if (pk.x == pk.y) w *= 0.5;
       if (qd.x == qd.y) w *= 0.5;
-      const long long a0 = shell_ao[pb.x], b0 = shell_ao[pb.y], c0 = shell_ao[pk.x], d0 = shell_ao[pk.y];
+      const int4 sa = shell_dm[pb.x];
+      const long long a0 = sa.x, b0 = shell_dm[pb.y].x, c0 = shell_dm[pk.x].x, d0 = shell_dm[pk.y].x;
+      const long long nbf = sa.y;
+      const long long mb = (long long)(unsigned)sa.z | ((long long)sa.w << 32);
+      const double* DJm = DJ + mb;
       const long long* sl = slots + q * (2 + 4 * NK);
       const int na = C.na, nb = C.nb, nc = C.nc, nd = C.nd;
       auto I = [&](int a, int b, int c, int d) { return (double)sGI[((a * nb + b) * nc + c) * nd + d]; };
@@ -221,18 +225,18 @@ __global__ __launch_bounds__(64) void eri_wave_kernel(WaveClass C, const int2* _
         if (o < nab) {
           const int a = o / nb, b = o - a * nb;
           for (int c = 0; c < nc; ++c)
-            for (int d = 0; d < nd; ++d) s = fma(I(a, b, c, d), DJ[(c0 + c) * nbf + d0 + d], s);
+            for (int d = 0; d < nd; ++d) s = fma(I(a, b, c, d), DJm[(c0 + c) * nbf + d0 + d], s);
           out[sl[0] + o] = 2.0 * w * s;
         } else if (o < nab + ncd) {
           const int x = o - nab, c = x / nd, d = x - c * nd;
           for (int a = 0; a < na; ++a)
-            for (int b = 0; b < nb; ++b) s = fma(I(a, b, c, d), DJ[(a0 + a) * nbf + b0 + b], s);
+            for (int b = 0; b < nb; ++b) s = fma(I(a, b, c, d), DJm[(a0 + a) * nbf + b0 + b], s);
           out[sl[1] + x] = 2.0 * w * s;
         } else {
           int x = o - nab - ncd;
           const int kk = x / nK;
           x -= kk * nK;
-          const double* D = kk == 0 ? DK0 : DK1;
+          const double* D = (kk == 0 ? DK0 : DK1) + mb;
           const long long* ks = sl + 2 + 4 * kk;
           if (x < na * nc) {
             const int a = x / nc, c = x - a * nc;
@@ -295,8 +299,8 @@ static WaveClass make_class(int la, int lb, int lc, int ld) {
 }
 
 hipError_t launch_eri_wave(int la, int lb, int lc, int ld, bool fp32, bool raw, const int2* q, const long long* sl,
-                           long long nq, const int4* pairs, const void* rec, const int* ao, const double* DJ,
-                           const double* DK0, const double* DK1, int nk, int nbf, double* out, const double* bt,
+                           long long nq, const int4* pairs, const void* rec, const int4* ao, const double* DJ,
+                           const double* DK0, const double* DK1, int nk, double* out, const double* bt,
                            int cap, hipStream_t s) {
   if (nq <= 0) return hipSuccess;
   if (la + lb + lc + ld > WMAX_L || la > 2 || lb > 2 || lc > 2 || ld > 2) return hipErrorInvalidValue;
@@ -304,21 +308,21 @@ hipError_t launch_eri_wave(int la, int lb, int lc, int ld, bool fp32, bool raw, 
   dim3 grid((unsigned)std::min<long long>(nq, cap)), block(64);
   if (raw) {
     hipLaunchKernelGGL((eri_wave_kernel<double, 1, true>), grid, block, 0, s, C, q, sl, nq, pairs,
-                       (const double*)rec, ao, DJ, DK0, DK1, nbf, out, bt);
+                       (const double*)rec, ao, DJ, DK0, DK1, out, bt);
   } else if (fp32) {
     if (nk == 1)
       hipLaunchKernelGGL((eri_wave_kernel<float, 1, false>), grid, block, 0, s, C, q, sl, nq, pairs,
-                         (const float*)rec, ao, DJ, DK0, DK1, nbf, out, bt);
+                         (const float*)rec, ao, DJ, DK0, DK1, out, bt);
     else
       hipLaunchKernelGGL((eri_wave_kernel<float, 2, false>), grid, block, 0, s, C, q, sl, nq, pairs,
-                         (const float*)rec, ao, DJ, DK0, DK1, nbf, out, bt);
+                         (const float*)rec, ao, DJ, DK0, DK1, out, bt);
   } else {
     if (nk == 1)
       hipLaunchKernelGGL((eri_wave_kernel<double, 1, false>), grid, block, 0, s, C, q, sl, nq, pairs,
-                         (const double*)rec, ao, DJ, DK0, DK1, nbf, out, bt);
+                         (const double*)rec, ao, DJ, DK0, DK1, out, bt);
     else
       hipLaunchKernelGGL((eri_wave_kernel<double, 2, false>), grid, block, 0, s, C, q, sl, nq, pairs,
-                         (const double*)rec, ao, DJ, DK0, DK1, nbf, out, bt);
+                         (const double*)rec, ao, DJ, DK0, DK1, out, bt);
   }
   return hipGetLastError();
 }

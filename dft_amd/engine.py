@@ -224,3 +224,134 @@ def profile_read(plan: Plan):
     _lib.check(plan.lib.dft_plan_profile_read(plan.handle, _lib.ptr(ms, C.c_double), ns.value, C.byref(ns),
                                               C.byref(st)))
     return ms, st.value
+
+
+def flatten_shells(shells):
+    """Segmented-shell arrays of one molecule: (l, nprim, ao, xyz, exps, coefs, segments)."""
+    seg_l, seg_np, seg_ao, seg_xyz, exps, coefs, segments = [], [], [], [], [], [], []
+    for sh in shells:
+        ids = []
+        for r in range(sh["n_contractions"]):
+            ids.append(len(seg_l))
+            seg_l.append(sh["angular_momentum"])
+            seg_np.append(sh["n_exponents"])
+            seg_ao.append(sh["ao_start"] + r * sh["n_components"])
+            seg_xyz.append(np.asarray(sh["atom_pos"], dtype=np.float64))
+            exps.append(np.asarray(sh["exponents"], dtype=np.float64))
+            coefs.append(np.asarray(sh["coefficients"], dtype=np.float64)[r])
+        segments.append(ids)
+    return (np.asarray(seg_l, np.int32), np.asarray(seg_np, np.int32), np.asarray(seg_ao, np.int32),
+            np.stack(seg_xyz), np.concatenate(exps), np.concatenate(coefs), segments)
+
+
+class Graph:
+    """An instantiated hipGraph of one J/K build (dft_jk_graph_create)."""
+
+    def __init__(self, plan, P, Pa, Pb, J, Ka, Kb, mode):
+        self.plan = plan
+        self.keep = (P, Pa, Pb, J, Ka, Kb)  # the graph binds these buffers
+        h = C.c_void_p()
+        _lib.check(plan.lib.dft_jk_graph_create(plan.handle, P.data_ptr(), Pa.data_ptr(),
+                                                Pb.data_ptr() if Pb is not None else None, J.data_ptr(),
+                                                Ka.data_ptr(), Kb.data_ptr() if Kb is not None else None,
+                                                _lib.MODES[mode], C.byref(h)))
+        self.handle = h
+
+    def launch(self, stream=None):
+        s = torch.cuda.current_stream(self.plan.device) if stream is None else stream
+        _lib.check(self.plan.lib.dft_graph_launch(self.handle, C.c_void_p(s.cuda_stream)))
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            self.plan.lib.dft_graph_destroy(h)
+            self.handle = None
+
+
+def tune(plan, P, Pa, Pb=None, reps=3):
+    """Autotune the plan's launch shapes on these densities; returns the
+    per-combination choices as an int array [ncombo, 6]."""
+    ncombo = len(plan.stats()["per_combo"])
+    rep = (C.c_int * (6 * ncombo))()
+    s = torch.cuda.current_stream(plan.device)
+    _lib.check(plan.lib.dft_plan_tune(plan.handle, P.data_ptr(), Pa.data_ptr(),
+                                      Pb.data_ptr() if Pb is not None else None, int(reps), rep, 6 * ncombo,
+                                      C.c_void_p(s.cuda_stream)))
+    torch.cuda.synchronize(plan.device)
+    return np.array(rep[:], dtype=np.int64).reshape(ncombo, 6)
+
+
+def set_shapes(plan, shapes):
+    arr = np.ascontiguousarray(np.asarray(shapes, dtype=np.int32).ravel())
+    _lib.check(plan.lib.dft_plan_set_shapes(plan.handle, _lib.ptr(arr, C.c_int), arr.size))
+
+
+class BatchPlan(Plan):
+    """Plan over many independent molecules (dft_plan_create_batch).
+
+    Matrices are packed: molecule m's nbf_m x nbf_m block starts at element
+    ``offsets[m]`` of a flat device vector; ``pack``/``unpack`` convert.
+    """
+
+    def __init__(self, molecules, schwarz_tau=1e-10, fp32_tau=1e-4, uhf=False, device=None):
+        self.lib = _lib.load()
+        if not torch.cuda.is_available():
+            raise _lib.DFTError("no HIP device visible: the integral engine runs on the GPU only")
+        self.device = torch.device("cuda", torch.cuda.current_device() if device is None else device)
+        self.uhf = bool(uhf)
+        self.molecules = molecules
+        parts = [flatten_shells(sh) for sh, _, _ in molecules]
+        self.nbfs = [int(sh[-1]["ao_stop"]) for sh, _, _ in molecules]
+        self.offsets = np.concatenate([[0], np.cumsum([n * n for n in self.nbfs])]).astype(np.int64)
+        self.nelem = int(self.offsets[-1])
+        mol_nsh = np.array([len(p[0]) for p in parts], np.int32)
+        mol_nat = np.array([len(Z) for _, Z, _ in molecules], np.int32)
+        mol_nbf = np.array(self.nbfs, np.int32)
+        cat = lambda k, dt: np.ascontiguousarray(np.concatenate([p[k] for p in parts]), dtype=dt)  # noqa: E731
+        l, npr, ao = cat(0, np.int32), cat(1, np.int32), cat(2, np.int32)
+        xyz = np.ascontiguousarray(np.concatenate([p[3] for p in parts]).reshape(-1), dtype=np.float64)
+        ex, co = cat(4, np.float64), cat(5, np.float64)
+        Zc = np.ascontiguousarray(np.concatenate([np.asarray(Z, np.float64) for _, Z, _ in molecules]))
+        axyz = np.ascontiguousarray(np.concatenate([np.asarray(c, np.float64).reshape(-1, 3)
+                                                    for _, _, c in molecules]).reshape(-1))
+        self.Z, self.centers = Zc, axyz.reshape(-1, 3)
+        handle = C.c_void_p()
+        with torch.cuda.device(self.device):
+            _lib.check(self.lib.dft_plan_create_batch(
+                len(molecules), _lib.ptr(mol_nsh, C.c_int32), _lib.ptr(mol_nat, C.c_int32),
+                _lib.ptr(mol_nbf, C.c_int32), _lib.ptr(l, C.c_int32), _lib.ptr(npr, C.c_int32),
+                _lib.ptr(ao, C.c_int32), _lib.ptr(xyz, C.c_double), _lib.ptr(ex, C.c_double),
+                _lib.ptr(co, C.c_double), _lib.ptr(axyz, C.c_double), _lib.ptr(Zc, C.c_double),
+                float(schwarz_tau), float(fp32_tau), _lib.DFT_PLAN_UHF if uhf else 0, C.byref(handle)))
+        self.handle = handle
+        self.nbf = None
+        self.schwarz_tau, self.fp32_tau = schwarz_tau, fp32_tau
+
+    def empty(self):
+        return torch.empty(self.nelem, dtype=torch.float64, device=self.device)
+
+    def pack(self, mats):
+        return torch.cat([torch.as_tensor(m, dtype=torch.float64, device=self.device).reshape(-1)
+                          for m in mats]).contiguous()
+
+    def unpack(self, flat):
+        return [flat[self.offsets[m]:self.offsets[m + 1]].reshape(n, n) for m, n in enumerate(self.nbfs)]
+
+    def one_electron(self):
+        S, T, V = self.empty(), self.empty(), self.empty()
+        s = torch.cuda.current_stream(self.device)
+        _lib.check(self.lib.dft_oneelec(self.handle, S.data_ptr(), T.data_ptr(), V.data_ptr(),
+                                        C.c_void_p(s.cuda_stream)))
+        return S, T, V
+
+    def jk(self, P, Pa, Pb=None, mode="fp64"):
+        J, Ka = self.empty(), self.empty()
+        Kb = self.empty() if self.uhf else None
+        self.jk_into(P, Pa, Pb if self.uhf else None, J, Ka, Kb, mode)
+        return J, Ka, Kb
+
+    def schwarz(self):
+        raise NotImplementedError("Schwarz matrices are per molecule: build a single-molecule Plan")
+
+    def eri_tensor(self):
+        raise NotImplementedError("full ERI tensors are per molecule: build a single-molecule Plan")

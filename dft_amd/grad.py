@@ -69,8 +69,8 @@ def energy_and_gradient(Z, xyz, basis, charge=0, mult=1, schwarz_tau=1e-12, e_to
     """(E, dE/dR [natom, 3], SCF result) at bohr coordinates ``xyz``."""
     xyz = np.asarray(xyz, dtype=np.float64).reshape(-1, 3)
     shells = _shells(Z, xyz, basis)
-    plan = Plan(shells, Z, xyz, schwarz_tau=schwarz_tau, fp32_tau=0.0)
     N_a, N_b = electron_counts(Z, charge, mult)
+    plan = Plan(shells, Z, xyz, schwarz_tau=schwarz_tau, fp32_tau=0.0, uhf=N_a != N_b)
     res = scf_diis(plan, Z, xyz, N_a, N_b, e_tol=e_tol, p_tol=p_tol, max_count=300)
     if not res.converged:
         raise RuntimeError("SCF did not converge; the analytic gradient needs a stationary density")

@@ -124,6 +124,8 @@ def scf_plain(plan: Plan, Z, centers, N_a, N_b, mode="fp64", e_tol=1e-8, p_tol=1
     reference silently exits there, main.py:806).
     """
     rhf = N_a == N_b and not plan.uhf
+    if not rhf and not plan.uhf:
+        raise ValueError("N_a != N_b needs a plan built with uhf=True (two exchange matrices)")
     S, H, X, E_NN = _setup(plan, Z, centers, A)
     _, Cp = torch.linalg.eigh(X.T @ H @ X)
     C = X @ Cp
@@ -178,6 +180,8 @@ def scf_diis(plan: Plan, Z, centers, N_a, N_b, mode="fp64", rate=0.4, diis_thres
     C = X @ Cp
     P_a, P_b = init_UHF_density(C, C, N_a, N_b)
     rhf = N_a == N_b and not plan.uhf
+    if not rhf and not plan.uhf:
+        raise ValueError("N_a != N_b needs a plan built with uhf=True (two exchange matrices)")
     fock = _Fock(plan, mode, rhf, A)
     hist = {k: [] for k in ("Pa", "Pb", "Fa", "Fb", "Ra", "Rb")}
     E_total = None

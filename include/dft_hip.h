@@ -79,7 +79,53 @@ int dft_plan_create(int nsh, const int32_t* shell_l, const int32_t* shell_nprim,
                     const double* atom_xyz, const double* atom_charge, int nbf, double schwarz_tau,
                     double fp32_tau, int flags, dft_plan* out);
 
+/*
+ * Plan of a batch of independent molecules: molecule m owns the next
+ * mol_nsh[m] shells and mol_natom[m] nuclei of the concatenated arrays, and
+ * its shell_ao values are relative to its own nbf = mol_nbf[m]. Matrices of
+ * a batch plan (P, J, K, S, T, V) are packed: molecule m's nbf_m x nbf_m
+ * row-major block starts at element sum_{m' < m} nbf_m'^2. All quartet
+ * classes of all molecules are sorted together, so one dft_jk call (or one
+ * graph launch) builds every molecule's J/K.
+ *
+ * Replaces running the reference's whole set-up and SCF once per molecule.
+ */
+int dft_plan_create_batch(int nmol, const int32_t* mol_nsh, const int32_t* mol_natom, const int32_t* mol_nbf,
+                          const int32_t* shell_l, const int32_t* shell_nprim, const int32_t* shell_ao,
+                          const double* shell_xyz, const double* prim_exp, const double* prim_coef,
+                          const double* atom_xyz, const double* atom_charge, double schwarz_tau, double fp32_tau,
+                          int flags, dft_plan* out);
+
 int dft_plan_destroy(dft_plan plan);
+
+typedef struct dft_graph_s* dft_graph;
+
+/*
+ * Capture one dft_jk (same arguments; the pointers are bound into the graph)
+ * as an instantiated hipGraph. Launching the graph rebuilds J/K from the
+ * current contents of P, Pa, Pb; update those buffers in place between
+ * launches. No counterpart in the reference.
+ */
+int dft_jk_graph_create(dft_plan plan, const double* P, const double* Pa, const double* Pb, double* J, double* Ka,
+                        double* Kb, int mode, dft_graph* out);
+int dft_graph_launch(dft_graph graph, void* stream);
+int dft_graph_destroy(dft_graph graph);
+
+/*
+ * Autotune the launch shape of every class-combination launch of the plan:
+ * for the FP64 launch (whole list) and the FP32 launch (weak part) it times
+ * the lane-per-quartet kernel at workgroup sizes 64/128/256 and grid caps
+ * 512..8192, and the wave-per-quartet kernel at three grid caps (median of
+ * reps runs each) and keeps the fastest. report (capacity report_cap ints)
+ * receives 6 ints per combination: FP64 {wave, block, cap}, FP32 {wave,
+ * block, cap}. Overwrites the plan's staging (J/K are not written).
+ * No counterpart in the reference.
+ */
+int dft_plan_tune(dft_plan plan, const double* P, const double* Pa, const double* Pb, int reps, int* report,
+                  int report_cap, void* stream);
+
+/* Set launch shapes (6 ints per combination, as reported by dft_plan_tune). */
+int dft_plan_set_shapes(dft_plan plan, const int* shapes, int n);
 
 /*
  * Plan statistics. counts receives, for each class combination c (pair

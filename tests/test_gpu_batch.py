@@ -1,0 +1,122 @@
+"""Batched plans (many molecules, one J/K build), hipGraph replay and the
+launch-shape autotuner, against single-molecule plans."""
+import numpy as np
+import pytest
+
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    return torch
+
+
+def _mols():
+    from dft_amd.basis import build_shells, build_shells_from_entries, load_basis
+    from dft_amd.geometry import methane, synthetic_spd_basis, water, water_cluster
+    b = load_basis()
+    out = []
+    Z, xyz = water()
+    out.append((build_shells(b, Z, xyz), Z, xyz))
+    Z, xyz = water_cluster((2, 2, 1), 3.1, seed=5)
+    out.append((build_shells(b, Z, xyz), Z, xyz))
+    Z, xyz = methane()
+    out.append((build_shells_from_entries(synthetic_spd_basis(Z, seed=1), xyz), Z, xyz))
+    g = golden("dimer_sto3g.npz")
+    Z = list(g["Z"])
+    out.append((build_shells(b, Z, g["xyz"]), Z, g["xyz"]))
+    return out
+
+
+def _dens(n, rng):
+    A = rng.standard_normal((n, n)) * 0.1
+    return A + A.T
+
+
+@pytest.mark.parametrize("uhf", [False, True])
+def test_batch_jk_bitwise_equal_to_single(torch_cuda, uhf):
+    torch = torch_cuda
+    from dft_amd.engine import BatchPlan, Plan
+    mols = _mols()
+    bp = BatchPlan(mols, schwarz_tau=1e-12, fp32_tau=0.0, uhf=uhf)
+    rng = np.random.default_rng(0)
+    Pa = [_dens(n, rng) for n in bp.nbfs]
+    Pb = [_dens(n, rng) for n in bp.nbfs]
+    Pt = bp.pack([a + b for a, b in zip(Pa, Pb)])
+    J, Ka, Kb = bp.jk(Pt, bp.pack(Pa), bp.pack(Pb) if uhf else None)
+    for m, (sh, Z, xyz) in enumerate(mols):
+        p = Plan(sh, Z, xyz, schwarz_tau=1e-12, fp32_tau=0.0, uhf=uhf)
+        j, ka, kb = p.jk(Pa[m] + Pb[m], Pa[m], Pb[m] if uhf else None)
+        assert torch.equal(bp.unpack(J)[m], j)
+        assert torch.equal(bp.unpack(Ka)[m], ka)
+        if uhf:
+            assert torch.equal(bp.unpack(Kb)[m], kb)
+
+
+def test_batch_one_electron_equals_single(torch_cuda):
+    torch = torch_cuda
+    from dft_amd.engine import BatchPlan, Plan
+    mols = _mols()
+    bp = BatchPlan(mols)
+    S, T, V = bp.one_electron()
+    for m, (sh, Z, xyz) in enumerate(mols):
+        s, t, v = Plan(sh, Z, xyz).one_electron()
+        for a, b in ((S, s), (T, t), (V, v)):
+            assert torch.allclose(bp.unpack(a)[m], b, rtol=0, atol=1e-14)
+
+
+def test_graph_replays_match_eager(torch_cuda):
+    torch = torch_cuda
+    from dft_amd.engine import BatchPlan, Graph
+    mols = _mols()
+    bp = BatchPlan(mols, schwarz_tau=1e-12)
+    rng = np.random.default_rng(1)
+    Pa = bp.pack([_dens(n, rng) for n in bp.nbfs])
+    P = (2 * Pa).contiguous()
+    J, K = bp.empty(), bp.empty()
+    g = Graph(bp, P, Pa, None, J, K, None, "mixed")
+    for rep in range(4):
+        new = bp.pack([_dens(n, rng) for n in bp.nbfs])
+        Pa.copy_(new)
+        torch.mul(Pa, 2.0, out=P)
+        torch.cuda.synchronize()
+        g.launch()
+        torch.cuda.synchronize()
+        Je, Ke, _ = bp.jk(P, Pa, None, mode="mixed")
+        assert torch.equal(J, Je) and torch.equal(K, Ke), rep
+
+
+def test_tuner_keeps_results(torch_cuda):
+    torch = torch_cuda
+    from dft_amd.engine import BatchPlan, tune
+    mols = _mols()
+    bp = BatchPlan(mols, schwarz_tau=1e-12, fp32_tau=1e-4)
+    rng = np.random.default_rng(2)
+    Pa = bp.pack([_dens(n, rng) for n in bp.nbfs])
+    P = (2 * Pa).contiguous()
+    J0, K0, _ = bp.jk(P, Pa, None, mode="fp64")
+    rep = tune(bp, P, Pa, None, reps=2)
+    assert rep.shape == (21, 6) and (rep[:, 2] > 0).all()
+    J1, K1, _ = bp.jk(P, Pa, None, mode="fp64")
+    scale = max(1.0, J0.abs().max().item())
+    assert (J1 - J0).abs().max().item() < 1e-12 * scale
+    assert (K1 - K0).abs().max().item() < 1e-12 * scale
+
+
+def test_scf_batch_matches_single(torch_cuda):
+    from dft_amd.batch import scf_batch
+    from dft_amd.engine import BatchPlan, Plan
+    from dft_amd.scf import scf_plain
+    mols = [m for m in _mols() if max(s["angular_momentum"] for s in m[0]) <= 1]
+    bp = BatchPlan(mols, schwarz_tau=1e-12)
+    nocc = [int(sum(Z)) // 2 for _, Z, _ in mols]
+    res = scf_batch(bp, nocc, e_tol=1e-10, p_tol=1e-7, max_iter=200)
+    assert res.converged.all()
+    for m, (sh, Z, xyz) in enumerate(mols):
+        single = scf_plain(Plan(sh, Z, xyz, schwarz_tau=1e-12), Z, xyz, nocc[m], nocc[m], e_tol=1e-10, p_tol=1e-7,
+                           max_count=200)
+        assert abs(res.energies[m] - single.energy) < 1e-8, (m, res.energies[m], single.energy)
