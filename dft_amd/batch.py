@@ -29,6 +29,8 @@ class BatchResult:
     iterations: int
     fock_times: list = field(default_factory=list)
     iter_times: list = field(default_factory=list)
+    P: object = None   # packed total density after the last iteration
+    Pa: object = None  # packed alpha density after the last iteration
 
 
 class _Pad:
@@ -86,7 +88,8 @@ def scf_batch(plan: BatchPlan, n_occ, mode="fp64", use_graph=True, e_tol=1e-8, p
     occ = torch.zeros((pad.nmol, pad.nmax), dtype=torch.float64, device=plan.device)
     for m, n in enumerate(n_occ):
         occ[m, :int(n)] = 1.0
-    E_NN = torch.as_tensor([nuclear_repulsion(Z, xyz) for _, Z, xyz in plan.molecules], device=plan.device)
+    E_NN = torch.as_tensor([nuclear_repulsion(Z, xyz) for _, Z, xyz in plan.molecules], dtype=torch.float64,
+                           device=plan.device)
 
     def density(Fp):
         _, Cp = torch.linalg.eigh(X.transpose(1, 2) @ Fp @ X)
@@ -132,7 +135,7 @@ def scf_batch(plan: BatchPlan, n_occ, mode="fp64", use_graph=True, e_tol=1e-8, p
         torch.cuda.synchronize()
         iter_t.append(time.perf_counter() - t0)
         if done:
-            return BatchResult(E.cpu().numpy(), converged.cpu().numpy(), it, fock_t, iter_t)
+            return BatchResult(E.cpu().numpy(), converged.cpu().numpy(), it, fock_t, iter_t, P.clone(), Pa.clone())
 
 
 def water_batch(n_mol=64, seed=2):

@@ -1,0 +1,176 @@
+"""The benchmark workloads: water (plain RHF), (H2O)16 direct J/K, the
+C12H26 s/p/d mixed-precision build, the batch of 64 water clusters and the
+C8H18 analytic gradient. Each function runs one workload on the GPU and
+returns a dict of measured numbers (device times from HIP events).
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+import torch
+
+from .basis import build_shells, build_shells_from_entries, load_basis
+from .engine import BatchPlan, Graph, Plan, tune
+from .geometry import alkane_bohr, synthetic_spd_basis, water
+from .scf import electron_counts, scf_diis, scf_plain
+
+
+def _events():
+    return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+
+def time_builds(fn, steps, warmup):
+    """Median-free total device time of ``steps`` calls of fn() (after warmup), ms per call."""
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    e0, e1 = _events()
+    e0.record()
+    for _ in range(steps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def water_rhf():
+    """Plain RHF water / STO-3G (SCF to 1e-8 Eh, max|dP| 1e-6 and RMS dP 1e-6, no DIIS)."""
+    Z, xyz = water()
+    shells = build_shells(load_basis(), Z, xyz)
+    t0 = time.perf_counter()
+    plan = Plan(shells, Z, xyz, schwarz_tau=0.0)
+    res = scf_plain(plan, Z, xyz, 5, 5, p_rms_tol=1e-6)
+    torch.cuda.synchronize()
+    return {"energy_Eh": res.energy, "converged": res.converged, "iterations": res.iterations,
+            "s_total": time.perf_counter() - t0, "s_per_fock_build": float(np.median(res.fock_times))}
+
+
+def alkane_spd_mixed(steps=3, warmup=1):
+    """C12H26 with the synthetic s/p/d basis (seed 1): FP64 vs mixed direct J/K.
+
+    SCF in the spherical basis with damping + DIIS (shelled.py conventions).
+    max|dF| is the Fock-matrix difference of the two modes at the converged
+    FP64 density; |dE| the difference of the converged SCF energies.
+    """
+    from .shelled import spherical_matrix
+    Z, xyz = alkane_bohr(12)
+    shells = build_shells_from_entries(synthetic_spd_basis(Z, seed=1), xyz)
+    t0 = time.perf_counter()
+    plan = Plan(shells, Z, xyz, schwarz_tau=1e-10, fp32_tau=1e-4)
+    torch.cuda.synchronize()
+    t_plan = time.perf_counter() - t0
+    st = plan.stats()
+    n_sph = shells[-1]["sphere_ao_stop"]
+    A = torch.as_tensor(spherical_matrix(shells, n_sph), device=plan.device)
+    S, _, _ = plan.one_electron()
+    smin = float(torch.linalg.eigvalsh(A @ S @ A.T)[0])
+    N_a, N_b = electron_counts(Z)
+    r64 = scf_diis(plan, Z, xyz, N_a, N_b, mode="fp64", A=A, max_count=200)
+    r32 = scf_diis(plan, Z, xyz, N_a, N_b, mode="mixed", A=A, max_count=200)
+    Pc = (A.T @ r64.P_a @ A).contiguous()
+    P2 = (2 * Pc).contiguous()
+    J64, K64, _ = plan.jk(P2, Pc, mode="fp64")
+    J32, K32, _ = plan.jk(P2, Pc, mode="mixed")
+    dF = float((A @ ((J64 - K64) - (J32 - K32)) @ A.T).abs().max())
+    J, K = torch.empty_like(P2), torch.empty_like(P2)
+    ms64 = time_builds(lambda: plan.jk_into(P2, Pc, None, J, K, None, "fp64"), steps, warmup)
+    ms32 = time_builds(lambda: plan.jk_into(P2, Pc, None, J, K, None, "mixed"), steps, warmup)
+    return {"nbf_cart": plan.nbf, "nbf_sph": n_sph, "n_shells": len(shells), "min_eig_S": smin,
+            "n_quartets": st["n_quartets"], "n_quartets_fp32": st["n_quartets_fp32"],
+            "n_unscreened": st["n_unscreened"], "n_prim_quartets": st["n_prim_quartets"], "plan_s": t_plan,
+            "energy_fp64_Eh": r64.energy, "energy_mixed_Eh": r32.energy, "converged": [r64.converged, r32.converged],
+            "abs_dE_Eh": abs(r64.energy - r32.energy), "max_abs_dF_Eh": dF,
+            "ms_per_build_fp64": ms64, "ms_per_build_mixed": ms32, "mixed_speedup": ms64 / ms32}
+
+
+def water_batch_scf(steps=5, warmup=2, tune_reps=2, scf_check=True):
+    """64 (H2O)_n clusters (n ~ U{1..24}, seed 2), STO-3G.
+
+    Plain FP64 path: one plan per molecule, one dft_jk per molecule per
+    iteration. Batched path: one plan for all molecules, one hipGraph launch
+    per iteration, autotuned launch shapes; FP64 and mixed.
+    """
+    from .batch import scf_batch, water_batch
+    mols = water_batch(64, 2)
+    t0 = time.perf_counter()
+    bp = BatchPlan(mols, schwarz_tau=1e-10, fp32_tau=1e-4)
+    torch.cuda.synchronize()
+    t_plan = time.perf_counter() - t0
+    st = bp.stats()
+    nocc = [int(sum(Z)) // 2 for _, Z, _ in mols]
+    warm = scf_batch(bp, nocc, fixed_iters=3, use_graph=False)
+    P, Pa = warm.P.contiguous(), warm.Pa.contiguous()
+    # plain FP64 HIP path: per-molecule plans and launches
+    singles = [Plan(sh, Z, xyz, schwarz_tau=1e-10, fp32_tau=1e-4) for sh, Z, xyz in mols]
+    Pm = [x.contiguous() for x in bp.unpack(P)]
+    Pam = [x.contiguous() for x in bp.unpack(Pa)]
+    Jm = [torch.empty_like(x) for x in Pm]
+    Km = [torch.empty_like(x) for x in Pm]
+
+    def per_molecule(mode):
+        for k, pl in enumerate(singles):
+            pl.jk_into(Pm[k], Pam[k], None, Jm[k], Km[k], None, mode)
+    ms_plain64 = time_builds(lambda: per_molecule("fp64"), steps, warmup)
+    J, K = bp.empty(), bp.empty()
+    ms_batch64_eager = time_builds(lambda: bp.jk_into(P, Pa, None, J, K, None, "fp64"), steps, warmup)
+    t0 = time.perf_counter()
+    shapes = tune(bp, P, Pa, None, reps=tune_reps)
+    t_tune = time.perf_counter() - t0
+    g64 = Graph(bp, P, Pa, None, J, K, None, "fp64")
+    ms_batch64 = time_builds(g64.launch, steps, warmup)
+    g32 = Graph(bp, P, Pa, None, J, K, None, "mixed")
+    ms_batch32 = time_builds(g32.launch, steps, warmup)
+    out = {"n_molecules": len(mols), "n_quartets": st["n_quartets"], "n_quartets_fp32": st["n_quartets_fp32"],
+           "n_unscreened": st["n_unscreened"], "n_prim_quartets": st["n_prim_quartets"], "plan_s": t_plan,
+           "tune_s": t_tune, "tuned_shapes": shapes.tolist(),
+           "ms_per_iter_plain_fp64_per_molecule": ms_plain64, "ms_per_iter_batched_fp64_eager_untuned": ms_batch64_eager,
+           "ms_per_iter_batched_fp64_graph_tuned": ms_batch64, "ms_per_iter_batched_mixed_graph_tuned": ms_batch32,
+           "speedup_batched_mixed_vs_plain_fp64": ms_plain64 / ms_batch32,
+           "speedup_batched_fp64_vs_plain_fp64": ms_plain64 / ms_batch64}
+    if scf_check:
+        r64 = scf_batch(bp, nocc, mode="fp64", max_iter=150)
+        r32 = scf_batch(bp, nocc, mode="mixed", max_iter=150)
+        out.update({"scf_converged_fp64": int(r64.converged.sum()), "scf_converged_mixed": int(r32.converged.sum()),
+                    "scf_iterations": [r64.iterations, r32.iterations],
+                    "max_abs_dE_mixed_Eh": float(np.abs(r64.energies - r32.energies).max()),
+                    "s_per_iter_wall_fp64": float(np.median(r64.iter_times[1:]))})
+    return out
+
+
+def alkane_gradient(fd_coords=12, sd_steps=10, h=1e-4):
+    """C8H18 (+ N(0, 0.05 A) noise, seed 3), STO-3G: analytic gradient, FD check
+    on a subset of coordinates, and timed steepest-descent steps."""
+    from .grad import electronic_gradient, energy_and_gradient, steepest_descent
+    Z, xyz = alkane_bohr(8, noise_sigma_ang=0.05, seed=3)
+    basis = load_basis()
+    t0 = time.perf_counter()
+    E, g, res = energy_and_gradient(Z, xyz, basis)
+    t_call = time.perf_counter() - t0
+    # device time of the gradient kernels alone, on the same plan and orbitals
+    shells = build_shells(basis, Z, xyz)
+    plan = Plan(shells, Z, xyz, schwarz_tau=1e-12, fp32_tau=0.0)
+    N_a, N_b = electron_counts(Z)
+    electronic_gradient(plan, res.C_a, res.eps_a, N_a, res.C_b, res.eps_b, N_b)  # builds the gradient data
+    ms_grad = time_builds(lambda: electronic_gradient(plan, res.C_a, res.eps_a, N_a, res.C_b, res.eps_b, N_b), 3, 1)
+    rng = np.random.default_rng(0)
+    coords = [(int(a), int(x)) for a, x in zip(rng.choice(len(Z), fd_coords, replace=False),
+                                                 rng.integers(0, 3, fd_coords))]
+    fd_err = []
+    t0 = time.perf_counter()
+    for a, x in coords:
+        p = xyz.copy()
+        p[a, x] += h
+        m = xyz.copy()
+        m[a, x] -= h
+        fd = (energy_and_gradient(Z, p, basis)[0] - energy_and_gradient(Z, m, basis)[0]) / (2 * h)
+        fd_err.append(abs(fd - g[a, x]))
+    t_fd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    _, hist = steepest_descent(Z, xyz, basis, steps=sd_steps, step_size=0.3)
+    t_sd = time.perf_counter() - t0
+    return {"n_atoms": len(Z), "energy_Eh": E, "max_abs_grad": float(np.abs(g).max()),
+            "s_per_gradient_call_total": t_call, "ms_gradient_kernels": ms_grad,
+            "fd_coords_checked": len(coords), "fd_max_abs_err_Eh_per_bohr": float(max(fd_err)),
+            "fd_seconds": t_fd, "sd_steps": sd_steps, "sd_seconds": t_sd,
+            "sd_energies": [hh[0] for hh in hist], "sd_s_per_step": [hh[2] for hh in hist]}
