@@ -69,12 +69,15 @@ class _Pad:
 
 
 def scf_batch(plan: BatchPlan, n_occ, mode="fp64", use_graph=True, e_tol=1e-8, p_tol=1e-6, max_iter=100,
-              fixed_iters=None):
+              fixed_iters=None, diis=True, history=10):
     """Closed-shell SCF of every molecule of the batch (N_alpha = N_beta = n_occ[m]).
 
     With ``fixed_iters`` the loop runs exactly that many iterations (for
     timing); otherwise until every molecule has |dE| < e_tol and
-    max|dP| < p_tol, or max_iter.
+    max|dP| < p_tol, or max_iter. ``diis`` extrapolates each molecule's Fock
+    matrix from the commutator residuals FPS - SPF as shelled.py:1196-1203
+    (B c = 1 without a Lagrange row, c normalised to sum 1), from the second
+    iteration on; the energy is always that of the un-extrapolated F.
     """
     if plan.uhf:
         raise ValueError("scf_batch is closed-shell: build the BatchPlan with uhf=False")
@@ -103,6 +106,7 @@ def scf_batch(plan: BatchPlan, n_occ, mode="fp64", use_graph=True, e_tol=1e-8, p
     graph = Graph(plan, P, Pa, None, J, K, None, mode) if use_graph else None
     E_old = torch.full((pad.nmol,), -1e9, dtype=torch.float64, device=plan.device)
     fock_t, iter_t = [], []
+    hist_F, hist_R = [], []
     it = 0
     converged = torch.zeros(pad.nmol, dtype=torch.bool, device=plan.device)
     while True:
@@ -118,9 +122,24 @@ def scf_batch(plan: BatchPlan, n_occ, mode="fp64", use_graph=True, e_tol=1e-8, p
         F = H + J - K
         Fp = pad.pad(F, diag=PAD_DIAG)
         Pp = pad.pad(P)
-        E = 0.5 * (Pp * Hp).sum((1, 2)) + (pad.pad(Pa) * Fp).sum((1, 2)) + E_NN
-        Pa_new_p = density(Fp)
-        dP = (Pa_new_p - pad.pad(Pa)).abs().amax((1, 2))
+        Pap = pad.pad(Pa)
+        E = 0.5 * (Pp * Hp).sum((1, 2)) + (Pap * Fp).sum((1, 2)) + E_NN
+        Fd = Fp
+        if diis:
+            R = Fp @ Pap @ Sp - Sp @ Pap @ Fp
+            hist_F.append(Fp)
+            hist_R.append(R)
+            del hist_F[:-history], hist_R[:-history]
+            if len(hist_F) > 1:
+                Rs = torch.stack(hist_R)
+                B = torch.einsum("imab,jmab->mij", Rs, Rs)
+                lam = 1e-14 * B.diagonal(dim1=1, dim2=2).amax(1).clamp_min(1e-300)
+                B = B + lam[:, None, None] * torch.eye(B.shape[1], dtype=B.dtype, device=B.device)
+                c = torch.linalg.solve(B, torch.ones(B.shape[:2], dtype=B.dtype, device=B.device))
+                c = c / c.sum(1, keepdim=True)
+                Fd = torch.einsum("mi,imab->mab", c, torch.stack(hist_F))
+        Pa_new_p = density(Fd)
+        dP = (Pa_new_p - Pap).abs().amax((1, 2))
         dE = (E - E_old).abs()
         e1.synchronize()
         fock_t.append(e0.elapsed_time(e1) * 1e-3)

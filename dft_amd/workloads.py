@@ -16,6 +16,11 @@ from .geometry import alkane_bohr, synthetic_spd_basis, water
 from .scf import electron_counts, scf_diis, scf_plain
 
 
+def log(msg):
+    import sys
+    print(f"[workload] {msg}", file=sys.stderr, flush=True)
+
+
 def _events():
     return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
 
@@ -66,8 +71,12 @@ def alkane_spd_mixed(steps=3, warmup=1):
     S, _, _ = plan.one_electron()
     smin = float(torch.linalg.eigvalsh(A @ S @ A.T)[0])
     N_a, N_b = electron_counts(Z)
+    log(f"C12H26 spd plan {t_plan:.1f} s: {st['n_quartets']} quartets ({st['n_quartets_fp32']} FP32), "
+        f"min eig S {smin:.2e}")
     r64 = scf_diis(plan, Z, xyz, N_a, N_b, mode="fp64", A=A, max_count=200)
+    log(f"SCF fp64: {r64.iterations} iterations, E {r64.energy:.10f}")
     r32 = scf_diis(plan, Z, xyz, N_a, N_b, mode="mixed", A=A, max_count=200)
+    log(f"SCF mixed: {r32.iterations} iterations, E {r32.energy:.10f}")
     Pc = (A.T @ r64.P_a @ A).contiguous()
     P2 = (2 * Pc).contiguous()
     J64, K64, _ = plan.jk(P2, Pc, mode="fp64")
@@ -93,16 +102,21 @@ def water_batch_scf(steps=5, warmup=2, tune_reps=2, scf_check=True):
     """
     from .batch import scf_batch, water_batch
     mols = water_batch(64, 2)
+    log(f"batch: {len(mols)} molecules, {sum(sh[-1]['ao_stop'] for sh, _, _ in mols)} bf")
     t0 = time.perf_counter()
     bp = BatchPlan(mols, schwarz_tau=1e-10, fp32_tau=1e-4)
     torch.cuda.synchronize()
     t_plan = time.perf_counter() - t0
     st = bp.stats()
+    log(f"batch plan {t_plan:.1f} s: {st['n_quartets']} quartets of {st['n_unscreened']}")
     nocc = [int(sum(Z)) // 2 for _, Z, _ in mols]
     warm = scf_batch(bp, nocc, fixed_iters=3, use_graph=False)
+    log("warm-up iterations done")
     P, Pa = warm.P.contiguous(), warm.Pa.contiguous()
     # plain FP64 HIP path: per-molecule plans and launches
+    t0 = time.perf_counter()
     singles = [Plan(sh, Z, xyz, schwarz_tau=1e-10, fp32_tau=1e-4) for sh, Z, xyz in mols]
+    log(f"64 single-molecule plans {time.perf_counter() - t0:.1f} s")
     Pm = [x.contiguous() for x in bp.unpack(P)]
     Pam = [x.contiguous() for x in bp.unpack(Pa)]
     Jm = [torch.empty_like(x) for x in Pm]
@@ -112,15 +126,20 @@ def water_batch_scf(steps=5, warmup=2, tune_reps=2, scf_check=True):
         for k, pl in enumerate(singles):
             pl.jk_into(Pm[k], Pam[k], None, Jm[k], Km[k], None, mode)
     ms_plain64 = time_builds(lambda: per_molecule("fp64"), steps, warmup)
+    log(f"plain fp64 per molecule: {ms_plain64:.2f} ms/iter")
+    del singles, Jm, Km
     J, K = bp.empty(), bp.empty()
     ms_batch64_eager = time_builds(lambda: bp.jk_into(P, Pa, None, J, K, None, "fp64"), steps, warmup)
+    log(f"batched fp64 eager untuned: {ms_batch64_eager:.2f} ms/iter; tuning ...")
     t0 = time.perf_counter()
     shapes = tune(bp, P, Pa, None, reps=tune_reps)
     t_tune = time.perf_counter() - t0
+    log(f"tuned in {t_tune:.1f} s")
     g64 = Graph(bp, P, Pa, None, J, K, None, "fp64")
     ms_batch64 = time_builds(g64.launch, steps, warmup)
     g32 = Graph(bp, P, Pa, None, J, K, None, "mixed")
     ms_batch32 = time_builds(g32.launch, steps, warmup)
+    log(f"batched graph tuned: fp64 {ms_batch64:.2f} ms, mixed {ms_batch32:.2f} ms")
     out = {"n_molecules": len(mols), "n_quartets": st["n_quartets"], "n_quartets_fp32": st["n_quartets_fp32"],
            "n_unscreened": st["n_unscreened"], "n_prim_quartets": st["n_prim_quartets"], "plan_s": t_plan,
            "tune_s": t_tune, "tuned_shapes": shapes.tolist(),
@@ -129,8 +148,10 @@ def water_batch_scf(steps=5, warmup=2, tune_reps=2, scf_check=True):
            "speedup_batched_mixed_vs_plain_fp64": ms_plain64 / ms_batch32,
            "speedup_batched_fp64_vs_plain_fp64": ms_plain64 / ms_batch64}
     if scf_check:
-        r64 = scf_batch(bp, nocc, mode="fp64", max_iter=150)
-        r32 = scf_batch(bp, nocc, mode="mixed", max_iter=150)
+        r64 = scf_batch(bp, nocc, mode="fp64", max_iter=60)
+        log(f"batched SCF fp64: {r64.iterations} iterations, {int(r64.converged.sum())} converged")
+        r32 = scf_batch(bp, nocc, mode="mixed", max_iter=60)
+        log(f"batched SCF mixed: {r32.iterations} iterations, {int(r32.converged.sum())} converged")
         out.update({"scf_converged_fp64": int(r64.converged.sum()), "scf_converged_mixed": int(r32.converged.sum()),
                     "scf_iterations": [r64.iterations, r32.iterations],
                     "max_abs_dE_mixed_Eh": float(np.abs(r64.energies - r32.energies).max()),
