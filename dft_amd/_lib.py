@@ -33,7 +33,7 @@ SIGNATURES = {
     "dft_jk_graph_create": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int, C.POINTER(_vp)]),
     "dft_graph_launch": (C.c_int, [_vp, _vp]),
     "dft_graph_destroy": (C.c_int, [_vp]),
-    "dft_plan_tune": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.POINTER(C.c_int), C.c_int, _vp]),
+    "dft_plan_tune": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.POINTER(C.c_int), C.c_int, _vp]),
     "dft_plan_set_shapes": (C.c_int, [_vp, C.POINTER(C.c_int), C.c_int]),
     "dft_plan_stats": (C.c_int, [_vp, _i64p, C.c_int, C.POINTER(C.c_int), _i64p, _i64p]),
     "dft_schwarz": (C.c_int, [_vp, _dp]),

@@ -92,6 +92,8 @@ int upload(T** d, const std::vector<T>& h) {
 
 int ncart_h(int l) { return (l + 1) * (l + 2) / 2; }
 
+constexpr double PRIM_CUT = 39.1439465808987;  // -ln(1e-17)
+
 // launch shape of one class-combination launch (chosen by dft_plan_tune)
 struct Shape {
   int wave = 0;         // 0: lane-per-quartet kernel, 1: wave-per-quartet kernel
@@ -122,6 +124,8 @@ struct dft_plan_s {
   std::vector<int> in2int;       // input shell -> internal shell
   std::vector<double> exps, coefs;
   std::vector<int4> pairs;       // (sh_a, sh_b, rec_off, nprim_pairs), class-sorted
+  std::vector<int2> prim_list;   // surviving primitive pairs (ia, ib) of all pairs
+  std::vector<int> pair_prim_begin;  // first prim_list entry of each pair
   std::vector<int> pair_pc;      // pair class per pair
   std::map<long long, int> pair_of;  // internal (i,j), i >= j -> pair id
   std::vector<double> Q;         // Schwarz per pair
@@ -299,16 +303,25 @@ int build_plan(dft_plan p, const Input& in) {
     for (const int2& ij : by_pc[pc]) {
       const ShellDev &A = p->sh[ij.x], &B = p->sh[ij.y];
       const int id = (int)p->pairs.size();
-      p->pairs.push_back(make_int4(ij.x, ij.y, (int)roff, A.nprim * B.nprim));
-      p->pair_pc.push_back(pc);
-      p->pair_of[pair_key(ij.x, ij.y)] = id;
-      class_pairs[A.mol][pc].push_back(id);
+      const double r2 = (A.x - B.x) * (A.x - B.x) + (A.y - B.y) * (A.y - B.y) + (A.z - B.z) * (A.z - B.z);
+      const int first = (int)p->prim_list.size();
+      // primitive-pair screening: exp(-ab/(a+b) R_AB^2) < 1e-17 makes every
+      // integral of the primitive pair negligible next to FP64 rounding
       for (int ia = 0; ia < A.nprim; ++ia)
         for (int ib = 0; ib < B.nprim; ++ib) {
+          const double a = p->exps[A.prim_off + ia], b = p->exps[B.prim_off + ib];
+          if (a * b / (a + b) * r2 > PRIM_CUT) continue;
+          p->prim_list.push_back(make_int2(ia, ib));
           pp.push_back(make_int4(ij.x, ij.y, ia, ib));
           pp_off.push_back((int)roff);
           roff += rs;
         }
+      p->pairs.push_back(make_int4(ij.x, ij.y, (int)(roff - (long long)rs * ((int)p->prim_list.size() - first)),
+                                   (int)p->prim_list.size() - first));
+      p->pair_prim_begin.push_back(first);
+      p->pair_pc.push_back(pc);
+      p->pair_of[pair_key(ij.x, ij.y)] = id;
+      class_pairs[A.mol][pc].push_back(id);
       if (roff > 0x7fffffffLL) return fail(DFT_EINVAL, "pair records exceed 2^31 reals");
     }
   }
@@ -585,12 +598,13 @@ int grad_prepare(dft_plan p, const int32_t* shell_atom) {
     const ShellDev &A = p->sh[pr.x], &B = p->sh[pr.y];
     gp[id] = make_int4(pr.x, pr.y, (int)roff, pr.w);
     const int rs = 6 + 3 * ne_axis(A.l + 1, B.l + 1);
-    for (int ia = 0; ia < A.nprim; ++ia)
-      for (int ib = 0; ib < B.nprim; ++ib) {
-        pp.push_back(make_int4(pr.x, pr.y, ia, ib));
-        off.push_back((int)roff);
-        roff += rs;
-      }
+    // the same screened primitive pairs as the J/K records
+    for (int k = 0; k < pr.w; ++k) {
+      const int2 ab = p->prim_list[p->pair_prim_begin[id] + k];
+      pp.push_back(make_int4(pr.x, pr.y, ab.x, ab.y));
+      off.push_back((int)roff);
+      roff += rs;
+    }
   }
   int rc;
   if ((rc = upload(&p->d_gpairs, gp))) return rc;
@@ -838,7 +852,7 @@ int dft_graph_destroy(dft_graph g) {
   return DFT_OK;
 }
 
-int dft_plan_tune(dft_plan p, const double* P, const double* Pa, const double* Pb, int reps, int* report,
+int dft_plan_tune(dft_plan p, const double* P, const double* Pa, const double* Pb, int reps, int combo, int* report,
                   int report_cap, void* stream) {
   if (!p || !P || !Pa) return fail(DFT_EINVAL, "NULL argument");
   if ((p->nk == 2) != (Pb != nullptr)) return fail(DFT_EINVAL, "Pb must be given exactly for a UHF plan");
@@ -865,6 +879,7 @@ int dft_plan_tune(dft_plan p, const double* P, const double* Pa, const double* P
   }
   int rc = DFT_OK;
   for (size_t ci = 0; ci < p->combos.size() && rc == DFT_OK; ++ci) {
+    if (combo >= 0 && (int)ci != combo) continue;
     Combo& c = p->combos[ci];
     const long long ntot = (long long)c.q.size();
     for (int part = 0; part < 2 && rc == DFT_OK; ++part) {
@@ -872,14 +887,19 @@ int dft_plan_tune(dft_plan p, const double* P, const double* Pa, const double* P
       // part, by the mixed mode); part 1: weak part in FP32
       const long long first = part ? c.n64 : 0, n = part ? ntot - c.n64 : ntot;
       if (n <= 0) continue;
-      std::vector<Shape> cand = wave_c;
-      if (lane_kernel_combo((int)ci)) cand.insert(cand.begin(), lane_c.begin(), lane_c.end());
+      // the wave kernel only competes where a lane holds many integrals
+      // ((pp|pp) and the d classes); for the light classes it is idle-heavy
+      std::vector<Shape> cand;
+      if (lane_kernel_combo((int)ci)) cand = lane_c;
+      if (!lane_kernel_combo((int)ci) || ci == 5 || c.pcb >= 3) cand.insert(cand.end(), wave_c.begin(), wave_c.end());
       float best = 1e30f;
       Shape best_sh = part ? c.shape32 : c.shape64;
       for (const Shape& sh : cand) {
         hipError_t e = launch_combo(p, ci, part == 1, first, n, sh, P, Pa, Pb, s);  // warm-up
         std::vector<float> t;
         for (int r = 0; r < reps && e == hipSuccess; ++r) {
+          // a shape already 2x slower than the best one is not timed again
+          if (r > 0 && !t.empty() && t[0] > 2.0f * best) break;
           (void)hipEventRecord(e0, s);
           e = launch_combo(p, ci, part == 1, first, n, sh, P, Pa, Pb, s);
           (void)hipEventRecord(e1, s);

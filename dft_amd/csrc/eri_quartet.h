@@ -7,7 +7,8 @@
 //                          E^cd_tau E^cd_nu E^cd_phi R[h + (tau,nu,phi)]
 //   bra step  I[ab][cd] += sum_h E^ab_t E^ab_u E^ab_v G[h][cd]
 // with the bra step applied once per bra primitive pair (G is summed over
-// ket primitives first). pref = 2 pi^{5/2} / (p q sqrt(p+q)) K_ab K_cd.
+// ket primitives first). pref = 2 pi^{5/2} / (p q sqrt(p+q)) K_ab K_cd, with the
+// 1/p, 1/q factors stored in the pair records.
 //
 // Reference: eri_kernel (shelled.py:452-532) computes the same integral per
 // primitive Cartesian quartet as C_x C_y C_z . R (convolution + box sum) and
@@ -61,10 +62,11 @@ __device__ __forceinline__ void quartet_integrals(const real* __restrict__ recb,
       const real* rk = reck + j * S::RSK;
       const real q = rk[0];
       const real X = Px - rk[1], Y = Py - rk[2], Z = Pz - rk[3];
-      const real pq = p + q;
-      const real rpq = real(1) / pq;
-      const real alpha = p * q * rpq;
-      const real pref = real(TWO_PI_2P5) * Kab * rk[4] / (p * q) * sqrt(rpq);
+      // K_ab and K_cd carry 1/p and 1/q (pair pre-pass), so
+      // pref = 2 pi^{5/2} K_ab K_cd / sqrt(p+q) and alpha = p q / (p+q)
+      const real rs = rsqrt_nr(p + q);
+      const real alpha = p * q * rs * rs;
+      const real pref = real(TWO_PI_2P5) * Kab * rk[4] * rs;
       const real T = alpha * (X * X + Y * Y + Z * Z);
       real F[S::L + 1];
       boys_eval<S::L, real>(T, btab, F);

@@ -7,7 +7,7 @@ namespace dft {
 
 // ------------------------------------------------------- pair pre-pass
 // One lane per primitive pair (i_a of shell a, i_b of shell b). Record:
-//   p, Px, Py, Pz, K_ab = c_a c_b exp(-ab/p |A-B|^2), then per axis the
+//   p, Px, Py, Pz, K_ab = c_a c_b exp(-ab/p |A-B|^2) / p, then per axis the
 //   table E_t^{ia,ib} (ia <= la, ib <= lb) of calc_E (shelled.py:228-294),
 //   with E_0^{00} = 1 and the angular factor 1/sqrt((2ia-1)!!(2ib-1)!!).
 // a is raised first (X_PA = -b Q/p), then b (X_PB = a Q/p), as in calc_E.
@@ -31,7 +31,7 @@ __global__ void pair_records_kernel(const int4* __restrict__ pp, int n, const in
   double head[5];
   for (int d = 0; d < 3; ++d) head[1 + d] = (a * Ac[d] + b * Bc[d]) / p;
   head[0] = p;
-  head[4] = ca * cb * exp(-mu * (AB[0] * AB[0] + AB[1] * AB[1] + AB[2] * AB[2]));
+  head[4] = ca * cb * exp(-mu * (AB[0] * AB[0] + AB[1] * AB[1] + AB[2] * AB[2])) / p;
   for (int k = 0; k < 5; ++k) { o64[k] = head[k]; o32[k] = (float)head[k]; }
   // (2l-1)!! for l <= 4: 1, 1, 3, 15, 105
   const double dfac[5] = {1.0, 1.0, 3.0, 15.0, 105.0};

@@ -98,9 +98,10 @@ __global__ __launch_bounds__(64) void eri_wave_kernel(WaveClass C, const int2* _
         const real* rk = rec + pk.z + (long long)j * C.rsk;
         const real qq = rk[0];
         const real X = Px - rk[1], Y = Py - rk[2], Z = Pz - rk[3];
-        const real pq = p + qq;
-        const real alpha = p * qq / pq;
-        const real pref = real(34.986836655249725693) * Kab * rk[4] / (p * qq) * sqrt(real(1) / pq);
+        // K_ab and K_cd carry 1/p and 1/q (pair pre-pass)
+        const real rs = rsqrt_nr(p + qq);
+        const real alpha = p * qq * rs * rs;
+        const real pref = real(34.986836655249725693) * Kab * rk[4] * rs;
         double F[WMAX_L + 2];
         boys_global(C.L, (double)(alpha * (X * X + Y * Y + Z * Z)), boys_gtab, F);
         // R levels: R^n_000 = (-2a)^n F_n

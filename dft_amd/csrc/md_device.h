@@ -23,6 +23,19 @@ __host__ __device__ __forceinline__ void static_for(F&& f) {
   }
 }
 
+// ----------------------------------------------------- reciprocal sqrt
+// FP64: hardware v_rsq_f64 estimate refined by two Newton steps
+// r <- r (3 - x r^2) / 2 (each step squares the relative error), instead of
+// the IEEE sqrt + divide sequences. FP32: the hardware estimate (~1 ulp).
+__device__ __forceinline__ double rsqrt_nr(double x) {
+  double r = __builtin_amdgcn_rsq(x);
+  double h = 0.5 * x;
+  r = r * fma(-h * r, r, 1.5);
+  r = r * fma(-h * r, r, 1.5);
+  return r;
+}
+__device__ __forceinline__ float rsqrt_nr(float x) { return __builtin_amdgcn_rsqf(x); }
+
 // -------------------------------------------------------- index algebra
 __host__ __device__ constexpr int ncart(int l) { return (l + 1) * (l + 2) / 2; }
 // number of Hermite functions with t+u+v <= L
@@ -127,9 +140,10 @@ __device__ __forceinline__ void boys_fp32(float Tf, const double* __restrict__ t
     int k = __double2int_rn(T * BOYS_INV_DT);
     double d = -(T - k * BOYS_DT);
     const double* c = tab + k * BOYS_TAYLOR;
-    double f = c[BOYS_TAYLOR - 1];
+    // five Taylor terms: truncation below 3e-10 relative, far under FP32 rounding
+    double f = c[4];
 #pragma unroll
-    for (int j = BOYS_TAYLOR - 2; j >= 0; --j) f = fma(f, d, c[j]);
+    for (int j = 3; j >= 0; --j) f = fma(f, d, c[j]);
     F[L] = (float)f;
     if constexpr (L > 0) {
       float e = __expf(-Tf);

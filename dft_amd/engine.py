@@ -268,15 +268,22 @@ class Graph:
             self.handle = None
 
 
-def tune(plan, P, Pa, Pb=None, reps=3):
-    """Autotune the plan's launch shapes on these densities; returns the
-    per-combination choices as an int array [ncombo, 6]."""
-    ncombo = len(plan.stats()["per_combo"])
+def tune(plan, P, Pa, Pb=None, reps=3, log=None):
+    """Autotune the plan's launch shapes on these densities, one class
+    combination at a time; returns the choices as an int array [ncombo, 6]
+    (FP64 {wave, block, cap}, FP32 {wave, block, cap})."""
+    per = plan.stats()["per_combo"]
+    ncombo = len(per)
     rep = (C.c_int * (6 * ncombo))()
     s = torch.cuda.current_stream(plan.device)
-    _lib.check(plan.lib.dft_plan_tune(plan.handle, P.data_ptr(), Pa.data_ptr(),
-                                      Pb.data_ptr() if Pb is not None else None, int(reps), rep, 6 * ncombo,
-                                      C.c_void_p(s.cuda_stream)))
+    for c in range(ncombo):
+        if per[c][0] + per[c][1] == 0:
+            continue
+        _lib.check(plan.lib.dft_plan_tune(plan.handle, P.data_ptr(), Pa.data_ptr(),
+                                          Pb.data_ptr() if Pb is not None else None, int(reps), c, rep,
+                                          6 * ncombo, C.c_void_p(s.cuda_stream)))
+        if log:
+            log(f"tuned combination {c}: {list(rep[6 * c:6 * c + 6])}")
     torch.cuda.synchronize(plan.device)
     return np.array(rep[:], dtype=np.int64).reshape(ncombo, 6)
 

@@ -132,7 +132,7 @@ def water_batch_scf(steps=5, warmup=2, tune_reps=2, scf_check=True):
     ms_batch64_eager = time_builds(lambda: bp.jk_into(P, Pa, None, J, K, None, "fp64"), steps, warmup)
     log(f"batched fp64 eager untuned: {ms_batch64_eager:.2f} ms/iter; tuning ...")
     t0 = time.perf_counter()
-    shapes = tune(bp, P, Pa, None, reps=tune_reps)
+    shapes = tune(bp, P, Pa, None, reps=tune_reps, log=log)
     t_tune = time.perf_counter() - t0
     log(f"tuned in {t_tune:.1f} s")
     g64 = Graph(bp, P, Pa, None, J, K, None, "fp64")

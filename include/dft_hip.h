@@ -118,11 +118,13 @@ int dft_graph_destroy(dft_graph graph);
  * 512..8192, and the wave-per-quartet kernel at three grid caps (median of
  * reps runs each) and keeps the fastest. report (capacity report_cap ints)
  * receives 6 ints per combination: FP64 {wave, block, cap}, FP32 {wave,
- * block, cap}. Overwrites the plan's staging (J/K are not written).
+ * block, cap}. combo >= 0 tunes only that combination (-1: all). The wave
+ * kernel is a candidate for (pp|pp) and the d classes only. Overwrites the
+ * plan's staging (J/K are not written).
  * No counterpart in the reference.
  */
-int dft_plan_tune(dft_plan plan, const double* P, const double* Pa, const double* Pb, int reps, int* report,
-                  int report_cap, void* stream);
+int dft_plan_tune(dft_plan plan, const double* P, const double* Pa, const double* Pb, int reps, int combo,
+                  int* report, int report_cap, void* stream);
 
 /* Set launch shapes (6 ints per combination, as reported by dft_plan_tune). */
 int dft_plan_set_shapes(dft_plan plan, const int* shapes, int n);
