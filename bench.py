@@ -5,9 +5,11 @@ uniform random quaternion (seed 0); STO-3G, 112 basis functions, 80 shells;
 Schwarz threshold 1e-10. A few plain-SCF iterations give a realistic
 density; then ``--steps`` direct J/K builds are timed back to back on the
 device (one step = one Fock build = every screened shell quartet evaluated
-and digested into J and K). The metric value is unique screened contracted
-shell quartets per second; ms_per_step is the Fock-build time per SCF
-iteration.
+and digested into J and K). The metric value is unique (8-fold) contracted
+Cartesian AO quartets surviving screening per second, counted at shell level
+(prod n(l) per screened unique shell quartet); shell and primitive shell
+quartets/s are reported beside it; ms_per_step is the Fock-build time per
+SCF iteration.
 
 Also reported: the mixed-precision build (FP32 for Q_ab Q_cd < 1e-4) and
 its SCF energy difference to FP64, per-launch HIP-event timings with the
@@ -62,11 +64,15 @@ def cpu_baseline(shells, Q, tau, P, seconds, n_quartets_gpu):
     t0 = time.perf_counter()
     ref.build_jk(seg, quart[perm[:n]], P, P / 2, None, nthreads=threads)
     t = time.perf_counter() - t0
+    nc = (seg.l + 1) * (seg.l + 2) // 2
+    s = quart[perm[:n]]
+    n_ao = int((nc[s[:, 0]] * nc[s[:, 1]] * nc[s[:, 2]] * nc[s[:, 3]]).sum())
     rate = n / t
-    return {"value": rate, "unit": "quartets/s", "cores": threads, "kind": "port",
-            "sample": f"{n} of {len(quart)} screened unique shell quartets, uniform random (seed 0), "
-                      f"{t:.1f} s; C restatement of the reference per-primitive-Cartesian MD path "
+    return {"value": n_ao / t, "unit": "AO quartets/s", "cores": threads, "kind": "port",
+            "sample": f"{n} of {len(quart)} screened unique shell quartets ({n_ao} AO quartets), uniform random "
+                      f"(seed 0), {t:.1f} s; C restatement of the reference per-primitive-Cartesian MD path "
                       f"(oracle/md_ref.c, OpenMP {threads} threads)",
+            "shell_quartets_per_s": rate,
             "s_per_fock_build_extrapolated": len(quart) / rate,
             "n_screened_list": int(len(quart)), "gpu_list_matches": int(len(quart)) == n_quartets_gpu}
 
@@ -78,9 +84,23 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--fp64-only", action="store_true",
                     help="only the FP64 timed region (for profiler runs)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run 5 device-to-device copies of MIB MiB (calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters against a known byte count)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        n = args.copy_probe * (1 << 20) // 8
+        src = torch.rand(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": n * 8, "copies": 5}), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: the Fock build shards by quartet blocks with one "
               "all-reduce of J/K per iteration (design only); run with --gpus 1", file=sys.stderr)
@@ -88,7 +108,7 @@ def main():
 
     import torch
     from dft_amd.basis import build_shells, load_basis
-    from dft_amd.engine import Plan, profile_read, profile_start
+    from dft_amd.engine import Plan, profile_read, profile_start, tune
     from dft_amd.flops import build_flops, combo_flops
     from dft_amd.geometry import water_cluster
     from dft_amd.scf import scf_plain
@@ -117,6 +137,14 @@ def main():
         for _ in range(k):
             plan.jk_into(P, P_a, None, J, K, None, mode)
 
+    # launch shapes per class combination from the autotuner (untimed)
+    shapes, t_tune = None, 0.0
+    if not args.no_tune:
+        t0 = time.perf_counter()
+        shapes = tune(plan, P, P_a, None, reps=3)
+        t_tune = time.perf_counter() - t0
+        log(f"tuned in {t_tune:.1f} s")
+
     # warmup, then the timed region (no per-launch events inside it)
     run(args.warmup, "fp64")
     torch.cuda.synchronize()
@@ -138,10 +166,13 @@ def main():
     profile_start(plan, 0)
     ms_per_step = ms_total / args.steps
     quartets = st["n_quartets"]
-    value = quartets / (ms_per_step * 1e-3)
-    log(f"fp64: {ms_per_step:.3f} ms/build, {value:.3e} quartets/s (wall {wall:.3f} s)")
+    value = st["n_ao_quartets"] / (ms_per_step * 1e-3)
+    log(f"fp64: {ms_per_step:.3f} ms/build, {value:.3e} AO quartets/s, "
+        f"{quartets / (ms_per_step * 1e-3):.3e} shell quartets/s (wall {wall:.3f} s)")
 
-    # dominant kernel roofline (FP64 part of one class combination)
+    # dominant kernel roofline (FP64 launch of one class combination); flops
+    # from the algorithmic model: K_a K_b K_c K_d primitive quartets per
+    # surviving contracted quartet
     per = st["per_combo"]
     ncombo = len(per)
     eri_ms = ms_slots[:2 * ncombo]
@@ -149,33 +180,40 @@ def main():
     combo = dom // 2
     # FP64 mode: one launch per class combination over its whole list
     nq_dom = int(per[combo][0] + per[combo][1])
-    npq_dom = int(per[combo][2] + per[combo][3])
-    f_dom = combo_flops(combo, nq_dom, npq_dom)
+    f_dom = combo_flops(combo, nq_dom, int(per[combo][4] + per[combo][5]))
+    f_dom_eval = combo_flops(combo, nq_dom, int(per[combo][2] + per[combo][3]))
     t_dom = eri_ms[dom] / nrec * 1e-3
     achieved = f_dom / t_dom / 1e12
-    f64_total, _ = build_flops(per)
-    f64_all = f64_total + build_flops(per)[1]  # whole list in FP64 in this mode
+    f_all = sum(build_flops(per))          # whole list in FP64 in this mode
+    f_all_eval = sum(build_flops(per, model=False))
     from dft_amd.flops import combo_moments
     roof = {"bound": "mfma", "pipe": "fp64 VALU (MFMA FP64 peak is the same 78.6 TFLOP/s)",
             "kernel": "eri_jk_kernel<%d,%d,%d,%d,double>" % combo_moments(combo),
             "achieved": achieved, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
             "frac": achieved * 1e12 / FP64_PEAK, "traffic": None,
             "avg_launch_ms": eri_ms[dom] / nrec, "flops_per_launch": f_dom,
-            "whole_build_tflops": f64_all / (ms_per_step * 1e-3) / 1e12,
-            "whole_build_frac": f64_all / (ms_per_step * 1e-3) / FP64_PEAK,
+            "flops_per_launch_evaluated": f_dom_eval,
+            "whole_build_tflops": f_all / (ms_per_step * 1e-3) / 1e12,
+            "whole_build_frac": f_all / (ms_per_step * 1e-3) / FP64_PEAK,
+            "whole_build_frac_evaluated": f_all_eval / (ms_per_step * 1e-3) / FP64_PEAK,
             "slot_ms_per_step": [float(x) / nrec for x in ms_slots]}
 
     out = {
         "metric": "Fock-build s/iter + ERI quartets/s, FP64 vs mixed at 1 µEh; gradient s/call",
-        "value": value, "unit": "quartets/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "value": value, "unit": "AO quartets/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": "(H2O)16 STO-3G direct J/K (config 1): 4x2x2 lattice 3.1 A, seed-0 rotations, "
-                               "112 bf, Schwarz 1e-10", "n_quartets": quartets,
+                               "112 bf, Schwarz 1e-10", "n_ao_quartets": st["n_ao_quartets"],
+                   "n_quartets": quartets,
                    "n_unscreened": st["n_unscreened"], "n_prim_quartets": st["n_prim_quartets"],
                    "parallelism": "single GPU"},
         "s_per_iter_fock_fp64": ms_per_step * 1e-3,
+        "shell_quartets_per_s": quartets / (ms_per_step * 1e-3),
+        "prim_shell_quartets_per_s": st["n_prim_quartets"] / (ms_per_step * 1e-3),
         "plan_seconds": t_plan,
+        "tune_seconds": t_tune,
+        "tuned_shapes": None if shapes is None else [list(map(int, r)) for r in shapes if r.any()],
         "roofline": roof,
         "cpu_baseline": None,
     }

@@ -107,7 +107,8 @@ struct Combo {
   long long n64 = 0;           // size of the FP64 part
   int2* d_q = nullptr;
   long long* d_slots = nullptr;  // per quartet 2 + 4 nk staging offsets
-  long long prim64 = 0, prim32 = 0;
+  long long prim64 = 0, prim32 = 0;      // primitive shell quartets evaluated
+  long long kprod64 = 0, kprod32 = 0;    // sum of K_a K_b K_c K_d (before primitive screening)
   Shape shape64, shape32;        // for the FP64 launch and the FP32 (weak part) launch
 };
 
@@ -142,6 +143,11 @@ struct dft_plan_s {
   int* d_shell_ao = nullptr;
   double* d_rec64 = nullptr;
   float* d_rec32 = nullptr;
+  // interleaved copies of the records, one block per pair class (lane kernels)
+  double* d_recT64 = nullptr;
+  float* d_recT32 = nullptr;
+  long long cls_base[NPC] = {};
+  int cls_first[NPC] = {}, cls_n[NPC] = {};
   double* d_stage = nullptr;
   Target* d_tg = nullptr;
   long long* d_tbase = nullptr;
@@ -182,7 +188,8 @@ void free_plan(dft_plan p) {
   void* ptrs[] = {p->d_sh,    p->d_exps,  p->d_coefs,   p->d_pairs,    p->d_shell_dm,  p->d_shell_ao,
                   p->d_rec64, p->d_rec32, p->d_stage,   p->d_tg,       p->d_tbase,     p->d_tcount,
                   p->d_half,  p->d_atoms, p->d_shpairs, p->d_mol_base, p->d_mol_nbf,   p->d_grec,
-                  p->d_gpairs, p->d_g1slots, p->d_gbase, p->d_gcount, p->d_gstage};
+                  p->d_gpairs, p->d_g1slots, p->d_gbase, p->d_gcount, p->d_gstage, p->d_recT64,
+                  p->d_recT32};
   for (void* x : ptrs) (void)hipFree(x);
   for (hipEvent_t e : p->prof_ev) (void)hipEventDestroy(e);
   for (auto* x : p->d_gq_swap) (void)hipFree(x);
@@ -300,6 +307,8 @@ int build_plan(dft_plan p, const Input& in) {
     int la, lb;
     pc_moments(pc, la, lb);
     const int rs = rec_size(la, lb);
+    p->cls_first[pc] = (int)p->pairs.size();
+    p->cls_n[pc] = (int)by_pc[pc].size();
     for (const int2& ij : by_pc[pc]) {
       const ShellDev &A = p->sh[ij.x], &B = p->sh[ij.y];
       const int id = (int)p->pairs.size();
@@ -322,7 +331,7 @@ int build_plan(dft_plan p, const Input& in) {
       p->pair_pc.push_back(pc);
       p->pair_of[pair_key(ij.x, ij.y)] = id;
       class_pairs[A.mol][pc].push_back(id);
-      if (roff > 0x7fffffffLL) return fail(DFT_EINVAL, "pair records exceed 2^31 reals");
+      if (roff * (long long)sizeof(double) > 0xffffffffLL) return fail(DFT_EINVAL, "pair records exceed 4 GiB");
     }
   }
   // ---- device data + pair pre-pass
@@ -356,6 +365,37 @@ int build_plan(dft_plan p, const Input& in) {
     (void)hipFree(dpp);
     (void)hipFree(doff);
     if (e != hipSuccess) return fail(DFT_EHIP, std::string("pair pre-pass: ") + hipGetErrorString(e));
+  }
+  // ---- interleaved record blocks per pair class: element f of primitive
+  // pair j of class-local pair u at base + (j * rs + f) * n + u
+  {
+    std::vector<int4> map;
+    long long tot = 0;
+    for (int pc = 0; pc < NPC; ++pc) {
+      int la, lb;
+      pc_moments(pc, la, lb);
+      const int rs = rec_size(la, lb), n = p->cls_n[pc], f0 = p->cls_first[pc];
+      int mp = 0;
+      for (int u = 0; u < n; ++u) mp = std::max(mp, p->pairs[f0 + u].w);
+      p->cls_base[pc] = tot;
+      for (int u = 0; u < n; ++u) {
+        const int4 pr = p->pairs[f0 + u];
+        for (int j = 0; j < pr.w; ++j)
+          map.push_back(make_int4(pr.z + j * rs, (int)(tot + (long long)j * rs * n + u), rs, n));
+      }
+      tot += (long long)mp * rs * n;
+      if (tot * (long long)sizeof(double) > 0xffffffffLL)
+        return fail(DFT_EINVAL, "interleaved pair records exceed 4 GiB");
+    }
+    int4* dmap = nullptr;
+    if ((rc = upload(&dmap, map))) return rc;
+    HIP_TRY(hipMalloc(&p->d_recT64, std::max<long long>(tot, 1) * sizeof(double)));
+    HIP_TRY(hipMalloc(&p->d_recT32, std::max<long long>(tot, 1) * sizeof(float)));
+    hipError_t e = launch_interleave_records(dmap, (int)map.size(), p->d_rec64, p->d_rec32, p->d_recT64,
+                                             p->d_recT32, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    (void)hipFree(dmap);
+    if (e != hipSuccess) return fail(DFT_EHIP, std::string("record interleave: ") + hipGetErrorString(e));
   }
   // ---- atoms and shell pairs for the 1-e kernel
   std::vector<double> at(4 * (size_t)p->natom);
@@ -407,20 +447,28 @@ int build_plan(dft_plan p, const Input& in) {
       c.pck = pck;
       if (!lane_kernel_combo(combo_index(pcb, pck))) c.shape64.wave = c.shape32.wave = 1;
       std::vector<int2> weak;
+      // ket-major: consecutive quartets (the lanes of a wave) share the ket
+      // pair, whose primitive records the lane kernel reads in its inner loop,
+      // so those loads are wave-uniform; the bra pairs of a wave are
+      // consecutive pairs of their class (coalesced in the interleaved layout)
       for (int m = 0; m < in.nmol; ++m)
-        for (int ib : class_pairs[m][pcb]) {
-          for (int ik : class_pairs[m][pck]) {
-            if (pcb == pck && ik > ib) break;
+        for (int ik : class_pairs[m][pck]) {
+          for (int ib : class_pairs[m][pcb]) {
+            if (pcb == pck && ib < ik) continue;
             ++p->n_unscreened;
             const double b = p->Q[ib] * p->Q[ik];
             if (b < p->tau) continue;
             const long long np = (long long)p->pairs[ib].w * p->pairs[ik].w;
+            const long long kp = (long long)p->sh[p->pairs[ib].x].nprim * p->sh[p->pairs[ib].y].nprim *
+                                 p->sh[p->pairs[ik].x].nprim * p->sh[p->pairs[ik].y].nprim;
             if (b < p->tau32) {
               weak.push_back(make_int2(ib, ik));
               c.prim32 += np;
+              c.kprod32 += kp;
             } else {
               c.q.push_back(make_int2(ib, ik));
               c.prim64 += np;
+              c.kprod64 += kp;
             }
           }
         }
@@ -516,9 +564,17 @@ hipError_t launch_combo(dft_plan p, size_t ci, bool f32, long long first, long l
   const Combo& c = p->combos[ci];
   const int nblk = 2 + 4 * p->nk;
   const void* rec = f32 ? (const void*)p->d_rec32 : (const void*)p->d_rec64;
-  if (!sh.wave && lane_kernel_combo((int)ci))
-    return launch_eri_jk((int)ci, f32, c.d_q + first, c.d_slots + first * nblk, n, p->d_pairs, rec, p->d_shell_dm, P,
-                         Pa, Pb, p->nk, p->d_stage, p->d_boys, sh.block, sh.cap, s);
+  if (!sh.wave && lane_kernel_combo((int)ci)) {
+    RecView rv;
+    const size_t esz = f32 ? sizeof(float) : sizeof(double);
+    const char* t = f32 ? (const char*)p->d_recT32 : (const char*)p->d_recT64;
+    rv.bra = t + p->cls_base[c.pcb] * esz;
+    rv.sb = p->cls_n[c.pcb];
+    rv.fb = p->cls_first[c.pcb];
+    rv.ket = rec;
+    return launch_eri_jk((int)ci, f32, c.d_q + first, c.d_slots + first * nblk, n, p->d_pairs, rv, p->d_shell_dm,
+                         P, Pa, Pb, p->nk, p->d_stage, p->d_boys, sh.block, sh.cap, s);
+  }
   int la, lb, lc, ld;
   pc_moments(c.pcb, la, lb);
   pc_moments(c.pck, lc, ld);
@@ -786,10 +842,12 @@ int dft_plan_stats(dft_plan p, int64_t* counts, int ncombo, int* ncombo_out, int
   if (ncombo_out) *ncombo_out = (int)p->combos.size();
   if (counts)
     for (int c = 0; c < std::min<int>(ncombo, (int)p->combos.size()); ++c) {
-      counts[4 * c] = (int64_t)p->combos[c].n64;
-      counts[4 * c + 1] = (int64_t)p->combos[c].q.size() - p->combos[c].n64;
-      counts[4 * c + 2] = p->combos[c].prim64;
-      counts[4 * c + 3] = p->combos[c].prim32;
+      counts[6 * c] = (int64_t)p->combos[c].n64;
+      counts[6 * c + 1] = (int64_t)p->combos[c].q.size() - p->combos[c].n64;
+      counts[6 * c + 2] = p->combos[c].prim64;
+      counts[6 * c + 3] = p->combos[c].prim32;
+      counts[6 * c + 4] = p->combos[c].kprod64;
+      counts[6 * c + 5] = p->combos[c].kprod32;
     }
   if (n_unscreened) *n_unscreened = p->n_unscreened;
   if (staging_bytes) *staging_bytes = p->stage_len * (int64_t)sizeof(double);
@@ -863,7 +921,7 @@ int dft_plan_tune(dft_plan p, const double* P, const double* Pa, const double* P
   HIP_TRY(hipEventCreate(&e1));
   // candidate launch shapes: lane kernel (block x grid cap) and wave kernel
   std::vector<Shape> lane_c, wave_c;
-  for (int b : {64, 128, 256})
+  for (int b : {64, 128, 256, 512})
     for (int cap : {256 * 2, 256 * 4, 256 * 8, 256 * 16, 256 * 32}) {
       Shape sh;
       sh.wave = 0;
@@ -890,7 +948,9 @@ int dft_plan_tune(dft_plan p, const double* P, const double* Pa, const double* P
       // the wave kernel only competes where a lane holds many integrals
       // ((pp|pp) and the d classes); for the light classes it is idle-heavy
       std::vector<Shape> cand;
-      if (lane_kernel_combo((int)ci)) cand = lane_c;
+      if (lane_kernel_combo((int)ci))
+        for (const Shape& sh : lane_c)
+          if (sh.block <= lane_block_max((int)ci)) cand.push_back(sh);
       if (!lane_kernel_combo((int)ci) || ci == 5 || c.pcb >= 3) cand.insert(cand.end(), wave_c.begin(), wave_c.end());
       float best = 1e30f;
       Shape best_sh = part ? c.shape32 : c.shape64;
@@ -953,8 +1013,10 @@ int dft_plan_set_shapes(dft_plan p, const int* shapes, int n) {
     s32.cap = shapes[6 * ci + 5];
     for (const Shape* s : {&s64, &s32}) {
       if (!s->wave && !lane_kernel_combo((int)ci)) return fail(DFT_EINVAL, "combination has no lane kernel");
-      if (!s->wave && s->block != 64 && s->block != 128 && s->block != 256)
-        return fail(DFT_EINVAL, "block must be 64, 128 or 256");
+      if (!s->wave && s->block != 64 && s->block != 128 && s->block != 256 && s->block != 512)
+        return fail(DFT_EINVAL, "block must be 64, 128, 256 or 512");
+      if (!s->wave && s->block > lane_block_max((int)ci))
+        return fail(DFT_EINVAL, "block exceeds the class's largest workgroup");
       if (s->cap < 1) return fail(DFT_EINVAL, "grid cap must be positive");
     }
     c.shape64 = s64;

@@ -22,12 +22,29 @@
 
 namespace dft {
 
-// E_t^{ia,ib} of one axis from a pair record; E_0^{00} = 1 by construction
-// (the exp(-mu X_AB^2) factor lives in K_ab), so that entry is never loaded.
-template <int LB, int IA, int IB, int T, typename real>
-__device__ __forceinline__ real e_at(const real* __restrict__ E) {
+// Pair-record access. Element f of primitive pair i of one shell pair sits
+// at base[f * s + off + i * RS * s]:
+//   plain layout        s = 1, off = the pair's record offset;
+//   interleaved layout  base = the pair class's block, s = pairs in the class,
+//                       off = the pair's index within the class,
+// so in the interleaved layout the lanes of a wave working on neighbouring
+// pairs of a class read neighbouring addresses (coalesced), and base + f*s
+// is wave-uniform (scalar base register, 32-bit vector offset).
+// Byte offsets are formed in 32 bits (record blocks stay below 4 GiB), so
+// the load takes the scalar-base + 32-bit vector-offset form.
+template <typename real>
+__device__ __forceinline__ real fld(const real* __restrict__ base, int s, int f, unsigned off) {
+  const char* __restrict__ b = reinterpret_cast<const char*>(base + f * s);
+  return *reinterpret_cast<const real*>(b + off * (unsigned)sizeof(real));
+}
+
+// E_t^{ia,ib} of one axis (first element AX of that axis' table in the
+// record); E_0^{00} = 1 by construction (the exp(-mu X_AB^2) factor lives in
+// K_ab), so that entry is never loaded.
+template <int LB, int IA, int IB, int T, int AX, typename real>
+__device__ __forceinline__ real e_at(const real* __restrict__ base, int s, unsigned off) {
   if constexpr (IA == 0 && IB == 0) return real(1);
-  else return E[eoff(LB, IA, IB) + T];
+  else return fld(base, s, AX + eoff(LB, IA, IB) + T, off);
 }
 
 constexpr double TWO_PI_2P5 = 34.986836655249725693;  // 2 pi^{5/2}
@@ -43,38 +60,49 @@ struct QuartetShape {
   static constexpr int RSB = rec_size(LA, LB), RSK = rec_size(LC, LD);
 };
 
-// I[ab * NCD + cd] for the quartet (bra pair record block, ket pair record block).
+// I[ab * NCD + cd] for the quartet (bra pair: base bb, stride sb, offset ob,
+// nb primitive pairs; ket pair likewise).
 template <int LA, int LB, int LC, int LD, typename real>
-__device__ __forceinline__ void quartet_integrals(const real* __restrict__ recb, int nb,
-                                                  const real* __restrict__ reck, int nk,
+__device__ __forceinline__ void quartet_integrals(const real* __restrict__ bb, int sb, unsigned ob, int nb,
+                                                  const real* __restrict__ bk, int sk, unsigned ok, int nk,
                                                   const double* __restrict__ btab,
                                                   real (&I)[QuartetShape<LA, LB, LC, LD, real>::NI]) {
   using S = QuartetShape<LA, LB, LC, LD, real>;
+  constexpr int EXB = 5, EYB = 5 + S::NEB, EZB = 5 + 2 * S::NEB;
+  constexpr int EXK = 5, EYK = 5 + S::NEK, EZK = 5 + 2 * S::NEK;
 #pragma unroll
   for (int x = 0; x < S::NI; ++x) I[x] = real(0);
   for (int i = 0; i < nb; ++i) {
-    const real* rb = recb + i * S::RSB;
-    const real p = rb[0], Px = rb[1], Py = rb[2], Pz = rb[3], Kab = rb[4];
+    const unsigned oi = ob + (unsigned)(i * S::RSB * sb);
+    const real p = fld(bb, sb, 0, oi), Px = fld(bb, sb, 1, oi), Py = fld(bb, sb, 2, oi);
+    const real Pz = fld(bb, sb, 3, oi), Kab = fld(bb, sb, 4, oi);
     real G[S::NHB * S::NCD];
 #pragma unroll
     for (int x = 0; x < S::NHB * S::NCD; ++x) G[x] = real(0);
+    // the head (q, Q, K_cd) of ket primitive j + 1 is loaded while primitive
+    // j is evaluated, which hides most of the L2 latency of the record loads
+    unsigned oj = ok;
+    real nq = real(0), nQx = real(0), nQy = real(0), nQz = real(0), nK = real(0);
+    if (nk > 0) {
+      nq = fld(bk, sk, 0, oj), nQx = fld(bk, sk, 1, oj), nQy = fld(bk, sk, 2, oj);
+      nQz = fld(bk, sk, 3, oj), nK = fld(bk, sk, 4, oj);
+    }
     for (int j = 0; j < nk; ++j) {
-      const real* rk = reck + j * S::RSK;
-      const real q = rk[0];
-      const real X = Px - rk[1], Y = Py - rk[2], Z = Pz - rk[3];
+      const real q = nq, X = Px - nQx, Y = Py - nQy, Z = Pz - nQz, Kcd = nK;
+      const unsigned ojc = oj;
+      if (j + 1 < nk) oj += (unsigned)(S::RSK * sk);
+      nq = fld(bk, sk, 0, oj), nQx = fld(bk, sk, 1, oj), nQy = fld(bk, sk, 2, oj);
+      nQz = fld(bk, sk, 3, oj), nK = fld(bk, sk, 4, oj);
       // K_ab and K_cd carry 1/p and 1/q (pair pre-pass), so
       // pref = 2 pi^{5/2} K_ab K_cd / sqrt(p+q) and alpha = p q / (p+q)
       const real rs = rsqrt_nr(p + q);
       const real alpha = p * q * rs * rs;
-      const real pref = real(TWO_PI_2P5) * Kab * rk[4] * rs;
+      const real pref = real(TWO_PI_2P5) * Kab * Kcd * rs;
       const real T = alpha * (X * X + Y * Y + Z * Z);
       real F[S::L + 1];
       boys_eval<S::L, real>(T, btab, F);
       real R[nherm(S::L)];
       hermite_R<S::L, real>(F, real(-2) * alpha, X, Y, Z, R);
-      const real* Exk = rk + 5;
-      const real* Eyk = Exk + S::NEK;
-      const real* Ezk = Eyk + S::NEK;
       static_for<0, S::NCD>([&](auto cdI) {
         constexpr int cd = decltype(cdI)::value;
         constexpr int c = cd / S::ND, d = cd % S::ND;
@@ -87,7 +115,8 @@ __device__ __forceinline__ void quartet_integrals(const real* __restrict__ recb,
             static_for<0, cz + dz + 1>([&](auto vI) {
               constexpr int ta = decltype(tI)::value, nu = decltype(uI)::value, ph = decltype(vI)::value;
               constexpr int k = (ta * (cy + dy + 1) + nu) * (cz + dz + 1) + ph;
-              real e = e_at<LD, cx, dx, ta>(Exk) * e_at<LD, cy, dy, nu>(Eyk) * e_at<LD, cz, dz, ph>(Ezk) * pref;
+              real e = e_at<LD, cx, dx, ta, EXK>(bk, sk, ojc) * e_at<LD, cy, dy, nu, EYK>(bk, sk, ojc) *
+                       e_at<LD, cz, dz, ph, EZK>(bk, sk, ojc) * pref;
               Ek[k] = ((ta + nu + ph) & 1) ? -e : e;
             });
           });
@@ -109,9 +138,6 @@ __device__ __forceinline__ void quartet_integrals(const real* __restrict__ recb,
         });
       });
     }
-    const real* Exb = rb + 5;
-    const real* Eyb = Exb + S::NEB;
-    const real* Ezb = Eyb + S::NEB;
     static_for<0, S::NAB>([&](auto abI) {
       constexpr int ab = decltype(abI)::value;
       constexpr int a = ab / S::NB, b = ab % S::NB;
@@ -122,7 +148,8 @@ __device__ __forceinline__ void quartet_integrals(const real* __restrict__ recb,
           static_for<0, az + bz + 1>([&](auto vI) {
             constexpr int t = decltype(tI)::value, u = decltype(uI)::value, v = decltype(vI)::value;
             constexpr int h = hidx(t, u, v);
-            const real e = e_at<LB, ax, bx, t>(Exb) * e_at<LB, ay, by, u>(Eyb) * e_at<LB, az, bz, v>(Ezb);
+            const real e = e_at<LB, ax, bx, t, EXB>(bb, sb, oi) * e_at<LB, ay, by, u, EYB>(bb, sb, oi) *
+                           e_at<LB, az, bz, v, EZB>(bb, sb, oi);
             static_for<0, S::NCD>([&](auto cdI) {
               constexpr int cd = decltype(cdI)::value;
               I[ab * S::NCD + cd] = fma(e, G[h * S::NCD + cd], I[ab * S::NCD + cd]);

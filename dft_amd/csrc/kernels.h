@@ -57,8 +57,29 @@ bool lane_kernel_combo(int combo);
 // fused ERI + J/K digest for one class combination (lane per quartet);
 // slots holds 2 + 4*nk staging offsets (doubles) per quartet; block is the
 // workgroup size (64, 128 or 256), grid_cap the maximum number of workgroups
+// Pair records of one lane-kernel launch. The bra pairs of a wave differ
+// from lane to lane and are read from the interleaved block of their class
+// (element f of primitive pair i of class-local pair u at
+// bra[(i * RS + f) * sb + u]; pair id = fb + u), so those loads coalesce;
+// the ket pair is shared by the wave (ket-major quartet order) and is read
+// from the plain records (ket + pair.z), a few cache lines per pair.
+struct RecView {
+  const void* bra;
+  int sb, fb;
+  const void* ket;
+};
+
+// Copies plain pair records (one record of rs reals per primitive pair at
+// src_off) into the interleaved layout: map[r] = {src_off, dst_off, rs, s}
+// with element f going to dst_off + f * s.
+hipError_t launch_interleave_records(const int4* map, int nrec, const double* r64, const float* r32, double* t64,
+                                     float* t32, hipStream_t s);
+
+// largest lane-kernel workgroup of a class combination (0 if none)
+int lane_block_max(int combo);
+
 hipError_t launch_eri_jk(int combo, bool fp32, const int2* quartets, const long long* slots, long long nq,
-                         const int4* pairs, const void* rec, const int4* shell_dm,
+                         const int4* pairs, const RecView& rv, const int4* shell_dm,
                          const double* DJ, const double* DK0, const double* DK1, int nk,
                          double* stage, const double* boys_tab, int block, int grid_cap, hipStream_t s);
 

@@ -5,7 +5,13 @@
 
 namespace dft {
 
-constexpr int ERI_BLOCK = 256;
+constexpr int ERI_BLOCK = 256;  // raw kernel
+
+// Largest J/K workgroup per class: 512 lanes leave 128 VGPRs per lane, which
+// the classes up to (pp|ps) fit; the larger ones keep 256 (and 256 VGPRs).
+constexpr int jk_block_max(int la, int lb, int lc, int ld) {
+  return ncart(la) * ncart(lb) * ncart(lc) * ncart(ld) <= 27 ? 512 : 256;
+}
 
 // Each lane evaluates one canonical quartet (bra pair >= ket pair in the
 // class-sorted pair order) and writes its contribution blocks
@@ -19,9 +25,10 @@ constexpr int ERI_BLOCK = 256;
 // (ordered shell pair of a half matrix) are contiguous, assigned in quartet
 // order at plan time, and summed by a separate fixed-order pass.
 template <int LA, int LB, int LC, int LD, typename real, int NK>
-__global__ __launch_bounds__(ERI_BLOCK) void eri_jk_kernel(
+__global__ __launch_bounds__(jk_block_max(LA, LB, LC, LD)) void eri_jk_kernel(
     const int2* __restrict__ quartets, const long long* __restrict__ slots, long long nq,
-    const int4* __restrict__ pairs, const real* __restrict__ rec, const int4* __restrict__ shell_dm,
+    const int4* __restrict__ pairs, const real* __restrict__ recb, int sb, int fb, const real* __restrict__ reck,
+    const int4* __restrict__ shell_dm,
     const double* __restrict__ DJ, const double* __restrict__ DK0, const double* __restrict__ DK1,
     double* __restrict__ stage, const double* __restrict__ boys_gtab) {
   using S = QuartetShape<LA, LB, LC, LD, real>;
@@ -35,7 +42,8 @@ __global__ __launch_bounds__(ERI_BLOCK) void eri_jk_kernel(
     const int4 pb = pairs[qd.x];
     const int4 pk = pairs[qd.y];
     real I[S::NI];
-    quartet_integrals<LA, LB, LC, LD, real>(rec + pb.z, pb.w, rec + pk.z, pk.w, btab, I);
+    quartet_integrals<LA, LB, LC, LD, real>(recb, sb, (unsigned)(qd.x - fb), pb.w, reck, 1, (unsigned)pk.z, pk.w,
+                                            btab, I);
     double w = 1.0;
     if (pb.x == pb.y) w *= 0.5;
     if (pk.x == pk.y) w *= 0.5;
@@ -146,7 +154,8 @@ __global__ __launch_bounds__(ERI_BLOCK) void eri_raw_kernel(const int2* __restri
     const int4 pb = pairs[qd.x];
     const int4 pk = pairs[qd.y];
     double I[S::NI];
-    quartet_integrals<LA, LB, LC, LD, double>(rec + pb.z, pb.w, rec + pk.z, pk.w, btab, I);
+    quartet_integrals<LA, LB, LC, LD, double>(rec, 1, (unsigned)pb.z, pb.w, rec, 1, (unsigned)pk.z, pk.w, btab,
+                                              I);
     double* o = out + q * S::NI;
 #pragma unroll
     for (int x = 0; x < S::NI; ++x) o[x] = I[x];
@@ -161,18 +170,21 @@ static int grid_for(long long nq, int cap, int block = ERI_BLOCK) {
 }
 
 template <int LA, int LB, int LC, int LD, typename real>
-static hipError_t jk_one(const int2* q, const long long* sl, long long nq, const int4* pairs, const void* rec,
+static hipError_t jk_one(const int2* q, const long long* sl, long long nq, const int4* pairs, const RecView& rv,
                          const int4* ao, const double* DJ, const double* DK0, const double* DK1, int nk,
                          double* stage, const double* bt, int blk, int cap, hipStream_t s) {
   if (nq <= 0) return hipSuccess;
-  if (blk != 64 && blk != 128 && blk != 256) return hipErrorInvalidValue;
+  if (blk != 64 && blk != 128 && blk != 256 && blk != 512) return hipErrorInvalidValue;
+  if (blk > jk_block_max(LA, LB, LC, LD)) return hipErrorInvalidValue;
   dim3 grid(grid_for(nq, cap, blk)), block(blk);
   if (nk == 1)
     hipLaunchKernelGGL((eri_jk_kernel<LA, LB, LC, LD, real, 1>), grid, block, 0, s, q, sl, nq, pairs,
-                       (const real*)rec, ao, DJ, DK0, DK1, stage, bt);
+                       (const real*)rv.bra, rv.sb, rv.fb, (const real*)rv.ket, ao, DJ, DK0, DK1,
+                       stage, bt);
   else
     hipLaunchKernelGGL((eri_jk_kernel<LA, LB, LC, LD, real, 2>), grid, block, 0, s, q, sl, nq, pairs,
-                       (const real*)rec, ao, DJ, DK0, DK1, stage, bt);
+                       (const real*)rv.bra, rv.sb, rv.fb, (const real*)rv.ket, ao, DJ, DK0, DK1,
+                       stage, bt);
   return hipGetLastError();
 }
 
@@ -201,6 +213,18 @@ static hipError_t raw_one(const int2* q, long long nq, const int4* pairs, const 
   X(10, 2, 1, 0, 0)       \
   X(15, 2, 2, 0, 0)
 
+int lane_block_max(int combo) {
+  switch (combo) {
+#define X(id, la, lb, lc, ld) \
+  case id:                    \
+    return jk_block_max(la, lb, lc, ld);
+    DFT_FOR_COMBOS(X)
+#undef X
+    default:
+      return 0;
+  }
+}
+
 bool lane_kernel_combo(int combo) {
   switch (combo) {
 #define X(id, la, lb, lc, ld) \
@@ -214,15 +238,15 @@ bool lane_kernel_combo(int combo) {
 }
 
 hipError_t launch_eri_jk(int combo, bool fp32, const int2* q, const long long* sl, long long nq,
-                         const int4* pairs, const void* rec, const int4* ao, const double* DJ, const double* DK0,
+                         const int4* pairs, const RecView& rv, const int4* ao, const double* DJ, const double* DK0,
                          const double* DK1, int nk, double* stage, const double* bt, int block, int cap,
                          hipStream_t s) {
   switch (combo) {
 #define X(id, la, lb, lc, ld)                                                                          \
   case id:                                                                                             \
-    return fp32 ? jk_one<la, lb, lc, ld, float>(q, sl, nq, pairs, rec, ao, DJ, DK0, DK1, nk, stage, \
-                                                 bt, block, cap, s)                                    \
-                : jk_one<la, lb, lc, ld, double>(q, sl, nq, pairs, rec, ao, DJ, DK0, DK1, nk, stage, \
+    return fp32 ? jk_one<la, lb, lc, ld, float>(q, sl, nq, pairs, rv, ao, DJ, DK0, DK1, nk, stage, \
+                                                 bt, block, cap, s)                                   \
+                : jk_one<la, lb, lc, ld, double>(q, sl, nq, pairs, rv, ao, DJ, DK0, DK1, nk, stage, \
                                                   bt, block, cap, s);
     DFT_FOR_COMBOS(X)
 #undef X

@@ -84,13 +84,37 @@ hipError_t launch_pair_records(const int4* pp, int n, const int* rec_off, const 
   return hipGetLastError();
 }
 
+__global__ void interleave_records_kernel(const int4* __restrict__ map, int nrec, const double* __restrict__ r64,
+                                          const float* __restrict__ r32, double* __restrict__ t64,
+                                          float* __restrict__ t32) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrec) return;
+  const int4 m = map[r];
+  for (int f = 0; f < m.z; ++f) {
+    t64[m.y + (long long)f * m.w] = r64[m.x + f];
+    t32[m.y + (long long)f * m.w] = r32[m.x + f];
+  }
+}
+
+hipError_t launch_interleave_records(const int4* map, int nrec, const double* r64, const float* r32, double* t64,
+                                     float* t32, hipStream_t s) {
+  if (nrec <= 0) return hipSuccess;
+  hipLaunchKernelGGL(interleave_records_kernel, dim3((nrec + 127) / 128), dim3(128), 0, s, map, nrec, r64, r32,
+                     t64, t32);
+  return hipGetLastError();
+}
+
 // ------------------------------------------------- fixed-order reduction
 // One 64-lane wave per target block (an ordered shell pair of one half
-// matrix). The target's contribution blocks are contiguous in the staging
-// buffer; lane l sums blocks l, l+64, ... in order, then a xor butterfly
-// combines the lanes. The summation tree is fixed, so J and K are bitwise
-// reproducible run to run (the reference's add.at scatter is not:
-// shelled.py:857-859).
+// matrix, ne = nrow * ncol <= 36 elements). The target's cnt contribution
+// blocks are contiguous in the staging buffer, so the wave streams them as
+// one contiguous array: with g = 64 / ne groups, lane l < g * ne takes
+// element l % ne of blocks l / ne, l / ne + g, ... (consecutive lanes read
+// consecutive addresses; four partial sums per lane). The groups are then
+// combined in a fixed order
+// (a xor butterfly when ne = 1, else group 0, 1, ... by lane shuffles), so
+// J and K are bitwise reproducible run to run (the reference's add.at
+// scatter is not: shelled.py:857-859).
 __global__ void reduce_kernel(int ntarget, const Target* __restrict__ tg, const long long* __restrict__ tbase,
                               const int* __restrict__ tcount, const double* __restrict__ stage,
                               double* __restrict__ half) {
@@ -100,13 +124,31 @@ __global__ void reduce_kernel(int ntarget, const Target* __restrict__ tg, const 
   const Target t = tg[wave];
   const int ne = t.nrow * t.ncol;
   const double* src = stage + tbase[wave];
-  const int cnt = tcount[wave];
-  for (int el = 0; el < ne; ++el) {
-    double s = 0.0;
-    for (int k = lane; k < cnt; k += 64) s += src[(long long)k * ne + el];
+  const long long tot = (long long)tcount[wave] * ne;
+  const int g = 64 / ne;
+  const int span = g * ne;
+  // four independent partial sums keep four loads in flight per lane
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  if (lane < span) {
+    long long i = lane;
+    for (; i + 3 * span < tot; i += 4 * span) {
+      s0 += src[i];
+      s1 += src[i + span];
+      s2 += src[i + 2 * span];
+      s3 += src[i + 3 * span];
+    }
+    for (; i < tot; i += span) s0 += src[i];
+  }
+  double s = (s0 + s1) + (s2 + s3);
+  if (ne == 1) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    if (lane == 0) half[t.base + (long long)(el / t.ncol) * t.stride + el % t.ncol] = s;
+    if (lane == 0) half[t.base] = s;
+  } else {
+    const int el = lane < ne ? lane : 0;
+    double r = 0.0;
+    for (int k = 0; k < g; ++k) r += __shfl(s, k * ne + el, 64);
+    if (lane < ne) half[t.base + (long long)(lane / t.ncol) * t.stride + lane % t.ncol] = r;
   }
 }
 

@@ -61,10 +61,17 @@ def combo_flops(combo, n_quartets, n_prim_quartets, nk=1):
     return n_prim_quartets * f_prim(la, lb, lc, ld) + n_quartets * f_digest(la, lb, lc, ld, nk)
 
 
-def build_flops(per_combo, nk=1):
-    """(FP64 flops, FP32 flops) of one Fock build from Plan.stats()['per_combo']."""
+def build_flops(per_combo, nk=1, model=True):
+    """(FP64 flops, FP32 flops) of one Fock build from Plan.stats()['per_combo'].
+
+    ``model=True`` counts K_a K_b K_c K_d primitive quartets per surviving
+    contracted quartet (the algorithmic model); ``model=False`` counts only the
+    primitive quartets that survive primitive-pair screening (what the
+    kernels evaluate).
+    """
+    col = 4 if model else 2
     f64 = f32 = 0
     for c, row in enumerate(per_combo):
-        f64 += combo_flops(c, int(row[0]), int(row[2]), nk)
-        f32 += combo_flops(c, int(row[1]), int(row[3]), nk)
+        f64 += combo_flops(c, int(row[0]), int(row[col]), nk)
+        f32 += combo_flops(c, int(row[1]), int(row[col + 1]), nk)
     return f64, f32

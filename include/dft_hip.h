@@ -133,8 +133,10 @@ int dft_plan_set_shapes(dft_plan plan, const int* shapes, int n);
  * Plan statistics. counts receives, for each class combination c (pair
  * classes ss=0, ps=1, pp=2, ds=3, dp=4, dd=5; combination index
  * pcb*(pcb+1)/2 + pck):
- *   counts[4*c+0] FP64 quartets, [4*c+1] FP32 quartets,
- *   [4*c+2] primitive shell quartets of the FP64 part, [4*c+3] of the FP32 part.
+ *   counts[6*c+0] FP64 quartets, [6*c+1] FP32 quartets,
+ *   [6*c+2] primitive shell quartets evaluated in the FP64 part, [6*c+3] in
+ *   the FP32 part (after primitive-pair screening), [6*c+4] and [6*c+5] the
+ *   sums of K_a K_b K_c K_d over the same quartets (before it).
  * ncombo is the capacity of counts in combinations; the number of
  * combinations is returned in *ncombo_out. Also reports the quartets before
  * screening and the staging size in bytes.
