@@ -21,6 +21,13 @@ using namespace dft;
 
 namespace {
 
+// lanes below which a lane-kernel class combination is split over primitives
+constexpr long long SPLIT_LANES = 65536;
+
+}  // namespace
+
+namespace {
+
 thread_local std::string g_err;
 
 int fail(int code, const std::string& msg) {
@@ -110,12 +117,18 @@ struct Combo {
   long long prim64 = 0, prim32 = 0;      // primitive shell quartets evaluated
   long long kprod64 = 0, kprod32 = 0;    // sum of K_a K_b K_c K_d (before primitive screening)
   Shape shape64, shape32;        // for the FP64 launch and the FP32 (weak part) launch
+  // primitive split of the lane kernel: every quartet is evaluated by
+  // splb * splk lanes, lane (cb, ck) taking bra chunk cb and ket chunk ck of
+  // the primitive pairs and writing its own partial J/K blocks; chunk-major
+  // slot order (slot row = (cb * splk + ck) * q.size() + quartet)
+  int splb = 1, splk = 1;
 };
 
 }  // namespace
 
 struct dft_plan_s {
   int nsh = 0, natom = 0, nk = 1, nmol = 1;
+  bool split = true;             // primitive split of small lane-kernel classes
   long long nelem = 0;           // packed matrix elements (sum of nbf_m^2)
   int max_nbf = 0;
   double tau = 0, tau32 = 0;
@@ -474,6 +487,21 @@ int build_plan(dft_plan p, const Input& in) {
         }
       c.n64 = (long long)c.q.size();
       c.q.insert(c.q.end(), weak.begin(), weak.end());
+      // primitive split where the class has too few quartets to fill the
+      // chip with one lane each (~64 Ki lanes): bra chunks first, then ket
+      const long long nq = (long long)c.q.size();
+      if (p->split && nq > 0 && lane_kernel_combo(combo_index(pcb, pck)) && nq < SPLIT_LANES) {
+        int nbmax = 0, nkmax = 0;
+        for (const int2& qq : c.q) {
+          nbmax = std::max(nbmax, p->pairs[qq.x].w);
+          nkmax = std::max(nkmax, p->pairs[qq.y].w);
+        }
+        const long long want = (SPLIT_LANES + nq - 1) / nq;
+        c.splb = (int)std::max<long long>(1, std::min<long long>(want, nbmax));
+        c.splk = (int)std::max<long long>(1, std::min<long long>((want + c.splb - 1) / c.splb, nkmax));
+        c.shape64.wave = c.shape32.wave = 0;
+      }
+      if (nq * c.splb * c.splk > 0x7fffffffLL) return fail(DFT_EINVAL, "class combination exceeds 2^31 lanes");
       if ((rc = upload(&c.d_q, c.q))) return rc;
     }
   // ---- target-major staging: a target is an ordered shell pair (X, Y) of
@@ -514,7 +542,7 @@ int build_plan(dft_plan p, const Input& in) {
   for (const Combo& c : p->combos)
     for (const int2& qq : c.q) {
       quartet_targets(qq, tq);
-      for (int k = 0; k < nblk; ++k) ++tcount[tq[k]];
+      for (int k = 0; k < nblk; ++k) tcount[tq[k]] += c.splb * c.splk;
     }
   std::vector<long long> tbase(p->ntarget, 0), tnext(p->ntarget, 0);
   std::vector<Target> tg(p->ntarget);
@@ -538,15 +566,17 @@ int build_plan(dft_plan p, const Input& in) {
         }
   p->stage_len = cur;
   for (Combo& c : p->combos) {
-    std::vector<long long> slots(c.q.size() * nblk);
-    for (size_t i = 0; i < c.q.size(); ++i) {
-      quartet_targets(c.q[i], tq);
-      for (int k = 0; k < nblk; ++k) {
-        const Target& T = tg[tq[k]];
-        slots[i * nblk + k] = tnext[tq[k]];
-        tnext[tq[k]] += (long long)T.nrow * T.ncol;
+    const size_t nq = c.q.size(), ns = (size_t)c.splb * c.splk;
+    std::vector<long long> slots(nq * ns * nblk);
+    for (size_t ch = 0; ch < ns; ++ch)
+      for (size_t i = 0; i < nq; ++i) {
+        quartet_targets(c.q[i], tq);
+        for (int k = 0; k < nblk; ++k) {
+          const Target& T = tg[tq[k]];
+          slots[(ch * nq + i) * nblk + k] = tnext[tq[k]];
+          tnext[tq[k]] += (long long)T.nrow * T.ncol;
+        }
       }
-    }
     if ((rc = upload(&c.d_slots, slots))) return rc;
   }
   if ((rc = upload(&p->d_tg, tg))) return rc;
@@ -563,6 +593,7 @@ hipError_t launch_combo(dft_plan p, size_t ci, bool f32, long long first, long l
                         const double* P, const double* Pa, const double* Pb, hipStream_t s) {
   const Combo& c = p->combos[ci];
   const int nblk = 2 + 4 * p->nk;
+  if (sh.wave && c.splb * c.splk != 1) return hipErrorInvalidValue;  // split classes run the lane kernel
   const void* rec = f32 ? (const void*)p->d_rec32 : (const void*)p->d_rec64;
   if (!sh.wave && lane_kernel_combo((int)ci)) {
     RecView rv;
@@ -572,8 +603,13 @@ hipError_t launch_combo(dft_plan p, size_t ci, bool f32, long long first, long l
     rv.sb = p->cls_n[c.pcb];
     rv.fb = p->cls_first[c.pcb];
     rv.ket = rec;
-    return launch_eri_jk((int)ci, f32, c.d_q + first, c.d_slots + first * nblk, n, p->d_pairs, rv, p->d_shell_dm,
-                         P, Pa, Pb, p->nk, p->d_stage, p->d_boys, sh.block, sh.cap, s);
+    rv.splb = c.splb;
+    rv.splk = c.splk;
+    rv.first = (int)first;
+    rv.n = (int)n;
+    rv.ntot = (int)c.q.size();
+    return launch_eri_jk((int)ci, f32, c.d_q, c.d_slots, p->d_pairs, rv, p->d_shell_dm, P, Pa, Pb, p->nk,
+                         p->d_stage, p->d_boys, sh.block, sh.cap, s);
   }
   int la, lb, lc, ld;
   pc_moments(c.pcb, la, lb);
@@ -790,6 +826,7 @@ int validate_and_build(int nmol, const int32_t* mol_nsh, const int32_t* mol_nato
   p->nsh = nsh;
   p->natom = natom;
   p->nk = (flags & DFT_PLAN_UHF) ? 2 : 1;
+  p->split = !(flags & DFT_PLAN_NOSPLIT);
   p->tau = schwarz_tau;
   p->tau32 = fp32_tau;
   int rc = build_plan(p, in);
@@ -951,7 +988,8 @@ int dft_plan_tune(dft_plan p, const double* P, const double* Pa, const double* P
       if (lane_kernel_combo((int)ci))
         for (const Shape& sh : lane_c)
           if (sh.block <= lane_block_max((int)ci)) cand.push_back(sh);
-      if (!lane_kernel_combo((int)ci) || ci == 5 || c.pcb >= 3) cand.insert(cand.end(), wave_c.begin(), wave_c.end());
+      if ((!lane_kernel_combo((int)ci) || ci == 5 || c.pcb >= 3) && c.splb * c.splk == 1)
+        cand.insert(cand.end(), wave_c.begin(), wave_c.end());
       float best = 1e30f;
       Shape best_sh = part ? c.shape32 : c.shape64;
       for (const Shape& sh : cand) {
@@ -1013,6 +1051,8 @@ int dft_plan_set_shapes(dft_plan p, const int* shapes, int n) {
     s32.cap = shapes[6 * ci + 5];
     for (const Shape* s : {&s64, &s32}) {
       if (!s->wave && !lane_kernel_combo((int)ci)) return fail(DFT_EINVAL, "combination has no lane kernel");
+      if (s->wave && c.splb * c.splk != 1)
+        return fail(DFT_EINVAL, "combination is split over primitives and runs the lane kernel");
       if (!s->wave && s->block != 64 && s->block != 128 && s->block != 256 && s->block != 512)
         return fail(DFT_EINVAL, "block must be 64, 128, 256 or 512");
       if (!s->wave && s->block > lane_block_max((int)ci))

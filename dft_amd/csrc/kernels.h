@@ -54,9 +54,6 @@ hipError_t launch_pair_records(const int4* pairs_prim /* sh_a, sh_b, i_a, i_b */
 // (s/p classes and the light d classes); the others use the wave kernel
 bool lane_kernel_combo(int combo);
 
-// fused ERI + J/K digest for one class combination (lane per quartet);
-// slots holds 2 + 4*nk staging offsets (doubles) per quartet; block is the
-// workgroup size (64, 128 or 256), grid_cap the maximum number of workgroups
 // Pair records of one lane-kernel launch. The bra pairs of a wave differ
 // from lane to lane and are read from the interleaved block of their class
 // (element f of primitive pair i of class-local pair u at
@@ -67,6 +64,9 @@ struct RecView {
   const void* bra;
   int sb, fb;
   const void* ket;
+  // quartets [first, first + n) of the combination's list of ntot, each
+  // evaluated by splb * splk lanes (primitive split)
+  int first, n, ntot, splb, splk;
 };
 
 // Copies plain pair records (one record of rs reals per primitive pair at
@@ -75,10 +75,14 @@ struct RecView {
 hipError_t launch_interleave_records(const int4* map, int nrec, const double* r64, const float* r32, double* t64,
                                      float* t32, hipStream_t s);
 
+// Fused ERI + J/K digest for one class combination (lane per quartet or per
+// quartet primitive chunk); slots holds 2 + 4*nk staging offsets (doubles)
+// per quartet and chunk; block is the workgroup size (64 ... 512, at most
+// lane_block_max), grid_cap the maximum number of workgroups.
 // largest lane-kernel workgroup of a class combination (0 if none)
 int lane_block_max(int combo);
 
-hipError_t launch_eri_jk(int combo, bool fp32, const int2* quartets, const long long* slots, long long nq,
+hipError_t launch_eri_jk(int combo, bool fp32, const int2* quartets, const long long* slots,
                          const int4* pairs, const RecView& rv, const int4* shell_dm,
                          const double* DJ, const double* DK0, const double* DK1, int nk,
                          double* stage, const double* boys_tab, int block, int grid_cap, hipStream_t s);

@@ -26,24 +26,46 @@ constexpr int jk_block_max(int la, int lb, int lc, int ld) {
 // order at plan time, and summed by a separate fixed-order pass.
 template <int LA, int LB, int LC, int LD, typename real, int NK>
 __global__ __launch_bounds__(jk_block_max(LA, LB, LC, LD)) void eri_jk_kernel(
-    const int2* __restrict__ quartets, const long long* __restrict__ slots, long long nq,
-    const int4* __restrict__ pairs, const real* __restrict__ recb, int sb, int fb, const real* __restrict__ reck,
-    const int4* __restrict__ shell_dm,
-    const double* __restrict__ DJ, const double* __restrict__ DK0, const double* __restrict__ DK1,
-    double* __restrict__ stage, const double* __restrict__ boys_gtab) {
+    const int2* __restrict__ quartets, const long long* __restrict__ slots, int first, int n, int ntot, int splb,
+    int splk, const int4* __restrict__ pairs, const real* __restrict__ recb, int sb, int fb,
+    const real* __restrict__ reck, const int4* __restrict__ shell_dm, const double* __restrict__ DJ,
+    const double* __restrict__ DK0, const double* __restrict__ DK1, double* __restrict__ stage,
+    const double* __restrict__ boys_gtab) {
   using S = QuartetShape<LA, LB, LC, LD, real>;
   constexpr int NA = S::NA, NB = S::NB, NC = S::NC, ND = S::ND;
   constexpr int NBLK = 2 + 4 * NK;
-  __shared__ double btab[BOYS_NG * BOYS_TAYLOR];
+  __shared__ double btab[BOYS_LDS];
   boys_stage<S::L>(boys_gtab, btab);
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += stride) {
+  // lane v: chunk ch = v / n (chunk-major, so the lanes of a wave share the
+  // chunk and mostly the ket pair), quartet first + v % n
+  const unsigned nv = (unsigned)n * (unsigned)(splb * splk);
+  const unsigned stride = gridDim.x * blockDim.x;
+  for (unsigned v = blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += stride) {
+    const unsigned ch = v / (unsigned)n;
+    const int q = first + (int)(v - ch * (unsigned)n);
     const int2 qd = quartets[q];
     const int4 pb = pairs[qd.x];
     const int4 pk = pairs[qd.y];
+    // this lane's primitive ranges [i0, i0 + nbp) of the bra pair and
+    // [j0, j0 + nkp) of the ket pair
+    const int cb = (int)ch / splk, ck = (int)ch - cb * splk;
+    const int szb = (pb.w + splb - 1) / splb, szk = (pk.w + splk - 1) / splk;
+    const int i0 = min(pb.w, cb * szb), j0 = min(pk.w, ck * szk);
+    const int nbp = min(pb.w, i0 + szb) - i0, nkp = min(pk.w, j0 + szk) - j0;
+    const unsigned ob = (unsigned)(qd.x - fb) + (unsigned)(i0 * S::RSB * sb);
+    const unsigned ok = (unsigned)pk.z + (unsigned)(j0 * S::RSK);
     real I[S::NI];
-    quartet_integrals<LA, LB, LC, LD, real>(recb, sb, (unsigned)(qd.x - fb), pb.w, reck, 1, (unsigned)pk.z, pk.w,
-                                            btab, I);
+    // When the whole wave shares the ket primitive range (the common case in
+    // ket-major order) its offset and length are made explicitly
+    // wave-uniform, so the inner-loop record loads become scalar loads and
+    // the loop a scalar loop; otherwise the same code runs per lane.
+    const unsigned okf = (unsigned)__builtin_amdgcn_readfirstlane((int)ok);
+    const int nkf = __builtin_amdgcn_readfirstlane(nkp);
+    if (__all(ok == okf && nkp == nkf)) {
+      quartet_integrals<LA, LB, LC, LD, real>(recb, sb, ob, nbp, reck, 1, okf, nkf, btab, I);
+    } else {
+      quartet_integrals<LA, LB, LC, LD, real>(recb, sb, ob, nbp, reck, 1, ok, nkp, btab, I);
+    }
     double w = 1.0;
     if (pb.x == pb.y) w *= 0.5;
     if (pk.x == pk.y) w *= 0.5;
@@ -55,7 +77,7 @@ __global__ __launch_bounds__(jk_block_max(LA, LB, LC, LD)) void eri_jk_kernel(
     const int nbf = sa.y;
     const long long mb = (long long)(unsigned)sa.z | ((long long)sa.w << 32);
     const double* DJm = DJ + mb;
-    const long long* sl = slots + q * NBLK;
+    const long long* sl = slots + ((long long)ch * ntot + q) * NBLK;
     // Coulomb
     {
       double Pcd[S::NCD], Pab[S::NAB];
@@ -146,7 +168,7 @@ __global__ __launch_bounds__(ERI_BLOCK) void eri_raw_kernel(const int2* __restri
                                                             double* __restrict__ out,
                                                             const double* __restrict__ boys_gtab) {
   using S = QuartetShape<LA, LB, LC, LD, double>;
-  __shared__ double btab[BOYS_NG * BOYS_TAYLOR];
+  __shared__ double btab[BOYS_LDS];
   boys_stage<S::L>(boys_gtab, btab);
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += stride) {
@@ -170,21 +192,22 @@ static int grid_for(long long nq, int cap, int block = ERI_BLOCK) {
 }
 
 template <int LA, int LB, int LC, int LD, typename real>
-static hipError_t jk_one(const int2* q, const long long* sl, long long nq, const int4* pairs, const RecView& rv,
+static hipError_t jk_one(const int2* q, const long long* sl, const int4* pairs, const RecView& rv,
                          const int4* ao, const double* DJ, const double* DK0, const double* DK1, int nk,
                          double* stage, const double* bt, int blk, int cap, hipStream_t s) {
-  if (nq <= 0) return hipSuccess;
+  if (rv.n <= 0) return hipSuccess;
+  const long long nq = (long long)rv.n * rv.splb * rv.splk;
   if (blk != 64 && blk != 128 && blk != 256 && blk != 512) return hipErrorInvalidValue;
   if (blk > jk_block_max(LA, LB, LC, LD)) return hipErrorInvalidValue;
   dim3 grid(grid_for(nq, cap, blk)), block(blk);
   if (nk == 1)
-    hipLaunchKernelGGL((eri_jk_kernel<LA, LB, LC, LD, real, 1>), grid, block, 0, s, q, sl, nq, pairs,
-                       (const real*)rv.bra, rv.sb, rv.fb, (const real*)rv.ket, ao, DJ, DK0, DK1,
-                       stage, bt);
+    hipLaunchKernelGGL((eri_jk_kernel<LA, LB, LC, LD, real, 1>), grid, block, 0, s, q, sl, rv.first, rv.n,
+                       rv.ntot, rv.splb, rv.splk, pairs, (const real*)rv.bra, rv.sb, rv.fb, (const real*)rv.ket,
+                       ao, DJ, DK0, DK1, stage, bt);
   else
-    hipLaunchKernelGGL((eri_jk_kernel<LA, LB, LC, LD, real, 2>), grid, block, 0, s, q, sl, nq, pairs,
-                       (const real*)rv.bra, rv.sb, rv.fb, (const real*)rv.ket, ao, DJ, DK0, DK1,
-                       stage, bt);
+    hipLaunchKernelGGL((eri_jk_kernel<LA, LB, LC, LD, real, 2>), grid, block, 0, s, q, sl, rv.first, rv.n,
+                       rv.ntot, rv.splb, rv.splk, pairs, (const real*)rv.bra, rv.sb, rv.fb, (const real*)rv.ket,
+                       ao, DJ, DK0, DK1, stage, bt);
   return hipGetLastError();
 }
 
@@ -237,17 +260,17 @@ bool lane_kernel_combo(int combo) {
   }
 }
 
-hipError_t launch_eri_jk(int combo, bool fp32, const int2* q, const long long* sl, long long nq,
+hipError_t launch_eri_jk(int combo, bool fp32, const int2* q, const long long* sl,
                          const int4* pairs, const RecView& rv, const int4* ao, const double* DJ, const double* DK0,
                          const double* DK1, int nk, double* stage, const double* bt, int block, int cap,
                          hipStream_t s) {
   switch (combo) {
 #define X(id, la, lb, lc, ld)                                                                          \
   case id:                                                                                             \
-    return fp32 ? jk_one<la, lb, lc, ld, float>(q, sl, nq, pairs, rv, ao, DJ, DK0, DK1, nk, stage, \
-                                                 bt, block, cap, s)                                   \
-                : jk_one<la, lb, lc, ld, double>(q, sl, nq, pairs, rv, ao, DJ, DK0, DK1, nk, stage, \
-                                                  bt, block, cap, s);
+    return fp32 ? jk_one<la, lb, lc, ld, float>(q, sl, pairs, rv, ao, DJ, DK0, DK1, nk, stage, bt, \
+                                                 block, cap, s)                                        \
+                : jk_one<la, lb, lc, ld, double>(q, sl, pairs, rv, ao, DJ, DK0, DK1, nk, stage, bt, \
+                                                  block, cap, s);
     DFT_FOR_COMBOS(X)
 #undef X
     default:

@@ -49,7 +49,7 @@ __global__ __launch_bounds__(256) void eri_grad_kernel(const int2* __restrict__ 
   constexpr int NH1 = nherm(LA + LB + 1);
   constexpr int RSB = grec_size(LA, LB), RSK = grec_size(LC, LD);
   constexpr int NEB = ne_axis(LA + 1, LB + 1), NEK = ne_axis(LC + 1, LD + 1);
-  __shared__ double btab[BOYS_NG * BOYS_TAYLOR];
+  __shared__ double btab[BOYS_LDS];
   boys_stage<L1>(boys_gtab, btab);
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += stride) {

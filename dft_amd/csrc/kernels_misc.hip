@@ -343,7 +343,7 @@ hipError_t launch_one_electron(const int2* sp, int npairs, const ShellDev* sh, c
 template <int L>
 __global__ void boys_kernel(const double* __restrict__ T, int n, double* __restrict__ out, int fp32,
                             const double* __restrict__ gtab) {
-  __shared__ double btab[BOYS_NG * BOYS_TAYLOR];
+  __shared__ double btab[BOYS_LDS];
   boys_stage<L>(gtab, btab);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;

@@ -86,21 +86,32 @@ __host__ __device__ constexpr int rec_size(int la, int lb) { return 5 + 3 * ne_a
 
 // ------------------------------------------------------------- Boys F_m
 // Table of F_m(T_k), T_k = k * BOYS_DT, k < BOYS_NG, m < BOYS_MT, generated on
-// the host in extended precision. A kernel of total momentum L stages
-// F_{L+j}(T_k)/j!, j < BOYS_TAYLOR, into LDS and evaluates
+// the host in extended precision. A kernel of total momentum L stages, per
+// grid point k, the row F_{L+j}(T_k)/j! (j < BOYS_TAYLOR) followed by
+// e^{-T_k} into LDS and evaluates
 //   F_L(T) = sum_j F_{L+j}(T_k)/j! (-d)^j,  d = T - T_k,  |d| <= DT/2
-// (dF_m/dT = -F_{m+1}), then recurses downward
+// (dF_m/dT = -F_{m+1}), e^{-T} = e^{-T_k} sum_j (-d)^j/j! (eight terms,
+// truncation below 1e-15 relative), then recurses downward
 //   F_m = (2T F_{m+1} + e^{-T}) / (2m+1).
 // For T >= BOYS_TMAX: F_0 = sqrt(pi/T)/2 (erfc(sqrt T) < 1e-18) and the upward
-// recursion F_{m+1} = ((2m+1) F_m - e^{-T}) / (2T), stable for m < T.
+// recursion F_{m+1} = ((2m+1) F_m - e^{-T}) / (2T), stable for m < T, with
+// the e^{-T} term dropped where it is negligible (boys_tnoexp).
 constexpr int BOYS_NG = 401;
 constexpr double BOYS_DT = 0.1;
 constexpr double BOYS_INV_DT = 10.0;
 constexpr double BOYS_TMAX = 40.0;
 constexpr int BOYS_TAYLOR = 8;
 constexpr int BOYS_MT = 32;  // m range held in the global table
+constexpr int BOYS_ROW = 10;  // LDS row: BOYS_TAYLOR coefficients, e^{-T_k}, pad (16-byte rows)
+constexpr int BOYS_LDS = BOYS_NG * BOYS_ROW;  // doubles of the staged table
 
 __host__ __device__ constexpr double inv_odd(int m) { return 1.0 / (2.0 * m + 1.0); }
+
+// Above BOYS_TMAX, F_m = Gamma(m+1/2) / (2 T^{m+1/2}) - O(e^{-T} / (2T)). The
+// e^{-T} term is dropped where it is below 1e-16 of F_L: from T = BOYS_TMAX
+// for L <= 1 and from BOYS_TMAX + 4 (L - 1) above (e.g. 68 for L = 8, where it
+// is needed up to T ~ 62); in between the upward recursion keeps it.
+__host__ __device__ constexpr double boys_tnoexp(int L) { return L <= 1 ? BOYS_TMAX : BOYS_TMAX + 4.0 * (L - 1); }
 
 // FP64 Boys function F_0..F_L from the LDS slice tab[k * BOYS_TAYLOR + j].
 template <int L>
@@ -108,25 +119,34 @@ __device__ __forceinline__ void boys_fp64(double T, const double* __restrict__ t
   if (T < BOYS_TMAX) {
     int k = __double2int_rn(T * BOYS_INV_DT);
     double d = -(T - k * BOYS_DT);
-    const double* c = tab + k * BOYS_TAYLOR;
+    const double* c = tab + k * BOYS_ROW;
     double f = c[BOYS_TAYLOR - 1];
 #pragma unroll
     for (int j = BOYS_TAYLOR - 2; j >= 0; --j) f = fma(f, d, c[j]);
     F[L] = f;
     if constexpr (L > 0) {
-      double e = exp(-T);
+      constexpr double inv_fact[8] = {1.0, 1.0, 0.5, 1.0 / 6, 1.0 / 24, 1.0 / 120, 1.0 / 720, 1.0 / 5040};
+      double pe = inv_fact[7];
+#pragma unroll
+      for (int j = 6; j >= 0; --j) pe = fma(pe, d, inv_fact[j]);
+      double e = c[BOYS_TAYLOR] * pe;
       double t2 = T + T;
 #pragma unroll
       for (int m = L - 1; m >= 0; --m) F[m] = fma(t2, F[m + 1], e) * inv_odd(m);
     }
   } else {
-    double rs = rsqrt(T);
+    const double rs = rsqrt_nr(T);
     F[0] = 0.88622692545275801365 * rs;  // sqrt(pi)/2 / sqrt(T)
     if constexpr (L > 0) {
-      double e = exp(-T);
-      double h = 0.5 / T;
+      const double h = 0.5 * rs * rs;  // 1 / (2T)
+      if (L <= 1 || T >= boys_tnoexp(L)) {
 #pragma unroll
-      for (int m = 0; m < L; ++m) F[m + 1] = ((2 * m + 1) * F[m] - e) * h;
+        for (int m = 0; m < L; ++m) F[m + 1] = (2 * m + 1) * F[m] * h;
+      } else {
+        const double e = exp(-T);
+#pragma unroll
+        for (int m = 0; m < L; ++m) F[m + 1] = ((2 * m + 1) * F[m] - e) * h;
+      }
     }
   }
 }
@@ -139,7 +159,7 @@ __device__ __forceinline__ void boys_fp32(float Tf, const double* __restrict__ t
     double T = (double)Tf;
     int k = __double2int_rn(T * BOYS_INV_DT);
     double d = -(T - k * BOYS_DT);
-    const double* c = tab + k * BOYS_TAYLOR;
+    const double* c = tab + k * BOYS_ROW;
     // five Taylor terms: truncation below 3e-10 relative, far under FP32 rounding
     double f = c[4];
 #pragma unroll
@@ -152,13 +172,13 @@ __device__ __forceinline__ void boys_fp32(float Tf, const double* __restrict__ t
       for (int m = L - 1; m >= 0; --m) F[m] = fmaf(t2, F[m + 1], e) * (float)inv_odd(m);
     }
   } else {
-    float rs = rsqrtf(Tf);
+    // e^{-T} < 5e-18 is below FP32 resolution of every F_m here
+    const float rs = rsqrt_nr(Tf);
     F[0] = 0.886226925f * rs;
     if constexpr (L > 0) {
-      float e = __expf(-Tf);
-      float h = 0.5f / Tf;
+      const float h = 0.5f * rs * rs;
 #pragma unroll
-      for (int m = 0; m < L; ++m) F[m + 1] = ((2 * m + 1) * F[m] - e) * h;
+      for (int m = 0; m < L; ++m) F[m + 1] = (2 * m + 1) * F[m] * h;
     }
   }
 }
@@ -169,13 +189,17 @@ __device__ __forceinline__ void boys_eval(real T, const double* __restrict__ tab
   else boys_fp32<L>(T, tab, F);
 }
 
-// Stage F_{L+j}(T_k)/j! for all k into LDS (cooperative, whole block).
+// Stage the rows F_{L+j}(T_k)/j!, e^{-T_k} for all k into LDS (cooperative,
+// whole block).
 template <int L>
 __device__ __forceinline__ void boys_stage(const double* __restrict__ gtab, double* __restrict__ lds) {
   constexpr double inv_fact[BOYS_TAYLOR] = {1.0, 1.0, 0.5, 1.0 / 6, 1.0 / 24, 1.0 / 120, 1.0 / 720, 1.0 / 5040};
-  for (int i = threadIdx.x; i < BOYS_NG * BOYS_TAYLOR; i += blockDim.x) {
-    int k = i / BOYS_TAYLOR, j = i - k * BOYS_TAYLOR;
-    lds[i] = gtab[k * BOYS_MT + L + j] * inv_fact[j];
+  for (int i = threadIdx.x; i < BOYS_LDS; i += blockDim.x) {
+    int k = i / BOYS_ROW, j = i - k * BOYS_ROW;
+    double v = 0.0;
+    if (j < BOYS_TAYLOR) v = gtab[k * BOYS_MT + L + j] * inv_fact[j];
+    else if (j == BOYS_TAYLOR) v = exp(-k * BOYS_DT);
+    lds[i] = v;
   }
   __syncthreads();
 }

@@ -22,16 +22,23 @@ def _ncart(l: int) -> int:
     return (l + 1) * (l + 2) // 2
 
 
+def _flags(uhf: bool, split: bool) -> int:
+    return (_lib.DFT_PLAN_UHF if uhf else 0) | (0 if split else _lib.DFT_PLAN_NOSPLIT)
+
+
 class Plan:
     """Integral plan of one molecule (pair data, Schwarz bounds, quartet lists).
 
     Parameters follow ``dft_plan_create``: ``schwarz_tau`` drops quartets with
     Q_ab Q_cd below it; ``fp32_tau`` marks quartets below it for FP32 in the
-    mixed mode; ``uhf`` builds K for two densities.
+    mixed mode; ``uhf`` builds K for two densities; ``split=False`` keeps
+    one lane per quartet in every class (no primitive split of classes with
+    few quartets), which makes a molecule's summation order independent of
+    the other molecules of a batch plan.
     """
 
     def __init__(self, shells, Z, centers, schwarz_tau=1e-10, fp32_tau=1e-4, uhf=False,
-                 device=None):
+                 device=None, split=True):
         self.lib = _lib.load()
         if not torch.cuda.is_available():
             raise _lib.DFTError("no HIP device visible: the integral engine runs on the GPU only")
@@ -70,8 +77,7 @@ class Plan:
                 _lib.ptr(seg_ao, C.c_int32), _lib.ptr(seg_xyz, C.c_double),
                 _lib.ptr(exps, C.c_double), _lib.ptr(coefs, C.c_double), len(Zc),
                 _lib.ptr(xyz, C.c_double), _lib.ptr(Zc, C.c_double), self.nbf,
-                float(schwarz_tau), float(fp32_tau), _lib.DFT_PLAN_UHF if uhf else 0,
-                C.byref(handle)))
+                float(schwarz_tau), float(fp32_tau), _flags(uhf, split), C.byref(handle)))
         self.handle = handle
         self.schwarz_tau, self.fp32_tau = schwarz_tau, fp32_tau
 
@@ -306,7 +312,7 @@ class BatchPlan(Plan):
     ``offsets[m]`` of a flat device vector; ``pack``/``unpack`` convert.
     """
 
-    def __init__(self, molecules, schwarz_tau=1e-10, fp32_tau=1e-4, uhf=False, device=None):
+    def __init__(self, molecules, schwarz_tau=1e-10, fp32_tau=1e-4, uhf=False, device=None, split=True):
         self.lib = _lib.load()
         if not torch.cuda.is_available():
             raise _lib.DFTError("no HIP device visible: the integral engine runs on the GPU only")
@@ -335,7 +341,7 @@ class BatchPlan(Plan):
                 _lib.ptr(mol_nbf, C.c_int32), _lib.ptr(l, C.c_int32), _lib.ptr(npr, C.c_int32),
                 _lib.ptr(ao, C.c_int32), _lib.ptr(xyz, C.c_double), _lib.ptr(ex, C.c_double),
                 _lib.ptr(co, C.c_double), _lib.ptr(axyz, C.c_double), _lib.ptr(Zc, C.c_double),
-                float(schwarz_tau), float(fp32_tau), _lib.DFT_PLAN_UHF if uhf else 0, C.byref(handle)))
+                float(schwarz_tau), float(fp32_tau), _flags(uhf, split), C.byref(handle)))
         self.handle = handle
         self.nbf = None
         self.schwarz_tau, self.fp32_tau = schwarz_tau, fp32_tau

@@ -33,7 +33,10 @@ extern "C" {
 #define DFT_EHIP -3
 
 /* plan flags */
-#define DFT_PLAN_UHF 1 /* K for two densities (alpha, beta) */
+#define DFT_PLAN_UHF 1     /* K for two densities (alpha, beta) */
+#define DFT_PLAN_NOSPLIT 2 /* one lane per quartet in every class (no primitive split
+                              of classes with few quartets; makes the summation
+                              order of a molecule the same in single and batch plans) */
 
 /* precision modes of dft_jk */
 #define DFT_FP64 0  /* every quartet in FP64 */
@@ -72,7 +75,7 @@ const char* dft_last_error(void);
  *  nbf                 number of Cartesian AOs
  *  schwarz_tau         quartets with Q_ab Q_cd < schwarz_tau are skipped
  *  fp32_tau            quartets with Q_ab Q_cd < fp32_tau form the FP32 part
- *  flags               DFT_PLAN_UHF or 0
+ *  flags               DFT_PLAN_UHF | DFT_PLAN_NOSPLIT or 0
  */
 int dft_plan_create(int nsh, const int32_t* shell_l, const int32_t* shell_nprim, const int32_t* shell_ao,
                     const double* shell_xyz, const double* prim_exp, const double* prim_coef, int natom,

@@ -42,14 +42,16 @@ def test_batch_jk_bitwise_equal_to_single(torch_cuda, uhf):
     torch = torch_cuda
     from dft_amd.engine import BatchPlan, Plan
     mols = _mols()
-    bp = BatchPlan(mols, schwarz_tau=1e-12, fp32_tau=0.0, uhf=uhf)
+    # no primitive split: the split of a class depends on its quartet count,
+    # which differs between the batch and the single plans
+    bp = BatchPlan(mols, schwarz_tau=1e-12, fp32_tau=0.0, uhf=uhf, split=False)
     rng = np.random.default_rng(0)
     Pa = [_dens(n, rng) for n in bp.nbfs]
     Pb = [_dens(n, rng) for n in bp.nbfs]
     Pt = bp.pack([a + b for a, b in zip(Pa, Pb)])
     J, Ka, Kb = bp.jk(Pt, bp.pack(Pa), bp.pack(Pb) if uhf else None)
     for m, (sh, Z, xyz) in enumerate(mols):
-        p = Plan(sh, Z, xyz, schwarz_tau=1e-12, fp32_tau=0.0, uhf=uhf)
+        p = Plan(sh, Z, xyz, schwarz_tau=1e-12, fp32_tau=0.0, uhf=uhf, split=False)
         j, ka, kb = p.jk(Pa[m] + Pb[m], Pa[m], Pb[m] if uhf else None)
         assert torch.equal(bp.unpack(J)[m], j)
         assert torch.equal(bp.unpack(Ka)[m], ka)
@@ -120,3 +122,24 @@ def test_scf_batch_matches_single(torch_cuda):
         single = scf_plain(Plan(sh, Z, xyz, schwarz_tau=1e-12), Z, xyz, nocc[m], nocc[m], e_tol=1e-10, p_tol=1e-7,
                            max_count=200)
         assert abs(res.energies[m] - single.energy) < 1e-8, (m, res.energies[m], single.energy)
+
+
+@pytest.mark.parametrize("uhf", [False, True])
+def test_primitive_split_matches_unsplit(torch_cuda, uhf):
+    """Classes with few quartets are split over primitive ranges (partial J/K
+    blocks summed by the reduction); the result equals the unsplit build to
+    rounding."""
+    torch = torch_cuda
+    from dft_amd.engine import Plan
+    for sh, Z, xyz in _mols():
+        a = Plan(sh, Z, xyz, schwarz_tau=1e-12, fp32_tau=0.0, uhf=uhf)
+        b = Plan(sh, Z, xyz, schwarz_tau=1e-12, fp32_tau=0.0, uhf=uhf, split=False)
+        rng = np.random.default_rng(1)
+        n = a.nbf
+        Pa, Pb = _dens(n, rng), _dens(n, rng)
+        ra = a.jk(Pa + Pb, Pa, Pb if uhf else None)
+        rb = b.jk(Pa + Pb, Pa, Pb if uhf else None)
+        for x, y in zip(ra, rb):
+            if x is None:
+                continue
+            assert float((x - y).abs().max()) < 1e-12
