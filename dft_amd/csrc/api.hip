@@ -23,6 +23,8 @@ namespace {
 
 // lanes below which a lane-kernel class combination is split over primitives
 constexpr long long SPLIT_LANES = 65536;
+// auxiliary streams per plan for concurrent class launches
+constexpr int JK_AUX = 3;
 
 }  // namespace
 
@@ -176,6 +178,11 @@ struct dft_plan_s {
   // FP32 launch, then reduce, symmetrize; one begin/end event pair per slot and step
   int prof_steps = 0, prof_used = 0;
   std::vector<hipEvent_t> prof_ev;
+  // class launches of one Fock build are spread over the caller's stream and
+  // JK_AUX more streams (fork/join by events; all default priority)
+  std::vector<hipStream_t> aux;
+  std::vector<hipEvent_t> ev_join;
+  hipEvent_t ev_fork = nullptr;
   // gradient data, built by the first dft_grad call (single molecule)
   bool grad_ready = false;
   std::vector<int> grad_atom;     // internal shell -> atom
@@ -205,6 +212,9 @@ void free_plan(dft_plan p) {
                   p->d_recT32};
   for (void* x : ptrs) (void)hipFree(x);
   for (hipEvent_t e : p->prof_ev) (void)hipEventDestroy(e);
+  for (hipStream_t x : p->aux) (void)hipStreamDestroy(x);
+  for (hipEvent_t e : p->ev_join) (void)hipEventDestroy(e);
+  if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
   for (auto* x : p->d_gq_swap) (void)hipFree(x);
   for (auto* x : p->d_gslots_ab) (void)hipFree(x);
   for (auto* x : p->d_gslots_cd) (void)hipFree(x);
@@ -618,43 +628,84 @@ hipError_t launch_combo(dft_plan p, size_t ci, bool f32, long long first, long l
                          p->d_shell_dm, P, Pa, Pb, p->nk, p->d_stage, p->d_boys, sh.cap, s);
 }
 
-// The whole Fock build: class launches, reduction, symmetrisation.
+// The whole Fock build: class launches, reduction, symmetrisation. The class
+// launches write disjoint staging slots, so they run concurrently: each goes
+// to the least-loaded of the caller's stream and the plan's auxiliary
+// streams (greedy by estimated cost, largest first), which fork from and
+// join back into the caller's stream by events (also under graph capture).
 int jk_enqueue(dft_plan p, const double* P, const double* Pa, const double* Pb, double* J, double* Ka, double* Kb,
                int mode, hipStream_t s, bool profile) {
   const int nslot = 2 * (int)p->combos.size() + 2;
   hipEvent_t* ev = nullptr;
   if (profile && p->prof_used < p->prof_steps) ev = &p->prof_ev[(size_t)p->prof_used++ * nslot * 2];
-  auto mark = [&](int slot, int end) {
-    if (ev) (void)hipEventRecord(ev[2 * slot + end], s);
+  auto mark = [&](int slot, int end, hipStream_t st) {
+    if (ev) (void)hipEventRecord(ev[2 * slot + end], st);
   };
+  struct Launch {
+    int ci;
+    bool f32;
+    long long first, n;
+    double cost;
+  };
+  std::vector<Launch> ls;
   for (size_t ci = 0; ci < p->combos.size(); ++ci) {
     const Combo& c = p->combos[ci];
     const long long ntot = (long long)c.q.size();
+    if (ntot == 0) continue;
     // FP64 mode: one launch over the whole list; mixed: FP64 part, then the
     // weak part through the FP32 kernel
     const long long n1 = mode == DFT_MIXED ? c.n64 : ntot;
-    mark(2 * (int)ci, 0);
-    hipError_t e = launch_combo(p, ci, false, 0, n1, c.shape64, P, Pa, Pb, s);
-    if (e != hipSuccess) return fail(DFT_EHIP, std::string("eri_jk fp64: ") + hipGetErrorString(e));
-    mark(2 * (int)ci, 1);
-    mark(2 * (int)ci + 1, 0);
-    if (mode == DFT_MIXED) {
-      e = launch_combo(p, ci, true, c.n64, ntot - c.n64, c.shape32, P, Pa, Pb, s);
-      if (e != hipSuccess) return fail(DFT_EHIP, std::string("eri_jk fp32: ") + hipGetErrorString(e));
-    }
-    mark(2 * (int)ci + 1, 1);
+    int la, lb, lc, ld;
+    pc_moments(c.pcb, la, lb);
+    pc_moments(c.pck, lc, ld);
+    const double w = 16.0 + ncart_h(la) * ncart_h(lb) * ncart_h(lc) * ncart_h(ld);  // per primitive quartet
+    if (n1 > 0) ls.push_back({(int)ci, false, 0, n1, w * (mode == DFT_MIXED ? c.prim64 : c.prim64 + c.prim32)});
+    if (mode == DFT_MIXED && ntot > c.n64) ls.push_back({(int)ci, true, c.n64, ntot - c.n64, 0.7 * w * c.prim32});
   }
-  mark(nslot - 2, 0);
+  std::stable_sort(ls.begin(), ls.end(), [](const Launch& a, const Launch& b) { return a.cost > b.cost; });
+  // slots without a launch read as zero
+  for (int slot = 0; slot < nslot - 2; ++slot) {
+    mark(slot, 0, s);
+    mark(slot, 1, s);
+  }
+  std::vector<hipStream_t> st(1, s);
+  for (hipStream_t a : p->aux) st.push_back(a);
+  const bool fork = ls.size() > 1 && st.size() > 1;
+  if (fork) {
+    HIP_TRY(hipEventRecord(p->ev_fork, s));
+    for (size_t k = 1; k < st.size(); ++k) HIP_TRY(hipStreamWaitEvent(st[k], p->ev_fork, 0));
+  }
+  std::vector<double> load(st.size(), 0.0);
+  for (const Launch& L : ls) {
+    size_t k = 0;
+    if (fork)
+      for (size_t x = 1; x < st.size(); ++x)
+        if (load[x] < load[k]) k = x;
+    load[k] += L.cost;
+    const Combo& c = p->combos[L.ci];
+    const int slot = 2 * L.ci + (L.f32 ? 1 : 0);
+    mark(slot, 0, st[k]);
+    hipError_t e = launch_combo(p, L.ci, L.f32, L.first, L.n, L.f32 ? c.shape32 : c.shape64, P, Pa, Pb, st[k]);
+    if (e != hipSuccess)
+      return fail(DFT_EHIP, std::string(L.f32 ? "eri_jk fp32: " : "eri_jk fp64: ") + hipGetErrorString(e));
+    mark(slot, 1, st[k]);
+  }
+  if (fork)
+    for (size_t k = 1; k < st.size(); ++k) {
+      HIP_TRY(hipEventRecord(p->ev_join[k - 1], st[k]));
+      HIP_TRY(hipStreamWaitEvent(s, p->ev_join[k - 1], 0));
+    }
+  mark(nslot - 2, 0, s);
   hipError_t e = launch_reduce(p->ntarget, p->d_tg, p->d_tbase, p->d_tcount, p->d_stage, p->d_half, s);
   if (e != hipSuccess) return fail(DFT_EHIP, std::string("reduce: ") + hipGetErrorString(e));
-  mark(nslot - 2, 1);
-  mark(nslot - 1, 0);
+  mark(nslot - 2, 1, s);
+  mark(nslot - 1, 0, s);
   double* outs[3] = {J, Ka, Kb};
   for (int m = 0; m < 1 + p->nk; ++m) {
     e = launch_symmetrize(p->nmol, p->d_mol_base, p->d_mol_nbf, p->max_nbf, p->d_half + m * p->nelem, outs[m], s);
     if (e != hipSuccess) return fail(DFT_EHIP, std::string("symmetrize: ") + hipGetErrorString(e));
   }
-  mark(nslot - 1, 1);
+  mark(nslot - 1, 1, s);
   return DFT_OK;
 }
 
@@ -780,6 +831,20 @@ int grad_prepare(dft_plan p, const int32_t* shell_atom) {
   return DFT_OK;
 }
 
+// auxiliary streams and fork/join events on the current device
+int create_streams(dft_plan p) {
+  HIP_TRY(hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming));
+  for (int k = 0; k < JK_AUX; ++k) {
+    hipStream_t x;
+    HIP_TRY(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
+    p->aux.push_back(x);
+    hipEvent_t e;
+    HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    p->ev_join.push_back(e);
+  }
+  return DFT_OK;
+}
+
 int validate_and_build(int nmol, const int32_t* mol_nsh, const int32_t* mol_natom, const int32_t* mol_nbf,
                        const int32_t* shell_l, const int32_t* shell_nprim, const int32_t* shell_ao,
                        const double* shell_xyz, const double* prim_exp, const double* prim_coef, const double* atom_xyz,
@@ -830,6 +895,7 @@ int validate_and_build(int nmol, const int32_t* mol_nsh, const int32_t* mol_nato
   p->tau = schwarz_tau;
   p->tau32 = fp32_tau;
   int rc = build_plan(p, in);
+  if (!rc) rc = create_streams(p);
   if (rc) {
     free_plan(p);
     return rc;
