@@ -49,6 +49,23 @@ def screened_quartets(Q, tau):
     return np.stack([ii[bra], jj[bra], ii[ket], jj[ket]], axis=1).astype(np.int32)
 
 
+TRAFFIC = os.path.join(ROOT, "profiles", "eri_jk_h2o16_sto3g_traffic.json")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of ``kernel`` from the committed counter summary
+    (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this benchmark, each
+    scaled by the same counter's reading for a device copy of known size),
+    or None when that summary has no entry for it."""
+    try:
+        with open(TRAFFIC) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = t.get("kernels", {}).get(kernel)
+    return None if k is None else k["fetch_bytes"] + k["write_bytes"]
+
+
 def cpu_baseline(shells, Q, tau, P, seconds, n_quartets_gpu):
     import oracle.ref as ref
     seg = ref.Segments(shells)
@@ -187,10 +204,11 @@ def main():
     f_all = sum(build_flops(per))          # whole list in FP64 in this mode
     f_all_eval = sum(build_flops(per, model=False))
     from dft_amd.flops import combo_moments
+    kname = "eri_jk_kernel<%d, %d, %d, %d, double, 1>" % combo_moments(combo)
     roof = {"bound": "mfma", "pipe": "fp64 VALU (MFMA FP64 peak is the same 78.6 TFLOP/s)",
-            "kernel": "eri_jk_kernel<%d,%d,%d,%d,double>" % combo_moments(combo),
+            "kernel": kname,
             "achieved": achieved, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
-            "frac": achieved * 1e12 / FP64_PEAK, "traffic": None,
+            "frac": achieved * 1e12 / FP64_PEAK, "traffic": measured_traffic(kname),
             "avg_launch_ms": eri_ms[dom] / nrec, "flops_per_launch": f_dom,
             "flops_per_launch_evaluated": f_dom_eval,
             "whole_build_tflops": f_all / (ms_per_step * 1e-3) / 1e12,
@@ -208,6 +226,9 @@ def main():
                    "n_quartets": quartets,
                    "n_unscreened": st["n_unscreened"], "n_prim_quartets": st["n_prim_quartets"],
                    "parallelism": "single GPU"},
+        "per_combo": {"columns": ["fp64 quartets", "fp32-eligible quartets", "fp64 prim quartets",
+                                  "fp32 prim quartets", "fp64 K-products", "fp32 K-products"],
+                      "rows": [list(map(int, r)) for r in per]},
         "s_per_iter_fock_fp64": ms_per_step * 1e-3,
         "shell_quartets_per_s": quartets / (ms_per_step * 1e-3),
         "prim_shell_quartets_per_s": st["n_prim_quartets"] / (ms_per_step * 1e-3),

@@ -14,6 +14,7 @@ LIB_PATH = os.environ.get("DFT_HIP_LIB", os.path.join(_HERE, "libdft_hip.so"))
 DFT_OK, DFT_EINVAL, DFT_EUNSUPPORTED, DFT_EHIP = 0, -1, -2, -3
 DFT_PLAN_UHF = 1
 DFT_PLAN_NOSPLIT = 2
+DFT_PLAN_SERIAL = 4
 DFT_FP64, DFT_MIXED = 0, 1
 MODES = {"fp64": DFT_FP64, "mixed": DFT_MIXED}
 

@@ -895,7 +895,7 @@ int validate_and_build(int nmol, const int32_t* mol_nsh, const int32_t* mol_nato
   p->tau = schwarz_tau;
   p->tau32 = fp32_tau;
   int rc = build_plan(p, in);
-  if (!rc) rc = create_streams(p);
+  if (!rc && !(flags & DFT_PLAN_SERIAL)) rc = create_streams(p);
   if (rc) {
     free_plan(p);
     return rc;

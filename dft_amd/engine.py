@@ -22,8 +22,9 @@ def _ncart(l: int) -> int:
     return (l + 1) * (l + 2) // 2
 
 
-def _flags(uhf: bool, split: bool) -> int:
-    return (_lib.DFT_PLAN_UHF if uhf else 0) | (0 if split else _lib.DFT_PLAN_NOSPLIT)
+def _flags(uhf: bool, split: bool, serial: bool = False) -> int:
+    return ((_lib.DFT_PLAN_UHF if uhf else 0) | (0 if split else _lib.DFT_PLAN_NOSPLIT)
+            | (_lib.DFT_PLAN_SERIAL if serial else 0))
 
 
 class Plan:
@@ -34,11 +35,12 @@ class Plan:
     mixed mode; ``uhf`` builds K for two densities; ``split=False`` keeps
     one lane per quartet in every class (no primitive split of classes with
     few quartets), which makes a molecule's summation order independent of
-    the other molecules of a batch plan.
+    the other molecules of a batch plan; ``serial=True`` issues all class
+    launches on the caller's stream (per-kernel profiling).
     """
 
     def __init__(self, shells, Z, centers, schwarz_tau=1e-10, fp32_tau=1e-4, uhf=False,
-                 device=None, split=True):
+                 device=None, split=True, serial=False):
         self.lib = _lib.load()
         if not torch.cuda.is_available():
             raise _lib.DFTError("no HIP device visible: the integral engine runs on the GPU only")
@@ -77,7 +79,7 @@ class Plan:
                 _lib.ptr(seg_ao, C.c_int32), _lib.ptr(seg_xyz, C.c_double),
                 _lib.ptr(exps, C.c_double), _lib.ptr(coefs, C.c_double), len(Zc),
                 _lib.ptr(xyz, C.c_double), _lib.ptr(Zc, C.c_double), self.nbf,
-                float(schwarz_tau), float(fp32_tau), _flags(uhf, split), C.byref(handle)))
+                float(schwarz_tau), float(fp32_tau), _flags(uhf, split, serial), C.byref(handle)))
         self.handle = handle
         self.schwarz_tau, self.fp32_tau = schwarz_tau, fp32_tau
 
@@ -312,7 +314,8 @@ class BatchPlan(Plan):
     ``offsets[m]`` of a flat device vector; ``pack``/``unpack`` convert.
     """
 
-    def __init__(self, molecules, schwarz_tau=1e-10, fp32_tau=1e-4, uhf=False, device=None, split=True):
+    def __init__(self, molecules, schwarz_tau=1e-10, fp32_tau=1e-4, uhf=False, device=None, split=True,
+                 serial=False):
         self.lib = _lib.load()
         if not torch.cuda.is_available():
             raise _lib.DFTError("no HIP device visible: the integral engine runs on the GPU only")
@@ -341,7 +344,7 @@ class BatchPlan(Plan):
                 _lib.ptr(mol_nbf, C.c_int32), _lib.ptr(l, C.c_int32), _lib.ptr(npr, C.c_int32),
                 _lib.ptr(ao, C.c_int32), _lib.ptr(xyz, C.c_double), _lib.ptr(ex, C.c_double),
                 _lib.ptr(co, C.c_double), _lib.ptr(axyz, C.c_double), _lib.ptr(Zc, C.c_double),
-                float(schwarz_tau), float(fp32_tau), _flags(uhf, split), C.byref(handle)))
+                float(schwarz_tau), float(fp32_tau), _flags(uhf, split, serial), C.byref(handle)))
         self.handle = handle
         self.nbf = None
         self.schwarz_tau, self.fp32_tau = schwarz_tau, fp32_tau

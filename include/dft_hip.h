@@ -37,6 +37,8 @@ extern "C" {
 #define DFT_PLAN_NOSPLIT 2 /* one lane per quartet in every class (no primitive split
                               of classes with few quartets; makes the summation
                               order of a molecule the same in single and batch plans) */
+#define DFT_PLAN_SERIAL 4  /* all class launches of a Fock build on the caller's stream
+                              (no concurrent launches; for per-kernel profiling) */
 
 /* precision modes of dft_jk */
 #define DFT_FP64 0  /* every quartet in FP64 */
@@ -75,7 +77,7 @@ const char* dft_last_error(void);
  *  nbf                 number of Cartesian AOs
  *  schwarz_tau         quartets with Q_ab Q_cd < schwarz_tau are skipped
  *  fp32_tau            quartets with Q_ab Q_cd < fp32_tau form the FP32 part
- *  flags               DFT_PLAN_UHF | DFT_PLAN_NOSPLIT or 0
+ *  flags               DFT_PLAN_UHF | DFT_PLAN_NOSPLIT | DFT_PLAN_SERIAL or 0
  */
 int dft_plan_create(int nsh, const int32_t* shell_l, const int32_t* shell_nprim, const int32_t* shell_ao,
                     const double* shell_xyz, const double* prim_exp, const double* prim_coef, int natom,
@@ -117,13 +119,14 @@ int dft_graph_destroy(dft_graph graph);
 /*
  * Autotune the launch shape of every class-combination launch of the plan:
  * for the FP64 launch (whole list) and the FP32 launch (weak part) it times
- * the lane-per-quartet kernel at workgroup sizes 64/128/256 and grid caps
- * 512..8192, and the wave-per-quartet kernel at three grid caps (median of
- * reps runs each) and keeps the fastest. report (capacity report_cap ints)
+ * the lane-per-quartet kernel at workgroup sizes 64/128/256/512 (up to the
+ * class's limit) and grid caps 512..8192, and the wave-per-quartet kernel at
+ * three grid caps (median of reps runs each) and keeps the fastest. report (capacity report_cap ints)
  * receives 6 ints per combination: FP64 {wave, block, cap}, FP32 {wave,
  * block, cap}. combo >= 0 tunes only that combination (-1: all). The wave
- * kernel is a candidate for (pp|pp) and the d classes only. Overwrites the
- * plan's staging (J/K are not written).
+ * kernel is a candidate for (pp|pp) and the d classes only, and not for a
+ * class split over primitive chunks. Overwrites the plan's staging (J/K are
+ * not written).
  * No counterpart in the reference.
  */
 int dft_plan_tune(dft_plan plan, const double* P, const double* Pa, const double* Pb, int reps, int combo,
@@ -156,7 +159,10 @@ int dft_schwarz(dft_plan plan, double* Q);
 /*
  * Direct J/K build: J = sum_cd (ab|cd) P_cd, K_a = sum_bd (ab|cd) Pa_bd and,
  * for a UHF plan, K_b from Pb. Fused ERI evaluation + digestion, then a
- * deterministic reduction. Overwrites J, Ka, Kb (no accumulation).
+ * deterministic reduction. Overwrites J, Ka, Kb (no accumulation). The
+ * class launches run concurrently on the caller's stream and the plan's
+ * auxiliary streams, joined back into the caller's stream before the
+ * reduction; calls on one plan must not overlap (they share the staging).
  *
  * Replaces fill_matrices(shells, P, P_a, P_b, J, K_a, K_b)
  * (shelled.py:770-877) together with the ERI store it reads
