@@ -105,7 +105,7 @@ constexpr double PRIM_CUT = 39.1439465808987;  // -ln(1e-17)
 
 // launch shape of one class-combination launch (chosen by dft_plan_tune)
 struct Shape {
-  int wave = 0;         // 0: lane-per-quartet kernel, 1: wave-per-quartet kernel
+  int wave = 0;         // reserved (0)
   int block = 256;      // lane kernel workgroup size
   int cap = 256 * 16;   // maximum workgroups (persistent grid-stride loop)
 };
@@ -124,6 +124,9 @@ struct Combo {
   // the primitive pairs and writing its own partial J/K blocks; chunk-major
   // slot order (slot row = (cb * splk + ck) * q.size() + quartet)
   int splb = 1, splk = 1;
+  // heavy (column-kernel) combinations: integrals of the whole list,
+  // q.size() x NI doubles, read by the digest kernel
+  double* d_ibuf = nullptr;
 };
 
 }  // namespace
@@ -204,6 +207,7 @@ void free_plan(dft_plan p) {
   for (auto& c : p->combos) {
     (void)hipFree(c.d_q);
     (void)hipFree(c.d_slots);
+    (void)hipFree(c.d_ibuf);
   }
   void* ptrs[] = {p->d_sh,    p->d_exps,  p->d_coefs,   p->d_pairs,    p->d_shell_dm,  p->d_shell_ao,
                   p->d_rec64, p->d_rec32, p->d_stage,   p->d_tg,       p->d_tbase,     p->d_tcount,
@@ -225,6 +229,8 @@ long long pair_key(int i, int j) { return ((long long)i << 32) | (unsigned)j; }
 
 // Raw contracted blocks for a list of (bra pair, ket pair) in one combo,
 // synchronous, host result.
+RecView rec_view(dft_plan p, int ci, bool f32);
+
 int raw_blocks(dft_plan p, int combo, const std::vector<int2>& q, std::vector<double>& out, int blk) {
   out.assign(q.size() * (size_t)blk, 0.0);
   if (q.empty()) return DFT_OK;
@@ -240,11 +246,9 @@ int raw_blocks(dft_plan p, int combo, const std::vector<int2>& q, std::vector<do
   if (lane_kernel_combo(combo)) {
     e = launch_eri_raw(combo, dq, (long long)q.size(), p->d_pairs, p->d_rec64, dout, p->d_boys, nullptr);
   } else {
-    int la, lb, lc, ld;
-    pc_moments(combo_pcb(combo), la, lb);
-    pc_moments(combo_pck(combo), lc, ld);
-    e = launch_eri_wave(la, lb, lc, ld, false, true, dq, nullptr, (long long)q.size(), p->d_pairs, p->d_rec64,
-                        nullptr, nullptr, nullptr, nullptr, 1, dout, p->d_boys, 1 << 20, nullptr);
+    const RecView rv = rec_view(p, combo, false);
+    e = launch_eri_chunk(combo, false, dq, 0, (int)q.size(), p->d_pairs, rv, dout, 1, blk, p->d_boys, 256, 1 << 20,
+                         nullptr);
   }
   if (e == hipSuccess) e = hipMemcpy(out.data(), dout, out.size() * sizeof(double), hipMemcpyDeviceToHost);
   (void)hipFree(dq);
@@ -468,7 +472,6 @@ int build_plan(dft_plan p, const Input& in) {
       Combo& c = p->combos[combo_index(pcb, pck)];
       c.pcb = pcb;
       c.pck = pck;
-      if (!lane_kernel_combo(combo_index(pcb, pck))) c.shape64.wave = c.shape32.wave = 1;
       std::vector<int2> weak;
       // ket-major: consecutive quartets (the lanes of a wave) share the ket
       // pair, whose primitive records the lane kernel reads in its inner loop,
@@ -509,9 +512,17 @@ int build_plan(dft_plan p, const Input& in) {
         const long long want = (SPLIT_LANES + nq - 1) / nq;
         c.splb = (int)std::max<long long>(1, std::min<long long>(want, nbmax));
         c.splk = (int)std::max<long long>(1, std::min<long long>((want + c.splb - 1) / c.splb, nkmax));
-        c.shape64.wave = c.shape32.wave = 0;
       }
       if (nq * c.splb * c.splk > 0x7fffffffLL) return fail(DFT_EINVAL, "class combination exceeds 2^31 lanes");
+      if (!lane_kernel_combo(combo_index(pcb, pck)) && nq > 0) {
+        int la, lb, lc, ld;
+        pc_moments(pcb, la, lb);
+        pc_moments(pck, lc, ld);
+        const long long ni = (long long)ncart_h(la) * ncart_h(lb) * ncart_h(lc) * ncart_h(ld);
+        if (nq * ncart_h(lc) * ncart_h(ld) > 0x7fffffffLL)
+          return fail(DFT_EINVAL, "class combination exceeds 2^31 lanes");
+        HIP_TRY(hipMalloc(&c.d_ibuf, nq * ni * sizeof(double)));
+      }
       if ((rc = upload(&c.d_q, c.q))) return rc;
     }
   // ---- target-major staging: a target is an ordered shell pair (X, Y) of
@@ -597,35 +608,64 @@ int build_plan(dft_plan p, const Input& in) {
   return DFT_OK;
 }
 
+// Largest workgroup of a combination's kernel (lane kernel: per class;
+// column kernel: 256, i.e. up to 512 VGPRs per lane).
+int block_max(int ci) { return lane_kernel_combo(ci) ? lane_block_max(ci) : 256; }
+
+// Record view of a combination's lane-kernel / column-kernel launch.
+// (Also used for the Schwarz diagonals, before the quartet lists exist: the
+// bra class comes from the combination index, the list fields only when
+// the combination is built.)
+RecView rec_view(dft_plan p, int ci, bool f32) {
+  const int pcb = combo_pcb(ci);
+  RecView rv;
+  const size_t esz = f32 ? sizeof(float) : sizeof(double);
+  const char* t = f32 ? (const char*)p->d_recT32 : (const char*)p->d_recT64;
+  rv.bra = t + p->cls_base[pcb] * esz;
+  rv.sb = p->cls_n[pcb];
+  rv.fb = p->cls_first[pcb];
+  rv.ket = f32 ? (const void*)p->d_rec32 : (const void*)p->d_rec64;
+  rv.first = 0;
+  rv.n = 0;
+  rv.ntot = 0;
+  rv.splb = rv.splk = 1;
+  if (ci < (int)p->combos.size()) {
+    const Combo& c = p->combos[ci];
+    rv.ntot = (int)c.q.size();
+    rv.splb = c.splb;
+    rv.splk = c.splk;
+  }
+  return rv;
+}
+
 // One class-combination launch of the FP64 or FP32 kernel on quartets
-// [first, first + n) with the given launch shape.
+// [first, first + n) with the given launch shape. Lane-kernel combinations
+// write their J/K blocks directly; heavy combinations write integrals to
+// the combination's buffer (launch_digest turns them into J/K blocks).
 hipError_t launch_combo(dft_plan p, size_t ci, bool f32, long long first, long long n, const Shape& sh,
                         const double* P, const double* Pa, const double* Pb, hipStream_t s) {
   const Combo& c = p->combos[ci];
-  const int nblk = 2 + 4 * p->nk;
-  if (sh.wave && c.splb * c.splk != 1) return hipErrorInvalidValue;  // split classes run the lane kernel
-  const void* rec = f32 ? (const void*)p->d_rec32 : (const void*)p->d_rec64;
-  if (!sh.wave && lane_kernel_combo((int)ci)) {
-    RecView rv;
-    const size_t esz = f32 ? sizeof(float) : sizeof(double);
-    const char* t = f32 ? (const char*)p->d_recT32 : (const char*)p->d_recT64;
-    rv.bra = t + p->cls_base[c.pcb] * esz;
-    rv.sb = p->cls_n[c.pcb];
-    rv.fb = p->cls_first[c.pcb];
-    rv.ket = rec;
-    rv.splb = c.splb;
-    rv.splk = c.splk;
-    rv.first = (int)first;
-    rv.n = (int)n;
-    rv.ntot = (int)c.q.size();
+  RecView rv = rec_view(p, (int)ci, f32);
+  rv.first = (int)first;
+  rv.n = (int)n;
+  if (lane_kernel_combo((int)ci))
     return launch_eri_jk((int)ci, f32, c.d_q, c.d_slots, p->d_pairs, rv, p->d_shell_dm, P, Pa, Pb, p->nk,
                          p->d_stage, p->d_boys, sh.block, sh.cap, s);
-  }
   int la, lb, lc, ld;
   pc_moments(c.pcb, la, lb);
   pc_moments(c.pck, lc, ld);
-  return launch_eri_wave(la, lb, lc, ld, f32, false, c.d_q + first, c.d_slots + first * nblk, n, p->d_pairs, rec,
-                         p->d_shell_dm, P, Pa, Pb, p->nk, p->d_stage, p->d_boys, sh.cap, s);
+  const long long ni = (long long)ncart_h(la) * ncart_h(lb) * ncart_h(lc) * ncart_h(ld);
+  return launch_eri_chunk((int)ci, f32, c.d_q, (int)first, (int)n, p->d_pairs, rv, c.d_ibuf, 1, ni, p->d_boys,
+                          sh.block, sh.cap, s);
+}
+
+// J/K blocks of a heavy combination from its integral buffer (whole list).
+hipError_t launch_digest(dft_plan p, size_t ci, const double* P, const double* Pa, const double* Pb,
+                         hipStream_t s) {
+  const Combo& c = p->combos[ci];
+  if (lane_kernel_combo((int)ci) || c.q.empty()) return hipSuccess;
+  return launch_jk_digest((int)ci, c.d_q, c.d_slots, (int)c.q.size(), p->d_pairs, p->d_shell_dm, P, Pa, Pb, p->nk,
+                          c.d_ibuf, p->d_stage, s);
 }
 
 // The whole Fock build: class launches, reduction, symmetrisation. The class
@@ -659,6 +699,11 @@ int jk_enqueue(dft_plan p, const double* P, const double* Pa, const double* Pb, 
     pc_moments(c.pcb, la, lb);
     pc_moments(c.pck, lc, ld);
     const double w = 16.0 + ncart_h(la) * ncart_h(lb) * ncart_h(lc) * ncart_h(ld);  // per primitive quartet
+    if (!lane_kernel_combo((int)ci)) {
+      // heavy combination: integral launches and digest in order on one stream
+      ls.push_back({(int)ci, false, 0, ntot, w * (c.prim64 + c.prim32)});
+      continue;
+    }
     if (n1 > 0) ls.push_back({(int)ci, false, 0, n1, w * (mode == DFT_MIXED ? c.prim64 : c.prim64 + c.prim32)});
     if (mode == DFT_MIXED && ntot > c.n64) ls.push_back({(int)ci, true, c.n64, ntot - c.n64, 0.7 * w * c.prim32});
   }
@@ -683,6 +728,19 @@ int jk_enqueue(dft_plan p, const double* P, const double* Pa, const double* Pb, 
         if (load[x] < load[k]) k = x;
     load[k] += L.cost;
     const Combo& c = p->combos[L.ci];
+    if (!lane_kernel_combo(L.ci)) {
+      const long long n1 = mode == DFT_MIXED ? c.n64 : (long long)c.q.size();
+      mark(2 * L.ci, 0, st[k]);
+      hipError_t e = launch_combo(p, L.ci, false, 0, n1, c.shape64, P, Pa, Pb, st[k]);
+      mark(2 * L.ci, 1, st[k]);
+      mark(2 * L.ci + 1, 0, st[k]);
+      if (e == hipSuccess && mode == DFT_MIXED)
+        e = launch_combo(p, L.ci, true, c.n64, (long long)c.q.size() - c.n64, c.shape32, P, Pa, Pb, st[k]);
+      if (e == hipSuccess) e = launch_digest(p, L.ci, P, Pa, Pb, st[k]);
+      mark(2 * L.ci + 1, 1, st[k]);
+      if (e != hipSuccess) return fail(DFT_EHIP, std::string("eri column / digest: ") + hipGetErrorString(e));
+      continue;
+    }
     const int slot = 2 * L.ci + (L.f32 ? 1 : 0);
     mark(slot, 0, st[k]);
     hipError_t e = launch_combo(p, L.ci, L.f32, L.first, L.n, L.f32 ? c.shape32 : c.shape64, P, Pa, Pb, st[k]);
@@ -1022,8 +1080,8 @@ int dft_plan_tune(dft_plan p, const double* P, const double* Pa, const double* P
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
-  // candidate launch shapes: lane kernel (block x grid cap) and wave kernel
-  std::vector<Shape> lane_c, wave_c;
+  // candidate launch shapes: workgroup size x grid cap (lane and column kernels)
+  std::vector<Shape> lane_c;
   for (int b : {64, 128, 256, 512})
     for (int cap : {256 * 2, 256 * 4, 256 * 8, 256 * 16, 256 * 32}) {
       Shape sh;
@@ -1032,12 +1090,6 @@ int dft_plan_tune(dft_plan p, const double* P, const double* Pa, const double* P
       sh.cap = cap;
       lane_c.push_back(sh);
     }
-  for (int cap : {256 * 4, 256 * 16, 256 * 64}) {
-    Shape sh;
-    sh.wave = 1;
-    sh.cap = cap;
-    wave_c.push_back(sh);
-  }
   int rc = DFT_OK;
   for (size_t ci = 0; ci < p->combos.size() && rc == DFT_OK; ++ci) {
     if (combo >= 0 && (int)ci != combo) continue;
@@ -1048,14 +1100,9 @@ int dft_plan_tune(dft_plan p, const double* P, const double* Pa, const double* P
       // part, by the mixed mode); part 1: weak part in FP32
       const long long first = part ? c.n64 : 0, n = part ? ntot - c.n64 : ntot;
       if (n <= 0) continue;
-      // the wave kernel only competes where a lane holds many integrals
-      // ((pp|pp) and the d classes); for the light classes it is idle-heavy
       std::vector<Shape> cand;
-      if (lane_kernel_combo((int)ci))
-        for (const Shape& sh : lane_c)
-          if (sh.block <= lane_block_max((int)ci)) cand.push_back(sh);
-      if ((!lane_kernel_combo((int)ci) || ci == 5 || c.pcb >= 3) && c.splb * c.splk == 1)
-        cand.insert(cand.end(), wave_c.begin(), wave_c.end());
+      for (const Shape& sh : lane_c)
+        if (sh.block <= block_max((int)ci)) cand.push_back(sh);
       float best = 1e30f;
       Shape best_sh = part ? c.shape32 : c.shape64;
       for (const Shape& sh : cand) {
@@ -1116,13 +1163,10 @@ int dft_plan_set_shapes(dft_plan p, const int* shapes, int n) {
     s32.block = shapes[6 * ci + 4];
     s32.cap = shapes[6 * ci + 5];
     for (const Shape* s : {&s64, &s32}) {
-      if (!s->wave && !lane_kernel_combo((int)ci)) return fail(DFT_EINVAL, "combination has no lane kernel");
-      if (s->wave && c.splb * c.splk != 1)
-        return fail(DFT_EINVAL, "combination is split over primitives and runs the lane kernel");
-      if (!s->wave && s->block != 64 && s->block != 128 && s->block != 256 && s->block != 512)
+      if (s->wave != 0) return fail(DFT_EINVAL, "the first int of a shape is reserved and must be 0");
+      if (s->block != 64 && s->block != 128 && s->block != 256 && s->block != 512)
         return fail(DFT_EINVAL, "block must be 64, 128, 256 or 512");
-      if (!s->wave && s->block > lane_block_max((int)ci))
-        return fail(DFT_EINVAL, "block exceeds the class's largest workgroup");
+      if (s->block > block_max((int)ci)) return fail(DFT_EINVAL, "block exceeds the class's largest workgroup");
       if (s->cap < 1) return fail(DFT_EINVAL, "grid cap must be positive");
     }
     c.shape64 = s64;

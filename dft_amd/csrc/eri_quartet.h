@@ -60,25 +60,32 @@ struct QuartetShape {
   static constexpr int RSB = rec_size(LA, LB), RSK = rec_size(LC, LD);
 };
 
-// I[ab * NCD + cd] for the quartet (bra pair: base bb, stride sb, offset ob,
-// nb primitive pairs; ket pair likewise).
-template <int LA, int LB, int LC, int LD, typename real>
-__device__ __forceinline__ void quartet_integrals(const real* __restrict__ bb, int sb, unsigned ob, int nb,
-                                                  const real* __restrict__ bk, int sk, unsigned ok, int nk,
-                                                  const double* __restrict__ btab,
-                                                  real (&I)[QuartetShape<LA, LB, LC, LD, real>::NI]) {
+// Number of ket component pairs cd evaluated by one call: all of them
+// (NCDK < 0) or the slice [CD0, CD0 + NCDK).
+template <int LA, int LB, int LC, int LD, int NCDK>
+constexpr int ncd_slice() { return NCDK < 0 ? ncart(LC) * ncart(LD) : NCDK; }
+
+// I[ab * NK + k] = (ab|cd) for cd = CD0 + k, k < NK (NK = ncd_slice), for the
+// quartet (bra pair: base bb, stride sb, offset ob, nb primitive pairs; ket
+// pair likewise). With the default slice this is the whole I[ab * NCD + cd].
+template <int LA, int LB, int LC, int LD, typename real, int CD0 = 0, int NCDK = -1>
+__device__ __forceinline__ void quartet_integrals(
+    const real* __restrict__ bb, int sb, unsigned ob, int nb, const real* __restrict__ bk, int sk, unsigned ok,
+    int nk, const double* __restrict__ btab,
+    real (&I)[QuartetShape<LA, LB, LC, LD, real>::NAB * ncd_slice<LA, LB, LC, LD, NCDK>()]) {
   using S = QuartetShape<LA, LB, LC, LD, real>;
+  constexpr int NK = ncd_slice<LA, LB, LC, LD, NCDK>();
   constexpr int EXB = 5, EYB = 5 + S::NEB, EZB = 5 + 2 * S::NEB;
   constexpr int EXK = 5, EYK = 5 + S::NEK, EZK = 5 + 2 * S::NEK;
 #pragma unroll
-  for (int x = 0; x < S::NI; ++x) I[x] = real(0);
+  for (int x = 0; x < S::NAB * NK; ++x) I[x] = real(0);
   for (int i = 0; i < nb; ++i) {
     const unsigned oi = ob + (unsigned)(i * S::RSB * sb);
     const real p = fld(bb, sb, 0, oi), Px = fld(bb, sb, 1, oi), Py = fld(bb, sb, 2, oi);
     const real Pz = fld(bb, sb, 3, oi), Kab = fld(bb, sb, 4, oi);
-    real G[S::NHB * S::NCD];
+    real G[S::NHB * NK];
 #pragma unroll
-    for (int x = 0; x < S::NHB * S::NCD; ++x) G[x] = real(0);
+    for (int x = 0; x < S::NHB * NK; ++x) G[x] = real(0);
     // the head (q, Q, K_cd) of ket primitive j + 1 is loaded while primitive
     // j is evaluated, which hides most of the L2 latency of the record loads
     unsigned oj = ok;
@@ -103,8 +110,9 @@ __device__ __forceinline__ void quartet_integrals(const real* __restrict__ bb, i
       boys_eval<S::L, real>(T, btab, F);
       real R[nherm(S::L)];
       hermite_R<S::L, real>(F, real(-2) * alpha, X, Y, Z, R);
-      static_for<0, S::NCD>([&](auto cdI) {
-        constexpr int cd = decltype(cdI)::value;
+      static_for<0, NK>([&](auto kI) {
+        constexpr int kk = decltype(kI)::value;
+        constexpr int cd = CD0 + kk;
         constexpr int c = cd / S::ND, d = cd % S::ND;
         constexpr int cx = comp_x(LC, c), cy = comp_y(LC, c), cz = comp_z(LC, c);
         constexpr int dx = comp_x(LD, d), dy = comp_y(LD, d), dz = comp_z(LD, d);
@@ -124,7 +132,7 @@ __device__ __forceinline__ void quartet_integrals(const real* __restrict__ bb, i
         static_for<0, S::NHB>([&](auto hI) {
           constexpr int h = decltype(hI)::value;
           constexpr int t = h_t(h), u = h_u(h), v = h_v(h);
-          real s = G[h * S::NCD + cd];
+          real s = G[h * NK + kk];
           static_for<0, cx + dx + 1>([&](auto tI) {
             static_for<0, cy + dy + 1>([&](auto uI) {
               static_for<0, cz + dz + 1>([&](auto vI) {
@@ -134,7 +142,7 @@ __device__ __forceinline__ void quartet_integrals(const real* __restrict__ bb, i
               });
             });
           });
-          G[h * S::NCD + cd] = s;
+          G[h * NK + kk] = s;
         });
       });
     }
@@ -150,9 +158,9 @@ __device__ __forceinline__ void quartet_integrals(const real* __restrict__ bb, i
             constexpr int h = hidx(t, u, v);
             const real e = e_at<LB, ax, bx, t, EXB>(bb, sb, oi) * e_at<LB, ay, by, u, EYB>(bb, sb, oi) *
                            e_at<LB, az, bz, v, EZB>(bb, sb, oi);
-            static_for<0, S::NCD>([&](auto cdI) {
-              constexpr int cd = decltype(cdI)::value;
-              I[ab * S::NCD + cd] = fma(e, G[h * S::NCD + cd], I[ab * S::NCD + cd]);
+            static_for<0, NK>([&](auto kI) {
+              constexpr int kk = decltype(kI)::value;
+              I[ab * NK + kk] = fma(e, G[h * NK + kk], I[ab * NK + kk]);
             });
           });
         });

@@ -51,7 +51,8 @@ hipError_t launch_pair_records(const int4* pairs_prim /* sh_a, sh_b, i_a, i_b */
                                double* rec64, float* rec32, hipStream_t s);
 
 // true if the lane-per-quartet kernel is instantiated for this combination
-// (s/p classes and the light d classes); the others use the wave kernel
+// (s/p classes and the light d classes); the others use the column kernel
+// (launch_eri_chunk) and its digest
 bool lane_kernel_combo(int combo);
 
 // Pair records of one lane-kernel launch. The bra pairs of a wave differ
@@ -87,12 +88,16 @@ hipError_t launch_eri_jk(int combo, bool fp32, const int2* quartets, const long 
                          const double* DJ, const double* DK0, const double* DK1, int nk,
                          double* stage, const double* boys_tab, int block, int grid_cap, hipStream_t s);
 
-// one-wave-per-quartet kernel (any class up to (dd|dd)); raw = write the
-// integral blocks to out + q * NA*NB*NC*ND instead of J/K contributions
-hipError_t launch_eri_wave(int la, int lb, int lc, int ld, bool fp32, bool raw, const int2* quartets,
-                           const long long* slots, long long nq, const int4* pairs, const void* rec,
-                           const int4* shell_dm, const double* DJ, const double* DK0, const double* DK1, int nk,
-                           double* out, const double* boys_tab, int grid_cap, hipStream_t s);
+// Heavy combinations (d with p/d; no lane kernel): integrals of quartets
+// [first, first + n), one lane per (quartet, ket component pair), stored at
+// out[x * xs + q * qs] (x = abcd row-major); and the J/K blocks of the whole
+// list of n quartets from integrals stored with xs = 1, qs = NI.
+hipError_t launch_eri_chunk(int combo, bool fp32, const int2* quartets, int first, int n, const int4* pairs,
+                            const RecView& rv, double* out, long long xs, long long qs, const double* boys_tab,
+                            int block, int grid_cap, hipStream_t s);
+hipError_t launch_jk_digest(int combo, const int2* quartets, const long long* slots, int n, const int4* pairs,
+                            const int4* shell_dm, const double* DJ, const double* DK0, const double* DK1, int nk,
+                            const double* ibuf, double* stage, hipStream_t s);
 
 // raw contracted blocks (internal orientation) for one class combination:
 // out + q * NA*NB*NC*ND

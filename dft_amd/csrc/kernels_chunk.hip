@@ -1,0 +1,329 @@
+// ERIs of the heavy class combinations (a d shell with p/d partners), one
+// lane per (quartet, ket component pair cd), and their J/K digest.
+//
+// A whole (dp|dp) or (dd|dd) quartet does not fit a lane's registers
+// ((dd|dd): 1296 integrals, 165 Hermite R values), so lane (q, cd) computes
+// the column I[ab][cd] only: per primitive quartet it evaluates Boys and R
+// (recomputed per cd, which costs less than the contraction it serves), the
+// ket step G[h] = pref sum_{tau nu phi} (-1)^{tau+nu+phi} E^c_tau E^d_nu E_phi
+// R[h + (tau,nu,phi)] over the dense ket Hermite set with the lane's E
+// factors loaded at run time (zero beyond its component sums), and, once per
+// bra primitive, the bra step for all ab with compile-time indices
+// (eri_quartet.h does the same with every index compile-time). cd is a
+// run-time index so that one kernel serves all columns of a class (a
+// compile-time column per instantiation would multiply the code by NCD).
+// The column goes to out[q * NI + ab * NCD + cd]: the raw integral blocks
+// (parity tests, Schwarz diagonals) or a per-combination buffer from which
+// a second kernel, one wave per quartet, forms the same J/K contribution
+// blocks as the lane kernel (kernels_eri.hip).
+#include <algorithm>
+
+#include "eri_quartet.h"
+#include "kernels.h"
+
+namespace dft {
+
+// I[ab] = (ab|cd) for one run-time ket column cd.
+template <int LA, int LB, int LC, int LD, typename real>
+__device__ __forceinline__ void quartet_column(const real* __restrict__ bb, int sb, unsigned ob, int nb,
+                                               const real* __restrict__ bk, unsigned ok, int nk, int cd,
+                                               const double* __restrict__ btab,
+                                               real (&I)[ncart(LA) * ncart(LB)]) {
+  using S = QuartetShape<LA, LB, LC, LD, real>;
+  constexpr int LCD = LC + LD;
+  constexpr int EXB = 5, EYB = 5 + S::NEB, EZB = 5 + 2 * S::NEB;
+  // component sums and E offsets of the column's (c, d) per axis (-1 marks
+  // E_0^{00} = 1, which is not stored); scalar work when cd is wave-uniform
+  const int c = cd / S::ND, d = cd - (cd / S::ND) * S::ND;
+  int exa[3], eoa[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int ic = comp_of(LC, c, a), id = comp_of(LD, d, a);
+    exa[a] = ic + id;
+    eoa[a] = (ic == 0 && id == 0) ? -1 : eoff(LD, ic, id);
+  }
+  const int exx = exa[0], exy = exa[1], exz = exa[2];
+  const int eox = eoa[0], eoy = eoa[1], eoz = eoa[2];
+#pragma unroll
+  for (int x = 0; x < S::NAB; ++x) I[x] = real(0);
+  for (int i = 0; i < nb; ++i) {
+    const unsigned oi = ob + (unsigned)(i * S::RSB * sb);
+    const real p = fld(bb, sb, 0, oi), Px = fld(bb, sb, 1, oi), Py = fld(bb, sb, 2, oi);
+    const real Pz = fld(bb, sb, 3, oi), Kab = fld(bb, sb, 4, oi);
+    real G[S::NHB];
+#pragma unroll
+    for (int x = 0; x < S::NHB; ++x) G[x] = real(0);
+    for (int j = 0; j < nk; ++j) {
+      const real* rk = bk + ok + (unsigned)(j * S::RSK);
+      const real q = rk[0];
+      const real X = Px - rk[1], Y = Py - rk[2], Z = Pz - rk[3];
+      const real rs = rsqrt_nr(p + q);
+      const real alpha = p * q * rs * rs;
+      const real pref = real(TWO_PI_2P5) * Kab * rk[4] * rs;
+      const real T = alpha * (X * X + Y * Y + Z * Z);
+      real F[S::L + 1];
+      boys_eval<S::L, real>(T, btab, F);
+      real R[nherm(S::L)];
+      hermite_R<S::L, real>(F, real(-2) * alpha, X, Y, Z, R);
+      // the lane's ket E factors with the (-1)^t sign and pref (x axis)
+      // folded in; zero beyond the component sums
+      real ex[LCD + 1], ey[LCD + 1], ez[LCD + 1];
+      const real* E = rk + 5;
+#pragma unroll
+      for (int t = 0; t <= LCD; ++t) {
+        const real sg = (t & 1) ? real(-1) : real(1);
+        ex[t] = t > exx ? real(0) : (eox < 0 ? real(1) : E[eox + t]) * sg * pref;
+        ey[t] = t > exy ? real(0) : (eoy < 0 ? real(1) : E[S::NEK + eoy + t]) * sg;
+        ez[t] = t > exz ? real(0) : (eoz < 0 ? real(1) : E[2 * S::NEK + eoz + t]) * sg;
+      }
+      real eyz[nherm(LCD) > 0 ? (LCD + 1) * (LCD + 2) / 2 : 1];
+      static_for<0, LCD + 1>([&](auto nI) {
+        constexpr int nu = decltype(nI)::value;
+        static_for<0, LCD + 1 - nu>([&](auto pI) {
+          constexpr int ph = decltype(pI)::value;
+          eyz[nu * (LCD + 1) - nu * (nu - 1) / 2 + ph] = ey[nu] * ez[ph];
+        });
+      });
+      static_for<0, S::NHB>([&](auto hI) {
+        constexpr int h = decltype(hI)::value;
+        constexpr int t = h_t(h), u = h_u(h), v = h_v(h);
+        real g = G[h];
+        static_for<0, LCD + 1>([&](auto tI) {
+          constexpr int ta = decltype(tI)::value;
+          real inner = real(0);
+          static_for<0, LCD + 1 - ta>([&](auto nI) {
+            constexpr int nu = decltype(nI)::value;
+            static_for<0, LCD + 1 - ta - nu>([&](auto pI) {
+              constexpr int ph = decltype(pI)::value;
+              inner = fma(eyz[nu * (LCD + 1) - nu * (nu - 1) / 2 + ph], R[hidx(t + ta, u + nu, v + ph)], inner);
+            });
+          });
+          g = fma(ex[ta], inner, g);
+        });
+        G[h] = g;
+      });
+    }
+    static_for<0, S::NAB>([&](auto abI) {
+      constexpr int ab = decltype(abI)::value;
+      constexpr int a = ab / S::NB, b = ab % S::NB;
+      constexpr int ax = comp_x(LA, a), ay = comp_y(LA, a), az = comp_z(LA, a);
+      constexpr int bx = comp_x(LB, b), by = comp_y(LB, b), bz = comp_z(LB, b);
+      static_for<0, ax + bx + 1>([&](auto tI) {
+        static_for<0, ay + by + 1>([&](auto uI) {
+          static_for<0, az + bz + 1>([&](auto vI) {
+            constexpr int t = decltype(tI)::value, u = decltype(uI)::value, v = decltype(vI)::value;
+            const real e = e_at<LB, ax, bx, t, EXB>(bb, sb, oi) * e_at<LB, ay, by, u, EYB>(bb, sb, oi) *
+                           e_at<LB, az, bz, v, EZB>(bb, sb, oi);
+            I[ab] = fma(e, G[hidx(t, u, v)], I[ab]);
+          });
+        });
+      });
+    });
+  }
+}
+
+template <int LA, int LB, int LC, int LD, typename real>
+__global__ __launch_bounds__(256) void eri_chunk_kernel(const int2* __restrict__ quartets, int first, int n,
+                                                        const int4* __restrict__ pairs, const real* __restrict__ recb,
+                                                        int sb, int fb, const real* __restrict__ reck,
+                                                        double* __restrict__ out, long long xs, long long qs,
+                                                        const double* __restrict__ boys_gtab) {
+  using S = QuartetShape<LA, LB, LC, LD, real>;
+  __shared__ double btab[BOYS_LDS];
+  boys_stage<S::L>(boys_gtab, btab);
+  // lane v: column cd = v / n (column-major, so a wave shares cd and mostly
+  // the ket pair), quartet first + v % n
+  const unsigned nv = (unsigned)n * (unsigned)S::NCD;
+  const unsigned stride = gridDim.x * blockDim.x;
+  for (unsigned v = blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += stride) {
+    const int cd = (int)(v / (unsigned)n);
+    const int q = first + (int)(v - (unsigned)cd * (unsigned)n);
+    const int2 qd = quartets[q];
+    const int4 pb = pairs[qd.x];
+    const int4 pk = pairs[qd.y];
+    const unsigned ob = (unsigned)(qd.x - fb);
+    real I[S::NAB];
+    // wave-uniform ket and column (the common case): scalar ket loads and
+    // table lookups, as in the lane kernel
+    const unsigned okf = (unsigned)__builtin_amdgcn_readfirstlane(pk.z);
+    const int nkf = __builtin_amdgcn_readfirstlane(pk.w);
+    const int cdf = __builtin_amdgcn_readfirstlane(cd);
+    if (__all((unsigned)pk.z == okf && pk.w == nkf && cd == cdf))
+      quartet_column<LA, LB, LC, LD, real>(recb, sb, ob, pb.w, reck, okf, nkf, cdf, btab, I);
+    else
+      quartet_column<LA, LB, LC, LD, real>(recb, sb, ob, pb.w, reck, (unsigned)pk.z, pk.w, cd, btab, I);
+#pragma unroll
+    for (int ab = 0; ab < S::NAB; ++ab) out[(long long)(ab * S::NCD + cd) * xs + (long long)q * qs] = (double)I[ab];
+  }
+}
+
+// J/K blocks of one quartet from its integrals I[q * NI + x] (x = abcd
+// row-major), one 64-lane wave per quartet, lanes over the output elements
+// of the quartet's blocks (run-time loops: the blocks of the heavy classes
+// are large and their digest is a small part of the work); same weights and
+// blocks as the lane kernel:
+//   Jab = 2w sum_cd I P_cd        Jcd = 2w sum_ab I P_ab
+//   Kac = w sum_bd I P_bd  Kad = w sum_bc I P_bc  Kbc = w sum_ad I P_ad  Kbd = w sum_ac I P_ac
+template <int NK>
+__global__ __launch_bounds__(256) void jk_digest_kernel(int na, int nb, int nc, int nd,
+                                                        const int2* __restrict__ quartets,
+                                                        const long long* __restrict__ slots, int n,
+                                                        const int4* __restrict__ pairs,
+                                                        const int4* __restrict__ shell_dm,
+                                                        const double* __restrict__ DJ, const double* __restrict__ DK0,
+                                                        const double* __restrict__ DK1,
+                                                        const double* __restrict__ ibuf, double* __restrict__ stage) {
+  const int lane = threadIdx.x & 63;
+  const int wpb = blockDim.x >> 6;
+  const int nab = na * nb, ncd = nc * nd, ni = nab * ncd;
+  const int nK = na * nc + na * nd + nb * nc + nb * nd;
+  const int nout = nab + ncd + NK * nK;
+  for (int q = blockIdx.x * wpb + (threadIdx.x >> 6); q < n; q += gridDim.x * wpb) {
+    const int2 qd = quartets[q];
+    const int4 pb = pairs[qd.x];
+    const int4 pk = pairs[qd.y];
+    const double* Iq = ibuf + (long long)q * ni;
+    auto I = [&](int a, int b, int c, int d) { return Iq[((a * nb + b) * nc + c) * nd + d]; };
+    double w = 1.0;
+    if (pb.x == pb.y) w *= 0.5;
+    if (pk.x == pk.y) w *= 0.5;
+    if (qd.x == qd.y) w *= 0.5;
+    const int4 sa = shell_dm[pb.x];
+    const long long a0 = sa.x, b0 = shell_dm[pb.y].x, c0 = shell_dm[pk.x].x, d0 = shell_dm[pk.y].x;
+    const long long nbf = sa.y;
+    const long long mb = (long long)(unsigned)sa.z | ((long long)sa.w << 32);
+    const double* DJm = DJ + mb;
+    const long long* sl = slots + (long long)q * (2 + 4 * NK);
+    for (int o = lane; o < nout; o += 64) {
+      double s = 0.0;
+      if (o < nab) {
+        const int a = o / nb, b = o - a * nb;
+        for (int c = 0; c < nc; ++c)
+          for (int d = 0; d < nd; ++d) s = fma(I(a, b, c, d), DJm[(c0 + c) * nbf + d0 + d], s);
+        stage[sl[0] + o] = 2.0 * w * s;
+      } else if (o < nab + ncd) {
+        const int x = o - nab, c = x / nd, d = x - c * nd;
+        for (int a = 0; a < na; ++a)
+          for (int b = 0; b < nb; ++b) s = fma(I(a, b, c, d), DJm[(a0 + a) * nbf + b0 + b], s);
+        stage[sl[1] + x] = 2.0 * w * s;
+      } else {
+        int x = o - nab - ncd;
+        const int kk = x / nK;
+        x -= kk * nK;
+        const double* D = (kk == 0 ? DK0 : DK1) + mb;
+        const long long* ks = sl + 2 + 4 * kk;
+        if (x < na * nc) {
+          const int a = x / nc, c = x - a * nc;
+          for (int b = 0; b < nb; ++b)
+            for (int d = 0; d < nd; ++d) s = fma(I(a, b, c, d), D[(b0 + b) * nbf + d0 + d], s);
+          stage[ks[0] + x] = w * s;
+        } else if ((x -= na * nc) < na * nd) {
+          const int a = x / nd, d = x - a * nd;
+          for (int b = 0; b < nb; ++b)
+            for (int c = 0; c < nc; ++c) s = fma(I(a, b, c, d), D[(b0 + b) * nbf + c0 + c], s);
+          stage[ks[1] + x] = w * s;
+        } else if ((x -= na * nd) < nb * nc) {
+          const int b = x / nc, c = x - b * nc;
+          for (int a = 0; a < na; ++a)
+            for (int d = 0; d < nd; ++d) s = fma(I(a, b, c, d), D[(a0 + a) * nbf + d0 + d], s);
+          stage[ks[2] + x] = w * s;
+        } else {
+          x -= nb * nc;
+          const int b = x / nd, d = x - b * nd;
+          for (int a = 0; a < na; ++a)
+            for (int c = 0; c < nc; ++c) s = fma(I(a, b, c, d), D[(a0 + a) * nbf + c0 + c], s);
+          stage[ks[3] + x] = w * s;
+        }
+      }
+    }
+  }
+}
+
+static int chunk_grid(long long lanes, int cap, int block) {
+  long long g = (lanes + block - 1) / block;
+  if (g > cap) g = cap;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+// heavy combinations: (ds|pp), (dp|ps), (dp|pp), (dp|ds), (dp|dp), (dd|ps),
+// (dd|pp), (dd|ds), (dd|dp), (dd|dd)
+#define DFT_FOR_HEAVY(X) \
+  X(8, 2, 0, 1, 1)       \
+  X(11, 2, 1, 1, 0)      \
+  X(12, 2, 1, 1, 1)      \
+  X(13, 2, 1, 2, 0)      \
+  X(14, 2, 1, 2, 1)      \
+  X(16, 2, 2, 1, 0)      \
+  X(17, 2, 2, 1, 1)      \
+  X(18, 2, 2, 2, 0)      \
+  X(19, 2, 2, 2, 1)      \
+  X(20, 2, 2, 2, 2)
+
+int chunk_slices(int combo) {
+  switch (combo) {
+#define X(id, la, lb, lc, ld) \
+  case id:                    \
+    return ncart(lc) * ncart(ld);
+    DFT_FOR_HEAVY(X)
+#undef X
+    default:
+      return 0;
+  }
+}
+
+hipError_t launch_eri_chunk(int combo, bool fp32, const int2* q, int first, int n, const int4* pairs,
+                            const RecView& rv, double* out, long long xs, long long qs, const double* bt, int block,
+                            int cap, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  if (block != 64 && block != 128 && block != 256) return hipErrorInvalidValue;
+  switch (combo) {
+#define X(id, la, lb, lc, ld)                                                                                   \
+  case id: {                                                                                                    \
+    const long long lanes = (long long)n * chunk_slices(id);                                                    \
+    dim3 grid(chunk_grid(lanes, cap, block)), blk(block);                                                       \
+    if (fp32)                                                                                                   \
+      hipLaunchKernelGGL((eri_chunk_kernel<la, lb, lc, ld, float>), grid, blk, 0, s, q, first, n, pairs,         \
+                         (const float*)rv.bra, rv.sb, rv.fb, (const float*)rv.ket, out, xs, qs, bt);           \
+    else                                                                                                        \
+      hipLaunchKernelGGL((eri_chunk_kernel<la, lb, lc, ld, double>), grid, blk, 0, s, q, first, n, pairs,        \
+                         (const double*)rv.bra, rv.sb, rv.fb, (const double*)rv.ket, out, xs, qs, bt);         \
+    return hipGetLastError();                                                                                   \
+  }
+    DFT_FOR_HEAVY(X)
+#undef X
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_jk_digest(int combo, const int2* q, const long long* sl, int n, const int4* pairs,
+                            const int4* shell_dm, const double* DJ, const double* DK0, const double* DK1, int nk,
+                            const double* ibuf, double* stage, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  int la, lb, lc, ld;
+  switch (combo) {
+#define X(id, a, b, c, d) \
+  case id:                \
+    la = a;               \
+    lb = b;               \
+    lc = c;               \
+    ld = d;               \
+    break;
+    DFT_FOR_HEAVY(X)
+#undef X
+    default:
+      return hipErrorInvalidValue;
+  }
+  const int wpb = 4;  // quartets (waves) per workgroup
+  dim3 grid((unsigned)std::min<long long>(((long long)n + wpb - 1) / wpb, 65536)), blk(64 * wpb);
+  if (nk == 1)
+    hipLaunchKernelGGL((jk_digest_kernel<1>), grid, blk, 0, s, ncart(la), ncart(lb), ncart(lc), ncart(ld), q, sl,
+                       n, pairs, shell_dm, DJ, DK0, DK1, ibuf, stage);
+  else
+    hipLaunchKernelGGL((jk_digest_kernel<2>), grid, blk, 0, s, ncart(la), ncart(lb), ncart(lc), ncart(ld), q, sl,
+                       n, pairs, shell_dm, DJ, DK0, DK1, ibuf, stage);
+  return hipGetLastError();
+}
+
+}  // namespace dft

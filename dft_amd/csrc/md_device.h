@@ -207,7 +207,10 @@ __device__ __forceinline__ void boys_stage(const double* __restrict__ gtab, doub
 // --------------------------------------------------- Hermite R^0_{tuv}
 // R^n_000 = (-2a)^n F_n; R^n_{t+1,u,v} = t R^{n+1}_{t-1,u,v} + X R^{n+1}_{tuv}
 // (and the same in u/Y, v/Z). Evaluated level by level from n = L down to 0
-// entirely in registers; returns R^0 for t+u+v <= L in hidx order.
+// in one register array, in place: within a level the entries are updated
+// from the highest t+u+v down, so every entry still read (t+u+v lower by one
+// or two) holds the previous level's value. Returns R^0 for t+u+v <= L in
+// hidx order.
 template <int L, typename real>
 __device__ __forceinline__ void hermite_R(const real (&F)[L + 1], real m2a, real X, real Y, real Z,
                                           real (&R)[nherm(L)]) {
@@ -215,33 +218,33 @@ __device__ __forceinline__ void hermite_R(const real (&F)[L + 1], real m2a, real
   real pw = real(1);
 #pragma unroll
   for (int n = 0; n <= L; ++n) { s[n] = F[n] * pw; pw *= m2a; }
-  real prev[nherm(L)];
-  prev[0] = s[L];
+  R[0] = s[L];
   static_for<0, L>([&](auto it) {
     constexpr int n = L - 1 - decltype(it)::value;
-    constexpr int S = L - n;
-    real cur[nherm(S)];
-    cur[0] = s[n];
-    static_for<1, nherm(S)>([&](auto hI) {
-      constexpr int h = decltype(hI)::value;
-      constexpr int t = h_t(h), u = h_u(h), v = h_v(h);
-      if constexpr (t > 0) {
-        real r = X * prev[hidx(t - 1, u, v)];
-        if constexpr (t > 1) r = fma(real(t - 1), prev[hidx(t - 2, u, v)], r);
-        cur[h] = r;
-      } else if constexpr (u > 0) {
-        real r = Y * prev[hidx(t, u - 1, v)];
-        if constexpr (u > 1) r = fma(real(u - 1), prev[hidx(t, u - 2, v)], r);
-        cur[h] = r;
-      } else {
-        real r = Z * prev[hidx(t, u, v - 1)];
-        if constexpr (v > 1) r = fma(real(v - 1), prev[hidx(t, u, v - 2)], r);
-        cur[h] = r;
-      }
+    constexpr int S = L - n;  // this level holds t+u+v <= S
+    // entries with t+u+v = k, from k = S down to 1
+    static_for<0, S>([&](auto kI) {
+      constexpr int k = S - decltype(kI)::value;
+      static_for<nherm(k - 1), nherm(k)>([&](auto hI) {
+        constexpr int h = decltype(hI)::value;
+        constexpr int t = h_t(h), u = h_u(h), v = h_v(h);
+        if constexpr (t > 0) {
+          real r = X * R[hidx(t - 1, u, v)];
+          if constexpr (t > 1) r = fma(real(t - 1), R[hidx(t - 2, u, v)], r);
+          R[h] = r;
+        } else if constexpr (u > 0) {
+          real r = Y * R[hidx(t, u - 1, v)];
+          if constexpr (u > 1) r = fma(real(u - 1), R[hidx(t, u - 2, v)], r);
+          R[h] = r;
+        } else {
+          real r = Z * R[hidx(t, u, v - 1)];
+          if constexpr (v > 1) r = fma(real(v - 1), R[hidx(t, u, v - 2)], r);
+          R[h] = r;
+        }
+      });
     });
-    static_for<0, nherm(S)>([&](auto hI) { prev[decltype(hI)::value] = cur[decltype(hI)::value]; });
+    R[0] = s[n];
   });
-  static_for<0, nherm(L)>([&](auto hI) { R[decltype(hI)::value] = prev[decltype(hI)::value]; });
 }
 
 }  // namespace dft

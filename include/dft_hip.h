@@ -119,14 +119,13 @@ int dft_graph_destroy(dft_graph graph);
 /*
  * Autotune the launch shape of every class-combination launch of the plan:
  * for the FP64 launch (whole list) and the FP32 launch (weak part) it times
- * the lane-per-quartet kernel at workgroup sizes 64/128/256/512 (up to the
- * class's limit) and grid caps 512..8192, and the wave-per-quartet kernel at
- * three grid caps (median of reps runs each) and keeps the fastest. report (capacity report_cap ints)
- * receives 6 ints per combination: FP64 {wave, block, cap}, FP32 {wave,
- * block, cap}. combo >= 0 tunes only that combination (-1: all). The wave
- * kernel is a candidate for (pp|pp) and the d classes only, and not for a
- * class split over primitive chunks. Overwrites the plan's staging (J/K are
- * not written).
+ * the class's kernel (lane-per-quartet kernel, or the column kernel of the
+ * heavy d classes) at workgroup sizes 64/128/256/512 (up to the class's
+ * limit) and grid caps 512..8192 (median of reps runs each) and keeps the
+ * fastest. report (capacity report_cap ints) receives 6 ints per
+ * combination: FP64 {0, block, cap}, FP32 {0, block, cap} (the first int of
+ * each triple is reserved). combo >= 0 tunes only that combination (-1:
+ * all). Overwrites the plan's staging (J/K are not written).
  * No counterpart in the reference.
  */
 int dft_plan_tune(dft_plan plan, const double* P, const double* Pa, const double* Pb, int reps, int combo,
