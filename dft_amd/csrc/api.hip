@@ -23,6 +23,8 @@ namespace {
 
 // lanes below which a lane-kernel class combination is split over primitives
 constexpr long long SPLIT_LANES = 65536;
+// the same for the column kernel of the heavy classes (longer lanes)
+constexpr long long SPLIT_LANES_HEAVY = 262144;
 // auxiliary streams per plan for concurrent class launches
 constexpr int JK_AUX = 3;
 
@@ -247,8 +249,8 @@ int raw_blocks(dft_plan p, int combo, const std::vector<int2>& q, std::vector<do
     e = launch_eri_raw(combo, dq, (long long)q.size(), p->d_pairs, p->d_rec64, dout, p->d_boys, nullptr);
   } else {
     const RecView rv = rec_view(p, combo, false);
-    e = launch_eri_chunk(combo, false, dq, 0, (int)q.size(), p->d_pairs, rv, dout, 1, blk, p->d_boys, 256, 1 << 20,
-                         nullptr);
+    e = launch_eri_chunk(combo, false, dq, 0, (int)q.size(), 1, p->d_pairs, rv, dout, 1, blk, 0, p->d_boys, 256,
+                         1 << 20, nullptr);
   }
   if (e == hipSuccess) e = hipMemcpy(out.data(), dout, out.size() * sizeof(double), hipMemcpyDeviceToHost);
   (void)hipFree(dq);
@@ -519,9 +521,17 @@ int build_plan(dft_plan p, const Input& in) {
         pc_moments(pcb, la, lb);
         pc_moments(pck, lc, ld);
         const long long ni = (long long)ncart_h(la) * ncart_h(lb) * ncart_h(lc) * ncart_h(ld);
-        if (nq * ncart_h(lc) * ncart_h(ld) > 0x7fffffffLL)
-          return fail(DFT_EINVAL, "class combination exceeds 2^31 lanes");
-        HIP_TRY(hipMalloc(&c.d_ibuf, nq * ni * sizeof(double)));
+        const long long ncd = (long long)ncart_h(lc) * ncart_h(ld);
+        // column kernel: split the bra primitives into chunks where the
+        // class has too few (quartet, column) lanes to fill the chip
+        if (p->split && nq * ncd < SPLIT_LANES_HEAVY) {
+          int nbmax = 0;
+          for (const int2& qq : c.q) nbmax = std::max(nbmax, p->pairs[qq.x].w);
+          const long long want = (SPLIT_LANES_HEAVY + nq * ncd - 1) / (nq * ncd);
+          c.splb = (int)std::max<long long>(1, std::min<long long>(want, nbmax));
+        }
+        if (nq * ncd * c.splb > 0x7fffffffLL) return fail(DFT_EINVAL, "class combination exceeds 2^31 lanes");
+        HIP_TRY(hipMalloc(&c.d_ibuf, nq * ni * c.splb * sizeof(double)));
       }
       if ((rc = upload(&c.d_q, c.q))) return rc;
     }
@@ -560,10 +570,17 @@ int build_plan(dft_plan p, const Input& in) {
   };
   std::vector<int> tcount(p->ntarget, 0);
   size_t tq[10];
-  for (const Combo& c : p->combos)
-    for (const int2& qq : c.q) {
+  // slot rows per quartet: one per primitive chunk for the lane kernel
+  // (each chunk writes partial blocks); one for the heavy classes (their
+  // digest sums the chunks' integrals first)
+  auto slot_rows = [&](size_t ci) {
+    const Combo& c = p->combos[ci];
+    return lane_kernel_combo((int)ci) ? c.splb * c.splk : 1;
+  };
+  for (size_t ci = 0; ci < p->combos.size(); ++ci)
+    for (const int2& qq : p->combos[ci].q) {
       quartet_targets(qq, tq);
-      for (int k = 0; k < nblk; ++k) tcount[tq[k]] += c.splb * c.splk;
+      for (int k = 0; k < nblk; ++k) tcount[tq[k]] += slot_rows(ci);
     }
   std::vector<long long> tbase(p->ntarget, 0), tnext(p->ntarget, 0);
   std::vector<Target> tg(p->ntarget);
@@ -586,8 +603,9 @@ int build_plan(dft_plan p, const Input& in) {
           cur += (long long)tcount[t] * nr * nc;
         }
   p->stage_len = cur;
-  for (Combo& c : p->combos) {
-    const size_t nq = c.q.size(), ns = (size_t)c.splb * c.splk;
+  for (size_t ci = 0; ci < p->combos.size(); ++ci) {
+    Combo& c = p->combos[ci];
+    const size_t nq = c.q.size(), ns = (size_t)slot_rows(ci);
     std::vector<long long> slots(nq * ns * nblk);
     for (size_t ch = 0; ch < ns; ++ch)
       for (size_t i = 0; i < nq; ++i) {
@@ -655,8 +673,8 @@ hipError_t launch_combo(dft_plan p, size_t ci, bool f32, long long first, long l
   pc_moments(c.pcb, la, lb);
   pc_moments(c.pck, lc, ld);
   const long long ni = (long long)ncart_h(la) * ncart_h(lb) * ncart_h(lc) * ncart_h(ld);
-  return launch_eri_chunk((int)ci, f32, c.d_q, (int)first, (int)n, p->d_pairs, rv, c.d_ibuf, 1, ni, p->d_boys,
-                          sh.block, sh.cap, s);
+  return launch_eri_chunk((int)ci, f32, c.d_q, (int)first, (int)n, c.splb, p->d_pairs, rv, c.d_ibuf, 1, ni,
+                          ni * (long long)c.q.size(), p->d_boys, sh.block, sh.cap, s);
 }
 
 // J/K blocks of a heavy combination from its integral buffer (whole list).
@@ -664,8 +682,12 @@ hipError_t launch_digest(dft_plan p, size_t ci, const double* P, const double* P
                          hipStream_t s) {
   const Combo& c = p->combos[ci];
   if (lane_kernel_combo((int)ci) || c.q.empty()) return hipSuccess;
-  return launch_jk_digest((int)ci, c.d_q, c.d_slots, (int)c.q.size(), p->d_pairs, p->d_shell_dm, P, Pa, Pb, p->nk,
-                          c.d_ibuf, p->d_stage, s);
+  int la, lb, lc, ld;
+  pc_moments(c.pcb, la, lb);
+  pc_moments(c.pck, lc, ld);
+  const long long ni = (long long)ncart_h(la) * ncart_h(lb) * ncart_h(lc) * ncart_h(ld);
+  return launch_jk_digest((int)ci, c.d_q, c.d_slots, (int)c.q.size(), c.splb, ni * (long long)c.q.size(),
+                          p->d_pairs, p->d_shell_dm, P, Pa, Pb, p->nk, c.d_ibuf, p->d_stage, s);
 }
 
 // The whole Fock build: class launches, reduction, symmetrisation. The class

@@ -89,15 +89,17 @@ hipError_t launch_eri_jk(int combo, bool fp32, const int2* quartets, const long 
                          double* stage, const double* boys_tab, int block, int grid_cap, hipStream_t s);
 
 // Heavy combinations (d with p/d; no lane kernel): integrals of quartets
-// [first, first + n), one lane per (quartet, ket component pair), stored at
-// out[x * xs + q * qs] (x = abcd row-major); and the J/K blocks of the whole
-// list of n quartets from integrals stored with xs = 1, qs = NI.
-hipError_t launch_eri_chunk(int combo, bool fp32, const int2* quartets, int first, int n, const int4* pairs,
-                            const RecView& rv, double* out, long long xs, long long qs, const double* boys_tab,
-                            int block, int grid_cap, hipStream_t s);
-hipError_t launch_jk_digest(int combo, const int2* quartets, const long long* slots, int n, const int4* pairs,
-                            const int4* shell_dm, const double* DJ, const double* DK0, const double* DK1, int nk,
-                            const double* ibuf, double* stage, hipStream_t s);
+// [first, first + n), one lane per (bra primitive chunk sp < splb, quartet,
+// ket component pair), chunk sp's partial integrals stored at
+// out[sp * ss + x * xs + q * qs] (x = abcd row-major); and the J/K blocks of
+// the whole list of n quartets from integrals stored with xs = 1, qs = NI,
+// summed over the splb chunks.
+hipError_t launch_eri_chunk(int combo, bool fp32, const int2* quartets, int first, int n, int splb,
+                            const int4* pairs, const RecView& rv, double* out, long long xs, long long qs,
+                            long long ss, const double* boys_tab, int block, int grid_cap, hipStream_t s);
+hipError_t launch_jk_digest(int combo, const int2* quartets, const long long* slots, int n, int splb, long long ss,
+                            const int4* pairs, const int4* shell_dm, const double* DJ, const double* DK0,
+                            const double* DK1, int nk, const double* ibuf, double* stage, hipStream_t s);
 
 // raw contracted blocks (internal orientation) for one class combination:
 // out + q * NA*NB*NC*ND

@@ -124,24 +124,31 @@ __device__ __forceinline__ void quartet_column(const real* __restrict__ bb, int 
 
 template <int LA, int LB, int LC, int LD, typename real>
 __global__ __launch_bounds__(256) void eri_chunk_kernel(const int2* __restrict__ quartets, int first, int n,
-                                                        const int4* __restrict__ pairs, const real* __restrict__ recb,
-                                                        int sb, int fb, const real* __restrict__ reck,
-                                                        double* __restrict__ out, long long xs, long long qs,
+                                                        int splb, const int4* __restrict__ pairs,
+                                                        const real* __restrict__ recb, int sb, int fb,
+                                                        const real* __restrict__ reck, double* __restrict__ out,
+                                                        long long xs, long long qs, long long ss,
                                                         const double* __restrict__ boys_gtab) {
   using S = QuartetShape<LA, LB, LC, LD, real>;
   __shared__ double btab[BOYS_LDS];
   boys_stage<S::L>(boys_gtab, btab);
-  // lane v: column cd = v / n (column-major, so a wave shares cd and mostly
-  // the ket pair), quartet first + v % n
-  const unsigned nv = (unsigned)n * (unsigned)S::NCD;
+  // lane v: bra primitive chunk sp, column cd, quartet first + qi with
+  // v = (sp * NCD + cd) * n + qi (a wave shares sp and cd, and mostly the
+  // ket pair); chunk sp's partial integrals go to out + sp * ss
+  const unsigned ncol = (unsigned)n * (unsigned)S::NCD;
+  const unsigned nv = ncol * (unsigned)splb;
   const unsigned stride = gridDim.x * blockDim.x;
   for (unsigned v = blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += stride) {
-    const int cd = (int)(v / (unsigned)n);
-    const int q = first + (int)(v - (unsigned)cd * (unsigned)n);
+    const int sp = (int)(v / ncol);
+    const unsigned rem = v - (unsigned)sp * ncol;
+    const int cd = (int)(rem / (unsigned)n);
+    const int q = first + (int)(rem - (unsigned)cd * (unsigned)n);
     const int2 qd = quartets[q];
     const int4 pb = pairs[qd.x];
     const int4 pk = pairs[qd.y];
-    const unsigned ob = (unsigned)(qd.x - fb);
+    const int szb = (pb.w + splb - 1) / splb;
+    const int i0 = min(pb.w, sp * szb), nbp = min(pb.w, i0 + szb) - i0;
+    const unsigned ob = (unsigned)(qd.x - fb) + (unsigned)(i0 * S::RSB * sb);
     real I[S::NAB];
     // wave-uniform ket and column (the common case): scalar ket loads and
     // table lookups, as in the lane kernel
@@ -149,11 +156,12 @@ __global__ __launch_bounds__(256) void eri_chunk_kernel(const int2* __restrict__
     const int nkf = __builtin_amdgcn_readfirstlane(pk.w);
     const int cdf = __builtin_amdgcn_readfirstlane(cd);
     if (__all((unsigned)pk.z == okf && pk.w == nkf && cd == cdf))
-      quartet_column<LA, LB, LC, LD, real>(recb, sb, ob, pb.w, reck, okf, nkf, cdf, btab, I);
+      quartet_column<LA, LB, LC, LD, real>(recb, sb, ob, nbp, reck, okf, nkf, cdf, btab, I);
     else
-      quartet_column<LA, LB, LC, LD, real>(recb, sb, ob, pb.w, reck, (unsigned)pk.z, pk.w, cd, btab, I);
+      quartet_column<LA, LB, LC, LD, real>(recb, sb, ob, nbp, reck, (unsigned)pk.z, pk.w, cd, btab, I);
+    double* o = out + (long long)sp * ss + (long long)q * qs;
 #pragma unroll
-    for (int ab = 0; ab < S::NAB; ++ab) out[(long long)(ab * S::NCD + cd) * xs + (long long)q * qs] = (double)I[ab];
+    for (int ab = 0; ab < S::NAB; ++ab) o[(long long)(ab * S::NCD + cd) * xs] = (double)I[ab];
   }
 }
 
@@ -165,7 +173,7 @@ __global__ __launch_bounds__(256) void eri_chunk_kernel(const int2* __restrict__
 //   Jab = 2w sum_cd I P_cd        Jcd = 2w sum_ab I P_ab
 //   Kac = w sum_bd I P_bd  Kad = w sum_bc I P_bc  Kbc = w sum_ad I P_ad  Kbd = w sum_ac I P_ac
 template <int NK>
-__global__ __launch_bounds__(256) void jk_digest_kernel(int na, int nb, int nc, int nd,
+__global__ __launch_bounds__(256) void jk_digest_kernel(int na, int nb, int nc, int nd, int splb, long long ss,
                                                         const int2* __restrict__ quartets,
                                                         const long long* __restrict__ slots, int n,
                                                         const int4* __restrict__ pairs,
@@ -183,7 +191,13 @@ __global__ __launch_bounds__(256) void jk_digest_kernel(int na, int nb, int nc, 
     const int4 pb = pairs[qd.x];
     const int4 pk = pairs[qd.y];
     const double* Iq = ibuf + (long long)q * ni;
-    auto I = [&](int a, int b, int c, int d) { return Iq[((a * nb + b) * nc + c) * nd + d]; };
+    // integrals summed over the bra primitive chunks (fixed order)
+    auto I = [&](int a, int b, int c, int d) {
+      const int x = ((a * nb + b) * nc + c) * nd + d;
+      double v = Iq[x];
+      for (int sp = 1; sp < splb; ++sp) v += Iq[sp * ss + x];
+      return v;
+    };
     double w = 1.0;
     if (pb.x == pb.y) w *= 0.5;
     if (pk.x == pk.y) w *= 0.5;
@@ -272,22 +286,22 @@ int chunk_slices(int combo) {
   }
 }
 
-hipError_t launch_eri_chunk(int combo, bool fp32, const int2* q, int first, int n, const int4* pairs,
-                            const RecView& rv, double* out, long long xs, long long qs, const double* bt, int block,
-                            int cap, hipStream_t s) {
+hipError_t launch_eri_chunk(int combo, bool fp32, const int2* q, int first, int n, int splb, const int4* pairs,
+                            const RecView& rv, double* out, long long xs, long long qs, long long ss,
+                            const double* bt, int block, int cap, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   if (block != 64 && block != 128 && block != 256) return hipErrorInvalidValue;
   switch (combo) {
 #define X(id, la, lb, lc, ld)                                                                                   \
   case id: {                                                                                                    \
-    const long long lanes = (long long)n * chunk_slices(id);                                                    \
+    const long long lanes = (long long)n * chunk_slices(id) * splb;                                             \
     dim3 grid(chunk_grid(lanes, cap, block)), blk(block);                                                       \
     if (fp32)                                                                                                   \
-      hipLaunchKernelGGL((eri_chunk_kernel<la, lb, lc, ld, float>), grid, blk, 0, s, q, first, n, pairs,         \
-                         (const float*)rv.bra, rv.sb, rv.fb, (const float*)rv.ket, out, xs, qs, bt);           \
+      hipLaunchKernelGGL((eri_chunk_kernel<la, lb, lc, ld, float>), grid, blk, 0, s, q, first, n, splb, pairs,   \
+                         (const float*)rv.bra, rv.sb, rv.fb, (const float*)rv.ket, out, xs, qs, ss, bt);       \
     else                                                                                                        \
-      hipLaunchKernelGGL((eri_chunk_kernel<la, lb, lc, ld, double>), grid, blk, 0, s, q, first, n, pairs,        \
-                         (const double*)rv.bra, rv.sb, rv.fb, (const double*)rv.ket, out, xs, qs, bt);         \
+      hipLaunchKernelGGL((eri_chunk_kernel<la, lb, lc, ld, double>), grid, blk, 0, s, q, first, n, splb, pairs,  \
+                         (const double*)rv.bra, rv.sb, rv.fb, (const double*)rv.ket, out, xs, qs, ss, bt);     \
     return hipGetLastError();                                                                                   \
   }
     DFT_FOR_HEAVY(X)
@@ -297,9 +311,9 @@ hipError_t launch_eri_chunk(int combo, bool fp32, const int2* q, int first, int 
   }
 }
 
-hipError_t launch_jk_digest(int combo, const int2* q, const long long* sl, int n, const int4* pairs,
-                            const int4* shell_dm, const double* DJ, const double* DK0, const double* DK1, int nk,
-                            const double* ibuf, double* stage, hipStream_t s) {
+hipError_t launch_jk_digest(int combo, const int2* q, const long long* sl, int n, int splb, long long ss,
+                            const int4* pairs, const int4* shell_dm, const double* DJ, const double* DK0,
+                            const double* DK1, int nk, const double* ibuf, double* stage, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   int la, lb, lc, ld;
   switch (combo) {
@@ -318,11 +332,11 @@ hipError_t launch_jk_digest(int combo, const int2* q, const long long* sl, int n
   const int wpb = 4;  // quartets (waves) per workgroup
   dim3 grid((unsigned)std::min<long long>(((long long)n + wpb - 1) / wpb, 65536)), blk(64 * wpb);
   if (nk == 1)
-    hipLaunchKernelGGL((jk_digest_kernel<1>), grid, blk, 0, s, ncart(la), ncart(lb), ncart(lc), ncart(ld), q, sl,
-                       n, pairs, shell_dm, DJ, DK0, DK1, ibuf, stage);
+    hipLaunchKernelGGL((jk_digest_kernel<1>), grid, blk, 0, s, ncart(la), ncart(lb), ncart(lc), ncart(ld), splb,
+                       ss, q, sl, n, pairs, shell_dm, DJ, DK0, DK1, ibuf, stage);
   else
-    hipLaunchKernelGGL((jk_digest_kernel<2>), grid, blk, 0, s, ncart(la), ncart(lb), ncart(lc), ncart(ld), q, sl,
-                       n, pairs, shell_dm, DJ, DK0, DK1, ibuf, stage);
+    hipLaunchKernelGGL((jk_digest_kernel<2>), grid, blk, 0, s, ncart(la), ncart(lb), ncart(lc), ncart(ld), splb,
+                       ss, q, sl, n, pairs, shell_dm, DJ, DK0, DK1, ibuf, stage);
   return hipGetLastError();
 }
 
