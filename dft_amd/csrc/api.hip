@@ -196,6 +196,8 @@ struct dft_plan_s {
   std::vector<int2*> d_gq_swap;
   std::vector<long long*> d_gslots_ab, d_gslots_cd;
   long long* d_g1slots = nullptr;
+  int4* d_g1tasks = nullptr;
+  int ng1tasks = 0;
   long long* d_gbase = nullptr;
   int* d_gcount = nullptr;
   double* d_gstage = nullptr;
@@ -214,7 +216,7 @@ void free_plan(dft_plan p) {
   void* ptrs[] = {p->d_sh,    p->d_exps,  p->d_coefs,   p->d_pairs,    p->d_shell_dm,  p->d_shell_ao,
                   p->d_rec64, p->d_rec32, p->d_stage,   p->d_tg,       p->d_tbase,     p->d_tcount,
                   p->d_half,  p->d_atoms, p->d_shpairs, p->d_mol_base, p->d_mol_nbf,   p->d_grec,
-                  p->d_gpairs, p->d_g1slots, p->d_gbase, p->d_gcount, p->d_gstage, p->d_recT64,
+                  p->d_gpairs, p->d_g1slots, p->d_gbase, p->d_gcount, p->d_gstage, p->d_recT64, p->d_g1tasks,
                   p->d_recT32};
   for (void* x : ptrs) (void)hipFree(x);
   for (hipEvent_t e : p->prof_ev) (void)hipEventDestroy(e);
@@ -904,6 +906,15 @@ int grad_prepare(dft_plan p, const int32_t* shell_atom) {
       s1[b + 2] = slot(cn);
     }
   if ((rc = upload(&p->d_g1slots, s1))) return rc;
+  // 1-e derivative tasks: per shell pair, blocks of g = 64 / (na nb) nuclei
+  std::vector<int4> tasks;
+  for (int k = 0; k < nsp; ++k) {
+    const int nc = ncart_h(p->sh[sp[k].x].l) * ncart_h(p->sh[sp[k].y].l);
+    const int g = 64 / nc;
+    for (int c0 = 0; c0 < natom; c0 += g) tasks.push_back(make_int4(k, c0, std::min(g, natom - c0), nc));
+  }
+  p->ng1tasks = (int)tasks.size();
+  if ((rc = upload(&p->d_g1tasks, tasks))) return rc;
   if ((rc = upload(&p->d_gbase, base))) return rc;
   if ((rc = upload(&p->d_gcount, cnt))) return rc;
   HIP_TRY(hipMalloc(&p->d_gstage, std::max<long long>(cur, 1) * sizeof(double)));
@@ -1335,8 +1346,8 @@ int dft_grad(dft_plan p, const int32_t* shell_atom, const double* P, const doubl
                           p->d_shell_ao, P, Pa, Pb, nbf, p->d_gstage, p->d_boys, s);
     if (e != hipSuccess) return fail(DFT_EHIP, std::string("eri_grad: ") + hipGetErrorString(e));
   }
-  hipError_t e = launch_grad1e(p->d_shpairs, p->nshpairs, p->d_sh, p->d_exps, p->d_coefs, p->natom, p->d_atoms, P, W,
-                               nbf, p->d_g1slots, p->d_gstage, p->d_boys, s);
+  hipError_t e = launch_grad1e(p->d_g1tasks, p->ng1tasks, p->d_shpairs, p->d_sh, p->d_exps, p->d_coefs, p->natom,
+                               p->d_atoms, P, W, nbf, p->d_g1slots, p->d_gstage, p->d_boys, s);
   if (e == hipSuccess) e = launch_grad_reduce(p->natom, p->d_gbase, p->d_gcount, p->d_gstage, grad, s);
   if (e != hipSuccess) return fail(DFT_EHIP, std::string("grad 1e/reduce: ") + hipGetErrorString(e));
   return DFT_OK;

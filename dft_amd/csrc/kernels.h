@@ -130,9 +130,10 @@ hipError_t launch_eri_grad(int la, int lb, int lc, int ld, const int2* quartets,
                            const int4* gpairs, const double* grec, const int* shell_ao, const double* DJ,
                            const double* DK0, const double* DK1, int nbf, double* stage, const double* boys_tab,
                            hipStream_t s);
-hipError_t launch_grad1e(const int2* shell_pairs, int npairs, const ShellDev* shells, const double* exps,
-                         const double* coefs, int natom, const double* atom_xyzq, const double* P, const double* W,
-                         int nbf, const long long* slots, double* stage, const double* boys_tab, hipStream_t s);
+hipError_t launch_grad1e(const int4* tasks /* pair, first nucleus, nuclei, components */, int ntask,
+                         const int2* shell_pairs, const ShellDev* shells, const double* exps, const double* coefs,
+                         int natom, const double* atoms, const double* P, const double* W, int nbf,
+                         const long long* slots, double* stage, const double* boys_tab, hipStream_t s);
 hipError_t launch_grad_reduce(int natom, const long long* base, const int* count, const double* stage, double* grad,
                               hipStream_t s);
 

@@ -309,25 +309,18 @@ __device__ void herm_g(int L, const double* F, double m2a, double X, double Y, d
   for (int h = 0; h < nherm(L); ++h) R0[h] = lev[cur][h];
 }
 
-// One block (64 lanes, lane = component pair) per (shell pair, nucleus).
-// Block (pair, c) writes three xyz contributions: to the atom of A, of B and
-// to nucleus c. For c == 0 the S (with W) and T (with P) derivatives are
-// added to the A and B parts.
-__global__ void grad1e_kernel(const int2* __restrict__ sp, int npairs, const ShellDev* __restrict__ sh,
-                              const double* __restrict__ exps, const double* __restrict__ coefs, int natom,
-                              const double* __restrict__ atoms, const double* __restrict__ P,
-                              const double* __restrict__ W, int nbf, const long long* __restrict__ slots,
-                              double* __restrict__ stage, const double* __restrict__ gtab) {
-  const int pid = blockIdx.x / natom, cn = blockIdx.x % natom;
-  const int lane = threadIdx.x;
-  if (pid >= npairs) return;
-  const int2 pr = sp[pid];
+// Contribution of (shell pair pr, nucleus cn, component pair comp) to the
+// three xyz gradients: of the atom of A, of B and of nucleus cn (acc[0..8]).
+// For cn == 0 the S (with W) and T (with P) derivatives are added to the A
+// and B parts.
+__device__ void grad1e_item(int2 pr, int cn, int comp, const ShellDev* __restrict__ sh,
+                            const double* __restrict__ exps, const double* __restrict__ coefs,
+                            const double* __restrict__ atoms, const double* __restrict__ P,
+                            const double* __restrict__ W, int nbf, const double* __restrict__ gtab, double* acc) {
   const ShellDev A = sh[pr.x], B = sh[pr.y];
-  const int na = ncart(A.l), nb = ncart(B.l);
-  double acc[9];
-  for (int k = 0; k < 9; ++k) acc[k] = 0.0;  // gA xyz, gB xyz, gC xyz
-  if (lane < na * nb) {
-    const int ca = lane / nb, cb = lane % nb;
+  const int nb = ncart(B.l);
+  {
+    const int ca = comp / nb, cb = comp % nb;
     int pa[3], pb[3];
     for (int d = 0; d < 3; ++d) {
       pa[d] = comp_of(A.l, ca, d);
@@ -408,28 +401,53 @@ __global__ void grad1e_kernel(const int2* __restrict__ sp, int npairs, const She
         }
       }
   }
-#pragma unroll
-  for (int k = 0; k < 9; ++k) {
-    double v = acc[k];
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    acc[k] = v;
-  }
-  if (lane == 0) {
-    const long long* sl = slots + 3 * (long long)blockIdx.x;
+}
+
+// One 64-lane block per task {pair, first nucleus c0, nuclei g, components
+// nc}: lane = item * nc + comp for item < g (nucleus c0 + item), so the
+// lanes of a block cover g nuclei of one shell pair. Each (pair, nucleus)
+// sums its components in a fixed order and writes three xyz contributions
+// (atom of A, atom of B, nucleus) to its slots 3 (pair * natom + nucleus).
+__global__ __launch_bounds__(64) void grad1e_kernel(const int4* __restrict__ tasks, const int2* __restrict__ sp,
+                                                    const ShellDev* __restrict__ sh, const double* __restrict__ exps,
+                                                    const double* __restrict__ coefs, int natom,
+                                                    const double* __restrict__ atoms, const double* __restrict__ P,
+                                                    const double* __restrict__ W, int nbf,
+                                                    const long long* __restrict__ slots, double* __restrict__ stage,
+                                                    const double* __restrict__ gtab) {
+  __shared__ double red[64][9];
+  const int4 tk = tasks[blockIdx.x];
+  const int lane = threadIdx.x;
+  const int item = lane / tk.w, comp = lane - item * tk.w;
+  const int cn = tk.y + item;
+  double acc[9];
+  for (int k = 0; k < 9; ++k) acc[k] = 0.0;
+  if (item < tk.z && cn < natom)
+    grad1e_item(sp[tk.x], cn, comp, sh, exps, coefs, atoms, P, W, nbf, gtab, acc);
+  for (int k = 0; k < 9; ++k) red[lane][k] = acc[k];
+  __syncthreads();
+  if (comp == 0 && item < tk.z && cn < natom) {
+    double tot[9];
+    for (int k = 0; k < 9; ++k) {
+      double v = 0.0;
+      for (int c = 0; c < tk.w; ++c) v += red[lane + c][k];
+      tot[k] = v;
+    }
+    const long long* sl = slots + 3 * ((long long)tk.x * natom + cn);
     for (int x = 0; x < 3; ++x) {
-      stage[sl[0] + x] = acc[x];
-      stage[sl[1] + x] = acc[3 + x];
-      stage[sl[2] + x] = acc[6 + x];
+      stage[sl[0] + x] = tot[x];
+      stage[sl[1] + x] = tot[3 + x];
+      stage[sl[2] + x] = tot[6 + x];
     }
   }
 }
 
-hipError_t launch_grad1e(const int2* sp, int npairs, const ShellDev* sh, const double* exps, const double* coefs,
-                         int natom, const double* atoms, const double* P, const double* W, int nbf,
-                         const long long* slots, double* stage, const double* gtab, hipStream_t s) {
-  if (npairs <= 0 || natom <= 0) return hipSuccess;
-  hipLaunchKernelGGL(grad1e_kernel, dim3(npairs * natom), dim3(64), 0, s, sp, npairs, sh, exps, coefs, natom, atoms,
-                     P, W, nbf, slots, stage, gtab);
+hipError_t launch_grad1e(const int4* tasks, int ntask, const int2* sp, const ShellDev* sh, const double* exps,
+                         const double* coefs, int natom, const double* atoms, const double* P, const double* W,
+                         int nbf, const long long* slots, double* stage, const double* gtab, hipStream_t s) {
+  if (ntask <= 0 || natom <= 0) return hipSuccess;
+  hipLaunchKernelGGL(grad1e_kernel, dim3(ntask), dim3(64), 0, s, tasks, sp, sh, exps, coefs, natom, atoms, P, W, nbf,
+                     slots, stage, gtab);
   return hipGetLastError();
 }
 

@@ -265,14 +265,21 @@ __global__ void one_electron_kernel(const int2* __restrict__ sp, int npairs, con
                                     const double* __restrict__ atoms, double* __restrict__ S,
                                     double* __restrict__ T, double* __restrict__ V,
                                     const double* __restrict__ gtab) {
+  // lane = item * ncomp + comp: the block's lanes cover the component
+  // pairs of one shell pair and g = 64 / ncomp interleaved subsets of the
+  // nuclei (lane item takes nuclei item, item + g, ...); the V parts are
+  // summed over the subsets in a fixed order through LDS
+  __shared__ double vred[64];
   const int pidx = blockIdx.x;
   const int lane = threadIdx.x;
   if (pidx >= npairs) return;
   const int2 pr = sp[pidx];
   const ShellDev A = sh[pr.x], B = sh[pr.y];
   const int na = ncart(A.l), nb = ncart(B.l);
-  if (lane >= na * nb) return;
-  const int ca = lane / nb, cb = lane % nb;
+  const int ncomp = na * nb, g = 64 / ncomp;
+  const int item = lane / ncomp, comp = lane - item * ncomp;
+  const bool active = item < g;
+  const int ca = (active ? comp : 0) / nb, cb = (active ? comp : 0) % nb;
   int pa[3], pb[3];
   for (int d = 0; d < 3; ++d) { pa[d] = comp_of(A.l, ca, d); pb[d] = comp_of(B.l, cb, d); }
   const double Ac[3] = {A.x, A.y, A.z}, Bc[3] = {B.x, B.y, B.z};
@@ -280,7 +287,7 @@ __global__ void one_electron_kernel(const int2* __restrict__ sp, int npairs, con
   double nang = 1.0;
   for (int d = 0; d < 3; ++d) nang /= sqrt(dfac[pa[d]] * dfac[pb[d]]);
   double s_acc = 0.0, t_acc = 0.0, v_acc = 0.0;
-  for (int i = 0; i < A.nprim; ++i) {
+  for (int i = 0; i < (active ? A.nprim : 0); ++i) {
     for (int j = 0; j < B.nprim; ++j) {
       const double a = exps[A.prim_off + i], b = exps[B.prim_off + j];
       const double cc = coefs[A.prim_off + i] * coefs[B.prim_off + j] * nang;
@@ -306,7 +313,7 @@ __global__ void one_electron_kernel(const int2* __restrict__ sp, int npairs, con
       const int L = A.l + B.l;
       const double P[3] = {(a * A.x + b * B.x) / p, (a * A.y + b * B.y) / p, (a * A.z + b * B.z) / p};
       double vsum = 0.0;
-      for (int c = A.atom_begin; c < A.atom_end; ++c) {
+      for (int c = A.atom_begin + item; c < A.atom_end; c += g) {
         const double X = P[0] - atoms[4 * c], Y = P[1] - atoms[4 * c + 1], Z = P[2] - atoms[4 * c + 2];
         double F[9], R0[nherm(8)];
         boys_rt(L, p * (X * X + Y * Y + Z * Z), gtab, F);
@@ -320,6 +327,11 @@ __global__ void one_electron_kernel(const int2* __restrict__ sp, int npairs, con
       v_acc += cc * vsum * (-2.0 * M_PI / p);
     }
   }
+  vred[lane] = v_acc;
+  __syncthreads();
+  if (!active || item != 0) return;
+  v_acc = 0.0;
+  for (int it = 0; it < g; ++it) v_acc += vred[it * ncomp + comp];
   const long long r = A.ao + ca, c = B.ao + cb;
   const long long nbf = A.nbf, rc = A.mbase + r * nbf + c, cr = A.mbase + c * nbf + r;
   S[rc] = s_acc;

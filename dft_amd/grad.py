@@ -65,13 +65,17 @@ def _shells(Z, xyz, basis):
     return build_shells_from_entries(basis, xyz)  # per-atom entries
 
 
-def energy_and_gradient(Z, xyz, basis, charge=0, mult=1, schwarz_tau=1e-12, e_tol=1e-11, p_tol=1e-9):
-    """(E, dE/dR [natom, 3], SCF result) at bohr coordinates ``xyz``."""
+def energy_and_gradient(Z, xyz, basis, charge=0, mult=1, schwarz_tau=1e-12, e_tol=1e-11, p_tol=1e-9, guess=None):
+    """(E, dE/dR [natom, 3], SCF result) at bohr coordinates ``xyz``;
+    ``guess`` = (P_a, P_b) starts the SCF from those densities."""
     xyz = np.asarray(xyz, dtype=np.float64).reshape(-1, 3)
     shells = _shells(Z, xyz, basis)
     N_a, N_b = electron_counts(Z, charge, mult)
     plan = Plan(shells, Z, xyz, schwarz_tau=schwarz_tau, fp32_tau=0.0, uhf=N_a != N_b)
-    res = scf_diis(plan, Z, xyz, N_a, N_b, e_tol=e_tol, p_tol=p_tol, max_count=300)
+    # undamped DIIS from the first iteration: the gradient needs a tightly
+    # converged density, which damping (the reference's schedule) slows down
+    res = scf_diis(plan, Z, xyz, N_a, N_b, e_tol=e_tol, p_tol=p_tol, max_count=300, guess=guess, rate=1.0,
+                   diis_threshold=float("inf"))
     if not res.converged:
         raise RuntimeError("SCF did not converge; the analytic gradient needs a stationary density")
     g = electronic_gradient(plan, res.C_a, res.eps_a, N_a, res.C_b, res.eps_b, N_b).cpu().numpy()
@@ -79,35 +83,43 @@ def energy_and_gradient(Z, xyz, basis, charge=0, mult=1, schwarz_tau=1e-12, e_to
 
 
 class SCFEnergy(torch.autograd.Function):
-    """E(R) with an analytic backward (derivative-ERI kernels)."""
+    """E(R) with an analytic backward (derivative-ERI kernels). ``state``
+    (a dict or None) carries the converged densities from one call to the
+    next as the SCF guess (geometry optimisation, finite differences)."""
 
     @staticmethod
-    def forward(ctx, coords, Z, basis, charge, mult):
-        E, g, _ = energy_and_gradient(Z, coords.detach().cpu().numpy(), basis, charge, mult)
+    def forward(ctx, coords, Z, basis, charge, mult, state=None):
+        guess = state.get("P") if state is not None else None
+        E, g, res = energy_and_gradient(Z, coords.detach().cpu().numpy(), basis, charge, mult, guess=guess)
+        if state is not None:
+            state["P"] = (res.P_a, res.P_b)
+            state["iterations"] = res.iterations
         ctx.save_for_backward(torch.as_tensor(g, dtype=coords.dtype, device=coords.device))
         return coords.new_tensor(E)
 
     @staticmethod
     def backward(ctx, grad_out):
         (g,) = ctx.saved_tensors
-        return grad_out * g, None, None, None, None
+        return grad_out * g, None, None, None, None, None
 
 
-def scf_energy(coords: torch.Tensor, Z, basis, charge=0, mult=1) -> torch.Tensor:
-    return SCFEnergy.apply(coords, Z, basis, charge, mult)
+def scf_energy(coords: torch.Tensor, Z, basis, charge=0, mult=1, state=None) -> torch.Tensor:
+    return SCFEnergy.apply(coords, Z, basis, charge, mult, state)
 
 
 def steepest_descent(Z, xyz, basis, steps=10, step_size=0.2, charge=0, mult=1):
     """Fixed-step steepest descent in bohr; returns per-step (E, |g|max, s)."""
     coords = torch.tensor(np.asarray(xyz, dtype=np.float64).reshape(-1, 3), requires_grad=True)
     history = []
+    state = {}  # previous step's densities seed the next SCF
     for _ in range(steps):
         t0 = time.perf_counter()
-        E = scf_energy(coords, Z, basis, charge, mult)
+        E = scf_energy(coords, Z, basis, charge, mult, state)
         E.backward()
         with torch.no_grad():
             g = coords.grad.clone()
             coords -= step_size * g
         coords.grad = None
-        history.append((float(E), float(g.abs().max()), time.perf_counter() - t0))
+        history.append((float(E.detach()), float(g.abs().max()), time.perf_counter() - t0,
+                        state.get("iterations")))
     return coords.detach().numpy(), history

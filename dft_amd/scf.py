@@ -115,6 +115,16 @@ def _setup(plan: Plan, Z, centers, A=None):
     return S, H, X, nuclear_repulsion(Z, centers)
 
 
+def _gram(R):
+    """B_ij = sum_ab R_i[a,b] R_j[a,b] for a stack of residuals. Small stacks
+    go through one elementwise product + reduction (a GEMM with K = n^2 and
+    a 10 x 10 output runs on a handful of workgroups)."""
+    h, n = R.shape[0], R.shape[1]
+    if h * h * n * n <= (1 << 22):
+        return (R.unsqueeze(1) * R.unsqueeze(0)).sum(dim=(2, 3))
+    return torch.einsum("iab,jab->ij", R, R)
+
+
 def scf_plain(plan: Plan, Z, centers, N_a, N_b, mode="fp64", e_tol=1e-8, p_tol=1e-6, max_count=100,
               p_rms_tol=None, A=None):
     """Plain UHF loop of main.py:753-828; RHF fast path when N_a == N_b.
@@ -166,7 +176,7 @@ def scf_plain(plan: Plan, Z, centers, N_a, N_b, mode="fp64", e_tol=1e-8, p_tol=1
 
 
 def scf_diis(plan: Plan, Z, centers, N_a, N_b, mode="fp64", rate=0.4, diis_threshold=1e-3, history=10,
-             e_tol=1e-8, p_tol=1e-6, max_count=500, A=None):
+             e_tol=1e-8, p_tol=1e-6, max_count=500, A=None, guess=None):
     """Damped + DIIS UHF loop of shelled.py:1103-1292.
 
     Damping rate 0.4, divided by 1.1 every fourth strike (an iteration whose
@@ -174,11 +184,17 @@ def scf_diis(plan: Plan, Z, centers, N_a, N_b, mode="fp64", rate=0.4, diis_thres
     first iteration with dE < 1e-3, history 10, no Lagrange row and the
     coefficients normalised to sum 1 (shelled.py:1196-1201). Unlike the
     reference it stops after ``max_count`` iterations with converged False.
+    ``guess`` = (P_a, P_b) starts from those densities instead of the core
+    Hamiltonian (e.g. the converged densities of a nearby geometry). For RHF
+    (N_a == N_b, P_a == P_b throughout) one diagonalisation serves both spins.
     """
     S, H, X, E_NN = _setup(plan, Z, centers, A)
-    _, Cp = torch.linalg.eigh(X.T @ H @ X)
-    C = X @ Cp
-    P_a, P_b = init_UHF_density(C, C, N_a, N_b)
+    if guess is not None:
+        P_a, P_b = (torch.as_tensor(g, dtype=torch.float64, device=plan.device).clone() for g in guess)
+    else:
+        _, Cp = torch.linalg.eigh(X.T @ H @ X)
+        C = X @ Cp
+        P_a, P_b = init_UHF_density(C, C, N_a, N_b)
     rhf = N_a == N_b and not plan.uhf
     if not rhf and not plan.uhf:
         raise ValueError("N_a != N_b needs a plan built with uhf=True (two exchange matrices)")
@@ -202,7 +218,7 @@ def scf_diis(plan: Plan, Z, centers, N_a, N_b, mode="fp64", rate=0.4, diis_thres
         if use_diis:
             Ra = torch.stack(hist["Ra"])
             Rb = torch.stack(hist["Rb"])
-            B = torch.einsum("iab,jab->ij", Ra, Ra) + torch.einsum("iab,jab->ij", Rb, Rb)
+            B = _gram(Ra) + _gram(Rb)
             c = torch.linalg.solve(B, torch.ones(B.shape[0], dtype=B.dtype, device=B.device))
             c = c / c.sum()
             Fa = torch.einsum("i,iab->ab", c, torch.stack(hist["Fa"]))
@@ -210,7 +226,7 @@ def scf_diis(plan: Plan, Z, centers, N_a, N_b, mode="fp64", rate=0.4, diis_thres
         E_new = float(0.5 * torch.sum(P * H + P_a * Fa + P_b * Fb)) + E_NN
         dE = float("inf") if E_total is None else abs(E_new - E_total)
         _, Ca = torch.linalg.eigh(X.T @ Fa @ X)
-        _, Cb = torch.linalg.eigh(X.T @ Fb @ X)
+        Cb = Ca if rhf else torch.linalg.eigh(X.T @ Fb @ X)[1]
         Ca, Cb = X @ Ca, X @ Cb
         Pa_new, Pb_new = UHF_density(Ca, Cb, N_a, N_b, P_a, P_b, rate)
         dP = float(torch.maximum((Pa_new - P_a).abs().max(), (Pb_new - P_b).abs().max()))
@@ -231,7 +247,7 @@ def scf_diis(plan: Plan, Z, centers, N_a, N_b, mode="fp64", rate=0.4, diis_thres
             Fb = H + J - Kb
             E_fin = float(0.5 * torch.sum(P * H + P_a * Fa + P_b * Fb)) + E_NN
             ea, Ca = torch.linalg.eigh(X.T @ Fa @ X)
-            eb, Cb = torch.linalg.eigh(X.T @ Fb @ X)
+            eb, Cb = (ea, Ca) if rhf else torch.linalg.eigh(X.T @ Fb @ X)
             return SCFResult(E_fin, dE < e_tol and dP < p_tol, count, P_a, P_b, Fa, Fb, X @ Ca, X @ Cb, ea, eb,
                              fock.times, [], energies)
         E_total = E_new

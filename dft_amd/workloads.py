@@ -184,7 +184,9 @@ def alkane_gradient(fd_coords=12, sd_steps=10, h=1e-4):
         p[a, x] += h
         m = xyz.copy()
         m[a, x] -= h
-        fd = (energy_and_gradient(Z, p, basis)[0] - energy_and_gradient(Z, m, basis)[0]) / (2 * h)
+        guess = (res.P_a, res.P_b)  # the reference geometry's densities seed both SCFs
+        fd = (energy_and_gradient(Z, p, basis, guess=guess)[0]
+              - energy_and_gradient(Z, m, basis, guess=guess)[0]) / (2 * h)
         fd_err.append(abs(fd - g[a, x]))
     t_fd = time.perf_counter() - t0
     t0 = time.perf_counter()
@@ -194,4 +196,5 @@ def alkane_gradient(fd_coords=12, sd_steps=10, h=1e-4):
             "s_per_gradient_call_total": t_call, "ms_gradient_kernels": ms_grad,
             "fd_coords_checked": len(coords), "fd_max_abs_err_Eh_per_bohr": float(max(fd_err)),
             "fd_seconds": t_fd, "sd_steps": sd_steps, "sd_seconds": t_sd,
-            "sd_energies": [hh[0] for hh in hist], "sd_s_per_step": [hh[2] for hh in hist]}
+            "sd_energies": [hh[0] for hh in hist], "sd_s_per_step": [hh[2] for hh in hist],
+            "sd_scf_iterations": [hh[3] for hh in hist]}
