@@ -27,6 +27,9 @@ constexpr long long SPLIT_LANES = 65536;
 constexpr long long SPLIT_LANES_HEAVY = 262144;
 // auxiliary streams per plan for concurrent class launches
 constexpr int JK_AUX = 3;
+// screened quartets below which a plan's class launches run concurrently by
+// default (larger builds fill the chip with each launch and run faster serially)
+constexpr long long CONCURRENT_QUARTETS = 8000000;
 
 }  // namespace
 
@@ -136,6 +139,10 @@ struct Combo {
 struct dft_plan_s {
   int nsh = 0, natom = 0, nk = 1, nmol = 1;
   bool split = true;             // primitive split of small lane-kernel classes
+  // class launches of a build on several streams: helps builds whose class
+  // launches cannot fill the chip one at a time; large builds run faster
+  // serially (set by build size, then by dft_plan_tune)
+  bool concurrent = true;
   long long nelem = 0;           // packed matrix elements (sum of nbf_m^2)
   int max_nbf = 0;
   double tau = 0, tau32 = 0;
@@ -738,7 +745,8 @@ int jk_enqueue(dft_plan p, const double* P, const double* Pa, const double* Pb, 
     mark(slot, 1, s);
   }
   std::vector<hipStream_t> st(1, s);
-  for (hipStream_t a : p->aux) st.push_back(a);
+  if (p->concurrent)
+    for (hipStream_t a : p->aux) st.push_back(a);
   const bool fork = ls.size() > 1 && st.size() > 1;
   if (fork) {
     HIP_TRY(hipEventRecord(p->ev_fork, s));
@@ -987,6 +995,11 @@ int validate_and_build(int nmol, const int32_t* mol_nsh, const int32_t* mol_nato
   p->tau32 = fp32_tau;
   int rc = build_plan(p, in);
   if (!rc && !(flags & DFT_PLAN_SERIAL)) rc = create_streams(p);
+  if (!rc) {
+    long long nq = 0;
+    for (const Combo& c : p->combos) nq += (long long)c.q.size();
+    p->concurrent = !(flags & DFT_PLAN_SERIAL) && nq < CONCURRENT_QUARTETS;
+  }
   if (rc) {
     free_plan(p);
     return rc;
@@ -1125,7 +1138,7 @@ int dft_plan_tune(dft_plan p, const double* P, const double* Pa, const double* P
     }
   int rc = DFT_OK;
   for (size_t ci = 0; ci < p->combos.size() && rc == DFT_OK; ++ci) {
-    if (combo >= 0 && (int)ci != combo) continue;
+    if (combo == -2 || (combo >= 0 && (int)ci != combo)) continue;
     Combo& c = p->combos[ci];
     const long long ntot = (long long)c.q.size();
     for (int part = 0; part < 2 && rc == DFT_OK; ++part) {
@@ -1170,6 +1183,29 @@ int dft_plan_tune(dft_plan p, const double* P, const double* Pa, const double* P
         c.shape32 = best_sh;
       }
     }
+  }
+  // whole builds with the class launches serial and concurrent (FP64 mode,
+  // whole-plan tuning only); keep the faster
+  if (rc == DFT_OK && combo < 0 && !p->aux.empty()) {
+    float tbest[2] = {1e30f, 1e30f};
+    for (int conc = 0; conc < 2 && rc == DFT_OK; ++conc) {
+      p->concurrent = conc == 1;
+      std::vector<double*> tmp(3, nullptr);
+      for (int m = 0; m < 1 + p->nk && rc == DFT_OK; ++m)
+        if (hipMalloc(&tmp[m], std::max<long long>(p->nelem, 1) * sizeof(double)) != hipSuccess)
+          rc = fail(DFT_EHIP, "tune: hipMalloc");
+      for (int r = 0; r <= reps && rc == DFT_OK; ++r) {
+        (void)hipEventRecord(e0, s);
+        rc = jk_enqueue(p, P, Pa, Pb, tmp[0], tmp[1], tmp[2], DFT_FP64, s, false);
+        (void)hipEventRecord(e1, s);
+        float ms = 0.f;
+        if (rc == DFT_OK && hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess &&
+            r > 0)
+          tbest[conc] = std::min(tbest[conc], ms);
+      }
+      for (double* x : tmp) (void)hipFree(x);
+    }
+    p->concurrent = tbest[1] < tbest[0];
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

@@ -151,11 +151,17 @@ __device__ __forceinline__ void boys_fp64(double T, const double* __restrict__ t
   }
 }
 
-// FP32 Boys: the table seed F_L(T_k) and its Taylor step stay FP64; the
-// downward/upward recursion runs in FP32 with the native exp.
+// Above boys_tasym32(L) the asymptotic form Gamma(m+1/2) / (2 T^{m+1/2})
+// is exact to FP32 resolution for every m <= L (the neglected erfc / e^{-T}
+// terms are below 6e-8 relative, checked against quadrature for L <= 8).
+__host__ __device__ constexpr float boys_tasym32(int L) { return 16.0f + 2.5f * L; }
+
+// FP32 Boys: below boys_tasym32(L) the table seed F_L(T_k) and its Taylor
+// step stay FP64 and the downward recursion runs in FP32 with the native
+// exp; above it the asymptotic form in FP32 (no table access).
 template <int L>
 __device__ __forceinline__ void boys_fp32(float Tf, const double* __restrict__ tab, float (&F)[L + 1]) {
-  if (Tf < (float)BOYS_TMAX) {
+  if (Tf < boys_tasym32(L)) {
     double T = (double)Tf;
     int k = __double2int_rn(T * BOYS_INV_DT);
     double d = -(T - k * BOYS_DT);
@@ -172,7 +178,6 @@ __device__ __forceinline__ void boys_fp32(float Tf, const double* __restrict__ t
       for (int m = L - 1; m >= 0; --m) F[m] = fmaf(t2, F[m + 1], e) * (float)inv_odd(m);
     }
   } else {
-    // e^{-T} < 5e-18 is below FP32 resolution of every F_m here
     const float rs = rsqrt_nr(Tf);
     F[0] = 0.886226925f * rs;
     if constexpr (L > 0) {

@@ -298,6 +298,10 @@ def tune(plan, P, Pa, Pb=None, reps=3, log=None):
                                           6 * ncombo, C.c_void_p(s.cuda_stream)))
         if log:
             log(f"tuned combination {c}: {list(rep[6 * c:6 * c + 6])}")
+    # serial or concurrent class launches, from timed whole builds
+    _lib.check(plan.lib.dft_plan_tune(plan.handle, P.data_ptr(), Pa.data_ptr(),
+                                      Pb.data_ptr() if Pb is not None else None, int(reps), -2, rep,
+                                      6 * ncombo, C.c_void_p(s.cuda_stream)))
     torch.cuda.synchronize(plan.device)
     return np.array(rep[:], dtype=np.int64).reshape(ncombo, 6)
 

@@ -124,8 +124,10 @@ int dft_graph_destroy(dft_graph graph);
  * limit) and grid caps 512..8192 (median of reps runs each) and keeps the
  * fastest. report (capacity report_cap ints) receives 6 ints per
  * combination: FP64 {0, block, cap}, FP32 {0, block, cap} (the first int of
- * each triple is reserved). combo >= 0 tunes only that combination (-1:
- * all). Overwrites the plan's staging (J/K are not written).
+ * each triple is reserved). combo >= 0 tunes only that combination; -1 tunes
+ * all of them and then chooses between running the class launches of a
+ * build serially or concurrently (timing whole FP64 builds); -2 makes only
+ * that last choice. Overwrites the plan's staging (J/K are not written).
  * No counterpart in the reference.
  */
 int dft_plan_tune(dft_plan plan, const double* P, const double* Pa, const double* Pb, int reps, int combo,
