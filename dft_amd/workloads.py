@@ -198,3 +198,26 @@ def alkane_gradient(fd_coords=12, sd_steps=10, h=1e-4):
             "fd_seconds": t_fd, "sd_steps": sd_steps, "sd_seconds": t_sd,
             "sd_energies": [hh[0] for hh in hist], "sd_s_per_step": [hh[2] for hh in hist],
             "sd_scf_iterations": [hh[3] for hh in hist]}
+
+
+WORKLOADS = ("water_rhf", "alkane_spd_mixed", "water_batch_scf", "alkane_gradient")
+
+
+def main(argv=None):
+    """``python -m dft_amd.workloads NAME [NAME ...]``: run the named
+    workloads (default: all) and print one JSON line per workload."""
+    import json
+    import sys
+    names = list(argv if argv is not None else sys.argv[1:]) or list(WORKLOADS)
+    for name in names:
+        if name not in WORKLOADS:
+            raise SystemExit(f"unknown workload {name!r}; choose from {', '.join(WORKLOADS)}")
+        t0 = time.perf_counter()
+        log(f"== {name}")
+        r = globals()[name]()
+        r["wall_s"] = time.perf_counter() - t0
+        print(json.dumps({name: r}, default=str), flush=True)
+
+
+if __name__ == "__main__":
+    main()
