@@ -143,3 +143,20 @@ def test_primitive_split_matches_unsplit(torch_cuda, uhf):
             if x is None:
                 continue
             assert float((x - y).abs().max()) < 1e-12
+
+
+def test_serial_and_concurrent_launches_bitwise_equal(torch_cuda):
+    """The class launches of a build write disjoint staging slots and the
+    reduction order is fixed, so running them on one stream or spread over
+    four gives bitwise-identical J/K (s/p and the heavy d classes)."""
+    torch = torch_cuda
+    from dft_amd.engine import Plan
+    for sh, Z, xyz in _mols():
+        a = Plan(sh, Z, xyz, schwarz_tau=1e-12, fp32_tau=1e-4)
+        b = Plan(sh, Z, xyz, schwarz_tau=1e-12, fp32_tau=1e-4, serial=True)
+        rng = np.random.default_rng(3)
+        Pa = _dens(a.nbf, rng)
+        for mode in ("fp64", "mixed"):
+            ja, ka, _ = a.jk(2 * Pa, Pa, mode=mode)
+            jb, kb, _ = b.jk(2 * Pa, Pa, mode=mode)
+            assert torch.equal(ja, jb) and torch.equal(ka, kb), mode

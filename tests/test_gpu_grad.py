@@ -109,3 +109,18 @@ def test_autograd_function(torch_cuda):
     E2, g, _ = energy_and_gradient(Z, xyz, basis)
     assert abs(float(E) - E2) < 1e-10
     assert np.allclose(coords.grad.numpy(), g, atol=1e-12)
+
+
+def test_scf_guess_gives_same_energy_and_gradient(torch_cuda):
+    """Seeding the SCF with the densities of a nearby geometry (as the
+    steepest-descent driver does) converges to the same state."""
+    from dft_amd.basis import load_basis
+    from dft_amd.geometry import water
+    from dft_amd.grad import energy_and_gradient
+    Z, xyz = water()
+    basis = load_basis()
+    _, _, res0 = energy_and_gradient(Z, xyz + 0.02, basis)
+    E1, g1, _ = energy_and_gradient(Z, xyz, basis)
+    E2, g2, res2 = energy_and_gradient(Z, xyz, basis, guess=(res0.P_a, res0.P_b))
+    assert abs(E1 - E2) < 1e-9 and np.abs(g1 - g2).max() < 1e-7
+    assert res2.iterations <= 30
