@@ -24,13 +24,14 @@ __host__ __device__ __forceinline__ void static_for(F&& f) {
 }
 
 // ----------------------------------------------------- reciprocal sqrt
-// FP64: hardware v_rsq_f64 estimate refined by two Newton steps
-// r <- r (3 - x r^2) / 2 (each step squares the relative error), instead of
-// the IEEE sqrt + divide sequences. FP32: the hardware estimate (~1 ulp).
+// FP64: hardware v_rsq_f64 estimate (measured on gfx950: 5.2e-8 relative
+// over x in [e^-6, e^12]) refined by one Newton step r <- r (3 - x r^2) / 2,
+// which squares the error (measured 4.3e-15 relative, ~20 ulp: far below
+// what the integrals need), instead of the IEEE sqrt + divide sequences.
+// FP32: the hardware estimate (~1 ulp).
 __device__ __forceinline__ double rsqrt_nr(double x) {
   double r = __builtin_amdgcn_rsq(x);
   double h = 0.5 * x;
-  r = r * fma(-h * r, r, 1.5);
   r = r * fma(-h * r, r, 1.5);
   return r;
 }
