@@ -25,6 +25,9 @@ namespace {
 constexpr long long SPLIT_LANES = 65536;
 // the same for the column kernel of the heavy classes (longer lanes)
 constexpr long long SPLIT_LANES_HEAVY = 262144;
+// FP32 records drop primitive pairs whose summed Schwarz bounds stay within
+// this fraction of the shell pair's (FP32 resolution is 6e-8)
+constexpr double FP32_PRIM_REL = 1e-7;
 // auxiliary streams per plan for concurrent class launches
 constexpr int JK_AUX = 3;
 // screened quartets below which a plan's class launches run concurrently by
@@ -152,6 +155,7 @@ struct dft_plan_s {
   std::vector<int> in2int;       // input shell -> internal shell
   std::vector<double> exps, coefs;
   std::vector<int4> pairs;       // (sh_a, sh_b, rec_off, nprim_pairs), class-sorted
+  std::vector<int4> pairs32;     // the same for the FP32 records (looser primitive cut)
   std::vector<int2> prim_list;   // surviving primitive pairs (ia, ib) of all pairs
   std::vector<int> pair_prim_begin;  // first prim_list entry of each pair
   std::vector<int> pair_pc;      // pair class per pair
@@ -166,6 +170,7 @@ struct dft_plan_s {
   double* d_exps = nullptr;
   double* d_coefs = nullptr;
   int4* d_pairs = nullptr;
+  int4* d_pairs32 = nullptr;
   int4* d_shell_dm = nullptr;
   int* d_shell_ao = nullptr;
   double* d_rec64 = nullptr;
@@ -173,7 +178,7 @@ struct dft_plan_s {
   // interleaved copies of the records, one block per pair class (lane kernels)
   double* d_recT64 = nullptr;
   float* d_recT32 = nullptr;
-  long long cls_base[NPC] = {};
+  long long cls_base[NPC] = {}, cls_base32[NPC] = {};
   int cls_first[NPC] = {}, cls_n[NPC] = {};
   double* d_stage = nullptr;
   Target* d_tg = nullptr;
@@ -224,6 +229,7 @@ void free_plan(dft_plan p) {
                   p->d_rec64, p->d_rec32, p->d_stage,   p->d_tg,       p->d_tbase,     p->d_tcount,
                   p->d_half,  p->d_atoms, p->d_shpairs, p->d_mol_base, p->d_mol_nbf,   p->d_grec,
                   p->d_gpairs, p->d_g1slots, p->d_gbase, p->d_gcount, p->d_gstage, p->d_recT64, p->d_g1tasks,
+                  p->d_pairs32,
                   p->d_recT32};
   for (void* x : ptrs) (void)hipFree(x);
   for (hipEvent_t e : p->prof_ev) (void)hipEventDestroy(e);
@@ -238,11 +244,58 @@ void free_plan(dft_plan p) {
 
 long long pair_key(int i, int j) { return ((long long)i << 32) | (unsigned)j; }
 
-// Raw contracted blocks for a list of (bra pair, ket pair) in one combo,
-// synchronous, host result.
 RecView rec_view(dft_plan p, int ci, bool f32);
 
-int raw_blocks(dft_plan p, int combo, const std::vector<int2>& q, std::vector<double>& out, int blk) {
+// Interleaved record blocks per pair class (prec 0: FP64 records and pair
+// table, 1: FP32 ones): element f of primitive pair j of class-local pair u
+// at base + (j * rs + f) * n + u, n the class's pair count.
+int interleave_records(dft_plan p, int prec) {
+  const std::vector<int4>& prs = prec ? p->pairs32 : p->pairs;
+  long long* cbase = prec ? p->cls_base32 : p->cls_base;
+  std::vector<int4> map;
+  long long tot = 0;
+  for (int pc = 0; pc < NPC; ++pc) {
+    int la, lb;
+    pc_moments(pc, la, lb);
+    const int rs = rec_size(la, lb), n = p->cls_n[pc], f0 = p->cls_first[pc];
+    int mp = 0;
+    for (int u = 0; u < n; ++u) mp = std::max(mp, prs[f0 + u].w);
+    cbase[pc] = tot;
+    for (int u = 0; u < n; ++u) {
+      const int4 pr = prs[f0 + u];
+      for (int j = 0; j < pr.w; ++j)
+        map.push_back(make_int4(pr.z + j * rs, (int)(tot + (long long)j * rs * n + u), rs, n));
+    }
+    tot += (long long)mp * rs * n;
+    if (tot * (long long)sizeof(double) > 0xffffffffLL)
+      return fail(DFT_EINVAL, "interleaved pair records exceed 4 GiB");
+  }
+  int4* dmap = nullptr;
+  int rc = upload(&dmap, map);
+  if (rc) return rc;
+  hipError_t e;
+  if (prec == 0) {
+    e = hipMalloc(&p->d_recT64, std::max<long long>(tot, 1) * sizeof(double));
+    if (e == hipSuccess)
+      e = launch_interleave_records(dmap, (int)map.size(), p->d_rec64, nullptr, p->d_recT64, nullptr, nullptr);
+  } else {
+    e = hipMalloc(&p->d_recT32, std::max<long long>(tot, 1) * sizeof(float));
+    if (e == hipSuccess)
+      e = launch_interleave_records(dmap, (int)map.size(), nullptr, p->d_rec32, nullptr, p->d_recT32, nullptr);
+  }
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  (void)hipFree(dmap);
+  if (e != hipSuccess) return fail(DFT_EHIP, std::string("record interleave: ") + hipGetErrorString(e));
+  return DFT_OK;
+}
+
+// Raw contracted blocks for a list of (bra pair, ket pair) in one combo,
+// synchronous, host result. By default over the plan's FP64 pair table;
+// pairs / rv name another table of the same FP64 records (the primitive
+// Schwarz pass) and, for the column-kernel classes, its interleaved view.
+int raw_blocks(dft_plan p, int combo, const std::vector<int2>& q, std::vector<double>& out, int blk,
+               const int4* pairs = nullptr, const RecView* rvp = nullptr) {
+  if (!pairs) pairs = p->d_pairs;
   out.assign(q.size() * (size_t)blk, 0.0);
   if (q.empty()) return DFT_OK;
   int2* dq = nullptr;
@@ -255,10 +308,10 @@ int raw_blocks(dft_plan p, int combo, const std::vector<int2>& q, std::vector<do
     return fail(DFT_EHIP, std::string("hipMalloc: ") + hipGetErrorString(e));
   }
   if (lane_kernel_combo(combo)) {
-    e = launch_eri_raw(combo, dq, (long long)q.size(), p->d_pairs, p->d_rec64, dout, p->d_boys, nullptr);
+    e = launch_eri_raw(combo, dq, (long long)q.size(), pairs, p->d_rec64, dout, p->d_boys, nullptr);
   } else {
-    const RecView rv = rec_view(p, combo, false);
-    e = launch_eri_chunk(combo, false, dq, 0, (int)q.size(), 1, p->d_pairs, rv, dout, 1, blk, 0, p->d_boys, 256,
+    const RecView rv = rvp ? *rvp : rec_view(p, combo, false);
+    e = launch_eri_chunk(combo, false, dq, 0, (int)q.size(), 1, pairs, rv, dout, 1, blk, 0, p->d_boys, 256,
                          1 << 20, nullptr);
   }
   if (e == hipSuccess) e = hipMemcpy(out.data(), dout, out.size() * sizeof(double), hipMemcpyDeviceToHost);
@@ -396,45 +449,14 @@ int build_plan(dft_plan p, const Input& in) {
     if ((rc = upload(&dpp, pp))) return rc;
     if ((rc = upload(&doff, pp_off))) return rc;
     HIP_TRY(hipMalloc(&p->d_rec64, std::max<long long>(roff, 1) * sizeof(double)));
-    HIP_TRY(hipMalloc(&p->d_rec32, std::max<long long>(roff, 1) * sizeof(float)));
     hipError_t e = launch_pair_records(dpp, (int)pp.size(), doff, p->d_sh, p->d_exps, p->d_coefs, p->d_rec64,
-                                       p->d_rec32, nullptr);
+                                       nullptr, nullptr);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     (void)hipFree(dpp);
     (void)hipFree(doff);
     if (e != hipSuccess) return fail(DFT_EHIP, std::string("pair pre-pass: ") + hipGetErrorString(e));
   }
-  // ---- interleaved record blocks per pair class: element f of primitive
-  // pair j of class-local pair u at base + (j * rs + f) * n + u
-  {
-    std::vector<int4> map;
-    long long tot = 0;
-    for (int pc = 0; pc < NPC; ++pc) {
-      int la, lb;
-      pc_moments(pc, la, lb);
-      const int rs = rec_size(la, lb), n = p->cls_n[pc], f0 = p->cls_first[pc];
-      int mp = 0;
-      for (int u = 0; u < n; ++u) mp = std::max(mp, p->pairs[f0 + u].w);
-      p->cls_base[pc] = tot;
-      for (int u = 0; u < n; ++u) {
-        const int4 pr = p->pairs[f0 + u];
-        for (int j = 0; j < pr.w; ++j)
-          map.push_back(make_int4(pr.z + j * rs, (int)(tot + (long long)j * rs * n + u), rs, n));
-      }
-      tot += (long long)mp * rs * n;
-      if (tot * (long long)sizeof(double) > 0xffffffffLL)
-        return fail(DFT_EINVAL, "interleaved pair records exceed 4 GiB");
-    }
-    int4* dmap = nullptr;
-    if ((rc = upload(&dmap, map))) return rc;
-    HIP_TRY(hipMalloc(&p->d_recT64, std::max<long long>(tot, 1) * sizeof(double)));
-    HIP_TRY(hipMalloc(&p->d_recT32, std::max<long long>(tot, 1) * sizeof(float)));
-    hipError_t e = launch_interleave_records(dmap, (int)map.size(), p->d_rec64, p->d_rec32, p->d_recT64,
-                                             p->d_recT32, nullptr);
-    if (e == hipSuccess) e = hipDeviceSynchronize();
-    (void)hipFree(dmap);
-    if (e != hipSuccess) return fail(DFT_EHIP, std::string("record interleave: ") + hipGetErrorString(e));
-  }
+  if ((rc = interleave_records(p, 0))) return rc;
   // ---- atoms and shell pairs for the 1-e kernel
   std::vector<double> at(4 * (size_t)p->natom);
   for (int a = 0; a < p->natom; ++a) {
@@ -474,6 +496,118 @@ int build_plan(dft_plan p, const Input& in) {
       p->Q[ids[k]] = std::sqrt(mx);
     }
   }
+  // ---- FP32 primitive lists. q_k = sqrt(max_x (k k|k k)_xx) of each
+  // primitive pair k (its records carry c_a c_b) bounds its share of any
+  // integral by q_k Q_cd (Cauchy-Schwarz), so dropping the smallest ones of a
+  // pair while their sum stays within FP32_PRIM_REL * Q_ab changes every
+  // integral of the pair by at most FP32_PRIM_REL times its Schwarz bound.
+  // The FP32 records keep the rest (at least one per pair).
+  {
+    std::vector<int4> ppairs(pp.size());
+    for (size_t k = 0; k < pp.size(); ++k) ppairs[k] = make_int4(pp[k].x, pp[k].y, pp_off[k], 1);
+    int4* dpp = nullptr;
+    if ((rc = upload(&dpp, ppairs))) return rc;
+    std::vector<double> qprim(pp.size(), 0.0);
+    for (int pc = 0; pc < NPC && rc == DFT_OK; ++pc) {
+      const int n = p->cls_n[pc];
+      if (n == 0) continue;
+      int la, lb;
+      pc_moments(pc, la, lb);
+      const int nab = ncart_h(la) * ncart_h(lb), rs = rec_size(la, lb), ci = combo_index(pc, pc);
+      const int k0 = p->pair_prim_begin[p->cls_first[pc]];
+      const int k1 = p->cls_first[pc] + n < npair ? p->pair_prim_begin[p->cls_first[pc] + n] : (int)pp.size();
+      // in pieces of at most 32 Ki primitive pairs (each a full diagonal block)
+      for (int c0 = k0; c0 < k1 && rc == DFT_OK; c0 += 32768) {
+        const int nk = std::min(k1 - c0, 32768);
+        std::vector<int2> q(nk);
+        for (int k = 0; k < nk; ++k) q[k] = make_int2(c0 + k, c0 + k);
+        std::vector<double> blk;
+        if (lane_kernel_combo(ci)) {
+          rc = raw_blocks(p, ci, q, blk, nab * nab, dpp);
+        } else {
+          // the column kernel reads bra records interleaved over the list
+          std::vector<int4> map(nk);
+          for (int k = 0; k < nk; ++k) map[k] = make_int4(pp_off[c0 + k], k, rs, nk);
+          int4* dmap = nullptr;
+          double* tmp = nullptr;
+          if ((rc = upload(&dmap, map))) break;
+          hipError_t e = hipMalloc(&tmp, (size_t)nk * rs * sizeof(double));
+          if (e == hipSuccess) e = launch_interleave_records(dmap, nk, p->d_rec64, nullptr, tmp, nullptr, nullptr);
+          if (e == hipSuccess) e = hipDeviceSynchronize();
+          if (e == hipSuccess) {
+            RecView rv{};
+            rv.bra = tmp;
+            rv.sb = nk;
+            rv.fb = c0;
+            rv.ket = p->d_rec64;
+            rv.first = 0;
+            rv.n = rv.ntot = nk;
+            rv.splb = rv.splk = 1;
+            rc = raw_blocks(p, ci, q, blk, nab * nab, dpp, &rv);
+          } else {
+            rc = fail(DFT_EHIP, std::string("primitive Schwarz: ") + hipGetErrorString(e));
+          }
+          (void)hipFree(dmap);
+          (void)hipFree(tmp);
+        }
+        if (rc) break;
+        for (int k = 0; k < nk; ++k) {
+          double mx = 0.0;
+          for (int x = 0; x < nab; ++x)
+            mx = std::max(mx, std::fabs(blk[(size_t)k * nab * nab + (size_t)x * nab + x]));
+          qprim[c0 + k] = std::sqrt(mx);
+        }
+      }
+    }
+    (void)hipFree(dpp);
+    if (rc) return rc;
+    std::vector<int4> pp32;
+    std::vector<int> pp_off32;
+    long long roff32 = 0;
+    p->pairs32.resize(npair);
+    for (int id = 0; id < npair; ++id) {
+      const int4 pr = p->pairs[id];
+      const int first = p->pair_prim_begin[id];
+      int la, lb;
+      pc_moments(p->pair_pc[id], la, lb);
+      const int rs = rec_size(la, lb);
+      std::vector<int> ord(pr.w);
+      for (int k = 0; k < pr.w; ++k) ord[k] = k;
+      std::sort(ord.begin(), ord.end(), [&](int u, int v) { return qprim[first + u] < qprim[first + v]; });
+      std::vector<char> keep(pr.w, 1);
+      double dropped = 0.0;
+      for (int r = 0; r + 1 < pr.w; ++r) {
+        const double qk = qprim[first + ord[r]];
+        if (dropped + qk > FP32_PRIM_REL * p->Q[id]) break;
+        dropped += qk;
+        keep[ord[r]] = 0;
+      }
+      const long long r0 = roff32;
+      int n32 = 0;
+      for (int k = 0; k < pr.w; ++k) {
+        if (!keep[k]) continue;
+        pp32.push_back(pp[first + k]);
+        pp_off32.push_back((int)roff32);
+        roff32 += rs;
+        ++n32;
+      }
+      p->pairs32[id] = make_int4(pr.x, pr.y, (int)r0, n32);
+    }
+    if ((rc = upload(&p->d_pairs32, p->pairs32))) return rc;
+    int* doff = nullptr;
+    dpp = nullptr;
+    if ((rc = upload(&dpp, pp32))) return rc;
+    if ((rc = upload(&doff, pp_off32))) return rc;
+    hipError_t e = hipMalloc(&p->d_rec32, std::max<long long>(roff32, 1) * sizeof(float));
+    if (e == hipSuccess)
+      e = launch_pair_records(dpp, (int)pp32.size(), doff, p->d_sh, p->d_exps, p->d_coefs, nullptr, p->d_rec32,
+                              nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    (void)hipFree(dpp);
+    (void)hipFree(doff);
+    if (e != hipSuccess) return fail(DFT_EHIP, std::string("FP32 pair pre-pass: ") + hipGetErrorString(e));
+    if ((rc = interleave_records(p, 1))) return rc;
+  }
   // ---- screened, class-sorted quartet lists (FP64 part, then weak part),
   // molecule-major within each class combination; bra and ket always come
   // from the same molecule
@@ -500,7 +634,7 @@ int build_plan(dft_plan p, const Input& in) {
                                  p->sh[p->pairs[ik].x].nprim * p->sh[p->pairs[ik].y].nprim;
             if (b < p->tau32) {
               weak.push_back(make_int2(ib, ik));
-              c.prim32 += np;
+              c.prim32 += (long long)p->pairs32[ib].w * p->pairs32[ik].w;
               c.kprod32 += kp;
             } else {
               c.q.push_back(make_int2(ib, ik));
@@ -648,7 +782,7 @@ RecView rec_view(dft_plan p, int ci, bool f32) {
   RecView rv;
   const size_t esz = f32 ? sizeof(float) : sizeof(double);
   const char* t = f32 ? (const char*)p->d_recT32 : (const char*)p->d_recT64;
-  rv.bra = t + p->cls_base[pcb] * esz;
+  rv.bra = t + (f32 ? p->cls_base32[pcb] : p->cls_base[pcb]) * esz;
   rv.sb = p->cls_n[pcb];
   rv.fb = p->cls_first[pcb];
   rv.ket = f32 ? (const void*)p->d_rec32 : (const void*)p->d_rec64;
@@ -675,14 +809,15 @@ hipError_t launch_combo(dft_plan p, size_t ci, bool f32, long long first, long l
   RecView rv = rec_view(p, (int)ci, f32);
   rv.first = (int)first;
   rv.n = (int)n;
+  const int4* prs = f32 ? p->d_pairs32 : p->d_pairs;  // FP32 launches: FP32 primitive lists
   if (lane_kernel_combo((int)ci))
-    return launch_eri_jk((int)ci, f32, c.d_q, c.d_slots, p->d_pairs, rv, p->d_shell_dm, P, Pa, Pb, p->nk,
-                         p->d_stage, p->d_boys, sh.block, sh.cap, s);
+    return launch_eri_jk((int)ci, f32, c.d_q, c.d_slots, prs, rv, p->d_shell_dm, P, Pa, Pb, p->nk, p->d_stage,
+                         p->d_boys, sh.block, sh.cap, s);
   int la, lb, lc, ld;
   pc_moments(c.pcb, la, lb);
   pc_moments(c.pck, lc, ld);
   const long long ni = (long long)ncart_h(la) * ncart_h(lb) * ncart_h(lc) * ncart_h(ld);
-  return launch_eri_chunk((int)ci, f32, c.d_q, (int)first, (int)n, c.splb, p->d_pairs, rv, c.d_ibuf, 1, ni,
+  return launch_eri_chunk((int)ci, f32, c.d_q, (int)first, (int)n, c.splb, prs, rv, c.d_ibuf, 1, ni,
                           ni * (long long)c.q.size(), p->d_boys, sh.block, sh.cap, s);
 }
 

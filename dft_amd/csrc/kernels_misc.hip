@@ -26,13 +26,17 @@ __global__ void pair_records_kernel(const int4* __restrict__ pp, int n, const in
   const double AB[3] = {A.x - B.x, A.y - B.y, A.z - B.z};
   const double Ac[3] = {A.x, A.y, A.z}, Bc[3] = {B.x, B.y, B.z};
   const int NE = ne_axis(la, lb);
-  double* o64 = r64 + rec_off[idx];
-  float* o32 = r32 + rec_off[idx];
+  // either output may be absent (FP64 and FP32 records have separate lists)
+  double* o64 = r64 ? r64 + rec_off[idx] : nullptr;
+  float* o32 = r32 ? r32 + rec_off[idx] : nullptr;
   double head[5];
   for (int d = 0; d < 3; ++d) head[1 + d] = (a * Ac[d] + b * Bc[d]) / p;
   head[0] = p;
   head[4] = ca * cb * exp(-mu * (AB[0] * AB[0] + AB[1] * AB[1] + AB[2] * AB[2])) / p;
-  for (int k = 0; k < 5; ++k) { o64[k] = head[k]; o32[k] = (float)head[k]; }
+  for (int k = 0; k < 5; ++k) {
+    if (o64) o64[k] = head[k];
+    if (o32) o32[k] = (float)head[k];
+  }
   // (2l-1)!! for l <= 4: 1, 1, 3, 15, 105
   const double dfac[5] = {1.0, 1.0, 3.0, 15.0, 105.0};
   for (int d = 0; d < 3; ++d) {
@@ -58,8 +62,8 @@ __global__ void pair_records_kernel(const int4* __restrict__ pp, int n, const in
         const double nrm = 1.0 / sqrt(dfac[ia] * dfac[ib]);
         const int o = 5 + d * NE + eoff(lb, ia, ib);
         for (int t = 0; t <= ord; ++t) {
-          o64[o + t] = cur[t] * nrm;
-          o32[o + t] = (float)(cur[t] * nrm);
+          if (o64) o64[o + t] = cur[t] * nrm;
+          if (o32) o32[o + t] = (float)(cur[t] * nrm);
         }
         if (ib == lb) break;
         for (int t = 0; t <= ord + 1; ++t) {
@@ -91,8 +95,8 @@ __global__ void interleave_records_kernel(const int4* __restrict__ map, int nrec
   if (r >= nrec) return;
   const int4 m = map[r];
   for (int f = 0; f < m.z; ++f) {
-    t64[m.y + (long long)f * m.w] = r64[m.x + f];
-    t32[m.y + (long long)f * m.w] = r32[m.x + f];
+    if (t64) t64[m.y + (long long)f * m.w] = r64[m.x + f];
+    if (t32) t32[m.y + (long long)f * m.w] = r32[m.x + f];
   }
 }
 
