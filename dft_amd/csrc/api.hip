@@ -378,17 +378,45 @@ int build_plan(dft_plan p, const Input& in) {
     }
     p->sh[k] = s;
   }
-  // ---- pairs (within one molecule), sorted by pair class, then (i, j)
+  // ---- pairs (within one molecule), sorted by pair class, then molecule,
+  // then by surviving primitive-pair count, largest first (stable: (i, j)
+  // order among equal counts). The lane kernel runs a wave's bra loop for
+  // its lanes' largest count; consecutive bra pairs of similar counts keep
+  // those lanes busy (about half the bra iterations idle in (i, j) order).
+  auto prim_count = [&](int i, int j) {
+    const ShellDev &A = p->sh[i], &B = p->sh[j];
+    const double r2 = (A.x - B.x) * (A.x - B.x) + (A.y - B.y) * (A.y - B.y) + (A.z - B.z) * (A.z - B.z);
+    int n = 0;
+    for (int ia = 0; ia < A.nprim; ++ia)
+      for (int ib = 0; ib < B.nprim; ++ib) {
+        const double a = p->exps[A.prim_off + ia], b = p->exps[B.prim_off + ib];
+        n += a * b / (a + b) * r2 <= PRIM_CUT;
+      }
+    return n;
+  };
   std::vector<std::vector<int>> mol_shells(in.nmol);
   for (int k = 0; k < nsh; ++k) mol_shells[p->sh[k].mol].push_back(k);
   std::vector<std::vector<int2>> by_pc(NPC);
   for (int m = 0; m < in.nmol; ++m) {
     const auto& ms = mol_shells[m];
+    size_t seg[NPC];
+    for (int pc = 0; pc < NPC; ++pc) seg[pc] = by_pc[pc].size();
     for (size_t a = 0; a < ms.size(); ++a)
       for (size_t b = 0; b <= a; ++b) {
         const int i = ms[a], j = ms[b];
         by_pc[pc_index(p->sh[i].l, p->sh[j].l)].push_back(make_int2(i, j));
       }
+    for (int pc = 0; pc < NPC; ++pc) {
+      std::vector<std::pair<int, int2>> v;
+      for (size_t k = seg[pc]; k < by_pc[pc].size(); ++k) {
+        const int2 ij = by_pc[pc][k];
+        v.push_back({prim_count(ij.x, ij.y), ij});
+      }
+      std::stable_sort(v.begin(), v.end(), [](const std::pair<int, int2>& x, const std::pair<int, int2>& y) {
+        return x.first > y.first;
+      });
+      for (size_t k = 0; k < v.size(); ++k) by_pc[pc][seg[pc] + k] = v[k].second;
+    }
   }
   std::vector<int4> pp;       // primitive pairs: sh_a, sh_b, ia, ib
   std::vector<int> pp_off;    // record offsets
