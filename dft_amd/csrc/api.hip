@@ -28,6 +28,10 @@ constexpr long long SPLIT_LANES_HEAVY = 262144;
 // FP32 records drop primitive pairs whose summed Schwarz bounds stay within
 // this fraction of the shell pair's (FP32 resolution is 6e-8)
 constexpr double FP32_PRIM_REL = 1e-7;
+// FP32 quartets with Q_ab Q_cd below FP32_LEVEL2 * tau32 use a second record
+// set cut at FP32_PRIM_REL / FP32_LEVEL2 (the same absolute bound)
+constexpr double FP32_LEVEL2 = 1e-3;
+constexpr long long FP32_LEVEL2_MIN = 1 << 20;  // quartets
 // auxiliary streams per plan for concurrent class launches
 constexpr int JK_AUX = 3;
 // screened quartets below which a plan's class launches run concurrently by
@@ -122,9 +126,11 @@ struct Combo {
   int pcb = 0, pck = 0;
   std::vector<int2> q;         // (bra pair, ket pair) plan pair ids: FP64 part, then weak part
   long long n64 = 0;           // size of the FP64 part
+  long long n32a = 0;          // size of the first FP32 level (the rest is level 2)
   int2* d_q = nullptr;
   long long* d_slots = nullptr;  // per quartet 2 + 4 nk staging offsets
   long long prim64 = 0, prim32 = 0;      // primitive shell quartets evaluated
+  long long prim64w = 0;                 // FP64 primitive quartets of the FP32-eligible part
   long long kprod64 = 0, kprod32 = 0;    // sum of K_a K_b K_c K_d (before primitive screening)
   Shape shape64, shape32;        // for the FP64 launch and the FP32 (weak part) launch
   // primitive split of the lane kernel: every quartet is evaluated by
@@ -155,7 +161,9 @@ struct dft_plan_s {
   std::vector<int> in2int;       // input shell -> internal shell
   std::vector<double> exps, coefs;
   std::vector<int4> pairs;       // (sh_a, sh_b, rec_off, nprim_pairs), class-sorted
-  std::vector<int4> pairs32;     // the same for the FP32 records (looser primitive cut)
+  // the same for the two FP32 record levels (looser primitive cuts; level 1
+  // serves quartets with Q_ab Q_cd < FP32_LEVEL2 * tau32)
+  std::vector<int4> pairs32[2];
   std::vector<int2> prim_list;   // surviving primitive pairs (ia, ib) of all pairs
   std::vector<int> pair_prim_begin;  // first prim_list entry of each pair
   std::vector<int> pair_pc;      // pair class per pair
@@ -170,15 +178,15 @@ struct dft_plan_s {
   double* d_exps = nullptr;
   double* d_coefs = nullptr;
   int4* d_pairs = nullptr;
-  int4* d_pairs32 = nullptr;
+  int4* d_pairs32[2] = {};
   int4* d_shell_dm = nullptr;
   int* d_shell_ao = nullptr;
   double* d_rec64 = nullptr;
-  float* d_rec32 = nullptr;
+  float* d_rec32[2] = {};
   // interleaved copies of the records, one block per pair class (lane kernels)
   double* d_recT64 = nullptr;
-  float* d_recT32 = nullptr;
-  long long cls_base[NPC] = {}, cls_base32[NPC] = {};
+  float* d_recT32[2] = {};
+  long long cls_base[NPC] = {}, cls_base32[2][NPC] = {};
   int cls_first[NPC] = {}, cls_n[NPC] = {};
   double* d_stage = nullptr;
   Target* d_tg = nullptr;
@@ -226,11 +234,10 @@ void free_plan(dft_plan p) {
     (void)hipFree(c.d_ibuf);
   }
   void* ptrs[] = {p->d_sh,    p->d_exps,  p->d_coefs,   p->d_pairs,    p->d_shell_dm,  p->d_shell_ao,
-                  p->d_rec64, p->d_rec32, p->d_stage,   p->d_tg,       p->d_tbase,     p->d_tcount,
+                  p->d_rec64, p->d_rec32[0], p->d_stage,   p->d_tg,       p->d_tbase,     p->d_tcount,
                   p->d_half,  p->d_atoms, p->d_shpairs, p->d_mol_base, p->d_mol_nbf,   p->d_grec,
                   p->d_gpairs, p->d_g1slots, p->d_gbase, p->d_gcount, p->d_gstage, p->d_recT64, p->d_g1tasks,
-                  p->d_pairs32,
-                  p->d_recT32};
+                  p->d_pairs32[0], p->d_pairs32[1], p->d_rec32[1], p->d_recT32[0], p->d_recT32[1]};
   for (void* x : ptrs) (void)hipFree(x);
   for (hipEvent_t e : p->prof_ev) (void)hipEventDestroy(e);
   for (hipStream_t x : p->aux) (void)hipStreamDestroy(x);
@@ -244,14 +251,14 @@ void free_plan(dft_plan p) {
 
 long long pair_key(int i, int j) { return ((long long)i << 32) | (unsigned)j; }
 
-RecView rec_view(dft_plan p, int ci, bool f32);
+RecView rec_view(dft_plan p, int ci, bool f32, int lev = 0);
 
 // Interleaved record blocks per pair class (prec 0: FP64 records and pair
-// table, 1: FP32 ones): element f of primitive pair j of class-local pair u
-// at base + (j * rs + f) * n + u, n the class's pair count.
+// table, 1 / 2: FP32 levels 0 / 1): element f of primitive pair j of
+// class-local pair u at base + (j * rs + f) * n + u, n the class's pair count.
 int interleave_records(dft_plan p, int prec) {
-  const std::vector<int4>& prs = prec ? p->pairs32 : p->pairs;
-  long long* cbase = prec ? p->cls_base32 : p->cls_base;
+  const std::vector<int4>& prs = prec ? p->pairs32[prec - 1] : p->pairs;
+  long long* cbase = prec ? p->cls_base32[prec - 1] : p->cls_base;
   std::vector<int4> map;
   long long tot = 0;
   for (int pc = 0; pc < NPC; ++pc) {
@@ -279,9 +286,10 @@ int interleave_records(dft_plan p, int prec) {
     if (e == hipSuccess)
       e = launch_interleave_records(dmap, (int)map.size(), p->d_rec64, nullptr, p->d_recT64, nullptr, nullptr);
   } else {
-    e = hipMalloc(&p->d_recT32, std::max<long long>(tot, 1) * sizeof(float));
+    float*& t = p->d_recT32[prec - 1];
+    e = hipMalloc(&t, std::max<long long>(tot, 1) * sizeof(float));
     if (e == hipSuccess)
-      e = launch_interleave_records(dmap, (int)map.size(), nullptr, p->d_rec32, nullptr, p->d_recT32, nullptr);
+      e = launch_interleave_records(dmap, (int)map.size(), nullptr, p->d_rec32[prec - 1], nullptr, t, nullptr);
   }
   if (e == hipSuccess) e = hipDeviceSynchronize();
   (void)hipFree(dmap);
@@ -589,52 +597,56 @@ int build_plan(dft_plan p, const Input& in) {
     }
     (void)hipFree(dpp);
     if (rc) return rc;
-    std::vector<int4> pp32;
-    std::vector<int> pp_off32;
-    long long roff32 = 0;
-    p->pairs32.resize(npair);
-    for (int id = 0; id < npair; ++id) {
-      const int4 pr = p->pairs[id];
-      const int first = p->pair_prim_begin[id];
-      int la, lb;
-      pc_moments(p->pair_pc[id], la, lb);
-      const int rs = rec_size(la, lb);
-      std::vector<int> ord(pr.w);
-      for (int k = 0; k < pr.w; ++k) ord[k] = k;
-      std::sort(ord.begin(), ord.end(), [&](int u, int v) { return qprim[first + u] < qprim[first + v]; });
-      std::vector<char> keep(pr.w, 1);
-      double dropped = 0.0;
-      for (int r = 0; r + 1 < pr.w; ++r) {
-        const double qk = qprim[first + ord[r]];
-        if (dropped + qk > FP32_PRIM_REL * p->Q[id]) break;
-        dropped += qk;
-        keep[ord[r]] = 0;
+    for (int lev = 0; lev < 2; ++lev) {
+      const double eps = lev ? FP32_PRIM_REL / FP32_LEVEL2 : FP32_PRIM_REL;
+      std::vector<int4> pp32;
+      std::vector<int> pp_off32;
+      long long roff32 = 0;
+      std::vector<int4>& prs = p->pairs32[lev];
+      prs.resize(npair);
+      for (int id = 0; id < npair; ++id) {
+        const int4 pr = p->pairs[id];
+        const int first = p->pair_prim_begin[id];
+        int la, lb;
+        pc_moments(p->pair_pc[id], la, lb);
+        const int rs = rec_size(la, lb);
+        std::vector<int> ord(pr.w);
+        for (int k = 0; k < pr.w; ++k) ord[k] = k;
+        std::sort(ord.begin(), ord.end(), [&](int u, int v) { return qprim[first + u] < qprim[first + v]; });
+        std::vector<char> keep(pr.w, 1);
+        double dropped = 0.0;
+        for (int r = 0; r + 1 < pr.w; ++r) {
+          const double qk = qprim[first + ord[r]];
+          if (dropped + qk > eps * p->Q[id]) break;
+          dropped += qk;
+          keep[ord[r]] = 0;
+        }
+        const long long r0 = roff32;
+        int n32 = 0;
+        for (int k = 0; k < pr.w; ++k) {
+          if (!keep[k]) continue;
+          pp32.push_back(pp[first + k]);
+          pp_off32.push_back((int)roff32);
+          roff32 += rs;
+          ++n32;
+        }
+        prs[id] = make_int4(pr.x, pr.y, (int)r0, n32);
       }
-      const long long r0 = roff32;
-      int n32 = 0;
-      for (int k = 0; k < pr.w; ++k) {
-        if (!keep[k]) continue;
-        pp32.push_back(pp[first + k]);
-        pp_off32.push_back((int)roff32);
-        roff32 += rs;
-        ++n32;
-      }
-      p->pairs32[id] = make_int4(pr.x, pr.y, (int)r0, n32);
+      if ((rc = upload(&p->d_pairs32[lev], prs))) return rc;
+      int* doff = nullptr;
+      dpp = nullptr;
+      if ((rc = upload(&dpp, pp32))) return rc;
+      if ((rc = upload(&doff, pp_off32))) return rc;
+      hipError_t e = hipMalloc(&p->d_rec32[lev], std::max<long long>(roff32, 1) * sizeof(float));
+      if (e == hipSuccess)
+        e = launch_pair_records(dpp, (int)pp32.size(), doff, p->d_sh, p->d_exps, p->d_coefs, nullptr,
+                                p->d_rec32[lev], nullptr);
+      if (e == hipSuccess) e = hipDeviceSynchronize();
+      (void)hipFree(dpp);
+      (void)hipFree(doff);
+      if (e != hipSuccess) return fail(DFT_EHIP, std::string("FP32 pair pre-pass: ") + hipGetErrorString(e));
+      if ((rc = interleave_records(p, 1 + lev))) return rc;
     }
-    if ((rc = upload(&p->d_pairs32, p->pairs32))) return rc;
-    int* doff = nullptr;
-    dpp = nullptr;
-    if ((rc = upload(&dpp, pp32))) return rc;
-    if ((rc = upload(&doff, pp_off32))) return rc;
-    hipError_t e = hipMalloc(&p->d_rec32, std::max<long long>(roff32, 1) * sizeof(float));
-    if (e == hipSuccess)
-      e = launch_pair_records(dpp, (int)pp32.size(), doff, p->d_sh, p->d_exps, p->d_coefs, nullptr, p->d_rec32,
-                              nullptr);
-    if (e == hipSuccess) e = hipDeviceSynchronize();
-    (void)hipFree(dpp);
-    (void)hipFree(doff);
-    if (e != hipSuccess) return fail(DFT_EHIP, std::string("FP32 pair pre-pass: ") + hipGetErrorString(e));
-    if ((rc = interleave_records(p, 1))) return rc;
   }
   // ---- screened, class-sorted quartet lists (FP64 part, then weak part),
   // molecule-major within each class combination; bra and ket always come
@@ -645,7 +657,7 @@ int build_plan(dft_plan p, const Input& in) {
       Combo& c = p->combos[combo_index(pcb, pck)];
       c.pcb = pcb;
       c.pck = pck;
-      std::vector<int2> weak;
+      std::vector<int2> weak, weak2;
       // ket-major: consecutive quartets (the lanes of a wave) share the ket
       // pair, whose primitive records the lane kernel reads in its inner loop,
       // so those loads are wave-uniform; the bra pairs of a wave are
@@ -661,8 +673,8 @@ int build_plan(dft_plan p, const Input& in) {
             const long long kp = (long long)p->sh[p->pairs[ib].x].nprim * p->sh[p->pairs[ib].y].nprim *
                                  p->sh[p->pairs[ik].x].nprim * p->sh[p->pairs[ik].y].nprim;
             if (b < p->tau32) {
-              weak.push_back(make_int2(ib, ik));
-              c.prim32 += (long long)p->pairs32[ib].w * p->pairs32[ik].w;
+              (b < FP32_LEVEL2 * p->tau32 ? weak2 : weak).push_back(make_int2(ib, ik));
+              c.prim64w += np;
               c.kprod32 += kp;
             } else {
               c.q.push_back(make_int2(ib, ik));
@@ -671,8 +683,18 @@ int build_plan(dft_plan p, const Input& in) {
             }
           }
         }
+      // the second FP32 level only where its part is large enough to pay
+      // for a launch of its own
+      if ((long long)weak2.size() < FP32_LEVEL2_MIN) {
+        weak.insert(weak.end(), weak2.begin(), weak2.end());
+        weak2.clear();
+      }
+      for (const int2& qq : weak) c.prim32 += (long long)p->pairs32[0][qq.x].w * p->pairs32[0][qq.y].w;
+      for (const int2& qq : weak2) c.prim32 += (long long)p->pairs32[1][qq.x].w * p->pairs32[1][qq.y].w;
       c.n64 = (long long)c.q.size();
+      c.n32a = (long long)weak.size();
       c.q.insert(c.q.end(), weak.begin(), weak.end());
+      c.q.insert(c.q.end(), weak2.begin(), weak2.end());
       // primitive split where the class has too few quartets to fill the
       // chip with one lane each (~64 Ki lanes): bra chunks first, then ket
       const long long nq = (long long)c.q.size();
@@ -805,15 +827,15 @@ int block_max(int ci) { return lane_kernel_combo(ci) ? lane_block_max(ci) : 256;
 // (Also used for the Schwarz diagonals, before the quartet lists exist: the
 // bra class comes from the combination index, the list fields only when
 // the combination is built.)
-RecView rec_view(dft_plan p, int ci, bool f32) {
+RecView rec_view(dft_plan p, int ci, bool f32, int lev) {
   const int pcb = combo_pcb(ci);
   RecView rv;
   const size_t esz = f32 ? sizeof(float) : sizeof(double);
-  const char* t = f32 ? (const char*)p->d_recT32 : (const char*)p->d_recT64;
-  rv.bra = t + (f32 ? p->cls_base32[pcb] : p->cls_base[pcb]) * esz;
+  const char* t = f32 ? (const char*)p->d_recT32[lev] : (const char*)p->d_recT64;
+  rv.bra = t + (f32 ? p->cls_base32[lev][pcb] : p->cls_base[pcb]) * esz;
   rv.sb = p->cls_n[pcb];
   rv.fb = p->cls_first[pcb];
-  rv.ket = f32 ? (const void*)p->d_rec32 : (const void*)p->d_rec64;
+  rv.ket = f32 ? (const void*)p->d_rec32[lev] : (const void*)p->d_rec64;
   rv.first = 0;
   rv.n = 0;
   rv.ntot = 0;
@@ -827,17 +849,20 @@ RecView rec_view(dft_plan p, int ci, bool f32) {
   return rv;
 }
 
-// One class-combination launch of the FP64 or FP32 kernel on quartets
-// [first, first + n) with the given launch shape. Lane-kernel combinations
-// write their J/K blocks directly; heavy combinations write integrals to
-// the combination's buffer (launch_digest turns them into J/K blocks).
-hipError_t launch_combo(dft_plan p, size_t ci, bool f32, long long first, long long n, const Shape& sh,
+// One launch over quartets [first, first + n) of one record set (lev -1:
+// FP64, 0 / 1: FP32 levels) with the given launch shape. Lane-kernel
+// combinations write their J/K blocks directly; heavy combinations write
+// integrals to the combination's buffer (launch_digest turns them into J/K
+// blocks).
+hipError_t launch_level(dft_plan p, size_t ci, int lev, long long first, long long n, const Shape& sh,
                         const double* P, const double* Pa, const double* Pb, hipStream_t s) {
   const Combo& c = p->combos[ci];
-  RecView rv = rec_view(p, (int)ci, f32);
+  if (n <= 0) return hipSuccess;
+  const bool f32 = lev >= 0;
+  RecView rv = rec_view(p, (int)ci, f32, f32 ? lev : 0);
   rv.first = (int)first;
   rv.n = (int)n;
-  const int4* prs = f32 ? p->d_pairs32 : p->d_pairs;  // FP32 launches: FP32 primitive lists
+  const int4* prs = f32 ? p->d_pairs32[lev] : p->d_pairs;  // FP32 launches: that level's primitive lists
   if (lane_kernel_combo((int)ci))
     return launch_eri_jk((int)ci, f32, c.d_q, c.d_slots, prs, rv, p->d_shell_dm, P, Pa, Pb, p->nk, p->d_stage,
                          p->d_boys, sh.block, sh.cap, s);
@@ -847,6 +872,22 @@ hipError_t launch_combo(dft_plan p, size_t ci, bool f32, long long first, long l
   const long long ni = (long long)ncart_h(la) * ncart_h(lb) * ncart_h(lc) * ncart_h(ld);
   return launch_eri_chunk((int)ci, f32, c.d_q, (int)first, (int)n, c.splb, prs, rv, c.d_ibuf, 1, ni,
                           ni * (long long)c.q.size(), p->d_boys, sh.block, sh.cap, s);
+}
+
+// The FP64 or FP32 kernel of a class combination on quartets [first, first + n).
+hipError_t launch_combo(dft_plan p, size_t ci, bool f32, long long first, long long n, const Shape& sh,
+                        const double* P, const double* Pa, const double* Pb, hipStream_t s) {
+  const Combo& c = p->combos[ci];
+  if (f32) {
+    // FP32 quartets [n64, n64 + n32a) use the level-0 records, the rest level 1
+    const long long bnd = c.n64 + c.n32a, end = first + n;
+    if (first < bnd && end > bnd) {
+      hipError_t e = launch_level(p, ci, 0, first, bnd - first, sh, P, Pa, Pb, s);
+      return e != hipSuccess ? e : launch_level(p, ci, 1, bnd, end - bnd, sh, P, Pa, Pb, s);
+    }
+    return launch_level(p, ci, first >= bnd ? 1 : 0, first, n, sh, P, Pa, Pb, s);
+  }
+  return launch_level(p, ci, -1, first, n, sh, P, Pa, Pb, s);
 }
 
 // J/K blocks of a heavy combination from its integral buffer (whole list).
@@ -895,10 +936,10 @@ int jk_enqueue(dft_plan p, const double* P, const double* Pa, const double* Pb, 
     const double w = 16.0 + ncart_h(la) * ncart_h(lb) * ncart_h(lc) * ncart_h(ld);  // per primitive quartet
     if (!lane_kernel_combo((int)ci)) {
       // heavy combination: integral launches and digest in order on one stream
-      ls.push_back({(int)ci, false, 0, ntot, w * (c.prim64 + c.prim32)});
+      ls.push_back({(int)ci, false, 0, ntot, w * (c.prim64 + (mode == DFT_MIXED ? c.prim32 : c.prim64w))});
       continue;
     }
-    if (n1 > 0) ls.push_back({(int)ci, false, 0, n1, w * (mode == DFT_MIXED ? c.prim64 : c.prim64 + c.prim32)});
+    if (n1 > 0) ls.push_back({(int)ci, false, 0, n1, w * (mode == DFT_MIXED ? c.prim64 : c.prim64 + c.prim64w)});
     if (mode == DFT_MIXED && ntot > c.n64) ls.push_back({(int)ci, true, c.n64, ntot - c.n64, 0.7 * w * c.prim32});
   }
   std::stable_sort(ls.begin(), ls.end(), [](const Launch& a, const Launch& b) { return a.cost > b.cost; });

@@ -115,12 +115,20 @@ __host__ __device__ constexpr double inv_odd(int m) { return 1.0 / (2.0 * m + 1.
 __host__ __device__ constexpr double boys_tnoexp(int L) { return L <= 1 ? BOYS_TMAX : BOYS_TMAX + 4.0 * (L - 1); }
 
 // FP64 Boys function F_0..F_L from the LDS slice tab[k * BOYS_TAYLOR + j].
+// Nearest grid point of 0 <= T < BOYS_TMAX (truncation of T/DT + 1/2) and its
+// LDS row; the row offset is a 24-bit multiply (full rate, k < 2^24) rather
+// than a 32-bit one.
+__device__ __forceinline__ int boys_point(double T) { return (int)fma(T, BOYS_INV_DT, 0.5); }
+__device__ __forceinline__ const double* boys_row(const double* tab, int k) {
+  return tab + __umul24((unsigned)k, (unsigned)BOYS_ROW);
+}
+
 template <int L>
 __device__ __forceinline__ void boys_fp64(double T, const double* __restrict__ tab, double (&F)[L + 1]) {
   if (T < BOYS_TMAX) {
-    int k = __double2int_rn(T * BOYS_INV_DT);
+    const int k = boys_point(T);
     double d = -(T - k * BOYS_DT);
-    const double* c = tab + k * BOYS_ROW;
+    const double* c = boys_row(tab, k);
     double f = c[BOYS_TAYLOR - 1];
 #pragma unroll
     for (int j = BOYS_TAYLOR - 2; j >= 0; --j) f = fma(f, d, c[j]);
@@ -164,9 +172,9 @@ template <int L>
 __device__ __forceinline__ void boys_fp32(float Tf, const double* __restrict__ tab, float (&F)[L + 1]) {
   if (Tf < boys_tasym32(L)) {
     double T = (double)Tf;
-    int k = __double2int_rn(T * BOYS_INV_DT);
+    const int k = (int)fmaf(Tf, (float)BOYS_INV_DT, 0.5f);  // |d| <= DT/2 + 4e-6
     double d = -(T - k * BOYS_DT);
-    const double* c = tab + k * BOYS_ROW;
+    const double* c = boys_row(tab, k);
     // five Taylor terms: truncation below 3e-10 relative, far under FP32 rounding
     double f = c[4];
 #pragma unroll
