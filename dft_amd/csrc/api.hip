@@ -320,7 +320,7 @@ int raw_blocks(dft_plan p, int combo, const std::vector<int2>& q, std::vector<do
   } else {
     const RecView rv = rvp ? *rvp : rec_view(p, combo, false);
     e = launch_eri_chunk(combo, false, dq, 0, (int)q.size(), 1, pairs, rv, dout, 1, blk, 0, p->d_boys, 256,
-                         1 << 20, nullptr);
+                         1 << 20, false, nullptr);
   }
   if (e == hipSuccess) e = hipMemcpy(out.data(), dout, out.size() * sizeof(double), hipMemcpyDeviceToHost);
   (void)hipFree(dq);
@@ -485,8 +485,8 @@ int build_plan(dft_plan p, const Input& in) {
     if ((rc = upload(&dpp, pp))) return rc;
     if ((rc = upload(&doff, pp_off))) return rc;
     HIP_TRY(hipMalloc(&p->d_rec64, std::max<long long>(roff, 1) * sizeof(double)));
-    hipError_t e = launch_pair_records(dpp, (int)pp.size(), doff, p->d_sh, p->d_exps, p->d_coefs, p->d_rec64,
-                                       nullptr, nullptr);
+    hipError_t e = launch_pair_records(dpp, (int)pp.size(), doff, p->d_sh, p->d_exps, p->d_coefs, nullptr,
+                                       p->d_rec64, nullptr, nullptr);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     (void)hipFree(dpp);
     (void)hipFree(doff);
@@ -597,10 +597,52 @@ int build_plan(dft_plan p, const Input& in) {
     }
     (void)hipFree(dpp);
     if (rc) return rc;
+    // Primitive pairs of each pair in decreasing Schwarz factor, so that the
+    // integral loops can stop at the first primitive quartet below the cut;
+    // the FP64 records are rewritten in that order with their factors.
+    for (int id = 0; id < npair; ++id) {
+      const int first = p->pair_prim_begin[id], w = p->pairs[id].w;
+      std::vector<int> ord(w);
+      for (int k = 0; k < w; ++k) ord[k] = k;
+      std::stable_sort(ord.begin(), ord.end(),
+                       [&](int u, int v) { return qprim[first + u] > qprim[first + v]; });
+      std::vector<int2> pl(w);
+      std::vector<int4> pq(w);
+      std::vector<double> qq(w);
+      for (int k = 0; k < w; ++k) {
+        pl[k] = p->prim_list[first + ord[k]];
+        pq[k] = pp[first + ord[k]];
+        qq[k] = qprim[first + ord[k]];
+      }
+      for (int k = 0; k < w; ++k) {
+        p->prim_list[first + k] = pl[k];
+        pp[first + k] = pq[k];
+        qprim[first + k] = qq[k];
+      }
+    }
+    {
+      int* doff = nullptr;
+      double* dq = nullptr;
+      dpp = nullptr;
+      if ((rc = upload(&dpp, pp))) return rc;
+      if ((rc = upload(&doff, pp_off))) return rc;
+      if ((rc = upload(&dq, qprim))) return rc;
+      hipError_t e = launch_pair_records(dpp, (int)pp.size(), doff, p->d_sh, p->d_exps, p->d_coefs, dq,
+                                         p->d_rec64, nullptr, nullptr);
+      if (e == hipSuccess) e = hipDeviceSynchronize();
+      (void)hipFree(dpp);
+      (void)hipFree(doff);
+      (void)hipFree(dq);
+      if (e != hipSuccess) return fail(DFT_EHIP, std::string("pair pre-pass: ") + hipGetErrorString(e));
+      (void)hipFree(p->d_recT64);
+      p->d_recT64 = nullptr;
+      if ((rc = interleave_records(p, 0))) return rc;
+    }
     for (int lev = 0; lev < 2; ++lev) {
       const double eps = lev ? FP32_PRIM_REL / FP32_LEVEL2 : FP32_PRIM_REL;
       std::vector<int4> pp32;
       std::vector<int> pp_off32;
+      std::vector<double> q32;
       long long roff32 = 0;
       std::vector<int4>& prs = p->pairs32[lev];
       prs.resize(npair);
@@ -626,6 +668,7 @@ int build_plan(dft_plan p, const Input& in) {
         for (int k = 0; k < pr.w; ++k) {
           if (!keep[k]) continue;
           pp32.push_back(pp[first + k]);
+          q32.push_back(qprim[first + k]);
           pp_off32.push_back((int)roff32);
           roff32 += rs;
           ++n32;
@@ -634,16 +677,19 @@ int build_plan(dft_plan p, const Input& in) {
       }
       if ((rc = upload(&p->d_pairs32[lev], prs))) return rc;
       int* doff = nullptr;
+      double* dq = nullptr;
       dpp = nullptr;
       if ((rc = upload(&dpp, pp32))) return rc;
       if ((rc = upload(&doff, pp_off32))) return rc;
+      if ((rc = upload(&dq, q32))) return rc;
       hipError_t e = hipMalloc(&p->d_rec32[lev], std::max<long long>(roff32, 1) * sizeof(float));
       if (e == hipSuccess)
-        e = launch_pair_records(dpp, (int)pp32.size(), doff, p->d_sh, p->d_exps, p->d_coefs, nullptr,
+        e = launch_pair_records(dpp, (int)pp32.size(), doff, p->d_sh, p->d_exps, p->d_coefs, dq, nullptr,
                                 p->d_rec32[lev], nullptr);
       if (e == hipSuccess) e = hipDeviceSynchronize();
       (void)hipFree(dpp);
       (void)hipFree(doff);
+      (void)hipFree(dq);
       if (e != hipSuccess) return fail(DFT_EHIP, std::string("FP32 pair pre-pass: ") + hipGetErrorString(e));
       if ((rc = interleave_records(p, 1 + lev))) return rc;
     }
@@ -871,7 +917,7 @@ hipError_t launch_level(dft_plan p, size_t ci, int lev, long long first, long lo
   pc_moments(c.pck, lc, ld);
   const long long ni = (long long)ncart_h(la) * ncart_h(lb) * ncart_h(lc) * ncart_h(ld);
   return launch_eri_chunk((int)ci, f32, c.d_q, (int)first, (int)n, c.splb, prs, rv, c.d_ibuf, 1, ni,
-                          ni * (long long)c.q.size(), p->d_boys, sh.block, sh.cap, s);
+                          ni * (long long)c.q.size(), p->d_boys, sh.block, sh.cap, true, s);
 }
 
 // The FP64 or FP32 kernel of a class combination on quartets [first, first + n).

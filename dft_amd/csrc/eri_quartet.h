@@ -65,22 +65,37 @@ struct QuartetShape {
 template <int LA, int LB, int LC, int LD, int NCDK>
 constexpr int ncd_slice() { return NCDK < 0 ? ncart(LC) * ncart(LD) : NCDK; }
 
+// Primitive-quartet cut of the J/K kernels: a primitive quartet whose
+// Schwarz bound s_ab s_cd (record heads) is below it is skipped. At most K^4
+// such terms (K^4 <= 1296 here) change an integral by < 1.3e-11 (FP64) or
+// 1.3e-10 (FP32, whose integrals carry larger errors anyway).
+constexpr double PRIM_QCUT64 = 1e-14;
+constexpr float PRIM_QCUT32 = 1e-13f;
+
 // I[ab * NK + k] = (ab|cd) for cd = CD0 + k, k < NK (NK = ncd_slice), for the
 // quartet (bra pair: base bb, stride sb, offset ob, nb primitive pairs; ket
 // pair likewise). With the default slice this is the whole I[ab * NCD + cd].
-template <int LA, int LB, int LC, int LD, typename real, int CD0 = 0, int NCDK = -1>
+// With CUT, primitive quartets with s_ab s_cd < qcut are left out: the
+// primitive pairs of a pair are stored in decreasing s, so each lane stops
+// its bra loop at the first bra primitive below the cut with the largest ket
+// factor, and the ket loop ends once no lane of the wave is above it (the
+// terms a lane leaves out are exactly its own sub-cut ones).
+template <int LA, int LB, int LC, int LD, typename real, bool CUT = false, int CD0 = 0, int NCDK = -1>
 __device__ __forceinline__ void quartet_integrals(
     const real* __restrict__ bb, int sb, unsigned ob, int nb, const real* __restrict__ bk, int sk, unsigned ok,
     int nk, const double* __restrict__ btab,
-    real (&I)[QuartetShape<LA, LB, LC, LD, real>::NAB * ncd_slice<LA, LB, LC, LD, NCDK>()]) {
+    real (&I)[QuartetShape<LA, LB, LC, LD, real>::NAB * ncd_slice<LA, LB, LC, LD, NCDK>()], real qcut = real(0)) {
   using S = QuartetShape<LA, LB, LC, LD, real>;
   constexpr int NK = ncd_slice<LA, LB, LC, LD, NCDK>();
-  constexpr int EXB = 5, EYB = 5 + S::NEB, EZB = 5 + 2 * S::NEB;
-  constexpr int EXK = 5, EYK = 5 + S::NEK, EZK = 5 + 2 * S::NEK;
+  constexpr int EXB = REC_HEAD, EYB = REC_HEAD + S::NEB, EZB = REC_HEAD + 2 * S::NEB;
+  constexpr int EXK = REC_HEAD, EYK = REC_HEAD + S::NEK, EZK = REC_HEAD + 2 * S::NEK;
 #pragma unroll
   for (int x = 0; x < S::NAB * NK; ++x) I[x] = real(0);
+  const real sk0 = CUT && nk > 0 ? fld(bk, sk, 5, ok) : real(0);  // largest ket factor
   for (int i = 0; i < nb; ++i) {
     const unsigned oi = ob + (unsigned)(i * S::RSB * sb);
+    const real sab = CUT ? fld(bb, sb, 5, oi) : real(0);
+    if (CUT && sab * sk0 < qcut) break;
     const real p = fld(bb, sb, 0, oi), Px = fld(bb, sb, 1, oi), Py = fld(bb, sb, 2, oi);
     const real Pz = fld(bb, sb, 3, oi), Kab = fld(bb, sb, 4, oi);
     real G[S::NHB * NK];
@@ -89,22 +104,28 @@ __device__ __forceinline__ void quartet_integrals(
     // the head (q, Q, K_cd) of ket primitive j + 1 is loaded while primitive
     // j is evaluated, which hides most of the L2 latency of the record loads
     unsigned oj = ok;
-    real nq = real(0), nQx = real(0), nQy = real(0), nQz = real(0), nK = real(0);
+    real nq = real(0), nQx = real(0), nQy = real(0), nQz = real(0), nK = real(0), nS = real(0);
     if (nk > 0) {
       nq = fld(bk, sk, 0, oj), nQx = fld(bk, sk, 1, oj), nQy = fld(bk, sk, 2, oj);
       nQz = fld(bk, sk, 3, oj), nK = fld(bk, sk, 4, oj);
+      if constexpr (CUT) nS = fld(bk, sk, 5, oj);
     }
     for (int j = 0; j < nk; ++j) {
       const real q = nq, X = Px - nQx, Y = Py - nQy, Z = Pz - nQz, Kcd = nK;
+      const bool on = !CUT || sab * nS >= qcut;
+      if (CUT && !__any(on)) break;
       const unsigned ojc = oj;
       if (j + 1 < nk) oj += (unsigned)(S::RSK * sk);
       nq = fld(bk, sk, 0, oj), nQx = fld(bk, sk, 1, oj), nQy = fld(bk, sk, 2, oj);
       nQz = fld(bk, sk, 3, oj), nK = fld(bk, sk, 4, oj);
+      if constexpr (CUT) nS = fld(bk, sk, 5, oj);
       // K_ab and K_cd carry 1/p and 1/q (pair pre-pass), so
       // pref = 2 pi^{5/2} K_ab K_cd / sqrt(p+q) and alpha = p q / (p+q)
       const real rs = rsqrt_nr(p + q);
       const real alpha = p * q * rs * rs;
-      const real pref = real(TWO_PI_2P5) * Kab * Kcd * rs;
+      // a lane's sub-cut terms are multiplied by exactly 0 (the wave runs on
+      // while any lane is above the cut)
+      const real pref = (!CUT || on) ? real(TWO_PI_2P5) * Kab * Kcd * rs : real(0);
       const real T = alpha * (X * X + Y * Y + Z * Z);
       real F[S::L + 1];
       boys_eval<S::L, real>(T, btab, F);

@@ -44,11 +44,12 @@ struct Target {        // one ordered shell pair (X, Y) of a half matrix
   int nrow, ncol, stride, pad;
 };
 
-// pair pre-pass: one lane per primitive pair; writes FP64 and FP32 records
+// pair pre-pass: one lane per primitive pair; writes FP64 and / or FP32
+// records (either output may be null), with Schwarz factors sq (or 0 if null)
 hipError_t launch_pair_records(const int4* pairs_prim /* sh_a, sh_b, i_a, i_b */, int nprim_pairs,
                                const int* rec_off /* per primitive pair, in reals */,
                                const ShellDev* shells, const double* exps, const double* coefs,
-                               double* rec64, float* rec32, hipStream_t s);
+                               const double* sq, double* rec64, float* rec32, hipStream_t s);
 
 // true if the lane-per-quartet kernel is instantiated for this combination
 // (s/p classes and the light d classes); the others use the column kernel
@@ -96,7 +97,8 @@ hipError_t launch_eri_jk(int combo, bool fp32, const int2* quartets, const long 
 // summed over the splb chunks.
 hipError_t launch_eri_chunk(int combo, bool fp32, const int2* quartets, int first, int n, int splb,
                             const int4* pairs, const RecView& rv, double* out, long long xs, long long qs,
-                            long long ss, const double* boys_tab, int block, int grid_cap, hipStream_t s);
+                            long long ss, const double* boys_tab, int block, int grid_cap, bool prim_cut,
+                            hipStream_t s);
 hipError_t launch_jk_digest(int combo, const int2* quartets, const long long* slots, int n, int splb, long long ss,
                             const int4* pairs, const int4* shell_dm, const double* DJ, const double* DK0,
                             const double* DK1, int nk, const double* ibuf, double* stage, hipStream_t s);

@@ -28,10 +28,10 @@ template <int LA, int LB, int LC, int LD, typename real>
 __device__ __forceinline__ void quartet_column(const real* __restrict__ bb, int sb, unsigned ob, int nb,
                                                const real* __restrict__ bk, unsigned ok, int nk, int cd,
                                                const double* __restrict__ btab,
-                                               real (&I)[ncart(LA) * ncart(LB)]) {
+                                               real (&I)[ncart(LA) * ncart(LB)], real qcut) {
   using S = QuartetShape<LA, LB, LC, LD, real>;
   constexpr int LCD = LC + LD;
-  constexpr int EXB = 5, EYB = 5 + S::NEB, EZB = 5 + 2 * S::NEB;
+  constexpr int EXB = REC_HEAD, EYB = REC_HEAD + S::NEB, EZB = REC_HEAD + 2 * S::NEB;
   // component sums and E offsets of the column's (c, d) per axis (-1 marks
   // E_0^{00} = 1, which is not stored); scalar work when cd is wave-uniform
   const int c = cd / S::ND, d = cd - (cd / S::ND) * S::ND;
@@ -46,8 +46,14 @@ __device__ __forceinline__ void quartet_column(const real* __restrict__ bb, int 
   const int eox = eoa[0], eoy = eoa[1], eoz = eoa[2];
 #pragma unroll
   for (int x = 0; x < S::NAB; ++x) I[x] = real(0);
+  // primitive-quartet cut as in quartet_integrals (records in decreasing
+  // Schwarz factor; a lane's sub-cut terms are left out exactly; qcut = 0:
+  // none)
+  const real sk0 = nk > 0 ? bk[ok + 5] : real(0);
   for (int i = 0; i < nb; ++i) {
     const unsigned oi = ob + (unsigned)(i * S::RSB * sb);
+    const real sab = fld(bb, sb, 5, oi);
+    if (sab * sk0 < qcut) break;
     const real p = fld(bb, sb, 0, oi), Px = fld(bb, sb, 1, oi), Py = fld(bb, sb, 2, oi);
     const real Pz = fld(bb, sb, 3, oi), Kab = fld(bb, sb, 4, oi);
     real G[S::NHB];
@@ -55,11 +61,13 @@ __device__ __forceinline__ void quartet_column(const real* __restrict__ bb, int 
     for (int x = 0; x < S::NHB; ++x) G[x] = real(0);
     for (int j = 0; j < nk; ++j) {
       const real* rk = bk + ok + (unsigned)(j * S::RSK);
+      const bool on = sab * rk[5] >= qcut;
+      if (!__any(on)) break;
       const real q = rk[0];
       const real X = Px - rk[1], Y = Py - rk[2], Z = Pz - rk[3];
       const real rs = rsqrt_nr(p + q);
       const real alpha = p * q * rs * rs;
-      const real pref = real(TWO_PI_2P5) * Kab * rk[4] * rs;
+      const real pref = on ? real(TWO_PI_2P5) * Kab * rk[4] * rs : real(0);
       const real T = alpha * (X * X + Y * Y + Z * Z);
       real F[S::L + 1];
       boys_eval<S::L, real>(T, btab, F);
@@ -68,7 +76,7 @@ __device__ __forceinline__ void quartet_column(const real* __restrict__ bb, int 
       // the lane's ket E factors with the (-1)^t sign and pref (x axis)
       // folded in; zero beyond the component sums
       real ex[LCD + 1], ey[LCD + 1], ez[LCD + 1];
-      const real* E = rk + 5;
+      const real* E = rk + REC_HEAD;
 #pragma unroll
       for (int t = 0; t <= LCD; ++t) {
         const real sg = (t & 1) ? real(-1) : real(1);
@@ -128,7 +136,7 @@ __global__ __launch_bounds__(256) void eri_chunk_kernel(const int2* __restrict__
                                                         const real* __restrict__ recb, int sb, int fb,
                                                         const real* __restrict__ reck, double* __restrict__ out,
                                                         long long xs, long long qs, long long ss,
-                                                        const double* __restrict__ boys_gtab) {
+                                                        const double* __restrict__ boys_gtab, real qcut) {
   using S = QuartetShape<LA, LB, LC, LD, real>;
   __shared__ double btab[BOYS_LDS];
   boys_stage<S::L>(boys_gtab, btab);
@@ -156,9 +164,9 @@ __global__ __launch_bounds__(256) void eri_chunk_kernel(const int2* __restrict__
     const int nkf = __builtin_amdgcn_readfirstlane(pk.w);
     const int cdf = __builtin_amdgcn_readfirstlane(cd);
     if (__all((unsigned)pk.z == okf && pk.w == nkf && cd == cdf))
-      quartet_column<LA, LB, LC, LD, real>(recb, sb, ob, nbp, reck, okf, nkf, cdf, btab, I);
+      quartet_column<LA, LB, LC, LD, real>(recb, sb, ob, nbp, reck, okf, nkf, cdf, btab, I, qcut);
     else
-      quartet_column<LA, LB, LC, LD, real>(recb, sb, ob, nbp, reck, (unsigned)pk.z, pk.w, cd, btab, I);
+      quartet_column<LA, LB, LC, LD, real>(recb, sb, ob, nbp, reck, (unsigned)pk.z, pk.w, cd, btab, I, qcut);
     double* o = out + (long long)sp * ss + (long long)q * qs;
 #pragma unroll
     for (int ab = 0; ab < S::NAB; ++ab) o[(long long)(ab * S::NCD + cd) * xs] = (double)I[ab];
@@ -288,7 +296,7 @@ int chunk_slices(int combo) {
 
 hipError_t launch_eri_chunk(int combo, bool fp32, const int2* q, int first, int n, int splb, const int4* pairs,
                             const RecView& rv, double* out, long long xs, long long qs, long long ss,
-                            const double* bt, int block, int cap, hipStream_t s) {
+                            const double* bt, int block, int cap, bool cut, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   if (block != 64 && block != 128 && block != 256) return hipErrorInvalidValue;
   switch (combo) {
@@ -298,10 +306,12 @@ hipError_t launch_eri_chunk(int combo, bool fp32, const int2* q, int first, int 
     dim3 grid(chunk_grid(lanes, cap, block)), blk(block);                                                       \
     if (fp32)                                                                                                   \
       hipLaunchKernelGGL((eri_chunk_kernel<la, lb, lc, ld, float>), grid, blk, 0, s, q, first, n, splb, pairs,   \
-                         (const float*)rv.bra, rv.sb, rv.fb, (const float*)rv.ket, out, xs, qs, ss, bt);       \
+                         (const float*)rv.bra, rv.sb, rv.fb, (const float*)rv.ket, out, xs, qs, ss, bt,        \
+                         cut ? PRIM_QCUT32 : 0.0f);                                                             \
     else                                                                                                        \
       hipLaunchKernelGGL((eri_chunk_kernel<la, lb, lc, ld, double>), grid, blk, 0, s, q, first, n, splb, pairs,  \
-                         (const double*)rv.bra, rv.sb, rv.fb, (const double*)rv.ket, out, xs, qs, ss, bt);     \
+                         (const double*)rv.bra, rv.sb, rv.fb, (const double*)rv.ket, out, xs, qs, ss, bt,      \
+                         cut ? PRIM_QCUT64 : 0.0);                                                              \
     return hipGetLastError();                                                                                   \
   }
     DFT_FOR_HEAVY(X)

@@ -61,10 +61,13 @@ __global__ __launch_bounds__(jk_block_max(LA, LB, LC, LD)) void eri_jk_kernel(
     // the loop a scalar loop; otherwise the same code runs per lane.
     const unsigned okf = (unsigned)__builtin_amdgcn_readfirstlane((int)ok);
     const int nkf = __builtin_amdgcn_readfirstlane(nkp);
+    // the primitive-quartet cut pays in FP64 (the FP32 records are already
+    // cut per pair, and the test costs a larger share of the FP32 loop)
+    constexpr bool cut = std::is_same<real, double>::value;
     if (__all(ok == okf && nkp == nkf)) {
-      quartet_integrals<LA, LB, LC, LD, real>(recb, sb, ob, nbp, reck, 1, okf, nkf, btab, I);
+      quartet_integrals<LA, LB, LC, LD, real, cut>(recb, sb, ob, nbp, reck, 1, okf, nkf, btab, I, PRIM_QCUT64);
     } else {
-      quartet_integrals<LA, LB, LC, LD, real>(recb, sb, ob, nbp, reck, 1, ok, nkp, btab, I);
+      quartet_integrals<LA, LB, LC, LD, real, cut>(recb, sb, ob, nbp, reck, 1, ok, nkp, btab, I, PRIM_QCUT64);
     }
     double w = 1.0;
     if (pb.x == pb.y) w *= 0.5;

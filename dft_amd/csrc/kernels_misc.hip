@@ -7,14 +7,15 @@ namespace dft {
 
 // ------------------------------------------------------- pair pre-pass
 // One lane per primitive pair (i_a of shell a, i_b of shell b). Record:
-//   p, Px, Py, Pz, K_ab = c_a c_b exp(-ab/p |A-B|^2) / p, then per axis the
+//   p, Px, Py, Pz, K_ab = c_a c_b exp(-ab/p |A-B|^2) / p, the Schwarz factor
+//   sq[idx] (0 without sq), then per axis the
 //   table E_t^{ia,ib} (ia <= la, ib <= lb) of calc_E (shelled.py:228-294),
 //   with E_0^{00} = 1 and the angular factor 1/sqrt((2ia-1)!!(2ib-1)!!).
 // a is raised first (X_PA = -b Q/p), then b (X_PB = a Q/p), as in calc_E.
 __global__ void pair_records_kernel(const int4* __restrict__ pp, int n, const int* __restrict__ rec_off,
                                     const ShellDev* __restrict__ sh, const double* __restrict__ exps,
-                                    const double* __restrict__ coefs, double* __restrict__ r64,
-                                    float* __restrict__ r32) {
+                                    const double* __restrict__ coefs, const double* __restrict__ sq,
+                                    double* __restrict__ r64, float* __restrict__ r32) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= n) return;
   const int4 q = pp[idx];
@@ -29,11 +30,12 @@ __global__ void pair_records_kernel(const int4* __restrict__ pp, int n, const in
   // either output may be absent (FP64 and FP32 records have separate lists)
   double* o64 = r64 ? r64 + rec_off[idx] : nullptr;
   float* o32 = r32 ? r32 + rec_off[idx] : nullptr;
-  double head[5];
+  double head[REC_HEAD];
   for (int d = 0; d < 3; ++d) head[1 + d] = (a * Ac[d] + b * Bc[d]) / p;
   head[0] = p;
   head[4] = ca * cb * exp(-mu * (AB[0] * AB[0] + AB[1] * AB[1] + AB[2] * AB[2])) / p;
-  for (int k = 0; k < 5; ++k) {
+  head[5] = sq ? sq[idx] : 0.0;
+  for (int k = 0; k < REC_HEAD; ++k) {
     if (o64) o64[k] = head[k];
     if (o32) o32[k] = (float)head[k];
   }
@@ -60,7 +62,7 @@ __global__ void pair_records_kernel(const int4* __restrict__ pp, int n, const in
       for (int ib = 0; ib <= lb; ++ib) {
         const int ord = ia + ib;
         const double nrm = 1.0 / sqrt(dfac[ia] * dfac[ib]);
-        const int o = 5 + d * NE + eoff(lb, ia, ib);
+        const int o = REC_HEAD + d * NE + eoff(lb, ia, ib);
         for (int t = 0; t <= ord; ++t) {
           if (o64) o64[o + t] = cur[t] * nrm;
           if (o32) o32[o + t] = (float)(cur[t] * nrm);
@@ -80,11 +82,11 @@ __global__ void pair_records_kernel(const int4* __restrict__ pp, int n, const in
 }
 
 hipError_t launch_pair_records(const int4* pp, int n, const int* rec_off, const ShellDev* sh,
-                               const double* exps, const double* coefs, double* r64, float* r32,
-                               hipStream_t s) {
+                               const double* exps, const double* coefs, const double* sq, double* r64,
+                               float* r32, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(pair_records_kernel, dim3((n + 127) / 128), dim3(128), 0, s, pp, n, rec_off, sh, exps,
-                     coefs, r64, r32);
+                     coefs, sq, r64, r32);
   return hipGetLastError();
 }
 

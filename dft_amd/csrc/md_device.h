@@ -82,8 +82,11 @@ __host__ __device__ constexpr int eoff(int lb, int ia, int ib) {
   for (int b = 0; b < ib; ++b) o += ia + b + 1;
   return o;
 }
-// primitive-pair record: p, Px, Py, Pz, Kab, then Ex[NE], Ey[NE], Ez[NE]
-__host__ __device__ constexpr int rec_size(int la, int lb) { return 5 + 3 * ne_axis(la, lb); }
+// primitive-pair record: p, Px, Py, Pz, Kab, s (Schwarz factor
+// sqrt(max_x (k k|k k)_xx) of the primitive pair, 0 where not computed), then
+// Ex[NE], Ey[NE], Ez[NE]
+constexpr int REC_HEAD = 6;
+__host__ __device__ constexpr int rec_size(int la, int lb) { return REC_HEAD + 3 * ne_axis(la, lb); }
 
 // ------------------------------------------------------------- Boys F_m
 // Table of F_m(T_k), T_k = k * BOYS_DT, k < BOYS_NG, m < BOYS_MT, generated on
