@@ -240,7 +240,8 @@ def main():
                    "n_unscreened": st["n_unscreened"], "n_prim_quartets": st["n_prim_quartets"],
                    "parallelism": "single GPU"},
         "per_combo": {"columns": ["fp64 quartets", "fp32-eligible quartets", "fp64 prim quartets",
-                                  "fp32 prim quartets", "fp64 K-products", "fp32 K-products"],
+                                  "fp32 prim quartets", "fp64 K-products", "fp32 K-products",
+                                  "fp32 level-2 quartets"],
                       "rows": [list(map(int, r)) for r in per]},
         "s_per_iter_fock_fp64": ms_per_step * 1e-3,
         "shell_quartets_per_s": quartets / (ms_per_step * 1e-3),

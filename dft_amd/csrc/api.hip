@@ -1299,12 +1299,14 @@ int dft_plan_stats(dft_plan p, int64_t* counts, int ncombo, int* ncombo_out, int
   if (ncombo_out) *ncombo_out = (int)p->combos.size();
   if (counts)
     for (int c = 0; c < std::min<int>(ncombo, (int)p->combos.size()); ++c) {
-      counts[6 * c] = (int64_t)p->combos[c].n64;
-      counts[6 * c + 1] = (int64_t)p->combos[c].q.size() - p->combos[c].n64;
-      counts[6 * c + 2] = p->combos[c].prim64;
-      counts[6 * c + 3] = p->combos[c].prim32;
-      counts[6 * c + 4] = p->combos[c].kprod64;
-      counts[6 * c + 5] = p->combos[c].kprod32;
+      const Combo& cb = p->combos[c];
+      counts[DFT_STATS_COLS * c] = (int64_t)cb.n64;
+      counts[DFT_STATS_COLS * c + 1] = (int64_t)cb.q.size() - cb.n64;
+      counts[DFT_STATS_COLS * c + 2] = cb.prim64;
+      counts[DFT_STATS_COLS * c + 3] = cb.prim32;
+      counts[DFT_STATS_COLS * c + 4] = cb.kprod64;
+      counts[DFT_STATS_COLS * c + 5] = cb.kprod32;
+      counts[DFT_STATS_COLS * c + 6] = (int64_t)cb.q.size() - cb.n64 - cb.n32a;
     }
   if (n_unscreened) *n_unscreened = p->n_unscreened;
   if (staging_bytes) *staging_bytes = p->stage_len * (int64_t)sizeof(double);

@@ -91,19 +91,20 @@ class Plan:
 
     # ------------------------------------------------------------ stats
     def stats(self) -> dict:
-        counts = np.zeros(6 * 64, dtype=np.int64)
+        counts = np.zeros(_lib.STATS_COLS * 64, dtype=np.int64)
         nc = C.c_int()
         nun = C.c_int64()
         sb = C.c_int64()
         _lib.check(self.lib.dft_plan_stats(self.handle, _lib.ptr(counts, C.c_int64), 64, C.byref(nc),
                                            C.byref(nun), C.byref(sb)))
-        c = counts[:6 * nc.value].reshape(-1, 6)
+        c = counts[:_lib.STATS_COLS * nc.value].reshape(-1, _lib.STATS_COLS)
         # contracted Cartesian AO quartets, counted at shell level (prod n(l))
         ao = [np.prod([_ncart(l) for l in combo_moments(k)]) for k in range(len(c))]
         n_ao = int(sum(int(a) * int(row[0] + row[1]) for a, row in zip(ao, c)))
         return {"per_combo": c, "n_quartets_fp64": int(c[:, 0].sum()), "n_quartets_fp32": int(c[:, 1].sum()),
                 "n_quartets": int(c[:, :2].sum()), "n_prim_quartets": int(c[:, 2:4].sum()),
                 "n_prim_quartets_unscreened": int(c[:, 4:6].sum()),
+                "n_quartets_fp32_level2": int(c[:, 6].sum()),
                 "n_ao_quartets": n_ao,
                 "n_unscreened": int(nun.value), "staging_bytes": int(sb.value)}
 

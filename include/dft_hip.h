@@ -137,17 +137,22 @@ int dft_plan_tune(dft_plan plan, const double* P, const double* Pa, const double
 int dft_plan_set_shapes(dft_plan plan, const int* shapes, int n);
 
 /*
- * Plan statistics. counts receives, for each class combination c (pair
- * classes ss=0, ps=1, pp=2, ds=3, dp=4, dd=5; combination index
- * pcb*(pcb+1)/2 + pck):
- *   counts[6*c+0] FP64 quartets, [6*c+1] FP32 quartets,
- *   [6*c+2] primitive shell quartets evaluated in the FP64 part, [6*c+3] in
- *   the FP32 part (after primitive-pair screening), [6*c+4] and [6*c+5] the
- *   sums of K_a K_b K_c K_d over the same quartets (before it).
+ * Plan statistics. counts receives DFT_STATS_COLS values for each class
+ * combination c (pair classes ss=0, ps=1, pp=2, ds=3, dp=4, dd=5;
+ * combination index pcb*(pcb+1)/2 + pck), at counts[DFT_STATS_COLS*c + k]:
+ *   k=0 FP64 quartets, 1 FP32 quartets,
+ *   2 primitive shell quartets of the FP64 part, 3 of the FP32 part (after
+ *     primitive-pair screening and, for FP32, the FP32 primitive lists; the
+ *     kernels' run-time primitive-quartet cut is not counted),
+ *   4 and 5 the sums of K_a K_b K_c K_d over the same quartets (before any
+ *     primitive screening),
+ *   6 FP32 quartets on the second FP32 record level (Schwarz product below
+ *     1e-3 of the FP32 threshold, in classes where that part is large).
  * ncombo is the capacity of counts in combinations; the number of
  * combinations is returned in *ncombo_out. Also reports the quartets before
  * screening and the staging size in bytes.
  */
+#define DFT_STATS_COLS 7
 int dft_plan_stats(dft_plan plan, int64_t* counts, int ncombo, int* ncombo_out, int64_t* n_unscreened,
                    int64_t* staging_bytes);
 

@@ -180,6 +180,30 @@ def test_jk_spd_vs_dense(torch_cuda, uhf):
         assert np.abs(Kb.cpu().numpy() - Kbr).max() < scale
 
 
+def test_mixed_two_fp32_levels_water_cluster(torch_cuda):
+    """(H2O)48 STO-3G is large enough for the second FP32 record level (over
+    2^20 quartets of a class below 1e-3 of the FP32 threshold, with a 1000x
+    looser primitive-pair cut). Mixed J/K stay within 1e-7 of the largest
+    FP64 element (the FP64 path itself is checked against the oracle in the
+    tests above)."""
+    from dft_amd.basis import build_shells, load_basis
+    from dft_amd.engine import Plan
+    from dft_amd.geometry import water_cluster
+    Z, xyz = water_cluster((4, 4, 3), 3.1, seed=3)
+    shells = build_shells(load_basis(), Z, xyz)
+    plan = Plan(shells, Z, xyz, schwarz_tau=1e-10, fp32_tau=1e-4)
+    st = plan.stats()
+    assert st["n_quartets_fp32_level2"] > 0
+    rng = np.random.default_rng(5)
+    Pa = _rand_density(plan.nbf, rng) * 0.05
+    J64, K64, _ = plan.jk(2 * Pa, Pa, mode="fp64")
+    J32, K32, _ = plan.jk(2 * Pa, Pa, mode="mixed")
+    dJ = (J64 - J32).abs().max().item() / J64.abs().max().item()
+    dK = (K64 - K32).abs().max().item() / K64.abs().max().item()
+    print(f"level-2 quartets {st['n_quartets_fp32_level2']}, rel dJ {dJ:.2e}, rel dK {dK:.2e}")
+    assert dJ < 1e-7 and dK < 1e-7
+
+
 def test_mixed_precision_spd_error_small(torch_cuda):
     g = golden("methane_spd.npz")
     n = g["ERI"].shape[0]
