@@ -210,11 +210,11 @@ def main():
         combo = dom // 2
         nq_dom = int(per[combo][0] + per[combo][1])
         f_dom = combo_flops(combo, nq_dom, int(per[combo][4] + per[combo][5]))
-        f_dom_eval = combo_flops(combo, nq_dom, int(per[combo][2] + per[combo][3]))
+        f_dom_eval = combo_flops(combo, nq_dom, int(per[combo][2] + per[combo][7]))
         t_dom = eri_ms[dom] / nrec * 1e-3
         achieved = f_dom / t_dom / 1e12
         f_all = sum(build_flops(per))          # whole list in FP64 in this mode
-        f_all_eval = sum(build_flops(per, model=False))
+        f_all_eval = sum(build_flops(per, model=False, mode="fp64"))
         from dft_amd.flops import combo_moments
         kname = "eri_jk_kernel<%d, %d, %d, %d, double, 1>" % combo_moments(combo)
         return {"bound": "mfma", "pipe": "fp64 VALU (MFMA FP64 peak is the same 78.6 TFLOP/s)",
@@ -241,7 +241,7 @@ def main():
                    "parallelism": "single GPU"},
         "per_combo": {"columns": ["fp64 quartets", "fp32-eligible quartets", "fp64 prim quartets",
                                   "fp32 prim quartets", "fp64 K-products", "fp32 K-products",
-                                  "fp32 level-2 quartets"],
+                                  "fp32 level-2 quartets", "fp64 prim quartets of the fp32-eligible part"],
                       "rows": [list(map(int, r)) for r in per]},
         "s_per_iter_fock_fp64": ms_per_step * 1e-3,
         "shell_quartets_per_s": quartets / (ms_per_step * 1e-3),

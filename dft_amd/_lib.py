@@ -15,7 +15,7 @@ DFT_OK, DFT_EINVAL, DFT_EUNSUPPORTED, DFT_EHIP = 0, -1, -2, -3
 DFT_PLAN_UHF = 1
 DFT_PLAN_NOSPLIT = 2
 DFT_PLAN_SERIAL = 4
-STATS_COLS = 7  # DFT_STATS_COLS: values per class combination in dft_plan_stats
+STATS_COLS = 8  # DFT_STATS_COLS: values per class combination in dft_plan_stats
 DFT_FP64, DFT_MIXED = 0, 1
 MODES = {"fp64": DFT_FP64, "mixed": DFT_MIXED}
 

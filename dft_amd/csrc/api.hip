@@ -1307,6 +1307,7 @@ int dft_plan_stats(dft_plan p, int64_t* counts, int ncombo, int* ncombo_out, int
       counts[DFT_STATS_COLS * c + 4] = cb.kprod64;
       counts[DFT_STATS_COLS * c + 5] = cb.kprod32;
       counts[DFT_STATS_COLS * c + 6] = (int64_t)cb.q.size() - cb.n64 - cb.n32a;
+      counts[DFT_STATS_COLS * c + 7] = cb.prim64w;
     }
   if (n_unscreened) *n_unscreened = p->n_unscreened;
   if (staging_bytes) *staging_bytes = p->stage_len * (int64_t)sizeof(double);

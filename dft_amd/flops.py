@@ -61,17 +61,23 @@ def combo_flops(combo, n_quartets, n_prim_quartets, nk=1):
     return n_prim_quartets * f_prim(la, lb, lc, ld) + n_quartets * f_digest(la, lb, lc, ld, nk)
 
 
-def build_flops(per_combo, nk=1, model=True):
-    """(FP64 flops, FP32 flops) of one Fock build from Plan.stats()['per_combo'].
+def build_flops(per_combo, nk=1, model=True, mode="mixed"):
+    """(FP64-part flops, FP32-eligible-part flops) of one Fock build from
+    Plan.stats()['per_combo'].
 
     ``model=True`` counts K_a K_b K_c K_d primitive quartets per surviving
-    contracted quartet (the algorithmic model); ``model=False`` counts only the
-    primitive quartets that survive primitive-pair screening (what the
-    kernels evaluate).
+    contracted quartet (the algorithmic model); ``model=False`` counts the
+    primitive quartets left by primitive-pair screening (the kernels' own
+    run-time primitive-quartet cut removes more), with the FP32 records in
+    ``mode="mixed"`` and the FP64 ones for the FP32-eligible part in
+    ``mode="fp64"``.
     """
-    col = 4 if model else 2
     f64 = f32 = 0
     for c, row in enumerate(per_combo):
-        f64 += combo_flops(c, int(row[0]), int(row[col]), nk)
-        f32 += combo_flops(c, int(row[1]), int(row[col + 1]), nk)
+        if model:
+            p64, p32 = row[4], row[5]
+        else:
+            p64, p32 = row[2], (row[3] if mode == "mixed" else row[7])
+        f64 += combo_flops(c, int(row[0]), int(p64), nk)
+        f32 += combo_flops(c, int(row[1]), int(p32), nk)
     return f64, f32

@@ -147,12 +147,14 @@ int dft_plan_set_shapes(dft_plan plan, const int* shapes, int n);
  *   4 and 5 the sums of K_a K_b K_c K_d over the same quartets (before any
  *     primitive screening),
  *   6 FP32 quartets on the second FP32 record level (Schwarz product below
- *     1e-3 of the FP32 threshold, in classes where that part is large).
+ *     1e-3 of the FP32 threshold, in classes where that part is large),
+ *   7 primitive shell quartets of the FP32 part with the FP64 records (what
+ *     an all-FP64 build evaluates there).
  * ncombo is the capacity of counts in combinations; the number of
  * combinations is returned in *ncombo_out. Also reports the quartets before
  * screening and the staging size in bytes.
  */
-#define DFT_STATS_COLS 7
+#define DFT_STATS_COLS 8
 int dft_plan_stats(dft_plan plan, int64_t* counts, int ncombo, int* ncombo_out, int64_t* n_unscreened,
                    int64_t* staging_bytes);
 
