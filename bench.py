@@ -234,7 +234,7 @@ def main():
         "value": value, "unit": "AO quartets/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "(H2O)16 STO-3G direct J/K (config 1): 4x2x2 lattice 3.1 A, seed-0 rotations, "
+        "config": {"workload": "(H2O)16 STO-3G direct J/K: 4x2x2 lattice 3.1 A, seed-0 rotations, "
                                "112 bf, Schwarz 1e-10", "n_ao_quartets": st["n_ao_quartets"],
                    "n_quartets": quartets,
                    "n_unscreened": st["n_unscreened"], "n_prim_quartets": st["n_prim_quartets"],
