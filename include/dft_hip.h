@@ -54,7 +54,8 @@ const char* dft_last_error(void);
 
 /*
  * Build the integral plan of one molecule: primitive-pair records (pair
- * pre-pass), Schwarz bounds Q_ab = sqrt(max |(ab|ab)|), the screened,
+ * pre-pass; FP64 and two FP32 record sets, each primitive pair with its own
+ * Schwarz factor), Schwarz bounds Q_ab = sqrt(max |(ab|ab)|), the screened,
  * angular-momentum-class-sorted quartet lists (FP64 and FP32 parts) and the
  * fixed-order reduction map.
  *
@@ -171,6 +172,10 @@ int dft_schwarz(dft_plan plan, double* Q);
  * class launches run concurrently on the caller's stream and the plan's
  * auxiliary streams, joined back into the caller's stream before the
  * reduction; calls on one plan must not overlap (they share the staging).
+ * Primitive quartets whose primitive Schwarz bound is below 1e-14 are left
+ * out (each integral changes by < 1.3e-11); in DFT_MIXED the FP32 quartets
+ * use FP32 primitive lists cut to the same order of error relative to their
+ * Schwarz bound (below 1e-11 absolute).
  *
  * Replaces fill_matrices(shells, P, P_a, P_b, J, K_a, K_b)
  * (shelled.py:770-877) together with the ERI store it reads
