@@ -32,6 +32,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_PEAK = 78.6e12  # MI355X dense FP64 (vector and MFMA), FLOP/s
+FP32_PEAK = 157.3e12  # MI355X FP32 vector, FLOP/s
 
 
 def log(msg):
@@ -85,7 +86,22 @@ def cpu_baseline(shells, Q, tau, P, seconds, n_quartets_gpu):
     s = quart[perm[:n]]
     n_ao = int((nc[s[:, 0]] * nc[s[:, 1]] * nc[s[:, 2]] * nc[s[:, 3]]).sum())
     rate = n / t
+    # one thread on 1/threads of the sample (about the threaded run's time)
+    n1 = max(1, n // max(threads, 1))
+    t1 = time.perf_counter()
+    ref.build_jk(seg, quart[perm[:n1]], P, P / 2, None, nthreads=1)
+    t1 = time.perf_counter() - t1
+    s1 = quart[perm[:n1]]
+    n_ao1 = int((nc[s1[:, 0]] * nc[s1[:, 1]] * nc[s1[:, 2]] * nc[s1[:, 3]]).sum())
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {"value": n_ao / t, "unit": "AO quartets/s", "cores": threads, "kind": "port",
+            "cpu_model": model, "value_1_thread": n_ao1 / t1,
+            "sample_1_thread": f"{n1} quartets of the same sample, {t1:.1f} s",
             "sample": f"{n} of {len(quart)} screened unique shell quartets ({n_ao} AO quartets), uniform random "
                       f"(seed 0), {t:.1f} s; C restatement of the reference per-primitive-Cartesian MD path "
                       f"(oracle/md_ref.c, OpenMP {threads} threads)",
@@ -128,7 +144,7 @@ def main():
     from dft_amd.engine import Plan, profile_read, profile_start, tune
     from dft_amd.flops import build_flops, combo_flops
     from dft_amd.geometry import water_cluster
-    from dft_amd.scf import scf_plain
+    from dft_amd.scf import scf_diis, scf_plain
 
     tau = 1e-10
     Z, xyz = water_cluster((4, 2, 2), 3.1, seed=0)
@@ -175,6 +191,16 @@ def main():
     wall = time.perf_counter() - t0
     ms_total = e0.elapsed_time(e1)
     ms_per_step = ms_total / args.steps
+    # per-build times (outside the timed region: one event pair per build),
+    # median over builds 2..K
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+    ev[0].record()
+    for i in range(args.steps):
+        run(1, "fp64")
+        ev[i + 1].record()
+    torch.cuda.synchronize()
+    per_build = [ev[i].elapsed_time(ev[i + 1]) for i in range(args.steps)]
+    ms_median = float(np.median(per_build[1:])) if args.steps > 1 else per_build[0]
     quartets = st["n_quartets"]
     value = st["n_ao_quartets"] / (ms_per_step * 1e-3)
     log(f"fp64: {ms_per_step:.3f} ms/build, {value:.3e} AO quartets/s, "
@@ -244,6 +270,7 @@ def main():
                                   "fp32 level-2 quartets", "fp64 prim quartets of the fp32-eligible part"],
                       "rows": [list(map(int, r)) for r in per]},
         "s_per_iter_fock_fp64": ms_per_step * 1e-3,
+        "s_per_iter_fock_fp64_median_2_to_K": ms_median * 1e-3,
         "shell_quartets_per_s": quartets / (ms_per_step * 1e-3),
         "prim_shell_quartets_per_s": st["n_prim_quartets"] / (ms_per_step * 1e-3),
         "plan_seconds": t_plan,
@@ -265,17 +292,24 @@ def main():
     e1.record()
     torch.cuda.synchronize()
     ms_mixed = e0.elapsed_time(e1) / args.steps
-    scf64 = scf_plain(plan, Z, xyz, nocc, nocc, mode="fp64")
-    scf32 = scf_plain(plan, Z, xyz, nocc, nocc, mode="mixed")
+    # converged SCF energies in both modes (damped + DIIS loop; the plain
+    # Roothaan loop oscillates on this cluster)
+    scf64 = scf_diis(plan, Z, xyz, nocc, nocc, mode="fp64")
+    scf32 = scf_diis(plan, Z, xyz, nocc, nocc, mode="mixed")
+    f64_part, f32_part = build_flops(per)
     log(f"mixed: {ms_mixed:.3f} ms/build; E64 {scf64.energy:.10f} E_mixed {scf32.energy:.10f}")
 
     out.update({
         "s_per_iter_fock_mixed": ms_mixed * 1e-3,
         "mixed_speedup": ms_per_step / ms_mixed,
         "mixed_fp32_quartets": st["n_quartets_fp32"],
+        # model flops of the FP64 part at the FP64 peak plus the FP32 part at
+        # the FP32 vector peak, over the mixed build time
+        "mixed_whole_build_frac": (f64_part / FP64_PEAK + f32_part / FP32_PEAK) / (ms_mixed * 1e-3),
         "scf_energy_fp64": scf64.energy, "scf_energy_mixed": scf32.energy,
         "abs_dE_mixed_Eh": abs(scf64.energy - scf32.energy),
         "scf_iterations_fp64": scf64.iterations,
+        "scf_converged": [bool(scf64.converged), bool(scf32.converged)],
         "scf_s_per_iter_wall": float(np.median(scf64.iter_times[1:])) if len(scf64.iter_times) > 1 else None,
     })
     if not args.no_cpu:

@@ -205,8 +205,9 @@ def scf_diis(plan: Plan, Z, centers, N_a, N_b, mode="fp64", rate=0.4, diis_thres
     strikes = 0
     use_diis = False
     count = 0
-    energies = []
+    energies, iter_times = [], []
     while True:
+        t0 = time.perf_counter()
         P = P_a + P_b
         J, Ka, Kb = fock(P, P_a, P_b)
         Fa = H + J - Ka
@@ -231,6 +232,7 @@ def scf_diis(plan: Plan, Z, centers, N_a, N_b, mode="fp64", rate=0.4, diis_thres
         Pa_new, Pb_new = UHF_density(Ca, Cb, N_a, N_b, P_a, P_b, rate)
         dP = float(torch.maximum((Pa_new - P_a).abs().max(), (Pb_new - P_b).abs().max()))
         energies.append(E_new)
+        iter_times.append(time.perf_counter() - t0)  # wall, synchronised by the float() reads
         if (dE > prev_dE or dP > prev_dP) and not use_diis:
             if strikes == 3:
                 rate /= 1.1
@@ -249,7 +251,7 @@ def scf_diis(plan: Plan, Z, centers, N_a, N_b, mode="fp64", rate=0.4, diis_thres
             ea, Ca = torch.linalg.eigh(X.T @ Fa @ X)
             eb, Cb = (ea, Ca) if rhf else torch.linalg.eigh(X.T @ Fb @ X)
             return SCFResult(E_fin, dE < e_tol and dP < p_tol, count, P_a, P_b, Fa, Fb, X @ Ca, X @ Cb, ea, eb,
-                             fock.times, [], energies)
+                             fock.times, iter_times, energies)
         E_total = E_new
         prev_dE, prev_dP = dE, dP
         P_a, P_b = Pa_new, Pb_new
