@@ -217,3 +217,57 @@ def test_mixed_precision_spd_error_small(torch_cuda):
     # FP32 only on quartets with Q_ab Q_cd < 1e-4: error far below the largest element
     assert (J64 - J32).abs().max().item() < 1e-6 * max(1.0, J64.abs().max().item())
     assert (K64 - K32).abs().max().item() < 1e-6 * max(1.0, K64.abs().max().item())
+
+
+# ----------------------------------------------------------- edge cases
+
+def _dense_check(shells, plan, rng, uhf=False, tol=1e-10):
+    import oracle.md_oracle as O
+    ERI = O.eri_tensor(shells)
+    n = ERI.shape[0]
+    T = np.asarray(plan.eri_tensor())
+    assert np.abs(T - ERI).max() < tol
+    Pa, Pb = _rand_density(n, rng), _rand_density(n, rng)
+    P = Pa + Pb
+    J, Ka, Kb = plan.jk(P, Pa, Pb if uhf else None)
+    Jr, Kar, Kbr = O.jk_dense(ERI, P, Pa, Pb)
+    assert np.abs(J.cpu().numpy() - Jr).max() < tol
+    assert np.abs(Ka.cpu().numpy() - Kar).max() < tol
+    if uhf:
+        assert np.abs(Kb.cpu().numpy() - Kbr).max() < tol
+
+
+def test_single_shell_hydrogen_atom(torch_cuda):
+    """One s shell, one basis function, one quartet (both precisions)."""
+    shells, plan = _plan([1], np.zeros((1, 3)), uhf=True)
+    _dense_check(shells, plan, np.random.default_rng(1), uhf=True)
+    st = plan.stats()
+    assert st["n_quartets"] == 1
+
+
+def test_far_apart_atoms_have_empty_pairs(torch_cuda):
+    """H at 60 bohr from water: every primitive pair of a cross shell pair
+    is below the exp(-mu R^2) cut, so those pairs have no primitive records;
+    without Schwarz screening their quartets still run (and give 0)."""
+    from dft_amd.geometry import water
+    Z, xyz = water()
+    Z = list(Z) + [1]
+    xyz = np.vstack([xyz, [[60.0, 0.0, 0.0]]])
+    shells, plan = _plan(Z, xyz, tau=0.0)
+    _dense_check(shells, plan, np.random.default_rng(2))
+    shells, plan = _plan(Z, xyz, tau=1e-10, fp32_tau=1e-4)
+    for mode in ("fp64", "mixed"):
+        J, K, _ = plan.jk(np.eye(plan.nbf) * 0.1, np.eye(plan.nbf) * 0.05, mode=mode)
+        assert bool(torch_cuda.isfinite(J).all()) and bool(torch_cuda.isfinite(K).all())
+
+
+def test_everything_screened_gives_zero_jk(torch_cuda):
+    """A Schwarz threshold above every bound leaves no quartet: J = K = 0."""
+    from dft_amd.geometry import water
+    Z, xyz = water()
+    _, plan = _plan(Z, xyz, tau=1e6, fp32_tau=1e7)
+    assert plan.stats()["n_quartets"] == 0
+    n = plan.nbf
+    for mode in ("fp64", "mixed"):
+        J, K, _ = plan.jk(np.ones((n, n)), np.ones((n, n)) * 0.5, mode=mode)
+        assert float(J.abs().max()) == 0.0 and float(K.abs().max()) == 0.0
