@@ -303,6 +303,9 @@ def main():
         "s_per_iter_fock_mixed": ms_mixed * 1e-3,
         "mixed_speedup": ms_per_step / ms_mixed,
         "mixed_fp32_quartets": st["n_quartets_fp32"],
+        # the tuner keeps the FP32 launch of a class only where it pays
+        "mixed_fp32_quartets_run": (int(sum(int(per[i][1]) for i in range(len(per)) if shapes[i][3] == 0))
+                                    if shapes is not None else st["n_quartets_fp32"]),
         # model flops of the FP64 part at the FP64 peak plus the FP32 part at
         # the FP32 vector peak, over the mixed build time
         "mixed_whole_build_frac": (f64_part / FP64_PEAK + f32_part / FP32_PEAK) / (ms_mixed * 1e-3),

@@ -133,6 +133,9 @@ struct Combo {
   long long prim64w = 0;                 // FP64 primitive quartets of the FP32-eligible part
   long long kprod64 = 0, kprod32 = 0;    // sum of K_a K_b K_c K_d (before primitive screening)
   Shape shape64, shape32;        // for the FP64 launch and the FP32 (weak part) launch
+  // mixed mode runs the weak part in FP32 (the tuner clears this where one
+  // FP64 launch over the whole list is faster than the two launches)
+  bool use32 = true;
   // primitive split of the lane kernel: every quartet is evaluated by
   // splb * splk lanes, lane (cb, ck) taking bra chunk cb and ket chunk ck of
   // the primitive pairs and writing its own partial J/K blocks; chunk-major
@@ -974,19 +977,21 @@ int jk_enqueue(dft_plan p, const double* P, const double* Pa, const double* Pb, 
     const long long ntot = (long long)c.q.size();
     if (ntot == 0) continue;
     // FP64 mode: one launch over the whole list; mixed: FP64 part, then the
-    // weak part through the FP32 kernel
-    const long long n1 = mode == DFT_MIXED ? c.n64 : ntot;
+    // weak part through the FP32 kernel (classes whose FP32 launch does not
+    // pay run as in FP64 mode)
+    const bool f32 = mode == DFT_MIXED && c.use32;
+    const long long n1 = f32 ? c.n64 : ntot;
     int la, lb, lc, ld;
     pc_moments(c.pcb, la, lb);
     pc_moments(c.pck, lc, ld);
     const double w = 16.0 + ncart_h(la) * ncart_h(lb) * ncart_h(lc) * ncart_h(ld);  // per primitive quartet
     if (!lane_kernel_combo((int)ci)) {
       // heavy combination: integral launches and digest in order on one stream
-      ls.push_back({(int)ci, false, 0, ntot, w * (c.prim64 + (mode == DFT_MIXED ? c.prim32 : c.prim64w))});
+      ls.push_back({(int)ci, false, 0, ntot, w * (c.prim64 + (f32 ? c.prim32 : c.prim64w))});
       continue;
     }
-    if (n1 > 0) ls.push_back({(int)ci, false, 0, n1, w * (mode == DFT_MIXED ? c.prim64 : c.prim64 + c.prim64w)});
-    if (mode == DFT_MIXED && ntot > c.n64) ls.push_back({(int)ci, true, c.n64, ntot - c.n64, 0.7 * w * c.prim32});
+    if (n1 > 0) ls.push_back({(int)ci, false, 0, n1, w * (f32 ? c.prim64 : c.prim64 + c.prim64w)});
+    if (f32 && ntot > c.n64) ls.push_back({(int)ci, true, c.n64, ntot - c.n64, 0.7 * w * c.prim32});
   }
   std::stable_sort(ls.begin(), ls.end(), [](const Launch& a, const Launch& b) { return a.cost > b.cost; });
   // slots without a launch read as zero
@@ -1011,12 +1016,13 @@ int jk_enqueue(dft_plan p, const double* P, const double* Pa, const double* Pb, 
     load[k] += L.cost;
     const Combo& c = p->combos[L.ci];
     if (!lane_kernel_combo(L.ci)) {
-      const long long n1 = mode == DFT_MIXED ? c.n64 : (long long)c.q.size();
+      const bool f32 = mode == DFT_MIXED && c.use32;
+      const long long n1 = f32 ? c.n64 : (long long)c.q.size();
       mark(2 * L.ci, 0, st[k]);
       hipError_t e = launch_combo(p, L.ci, false, 0, n1, c.shape64, P, Pa, Pb, st[k]);
       mark(2 * L.ci, 1, st[k]);
       mark(2 * L.ci + 1, 0, st[k]);
-      if (e == hipSuccess && mode == DFT_MIXED)
+      if (e == hipSuccess && f32)
         e = launch_combo(p, L.ci, true, c.n64, (long long)c.q.size() - c.n64, c.shape32, P, Pa, Pb, st[k]);
       if (e == hipSuccess) e = launch_digest(p, L.ci, P, Pa, Pb, st[k]);
       mark(2 * L.ci + 1, 1, st[k]);
@@ -1394,6 +1400,7 @@ int dft_plan_tune(dft_plan p, const double* P, const double* Pa, const double* P
     if (combo == -2 || (combo >= 0 && (int)ci != combo)) continue;
     Combo& c = p->combos[ci];
     const long long ntot = (long long)c.q.size();
+    float t_all = 0.f, t_32 = 0.f;
     for (int part = 0; part < 2 && rc == DFT_OK; ++part) {
       // part 0: whole list in FP64 (used by the FP64 mode and, for its FP64
       // part, by the mixed mode); part 1: weak part in FP32
@@ -1432,9 +1439,35 @@ int dft_plan_tune(dft_plan p, const double* P, const double* Pa, const double* P
       if (part == 0) {
         c.shape64 = best_sh;
         c.shape32 = best_sh;  // default for the FP32 launch until tuned below
+        t_all = best;
       } else {
         c.shape32 = best_sh;
+        t_32 = best;
       }
+    }
+    // mixed mode: FP64 part + FP32 weak part against one FP64 launch over
+    // the whole list (the FP32 kernel runs FP32 FMAs at the FP64 rate, so
+    // for some classes the split does not pay)
+    if (rc == DFT_OK && ntot > c.n64) {
+      float t_64 = 0.f;
+      if (c.n64 > 0) {
+        std::vector<float> t;
+        for (int r = 0; r <= reps && rc == DFT_OK; ++r) {
+          (void)hipEventRecord(e0, s);
+          hipError_t e = launch_combo(p, ci, false, 0, c.n64, c.shape64, P, Pa, Pb, s);
+          (void)hipEventRecord(e1, s);
+          if (e == hipSuccess) e = hipEventSynchronize(e1);
+          float ms = 0.f;
+          if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+          if (e != hipSuccess) rc = fail(DFT_EHIP, std::string("tune: ") + hipGetErrorString(e));
+          if (r > 0) t.push_back(ms);
+        }
+        if (rc == DFT_OK) {
+          std::sort(t.begin(), t.end());
+          t_64 = t[t.size() / 2];
+        }
+      }
+      c.use32 = t_64 + t_32 < 0.97f * t_all;
     }
   }
   // whole builds with the class launches serial and concurrent (FP64 mode,
@@ -1467,7 +1500,7 @@ int dft_plan_tune(dft_plan p, const double* P, const double* Pa, const double* P
   if (report)
     for (size_t ci = 0; ci < p->combos.size() && (int)(6 * ci + 5) < report_cap; ++ci) {
       const Combo& c = p->combos[ci];
-      const int v[6] = {c.shape64.wave, c.shape64.block, c.shape64.cap, c.shape32.wave, c.shape32.block, c.shape32.cap};
+      const int v[6] = {0, c.shape64.block, c.shape64.cap, c.use32 ? 0 : 1, c.shape32.block, c.shape32.cap};
       for (int k = 0; k < 6; ++k) report[6 * ci + k] = v[k];
     }
   return DFT_OK;
@@ -1484,8 +1517,11 @@ int dft_plan_set_shapes(dft_plan p, const int* shapes, int n) {
     s32.wave = shapes[6 * ci + 3];
     s32.block = shapes[6 * ci + 4];
     s32.cap = shapes[6 * ci + 5];
+    if (s64.wave != 0) return fail(DFT_EINVAL, "the first int of the FP64 shape is reserved and must be 0");
+    if (s32.wave != 0 && s32.wave != 1) return fail(DFT_EINVAL, "the first int of the FP32 shape must be 0 or 1");
+    const bool use32 = s32.wave == 0;
+    s32.wave = 0;
     for (const Shape* s : {&s64, &s32}) {
-      if (s->wave != 0) return fail(DFT_EINVAL, "the first int of a shape is reserved and must be 0");
       if (s->block != 64 && s->block != 128 && s->block != 256 && s->block != 512)
         return fail(DFT_EINVAL, "block must be 64, 128, 256 or 512");
       if (s->block > block_max((int)ci)) return fail(DFT_EINVAL, "block exceeds the class's largest workgroup");
@@ -1493,6 +1529,7 @@ int dft_plan_set_shapes(dft_plan p, const int* shapes, int n) {
     }
     c.shape64 = s64;
     c.shape32 = s32;
+    c.use32 = use32;
   }
   return DFT_OK;
 }

@@ -123,9 +123,12 @@ int dft_graph_destroy(dft_graph graph);
  * the class's kernel (lane-per-quartet kernel, or the column kernel of the
  * heavy d classes) at workgroup sizes 64/128/256/512 (up to the class's
  * limit) and grid caps 512..8192 (median of reps runs each) and keeps the
- * fastest. report (capacity report_cap ints) receives 6 ints per
- * combination: FP64 {0, block, cap}, FP32 {0, block, cap} (the first int of
- * each triple is reserved). combo >= 0 tunes only that combination; -1 tunes
+ * fastest; then it times the FP64 part alone and keeps the FP32 launch for
+ * DFT_MIXED only if FP64 part + FP32 part beat one FP64 launch over the whole
+ * list by 3% (else that class runs all-FP64 in DFT_MIXED too). report
+ * (capacity report_cap ints) receives 6 ints per combination: FP64
+ * {0, block, cap}, FP32 {f, block, cap} with f = 0 (FP32 used) or 1 (FP64
+ * instead). combo >= 0 tunes only that combination; -1 tunes
  * all of them and then chooses between running the class launches of a
  * build serially or concurrently (timing whole FP64 builds); -2 makes only
  * that last choice. Overwrites the plan's staging (J/K are not written).
@@ -134,7 +137,8 @@ int dft_graph_destroy(dft_graph graph);
 int dft_plan_tune(dft_plan plan, const double* P, const double* Pa, const double* Pb, int reps, int combo,
                   int* report, int report_cap, void* stream);
 
-/* Set launch shapes (6 ints per combination, as reported by dft_plan_tune). */
+/* Set launch shapes and the FP32 choice (6 ints per combination, as
+ * reported by dft_plan_tune). */
 int dft_plan_set_shapes(dft_plan plan, const int* shapes, int n);
 
 /*
