@@ -555,9 +555,11 @@ int build_plan(dft_plan p, const Input& in) {
       const int nab = ncart_h(la) * ncart_h(lb), rs = rec_size(la, lb), ci = combo_index(pc, pc);
       const int k0 = p->pair_prim_begin[p->cls_first[pc]];
       const int k1 = p->cls_first[pc] + n < npair ? p->pair_prim_begin[p->cls_first[pc] + n] : (int)pp.size();
-      // in pieces of at most 32 Ki primitive pairs (each a full diagonal block)
-      for (int c0 = k0; c0 < k1 && rc == DFT_OK; c0 += 32768) {
-        const int nk = std::min(k1 - c0, 32768);
+      // in pieces of at most 32 Ki primitive pairs and 2^24 block elements
+      // (each primitive pair's whole diagonal block comes back)
+      const int piece = std::max(256, std::min(32768, (1 << 24) / (nab * nab)));
+      for (int c0 = k0; c0 < k1 && rc == DFT_OK; c0 += piece) {
+        const int nk = std::min(k1 - c0, piece);
         std::vector<int2> q(nk);
         for (int k = 0; k < nk; ++k) q[k] = make_int2(c0 + k, c0 + k);
         std::vector<double> blk;
