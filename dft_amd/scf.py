@@ -175,8 +175,24 @@ def scf_plain(plan: Plan, Z, centers, N_a, N_b, mode="fp64", e_tol=1e-8, p_tol=1
         count += 1
 
 
+# Keys of the reference's SCF checkpoint (shelled.py:1220-1237, read back at
+# :1136-1152).
+CHECKPOINT_KEYS = ("count", "switch_to_DIIS", "DIIS_P_a_history", "DIIS_P_b_history", "DIIS_R_a_history",
+                   "DIIS_R_b_history", "DIIS_Fock_a_history", "DIIS_Fock_b_history", "rate", "strikes", "P_a",
+                   "P_b", "delta_E", "delta_P", "prev_delta_E", "prev_delta_P")
+
+
+def _history_array(items, history, n):
+    """Newest-first history list as the reference's zero-padded
+    (history, n, n) array."""
+    out = np.zeros((history, n, n))
+    for k, v in enumerate(items[:history]):
+        out[k] = v.cpu().numpy()
+    return out
+
+
 def scf_diis(plan: Plan, Z, centers, N_a, N_b, mode="fp64", rate=0.4, diis_threshold=1e-3, history=10,
-             e_tol=1e-8, p_tol=1e-6, max_count=500, A=None, guess=None):
+             e_tol=1e-8, p_tol=1e-6, max_count=500, A=None, guess=None, checkpoint=None, resume=None):
     """Damped + DIIS UHF loop of shelled.py:1103-1292.
 
     Damping rate 0.4, divided by 1.1 every fourth strike (an iteration whose
@@ -187,8 +203,20 @@ def scf_diis(plan: Plan, Z, centers, N_a, N_b, mode="fp64", rate=0.4, diis_thres
     ``guess`` = (P_a, P_b) starts from those densities instead of the core
     Hamiltonian (e.g. the converged densities of a nearby geometry). For RHF
     (N_a == N_b, P_a == P_b throughout) one diagonalisation serves both spins.
+    ``checkpoint``: path written every 10th iteration in the reference's
+    ``scf_checkpoint.npz`` layout (shelled.py:1219-1237); ``resume``: such a
+    file to continue from (counter, DIIS state, damping and densities, as
+    shelled.py:1136-1162; the energy of the previous iteration is not in it,
+    so the first dE after a resume is infinite there too).
     """
     S, H, X, E_NN = _setup(plan, Z, centers, A)
+    n = S.shape[0]
+    if resume is not None:
+        with np.load(resume) as ck:
+            state = {k: ck[k] for k in CHECKPOINT_KEYS}
+            if "E_total" in ck.files:
+                state["E_total"] = ck["E_total"]
+        guess = (state["P_a"], state["P_b"])
     if guess is not None:
         P_a, P_b = (torch.as_tensor(g, dtype=torch.float64, device=plan.device).clone() for g in guess)
     else:
@@ -205,6 +233,30 @@ def scf_diis(plan: Plan, Z, centers, N_a, N_b, mode="fp64", rate=0.4, diis_thres
     strikes = 0
     use_diis = False
     count = 0
+    if resume is not None:
+        # the file holds iteration `count` before its damping / DIIS-switch
+        # update (as written by shelled.py:1219-1237): apply that update and
+        # go on with the next iteration
+        count = int(state["count"])
+        use_diis = bool(state["switch_to_DIIS"])
+        rate = float(state["rate"])
+        strikes = int(state["strikes"])
+        dE, dP = float(state["delta_E"]), float(state["delta_P"])
+        if (dE > float(state["prev_delta_E"]) or dP > float(state["prev_delta_P"])) and not use_diis:
+            if strikes == 3:
+                rate /= 1.1
+                strikes = 0
+            else:
+                strikes += 1
+        if dE < diis_threshold:
+            use_diis = True
+        prev_dE, prev_dP = dE, dP
+        E_total = float(state["E_total"]) if "E_total" in state else None
+        keep = min(count + 1, history)
+        count += 1
+        for k, key in (("Pa", "DIIS_P_a_history"), ("Pb", "DIIS_P_b_history"), ("Fa", "DIIS_Fock_a_history"),
+                       ("Fb", "DIIS_Fock_b_history"), ("Ra", "DIIS_R_a_history"), ("Rb", "DIIS_R_b_history")):
+            hist[k] = [torch.as_tensor(m, dtype=torch.float64, device=plan.device) for m in state[key][:keep]]
     energies, iter_times = [], []
     while True:
         t0 = time.perf_counter()
@@ -233,6 +285,19 @@ def scf_diis(plan: Plan, Z, centers, N_a, N_b, mode="fp64", rate=0.4, diis_thres
         dP = float(torch.maximum((Pa_new - P_a).abs().max(), (Pb_new - P_b).abs().max()))
         energies.append(E_new)
         iter_times.append(time.perf_counter() - t0)  # wall, synchronised by the float() reads
+        if checkpoint is not None and count != 0 and count % 10 == 0:
+            np.savez(checkpoint, count=count, switch_to_DIIS=use_diis,
+                     DIIS_P_a_history=_history_array(hist["Pa"], history, n),
+                     DIIS_P_b_history=_history_array(hist["Pb"], history, n),
+                     DIIS_R_a_history=_history_array(hist["Ra"], history, n),
+                     DIIS_R_b_history=_history_array(hist["Rb"], history, n),
+                     DIIS_Fock_a_history=_history_array(hist["Fa"], history, n),
+                     DIIS_Fock_b_history=_history_array(hist["Fb"], history, n),
+                     rate=rate, strikes=strikes, P_a=Pa_new.cpu().numpy(), P_b=Pb_new.cpu().numpy(),
+                     delta_E=dE, delta_P=dP, prev_delta_E=prev_dE, prev_delta_P=prev_dP,
+                     # not in the reference's file: the energy of this
+                     # iteration, so that a resume continues exactly
+                     E_total=E_new)
         if (dE > prev_dE or dP > prev_dP) and not use_diis:
             if strikes == 3:
                 rate /= 1.1

@@ -271,3 +271,24 @@ def test_everything_screened_gives_zero_jk(torch_cuda):
     for mode in ("fp64", "mixed"):
         J, K, _ = plan.jk(np.ones((n, n)), np.ones((n, n)) * 0.5, mode=mode)
         assert float(J.abs().max()) == 0.0 and float(K.abs().max()) == 0.0
+
+
+def test_scf_checkpoint_resume_continues_exactly(torch_cuda, tmp_path):
+    """scf_diis writes the reference's scf_checkpoint.npz layout every 10th
+    iteration; resuming from it reproduces the uninterrupted run."""
+    from dft_amd.geometry import water_cluster
+    from dft_amd.scf import CHECKPOINT_KEYS, scf_diis
+    Z, xyz = water_cluster((2, 2, 1), 3.1, seed=5)
+    _, plan = _plan(Z, xyz, tau=1e-12)
+    nocc = sum(Z) // 2
+    ck = str(tmp_path / "scf_checkpoint.npz")
+    full = scf_diis(plan, Z, xyz, nocc, nocc, checkpoint=ck)
+    assert full.converged and full.iterations >= 10
+    with np.load(ck) as f:
+        assert set(CHECKPOINT_KEYS) <= set(f.files)
+        assert f["DIIS_P_a_history"].shape == (10, plan.nbf, plan.nbf)
+        saved = int(f["count"])
+    res = scf_diis(plan, Z, xyz, nocc, nocc, resume=ck)
+    assert res.converged
+    assert res.iterations == full.iterations and saved <= full.iterations
+    assert abs(res.energy - full.energy) < 1e-10
