@@ -1648,6 +1648,16 @@ int dft_oneelec(dft_plan p, double* S, double* T, double* V, void* stream) {
   return DFT_OK;
 }
 
+int dft_ao_grid(dft_plan p, const float* grid_xyz, int64_t npts, double* ao, int64_t ld, void* stream) {
+  if (!p || !grid_xyz || !ao) return fail(DFT_EINVAL, "NULL argument");
+  if (p->nmol != 1) return fail(DFT_EUNSUPPORTED, "grid values take a single-molecule plan");
+  if (npts < 0 || ld < npts) return fail(DFT_EINVAL, "need 0 <= npts <= ld");
+  hipError_t e = launch_ao_grid(p->nsh, p->d_sh, p->d_exps, p->d_coefs, grid_xyz, (long long)npts, ao, (long long)ld,
+                                (hipStream_t)stream);
+  if (e != hipSuccess) return fail(DFT_EHIP, std::string("ao grid: ") + hipGetErrorString(e));
+  return DFT_OK;
+}
+
 int dft_grad(dft_plan p, const int32_t* shell_atom, const double* P, const double* Pa, const double* Pb,
              const double* W, double* grad, void* stream) {
   if (!p) return fail(DFT_EINVAL, "plan is NULL");

@@ -113,6 +113,10 @@ hipError_t launch_eri_raw(int combo, const int2* quartets, long long nq, const i
 hipError_t launch_reduce(int ntarget, const Target* targets, const long long* tgt_base, const int* tgt_count,
                          const double* stage, double* half, hipStream_t s);
 
+// AO values on a grid: out[ao * ld + g], g < npts (grid: npts x 3 FP32)
+hipError_t launch_ao_grid(int nsh, const ShellDev* shells, const double* exps, const double* coefs,
+                          const float* grid, long long npts, double* out, long long ld, hipStream_t s);
+
 // M = H + H^T per molecule block (packed layout), for one half matrix
 hipError_t launch_symmetrize(int nmol, const long long* mol_base, const int* mol_nbf, int max_nbf,
                              const double* half, double* out, hipStream_t s);

@@ -158,6 +158,21 @@ class Plan:
                                         C.c_void_p(s.cuda_stream)))
         return S, T, V
 
+    def ao_values(self, grid, out=None):
+        """Cartesian AO values at grid points: (nbf, npts) FP64 on the device
+        (shelled_eval.py:98-113); ``grid`` (npts, 3) bohr, sent as FP32 like
+        the reference's grid."""
+        g = torch.as_tensor(grid, device=self.device).to(torch.float32).reshape(-1, 3).contiguous()
+        npts = g.shape[0]
+        if out is None:
+            out = torch.empty((self.nbf, npts), dtype=torch.float64, device=self.device)
+        if out.shape != (self.nbf, npts) or out.dtype != torch.float64 or not out.is_contiguous():
+            raise ValueError("out must be a contiguous float64 (nbf, npts) tensor")
+        s = torch.cuda.current_stream(self.device)
+        _lib.check(self.lib.dft_ao_grid(self.handle, g.data_ptr(), npts, out.data_ptr(), npts,
+                                        C.c_void_p(s.cuda_stream)))
+        return out
+
     # ----------------------------------------------------- raw ERIs
     def eri_segment_blocks(self, quartets) -> np.ndarray:
         """Concatenated blocks for segment quartets (int array [nq, 4])."""

@@ -243,6 +243,17 @@ int dft_eri_quartets(dft_plan plan, const int32_t* quartets, int64_t nq, double*
 int dft_oneelec(dft_plan plan, double* S, double* T, double* V, void* stream);
 
 /*
+ * Values of every Cartesian basis function at npts points:
+ * ao[mu * ld + g] = phi_mu(r_g) (device, nbf rows of stride ld >= npts),
+ * grid_xyz device FP32 [npts][3] in bohr. Single-molecule plans.
+ *
+ * Replaces the AO evaluation of the density post-processing
+ * (shelled_eval.py:98-113; flat variant main.py:905-912); the MO and density
+ * contractions that follow there are plain GEMMs on these values.
+ */
+int dft_ao_grid(dft_plan plan, const float* grid_xyz, int64_t npts, double* ao, int64_t ld, void* stream);
+
+/*
  * Primitive one-electron pass in the buffer layout of the reference's
  * nuclear_kernel: one entry q = a * nfb + b per primitive Cartesian pair.
  * Outputs V[q] = -(2 pi/p) sum_C Z_C sum_tuv E_t E_u E_v R_tuv(p, P - C),

@@ -417,6 +417,26 @@ def one_electron(shells, Z, centers):
     return S, T, V
 
 
+def ao_values(shells, grid) -> np.ndarray:
+    """Cartesian AO values (n_ao, npts) at grid points, shelled_eval.py:98-113:
+    normalised primitives N x^i y^j z^k exp(-a r^2) contracted with the
+    coefficient rows (FP64 arithmetic on the given points)."""
+    grid = np.asarray(grid, dtype=np.float64).reshape(-1, 3)
+    n_ao = shells[-1]["ao_stop"]
+    out = np.zeros((n_ao, grid.shape[0]))
+    for sh in shells:
+        pw = np.asarray(sh["components"])
+        norm = np.asarray(sh["normalization"]).reshape(sh["n_components"], sh["n_exponents"])
+        d = grid - np.asarray(sh["atom_pos"])
+        r2 = np.sum(d * d, axis=-1)
+        radial = np.exp(-np.asarray(sh["exponents"])[:, None] * r2[None, :])
+        mono = d[None, :, 0] ** pw[:, None, 0] * d[None, :, 1] ** pw[:, None, 1] * d[None, :, 2] ** pw[:, None, 2]
+        vals = norm[:, :, None] * mono[:, None, :] * radial[None, :, :]
+        vals = np.einsum("kp,apg->kag", np.asarray(sh["coefficients"]), vals)
+        out[sh["ao_start"]:sh["ao_stop"]] = vals.reshape(sh["n_ao"], -1)
+    return out
+
+
 def nuclear_repulsion(Z, centers) -> float:
     """sum_{i<j} Z_i Z_j / |R_i - R_j| (shelled.py:157-163)."""
     centers = np.asarray(centers, float)

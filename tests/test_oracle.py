@@ -235,3 +235,36 @@ def test_c_oracle_matches_numpy_oracle_d():
         ref = O.shell_quartet(sh, a, b, c, d, pairs).ravel()
         assert np.allclose(flat[pos:pos + sz], ref, rtol=1e-12, atol=1e-14)
         pos += sz
+
+
+def _grid_overlap(shells, half, h, block):
+    """sum_g phi_m(r_g) phi_n(r_g) h^3 on a cube of side 2*half (spacing h),
+    evaluated in slabs; an exact-in-the-limit quadrature of S_mn for
+    functions much wider than h."""
+    ax = np.arange(-half, half + 1e-9, h)
+    S = np.zeros((len(block), len(block)))
+    for x in ax:
+        g = np.stack(np.meshgrid([x], ax, ax, indexing="ij"), axis=-1).reshape(-1, 3)
+        v = O.ao_values(shells, g)[block]
+        S += v @ v.T
+    return S * h ** 3
+
+
+def test_ao_values_integrate_to_the_overlap_matrix():
+    """The grid AO values (shelled_eval.py:98-113) squared and integrated
+    give the overlap matrix of the independent closed form: O 2p block of
+    STO-3G (a wrong monomial or angular factor fails) and one Cartesian d
+    primitive with every component normalised (shelled.py:88-101):
+    <xx|xx> = <xy|xy> = 1, <xx|yy> = 1/3, <xx|xy> = 0."""
+    from dft_amd.basis import build_shells, build_shells_from_entries, load_basis
+    sh = build_shells(load_basis(), [8], np.zeros((1, 3)))
+    p = [k for k, s in enumerate(sh) if s["angular_momentum"] == 1][0]
+    blk = list(range(sh[p]["ao_start"], sh[p]["ao_stop"]))
+    S, _, _ = O.one_electron(sh, [8], np.zeros((1, 3)))
+    Sg = _grid_overlap(sh, 7.0, 0.1, blk)
+    assert np.abs(Sg - S[np.ix_(blk, blk)]).max() < 1e-6
+    d = build_shells_from_entries([[[[2], [0.8], [[1.0]]]]], np.zeros((1, 3)))
+    Sd = _grid_overlap(d, 6.0, 0.1, list(range(6)))
+    # components xx, xy, yy, xz, yz, zz (combinations order)
+    assert abs(Sd[0, 0] - 1.0) < 1e-6 and abs(Sd[0, 2] - 1.0 / 3.0) < 1e-6
+    assert abs(Sd[1, 1] - 1.0) < 1e-6 and abs(Sd[0, 1]) < 1e-9
