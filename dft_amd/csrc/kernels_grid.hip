@@ -20,8 +20,8 @@ __global__ __launch_bounds__(256) void ao_grid_kernel(int nsh, const ShellDev* _
   const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= npts) return;
   const double gx = grid[3 * g], gy = grid[3 * g + 1], gz = grid[3 * g + 2];
-  // (2l-1)!! for l <= 4
-  const double dfac[5] = {1.0, 1.0, 3.0, 15.0, 105.0};
+  // 1/sqrt((2l-1)!!) for l <= 4
+  const double isd[5] = {1.0, 1.0, 0.57735026918962576451, 0.25819888974716112568, 0.097590007294853320675};
   for (int s = 0; s < nsh; ++s) {
     const ShellDev S = sh[s];
     const double dx = gx - S.x, dy = gy - S.y, dz = gz - S.z;
@@ -29,13 +29,14 @@ __global__ __launch_bounds__(256) void ao_grid_kernel(int nsh, const ShellDev* _
     double rad = 0.0;
     for (int i = 0; i < S.nprim; ++i) {
       const double a = exps[S.prim_off + i];
-      // exp(-a r^2) below 1e-300 adds nothing (and stays finite)
-      if (a * r2 < 690.0) rad += coefs[S.prim_off + i] * exp(-a * r2);
+      // exp(-a r^2) < 1e-16 (a r^2 > 36.84) is left out: most shells are far
+      // from most points of a box grid, and the exp is the kernel's main cost
+      if (a * r2 < 36.84) rad += coefs[S.prim_off + i] * exp(-a * r2);
     }
     const int nc = (S.l + 1) * (S.l + 2) / 2;
     for (int k = 0; k < nc; ++k) {
       const int ix = comp_of(S.l, k, 0), iy = comp_of(S.l, k, 1), iz = comp_of(S.l, k, 2);
-      double m = rad / sqrt(dfac[ix] * dfac[iy] * dfac[iz]);
+      double m = rad * (isd[ix] * isd[iy] * isd[iz]);
       for (int t = 0; t < ix; ++t) m *= dx;
       for (int t = 0; t < iy; ++t) m *= dy;
       for (int t = 0; t < iz; ++t) m *= dz;

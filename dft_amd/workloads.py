@@ -1,6 +1,6 @@
 """The benchmark workloads: water (plain RHF), (H2O)16 direct J/K, the
-C12H26 s/p/d mixed-precision build, the batch of 64 water clusters and the
-C8H18 analytic gradient. Each function runs one workload on the GPU and
+C12H26 s/p/d mixed-precision build, the batch of 64 water clusters, the
+C8H18 analytic gradient and the (H2O)16 grid density. Each function runs one workload on the GPU and
 returns a dict of measured numbers (device times from HIP events).
 """
 from __future__ import annotations
@@ -200,7 +200,48 @@ def alkane_gradient(fd_coords=12, sd_steps=10, h=1e-4):
             "sd_scf_iterations": [hh[3] for hh in hist]}
 
 
-WORKLOADS = ("water_rhf", "alkane_spd_mixed", "water_batch_scf", "alkane_gradient")
+def water16_grid(spacing=0.05, chunk=1 << 22, steps=5):
+    """(H2O)16 STO-3G (112 bf): AO values on the box grid around the cluster
+    (3 bohr padding, the given spacing), in chunks; device time per chunk of
+    the AO kernel against the HBM bytes it writes (8 B per value), and the
+    whole density pass (AO values + rho = sum (P phi) phi) per point."""
+    from .geometry import water_cluster
+    from .grid import box_grid
+    Z, xyz = water_cluster((4, 2, 2), 3.1, seed=0)
+    xyz = np.asarray(xyz)
+    shells = build_shells(load_basis(), Z, xyz)
+    plan = Plan(shells, Z, xyz, schwarz_tau=1e-10)
+    N_a, N_b = electron_counts(Z)
+    res = scf_diis(plan, Z, xyz, N_a, N_b)
+    P = (res.P_a + res.P_b).contiguous()
+    lo, hi = xyz.min(0) - 3.0, xyz.max(0) + 3.0
+    g, shape = box_grid(lo, hi, spacing)
+    dv = float(np.prod((hi - lo) / (np.asarray(shape) - 1)))
+    gd = torch.as_tensor(g[:chunk], device=plan.device)
+    out = torch.empty((plan.nbf, gd.shape[0]), dtype=torch.float64, device=plan.device)
+    ms_ao = time_builds(lambda: plan.ao_values(gd, out=out), steps, 1)
+    gbps = plan.nbf * gd.shape[0] * 8 / (ms_ao * 1e-3) / 1e9
+    # whole grid: electron count and time of the density pass
+    torch.cuda.synchronize()
+    e0, e1 = _events()
+    e0.record()
+    n = torch.zeros((), dtype=torch.float64, device=plan.device)
+    for a in range(0, len(g), chunk):
+        ao = plan.ao_values(g[a:a + chunk])
+        n += ((P @ ao) * ao).sum()
+    e1.record()
+    torch.cuda.synchronize()
+    ms_all = e0.elapsed_time(e1)
+    log(f"grid {shape}: AO kernel {ms_ao:.3f} ms per {gd.shape[0]} points ({gbps:.0f} GB/s written), "
+        f"electrons {float(n) * dv:.6f}")
+    return {"grid_shape": list(shape), "n_points": int(len(g)), "nbf": plan.nbf, "chunk_points": int(gd.shape[0]),
+            "ms_ao_kernel_per_chunk": ms_ao, "ao_kernel_write_GBps": gbps, "hbm_peak_GBps": 8000.0,
+            "ao_kernel_hbm_frac": gbps / 8000.0, "ms_density_pass_whole_grid": ms_all,
+            "ns_per_point_density_pass": ms_all * 1e6 / len(g), "electrons_integrated": float(n) * dv,
+            "electrons": int(N_a + N_b)}
+
+
+WORKLOADS = ("water_rhf", "alkane_spd_mixed", "water_batch_scf", "alkane_gradient", "water16_grid")
 
 
 def main(argv=None):
