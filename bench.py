@@ -293,9 +293,13 @@ def main():
     torch.cuda.synchronize()
     ms_mixed = e0.elapsed_time(e1) / args.steps
     # converged SCF energies in both modes (damped + DIIS loop; the plain
-    # Roothaan loop oscillates on this cluster)
-    scf64 = scf_diis(plan, Z, xyz, nocc, nocc, mode="fp64")
-    scf32 = scf_diis(plan, Z, xyz, nocc, nocc, mode="mixed")
+    # Roothaan loop oscillates on this cluster), on an untuned plan of the
+    # same molecule so that every FP32-eligible quartet runs in FP32 (the
+    # tuned plan may run some classes all-FP64 in mixed mode)
+    acc = Plan(shells, Z, xyz, schwarz_tau=tau, fp32_tau=1e-4)
+    scf64 = scf_diis(acc, Z, xyz, nocc, nocc, mode="fp64")
+    scf32 = scf_diis(acc, Z, xyz, nocc, nocc, mode="mixed")
+    del acc
     f64_part, f32_part = build_flops(per)
     log(f"mixed: {ms_mixed:.3f} ms/build; E64 {scf64.energy:.10f} E_mixed {scf32.energy:.10f}")
 
