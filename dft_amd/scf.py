@@ -264,6 +264,8 @@ def scf_diis(plan: Plan, Z, centers, N_a, N_b, mode="fp64", rate=0.4, diis_thres
         J, Ka, Kb = fock(P, P_a, P_b)
         Fa = H + J - Ka
         Fb = H + J - Kb
+        if not bool(torch.isfinite(Fa).all() & torch.isfinite(Fb).all()):
+            raise FloatingPointError(f"non-finite Fock matrix at iteration {count}")
         for k, v in (("Pa", P_a), ("Pb", P_b), ("Fa", Fa), ("Fb", Fb), ("Ra", Fa @ P_a @ S - S @ P_a @ Fa),
                      ("Rb", Fb @ P_b @ S - S @ P_b @ Fb)):
             hist[k].insert(0, v)
