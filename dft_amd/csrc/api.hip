@@ -1494,6 +1494,37 @@ int dft_plan_tune(dft_plan p, const double* P, const double* Pa, const double* P
       for (double* x : tmp) (void)hipFree(x);
     }
     p->concurrent = tbest[1] < tbest[0];
+    // mixed mode: the per-class FP32 choices above come from launches timed
+    // alone; keep them only if the whole mixed build is faster with them
+    // than with FP32 in every class
+    std::vector<char> chosen;
+    bool any64 = false;
+    for (const Combo& c : p->combos) {
+      chosen.push_back(c.use32);
+      any64 = any64 || (!c.use32 && c.q.size() > (size_t)c.n64);
+    }
+    if (any64 && rc == DFT_OK) {
+      std::vector<double*> tmp(3, nullptr);
+      for (int m = 0; m < 1 + p->nk && rc == DFT_OK; ++m)
+        if (hipMalloc(&tmp[m], std::max<long long>(p->nelem, 1) * sizeof(double)) != hipSuccess)
+          rc = fail(DFT_EHIP, "tune: hipMalloc");
+      float tm[2] = {1e30f, 1e30f};
+      for (int all32 = 0; all32 < 2 && rc == DFT_OK; ++all32) {
+        for (size_t ci = 0; ci < p->combos.size(); ++ci) p->combos[ci].use32 = all32 ? true : (bool)chosen[ci];
+        for (int r = 0; r <= reps && rc == DFT_OK; ++r) {
+          (void)hipEventRecord(e0, s);
+          rc = jk_enqueue(p, P, Pa, Pb, tmp[0], tmp[1], tmp[2], DFT_MIXED, s, false);
+          (void)hipEventRecord(e1, s);
+          float ms = 0.f;
+          if (rc == DFT_OK && hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess &&
+              r > 0)
+            tm[all32] = std::min(tm[all32], ms);
+        }
+      }
+      for (double* x : tmp) (void)hipFree(x);
+      const bool keep = tm[0] <= tm[1];
+      for (size_t ci = 0; ci < p->combos.size(); ++ci) p->combos[ci].use32 = keep ? (bool)chosen[ci] : true;
+    }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

@@ -85,12 +85,21 @@ def alkane_spd_mixed(steps=3, warmup=1):
     J, K = torch.empty_like(P2), torch.empty_like(P2)
     ms64 = time_builds(lambda: plan.jk_into(P2, Pc, None, J, K, None, "fp64"), steps, warmup)
     ms32 = time_builds(lambda: plan.jk_into(P2, Pc, None, J, K, None, "mixed"), steps, warmup)
+    # the same builds after the autotuner (launch shapes, FP32 choice per class)
+    t0 = time.perf_counter()
+    tune(plan, P2, Pc, None, reps=2)
+    t_tune = time.perf_counter() - t0
+    ms64t = time_builds(lambda: plan.jk_into(P2, Pc, None, J, K, None, "fp64"), steps, warmup)
+    ms32t = time_builds(lambda: plan.jk_into(P2, Pc, None, J, K, None, "mixed"), steps, warmup)
+    log(f"builds untuned fp64 {ms64:.2f} / mixed {ms32:.2f} ms; tuned ({t_tune:.1f} s) {ms64t:.2f} / {ms32t:.2f} ms")
     return {"nbf_cart": plan.nbf, "nbf_sph": n_sph, "n_shells": len(shells), "min_eig_S": smin,
             "n_quartets": st["n_quartets"], "n_quartets_fp32": st["n_quartets_fp32"],
             "n_unscreened": st["n_unscreened"], "n_prim_quartets": st["n_prim_quartets"], "plan_s": t_plan,
             "energy_fp64_Eh": r64.energy, "energy_mixed_Eh": r32.energy, "converged": [r64.converged, r32.converged],
             "abs_dE_Eh": abs(r64.energy - r32.energy), "max_abs_dF_Eh": dF,
-            "ms_per_build_fp64": ms64, "ms_per_build_mixed": ms32, "mixed_speedup": ms64 / ms32}
+            "ms_per_build_fp64": ms64, "ms_per_build_mixed": ms32, "mixed_speedup": ms64 / ms32,
+            "tune_s": t_tune, "ms_per_build_fp64_tuned": ms64t, "ms_per_build_mixed_tuned": ms32t,
+            "mixed_speedup_tuned": ms64t / ms32t}
 
 
 def water_batch_scf(steps=5, warmup=2, tune_reps=2, scf_check=True):

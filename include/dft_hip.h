@@ -125,7 +125,9 @@ int dft_graph_destroy(dft_graph graph);
  * limit) and grid caps 512..8192 (median of reps runs each) and keeps the
  * fastest; then it times the FP64 part alone and keeps the FP32 launch for
  * DFT_MIXED only if FP64 part + FP32 part beat one FP64 launch over the whole
- * list by 3% (else that class runs all-FP64 in DFT_MIXED too). report
+ * list by 3% (else that class runs all-FP64 in DFT_MIXED too); whole-plan
+ * tuning (combo -1) keeps those choices only if whole mixed builds with them
+ * beat whole mixed builds with FP32 in every class. report
  * (capacity report_cap ints) receives 6 ints per combination: FP64
  * {0, block, cap}, FP32 {f, block, cap} with f = 0 (FP32 used) or 1 (FP64
  * instead). combo >= 0 tunes only that combination; -1 tunes
