@@ -180,6 +180,9 @@ __global__ __launch_bounds__(256) void eri_chunk_kernel(const int2* __restrict__
 // blocks as the lane kernel:
 //   Jab = 2w sum_cd I P_cd        Jcd = 2w sum_ab I P_ab
 //   Kac = w sum_bd I P_bd  Kad = w sum_bc I P_bc  Kbc = w sum_ad I P_ad  Kbd = w sum_ac I P_ac
+constexpr int DIGEST_WAVES = 4;       // quartets (waves) per workgroup
+constexpr int DIGEST_NI_MAX = 1296;   // integrals of a (dd|dd) quartet
+
 template <int NK>
 __global__ __launch_bounds__(256) void jk_digest_kernel(int na, int nb, int nc, int nd, int splb, long long ss,
                                                         const int2* __restrict__ quartets,
@@ -194,18 +197,26 @@ __global__ __launch_bounds__(256) void jk_digest_kernel(int na, int nb, int nc, 
   const int nab = na * nb, ncd = nc * nd, ni = nab * ncd;
   const int nK = na * nc + na * nd + nb * nc + nb * nd;
   const int nout = nab + ncd + NK * nK;
+  // the wave's quartet integrals, summed over the bra primitive chunks in a
+  // fixed order, staged in LDS with coalesced loads; the output loops below
+  // then gather from LDS instead of strided global reads
+  __shared__ double sI[DIGEST_WAVES][DIGEST_NI_MAX];
+  double* Iw = sI[threadIdx.x >> 6];
   for (int q = blockIdx.x * wpb + (threadIdx.x >> 6); q < n; q += gridDim.x * wpb) {
     const int2 qd = quartets[q];
     const int4 pb = pairs[qd.x];
     const int4 pk = pairs[qd.y];
     const double* Iq = ibuf + (long long)q * ni;
-    // integrals summed over the bra primitive chunks (fixed order)
-    auto I = [&](int a, int b, int c, int d) {
-      const int x = ((a * nb + b) * nc + c) * nd + d;
+    __builtin_amdgcn_wave_barrier();  // the previous quartet's reads are done
+    for (int x = lane; x < ni; x += 64) {
       double v = Iq[x];
       for (int sp = 1; sp < splb; ++sp) v += Iq[sp * ss + x];
-      return v;
-    };
+      Iw[x] = v;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    auto I = [&](int a, int b, int c, int d) { return Iw[((a * nb + b) * nc + c) * nd + d]; };
     double w = 1.0;
     if (pb.x == pb.y) w *= 0.5;
     if (pk.x == pk.y) w *= 0.5;
@@ -339,7 +350,8 @@ hipError_t launch_jk_digest(int combo, const int2* q, const long long* sl, int n
     default:
       return hipErrorInvalidValue;
   }
-  const int wpb = 4;  // quartets (waves) per workgroup
+  const int wpb = DIGEST_WAVES;  // quartets (waves) per workgroup (the LDS staging is sized for it)
+  if (ncart(la) * ncart(lb) * ncart(lc) * ncart(ld) > DIGEST_NI_MAX) return hipErrorInvalidValue;
   dim3 grid((unsigned)std::min<long long>(((long long)n + wpb - 1) / wpb, 65536)), blk(64 * wpb);
   if (nk == 1)
     hipLaunchKernelGGL((jk_digest_kernel<1>), grid, blk, 0, s, ncart(la), ncart(lb), ncart(lc), ncart(ld), splb,
