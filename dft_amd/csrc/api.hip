@@ -629,9 +629,13 @@ int build_plan(dft_plan p, const Input& in) {
       int* doff = nullptr;
       double* dq = nullptr;
       dpp = nullptr;
-      if ((rc = upload(&dpp, pp))) return rc;
-      if ((rc = upload(&doff, pp_off))) return rc;
-      if ((rc = upload(&dq, qprim))) return rc;
+      if (!(rc = upload(&dpp, pp)) && !(rc = upload(&doff, pp_off))) rc = upload(&dq, qprim);
+      if (rc) {
+        (void)hipFree(dpp);
+        (void)hipFree(doff);
+        (void)hipFree(dq);
+        return rc;
+      }
       hipError_t e = launch_pair_records(dpp, (int)pp.size(), doff, p->d_sh, p->d_exps, p->d_coefs, dq,
                                          p->d_rec64, nullptr, nullptr);
       if (e == hipSuccess) e = hipDeviceSynchronize();
@@ -684,9 +688,13 @@ int build_plan(dft_plan p, const Input& in) {
       int* doff = nullptr;
       double* dq = nullptr;
       dpp = nullptr;
-      if ((rc = upload(&dpp, pp32))) return rc;
-      if ((rc = upload(&doff, pp_off32))) return rc;
-      if ((rc = upload(&dq, q32))) return rc;
+      if (!(rc = upload(&dpp, pp32)) && !(rc = upload(&doff, pp_off32))) rc = upload(&dq, q32);
+      if (rc) {
+        (void)hipFree(dpp);
+        (void)hipFree(doff);
+        (void)hipFree(dq);
+        return rc;
+      }
       hipError_t e = hipMalloc(&p->d_rec32[lev], std::max<long long>(roff32, 1) * sizeof(float));
       if (e == hipSuccess)
         e = launch_pair_records(dpp, (int)pp32.size(), doff, p->d_sh, p->d_exps, p->d_coefs, dq, nullptr,
