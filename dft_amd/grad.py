@@ -7,7 +7,8 @@ backward pass calls the derivative-integral kernels of libdft_hip.so
 derivatives with P and the energy-weighted density W) and adds the nuclear
 repulsion gradient.
 
-The reference has no gradients; this is the differentiation north-star item.
+The reference has no gradients; this module adds them (checked against
+finite differences of the SCF energy).
 """
 from __future__ import annotations
 

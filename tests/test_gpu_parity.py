@@ -1,6 +1,6 @@
 """GPU parity: the HIP path against the golden fixtures and the oracle.
 
-Bars (north-star): ERIs within 1e-10 absolute, total energy within 1e-8 Eh,
+Bars: ERIs within 1e-10 absolute, total energy within 1e-8 Eh,
 J/K as the dense contraction of the oracle ERI tensor.
 """
 import numpy as np
