@@ -84,14 +84,9 @@ __device__ __forceinline__ void quartet_column(const real* __restrict__ bb, int 
         ey[t] = t > exy ? real(0) : (eoy < 0 ? real(1) : E[S::NEK + eoy + t]) * sg;
         ez[t] = t > exz ? real(0) : (eoz < 0 ? real(1) : E[2 * S::NEK + eoz + t]) * sg;
       }
-      real eyz[nherm(LCD) > 0 ? (LCD + 1) * (LCD + 2) / 2 : 1];
-      static_for<0, LCD + 1>([&](auto nI) {
-        constexpr int nu = decltype(nI)::value;
-        static_for<0, LCD + 1 - nu>([&](auto pI) {
-          constexpr int ph = decltype(pI)::value;
-          eyz[nu * (LCD + 1) - nu * (nu - 1) / 2 + ph] = ey[nu] * ez[ph];
-        });
-      });
+      // G[h] += sum_tau ex[tau] sum_nu ey[nu] sum_phi ez[phi] R[h + (tau,nu,phi)],
+      // nested so that no ey*ez table is held (registers are what limits
+      // these lanes)
       static_for<0, S::NHB>([&](auto hI) {
         constexpr int h = decltype(hI)::value;
         constexpr int t = h_t(h), u = h_u(h), v = h_v(h);
@@ -101,10 +96,12 @@ __device__ __forceinline__ void quartet_column(const real* __restrict__ bb, int 
           real inner = real(0);
           static_for<0, LCD + 1 - ta>([&](auto nI) {
             constexpr int nu = decltype(nI)::value;
+            real in2 = real(0);
             static_for<0, LCD + 1 - ta - nu>([&](auto pI) {
               constexpr int ph = decltype(pI)::value;
-              inner = fma(eyz[nu * (LCD + 1) - nu * (nu - 1) / 2 + ph], R[hidx(t + ta, u + nu, v + ph)], inner);
+              in2 = fma(ez[ph], R[hidx(t + ta, u + nu, v + ph)], in2);
             });
+            inner = fma(ey[nu], in2, inner);
           });
           g = fma(ex[ta], inner, g);
         });
