@@ -855,16 +855,20 @@ int build_plan(dft_plan p, const Input& in) {
           cur += (long long)tcount[t] * nr * nc;
         }
   p->stage_len = cur;
+  // slot tables: block-major for the lane kernel (a wave's loads of one
+  // block's slots are contiguous), quartet-major for the heavy-class digest
+  // (one wave per quartet)
   for (size_t ci = 0; ci < p->combos.size(); ++ci) {
     Combo& c = p->combos[ci];
     const size_t nq = c.q.size(), ns = (size_t)slot_rows(ci);
+    const bool lane = lane_kernel_combo((int)ci);
     std::vector<long long> slots(nq * ns * nblk);
     for (size_t ch = 0; ch < ns; ++ch)
       for (size_t i = 0; i < nq; ++i) {
         quartet_targets(c.q[i], tq);
         for (int k = 0; k < nblk; ++k) {
           const Target& T = tg[tq[k]];
-          slots[(ch * nq + i) * nblk + k] = tnext[tq[k]];
+          slots[lane ? ((size_t)k * ns + ch) * nq + i : (ch * nq + i) * nblk + k] = tnext[tq[k]];
           tnext[tq[k]] += (long long)T.nrow * T.ncol;
         }
       }

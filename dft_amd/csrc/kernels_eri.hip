@@ -16,7 +16,8 @@ constexpr int jk_block_max(int la, int lb, int lc, int ld) {
 // Each lane evaluates one canonical quartet (bra pair >= ket pair in the
 // class-sorted pair order) and writes its contribution blocks
 //   Jab, Jcd, then per exchange density Kac, Kad, Kbc, Kbd
-// to stage + slots[q * NBLK + k]. With w = 2^-([a=b]+[c=d]+[ab=cd]):
+// to stage + slots[(k * nchunk + ch) * ntot + q] (block-major, so that each
+// slot load of a wave is one contiguous run). With w = 2^-([a=b]+[c=d]+[ab=cd]):
 //   Jab = 2w sum_cd I P_cd        Jcd = 2w sum_ab I P_ab
 //   Kac = w sum_bd I P_bd  Kad = w sum_bc I P_bc  Kbc = w sum_ad I P_ad  Kbd = w sum_ac I P_ac
 // and the half matrices Jh, Kh summed over all blocks give J = Jh + Jh^T and
@@ -80,7 +81,8 @@ __global__ __launch_bounds__(jk_block_max(LA, LB, LC, LD)) void eri_jk_kernel(
     const int nbf = sa.y;
     const long long mb = (long long)(unsigned)sa.z | ((long long)sa.w << 32);
     const double* DJm = DJ + mb;
-    const long long* sl = slots + ((long long)ch * ntot + q) * NBLK;
+    const long long sstr = (long long)ntot * (splb * splk);
+    const long long* sl = slots + (long long)ch * ntot + q;
     // Coulomb
     {
       double Pcd[S::NCD], Pab[S::NAB];
@@ -107,9 +109,12 @@ __global__ __launch_bounds__(jk_block_max(LA, LB, LC, LD)) void eri_jk_kernel(
         stage[sl[0] + ab] = 2.0 * w * s;
       }
 #pragma unroll
-      for (int cd = 0; cd < S::NCD; ++cd) stage[sl[1] + cd] = 2.0 * w * jcd[cd];
+      for (int cd = 0; cd < S::NCD; ++cd) stage[sl[sstr] + cd] = 2.0 * w * jcd[cd];
     }
-    // Exchange, one density at a time
+    // Exchange, one density at a time. The densities are symmetric, and
+    // P_bd is read as P_db (likewise P_bc, P_ad, P_ac): the lanes of a wave
+    // mostly share the ket shells c, d, so their reads fall in the rows of
+    // c and d instead of one row per lane.
 #pragma unroll
     for (int k = 0; k < NK; ++k) {
       const double* D = (k == 0 ? DK0 : DK1) + mb;
@@ -117,16 +122,16 @@ __global__ __launch_bounds__(jk_block_max(LA, LB, LC, LD)) void eri_jk_kernel(
 #pragma unroll
       for (int b = 0; b < NB; ++b) {
 #pragma unroll
-        for (int d = 0; d < ND; ++d) Pbd[b * ND + d] = D[(long long)(b0 + b) * nbf + d0 + d];
+        for (int d = 0; d < ND; ++d) Pbd[b * ND + d] = D[(long long)(d0 + d) * nbf + b0 + b];
 #pragma unroll
-        for (int c = 0; c < NC; ++c) Pbc[b * NC + c] = D[(long long)(b0 + b) * nbf + c0 + c];
+        for (int c = 0; c < NC; ++c) Pbc[b * NC + c] = D[(long long)(c0 + c) * nbf + b0 + b];
       }
 #pragma unroll
       for (int a = 0; a < NA; ++a) {
 #pragma unroll
-        for (int d = 0; d < ND; ++d) Pad[a * ND + d] = D[(long long)(a0 + a) * nbf + d0 + d];
+        for (int d = 0; d < ND; ++d) Pad[a * ND + d] = D[(long long)(d0 + d) * nbf + a0 + a];
 #pragma unroll
-        for (int c = 0; c < NC; ++c) Pac[a * NC + c] = D[(long long)(a0 + a) * nbf + c0 + c];
+        for (int c = 0; c < NC; ++c) Pac[a * NC + c] = D[(long long)(c0 + c) * nbf + a0 + a];
       }
       double kac[NA * NC], kad[NA * ND], kbc[NB * NC], kbd[NB * ND];
 #pragma unroll
@@ -151,15 +156,15 @@ __global__ __launch_bounds__(jk_block_max(LA, LB, LC, LD)) void eri_jk_kernel(
               kbc[b * NC + c] = fma(v, Pad[a * ND + d], kbc[b * NC + c]);
               kbd[b * ND + d] = fma(v, Pac[a * NC + c], kbd[b * ND + d]);
             }
-      const long long* ks = sl + 2 + 4 * k;
+      const long long* ks = sl + (2 + 4 * k) * sstr;
 #pragma unroll
       for (int x = 0; x < NA * NC; ++x) stage[ks[0] + x] = w * kac[x];
 #pragma unroll
-      for (int x = 0; x < NA * ND; ++x) stage[ks[1] + x] = w * kad[x];
+      for (int x = 0; x < NA * ND; ++x) stage[ks[sstr] + x] = w * kad[x];
 #pragma unroll
-      for (int x = 0; x < NB * NC; ++x) stage[ks[2] + x] = w * kbc[x];
+      for (int x = 0; x < NB * NC; ++x) stage[ks[2 * sstr] + x] = w * kbc[x];
 #pragma unroll
-      for (int x = 0; x < NB * ND; ++x) stage[ks[3] + x] = w * kbd[x];
+      for (int x = 0; x < NB * ND; ++x) stage[ks[3 * sstr] + x] = w * kbd[x];
     }
   }
 }

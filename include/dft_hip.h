@@ -188,7 +188,9 @@ int dft_schwarz(dft_plan plan, double* Q);
  * (calc_eri/process_eri, shelled.py:596-768), and the dense J/K of
  * main.py:787-789.
  *
- *  P, Pa, Pb   device nbf*nbf; Pb and Kb must be NULL for a non-UHF plan
+ *  P, Pa, Pb   device nbf*nbf, symmetric (the eight-fold quartet symmetry
+ *              is applied, so each density is read in one orientation only);
+ *              Pb and Kb must be NULL for a non-UHF plan
  *  mode        DFT_FP64 or DFT_MIXED
  */
 int dft_jk(dft_plan plan, const double* P, const double* Pa, const double* Pb, double* J, double* Ka,
