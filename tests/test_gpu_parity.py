@@ -122,6 +122,24 @@ def test_scf_energy_dimer(torch_cuda):
     assert abs(res.energy - float(g["E"])) < 1e-8, res.energy - float(g["E"])
 
 
+def test_size_consistency_far_apart_waters(torch_cuda):
+    """E(A + B) = E(A) + E(B) for two waters 1000 bohr apart: their
+    interaction (dipole-dipole, < 1e-9 Eh there) is below the tolerance, so
+    the intermolecular nuclear repulsion, nuclear attraction and Coulomb
+    terms (each ~0.1 Eh) must cancel to 1e-8."""
+    from dft_amd.geometry import water
+    from dft_amd.scf import scf_plain
+    Z, xyz = water()
+    _, p1 = _plan(Z, xyz, tau=1e-12)
+    r1 = scf_plain(p1, Z, xyz, 5, 5)
+    Z2 = list(Z) + list(Z)
+    xyz2 = np.vstack([xyz, xyz + np.array([1000.0, 0.0, 0.0])])
+    _, p2 = _plan(Z2, xyz2, tau=1e-12)
+    r2 = scf_plain(p2, Z2, xyz2, 10, 10)
+    assert r1.converged and r2.converged
+    assert abs(r2.energy - 2 * r1.energy) < 1e-8, r2.energy - 2 * r1.energy
+
+
 def test_schwarz_bound_holds(torch_cuda):
     g = golden("dimer_sto3g.npz")
     ERI = g["ERI"]
