@@ -179,6 +179,7 @@ __global__ __launch_bounds__(256) void eri_chunk_kernel(const int2* __restrict__
 //   Kac = w sum_bd I P_bd  Kad = w sum_bc I P_bc  Kbc = w sum_ad I P_ad  Kbd = w sum_ac I P_ac
 constexpr int DIGEST_WAVES = 4;       // quartets (waves) per workgroup
 constexpr int DIGEST_NI_MAX = 1296;   // integrals of a (dd|dd) quartet
+constexpr int DIGEST_BLK_MAX = 36;    // elements of a (d, d) density block
 
 template <int NK>
 __global__ __launch_bounds__(256) void jk_digest_kernel(int na, int nb, int nc, int nd, int splb, long long ss,
@@ -195,25 +196,19 @@ __global__ __launch_bounds__(256) void jk_digest_kernel(int na, int nb, int nc, 
   const int nK = na * nc + na * nd + nb * nc + nb * nd;
   const int nout = nab + ncd + NK * nK;
   // the wave's quartet integrals, summed over the bra primitive chunks in a
-  // fixed order, staged in LDS with coalesced loads; the output loops below
-  // then gather from LDS instead of strided global reads
+  // fixed order, and the six density blocks the quartet contracts with
+  // (P_cd, P_ab, then per exchange density P_bd, P_bc, P_ad, P_ac) are
+  // staged in LDS with coalesced loads; the output loops below then read LDS
+  // only (the density reads were one global load per multiply-add)
   __shared__ double sI[DIGEST_WAVES][DIGEST_NI_MAX];
-  double* Iw = sI[threadIdx.x >> 6];
-  for (int q = blockIdx.x * wpb + (threadIdx.x >> 6); q < n; q += gridDim.x * wpb) {
+  __shared__ double sP[DIGEST_WAVES][2 + 4 * NK][DIGEST_BLK_MAX];
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  double* Iw = sI[wv];
+  double* Pw = &sP[wv][0][0];
+  for (int q = blockIdx.x * wpb + wv; q < n; q += gridDim.x * wpb) {
     const int2 qd = quartets[q];
     const int4 pb = pairs[qd.x];
     const int4 pk = pairs[qd.y];
-    const double* Iq = ibuf + (long long)q * ni;
-    __builtin_amdgcn_wave_barrier();  // the previous quartet's reads are done
-    for (int x = lane; x < ni; x += 64) {
-      double v = Iq[x];
-      for (int sp = 1; sp < splb; ++sp) v += Iq[sp * ss + x];
-      Iw[x] = v;
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    auto I = [&](int a, int b, int c, int d) { return Iw[((a * nb + b) * nc + c) * nd + d]; };
     double w = 1.0;
     if (pb.x == pb.y) w *= 0.5;
     if (pk.x == pk.y) w *= 0.5;
@@ -222,49 +217,81 @@ __global__ __launch_bounds__(256) void jk_digest_kernel(int na, int nb, int nc, 
     const long long a0 = sa.x, b0 = shell_dm[pb.y].x, c0 = shell_dm[pk.x].x, d0 = shell_dm[pk.y].x;
     const long long nbf = sa.y;
     const long long mb = (long long)(unsigned)sa.z | ((long long)sa.w << 32);
-    const double* DJm = DJ + mb;
+    const double* Iq = ibuf + (long long)q * ni;
+    __builtin_amdgcn_wave_barrier();  // the previous quartet's reads are done
+    for (int x = lane; x < ni; x += 64) {
+      double v = Iq[x];
+      for (int sp = 1; sp < splb; ++sp) v += Iq[sp * ss + x];
+      Iw[x] = v;
+    }
+    // block t: rows r0.. (nr of them) and columns k0.. (nc_ of them) of D
+    auto stage_block = [&](int t, const double* D, long long r0, int nr, long long k0, int nc_) {
+      for (int x = lane; x < nr * nc_; x += 64) {
+        const int r = x / nc_, k = x - r * nc_;
+        Pw[t * DIGEST_BLK_MAX + x] = D[(r0 + r) * nbf + k0 + k];
+      }
+    };
+    stage_block(0, DJ + mb, c0, nc, d0, nd);
+    stage_block(1, DJ + mb, a0, na, b0, nb);
+    for (int kk = 0; kk < NK; ++kk) {
+      const double* D = (kk == 0 ? DK0 : DK1) + mb;
+      stage_block(2 + 4 * kk, D, b0, nb, d0, nd);
+      stage_block(3 + 4 * kk, D, b0, nb, c0, nc);
+      stage_block(4 + 4 * kk, D, a0, na, d0, nd);
+      stage_block(5 + 4 * kk, D, a0, na, c0, nc);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // Every output element is a sum over two indices, s = sum_i sum_j
+    // I[ib + i * i1 + j * i2] * P[pb + i * p1 + j * p2] (in the order the
+    // block formulas above list them), so one loop body serves all blocks
+    // and a wave's lanes do not diverge over the block kinds.
     const long long* sl = slots + (long long)q * (2 + 4 * NK);
     for (int o = lane; o < nout; o += 64) {
-      double s = 0.0;
+      int ib, i1, i2, n1, n2, pb_, p1, p2, blk, x;
+      double scale;
       if (o < nab) {
         const int a = o / nb, b = o - a * nb;
-        for (int c = 0; c < nc; ++c)
-          for (int d = 0; d < nd; ++d) s = fma(I(a, b, c, d), DJm[(c0 + c) * nbf + d0 + d], s);
-        stage[sl[0] + o] = 2.0 * w * s;
+        ib = (a * nb + b) * ncd, i1 = nd, i2 = 1, n1 = nc, n2 = nd;
+        pb_ = 0, p1 = nd, p2 = 1, blk = 0, x = o, scale = 2.0 * w;
       } else if (o < nab + ncd) {
-        const int x = o - nab, c = x / nd, d = x - c * nd;
-        for (int a = 0; a < na; ++a)
-          for (int b = 0; b < nb; ++b) s = fma(I(a, b, c, d), DJm[(a0 + a) * nbf + b0 + b], s);
-        stage[sl[1] + x] = 2.0 * w * s;
+        x = o - nab;
+        const int c = x / nd, d = x - c * nd;
+        ib = c * nd + d, i1 = nb * ncd, i2 = ncd, n1 = na, n2 = nb;
+        pb_ = DIGEST_BLK_MAX, p1 = nb, p2 = 1, blk = 1, scale = 2.0 * w;
       } else {
-        int x = o - nab - ncd;
+        x = o - nab - ncd;
         const int kk = x / nK;
         x -= kk * nK;
-        const double* D = (kk == 0 ? DK0 : DK1) + mb;
-        const long long* ks = sl + 2 + 4 * kk;
+        const int t0 = 2 + 4 * kk;
+        scale = w;
         if (x < na * nc) {
-          const int a = x / nc, c = x - a * nc;
-          for (int b = 0; b < nb; ++b)
-            for (int d = 0; d < nd; ++d) s = fma(I(a, b, c, d), D[(b0 + b) * nbf + d0 + d], s);
-          stage[ks[0] + x] = w * s;
+          const int a = x / nc, c = x - a * nc;  // K_ac: sum over b, d of I P_bd
+          ib = a * nb * ncd + c * nd, i1 = ncd, i2 = 1, n1 = nb, n2 = nd;
+          pb_ = t0 * DIGEST_BLK_MAX, p1 = nd, p2 = 1, blk = t0;
         } else if ((x -= na * nc) < na * nd) {
-          const int a = x / nd, d = x - a * nd;
-          for (int b = 0; b < nb; ++b)
-            for (int c = 0; c < nc; ++c) s = fma(I(a, b, c, d), D[(b0 + b) * nbf + c0 + c], s);
-          stage[ks[1] + x] = w * s;
+          const int a = x / nd, d = x - a * nd;  // K_ad: sum over b, c of I P_bc
+          ib = a * nb * ncd + d, i1 = ncd, i2 = nd, n1 = nb, n2 = nc;
+          pb_ = (t0 + 1) * DIGEST_BLK_MAX, p1 = nc, p2 = 1, blk = t0 + 1;
         } else if ((x -= na * nd) < nb * nc) {
-          const int b = x / nc, c = x - b * nc;
-          for (int a = 0; a < na; ++a)
-            for (int d = 0; d < nd; ++d) s = fma(I(a, b, c, d), D[(a0 + a) * nbf + d0 + d], s);
-          stage[ks[2] + x] = w * s;
+          const int b = x / nc, c = x - b * nc;  // K_bc: sum over a, d of I P_ad
+          ib = b * ncd + c * nd, i1 = nb * ncd, i2 = 1, n1 = na, n2 = nd;
+          pb_ = (t0 + 2) * DIGEST_BLK_MAX, p1 = nd, p2 = 1, blk = t0 + 2;
         } else {
           x -= nb * nc;
-          const int b = x / nd, d = x - b * nd;
-          for (int a = 0; a < na; ++a)
-            for (int c = 0; c < nc; ++c) s = fma(I(a, b, c, d), D[(a0 + a) * nbf + c0 + c], s);
-          stage[ks[3] + x] = w * s;
+          const int b = x / nd, d = x - b * nd;  // K_bd: sum over a, c of I P_ac
+          ib = b * ncd + d, i1 = nb * ncd, i2 = nd, n1 = na, n2 = nc;
+          pb_ = (t0 + 3) * DIGEST_BLK_MAX, p1 = nc, p2 = 1, blk = t0 + 3;
         }
       }
+      double s = 0.0;
+      for (int i = 0; i < n1; ++i) {
+        const double* Ir = Iw + ib + i * i1;
+        const double* Pr = Pw + pb_ + i * p1;
+        for (int j = 0; j < n2; ++j) s = fma(Ir[j * i2], Pr[j * p2], s);
+      }
+      stage[sl[blk] + x] = scale * s;
     }
   }
 }
@@ -349,6 +376,8 @@ hipError_t launch_jk_digest(int combo, const int2* q, const long long* sl, int n
   }
   const int wpb = DIGEST_WAVES;  // quartets (waves) per workgroup (the LDS staging is sized for it)
   if (ncart(la) * ncart(lb) * ncart(lc) * ncart(ld) > DIGEST_NI_MAX) return hipErrorInvalidValue;
+  if (ncart(2) * ncart(2) > DIGEST_BLK_MAX || std::max(std::max(la, lb), std::max(lc, ld)) > 2)
+    return hipErrorInvalidValue;  // density blocks of at most (d, d)
   dim3 grid((unsigned)std::min<long long>(((long long)n + wpb - 1) / wpb, 65536)), blk(64 * wpb);
   if (nk == 1)
     hipLaunchKernelGGL((jk_digest_kernel<1>), grid, blk, 0, s, ncart(la), ncart(lb), ncart(lc), ncart(ld), splb,
