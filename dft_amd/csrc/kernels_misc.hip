@@ -170,7 +170,9 @@ __global__ void reduce_kernel(int ntarget, const Target* __restrict__ tg, const 
 hipError_t launch_reduce(int ntarget, const Target* tg, const long long* tbase, const int* tcount,
                          const double* stage, double* half, hipStream_t s) {
   if (ntarget <= 0) return hipSuccess;
-  const int waves_per_block = 4;
+  // one-wave workgroups: 38 vs 43 us (four waves per group) for the 168 MB
+  // staging of a (H2O)16 STO-3G build
+  const int waves_per_block = 1;
   hipLaunchKernelGGL(reduce_kernel, dim3((ntarget + waves_per_block - 1) / waves_per_block),
                      dim3(64 * waves_per_block), 0, s, ntarget, tg, tbase, tcount, stage, half);
   return hipGetLastError();
