@@ -224,6 +224,11 @@ struct dft_plan_s {
   long long* d_gbase = nullptr;
   int* d_gcount = nullptr;
   double* d_gstage = nullptr;
+  // per-atom sums: chunks (atom, first slot, slots) and their partial sums
+  int4* d_gchunks = nullptr;
+  int* d_gcfirst = nullptr;
+  double* d_gpart = nullptr;
+  int n_gchunk = 0;
   bool tuned = false;
 };
 
@@ -240,6 +245,7 @@ void free_plan(dft_plan p) {
                   p->d_rec64, p->d_rec32[0], p->d_stage,   p->d_tg,       p->d_tbase,     p->d_tcount,
                   p->d_half,  p->d_atoms, p->d_shpairs, p->d_mol_base, p->d_mol_nbf,   p->d_grec,
                   p->d_gpairs, p->d_g1slots, p->d_gbase, p->d_gcount, p->d_gstage, p->d_recT64, p->d_g1tasks,
+                  p->d_gchunks, p->d_gcfirst, p->d_gpart,
                   p->d_pairs32[0], p->d_pairs32[1], p->d_rec32[1], p->d_recT32[0], p->d_recT32[1]};
   for (void* x : ptrs) (void)hipFree(x);
   for (hipEvent_t e : p->prof_ev) (void)hipEventDestroy(e);
@@ -1196,6 +1202,19 @@ int grad_prepare(dft_plan p, const int32_t* shell_atom) {
   if ((rc = upload(&p->d_gbase, base))) return rc;
   if ((rc = upload(&p->d_gcount, cnt))) return rc;
   HIP_TRY(hipMalloc(&p->d_gstage, std::max<long long>(cur, 1) * sizeof(double)));
+  {
+    std::vector<int4> chunks;
+    std::vector<int> cfirst(natom + 1, 0);
+    for (int a = 0; a < natom; ++a) {
+      cfirst[a] = (int)chunks.size();
+      for (int k = 0; k < cnt[a]; k += GRAD_CHUNK) chunks.push_back(make_int4(a, k, std::min(GRAD_CHUNK, cnt[a] - k), 0));
+    }
+    cfirst[natom] = (int)chunks.size();
+    p->n_gchunk = (int)chunks.size();
+    if ((rc = upload(&p->d_gchunks, chunks))) return rc;
+    if ((rc = upload(&p->d_gcfirst, cfirst))) return rc;
+    HIP_TRY(hipMalloc(&p->d_gpart, std::max<long long>(3LL * p->n_gchunk, 1) * sizeof(double)));
+  }
   p->grad_ready = true;
   return DFT_OK;
 }
@@ -1729,7 +1748,9 @@ int dft_grad(dft_plan p, const int32_t* shell_atom, const double* P, const doubl
   }
   hipError_t e = launch_grad1e(p->d_g1tasks, p->ng1tasks, p->d_shpairs, p->d_sh, p->d_exps, p->d_coefs, p->natom,
                                p->d_atoms, P, W, nbf, p->d_g1slots, p->d_gstage, p->d_boys, s);
-  if (e == hipSuccess) e = launch_grad_reduce(p->natom, p->d_gbase, p->d_gcount, p->d_gstage, grad, s);
+  if (e == hipSuccess)
+    e = launch_grad_reduce(p->natom, p->n_gchunk, p->d_gchunks, p->d_gcfirst, p->d_gbase, p->d_gstage, p->d_gpart,
+                           grad, s);
   if (e != hipSuccess) return fail(DFT_EHIP, std::string("grad 1e/reduce: ") + hipGetErrorString(e));
   return DFT_OK;
 }

@@ -140,8 +140,10 @@ hipError_t launch_grad1e(const int4* tasks /* pair, first nucleus, nuclei, compo
                          const int2* shell_pairs, const ShellDev* shells, const double* exps, const double* coefs,
                          int natom, const double* atoms, const double* P, const double* W, int nbf,
                          const long long* slots, double* stage, const double* boys_tab, hipStream_t s);
-hipError_t launch_grad_reduce(int natom, const long long* base, const int* count, const double* stage, double* grad,
-                              hipStream_t s);
+// GRAD_CHUNK: slots per stage-1 chunk of the per-atom gradient sums
+constexpr int GRAD_CHUNK = 1024;
+hipError_t launch_grad_reduce(int natom, int nchunk, const int4* chunks, const int* cfirst, const long long* base,
+                              const double* stage, double* part, double* grad, hipStream_t s);
 
 // reference-layout primitive kernels (nuclear_kernel / eri_kernel buffers)
 hipError_t launch_nuclear_prim(int nfa, int nfb, const double* ea, const double* eb, const double* ca, int sa,

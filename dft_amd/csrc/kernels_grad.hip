@@ -451,25 +451,58 @@ hipError_t launch_grad1e(const int4* tasks, int ntask, const int2* sp, const She
   return hipGetLastError();
 }
 
-// per-atom fixed-order sum of 3-vectors: wave per atom
-__global__ void grad_reduce_kernel(int natom, const long long* __restrict__ base, const int* __restrict__ count,
-                                   const double* __restrict__ stage, double* __restrict__ grad) {
+// Per-atom sums of the staged 3-vectors in two fixed-order stages. Stage 1:
+// one wave per chunk of at most GRAD_CHUNK slots of one atom; lanes 0..62
+// read elements lane + 63 i (component lane % 3) into four partial sums,
+// then component x sums lanes x, x + 3, ..., x + 60 in order.
+__global__ void grad_chunk_kernel(int nchunk, const int4* __restrict__ chunks, const long long* __restrict__ base,
+                                  const double* __restrict__ stage, double* __restrict__ part) {
+  const int c = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (c >= nchunk) return;
+  const int4 ck = chunks[c];
+  const double* src = stage + base[ck.x] + 3LL * ck.y;
+  const int ne = 3 * ck.z;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  if (lane < 63) {
+    int i = lane;
+    for (; i + 3 * 63 < ne; i += 4 * 63) {
+      a0 += src[i];
+      a1 += src[i + 63];
+      a2 += src[i + 2 * 63];
+      a3 += src[i + 3 * 63];
+    }
+    for (; i < ne; i += 63) a0 += src[i];
+  }
+  const double v = (a0 + a1) + (a2 + a3);
+  const int x = lane < 3 ? lane : 0;
+  double r = 0.0;
+  for (int k = 0; k < 21; ++k) r += __shfl(v, 3 * k + x, 64);
+  if (lane < 3) part[3LL * c + lane] = r;
+}
+
+// Stage 2: one wave per atom over its chunks' partial sums.
+__global__ void grad_reduce_kernel(int natom, const int* __restrict__ cfirst, const double* __restrict__ part,
+                                   double* __restrict__ grad) {
   const int atom = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   if (atom >= natom) return;
-  const double* src = stage + base[atom];
+  const int c0 = cfirst[atom], c1 = cfirst[atom + 1];
   for (int x = 0; x < 3; ++x) {
     double s = 0.0;
-    for (int k = lane; k < count[atom]; k += 64) s += src[3 * (long long)k + x];
+    for (int c = c0 + lane; c < c1; c += 64) s += part[3LL * c + x];
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
     if (lane == 0) grad[3 * atom + x] = s;
   }
 }
 
-hipError_t launch_grad_reduce(int natom, const long long* base, const int* count, const double* stage, double* grad,
-                              hipStream_t s) {
+hipError_t launch_grad_reduce(int natom, int nchunk, const int4* chunks, const int* cfirst, const long long* base,
+                              const double* stage, double* part, double* grad, hipStream_t s) {
   if (natom <= 0) return hipSuccess;
-  hipLaunchKernelGGL(grad_reduce_kernel, dim3((natom + 3) / 4), dim3(256), 0, s, natom, base, count, stage, grad);
+  if (nchunk > 0)
+    hipLaunchKernelGGL(grad_chunk_kernel, dim3((nchunk + 3) / 4), dim3(256), 0, s, nchunk, chunks, base, stage,
+                       part);
+  hipLaunchKernelGGL(grad_reduce_kernel, dim3((natom + 3) / 4), dim3(256), 0, s, natom, cfirst, part, grad);
   return hipGetLastError();
 }
 
