@@ -1062,7 +1062,7 @@ int jk_enqueue(dft_plan p, const double* P, const double* Pa, const double* Pb, 
       HIP_TRY(hipStreamWaitEvent(s, p->ev_join[k - 1], 0));
     }
   mark(nslot - 2, 0, s);
-  hipError_t e = launch_reduce(p->ntarget, p->d_tg, p->d_tbase, p->d_tcount, p->d_stage, p->d_half, s);
+  hipError_t e = launch_reduce(p->ntarget, p->d_tg, p->d_tbase, p->d_tcount, p->d_stage, p->stage_len, p->d_half, s);
   if (e != hipSuccess) return fail(DFT_EHIP, std::string("reduce: ") + hipGetErrorString(e));
   mark(nslot - 2, 1, s);
   mark(nslot - 1, 0, s);

@@ -111,7 +111,7 @@ hipError_t launch_eri_raw(int combo, const int2* quartets, long long nq, const i
 // deterministic target reduction: one wave per target; the blocks of target
 // t are contiguous: stage[tgt_base[t] + k * nrow * ncol + e], k < tgt_count[t]
 hipError_t launch_reduce(int ntarget, const Target* targets, const long long* tgt_base, const int* tgt_count,
-                         const double* stage, double* half, hipStream_t s);
+                         const double* stage, long long stage_len /* doubles */, double* half, hipStream_t s);
 
 // AO values on a grid: out[ao * ld + g], g < npts (grid: npts x 3 FP32)
 hipError_t launch_ao_grid(int nsh, const ShellDev* shells, const double* exps, const double* coefs,

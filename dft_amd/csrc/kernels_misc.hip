@@ -123,10 +123,20 @@ hipError_t launch_interleave_records(const int4* map, int nrec, const double* r6
 // scatter is not: shelled.py:857-859).
 constexpr int RED_ACC = 16;
 
+//
+// Two register layouts of the same sums. UNIFORM makes the wave index
+// explicitly wave-uniform, so the target's base pointer sits in scalar
+// registers (64 VGPRs, 8 waves per SIMD instead of 122 VGPRs and 4): faster
+// when the staging buffer streams from HBM (64 uneven (H2O)_n clusters,
+// 7.2 GB: batch build 15.71 -> 15.48 ms), slower when it fits in the
+// 256 MB Infinity Cache ((H2O)16, 168 MB: 38 -> 49 us), where the extra
+// waves in flight cost more than they hide.
+template <bool UNIFORM>
 __global__ void reduce_kernel(int ntarget, const Target* __restrict__ tg, const long long* __restrict__ tbase,
                               const int* __restrict__ tcount, const double* __restrict__ stage,
                               double* __restrict__ half) {
-  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int w0 = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int wave = UNIFORM ? __builtin_amdgcn_readfirstlane(w0) : w0;
   const int lane = threadIdx.x & 63;
   if (wave >= ntarget) return;
   const Target t = tg[wave];
@@ -168,13 +178,17 @@ __global__ void reduce_kernel(int ntarget, const Target* __restrict__ tg, const 
 }
 
 hipError_t launch_reduce(int ntarget, const Target* tg, const long long* tbase, const int* tcount,
-                         const double* stage, double* half, hipStream_t s) {
+                         const double* stage, long long stage_len, double* half, hipStream_t s) {
   if (ntarget <= 0) return hipSuccess;
   // one-wave workgroups: 38 vs 43 us (four waves per group) for the 168 MB
   // staging of a (H2O)16 STO-3G build
   const int waves_per_block = 1;
-  hipLaunchKernelGGL(reduce_kernel, dim3((ntarget + waves_per_block - 1) / waves_per_block),
-                     dim3(64 * waves_per_block), 0, s, ntarget, tg, tbase, tcount, stage, half);
+  const dim3 grid((ntarget + waves_per_block - 1) / waves_per_block), block(64 * waves_per_block);
+  // staging larger than the 256 MB Infinity Cache streams from HBM
+  if (stage_len * (long long)sizeof(double) > (256LL << 20))
+    hipLaunchKernelGGL(reduce_kernel<true>, grid, block, 0, s, ntarget, tg, tbase, tcount, stage, half);
+  else
+    hipLaunchKernelGGL(reduce_kernel<false>, grid, block, 0, s, ntarget, tg, tbase, tcount, stage, half);
   return hipGetLastError();
 }
 
