@@ -124,8 +124,37 @@ class Plan:
             raise ValueError(f"density must be {self.nbf}x{self.nbf}, got {tuple(t.shape)}")
         return t
 
+    def _nelem(self) -> int:
+        return self.nbf * self.nbf
+
+    def check_buffers(self, P, Pa, Pb, J, Ka, Kb, mode="fp64"):
+        """Raise ValueError unless the densities and outputs are what the C
+        entry points read and write through raw pointers: contiguous float64
+        tensors on the plan's device with nbf*nbf (packed batch: nelem)
+        elements, Pb/Kb present exactly for a UHF plan. A wrong buffer would
+        otherwise be read or written out of bounds on the device."""
+        if mode not in _lib.MODES:
+            raise ValueError(f"mode must be one of {sorted(_lib.MODES)}, got {mode!r}")
+        if self.uhf != (Pb is not None) or self.uhf != (Kb is not None):
+            raise ValueError("Pb and Kb must be given exactly for a UHF plan")
+        n = self._nelem()
+        for name, t in (("P", P), ("Pa", Pa), ("Pb", Pb), ("J", J), ("Ka", Ka), ("Kb", Kb)):
+            if t is None:
+                continue
+            if not isinstance(t, torch.Tensor):
+                raise ValueError(f"{name} must be a torch tensor")
+            if t.device != self.device:
+                raise ValueError(f"{name} is on {t.device}, the plan on {self.device}")
+            if t.dtype != torch.float64:
+                raise ValueError(f"{name} must be float64, got {t.dtype}")
+            if not t.is_contiguous():
+                raise ValueError(f"{name} must be contiguous")
+            if t.numel() != n:
+                raise ValueError(f"{name} has {t.numel()} elements, the plan needs {n}")
+
     def jk_into(self, P, Pa, Pb, J, Ka, Kb=None, mode="fp64", stream=None):
         """Enqueue J/K on ``stream`` into preallocated device tensors (no sync)."""
+        self.check_buffers(P, Pa, Pb, J, Ka, Kb, mode)
         s = torch.cuda.current_stream(self.device) if stream is None else stream
         _lib.check(self.lib.dft_jk(
             self.handle, P.data_ptr(), Pa.data_ptr(), Pb.data_ptr() if Pb is not None else None,
@@ -279,6 +308,7 @@ class Graph:
 
     def __init__(self, plan, P, Pa, Pb, J, Ka, Kb, mode):
         self.plan = plan
+        plan.check_buffers(P, Pa, Pb, J, Ka, Kb, mode)
         self.keep = (P, Pa, Pb, J, Ka, Kb)  # the graph binds these buffers
         h = C.c_void_p()
         _lib.check(plan.lib.dft_jk_graph_create(plan.handle, P.data_ptr(), Pa.data_ptr(),
@@ -302,6 +332,7 @@ def tune(plan, P, Pa, Pb=None, reps=3, log=None):
     """Autotune the plan's launch shapes on these densities, one class
     combination at a time; returns the choices as an int array [ncombo, 6]
     (FP64 {wave, block, cap}, FP32 {wave, block, cap})."""
+    plan.check_buffers(P, Pa, Pb, P, Pa, Pb)  # densities only (the tuner writes no J/K)
     per = plan.stats()["per_combo"]
     ncombo = len(per)
     rep = (C.c_int * (6 * ncombo))()
@@ -368,6 +399,9 @@ class BatchPlan(Plan):
         self.handle = handle
         self.nbf = None
         self.schwarz_tau, self.fp32_tau = schwarz_tau, fp32_tau
+
+    def _nelem(self) -> int:
+        return self.nelem
 
     def empty(self):
         return torch.empty(self.nelem, dtype=torch.float64, device=self.device)

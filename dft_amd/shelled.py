@@ -220,15 +220,36 @@ def independent(shells, sphere_total_ao, C):
 
 
 # ---------------------------------------------------- ERI store + J/K
-_plans = {}
+# Plans of the drop-in calls, most recently used last. A plan is found again
+# only for shells of identical content (momenta, AO offsets, centres,
+# exponents, coefficients), so a caller that moves atoms in place or reuses a
+# list object gets a new plan; at most _PLAN_CACHE plans (and their device
+# buffers) are kept.
+_PLAN_CACHE = 2
+_plans: list = []
+
+
+def _shell_key(shells, uhf) -> bytes:
+    import hashlib
+    h = hashlib.sha1()
+    for sh in shells:
+        h.update(np.asarray([sh["angular_momentum"], sh["ao_start"], sh["n_contractions"],
+                             sh["n_exponents"]], dtype=np.int64).tobytes())
+        for k in ("atom_pos", "exponents", "coefficients"):
+            h.update(np.ascontiguousarray(np.asarray(sh[k], dtype=np.float64)).tobytes())
+    h.update(b"uhf" if uhf else b"rhf")
+    return h.digest()
 
 
 def _plan_for(shells, uhf):
-    key = (id(shells), len(shells), shells[-1]["ao_stop"], bool(uhf))
-    plan = _plans.get(key)
-    if plan is None:
-        plan = Plan(shells, [], np.zeros((0, 3)), schwarz_tau=0.0, fp32_tau=0.0, uhf=uhf)
-        _plans[key] = plan
+    key = _shell_key(shells, bool(uhf))
+    for k, (kk, plan) in enumerate(_plans):
+        if kk == key:
+            _plans.append(_plans.pop(k))
+            return plan
+    plan = Plan(shells, [], np.zeros((0, 3)), schwarz_tau=0.0, fp32_tau=0.0, uhf=uhf)
+    _plans.append((key, plan))
+    del _plans[:-_PLAN_CACHE]
     return plan
 
 
@@ -300,8 +321,20 @@ def fill_matrices(shells, P, P_a, P_b, J_matrix, K_a, K_b, mode="fp64"):
 
     Computed by the fused direct J/K build (no ERI store is read). When P_a
     and P_b are equal one exchange matrix is built and added to both.
+
+    The build applies the eight-fold quartet symmetry, which reads each
+    density in one orientation; the reference reads all eight
+    (shelled.py:791-800), which differs for a non-symmetric density, so such
+    a density is rejected with ValueError.
     """
     Pt, Pa, Pb = (_dev(x, torch.float64) for x in (P, P_a, P_b))
+    for name, M in (("P", Pt), ("P_a", Pa), ("P_b", Pb)):
+        if M.dim() != 2 or M.shape[0] != M.shape[1]:
+            raise ValueError(f"{name} must be a square matrix, got {tuple(M.shape)}")
+        scale = float(M.abs().max()) if M.numel() else 0.0
+        if float((M - M.T).abs().max()) > 1e-12 * max(scale, 1.0):
+            raise ValueError(f"{name} is not symmetric: the direct build reads each density in one "
+                             "orientation (eight-fold symmetry)")
     rhf = torch.equal(Pa, Pb)
     plan = _plan_for(shells, not rhf)
     J, Ka, Kb = plan.jk(Pt, Pa, None if rhf else Pb, mode=mode)
