@@ -67,7 +67,22 @@ def measured_traffic(kernel):
     return None if k is None else k["fetch_bytes"] + k["write_bytes"]
 
 
-def cpu_baseline(shells, Q, tau, P, seconds, n_quartets_gpu):
+def traffic_source():
+    """Where ``traffic`` comes from: the committed counter summary and the
+    commit of the code it was measured on."""
+    try:
+        with open(TRAFFIC) as f:
+            t = json.load(f)
+        return {"file": "profiles/eri_jk_h2o16_sto3g_traffic.json", "measured_at_commit": t.get("commit"),
+                "method": t.get("method")}
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(shells, Q, tau, P, seconds, n_quartets_gpu, jk_gpu=None):
+    """C restatement timed on the host; when the timed sample is the whole
+    screened list (it is for (H2O)16 at the default time budget), its J/K are
+    also compared with the HIP FP64 J/K ``jk_gpu`` of the same density."""
     import oracle.ref as ref
     seg = ref.Segments(shells)
     quart = screened_quartets(Q, tau)
@@ -80,8 +95,13 @@ def cpu_baseline(shells, Q, tau, P, seconds, n_quartets_gpu):
     t_try = time.perf_counter() - t0
     n = min(len(quart), max(n_try, int(n_try * seconds / max(t_try, 1e-6))))
     t0 = time.perf_counter()
-    ref.build_jk(seg, quart[perm[:n]], P, P / 2, None, nthreads=threads)
+    Jc, Kc, _ = ref.build_jk(seg, quart[perm[:n]], P, P / 2, None, nthreads=threads)
     t = time.perf_counter() - t0
+    check = None
+    if n == len(quart) and jk_gpu is not None:
+        check = {"quartets": int(n), "max_abs_dJ": float(np.abs(jk_gpu[0] - Jc).max()),
+                 "max_abs_dK": float(np.abs(jk_gpu[1] - Kc).max()),
+                 "note": "HIP FP64 J/K vs the C restatement over the identical screened list"}
     nc = (seg.l + 1) * (seg.l + 2) // 2
     s = quart[perm[:n]]
     n_ao = int((nc[s[:, 0]] * nc[s[:, 1]] * nc[s[:, 2]] * nc[s[:, 3]]).sum())
@@ -107,7 +127,8 @@ def cpu_baseline(shells, Q, tau, P, seconds, n_quartets_gpu):
                       f"(oracle/md_ref.c, OpenMP {threads} threads)",
             "shell_quartets_per_s": rate,
             "s_per_fock_build_extrapolated": len(quart) / rate,
-            "n_screened_list": int(len(quart)), "gpu_list_matches": int(len(quart)) == n_quartets_gpu}
+            "n_screened_list": int(len(quart)), "gpu_list_matches": int(len(quart)) == n_quartets_gpu,
+            "jk_parity": check}
 
 
 def main():
@@ -117,6 +138,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the other workloads (C12H26 s/p/d mixed, the 64-molecule batch, the C8H18 "
+                         "gradient) reported beside the headline")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--fp64-only", action="store_true",
                     help="only the FP64 timed region (for profiler runs)")
@@ -243,9 +267,23 @@ def main():
         f_all_eval = sum(build_flops(per, model=False, mode="fp64"))
         from dft_amd.flops import combo_moments
         kname = "eri_jk_kernel<%d, %d, %d, %d, double, 1>" % combo_moments(combo)
-        return {"bound": "mfma", "pipe": "fp64 VALU (MFMA FP64 peak is the same 78.6 TFLOP/s)",
+        # every class launch: model flops over its serial launch time
+        per_class = []
+        for c in range(ncombo):
+            ms = eri_ms[2 * c] / nrec
+            nq_c = int(per[c][0] + per[c][1])
+            if nq_c == 0 or ms <= 0:
+                continue
+            fc = combo_flops(c, nq_c, int(per[c][4] + per[c][5]))
+            per_class.append({"class": "(%d%d|%d%d)" % combo_moments(c), "quartets": nq_c,
+                              "model_flops": fc, "ms": ms, "tflops": fc / (ms * 1e-3) / 1e12,
+                              "frac": fc / (ms * 1e-3) / FP64_PEAK})
+        return {"bound": "mfma", "pipe": "fp64 VALU: compute-bound; no MFMA instruction is issued by this "
+                                         "kernel, MI355X's FP64 VALU and FP64 MFMA peaks are both 78.6 TFLOP/s",
                 "kernel": kname, "achieved": achieved, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
                 "frac": achieved * 1e12 / FP64_PEAK, "traffic": measured_traffic(kname),
+                "traffic_source": traffic_source(),
+                "per_class": per_class,
                 "avg_launch_ms": eri_ms[dom] / nrec, "launch_timing": "serial class launches",
                 "flops_per_launch": f_dom, "flops_per_launch_evaluated": f_dom_eval,
                 "whole_build_tflops": f_all / (ms_per_step * 1e-3) / 1e12,
@@ -321,13 +359,44 @@ def main():
     })
     if not args.no_cpu:
         log("cpu baseline ...")
+        plan.jk_into(P, P_a, None, J, K, None, "fp64")
+        jk_gpu = (J.cpu().numpy(), K.cpu().numpy())
         out["cpu_baseline"] = cpu_baseline(shells, plan.schwarz(), tau, P.cpu().numpy(), args.cpu_seconds,
-                                           quartets)
+                                           quartets, jk_gpu)
         log(f"cpu: {out['cpu_baseline']['value']:.3e} quartets/s on {out['cpu_baseline']['cores']} threads")
+    if not args.no_extras:
+        out.update(extras())
     out["roofline"] = roofline()
     log(f"dominant kernel {out['roofline']['kernel']}: {out['roofline']['achieved']:.2f} TFLOP/s "
         f"({100 * out['roofline']['frac']:.1f}% of FP64 peak)")
     print(json.dumps(out), flush=True)
+
+
+def extras():
+    """The other benchmark workloads, each measured by its own function of
+    dft_amd.workloads (device times from HIP events); a failure is recorded
+    in the line instead of ending the run."""
+    from dft_amd import workloads as W
+    res = {}
+    for key, fn in (("config2_c12h26_spd_mixed", lambda: W.alkane_spd_mixed(steps=5, warmup=2)),
+                    ("config3_batch64", lambda: W.water_batch_scf(steps=5, warmup=2, scf_check=False)),
+                    ("config4_c8h18_gradient", lambda: W.alkane_gradient(fd_coords=4, sd_steps=10))):
+        t0 = time.perf_counter()
+        log(f"extra workload {key} ...")
+        try:
+            r = fn()
+        except Exception as exc:  # recorded, not fatal: the headline line still prints
+            r = {"error": f"{type(exc).__name__}: {exc}"}
+        r["wall_s"] = time.perf_counter() - t0
+        res[key] = r
+        log(f"{key} done in {r['wall_s']:.1f} s")
+    b = res.get("config3_batch64", {})
+    if "ms_per_iter_batched_mixed_graph_tuned" in b:
+        res["north_star_batched_mixed_vs_plain_fp64"] = {
+            "ratio": b["speedup_batched_mixed_vs_plain_fp64"], "target": 3.0,
+            "ms_plain_fp64": b["ms_per_iter_plain_fp64_per_molecule"],
+            "ms_batched_mixed": b["ms_per_iter_batched_mixed_graph_tuned"]}
+    return res
 
 
 if __name__ == "__main__":

@@ -142,7 +142,29 @@ __global__ void reduce_kernel(int ntarget, const Target* __restrict__ tg, const 
   const Target t = tg[wave];
   const int ne = t.nrow * t.ncol;
   const double* src = stage + tbase[wave];
-  const long long tot = (long long)tcount[wave] * ne;
+  const int cnt = tcount[wave];
+  if (ne > 64) {
+    // blocks wider than a wave (f x f and above): lane e sums element e,
+    // e + 64, ... over the slots in slot order
+    for (int e = lane; e < ne; e += 64) {
+      double acc[RED_ACC];
+#pragma unroll
+      for (int k = 0; k < RED_ACC; ++k) acc[k] = 0.0;
+      int k = 0;
+      for (; k + RED_ACC <= cnt; k += RED_ACC) {
+#pragma unroll
+        for (int j = 0; j < RED_ACC; ++j) acc[j] += src[(long long)(k + j) * ne + e];
+      }
+      for (; k < cnt; ++k) acc[0] += src[(long long)k * ne + e];
+#pragma unroll
+      for (int w = 1; w < RED_ACC; w *= 2)
+#pragma unroll
+        for (int j = 0; j + w < RED_ACC; j += 2 * w) acc[j] += acc[j + w];
+      half[t.base + (long long)(e / t.ncol) * t.stride + e % t.ncol] = acc[0];
+    }
+    return;
+  }
+  const long long tot = (long long)cnt * ne;
   const int g = 64 / ne;
   const int span = g * ne;
   // RED_ACC independent partial sums keep that many loads in flight per

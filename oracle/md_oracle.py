@@ -21,7 +21,11 @@ Each function restates one reference function with its branch structure:
 * ``nuclear_repulsion`` shelled.py:157-163
 * ``independent``     shelled.py:878-920
 * ``scf_plain``       main.py:753-828 (plain Roothaan, no DIIS)
-* ``scf_diis``        shelled.py:1079-1292 (damping + DIIS)
+* ``scf_diis``        shelled.py:1079-1292 (damping + DIIS, the loop as written
+                                            there: zero-padded histories, B from
+                                            traces, no Lagrange row)
+* ``process_eri``     shelled.py:702-768   (lexsort by AO shapes, repack, the
+                                            while-loop block boundaries)
 
 Vectorisation is over primitive quartets of one Cartesian component quartet,
 which keeps the arithmetic of each reference thread intact.
@@ -521,3 +525,138 @@ def scf_plain(H, S, jk, E_NN, N_a, N_b, e_tol=1e-8, p_tol=1e-6, max_count=100):
         E_total = E_new
         P_a, P_b = Pa_new, Pb_new
         count += 1
+
+
+def scf_diis(H, S, jk, E_NN, N_a, N_b, rate=0.4, diis_threshold=1e-3, history=10, e_tol=1e-8, p_tol=1e-6,
+             max_count=500):
+    """Damped + DIIS UHF loop of shelled.py:1103-1292 (Cartesian basis, A = I).
+
+    Restated step by step: core-Hamiltonian guess (:1082-1091); each
+    iteration builds J/K from the current densities (:1164-1177), shifts the
+    zero-padded histories of P, F and the residuals R = F P S - S P F
+    (:1179-1194); once switched to DIIS, B_ij = tr(R_a,i^T R_a,j) +
+    tr(R_b,i^T R_b,j) over the whole padded history, cut to
+    min(count + 1, history), solved against ones without a Lagrange row and
+    normalised to sum 1 (:1196-1203); the energy uses the (extrapolated) F
+    (:1206-1208); damped densities (UHF_density, :561-574); a strike when
+    dE or dP grew outside DIIS, the rate divided by 1.1 on every fourth
+    (:1250-1255); DIIS from the first dE < 1e-3 (:1256-1257); on
+    convergence the exit block rebuilds F from the last undamped-loop
+    densities and recomputes the energy (:1261-1285). The reference forms
+    the extrapolated F as a sum over the whole ten-entry history times the
+    lim coefficients (:1202-1203), which broadcasts only for lim = 1 or 10;
+    the entries past lim are still zero there, so the sum over the first lim
+    entries taken here is the same value wherever the reference runs.
+    ``jk(P, P_a, P_b)``
+    returns (J, K_a, K_b). Unlike the reference (no cap) the loop stops after
+    ``max_count`` iterations with converged False. Returns a dict with the
+    final energy, the energy of every iteration and the iteration count.
+    """
+    n = H.shape[0]
+    X = orthogonaliser(S)
+    _, Cp = np.linalg.eigh(X.T @ H @ X)
+    C = X @ Cp
+    P_a, P_b = occupied_density(C, N_a), occupied_density(C, N_b)
+    hist = {k: np.zeros((history, n, n)) for k in ("Pa", "Pb", "Fa", "Fb", "Ra", "Rb")}
+    E_total = None
+    prev_dE = prev_dP = np.inf
+    strikes = 0
+    use_diis = False
+    count = 0
+    energies = []
+    diis_from = None
+    while True:
+        P = P_a + P_b
+        J, Ka, Kb = jk(P, P_a, P_b)
+        Fa = H + J - Ka
+        Fb = H + J - Kb
+        if use_diis and diis_from is None:
+            diis_from = count
+        for k, v in (("Pa", P_a), ("Pb", P_b), ("Fa", Fa), ("Fb", Fb), ("Ra", Fa @ P_a @ S - S @ P_a @ Fa),
+                     ("Rb", Fb @ P_b @ S - S @ P_b @ Fb)):
+            hist[k][1:] = hist[k][:-1].copy()
+            hist[k][0] = v
+        if use_diis:
+            Ra, Rb = hist["Ra"], hist["Rb"]
+            B = (np.trace(Ra.transpose(0, 2, 1)[:, None] @ Ra[None], axis1=2, axis2=3)
+                 + np.trace(Rb.transpose(0, 2, 1)[:, None] @ Rb[None], axis1=2, axis2=3))
+            lim = min(count + 1, history)
+            B = B[:lim, :lim]
+            c = np.linalg.solve(B, np.ones(B.shape[0]))
+            c /= np.sum(c)
+            Fa = np.sum(hist["Fa"][:lim] * c[:, None, None], axis=0)
+            Fb = np.sum(hist["Fb"][:lim] * c[:, None, None], axis=0)
+        E_new = 0.5 * np.sum(P * H + P_a * Fa + P_b * Fb) + E_NN
+        dE = np.inf if E_total is None else abs(E_new - E_total)
+        _, Ca = np.linalg.eigh(X.T @ Fa @ X)
+        _, Cb = np.linalg.eigh(X.T @ Fb @ X)
+        Pa_new = rate * occupied_density(X @ Ca, N_a) + (1 - rate) * P_a
+        Pb_new = rate * occupied_density(X @ Cb, N_b) + (1 - rate) * P_b
+        dP = max(np.abs(Pa_new - P_a).max(), np.abs(Pb_new - P_b).max())
+        energies.append(float(E_new))
+        if (dE > prev_dE or dP > prev_dP) and not use_diis:
+            if strikes == 3:
+                rate /= 1.1
+                strikes = 0
+            else:
+                strikes += 1
+        if dE < diis_threshold:
+            use_diis = True
+        if (dE < e_tol and dP < p_tol) or count >= max_count:
+            J, Ka, Kb = jk(P, P_a, P_b)
+            Fa = H + J - Ka
+            Fb = H + J - Kb
+            E_fin = 0.5 * np.sum(P * H + P_a * Fa + P_b * Fb) + E_NN
+            return {"E": float(E_fin), "energies": energies, "count": count, "diis_from": diis_from,
+                    "P_a": P_a, "P_b": P_b,
+                    "converged": bool(dE < e_tol and dP < p_tol), "F_a": Fa, "F_b": Fb}
+        E_total = E_new
+        prev_dE, prev_dP = dE, dP
+        P_a, P_b = Pa_new, Pb_new
+        count += 1
+
+
+def process_eri(out_dir):
+    """shelled.py:702-768: stable lexsort of the quartets by their AO-shape
+    tuple (last key first), values and [start, stop] indices repacked in that
+    order, quads as int32, then the block boundaries found by walking the
+    sorted shapes (:739-762). Writes the same five files into out_dir."""
+    import os
+    j = lambda name: os.path.join(out_dir, name)  # noqa: E731
+    vals = np.load(j("eri_values.npy"))
+    idx = np.load(j("eri_indices.npy"))
+    quads = np.load(j("eri_quads.npy"))
+    shapes = np.load(j("eri_quads_shapes.npy"))
+    lex = np.lexsort((shapes[:, 3], shapes[:, 2], shapes[:, 1], shapes[:, 0]))
+    sorted_shapes = shapes[lex, :]
+    pv = np.zeros(vals.shape, dtype=np.float64)
+    pi = np.zeros(idx.shape, dtype=np.int32)
+    pq = quads[lex, :].astype(np.int32)
+    write = 0
+    for count, (a, b) in enumerate(idx[lex, :]):
+        write_to = write + b - a
+        pv[write:write_to] = vals[a:b]
+        pi[count] = (write, write_to)
+        write = write_to
+    blocks, chunk_shapes = [], []
+    target = sorted_shapes[0]
+    end = quads.shape[0]
+    count = start = 0
+    while True:
+        if count == end:
+            blocks.append([start, count])
+            chunk_shapes.append(target)
+            break
+        if (sorted_shapes[count] == target).all():
+            count += 1
+        else:
+            blocks.append([start, count])
+            chunk_shapes.append(target)
+            start = count
+            target = sorted_shapes[count]
+            count += 1
+    np.save(j("processed_eri_values.npy"), pv)
+    np.save(j("processed_eri_indices.npy"), pi)
+    np.save(j("processed_eri_quads.npy"), pq)
+    np.save(j("block_idxs.npy"), np.asarray(blocks))
+    np.save(j("chunked_quad_shape.npy"), np.asarray(chunk_shapes))

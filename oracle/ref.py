@@ -82,9 +82,17 @@ def unique_quartets(nseg):
                     dtype=np.int32).reshape(-1, 4)
 
 
+def _check_order(q):
+    """md_ref.c builds pair data for i >= j only: every quartet must list its
+    pairs as (a >= b, c >= d)."""
+    if len(q) and ((q[:, 0] < q[:, 1]).any() or (q[:, 2] < q[:, 3]).any()):
+        raise ValueError("quartets must be ordered a >= b and c >= d")
+
+
 def build_jk(seg: Segments, quartets, P, Pa, Pb=None, nthreads=0):
     lib = load()
     q = np.ascontiguousarray(np.asarray(quartets, np.int32).reshape(-1, 4))
+    _check_order(q)
     n = seg.nbf
     P, Pa = (np.ascontiguousarray(x, dtype=np.float64) for x in (P, Pa))
     J, Ka = np.zeros((n, n)), np.zeros((n, n))
@@ -99,6 +107,7 @@ def build_jk(seg: Segments, quartets, P, Pa, Pb=None, nthreads=0):
 def eri_blocks(seg: Segments, quartets):
     lib = load()
     q = np.ascontiguousarray(np.asarray(quartets, np.int32).reshape(-1, 4))
+    _check_order(q)
     nc = (seg.l + 1) * (seg.l + 2) // 2
     sizes = np.prod(nc[q], axis=1) if len(q) else np.zeros(0, int)
     out = np.zeros(int(sizes.sum()))

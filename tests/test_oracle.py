@@ -268,3 +268,201 @@ def test_ao_values_integrate_to_the_overlap_matrix():
     # components xx, xy, yy, xz, yz, zz (combinations order)
     assert abs(Sd[0, 0] - 1.0) < 1e-6 and abs(Sd[0, 2] - 1.0 / 3.0) < 1e-6
     assert abs(Sd[1, 1] - 1.0) < 1e-6 and abs(Sd[0, 1]) < 1e-9
+
+
+# ------------------------------------------------------------------ 1-e pins
+def _prim_shells(specs):
+    """Single-primitive shells (coefficient 1): specs = [(l, alpha, centre)]."""
+    from dft_amd.basis import build_shells_from_entries
+    entries = [[[[l], [alpha], [[1.0]]]] for l, alpha, _ in specs]
+    return build_shells_from_entries(entries, np.array([c for _, _, c in specs], float))
+
+
+def _kin_axis(l, a):
+    """-1/2 <x^l e^{-a x^2} | d2/dx2 | x^l e^{-a x^2}> / <x^l | x^l> of one axis."""
+    return a * (2 * l + 1) / 2 - (2 * a * l * (l - 1) / (2 * l - 1) if l > 1 else 0.0)
+
+
+@pytest.mark.parametrize("l", [0, 1, 2, 3])
+def test_kinetic_diagonal_closed_forms(l):
+    """T of a normalised primitive is a sum of per-axis closed forms
+    (s 3a/2, p_x 5a/2, d_xx 13a/6, d_xy 7a/2, f_xxx 21a/10, ...); the d and f
+    rows take the pow_b >= 2 branch of T_raw (shelled.py:149-155)."""
+    from dft_amd.basis import combinations
+    a = 0.73
+    sh = _prim_shells([(l, a, (0.1, -0.2, 0.3))])
+    _, T, _ = O.one_electron(sh, [], np.zeros((0, 3)))
+    for k, c in enumerate(combinations(l)):
+        want = sum(_kin_axis(int(x), a) for x in c)
+        assert abs(T[k, k] - want) < 1e-12 * max(1.0, want), (l, c, T[k, k], want)
+    assert abs(_kin_axis(0, a) * 3 - 1.5 * a) < 1e-15
+    assert abs(_kin_axis(2, a) + 2 * _kin_axis(0, a) - 13 * a / 6) < 1e-15
+
+
+def _quad_1d(la, a, A, lb, b, B):
+    """Overlap and -1/2 <g_a|d2/dx2|g_b> = +1/2 <g_a'|g_b'> of two unnormalised
+    1-D Gaussians by the trapezoidal rule on a uniform grid (spectrally
+    accurate for these integrands): independent of the binomial overlap and
+    of the T_raw recurrence."""
+    w = 1.0 / math.sqrt(min(a, b))
+    x = np.linspace(min(A, B) - 14 * w, max(A, B) + 14 * w, 40001)
+    h = x[1] - x[0]
+    xa, xb = x - A, x - B
+    ga = xa ** la * np.exp(-a * xa ** 2)
+    gb = xb ** lb * np.exp(-b * xb ** 2)
+    da = ((la * xa ** (la - 1) if la else 0.0) - 2 * a * xa ** (la + 1)) * np.exp(-a * xa ** 2)
+    db = ((lb * xb ** (lb - 1) if lb else 0.0) - 2 * b * xb ** (lb + 1)) * np.exp(-b * xb ** 2)
+    return h * np.sum(ga * gb), 0.5 * h * np.sum(da * db)
+
+
+@pytest.mark.parametrize("la,lb", [(0, 0), (1, 0), (2, 1), (2, 2), (3, 2), (1, 3)])
+def test_kinetic_and_overlap_two_centres_vs_quadrature(la, lb):
+    """Two-centre S and T blocks of the oracle against 1-D quadrature of
+    S_x S_y S_z and T = T_x S_y S_z + S_x T_y S_z + S_x S_y T_z."""
+    from dft_amd.basis import combinations, normal
+    a, b = 0.9, 0.45
+    A, B = np.array([0.2, -0.4, 0.5]), np.array([-0.6, 0.3, 1.1])
+    sh = _prim_shells([(la, a, A), (lb, b, B)])
+    S, T, _ = O.one_electron(sh, [], np.zeros((0, 3)))
+    na = len(combinations(la))
+    for i, ca in enumerate(combinations(la)):
+        for j, cb in enumerate(combinations(lb)):
+            s1 = [_quad_1d(ca[d], a, A[d], cb[d], b, B[d]) for d in range(3)]
+            ov = np.prod([q[0] for q in s1])
+            kin = sum(s1[d][1] * np.prod([s1[e][0] for e in range(3) if e != d]) for d in range(3))
+            nrm = normal([a], [ca])[0] * normal([b], [cb])[0]
+            assert abs(S[i, na + j] - nrm * ov) < 1e-11, (ca, cb)
+            assert abs(T[i, na + j] - nrm * kin) < 1e-11, (ca, cb, T[i, na + j], nrm * kin)
+
+
+def _df2(n):
+    return float(np.prod(np.arange(n, 0, -2))) if n > 0 else 1.0
+
+
+@pytest.mark.parametrize("l", [0, 1, 2, 3])
+def test_nuclear_attraction_one_centre_closed_form(l):
+    """<g_i|-Z/r|g_j> for normalised primitives of one shell centred on the
+    nucleus: radial integral Gamma((L_i+L_j)/2+1) / (2 (2a)^(...)) times the
+    angular integral 4 pi prod (n_d - 1)!! / (n+1)!! of x^n over the sphere
+    (zero for any odd power); the diagonal is -Z Gamma(l+1) sqrt(2a) / Gamma(l+3/2)."""
+    from dft_amd.basis import combinations, normal
+    a, Zc, C = 1.3, 2.0, np.array([0.3, 0.1, -0.7])
+    sh = _prim_shells([(l, a, C)])
+    _, _, V = O.one_electron(sh, [Zc], C[None, :])
+    comps = combinations(l)
+    for i, ci in enumerate(comps):
+        for j, cj in enumerate(comps):
+            n = [ci[d] + cj[d] for d in range(3)]
+            if any(x % 2 for x in n):
+                want = 0.0
+            else:
+                ang = 4 * math.pi * np.prod([_df2(x - 1) for x in n]) / _df2(sum(n) + 1)
+                rad = math.gamma(sum(n) / 2 + 1) / (2 * (2 * a) ** (sum(n) / 2 + 1))
+                want = -Zc * normal([a], [ci])[0] * normal([a], [cj])[0] * ang * rad
+            assert abs(V[i, j] - want) < 1e-12 * max(1.0, abs(want)), (ci, cj, V[i, j], want)
+        diag = -Zc * math.gamma(l + 1) * math.sqrt(2 * a) / math.gamma(l + 1.5)
+        assert abs(V[i, i] - diag) < 1e-12 * abs(diag)
+
+
+def test_nuclear_attraction_off_centre_from_s_closed_form():
+    """p and d attraction integrals with the nucleus off both centres from the
+    s closed form V_ss = -Z (2 pi / p) exp(-ab/p R_AB^2) F_0(p |P - C|^2),
+    F_0 = sqrt(pi/T) erf(sqrt T) / 2, by the centre-derivative identities
+    x_A e^{-a r_A^2} = (1/2a) d/dA_x e^{-a r_A^2} and
+    x_A^2 e^{-a r_A^2} = (d2/dA_x^2 + 2a) e^{-a r_A^2} / (4 a^2)."""
+    from scipy.special import erf
+    from dft_amd.basis import combinations, normal
+    a, b, Zc = 0.8, 0.55, 1.7
+    A, B, C = np.array([0.1, 0.2, -0.3]), np.array([-0.7, 0.4, 0.6]), np.array([0.5, -0.9, 0.2])
+
+    def vss(Ap):
+        p = a + b
+        P = (a * Ap + b * B) / p
+        T = p * np.sum((P - C) ** 2)
+        f0 = 0.5 * math.sqrt(math.pi / T) * erf(math.sqrt(T))
+        return -Zc * 2 * math.pi / p * math.exp(-a * b / p * np.sum((Ap - B) ** 2)) * f0
+
+    h = 1e-4
+    e = np.eye(3)
+
+    def d1(k):
+        return (vss(A + h * e[k]) - vss(A - h * e[k])) / (2 * h)
+
+    def d2(k, m):
+        return (vss(A + h * e[k] + h * e[m]) - vss(A + h * e[k] - h * e[m]) - vss(A - h * e[k] + h * e[m])
+                + vss(A - h * e[k] - h * e[m])) / (4 * h * h)
+    for l in (1, 2):
+        sh = _prim_shells([(l, a, A), (0, b, B)])
+        _, _, V = O.one_electron(sh, [Zc], C[None, :])
+        nb = normal([b], [[0, 0, 0]])[0]
+        for i, c in enumerate(combinations(l)):
+            axes = [d for d in range(3) for _ in range(c[d])]
+            if l == 1:
+                raw = d1(axes[0]) / (2 * a)
+            elif axes[0] != axes[1]:
+                raw = d2(axes[0], axes[1]) / (4 * a * a)
+            else:
+                raw = (d2(axes[0], axes[0]) + 2 * a * vss(A)) / (4 * a * a)
+            want = normal([a], [c])[0] * nb * raw
+            got = V[i, len(combinations(l))]
+            assert abs(got - want) < 2e-6 * abs(want) + 1e-9, (l, c, got, want)
+
+
+# ---------------------------------------------------------------- SCF pins
+def test_oracle_scf_diis_converges_to_the_roothaan_energy(water_shells):
+    """The damped + DIIS loop of shelled.py:1103-1292 and the plain loop of
+    main.py:753-828 have the same fixed point: their converged energies agree
+    (the schedule changes the path, not the solution); the DIIS loop switches
+    to DIIS and needs more iterations than the undamped one."""
+    Z, xyz, sh = water_shells
+    S, T, V = O.one_electron(sh, Z, xyz)
+    ERI = O.eri_tensor(sh)
+    jk = lambda P, a, b: O.jk_dense(ERI, P, a, b)  # noqa: E731
+    E_NN = O.nuclear_repulsion(Z, xyz)
+    plain = O.scf_plain(T + V, S, jk, E_NN, 5, 5, e_tol=1e-12, p_tol=1e-10)
+    diis = O.scf_diis(T + V, S, jk, E_NN, 5, 5, e_tol=1e-12, p_tol=1e-10)
+    assert plain["converged"] and diis["converged"]
+    assert abs(plain["E"] - diis["E"]) < 1e-10
+    # at the reference's own tolerances the damped loop stops earlier on its
+    # path: its exit energy uses the damped (non-idempotent) density, whose
+    # energy error is first order in the remaining density change
+    d8 = O.scf_diis(T + V, S, jk, E_NN, 5, 5)
+    assert d8["converged"] and abs(d8["E"] - plain["E"]) < 1e-7
+    assert diis["count"] > plain["count"]
+    assert abs(diis["E"] - float(golden("water_sto3g.npz")["E"])) < 1e-8
+    # damping keeps the energy sequence of the first iterations monotone here
+    e = diis["energies"]
+    assert e[1] < e[0]
+
+
+def test_oracle_process_eri_invariants(tmp_path):
+    """Restated process_eri on a synthetic store: processed values are the
+    quartets' values in the stable lexsort order of their shapes, each block
+    holds one shape, consecutive blocks differ, and the blocks partition the
+    list."""
+    rng = np.random.default_rng(4)
+    nq = 300
+    shapes = rng.choice([1, 3, 6], size=(nq, 4)).astype(np.int16)
+    sizes = shapes.astype(np.int64).prod(1)
+    ends = np.cumsum(sizes)
+    idx = np.stack([ends - sizes, ends], 1).astype(np.int32)
+    vals = rng.standard_normal(int(ends[-1]))
+    quads = rng.integers(0, 50, size=(nq, 4)).astype(np.int16)
+    for name, arr in (("eri_values", vals), ("eri_indices", idx), ("eri_quads", quads), ("eri_quads_shapes", shapes)):
+        np.save(tmp_path / f"{name}.npy", arr)
+    O.process_eri(str(tmp_path))
+    pv = np.load(tmp_path / "processed_eri_values.npy")
+    pi = np.load(tmp_path / "processed_eri_indices.npy")
+    pq = np.load(tmp_path / "processed_eri_quads.npy")
+    bl = np.load(tmp_path / "block_idxs.npy")
+    cs = np.load(tmp_path / "chunked_quad_shape.npy")
+    order = sorted(range(nq), key=lambda k: tuple(shapes[k]))  # stable: ties keep input order
+    assert pq.dtype == np.int32 and np.array_equal(pq, quads[order])
+    assert np.array_equal(np.sort(pv), np.sort(vals))
+    for k, q in enumerate(order):
+        assert np.array_equal(pv[pi[k, 0]:pi[k, 1]], vals[idx[q, 0]:idx[q, 1]])
+    assert bl[0, 0] == 0 and bl[-1, 1] == nq and np.array_equal(bl[1:, 0], bl[:-1, 1])
+    ss = shapes[order]
+    for (s0, s1), shp in zip(bl, cs):
+        assert (ss[s0:s1] == shp).all()
+    assert all((cs[k] != cs[k + 1]).any() for k in range(len(cs) - 1))
