@@ -15,6 +15,8 @@ DFT_OK, DFT_EINVAL, DFT_EUNSUPPORTED, DFT_EHIP = 0, -1, -2, -3
 DFT_PLAN_UHF = 1
 DFT_PLAN_NOSPLIT = 2
 DFT_PLAN_SERIAL = 4
+DFT_PLAN_STATIC_WORKSPACE = 8
+DFT_STATIC_SCRATCH_BYTES = 32 << 20  # include/dft_hip.h
 STATS_COLS = 8  # DFT_STATS_COLS: values per class combination in dft_plan_stats
 DFT_FP64, DFT_MIXED = 0, 1
 MODES = {"fp64": DFT_FP64, "mixed": DFT_MIXED}
@@ -41,6 +43,7 @@ SIGNATURES = {
     "dft_plan_stats": (C.c_int, [_vp, _i64p, C.c_int, C.POINTER(C.c_int), _i64p, _i64p]),
     "dft_schwarz": (C.c_int, [_vp, _dp]),
     "dft_jk": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int, _vp]),
+    "dft_jk_static": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int, _vp]),
     "dft_eri_quartets": (C.c_int, [_vp, _i32p, C.c_int64, _dp]),
     "dft_oneelec": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "dft_ao_grid": (C.c_int, [_vp, _vp, C.c_int64, _vp, C.c_int64, _vp]),

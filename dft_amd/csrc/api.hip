@@ -113,6 +113,9 @@ int upload(T** d, const std::vector<T>& h) {
 
 int ncart_h(int l) { return (l + 1) * (l + 2) / 2; }
 
+// workspace regions start on 256-byte boundaries
+long long ws_align(long long n) { return (n + 31) / 32 * 32; }
+
 constexpr double PRIM_CUT = 39.1439465808987;  // -ln(1e-17)
 
 // launch shape of one class-combination launch (chosen by dft_plan_tune)
@@ -142,14 +145,26 @@ struct Combo {
   // slot order (slot row = (cb * splk + ck) * q.size() + quartet)
   int splb = 1, splk = 1;
   // heavy (column-kernel) combinations: integrals of the whole list,
-  // q.size() x NI doubles, read by the digest kernel
-  double* d_ibuf = nullptr;
+  // q.size() x NI x splb doubles at this offset of the workspace's integral
+  // region, read by the digest kernel
+  long long ibuf_off = 0;
+};
+
+// Device scratch of one Fock build: the half matrices ((1 + nk) x nelem),
+// the J/K staging slots and the heavy classes' integral buffers. A plan owns
+// one (dft_jk); dft_jk_static points the same regions into the library's
+// static scratch array.
+struct Workspace {
+  double* half = nullptr;
+  double* stage = nullptr;
+  double* ibuf = nullptr;
 };
 
 }  // namespace
 
 struct dft_plan_s {
   int nsh = 0, natom = 0, nk = 1, nmol = 1;
+  int device = 0;                // HIP device the plan was built on
   bool split = true;             // primitive split of small lane-kernel classes
   // class launches of a build on several streams: helps builds whose class
   // launches cannot fill the chip one at a time; large builds run faster
@@ -191,11 +206,15 @@ struct dft_plan_s {
   float* d_recT32[2] = {};
   long long cls_base[NPC] = {}, cls_base32[2][NPC] = {};
   int cls_first[NPC] = {}, cls_n[NPC] = {};
-  double* d_stage = nullptr;
+  // the plan's own workspace (one allocation; not made for a plan built with
+  // DFT_PLAN_STATIC_WORKSPACE) and the sizes of its regions in doubles
+  double* d_ws = nullptr;
+  Workspace ws;
+  long long ibuf_len = 0;
+  bool own_ws = true;
   Target* d_tg = nullptr;
   long long* d_tbase = nullptr;
   int* d_tcount = nullptr;
-  double* d_half = nullptr;      // (1 + nk) * nelem
   long long* d_mol_base = nullptr;
   int* d_mol_nbf = nullptr;
   double* d_atoms = nullptr;     // x, y, z, Z
@@ -239,11 +258,10 @@ void free_plan(dft_plan p) {
   for (auto& c : p->combos) {
     (void)hipFree(c.d_q);
     (void)hipFree(c.d_slots);
-    (void)hipFree(c.d_ibuf);
   }
   void* ptrs[] = {p->d_sh,    p->d_exps,  p->d_coefs,   p->d_pairs,    p->d_shell_dm,  p->d_shell_ao,
-                  p->d_rec64, p->d_rec32[0], p->d_stage,   p->d_tg,       p->d_tbase,     p->d_tcount,
-                  p->d_half,  p->d_atoms, p->d_shpairs, p->d_mol_base, p->d_mol_nbf,   p->d_grec,
+                  p->d_rec64, p->d_rec32[0], p->d_ws,      p->d_tg,       p->d_tbase,     p->d_tcount,
+                  p->d_atoms, p->d_shpairs, p->d_mol_base, p->d_mol_nbf,   p->d_grec,
                   p->d_gpairs, p->d_g1slots, p->d_gbase, p->d_gcount, p->d_gstage, p->d_recT64, p->d_g1tasks,
                   p->d_gchunks, p->d_gcfirst, p->d_gpart,
                   p->d_pairs32[0], p->d_pairs32[1], p->d_rec32[1], p->d_recT32[0], p->d_recT32[1]};
@@ -259,6 +277,21 @@ void free_plan(dft_plan p) {
 }
 
 long long pair_key(int i, int j) { return ((long long)i << 32) | (unsigned)j; }
+
+// Doubles of a plan's workspace: half matrices, staging, integral buffers.
+long long workspace_len(dft_plan p) {
+  return ws_align((1 + p->nk) * p->nelem) + ws_align(p->stage_len) + p->ibuf_len;
+}
+
+// The workspace regions of a plan laid out from base.
+Workspace workspace_at(dft_plan p, double* base) {
+  Workspace w;
+  w.half = base;
+  w.stage = base + ws_align((1 + p->nk) * p->nelem);
+  w.ibuf = w.stage + ws_align(p->stage_len);
+  return w;
+}
+
 
 RecView rec_view(dft_plan p, int ci, bool f32, int lev = 0);
 
@@ -789,7 +822,8 @@ int build_plan(dft_plan p, const Input& in) {
           c.splb = (int)std::max<long long>(1, std::min<long long>(want, nbmax));
         }
         if (nq * ncd * c.splb > 0x7fffffffLL) return fail(DFT_EINVAL, "class combination exceeds 2^31 lanes");
-        HIP_TRY(hipMalloc(&c.d_ibuf, nq * ni * c.splb * sizeof(double)));
+        c.ibuf_off = p->ibuf_len;
+        p->ibuf_len += ws_align(nq * ni * c.splb);
       }
       if ((rc = upload(&c.d_q, c.q))) return rc;
     }
@@ -883,8 +917,10 @@ int build_plan(dft_plan p, const Input& in) {
   if ((rc = upload(&p->d_tg, tg))) return rc;
   if ((rc = upload(&p->d_tbase, tbase))) return rc;
   if ((rc = upload(&p->d_tcount, tcount))) return rc;
-  HIP_TRY(hipMalloc(&p->d_stage, std::max<long long>(p->stage_len, 1) * sizeof(double)));
-  HIP_TRY(hipMalloc(&p->d_half, std::max<long long>(nmat * p->nelem, 1) * sizeof(double)));
+  if (p->own_ws) {
+    HIP_TRY(hipMalloc(&p->d_ws, std::max<long long>(workspace_len(p), 1) * sizeof(double)));
+    p->ws = workspace_at(p, p->d_ws);
+  }
   return DFT_OK;
 }
 
@@ -923,8 +959,8 @@ RecView rec_view(dft_plan p, int ci, bool f32, int lev) {
 // combinations write their J/K blocks directly; heavy combinations write
 // integrals to the combination's buffer (launch_digest turns them into J/K
 // blocks).
-hipError_t launch_level(dft_plan p, size_t ci, int lev, long long first, long long n, const Shape& sh,
-                        const double* P, const double* Pa, const double* Pb, hipStream_t s) {
+hipError_t launch_level(dft_plan p, const Workspace& ws, size_t ci, int lev, long long first, long long n,
+                        const Shape& sh, const double* P, const double* Pa, const double* Pb, hipStream_t s) {
   const Combo& c = p->combos[ci];
   if (n <= 0) return hipSuccess;
   const bool f32 = lev >= 0;
@@ -933,35 +969,35 @@ hipError_t launch_level(dft_plan p, size_t ci, int lev, long long first, long lo
   rv.n = (int)n;
   const int4* prs = f32 ? p->d_pairs32[lev] : p->d_pairs;  // FP32 launches: that level's primitive lists
   if (lane_kernel_combo((int)ci))
-    return launch_eri_jk((int)ci, f32, c.d_q, c.d_slots, prs, rv, p->d_shell_dm, P, Pa, Pb, p->nk, p->d_stage,
+    return launch_eri_jk((int)ci, f32, c.d_q, c.d_slots, prs, rv, p->d_shell_dm, P, Pa, Pb, p->nk, ws.stage,
                          p->d_boys, sh.block, sh.cap, s);
   int la, lb, lc, ld;
   pc_moments(c.pcb, la, lb);
   pc_moments(c.pck, lc, ld);
   const long long ni = (long long)ncart_h(la) * ncart_h(lb) * ncart_h(lc) * ncart_h(ld);
-  return launch_eri_chunk((int)ci, f32, c.d_q, (int)first, (int)n, c.splb, prs, rv, c.d_ibuf, 1, ni,
+  return launch_eri_chunk((int)ci, f32, c.d_q, (int)first, (int)n, c.splb, prs, rv, ws.ibuf + c.ibuf_off, 1, ni,
                           ni * (long long)c.q.size(), p->d_boys, sh.block, sh.cap, true, s);
 }
 
 // The FP64 or FP32 kernel of a class combination on quartets [first, first + n).
-hipError_t launch_combo(dft_plan p, size_t ci, bool f32, long long first, long long n, const Shape& sh,
-                        const double* P, const double* Pa, const double* Pb, hipStream_t s) {
+hipError_t launch_combo(dft_plan p, const Workspace& ws, size_t ci, bool f32, long long first, long long n,
+                        const Shape& sh, const double* P, const double* Pa, const double* Pb, hipStream_t s) {
   const Combo& c = p->combos[ci];
   if (f32) {
     // FP32 quartets [n64, n64 + n32a) use the level-0 records, the rest level 1
     const long long bnd = c.n64 + c.n32a, end = first + n;
     if (first < bnd && end > bnd) {
-      hipError_t e = launch_level(p, ci, 0, first, bnd - first, sh, P, Pa, Pb, s);
-      return e != hipSuccess ? e : launch_level(p, ci, 1, bnd, end - bnd, sh, P, Pa, Pb, s);
+      hipError_t e = launch_level(p, ws, ci, 0, first, bnd - first, sh, P, Pa, Pb, s);
+      return e != hipSuccess ? e : launch_level(p, ws, ci, 1, bnd, end - bnd, sh, P, Pa, Pb, s);
     }
-    return launch_level(p, ci, first >= bnd ? 1 : 0, first, n, sh, P, Pa, Pb, s);
+    return launch_level(p, ws, ci, first >= bnd ? 1 : 0, first, n, sh, P, Pa, Pb, s);
   }
-  return launch_level(p, ci, -1, first, n, sh, P, Pa, Pb, s);
+  return launch_level(p, ws, ci, -1, first, n, sh, P, Pa, Pb, s);
 }
 
 // J/K blocks of a heavy combination from its integral buffer (whole list).
-hipError_t launch_digest(dft_plan p, size_t ci, const double* P, const double* Pa, const double* Pb,
-                         hipStream_t s) {
+hipError_t launch_digest(dft_plan p, const Workspace& ws, size_t ci, const double* P, const double* Pa,
+                         const double* Pb, hipStream_t s) {
   const Combo& c = p->combos[ci];
   if (lane_kernel_combo((int)ci) || c.q.empty()) return hipSuccess;
   int la, lb, lc, ld;
@@ -969,7 +1005,7 @@ hipError_t launch_digest(dft_plan p, size_t ci, const double* P, const double* P
   pc_moments(c.pck, lc, ld);
   const long long ni = (long long)ncart_h(la) * ncart_h(lb) * ncart_h(lc) * ncart_h(ld);
   return launch_jk_digest((int)ci, c.d_q, c.d_slots, (int)c.q.size(), c.splb, ni * (long long)c.q.size(),
-                          p->d_pairs, p->d_shell_dm, P, Pa, Pb, p->nk, c.d_ibuf, p->d_stage, s);
+                          p->d_pairs, p->d_shell_dm, P, Pa, Pb, p->nk, ws.ibuf + c.ibuf_off, ws.stage, s);
 }
 
 // The whole Fock build: class launches, reduction, symmetrisation. The class
@@ -977,8 +1013,8 @@ hipError_t launch_digest(dft_plan p, size_t ci, const double* P, const double* P
 // to the least-loaded of the caller's stream and the plan's auxiliary
 // streams (greedy by estimated cost, largest first), which fork from and
 // join back into the caller's stream by events (also under graph capture).
-int jk_enqueue(dft_plan p, const double* P, const double* Pa, const double* Pb, double* J, double* Ka, double* Kb,
-               int mode, hipStream_t s, bool profile) {
+int jk_enqueue(dft_plan p, const Workspace& ws, const double* P, const double* Pa, const double* Pb, double* J,
+               double* Ka, double* Kb, int mode, hipStream_t s, bool profile) {
   const int nslot = 2 * (int)p->combos.size() + 2;
   hipEvent_t* ev = nullptr;
   if (profile && p->prof_used < p->prof_steps) ev = &p->prof_ev[(size_t)p->prof_used++ * nslot * 2];
@@ -1039,19 +1075,20 @@ int jk_enqueue(dft_plan p, const double* P, const double* Pa, const double* Pb, 
       const bool f32 = mode == DFT_MIXED && c.use32;
       const long long n1 = f32 ? c.n64 : (long long)c.q.size();
       mark(2 * L.ci, 0, st[k]);
-      hipError_t e = launch_combo(p, L.ci, false, 0, n1, c.shape64, P, Pa, Pb, st[k]);
+      hipError_t e = launch_combo(p, ws, L.ci, false, 0, n1, c.shape64, P, Pa, Pb, st[k]);
       mark(2 * L.ci, 1, st[k]);
       mark(2 * L.ci + 1, 0, st[k]);
       if (e == hipSuccess && f32)
-        e = launch_combo(p, L.ci, true, c.n64, (long long)c.q.size() - c.n64, c.shape32, P, Pa, Pb, st[k]);
-      if (e == hipSuccess) e = launch_digest(p, L.ci, P, Pa, Pb, st[k]);
+        e = launch_combo(p, ws, L.ci, true, c.n64, (long long)c.q.size() - c.n64, c.shape32, P, Pa, Pb, st[k]);
+      if (e == hipSuccess) e = launch_digest(p, ws, L.ci, P, Pa, Pb, st[k]);
       mark(2 * L.ci + 1, 1, st[k]);
       if (e != hipSuccess) return fail(DFT_EHIP, std::string("eri column / digest: ") + hipGetErrorString(e));
       continue;
     }
     const int slot = 2 * L.ci + (L.f32 ? 1 : 0);
     mark(slot, 0, st[k]);
-    hipError_t e = launch_combo(p, L.ci, L.f32, L.first, L.n, L.f32 ? c.shape32 : c.shape64, P, Pa, Pb, st[k]);
+    hipError_t e =
+        launch_combo(p, ws, L.ci, L.f32, L.first, L.n, L.f32 ? c.shape32 : c.shape64, P, Pa, Pb, st[k]);
     if (e != hipSuccess)
       return fail(DFT_EHIP, std::string(L.f32 ? "eri_jk fp32: " : "eri_jk fp64: ") + hipGetErrorString(e));
     mark(slot, 1, st[k]);
@@ -1062,13 +1099,13 @@ int jk_enqueue(dft_plan p, const double* P, const double* Pa, const double* Pb, 
       HIP_TRY(hipStreamWaitEvent(s, p->ev_join[k - 1], 0));
     }
   mark(nslot - 2, 0, s);
-  hipError_t e = launch_reduce(p->ntarget, p->d_tg, p->d_tbase, p->d_tcount, p->d_stage, p->stage_len, p->d_half, s);
+  hipError_t e = launch_reduce(p->ntarget, p->d_tg, p->d_tbase, p->d_tcount, ws.stage, p->stage_len, ws.half, s);
   if (e != hipSuccess) return fail(DFT_EHIP, std::string("reduce: ") + hipGetErrorString(e));
   mark(nslot - 2, 1, s);
   mark(nslot - 1, 0, s);
   double* outs[3] = {J, Ka, Kb};
   for (int m = 0; m < 1 + p->nk; ++m) {
-    e = launch_symmetrize(p->nmol, p->d_mol_base, p->d_mol_nbf, p->max_nbf, p->d_half + m * p->nelem, outs[m], s);
+    e = launch_symmetrize(p->nmol, p->d_mol_base, p->d_mol_nbf, p->max_nbf, ws.half + m * p->nelem, outs[m], s);
     if (e != hipSuccess) return fail(DFT_EHIP, std::string("symmetrize: ") + hipGetErrorString(e));
   }
   mark(nslot - 1, 1, s);
@@ -1082,6 +1119,13 @@ int check_jk_args(dft_plan p, const double* P, const double* Pa, const double* P
   if ((p->nk == 2) != (Pb != nullptr) || (p->nk == 2) != (Kb != nullptr))
     return fail(DFT_EINVAL, "Pb/Kb must be given exactly for a UHF plan");
   if (mode != DFT_FP64 && mode != DFT_MIXED) return fail(DFT_EINVAL, "mode must be DFT_FP64 or DFT_MIXED");
+  return DFT_OK;
+}
+
+int need_own_workspace(dft_plan p) {
+  if (!p->own_ws)
+    return fail(DFT_EINVAL, "plan was built with DFT_PLAN_STATIC_WORKSPACE: it has no workspace of its own, "
+                            "build its J/K with dft_jk_static");
   return DFT_OK;
 }
 
@@ -1280,6 +1324,11 @@ int validate_and_build(int nmol, const int32_t* mol_nsh, const int32_t* mol_nato
   p->natom = natom;
   p->nk = (flags & DFT_PLAN_UHF) ? 2 : 1;
   p->split = !(flags & DFT_PLAN_NOSPLIT);
+  p->own_ws = !(flags & DFT_PLAN_STATIC_WORKSPACE);
+  if (hipGetDevice(&p->device) != hipSuccess) {
+    delete p;
+    return fail(DFT_EHIP, "hipGetDevice failed");
+  }
   p->tau = schwarz_tau;
   p->tau32 = fp32_tau;
   int rc = build_plan(p, in);
@@ -1294,6 +1343,39 @@ int validate_and_build(int nmol, const int32_t* mol_nsh, const int32_t* mol_nato
     return rc;
   }
   *out = p;
+  return DFT_OK;
+}
+
+}  // namespace
+
+// Static device scratch of dft_jk_static: one copy per GPU, held until the
+// process ends. Host code never uses this name as an address (on the host it
+// is only a handle of the symbol): each device's copy is reached through the
+// pointer hipGetSymbolAddress returns for it (scratch_on_current_device).
+static __device__ __align__(256) double scratch[DFT_STATIC_SCRATCH_BYTES / sizeof(double)];
+
+namespace {
+
+constexpr long long SCRATCH_DOUBLES = DFT_STATIC_SCRATCH_BYTES / (long long)sizeof(double);
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[SCRATCH_MAX_DEVICES] = {};
+
+// Device address of the current device's copy of the scratch, looked up once
+// per device (with that device current) and kept.
+int scratch_on_current_device(double** out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) return fail(DFT_EINVAL, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(DFT_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch[dev] = static_cast<double*>(ptr);
+  }
+  *out = g_scratch[dev];
   return DFT_OK;
 }
 
@@ -1349,7 +1431,7 @@ int dft_plan_stats(dft_plan p, int64_t* counts, int ncombo, int* ncombo_out, int
       counts[DFT_STATS_COLS * c + 7] = cb.prim64w;
     }
   if (n_unscreened) *n_unscreened = p->n_unscreened;
-  if (staging_bytes) *staging_bytes = p->stage_len * (int64_t)sizeof(double);
+  if (staging_bytes) *staging_bytes = workspace_len(p) * (int64_t)sizeof(double);
   return DFT_OK;
 }
 
@@ -1369,8 +1451,26 @@ int dft_schwarz(dft_plan p, double* Q) {
 int dft_jk(dft_plan p, const double* P, const double* Pa, const double* Pb, double* J, double* Ka, double* Kb,
            int mode, void* stream) {
   int rc = check_jk_args(p, P, Pa, Pb, J, Ka, Kb, mode);
+  if (!rc) rc = need_own_workspace(p);
   if (rc) return rc;
-  return jk_enqueue(p, P, Pa, Pb, J, Ka, Kb, mode, (hipStream_t)stream, true);
+  return jk_enqueue(p, p->ws, P, Pa, Pb, J, Ka, Kb, mode, (hipStream_t)stream, true);
+}
+
+int dft_jk_static(dft_plan p, const double* P, const double* Pa, const double* Pb, double* J, double* Ka, double* Kb,
+                  int mode, void* stream) {
+  int rc = check_jk_args(p, P, Pa, Pb, J, Ka, Kb, mode);
+  if (rc) return rc;
+  const long long need = workspace_len(p);
+  if (need > SCRATCH_DOUBLES)
+    return fail(DFT_EINVAL, "dft_jk_static: the plan needs " + std::to_string(need * (long long)sizeof(double)) +
+                                " bytes of workspace, the static scratch holds " +
+                                std::to_string(DFT_STATIC_SCRATCH_BYTES) + " (use dft_jk)");
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev != p->device) return fail(DFT_EINVAL, "dft_jk_static: the plan's device must be the current device");
+  double* base = nullptr;
+  if ((rc = scratch_on_current_device(&base))) return rc;
+  return jk_enqueue(p, workspace_at(p, base), P, Pa, Pb, J, Ka, Kb, mode, (hipStream_t)stream, true);
 }
 
 int dft_jk_graph_create(dft_plan p, const double* P, const double* Pa, const double* Pb, double* J, double* Ka,
@@ -1378,13 +1478,14 @@ int dft_jk_graph_create(dft_plan p, const double* P, const double* Pa, const dou
   if (!out) return fail(DFT_EINVAL, "out is NULL");
   *out = nullptr;
   int rc = check_jk_args(p, P, Pa, Pb, J, Ka, Kb, mode);
+  if (!rc) rc = need_own_workspace(p);
   if (rc) return rc;
   hipStream_t cs = nullptr;
   HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
   hipGraph_t graph = nullptr;
   hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
-    rc = jk_enqueue(p, P, Pa, Pb, J, Ka, Kb, mode, cs, false);
+    rc = jk_enqueue(p, p->ws, P, Pa, Pb, J, Ka, Kb, mode, cs, false);
     hipError_t e2 = hipStreamEndCapture(cs, &graph);
     if (rc == DFT_OK && e2 != hipSuccess) e = e2;
   }
@@ -1413,6 +1514,8 @@ int dft_plan_tune(dft_plan p, const double* P, const double* Pa, const double* P
                   int report_cap, void* stream) {
   if (!p || !P || !Pa) return fail(DFT_EINVAL, "NULL argument");
   if ((p->nk == 2) != (Pb != nullptr)) return fail(DFT_EINVAL, "Pb must be given exactly for a UHF plan");
+  int rc0 = need_own_workspace(p);
+  if (rc0) return rc0;
   if (reps < 1) reps = 1;
   hipStream_t s = (hipStream_t)stream;
   hipEvent_t e0, e1;
@@ -1445,13 +1548,13 @@ int dft_plan_tune(dft_plan p, const double* P, const double* Pa, const double* P
       float best = 1e30f;
       Shape best_sh = part ? c.shape32 : c.shape64;
       for (const Shape& sh : cand) {
-        hipError_t e = launch_combo(p, ci, part == 1, first, n, sh, P, Pa, Pb, s);  // warm-up
+        hipError_t e = launch_combo(p, p->ws, ci, part == 1, first, n, sh, P, Pa, Pb, s);  // warm-up
         std::vector<float> t;
         for (int r = 0; r < reps && e == hipSuccess; ++r) {
           // a shape already 2x slower than the best one is not timed again
           if (r > 0 && !t.empty() && t[0] > 2.0f * best) break;
           (void)hipEventRecord(e0, s);
-          e = launch_combo(p, ci, part == 1, first, n, sh, P, Pa, Pb, s);
+          e = launch_combo(p, p->ws, ci, part == 1, first, n, sh, P, Pa, Pb, s);
           (void)hipEventRecord(e1, s);
           if (e == hipSuccess) e = hipEventSynchronize(e1);
           float ms = 0.f;
@@ -1487,7 +1590,7 @@ int dft_plan_tune(dft_plan p, const double* P, const double* Pa, const double* P
         std::vector<float> t;
         for (int r = 0; r <= reps && rc == DFT_OK; ++r) {
           (void)hipEventRecord(e0, s);
-          hipError_t e = launch_combo(p, ci, false, 0, c.n64, c.shape64, P, Pa, Pb, s);
+          hipError_t e = launch_combo(p, p->ws, ci, false, 0, c.n64, c.shape64, P, Pa, Pb, s);
           (void)hipEventRecord(e1, s);
           if (e == hipSuccess) e = hipEventSynchronize(e1);
           float ms = 0.f;
@@ -1515,7 +1618,7 @@ int dft_plan_tune(dft_plan p, const double* P, const double* Pa, const double* P
           rc = fail(DFT_EHIP, "tune: hipMalloc");
       for (int r = 0; r <= reps && rc == DFT_OK; ++r) {
         (void)hipEventRecord(e0, s);
-        rc = jk_enqueue(p, P, Pa, Pb, tmp[0], tmp[1], tmp[2], DFT_FP64, s, false);
+        rc = jk_enqueue(p, p->ws, P, Pa, Pb, tmp[0], tmp[1], tmp[2], DFT_FP64, s, false);
         (void)hipEventRecord(e1, s);
         float ms = 0.f;
         if (rc == DFT_OK && hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess &&
@@ -1544,7 +1647,7 @@ int dft_plan_tune(dft_plan p, const double* P, const double* Pa, const double* P
         for (size_t ci = 0; ci < p->combos.size(); ++ci) p->combos[ci].use32 = all32 ? true : (bool)chosen[ci];
         for (int r = 0; r <= reps && rc == DFT_OK; ++r) {
           (void)hipEventRecord(e0, s);
-          rc = jk_enqueue(p, P, Pa, Pb, tmp[0], tmp[1], tmp[2], DFT_MIXED, s, false);
+          rc = jk_enqueue(p, p->ws, P, Pa, Pb, tmp[0], tmp[1], tmp[2], DFT_MIXED, s, false);
           (void)hipEventRecord(e1, s);
           float ms = 0.f;
           if (rc == DFT_OK && hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess &&

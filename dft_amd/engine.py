@@ -22,9 +22,10 @@ def _ncart(l: int) -> int:
     return (l + 1) * (l + 2) // 2
 
 
-def _flags(uhf: bool, split: bool, serial: bool = False) -> int:
+def _flags(uhf: bool, split: bool, serial: bool = False, static_workspace: bool = False) -> int:
     return ((_lib.DFT_PLAN_UHF if uhf else 0) | (0 if split else _lib.DFT_PLAN_NOSPLIT)
-            | (_lib.DFT_PLAN_SERIAL if serial else 0))
+            | (_lib.DFT_PLAN_SERIAL if serial else 0)
+            | (_lib.DFT_PLAN_STATIC_WORKSPACE if static_workspace else 0))
 
 
 class Plan:
@@ -40,7 +41,7 @@ class Plan:
     """
 
     def __init__(self, shells, Z, centers, schwarz_tau=1e-10, fp32_tau=1e-4, uhf=False,
-                 device=None, split=True, serial=False):
+                 device=None, split=True, serial=False, static_workspace=False):
         self.lib = _lib.load()
         if not torch.cuda.is_available():
             raise _lib.DFTError("no HIP device visible: the integral engine runs on the GPU only")
@@ -79,7 +80,8 @@ class Plan:
                 _lib.ptr(seg_ao, C.c_int32), _lib.ptr(seg_xyz, C.c_double),
                 _lib.ptr(exps, C.c_double), _lib.ptr(coefs, C.c_double), len(Zc),
                 _lib.ptr(xyz, C.c_double), _lib.ptr(Zc, C.c_double), self.nbf,
-                float(schwarz_tau), float(fp32_tau), _flags(uhf, split, serial), C.byref(handle)))
+                float(schwarz_tau), float(fp32_tau), _flags(uhf, split, serial, static_workspace),
+                C.byref(handle)))
         self.handle = handle
         self.schwarz_tau, self.fp32_tau = schwarz_tau, fp32_tau
 
@@ -106,7 +108,7 @@ class Plan:
                 "n_prim_quartets_unscreened": int(c[:, 4:6].sum()),
                 "n_quartets_fp32_level2": int(c[:, 6].sum()),
                 "n_ao_quartets": n_ao,
-                "n_unscreened": int(nun.value), "staging_bytes": int(sb.value)}
+                "n_unscreened": int(nun.value), "workspace_bytes": int(sb.value)}
 
     def schwarz(self) -> np.ndarray:
         Q = np.zeros(self.nseg * self.nseg)

@@ -39,6 +39,12 @@ extern "C" {
                               order of a molecule the same in single and batch plans) */
 #define DFT_PLAN_SERIAL 4  /* all class launches of a Fock build on the caller's stream
                               (no concurrent launches; for per-kernel profiling) */
+#define DFT_PLAN_STATIC_WORKSPACE 8 /* allocate no Fock-build workspace: the plan's J/K are
+                              built only by dft_jk_static (dft_jk, graphs and the tuner
+                              reject it) */
+
+/* Size of the library's static device scratch used by dft_jk_static. */
+#define DFT_STATIC_SCRATCH_BYTES (32LL << 20)
 
 /* precision modes of dft_jk */
 #define DFT_FP64 0  /* every quartet in FP64 */
@@ -78,7 +84,8 @@ const char* dft_last_error(void);
  *  nbf                 number of Cartesian AOs
  *  schwarz_tau         quartets with Q_ab Q_cd < schwarz_tau are skipped
  *  fp32_tau            quartets with Q_ab Q_cd < fp32_tau form the FP32 part
- *  flags               DFT_PLAN_UHF | DFT_PLAN_NOSPLIT | DFT_PLAN_SERIAL or 0
+ *  flags               DFT_PLAN_UHF | DFT_PLAN_NOSPLIT | DFT_PLAN_SERIAL |
+ *                      DFT_PLAN_STATIC_WORKSPACE or 0
  */
 int dft_plan_create(int nsh, const int32_t* shell_l, const int32_t* shell_nprim, const int32_t* shell_ao,
                     const double* shell_xyz, const double* prim_exp, const double* prim_coef, int natom,
@@ -159,7 +166,9 @@ int dft_plan_set_shapes(dft_plan plan, const int* shapes, int n);
  *     an all-FP64 build evaluates there).
  * ncombo is the capacity of counts in combinations; the number of
  * combinations is returned in *ncombo_out. Also reports the quartets before
- * screening and the staging size in bytes.
+ * screening and, in *staging_bytes, the bytes of device workspace one Fock
+ * build uses (half matrices, J/K staging slots, heavy-class integral
+ * buffers): what dft_jk_static needs to fit in DFT_STATIC_SCRATCH_BYTES.
  */
 #define DFT_STATS_COLS 8
 int dft_plan_stats(dft_plan plan, int64_t* counts, int ncombo, int* ncombo_out, int64_t* n_unscreened,
@@ -195,6 +204,27 @@ int dft_schwarz(dft_plan plan, double* Q);
  */
 int dft_jk(dft_plan plan, const double* P, const double* Pa, const double* Pb, double* J, double* Ka,
            double* Kb, int mode, void* stream);
+
+/*
+ * dft_jk with the Fock-build workspace (half matrices, J/K staging slots,
+ * heavy-class integral buffers) in the library's static device scratch
+ * instead of the plan's: allocates nothing, for callers that cannot call
+ * hipMalloc (code under hipGraph capture, tight serving loops, harnesses
+ * that forbid allocation). Same arguments, kernels and results as dft_jk
+ * (bitwise equal J/K). The plan may be one built with
+ * DFT_PLAN_STATIC_WORKSPACE (which then holds no workspace of its own).
+ * Limit: the plan's workspace (dft_plan_stats *staging_bytes) must fit in
+ * DFT_STATIC_SCRATCH_BYTES, else DFT_EINVAL; the call must be made with the
+ * plan's device current. Every call on one GPU shares that GPU's scratch,
+ * so calls that overlap in time on one GPU (other streams, other threads,
+ * other plans) race: serialise them. Nothing in the scratch needs clearing
+ * between calls (every slot is rewritten by each build).
+ *
+ * No counterpart in the reference: an allocation-free variant of
+ * fill_matrices (shelled.py:770-877) as replaced by dft_jk.
+ */
+int dft_jk_static(dft_plan plan, const double* P, const double* Pa, const double* Pb, double* J, double* Ka,
+                  double* Kb, int mode, void* stream);
 
 /*
  * Analytic nuclear gradient of the (U)HF energy without the nuclear
