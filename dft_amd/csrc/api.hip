@@ -128,8 +128,16 @@ struct Shape {
 struct Combo {
   int pcb = 0, pck = 0;
   std::vector<int2> q;         // (bra pair, ket pair) plan pair ids: FP64 part, then weak part
-  long long n64 = 0;           // size of the FP64 part
-  long long n32a = 0;          // size of the first FP32 level (the rest is level 2)
+  // lane-kernel lists are padded: every molecule's FP64 part and FP32
+  // level parts start on a wave (WAVE_QUARTETS) boundary, the gaps filled
+  // with dummy quartets (bra pair -1), so the quartets a wave holds are the
+  // same in every launch split (FP64 / mixed, single / batch plan)
+  long long n64 = 0;           // size of the FP64 part (padded)
+  long long n32a = 0;          // size of the first FP32 level (padded; the rest is level 2)
+  long long r64 = 0, r32a = 0, r32b = 0;  // quartets without the padding
+  // per (wave, block kind): bit l set where lane l starts a run of lanes
+  // whose block goes to the same target (the run's last lane stores its sum)
+  unsigned long long* d_segmask = nullptr;
   int2* d_q = nullptr;
   long long* d_slots = nullptr;  // per quartet 2 + 4 nk staging offsets
   long long prim64 = 0, prim32 = 0;      // primitive shell quartets evaluated
@@ -235,7 +243,8 @@ struct dft_plan_s {
   std::vector<int> grad_atom;     // internal shell -> atom
   double* d_grec = nullptr;
   int4* d_gpairs = nullptr;
-  std::vector<int2*> d_gq_swap;
+  std::vector<int2*> d_gq, d_gq_swap;  // per combination: quartets (no padding), bra and ket exchanged
+  std::vector<long long> gq_n;
   std::vector<long long*> d_gslots_ab, d_gslots_cd;
   long long* d_g1slots = nullptr;
   int4* d_g1tasks = nullptr;
@@ -258,6 +267,7 @@ void free_plan(dft_plan p) {
   for (auto& c : p->combos) {
     (void)hipFree(c.d_q);
     (void)hipFree(c.d_slots);
+    (void)hipFree(c.d_segmask);
   }
   void* ptrs[] = {p->d_sh,    p->d_exps,  p->d_coefs,   p->d_pairs,    p->d_shell_dm,  p->d_shell_ao,
                   p->d_rec64, p->d_rec32[0], p->d_ws,      p->d_tg,       p->d_tbase,     p->d_tcount,
@@ -271,6 +281,7 @@ void free_plan(dft_plan p) {
   for (hipEvent_t e : p->ev_join) (void)hipEventDestroy(e);
   if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
   for (auto* x : p->d_gq_swap) (void)hipFree(x);
+  for (auto* x : p->d_gq) (void)hipFree(x);
   for (auto* x : p->d_gslots_ab) (void)hipFree(x);
   for (auto* x : p->d_gslots_cd) (void)hipFree(x);
   delete p;
@@ -755,7 +766,10 @@ int build_plan(dft_plan p, const Input& in) {
       Combo& c = p->combos[combo_index(pcb, pck)];
       c.pcb = pcb;
       c.pck = pck;
-      std::vector<int2> weak, weak2;
+      const bool lane = lane_kernel_combo(combo_index(pcb, pck));
+      // per molecule: FP64 part, first and second FP32 level
+      std::vector<std::vector<int2>> part[3];
+      for (auto& x : part) x.resize(in.nmol);
       // ket-major: consecutive quartets (the lanes of a wave) share the ket
       // pair, whose primitive records the lane kernel reads in its inner loop,
       // so those loads are wave-uniform; the bra pairs of a wave are
@@ -771,34 +785,52 @@ int build_plan(dft_plan p, const Input& in) {
             const long long kp = (long long)p->sh[p->pairs[ib].x].nprim * p->sh[p->pairs[ib].y].nprim *
                                  p->sh[p->pairs[ik].x].nprim * p->sh[p->pairs[ik].y].nprim;
             if (b < p->tau32) {
-              (b < FP32_LEVEL2 * p->tau32 ? weak2 : weak).push_back(make_int2(ib, ik));
+              part[b < FP32_LEVEL2 * p->tau32 ? 2 : 1][m].push_back(make_int2(ib, ik));
               c.prim64w += np;
               c.kprod32 += kp;
             } else {
-              c.q.push_back(make_int2(ib, ik));
+              part[0][m].push_back(make_int2(ib, ik));
               c.prim64 += np;
               c.kprod64 += kp;
             }
           }
         }
+      long long nreal[3] = {0, 0, 0};
+      for (int k = 0; k < 3; ++k)
+        for (const auto& v : part[k]) nreal[k] += (long long)v.size();
       // the second FP32 level only where its part is large enough to pay
-      // for a launch of its own
-      if ((long long)weak2.size() < FP32_LEVEL2_MIN) {
-        weak.insert(weak.end(), weak2.begin(), weak2.end());
-        weak2.clear();
+      // for a launch of its own (otherwise its quartets use the first level's
+      // records; the list layout is the same either way)
+      const bool level2 = nreal[2] >= FP32_LEVEL2_MIN;
+      for (int m = 0; m < in.nmol; ++m) {
+        for (const int2& qq : part[1][m]) c.prim32 += (long long)p->pairs32[0][qq.x].w * p->pairs32[0][qq.y].w;
+        for (const int2& qq : part[2][m])
+          c.prim32 += (long long)p->pairs32[level2 ? 1 : 0][qq.x].w * p->pairs32[level2 ? 1 : 0][qq.y].w;
       }
-      for (const int2& qq : weak) c.prim32 += (long long)p->pairs32[0][qq.x].w * p->pairs32[0][qq.y].w;
-      for (const int2& qq : weak2) c.prim32 += (long long)p->pairs32[1][qq.x].w * p->pairs32[1][qq.y].w;
-      c.n64 = (long long)c.q.size();
-      c.n32a = (long long)weak.size();
-      c.q.insert(c.q.end(), weak.begin(), weak.end());
-      c.q.insert(c.q.end(), weak2.begin(), weak2.end());
+      c.r64 = nreal[0];
+      c.r32a = nreal[1];
+      c.r32b = nreal[2];
+      // [FP64 parts of all molecules][first FP32 level][second FP32 level];
+      // for the lane kernel each molecule's part padded to whole waves with
+      // dummies that repeat the last ket pair (so the wave's ket stays uniform)
+      for (int k = 0; k < 3; ++k) {
+        for (int m = 0; m < in.nmol; ++m) {
+          const auto& v = part[k][m];
+          c.q.insert(c.q.end(), v.begin(), v.end());
+          if (lane && !v.empty())
+            while (c.q.size() % WAVE_QUARTETS) c.q.push_back(make_int2(-1, v.back().y));
+        }
+        if (k == 0) c.n64 = (long long)c.q.size();
+        if (k == 1) c.n32a = (long long)c.q.size() - c.n64;
+      }
+      if (!level2) c.n32a = (long long)c.q.size() - c.n64;
       // primitive split where the class has too few quartets to fill the
       // chip with one lane each (~64 Ki lanes): bra chunks first, then ket
       const long long nq = (long long)c.q.size();
-      if (p->split && nq > 0 && lane_kernel_combo(combo_index(pcb, pck)) && nq < SPLIT_LANES) {
+      if (p->split && nq > 0 && lane && nq < SPLIT_LANES) {
         int nbmax = 0, nkmax = 0;
         for (const int2& qq : c.q) {
+          if (qq.x < 0) continue;
           nbmax = std::max(nbmax, p->pairs[qq.x].w);
           nkmax = std::max(nkmax, p->pairs[qq.y].w);
         }
@@ -869,11 +901,53 @@ int build_plan(dft_plan p, const Input& in) {
     const Combo& c = p->combos[ci];
     return lane_kernel_combo((int)ci) ? c.splb * c.splk : 1;
   };
-  for (size_t ci = 0; ci < p->combos.size(); ++ci)
+  // Lane-kernel combinations: the lanes of a wave (WAVE_QUARTETS consecutive
+  // quartets of one primitive chunk) whose block of kind k goes to the same
+  // target form runs; each run is summed across its lanes in the kernel and
+  // stored once, by its last lane. visit(k, ch, i, t) is called for every
+  // run end i (t = its target) in lane order; masks receive the run starts.
+  constexpr size_t NO_TARGET = ~(size_t)0;
+  auto lane_runs = [&](size_t ci, std::vector<unsigned long long>* masks, auto&& visit) {
+    const Combo& c = p->combos[ci];
+    const size_t nq = c.q.size(), ns = (size_t)slot_rows(ci);
+    std::vector<size_t> t(WAVE_QUARTETS * nblk);
+    for (size_t ch = 0; ch < ns; ++ch)
+      for (size_t w0 = 0; w0 < nq; w0 += WAVE_QUARTETS) {
+        for (int l = 0; l < WAVE_QUARTETS; ++l) {
+          const int2 qq = c.q[w0 + l];
+          if (qq.x < 0) {
+            for (int k = 0; k < nblk; ++k) t[l * nblk + k] = NO_TARGET;
+          } else {
+            quartet_targets(qq, tq);
+            for (int k = 0; k < nblk; ++k) t[l * nblk + k] = tq[k];
+          }
+        }
+        const size_t wave = (ch * nq + w0) / WAVE_QUARTETS;
+        for (int k = 0; k < nblk; ++k) {
+          unsigned long long m = 0;
+          for (int l = 0; l < WAVE_QUARTETS; ++l) {
+            const size_t tl = t[l * nblk + k];
+            if (l == 0 || tl == NO_TARGET || tl != t[(l - 1) * nblk + k]) m |= 1ull << l;
+          }
+          if (masks) (*masks)[wave * nblk + k] = m;
+          for (int l = 0; l < WAVE_QUARTETS; ++l) {
+            const bool last = l == WAVE_QUARTETS - 1 || ((m >> (l + 1)) & 1);
+            const size_t tl = t[l * nblk + k];
+            if (last && tl != NO_TARGET) visit(k, ch, w0 + l, tl);
+          }
+        }
+      }
+  };
+  for (size_t ci = 0; ci < p->combos.size(); ++ci) {
+    if (lane_kernel_combo((int)ci)) {
+      lane_runs(ci, nullptr, [&](int, size_t, size_t, size_t tl) { ++tcount[tl]; });
+      continue;
+    }
     for (const int2& qq : p->combos[ci].q) {
       quartet_targets(qq, tq);
       for (int k = 0; k < nblk; ++k) tcount[tq[k]] += slot_rows(ci);
     }
+  }
   std::vector<long long> tbase(p->ntarget, 0), tnext(p->ntarget, 0);
   std::vector<Target> tg(p->ntarget);
   long long cur = 0;
@@ -901,14 +975,25 @@ int build_plan(dft_plan p, const Input& in) {
   for (size_t ci = 0; ci < p->combos.size(); ++ci) {
     Combo& c = p->combos[ci];
     const size_t nq = c.q.size(), ns = (size_t)slot_rows(ci);
-    const bool lane = lane_kernel_combo((int)ci);
+    if (lane_kernel_combo((int)ci)) {
+      // slot of each run's last lane; -1 for the other lanes
+      std::vector<long long> slots(nq * ns * nblk, -1);
+      std::vector<unsigned long long> masks(nq * ns / WAVE_QUARTETS * nblk);
+      lane_runs(ci, &masks, [&](int k, size_t ch, size_t i, size_t tl) {
+        slots[((size_t)k * ns + ch) * nq + i] = tnext[tl];
+        tnext[tl] += (long long)tg[tl].nrow * tg[tl].ncol;
+      });
+      if ((rc = upload(&c.d_slots, slots))) return rc;
+      if ((rc = upload(&c.d_segmask, masks))) return rc;
+      continue;
+    }
     std::vector<long long> slots(nq * ns * nblk);
     for (size_t ch = 0; ch < ns; ++ch)
       for (size_t i = 0; i < nq; ++i) {
         quartet_targets(c.q[i], tq);
         for (int k = 0; k < nblk; ++k) {
           const Target& T = tg[tq[k]];
-          slots[lane ? ((size_t)k * ns + ch) * nq + i : (ch * nq + i) * nblk + k] = tnext[tq[k]];
+          slots[(ch * nq + i) * nblk + k] = tnext[tq[k]];
           tnext[tq[k]] += (long long)T.nrow * T.ncol;
         }
       }
@@ -969,7 +1054,8 @@ hipError_t launch_level(dft_plan p, const Workspace& ws, size_t ci, int lev, lon
   rv.n = (int)n;
   const int4* prs = f32 ? p->d_pairs32[lev] : p->d_pairs;  // FP32 launches: that level's primitive lists
   if (lane_kernel_combo((int)ci))
-    return launch_eri_jk((int)ci, f32, c.d_q, c.d_slots, prs, rv, p->d_shell_dm, P, Pa, Pb, p->nk, ws.stage,
+    return launch_eri_jk((int)ci, f32, c.d_q, c.d_slots, c.d_segmask, prs, rv, p->d_shell_dm, P, Pa, Pb, p->nk,
+                         ws.stage,
                          p->d_boys, sh.block, sh.cap, s);
   int la, lb, lc, ld;
   pc_moments(c.pcb, la, lb);
@@ -1178,6 +1264,7 @@ int grad_prepare(dft_plan p, const int32_t* shell_atom) {
   auto atom_of = [&](int pid, int which) { return p->grad_atom[which == 0 ? p->pairs[pid].x : p->pairs[pid].y]; };
   for (const Combo& c : p->combos)
     for (const int2& q : c.q) {
+      if (q.x < 0) continue;  // padding of the lane-kernel lists
       ++cnt[atom_of(q.x, 0)];
       ++cnt[atom_of(q.x, 1)];
       ++cnt[atom_of(q.y, 0)];
@@ -1205,22 +1292,29 @@ int grad_prepare(dft_plan p, const int32_t* shell_atom) {
     return s;
   };
   for (Combo& c : p->combos) {
-    std::vector<long long> sab(2 * c.q.size()), scd(2 * c.q.size());
-    std::vector<int2> sw(c.q.size());
-    for (size_t i = 0; i < c.q.size(); ++i) {
-      sab[2 * i] = slot(atom_of(c.q[i].x, 0));
-      sab[2 * i + 1] = slot(atom_of(c.q[i].x, 1));
+    // the combination's quartets without the padding
+    std::vector<int2> cq;
+    for (const int2& q : c.q)
+      if (q.x >= 0) cq.push_back(q);
+    std::vector<long long> sab(2 * cq.size()), scd(2 * cq.size());
+    std::vector<int2> sw(cq.size());
+    for (size_t i = 0; i < cq.size(); ++i) {
+      sab[2 * i] = slot(atom_of(cq[i].x, 0));
+      sab[2 * i + 1] = slot(atom_of(cq[i].x, 1));
     }
-    for (size_t i = 0; i < c.q.size(); ++i) {
-      scd[2 * i] = slot(atom_of(c.q[i].y, 0));
-      scd[2 * i + 1] = slot(atom_of(c.q[i].y, 1));
-      sw[i] = make_int2(c.q[i].y, c.q[i].x);
+    for (size_t i = 0; i < cq.size(); ++i) {
+      scd[2 * i] = slot(atom_of(cq[i].y, 0));
+      scd[2 * i + 1] = slot(atom_of(cq[i].y, 1));
+      sw[i] = make_int2(cq[i].y, cq[i].x);
     }
-    int2* dsw = nullptr;
+    int2 *dfw = nullptr, *dsw = nullptr;
     long long *dab = nullptr, *dcd = nullptr;
+    if ((rc = upload(&dfw, cq))) return rc;
     if ((rc = upload(&dsw, sw))) return rc;
     if ((rc = upload(&dab, sab))) return rc;
     if ((rc = upload(&dcd, scd))) return rc;
+    p->d_gq.push_back(dfw);
+    p->gq_n.push_back((long long)cq.size());
     p->d_gq_swap.push_back(dsw);
     p->d_gslots_ab.push_back(dab);
     p->d_gslots_cd.push_back(dcd);
@@ -1421,13 +1515,16 @@ int dft_plan_stats(dft_plan p, int64_t* counts, int ncombo, int* ncombo_out, int
   if (counts)
     for (int c = 0; c < std::min<int>(ncombo, (int)p->combos.size()); ++c) {
       const Combo& cb = p->combos[c];
-      counts[DFT_STATS_COLS * c] = (int64_t)cb.n64;
-      counts[DFT_STATS_COLS * c + 1] = (int64_t)cb.q.size() - cb.n64;
+      counts[DFT_STATS_COLS * c] = lane_kernel_combo(c) ? cb.r64 : (int64_t)cb.n64;
+      counts[DFT_STATS_COLS * c + 1] =
+          lane_kernel_combo(c) ? cb.r32a + cb.r32b : (int64_t)cb.q.size() - cb.n64;
       counts[DFT_STATS_COLS * c + 2] = cb.prim64;
       counts[DFT_STATS_COLS * c + 3] = cb.prim32;
       counts[DFT_STATS_COLS * c + 4] = cb.kprod64;
       counts[DFT_STATS_COLS * c + 5] = cb.kprod32;
-      counts[DFT_STATS_COLS * c + 6] = (int64_t)cb.q.size() - cb.n64 - cb.n32a;
+      counts[DFT_STATS_COLS * c + 6] =
+          lane_kernel_combo(c) ? (cb.n64 + cb.n32a < (long long)cb.q.size() ? cb.r32b : 0)
+                               : (int64_t)cb.q.size() - cb.n64 - cb.n32a;
       counts[DFT_STATS_COLS * c + 7] = cb.prim64w;
     }
   if (n_unscreened) *n_unscreened = p->n_unscreened;
@@ -1837,12 +1934,12 @@ int dft_grad(dft_plan p, const int32_t* shell_atom, const double* P, const doubl
   const int nbf = p->mol_nbf[0];
   for (size_t ci = 0; ci < p->combos.size(); ++ci) {
     const Combo& c = p->combos[ci];
-    if (c.q.empty()) continue;
+    const long long nq = p->gq_n[ci];
+    if (nq == 0) continue;
     int la, lb, lc, ld;
     pc_moments(c.pcb, la, lb);
     pc_moments(c.pck, lc, ld);
-    const long long nq = (long long)c.q.size();
-    hipError_t e = launch_eri_grad(la, lb, lc, ld, c.d_q, p->d_gslots_ab[ci], nq, p->d_gpairs, p->d_grec,
+    hipError_t e = launch_eri_grad(la, lb, lc, ld, p->d_gq[ci], p->d_gslots_ab[ci], nq, p->d_gpairs, p->d_grec,
                                    p->d_shell_ao, P, Pa, Pb, nbf, p->d_gstage, p->d_boys, s);
     if (e == hipSuccess)
       e = launch_eri_grad(lc, ld, la, lb, p->d_gq_swap[ci], p->d_gslots_cd[ci], nq, p->d_gpairs, p->d_grec,

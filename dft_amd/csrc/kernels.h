@@ -17,6 +17,10 @@ constexpr int LMAX = 2;
 constexpr int NPC = (LMAX + 1) * (LMAX + 2) / 2;   // pair classes: ss, ps, pp, ds, dp, dd
 constexpr int NCOMBO = NPC * (NPC + 1) / 2;        // (bra class >= ket class)
 
+// lanes of a wavefront, i.e. quartets per wave of a lane-kernel launch; the
+// lane-kernel quartet lists are padded to whole waves per molecule and part
+constexpr int WAVE_QUARTETS = 64;
+
 inline int pc_index(int la, int lb) { return la * (la + 1) / 2 + lb; }
 inline void pc_moments(int pc, int& la, int& lb) {
   la = 0;
@@ -79,13 +83,18 @@ hipError_t launch_interleave_records(const int4* map, int nrec, const double* r6
 
 // Fused ERI + J/K digest for one class combination (lane per quartet or per
 // quartet primitive chunk); slots holds 2 + 4*nk staging offsets (doubles)
-// per quartet and chunk; block is the workgroup size (64 ... 512, at most
-// lane_block_max), grid_cap the maximum number of workgroups.
+// per quartet and chunk (-1 where the lane's block is summed into a later
+// lane's); segmask one word per (wave, block kind) with bit l set where lane
+// l starts a run of lanes with the same target; block is the workgroup size
+// (64 ... 512, at most lane_block_max), grid_cap the maximum number of
+// workgroups. Quartets with a negative bra pair are padding (no work, no
+// stores).
 // largest lane-kernel workgroup of a class combination (0 if none)
 int lane_block_max(int combo);
 
 hipError_t launch_eri_jk(int combo, bool fp32, const int2* quartets, const long long* slots,
-                         const int4* pairs, const RecView& rv, const int4* shell_dm,
+                         const unsigned long long* segmask, const int4* pairs, const RecView& rv,
+                         const int4* shell_dm,
                          const double* DJ, const double* DK0, const double* DK1, int nk,
                          double* stage, const double* boys_tab, int block, int grid_cap, hipStream_t s);
 

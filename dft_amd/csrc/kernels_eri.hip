@@ -13,22 +13,55 @@ constexpr int jk_block_max(int la, int lb, int lc, int ld) {
   return ncart(la) * ncart(lb) * ncart(lc) * ncart(ld) <= 27 ? 512 : 256;
 }
 
+// Blocks of one kind from the 64 lanes of a wave: lanes whose block goes to
+// the same target form runs (mask bit l set where lane l starts a run); each
+// run is summed across its lanes by a segmented inclusive scan (six shift
+// steps, a fixed order, so the sums are reproducible) and its last lane
+// stores the sum at its slot (slot < 0: the lane stores nothing). A mask of
+// only singleton runs stores directly.
+template <int NE>
+__device__ __forceinline__ void store_runs(double (&v)[NE], unsigned long long mask, int lane, long long slot,
+                                           double* __restrict__ stage) {
+  if (mask != ~0ull) {
+    const unsigned long long upto = mask & (~0ull >> (63 - lane));
+    const int start = 63 - __builtin_clzll(upto);  // bit 0 is always set
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const bool take = lane - d >= start;
+#pragma unroll
+      for (int e = 0; e < NE; ++e) {
+        const double u = __shfl_up(v[e], d, 64);
+        if (take) v[e] += u;
+      }
+    }
+  }
+  if (slot >= 0) {
+#pragma unroll
+    for (int e = 0; e < NE; ++e) stage[slot + e] = v[e];
+  }
+}
+
 // Each lane evaluates one canonical quartet (bra pair >= ket pair in the
-// class-sorted pair order) and writes its contribution blocks
-//   Jab, Jcd, then per exchange density Kac, Kad, Kbc, Kbd
-// to stage + slots[(k * nchunk + ch) * ntot + q] (block-major, so that each
-// slot load of a wave is one contiguous run). With w = 2^-([a=b]+[c=d]+[ab=cd]):
+// class-sorted pair order) and forms its contribution blocks
+//   Jab, Jcd, then per exchange density Kac, Kad, Kbc, Kbd;
+// the blocks of a wave's lanes that share a target are summed in the wave
+// (store_runs) and stored once at stage + slots[(k * nchunk + ch) * ntot + q]
+// (block-major, so that each slot load of a wave is one contiguous run).
+// With w = 2^-([a=b]+[c=d]+[ab=cd]):
 //   Jab = 2w sum_cd I P_cd        Jcd = 2w sum_ab I P_ab
 //   Kac = w sum_bd I P_bd  Kad = w sum_bc I P_bc  Kbc = w sum_ad I P_ad  Kbd = w sum_ac I P_ac
 // and the half matrices Jh, Kh summed over all blocks give J = Jh + Jh^T and
 // K = Kh + Kh^T, i.e. the eight orientations and degeneracy weights of
 // fill_matrices (shelled.py:791-859). No atomics: the slots of one target
 // (ordered shell pair of a half matrix) are contiguous, assigned in quartet
-// order at plan time, and summed by a separate fixed-order pass.
+// order at plan time, and summed by a separate fixed-order pass. The quartet
+// lists are padded so that a wave's quartets (and hence its run sums) are
+// the same in every launch split.
 template <int LA, int LB, int LC, int LD, typename real, int NK>
 __global__ __launch_bounds__(jk_block_max(LA, LB, LC, LD)) void eri_jk_kernel(
-    const int2* __restrict__ quartets, const long long* __restrict__ slots, int first, int n, int ntot, int splb,
-    int splk, const int4* __restrict__ pairs, const real* __restrict__ recb, int sb, int fb,
+    const int2* __restrict__ quartets, const long long* __restrict__ slots,
+    const unsigned long long* __restrict__ segmask, int first, int n, int ntot, int splb, int splk,
+    const int4* __restrict__ pairs, const real* __restrict__ recb, int sb, int fb,
     const real* __restrict__ reck, const int4* __restrict__ shell_dm, const double* __restrict__ DJ,
     const double* __restrict__ DK0, const double* __restrict__ DK1, double* __restrict__ stage,
     const double* __restrict__ boys_gtab) {
@@ -45,15 +78,17 @@ __global__ __launch_bounds__(jk_block_max(LA, LB, LC, LD)) void eri_jk_kernel(
     const unsigned ch = v / (unsigned)n;
     const int q = first + (int)(v - ch * (unsigned)n);
     const int2 qd = quartets[q];
-    const int4 pb = pairs[qd.x];
+    // padding quartets (bra -1) take their ket as bra and evaluate nothing
+    const bool live = qd.x >= 0;
+    const int4 pb = pairs[live ? qd.x : qd.y];
     const int4 pk = pairs[qd.y];
     // this lane's primitive ranges [i0, i0 + nbp) of the bra pair and
     // [j0, j0 + nkp) of the ket pair
     const int cb = (int)ch / splk, ck = (int)ch - cb * splk;
     const int szb = (pb.w + splb - 1) / splb, szk = (pk.w + splk - 1) / splk;
     const int i0 = min(pb.w, cb * szb), j0 = min(pk.w, ck * szk);
-    const int nbp = min(pb.w, i0 + szb) - i0, nkp = min(pk.w, j0 + szk) - j0;
-    const unsigned ob = (unsigned)(qd.x - fb) + (unsigned)(i0 * S::RSB * sb);
+    const int nbp = live ? min(pb.w, i0 + szb) - i0 : 0, nkp = min(pk.w, j0 + szk) - j0;
+    const unsigned ob = (unsigned)((live ? qd.x : qd.y) - fb) + (unsigned)(i0 * S::RSB * sb);
     const unsigned ok = (unsigned)pk.z + (unsigned)(j0 * S::RSK);
     real I[S::NI];
     // When the whole wave shares the ket primitive range (the common case in
@@ -83,6 +118,12 @@ __global__ __launch_bounds__(jk_block_max(LA, LB, LC, LD)) void eri_jk_kernel(
     const double* DJm = DJ + mb;
     const long long sstr = (long long)ntot * (splb * splk);
     const long long* sl = slots + (long long)ch * ntot + q;
+    // the wave's index in the combination's (chunk, quartet) lane order, the
+    // same for its 64 lanes (n and first are multiples of the wave)
+    const int lane = threadIdx.x & 63;
+    const unsigned wid =
+        (unsigned)__builtin_amdgcn_readfirstlane((int)((ch * (unsigned)ntot + (unsigned)q) / WAVE_QUARTETS));
+    const unsigned long long* msk = segmask + (size_t)wid * NBLK;
     // Coulomb
     {
       double Pcd[S::NCD], Pab[S::NAB];
@@ -94,7 +135,7 @@ __global__ __launch_bounds__(jk_block_max(LA, LB, LC, LD)) void eri_jk_kernel(
       for (int a = 0; a < NA; ++a)
 #pragma unroll
         for (int b = 0; b < NB; ++b) Pab[a * NB + b] = DJm[(long long)(a0 + a) * nbf + b0 + b];
-      double jcd[S::NCD];
+      double jab[S::NAB], jcd[S::NCD];
 #pragma unroll
       for (int cd = 0; cd < S::NCD; ++cd) jcd[cd] = 0.0;
 #pragma unroll
@@ -106,10 +147,12 @@ __global__ __launch_bounds__(jk_block_max(LA, LB, LC, LD)) void eri_jk_kernel(
           s = fma(v, Pcd[cd], s);
           jcd[cd] = fma(v, Pab[ab], jcd[cd]);
         }
-        stage[sl[0] + ab] = 2.0 * w * s;
+        jab[ab] = 2.0 * w * s;
       }
 #pragma unroll
-      for (int cd = 0; cd < S::NCD; ++cd) stage[sl[sstr] + cd] = 2.0 * w * jcd[cd];
+      for (int cd = 0; cd < S::NCD; ++cd) jcd[cd] *= 2.0 * w;
+      store_runs(jab, msk[0], lane, sl[0], stage);
+      store_runs(jcd, msk[1], lane, sl[sstr], stage);
     }
     // Exchange, one density at a time. The densities are symmetric, and
     // P_bd is read as P_db (likewise P_bc, P_ad, P_ac): the lanes of a wave
@@ -156,15 +199,20 @@ __global__ __launch_bounds__(jk_block_max(LA, LB, LC, LD)) void eri_jk_kernel(
               kbc[b * NC + c] = fma(v, Pad[a * ND + d], kbc[b * NC + c]);
               kbd[b * ND + d] = fma(v, Pac[a * NC + c], kbd[b * ND + d]);
             }
+#pragma unroll
+      for (int x = 0; x < NA * NC; ++x) kac[x] *= w;
+#pragma unroll
+      for (int x = 0; x < NA * ND; ++x) kad[x] *= w;
+#pragma unroll
+      for (int x = 0; x < NB * NC; ++x) kbc[x] *= w;
+#pragma unroll
+      for (int x = 0; x < NB * ND; ++x) kbd[x] *= w;
       const long long* ks = sl + (2 + 4 * k) * sstr;
-#pragma unroll
-      for (int x = 0; x < NA * NC; ++x) stage[ks[0] + x] = w * kac[x];
-#pragma unroll
-      for (int x = 0; x < NA * ND; ++x) stage[ks[sstr] + x] = w * kad[x];
-#pragma unroll
-      for (int x = 0; x < NB * NC; ++x) stage[ks[2 * sstr] + x] = w * kbc[x];
-#pragma unroll
-      for (int x = 0; x < NB * ND; ++x) stage[ks[3 * sstr] + x] = w * kbd[x];
+      const unsigned long long* km = msk + 2 + 4 * k;
+      store_runs(kac, km[0], lane, ks[0], stage);
+      store_runs(kad, km[1], lane, ks[sstr], stage);
+      store_runs(kbc, km[2], lane, ks[2 * sstr], stage);
+      store_runs(kbd, km[3], lane, ks[3 * sstr], stage);
     }
   }
 }
@@ -200,20 +248,23 @@ static int grid_for(long long nq, int cap, int block = ERI_BLOCK) {
 }
 
 template <int LA, int LB, int LC, int LD, typename real>
-static hipError_t jk_one(const int2* q, const long long* sl, const int4* pairs, const RecView& rv,
+static hipError_t jk_one(const int2* q, const long long* sl, const unsigned long long* sm, const int4* pairs,
+                         const RecView& rv,
                          const int4* ao, const double* DJ, const double* DK0, const double* DK1, int nk,
                          double* stage, const double* bt, int blk, int cap, hipStream_t s) {
   if (rv.n <= 0) return hipSuccess;
+  // whole waves only: every wave runs its shuffles with all 64 lanes
+  if (rv.n % WAVE_QUARTETS || rv.first % WAVE_QUARTETS || rv.ntot % WAVE_QUARTETS) return hipErrorInvalidValue;
   const long long nq = (long long)rv.n * rv.splb * rv.splk;
   if (blk != 64 && blk != 128 && blk != 256 && blk != 512) return hipErrorInvalidValue;
   if (blk > jk_block_max(LA, LB, LC, LD)) return hipErrorInvalidValue;
   dim3 grid(grid_for(nq, cap, blk)), block(blk);
   if (nk == 1)
-    hipLaunchKernelGGL((eri_jk_kernel<LA, LB, LC, LD, real, 1>), grid, block, 0, s, q, sl, rv.first, rv.n,
+    hipLaunchKernelGGL((eri_jk_kernel<LA, LB, LC, LD, real, 1>), grid, block, 0, s, q, sl, sm, rv.first, rv.n,
                        rv.ntot, rv.splb, rv.splk, pairs, (const real*)rv.bra, rv.sb, rv.fb, (const real*)rv.ket,
                        ao, DJ, DK0, DK1, stage, bt);
   else
-    hipLaunchKernelGGL((eri_jk_kernel<LA, LB, LC, LD, real, 2>), grid, block, 0, s, q, sl, rv.first, rv.n,
+    hipLaunchKernelGGL((eri_jk_kernel<LA, LB, LC, LD, real, 2>), grid, block, 0, s, q, sl, sm, rv.first, rv.n,
                        rv.ntot, rv.splb, rv.splk, pairs, (const real*)rv.bra, rv.sb, rv.fb, (const real*)rv.ket,
                        ao, DJ, DK0, DK1, stage, bt);
   return hipGetLastError();
@@ -268,16 +319,16 @@ bool lane_kernel_combo(int combo) {
   }
 }
 
-hipError_t launch_eri_jk(int combo, bool fp32, const int2* q, const long long* sl,
+hipError_t launch_eri_jk(int combo, bool fp32, const int2* q, const long long* sl, const unsigned long long* sm,
                          const int4* pairs, const RecView& rv, const int4* ao, const double* DJ, const double* DK0,
                          const double* DK1, int nk, double* stage, const double* bt, int block, int cap,
                          hipStream_t s) {
   switch (combo) {
 #define X(id, la, lb, lc, ld)                                                                          \
   case id:                                                                                             \
-    return fp32 ? jk_one<la, lb, lc, ld, float>(q, sl, pairs, rv, ao, DJ, DK0, DK1, nk, stage, bt, \
-                                                 block, cap, s)                                        \
-                : jk_one<la, lb, lc, ld, double>(q, sl, pairs, rv, ao, DJ, DK0, DK1, nk, stage, bt, \
+    return fp32 ? jk_one<la, lb, lc, ld, float>(q, sl, sm, pairs, rv, ao, DJ, DK0, DK1, nk, stage, bt, \
+                                                 block, cap, s)                                            \
+                : jk_one<la, lb, lc, ld, double>(q, sl, sm, pairs, rv, ao, DJ, DK0, DK1, nk, stage, bt, \
                                                   block, cap, s);
     DFT_FOR_COMBOS(X)
 #undef X
