@@ -1987,6 +1987,17 @@ int dft_eri_prim(int64_t n_ab, const double* p_ab, const double* P_ab, const dou
   return DFT_OK;
 }
 
+int dft_syevj(int n, int batch, double* A, double* w, double* V, int init_v, void* stream) {
+  if (n <= 0 || batch < 0) return fail(DFT_EINVAL, "n must be positive and batch >= 0");
+  if (n > JAC_NMAX) return fail(DFT_EUNSUPPORTED, "dft_syevj: order " + std::to_string(n) + " above " +
+                                                      std::to_string(JAC_NMAX));
+  if (batch > 0 && (!A || !w || !V)) return fail(DFT_EINVAL, "NULL argument");
+  const long long nn = (long long)n * n;
+  hipError_t e = launch_syevj(n, batch, A, nn, w, n, V, nn, init_v ? 1 : 0, (hipStream_t)stream);
+  if (e != hipSuccess) return fail(DFT_EHIP, std::string("syevj: ") + hipGetErrorString(e));
+  return DFT_OK;
+}
+
 int dft_boys(const double* T, int n, int mmax, double* out, int fp32, void* stream) {
   if (n < 0 || (n > 0 && (!T || !out))) return fail(DFT_EINVAL, "bad arguments");
   if (mmax < 0 || mmax > 16) return fail(DFT_EUNSUPPORTED, "mmax must be in [0, 16]");

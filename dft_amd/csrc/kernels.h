@@ -165,6 +165,15 @@ hipError_t launch_eri_prim(long long nab, const double* pab, const double* Pab, 
                            const double* Eycd, const double* Ezcd, const int* nxcd, const int* nycd,
                            const int* nzcd, int e_stride, double* out, const double* boys_tab, hipStream_t s);
 
+// Batched symmetric eigensolver (two-sided cyclic Jacobi, one workgroup per
+// matrix, order n <= JAC_NMAX): A (batch x n x n, row-major, destroyed), w
+// (batch x n, ascending), V (batch x n x n, eigenvectors as columns; with
+// init_v = 0 it holds an orthonormal starting basis on entry and the result
+// is that basis times the rotations)
+constexpr int JAC_NMAX = 128;
+hipError_t launch_syevj(int n, int batch, double* A, long long sa, double* w, long long sw, double* V,
+                        long long sv, int init_v, hipStream_t s);
+
 // F_m(T) test hook
 hipError_t launch_boys(const double* T, int n, int mmax, double* out, int fp32,
                        const double* boys_tab, hipStream_t s);

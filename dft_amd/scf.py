@@ -5,8 +5,9 @@ Hamiltonian guess, symmetric orthogonalisation without cutoff, aufbau
 occupation, convergence on |dE| < 1e-8 and max|dP| < 1e-6). ``scf_diis`` is
 the damped + DIIS loop of ``shelled.py:1103-1292``. J/K come from
 ``Plan.jk`` (fused direct build on the GPU); orthogonalisation,
-diagonalisation, density build and energy stay FP64 on the GPU through
-torch.linalg. RHF (N_alpha == N_beta) uses one exchange matrix, since then
+diagonalisation, density build and energy stay FP64 on the GPU: the
+eigendecompositions go through the library's Jacobi kernel (``linalg.eigh``,
+started from the previous iteration's orbitals), the rest through torch. RHF (N_alpha == N_beta) uses one exchange matrix, since then
 P_a = P_b and K_a = K_b exactly.
 """
 from __future__ import annotations
@@ -18,6 +19,7 @@ import numpy as np
 import torch
 
 from .engine import Plan
+from .linalg import eigh
 
 
 def nuclear_repulsion(Z, centers) -> float:
@@ -110,7 +112,7 @@ def _setup(plan: Plan, Z, centers, A=None):
     H = T + V
     if A is not None:
         S, H = A @ S @ A.T, A @ H @ A.T
-    w, U = torch.linalg.eigh(S)
+    w, U = eigh(S)
     X = U @ torch.diag(w.rsqrt()) @ U.T
     return S, H, X, nuclear_repulsion(Z, centers)
 
@@ -137,10 +139,11 @@ def scf_plain(plan: Plan, Z, centers, N_a, N_b, mode="fp64", e_tol=1e-8, p_tol=1
     if not rhf and not plan.uhf:
         raise ValueError("N_a != N_b needs a plan built with uhf=True (two exchange matrices)")
     S, H, X, E_NN = _setup(plan, Z, centers, A)
-    _, Cp = torch.linalg.eigh(X.T @ H @ X)
+    _, Cp = eigh(X.T @ H @ X)
     C = X @ Cp
     P_a, P_b = init_UHF_density(C, C, N_a, N_b)
     fock = _Fock(plan, mode, rhf, A)
+    Qa = Qb = Cp  # orbitals of the last diagonalisation: the next one starts from them
     E_total = -1000.0
     count = 0
     energies, iter_times = [], []
@@ -154,12 +157,12 @@ def scf_plain(plan: Plan, Z, centers, N_a, N_b, mode="fp64", e_tol=1e-8, p_tol=1
             raise FloatingPointError(f"non-finite Fock matrix at iteration {count}")
         E_new = float(0.5 * torch.sum(P * H + P_a * Fa + P_b * Fb)) + E_NN
         dE = abs(E_new - E_total)
-        ea, Ca = torch.linalg.eigh(X.T @ Fa @ X)
+        ea, Qa = eigh(X.T @ Fa @ X, Qa)
         if rhf:
-            eb, Cb = ea, Ca
+            eb, Qb = ea, Qa
         else:
-            eb, Cb = torch.linalg.eigh(X.T @ Fb @ X)
-        Ca, Cb = X @ Ca, X @ Cb
+            eb, Qb = eigh(X.T @ Fb @ X, Qb)
+        Ca, Cb = X @ Qa, X @ Qb
         Pa_new, Pb_new = init_UHF_density(Ca, Cb, N_a, N_b)
         dPa, dPb = Pa_new - P_a, Pb_new - P_b
         dP = float(torch.maximum(dPa.abs().max(), dPb.abs().max()))
@@ -217,12 +220,14 @@ def scf_diis(plan: Plan, Z, centers, N_a, N_b, mode="fp64", rate=0.4, diis_thres
             if "E_total" in ck.files:
                 state["E_total"] = ck["E_total"]
         guess = (state["P_a"], state["P_b"])
+    Qa = Qb = None  # orbitals of the last diagonalisation: the next one starts from them
     if guess is not None:
         P_a, P_b = (torch.as_tensor(g, dtype=torch.float64, device=plan.device).clone() for g in guess)
     else:
-        _, Cp = torch.linalg.eigh(X.T @ H @ X)
+        _, Cp = eigh(X.T @ H @ X)
         C = X @ Cp
         P_a, P_b = init_UHF_density(C, C, N_a, N_b)
+        Qa = Qb = Cp
     rhf = N_a == N_b and not plan.uhf
     if not rhf and not plan.uhf:
         raise ValueError("N_a != N_b needs a plan built with uhf=True (two exchange matrices)")
@@ -258,6 +263,7 @@ def scf_diis(plan: Plan, Z, centers, N_a, N_b, mode="fp64", rate=0.4, diis_thres
                        ("Fb", "DIIS_Fock_b_history"), ("Ra", "DIIS_R_a_history"), ("Rb", "DIIS_R_b_history")):
             hist[k] = [torch.as_tensor(m, dtype=torch.float64, device=plan.device) for m in state[key][:keep]]
     energies, iter_times = [], []
+    gram = _gram(torch.stack(hist["Ra"])) + _gram(torch.stack(hist["Rb"])) if hist["Ra"] else None
     while True:
         t0 = time.perf_counter()
         P = P_a + P_b
@@ -270,19 +276,28 @@ def scf_diis(plan: Plan, Z, centers, N_a, N_b, mode="fp64", rate=0.4, diis_thres
                      ("Rb", Fb @ P_b @ S - S @ P_b @ Fb)):
             hist[k].insert(0, v)
             del hist[k][history:]
+        # Gram matrix of the residual history (newest first), kept from one
+        # iteration to the next: only the new residual's row is computed,
+        # as elementwise products and sums (a GEMM with K = n^2 and a 10 x 10
+        # output runs on a handful of workgroups)
+        d = sum((torch.stack(hist[k]) * hist[k][0]).sum((1, 2)) for k in ("Ra", "Rb"))
+        Bn = torch.empty((d.numel(), d.numel()), dtype=d.dtype, device=d.device)
+        if gram is not None and d.numel() > 1:
+            Bn[1:, 1:] = gram[:d.numel() - 1, :d.numel() - 1]
+        Bn[0, :] = d
+        Bn[:, 0] = d
+        gram = Bn
         if use_diis:
-            Ra = torch.stack(hist["Ra"])
-            Rb = torch.stack(hist["Rb"])
-            B = _gram(Ra) + _gram(Rb)
+            B = gram
             c = torch.linalg.solve(B, torch.ones(B.shape[0], dtype=B.dtype, device=B.device))
             c = c / c.sum()
             Fa = torch.einsum("i,iab->ab", c, torch.stack(hist["Fa"]))
             Fb = torch.einsum("i,iab->ab", c, torch.stack(hist["Fb"]))
         E_new = float(0.5 * torch.sum(P * H + P_a * Fa + P_b * Fb)) + E_NN
         dE = float("inf") if E_total is None else abs(E_new - E_total)
-        _, Ca = torch.linalg.eigh(X.T @ Fa @ X)
-        Cb = Ca if rhf else torch.linalg.eigh(X.T @ Fb @ X)[1]
-        Ca, Cb = X @ Ca, X @ Cb
+        _, Qa = eigh(X.T @ Fa @ X, Qa)
+        Qb = Qa if rhf else eigh(X.T @ Fb @ X, Qb)[1]
+        Ca, Cb = X @ Qa, X @ Qb
         Pa_new, Pb_new = UHF_density(Ca, Cb, N_a, N_b, P_a, P_b, rate)
         dP = float(torch.maximum((Pa_new - P_a).abs().max(), (Pb_new - P_b).abs().max()))
         energies.append(E_new)
@@ -315,8 +330,8 @@ def scf_diis(plan: Plan, Z, centers, N_a, N_b, mode="fp64", rate=0.4, diis_thres
             Fa = H + J - Ka
             Fb = H + J - Kb
             E_fin = float(0.5 * torch.sum(P * H + P_a * Fa + P_b * Fb)) + E_NN
-            ea, Ca = torch.linalg.eigh(X.T @ Fa @ X)
-            eb, Cb = (ea, Ca) if rhf else torch.linalg.eigh(X.T @ Fb @ X)
+            ea, Ca = eigh(X.T @ Fa @ X, Qa)
+            eb, Cb = (ea, Ca) if rhf else eigh(X.T @ Fb @ X, Qb)
             return SCFResult(E_fin, dE < e_tol and dP < p_tol, count, P_a, P_b, Fa, Fb, X @ Ca, X @ Cb, ea, eb,
                              fock.times, iter_times, energies)
         E_total = E_new

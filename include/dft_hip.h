@@ -321,6 +321,23 @@ int dft_eri_prim(int64_t n_ab, const double* p_ab, const double* P_ab, const dou
                  double* out, void* stream);
 
 /*
+ * Batched symmetric eigendecomposition for the SCF step: for each of the
+ * batch matrices A_b (n x n, row-major, symmetric; overwritten), the
+ * eigenvalues in ascending order in w[b*n ...] and the eigenvectors as the
+ * columns of V_b (n x n, row-major). With init_v = 1, V is set to the
+ * identity first; with init_v = 0, V must hold an orthonormal basis Q and A
+ * the matrix in that basis (Q^T F Q), and the eigenvectors of F are
+ * returned (Q times the rotations): a warm start from the previous
+ * iteration's orbitals. Two-sided cyclic Jacobi, one workgroup per matrix,
+ * n <= 128 (DFT_EUNSUPPORTED above). Only enqueues work (no allocation, no
+ * synchronisation), so it can be captured into a hipGraph. Device pointers.
+ *
+ * Replaces xp.linalg.eigh of the SCF loops (shelled.py:1210-1213,
+ * main.py:796-799) and of S (shelled.py:1082, main.py:754).
+ */
+int dft_syevj(int n, int batch, double* A, double* w, double* V, int init_v, void* stream);
+
+/*
  * Boys function test hook: out[i*(mmax+1)+m] = F_m(T[i]), m = 0..mmax, with
  * the tabulated evaluator the ERI kernels use. T and out are device arrays.
  * fp32 selects the FP32 recursion (FP64 seed). Replaces boys/taylor

@@ -52,3 +52,21 @@ def test_unsupported_angular_momentum_is_rejected():
                              0.0, 0.0, 0, ctypes.byref(ctypes.c_void_p()))
     assert rc == _lib.DFT_EUNSUPPORTED
     assert b"angular momentum" in lib.dft_last_error()
+
+
+def test_eigensolver_rejects_orders_above_its_limit():
+    from dft_amd import _lib
+    lib = _lib.load()
+    rc = lib.dft_syevj(129, 1, None, None, None, 1, None)
+    assert rc == _lib.DFT_EUNSUPPORTED
+    assert b"above 128" in lib.dft_last_error()
+
+
+def test_python_constants_match_the_header():
+    from dft_amd import _lib
+    text = open(os.path.join(ROOT, "include", "dft_hip.h")).read()
+    for name in ("DFT_PLAN_UHF", "DFT_PLAN_NOSPLIT", "DFT_PLAN_SERIAL", "DFT_PLAN_STATIC_WORKSPACE"):
+        m = re.search(r"#define\s+%s\s+(\d+)" % name, text)
+        assert m and int(m.group(1)) == getattr(_lib, name), name
+    m = re.search(r"#define\s+DFT_STATIC_SCRATCH_BYTES\s+\((\d+)LL << (\d+)\)", text)
+    assert m and int(m.group(1)) << int(m.group(2)) == _lib.DFT_STATIC_SCRATCH_BYTES
