@@ -168,7 +168,7 @@ def main():
     from dft_amd.engine import Plan, profile_read, profile_start, tune
     from dft_amd.flops import build_flops, combo_flops
     from dft_amd.geometry import water_cluster
-    from dft_amd.scf import scf_diis, scf_plain
+    from dft_amd.scf import scf_diis, scf_diis_graph, scf_plain
 
     tau = 1e-10
     Z, xyz = water_cluster((4, 2, 2), 3.1, seed=0)
@@ -337,6 +337,8 @@ def main():
     acc = Plan(shells, Z, xyz, schwarz_tau=tau, fp32_tau=1e-4)
     scf64 = scf_diis(acc, Z, xyz, nocc, nocc, mode="fp64")
     scf32 = scf_diis(acc, Z, xyz, nocc, nocc, mode="mixed")
+    # the same loop with each iteration one replay of a captured hipGraph
+    scfg = scf_diis_graph(acc, Z, xyz, nocc, mode="fp64")
     del acc
     f64_part, f32_part = build_flops(per)
     log(f"mixed: {ms_mixed:.3f} ms/build; E64 {scf64.energy:.10f} E_mixed {scf32.energy:.10f}")
@@ -356,6 +358,11 @@ def main():
         "scf_iterations_fp64": scf64.iterations,
         "scf_converged": [bool(scf64.converged), bool(scf32.converged)],
         "scf_s_per_iter_wall": float(np.median(scf64.iter_times[1:])) if len(scf64.iter_times) > 1 else None,
+        "scf_graph": {"s_per_iter_wall": float(np.median(scfg.iter_times[1:])), "iterations": scfg.iterations,
+                      "energy": scfg.energy, "abs_dE_vs_eager_Eh": abs(scfg.energy - scf64.energy),
+                      "converged": bool(scfg.converged),
+                      "note": "damped + DIIS loop, one hipGraph replay per iteration (J/K, DIIS stage, Jacobi "
+                              "eigensolver, density, control on the device; one 32-double read per iteration)"},
     })
     if not args.no_cpu:
         log("cpu baseline ...")

@@ -1987,14 +1987,34 @@ int dft_eri_prim(int64_t n_ab, const double* p_ab, const double* P_ab, const dou
   return DFT_OK;
 }
 
-int dft_syevj(int n, int batch, double* A, double* w, double* V, int init_v, void* stream) {
+int dft_syevj(int n, int batch, double* A, double* w, double* V, int init_v, int32_t* sweeps, void* stream) {
   if (n <= 0 || batch < 0) return fail(DFT_EINVAL, "n must be positive and batch >= 0");
   if (n > JAC_NMAX) return fail(DFT_EUNSUPPORTED, "dft_syevj: order " + std::to_string(n) + " above " +
                                                       std::to_string(JAC_NMAX));
   if (batch > 0 && (!A || !w || !V)) return fail(DFT_EINVAL, "NULL argument");
   const long long nn = (long long)n * n;
-  hipError_t e = launch_syevj(n, batch, A, nn, w, n, V, nn, init_v ? 1 : 0, (hipStream_t)stream);
+  hipError_t e = launch_syevj(n, batch, A, nn, w, n, V, nn, init_v ? 1 : 0, sweeps, (hipStream_t)stream);
   if (e != hipSuccess) return fail(DFT_EHIP, std::string("syevj: ") + hipGetErrorString(e));
+  return DFT_OK;
+}
+
+int dft_scf_diis_stage(int n, int history, const double* F, const double* R, double* hist_F, double* hist_R,
+                       double* G, const double* P, const double* H, const double* Pa, double* state, double* Fu,
+                       void* stream) {
+  static_assert(DFT_SCF_HMAX == SCF_HMAX && DFT_SCF_ST_SLOT == SCF_ST_SLOT, "state layout");
+  if (n <= 0 || n > JAC_NMAX) return fail(DFT_EUNSUPPORTED, "dft_scf_diis_stage: order must be in [1, 128]");
+  if (history < 1 || history > DFT_SCF_HMAX) return fail(DFT_EINVAL, "history must be in [1, 16]");
+  if (!F || !R || !hist_F || !hist_R || !G || !P || !H || !Pa || !state || !Fu) return fail(DFT_EINVAL, "NULL argument");
+  hipError_t e = launch_scf_diis_stage(n, history, F, R, hist_F, hist_R, G, P, H, Pa, state, Fu, (hipStream_t)stream);
+  if (e != hipSuccess) return fail(DFT_EHIP, std::string("scf diis stage: ") + hipGetErrorString(e));
+  return DFT_OK;
+}
+
+int dft_scf_damp_control(int n, double* Pa, const double* Pauf, double* P, double* state, void* stream) {
+  if (n <= 0 || n > JAC_NMAX) return fail(DFT_EUNSUPPORTED, "dft_scf_damp_control: order must be in [1, 128]");
+  if (!Pa || !Pauf || !P || !state) return fail(DFT_EINVAL, "NULL argument");
+  hipError_t e = launch_scf_damp_control(n, Pa, Pauf, P, state, (hipStream_t)stream);
+  if (e != hipSuccess) return fail(DFT_EHIP, std::string("scf damp/control: ") + hipGetErrorString(e));
   return DFT_OK;
 }
 

@@ -172,7 +172,20 @@ hipError_t launch_eri_prim(long long nab, const double* pab, const double* Pab, 
 // is that basis times the rotations)
 constexpr int JAC_NMAX = 128;
 hipError_t launch_syevj(int n, int batch, double* A, long long sa, double* w, long long sw, double* V,
-                        long long sv, int init_v, hipStream_t s);
+                        long long sv, int init_v, int* sweeps, hipStream_t s);
+
+// device SCF iteration (state layout: the DFT_SCF_ST_* indices of dft_hip.h)
+constexpr int SCF_HMAX = 16;
+enum ScfState {
+  SCF_ST_ESUM = 0, SCF_ST_ENN = 1, SCF_ST_E = 2, SCF_ST_HAS_E = 3, SCF_ST_DE = 4, SCF_ST_DP = 5,
+  SCF_ST_PREV_DE = 6, SCF_ST_PREV_DP = 7, SCF_ST_RATE = 8, SCF_ST_STRIKES = 9, SCF_ST_USE_DIIS = 10,
+  SCF_ST_COUNT = 11, SCF_ST_CONV = 12, SCF_ST_ETOL = 13, SCF_ST_PTOL = 14, SCF_ST_DIIS_TH = 15,
+  SCF_ST_ENEW = 16, SCF_ST_SLOT = 17
+};
+hipError_t launch_scf_diis_stage(int n, int h, const double* F, const double* R, double* histF, double* histR,
+                                 double* G, const double* P, const double* H, const double* Pa, double* st,
+                                 double* Fu, hipStream_t s);
+hipError_t launch_scf_damp_control(int n, double* Pa, const double* Pauf, double* P, double* st, hipStream_t s);
 
 // F_m(T) test hook
 hipError_t launch_boys(const double* T, int n, int mmax, double* out, int fp32,

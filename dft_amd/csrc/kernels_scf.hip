@@ -6,7 +6,8 @@
 // (xp.linalg.eigh of X^T F X, shelled.py:1210-1213, main.py:796-799; of S,
 // shelled.py:1082 / main.py:754) for matrices up to JAC_NMAX. The matrix
 // lives in LDS; the eigenvectors in LDS as well when both fit, else in the
-// caller's output buffer (global memory, L2-resident at these sizes).
+// caller's output buffer (global memory, L2-resident at these sizes). The
+// matrix is stored as its packed upper triangle, so both fit up to n = 116.
 //
 // Method: Golub & Van Loan, Matrix Computations (4th ed.), section 8.5: the
 // symmetric 2x2 Schur decomposition (tau = (a_qq - a_pp) / 2 a_pq,
@@ -14,10 +15,15 @@
 // applied to m/2 disjoint pairs at once in the round-robin (parallel) order,
 // m - 1 rounds per sweep; with J = [[c, s], [-s, c]] on (p, q) the update
 // A <- J^T A J touches every 2x2 block {p_k, q_k} x {p_l, q_l} once per
-// round. Rotations with |a_pq| below 2^-52 ||A||_F are skipped; the sweeps
-// stop after one without any rotation (at most JAC_MAX_SWEEPS). Eigenvalues
+// round. Rotations with |a_pq| below m 2^-52 ||A||_F are skipped (rounding
+// keeps the off-diagonal entries of a converged matrix near 2^-52 ||A|| times
+// the square root of the rotations that touched them, so a bound without the
+// factor m can be out of reach); the sweeps stop after one without any
+// rotation (at most JAC_MAX_SWEEPS). Eigenvalues
 // are returned in ascending order, eigenvectors as the columns of V.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "kernels.h"
 
@@ -27,14 +33,29 @@ constexpr int JAC_THREADS = 512;
 constexpr int JAC_MAX_SWEEPS = 40;
 // LDS of one workgroup, and the matrix order up to which A and V share it
 constexpr int JAC_LDS_BYTES = 160 * 1024;
+// per-thread share of a round's blocks (h <= 64 pairs): ceil(64 * 65 / 2 / 512)
+constexpr int JAC_BLK_PER_THREAD = (JAC_NMAX / 2 * (JAC_NMAX / 2 + 1) / 2 + JAC_THREADS - 1) / JAC_THREADS;
+constexpr int JAC_V_GROUP = 4;
 
 // partner positions of round r of the round-robin tournament over m players
 __device__ __forceinline__ int rr_pos(int i, int r, int m) { return i == 0 ? 0 : 1 + (i - 1 + r) % (m - 1); }
 
-__global__ __launch_bounds__(JAC_THREADS) void jacobi_eigh_kernel(int n, double* __restrict__ A, long long sa,
+// Upper triangle of the symmetric m x m matrix, packed by rows.
+__device__ __forceinline__ int tri(int r, int c, int m) {
+  if (r > c) {
+    const int t = r;
+    r = c;
+    c = t;
+  }
+  return r * m - (r * (r - 1)) / 2 + (c - r);
+}
+
+// VL: eigenvectors in LDS (with the packed matrix) or in the output buffer.
+template <bool VL>
+__global__ __launch_bounds__(JAC_THREADS) void jacobi_eigh_kernel(int n, const double* __restrict__ A, long long sa,
                                                                   double* __restrict__ w, long long sw,
-                                                                  double* __restrict__ V, long long sv, int init_v,
-                                                                  int v_in_lds) {
+                                                                  double* __restrict__ V, long long sv,
+                                                                  int init_v, int* __restrict__ sweeps) {
   extern __shared__ double lds[];
   __shared__ double cs_c[JAC_NMAX / 2], cs_s[JAC_NMAX / 2];
   __shared__ int cs_p[JAC_NMAX / 2], cs_q[JAC_NMAX / 2];
@@ -44,32 +65,54 @@ __global__ __launch_bounds__(JAC_THREADS) void jacobi_eigh_kernel(int n, double*
   const int tid = threadIdx.x, nt = blockDim.x;
   const int m = n + (n & 1);  // an odd order gets a decoupled zero row and column
   const int h = m / 2;
-  double* a = lds;                                  // m x m
-  double* vl = v_in_lds ? lds + (size_t)m * m : nullptr;  // n x n when it fits
-  double* Ab = A + (size_t)blockIdx.x * sa;
-  double* Vb = V + (size_t)blockIdx.x * sv;
-  double* v = v_in_lds ? vl : Vb;
+  const int ntri = m * (m + 1) / 2;
+  double* a = lds;  // packed upper triangle, ntri
+  const double* Ab = A + (size_t)blockIdx.x * sa;
+  double* __restrict__ Vb = V + (size_t)blockIdx.x * sv;
+  double* vl = lds + ntri;  // n x n (VL only)
   for (int i = tid; i < m * m; i += nt) {
     const int r = i / m, c = i - r * m;
-    a[i] = (r < n && c < n) ? Ab[(size_t)r * n + c] : 0.0;
+    if (r <= c) a[tri(r, c, m)] = (r < n && c < n) ? Ab[(size_t)r * n + c] : 0.0;
   }
   for (int i = tid; i < n * n; i += nt) {
     const int r = i / n, c = i - r * n;
-    v[i] = init_v ? (r == c ? 1.0 : 0.0) : Vb[i];
+    const double x = init_v ? (r == c ? 1.0 : 0.0) : Vb[i];
+    if (VL) vl[i] = x;
+    else Vb[i] = x;
   }
   if (tid == 0) rotated = 0;
   __syncthreads();
   // ||A||_F by a fixed-order reduction: per-thread partial, per-wave
-  // butterfly, then the waves in order
+  // butterfly, then the waves in order (off-diagonal entries count twice)
   double f = 0.0;
-  for (int i = tid; i < m * m; i += nt) f = fma(a[i], a[i], f);
+  for (int i = tid; i < m * m; i += nt) {
+    const int r = i / m, c = i - r * m;
+    const double x = a[tri(r, c, m)];
+    f = fma(x, x, f);
+  }
   for (int off = 32; off > 0; off >>= 1) f += __shfl_xor(f, off, 64);
   if ((tid & 63) == 0) red[tid >> 6] = f;
   __syncthreads();
   double fro = 0.0;
   for (int k = 0; k < (nt >> 6); ++k) fro += red[k];
-  const double thresh = 2.220446049250313e-16 * sqrt(fro);
-  for (int sweep = 0; sweep < JAC_MAX_SWEEPS; ++sweep) {
+  const double thresh = m * 2.220446049250313e-16 * sqrt(fro);
+  // this thread's fixed share of the round's work: blocks (k, l), k <= l, of
+  // the h x h block grid, and (row, pair) updates of the eigenvectors
+  const int nblk = h * (h + 1) / 2;
+  // (packed: block k | l << 8)
+  int bkl[JAC_BLK_PER_THREAD], nb = 0;
+  for (int idx = tid; idx < nblk; idx += nt) {
+    int k = 0, rem = idx;
+    while (rem >= h - k) {
+      rem -= h - k;
+      ++k;
+    }
+    bkl[nb++] = k | (k + rem) << 8;
+  }
+  const int vng = nt / h;  // row groups of the eigenvector update
+  const int vk = tid < vng * h ? tid % h : -1, vg = tid / h;
+  int sweep = 0;
+  for (; sweep < JAC_MAX_SWEEPS; ++sweep) {
     for (int r = 0; r < m - 1; ++r) {
       for (int k = tid; k < h; k += nt) {
         int p = rr_pos(k, r, m), q = rr_pos(m - 1 - k, r, m);
@@ -78,10 +121,10 @@ __global__ __launch_bounds__(JAC_THREADS) void jacobi_eigh_kernel(int n, double*
           p = q;
           q = t;
         }
-        const double apq = a[p * m + q];
+        const double apq = a[tri(p, q, m)];
         double c = 1.0, s = 0.0;
         if (fabs(apq) > thresh) {
-          const double tau = (a[q * m + q] - a[p * m + p]) / (2.0 * apq);
+          const double tau = (a[tri(q, q, m)] - a[tri(p, p, m)]) / (2.0 * apq);
           const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
           c = 1.0 / sqrt(1.0 + t * t);
           s = t * c;
@@ -93,37 +136,92 @@ __global__ __launch_bounds__(JAC_THREADS) void jacobi_eigh_kernel(int n, double*
         cs_q[k] = q;
       }
       __syncthreads();
-      for (int idx = tid; idx < h * h; idx += nt) {
-        const int k = idx / h, l = idx - k * h;
-        const double sk = cs_s[k], sl = cs_s[l];
-        if (sk == 0.0 && sl == 0.0) continue;
-        const double ck = cs_c[k], cl = cs_c[l];
-        const int pk = cs_p[k], qk = cs_q[k], pl = cs_p[l], ql = cs_q[l];
-        const double xpp = a[pk * m + pl], xpq = a[pk * m + ql], xqp = a[qk * m + pl], xqq = a[qk * m + ql];
-        // rows (J^T from the left), then columns (J from the right)
-        const double tpp = ck * xpp - sk * xqp, tpq = ck * xpq - sk * xqq;
-        const double tqp = sk * xpp + ck * xqp, tqq = sk * xpq + ck * xqq;
-        double ypp = cl * tpp - sl * tpq, ypq = sl * tpp + cl * tpq;
-        double yqp = cl * tqp - sl * tqq, yqq = sl * tqp + cl * tqq;
-        if (k == l) {
-          // the block the rotation diagonalises: its off-diagonal is zero
-          ypq = 0.0;
-          yqp = 0.0;
+      // blocks {p_k, q_k} x {p_l, q_l} with k <= l (the others are their
+      // transposes). The entries a thread touches in a round are its own, so
+      // all of them are loaded first and stored after the arithmetic (the
+      // compiler cannot see that the stores do not alias later loads).
+      {
+        double x[JAC_BLK_PER_THREAD][4];
+        int ix[JAC_BLK_PER_THREAD][4];
+#pragma unroll
+        for (int j = 0; j < JAC_BLK_PER_THREAD; ++j) {
+          if (j < nb) {
+            const int k = bkl[j] & 255, l = bkl[j] >> 8;
+            const int pk = cs_p[k], qk = cs_q[k], pl = cs_p[l], ql = cs_q[l];
+            if (k == l) {
+              ix[j][0] = tri(pk, pk, m);
+              ix[j][1] = tri(pk, qk, m);
+              ix[j][2] = tri(qk, qk, m);
+              ix[j][3] = ix[j][1];
+            } else {
+              ix[j][0] = tri(pk, pl, m);
+              ix[j][1] = tri(pk, ql, m);
+              ix[j][2] = tri(qk, pl, m);
+              ix[j][3] = tri(qk, ql, m);
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) x[j][e] = a[ix[j][e]];
+          }
         }
-        a[pk * m + pl] = ypp;
-        a[pk * m + ql] = ypq;
-        a[qk * m + pl] = yqp;
-        a[qk * m + ql] = yqq;
+#pragma unroll
+        for (int j = 0; j < JAC_BLK_PER_THREAD; ++j) {
+          if (j < nb) {
+            const int k = bkl[j] & 255, l = bkl[j] >> 8;
+            const double sk = cs_s[k], sl = cs_s[l];
+            if (sk == 0.0 && sl == 0.0) continue;
+            const double ck = cs_c[k], cl = cs_c[l];
+            if (k == l) {
+              // the block the rotation diagonalises (t = s / c): a_pp - t a_pq, a_qq + t a_pq, 0
+              const double t = sk / ck, apq = x[j][1];
+              a[ix[j][0]] = x[j][0] - t * apq;
+              a[ix[j][2]] = x[j][2] + t * apq;
+              a[ix[j][1]] = 0.0;
+              continue;
+            }
+            // rows (J^T from the left), then columns (J from the right)
+            const double tpp = ck * x[j][0] - sk * x[j][2], tpq = ck * x[j][1] - sk * x[j][3];
+            const double tqp = sk * x[j][0] + ck * x[j][2], tqq = sk * x[j][1] + ck * x[j][3];
+            a[ix[j][0]] = cl * tpp - sl * tpq;
+            a[ix[j][1]] = sl * tpp + cl * tpq;
+            a[ix[j][2]] = cl * tqp - sl * tqq;
+            a[ix[j][3]] = sl * tqp + cl * tqq;
+          }
+        }
       }
-      for (int idx = tid; idx < n * h; idx += nt) {
-        const int i = idx / h, k = idx - i * h;
-        const double s = cs_s[k];
-        const int p = cs_p[k], q = cs_q[k];
-        if (s == 0.0 || q >= n) continue;
-        const double c = cs_c[k];
-        const double vp = v[(size_t)i * n + p], vq = v[(size_t)i * n + q];
-        v[(size_t)i * n + p] = c * vp - s * vq;
-        v[(size_t)i * n + q] = s * vp + c * vq;
+      // eigenvector columns p_k, q_k of rows g, g + ng, ... for this
+      // thread's fixed pair k and row group g (its rotation read once), in
+      // groups of JAC_V_GROUP rows loaded before any is stored
+      if (vk >= 0) {
+        const double sr = cs_s[vk];
+        const int p = cs_p[vk], q = cs_q[vk];
+        if (sr != 0.0 && q < n) {
+          const double c = cs_c[vk];
+          for (int i0 = vg; i0 < n; i0 += JAC_V_GROUP * vng) {
+            double xp[JAC_V_GROUP], xq[JAC_V_GROUP];
+#pragma unroll
+            for (int u = 0; u < JAC_V_GROUP; ++u) {
+              const int i = i0 + u * vng;
+              if (i < n) {
+                xp[u] = VL ? vl[i * n + p] : Vb[(size_t)i * n + p];
+                xq[u] = VL ? vl[i * n + q] : Vb[(size_t)i * n + q];
+              }
+            }
+#pragma unroll
+            for (int u = 0; u < JAC_V_GROUP; ++u) {
+              const int i = i0 + u * vng;
+              if (i < n) {
+                const double np_ = c * xp[u] - sr * xq[u], nq_ = sr * xp[u] + c * xq[u];
+                if (VL) {
+                  vl[i * n + p] = np_;
+                  vl[i * n + q] = nq_;
+                } else {
+                  Vb[(size_t)i * n + p] = np_;
+                  Vb[(size_t)i * n + q] = nq_;
+                }
+              }
+            }
+          }
+        }
       }
       __syncthreads();
     }
@@ -133,15 +231,18 @@ __global__ __launch_bounds__(JAC_THREADS) void jacobi_eigh_kernel(int n, double*
     __syncthreads();
     if (!any) break;
   }
+  if (sweeps && tid == 0) sweeps[blockIdx.x] = sweep + 1;
   // ascending order: rank of each eigenvalue (ties by index)
-  for (int i = tid; i < n; i += nt) diag[i] = a[i * m + i];
+  for (int i = tid; i < n; i += nt) diag[i] = a[tri(i, i, m)];
   __syncthreads();
   double* wb = w + (size_t)blockIdx.x * sw;
-  if (!v_in_lds) {
-    // stage V through the (now free) matrix area to permute its columns
-    for (int i = tid; i < n * n; i += nt) a[i] = Vb[i];
+  const double* v = vl;
+  if (!VL) {
+    // stage V through LDS (the packed matrix is no longer needed) to permute its columns
+    double* st = lds;
+    for (int i = tid; i < n * n; i += nt) st[i] = Vb[i];
     __syncthreads();
-    v = a;
+    v = st;
   }
   for (int i = tid; i < n; i += nt) {
     const double di = diag[i];
@@ -151,28 +252,236 @@ __global__ __launch_bounds__(JAC_THREADS) void jacobi_eigh_kernel(int n, double*
       rank += (dj < di || (dj == di && j < i)) ? 1 : 0;
     }
     wb[rank] = di;
-    for (int r = 0; r < n; ++r) Vb[(size_t)r * n + rank] = v[(size_t)r * n + i];
+    for (int r = 0; r < n; ++r) Vb[(size_t)r * n + rank] = v[r * n + i];
   }
 }
 
 hipError_t launch_syevj(int n, int batch, double* A, long long sa, double* w, long long sw, double* V,
-                        long long sv, int init_v, hipStream_t s) {
+                        long long sv, int init_v, int* sweeps, hipStream_t s) {
   if (batch <= 0 || n <= 0) return hipSuccess;
   if (n > JAC_NMAX) return hipErrorInvalidValue;
   const int m = n + (n & 1);
-  const size_t a_bytes = (size_t)m * m * sizeof(double), v_bytes = (size_t)n * n * sizeof(double);
-  // static LDS of the kernel (rotation tables, diagonal, reduction) stays below 4 KB
-  const bool v_in_lds = a_bytes + v_bytes + 4096 <= (size_t)JAC_LDS_BYTES;
-  const size_t dyn = v_in_lds ? a_bytes + v_bytes : a_bytes;
+  const size_t a_bytes = (size_t)m * (m + 1) / 2 * sizeof(double), v_bytes = (size_t)n * n * sizeof(double);
+  // the kernel's static LDS (rotation tables, diagonal, reduction) stays below 4 KB
+  const bool vl = a_bytes + v_bytes + 4096 <= (size_t)JAC_LDS_BYTES;
+  // the staging of V for the final column permutation reuses the matrix area
+  const size_t dyn = vl ? a_bytes + v_bytes : std::max(a_bytes, v_bytes);
   static bool attr_set = false;
   if (!attr_set) {
-    hipError_t e = hipFuncSetAttribute((const void*)jacobi_eigh_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       JAC_LDS_BYTES - 4096);
-    if (e != hipSuccess) return e;
+    for (const void* k : {(const void*)jacobi_eigh_kernel<true>, (const void*)jacobi_eigh_kernel<false>}) {
+      hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, JAC_LDS_BYTES - 4096);
+      if (e != hipSuccess) return e;
+    }
     attr_set = true;
   }
-  hipLaunchKernelGGL(jacobi_eigh_kernel, dim3(batch), dim3(JAC_THREADS), dyn, s, n, A, sa, w, sw, V, sv, init_v,
-                     v_in_lds ? 1 : 0);
+  if (vl)
+    hipLaunchKernelGGL(jacobi_eigh_kernel<true>, dim3(batch), dim3(JAC_THREADS), dyn, s, n, A, sa, w, sw, V, sv,
+                       init_v, sweeps);
+  else
+    hipLaunchKernelGGL(jacobi_eigh_kernel<false>, dim3(batch), dim3(JAC_THREADS), dyn, s, n, A, sa, w, sw, V, sv,
+                       init_v, sweeps);
+  return hipGetLastError();
+}
+
+}  // namespace dft
+
+namespace dft {
+
+// ---------------------------------------------------------------------------
+// One damped + DIIS SCF iteration on the device (closed shell), as the loop of
+// shelled.py:1164-1260 with P_a = P_b: the two kernels below hold the
+// loop's control state in device memory (st[], SCF_ST_* slots), so an
+// iteration needs no host decision and can be replayed as a hipGraph; the
+// host reads st[] once per iteration to stop the loop.
+
+constexpr int SCF_THREADS = 1024;
+
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  // fixed order: wave butterflies, then the waves in order
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int k = 0; k < (int)(blockDim.x >> 6); ++k) s += red[k];
+  return s;
+}
+
+__device__ __forceinline__ double block_max(double v, double* red) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int k = 0; k < (int)(blockDim.x >> 6); ++k) s = fmax(s, red[k]);
+  return s;
+}
+
+// History (ring of h slots, this iteration's slot = count mod h): store F and
+// its residual R = F P S - S P F, update the Gram matrix G[s][t] = <R_s, R_t>
+// of the new slot's row and column, solve the DIIS equations on the used
+// slots (B c = 1 without a Lagrange row, c normalised to sum 1; unused slots
+// get c = 0), and form the Fock matrix of this iteration Fu (the
+// extrapolation once DIIS is on, F before), and E_sum = sum P.H + 2 sum Pa.Fu.
+__global__ __launch_bounds__(SCF_THREADS) void scf_diis_stage_kernel(int n, int h, const double* __restrict__ F,
+                                                                     const double* __restrict__ R,
+                                                                     double* __restrict__ histF,
+                                                                     double* __restrict__ histR,
+                                                                     double* __restrict__ G,
+                                                                     const double* __restrict__ P,
+                                                                     const double* __restrict__ H,
+                                                                     const double* __restrict__ Pa,
+                                                                     double* __restrict__ st, double* __restrict__ Fu) {
+  __shared__ double red[SCF_THREADS / 64];
+  __shared__ double c[SCF_HMAX];
+  __shared__ double gauss[SCF_HMAX][SCF_HMAX + 1];
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const long long nn = (long long)n * n;
+  const int count = (int)st[SCF_ST_COUNT];
+  const int slot = count % h;
+  const int used = min(count + 1, h);
+  for (long long i = tid; i < nn; i += nt) {
+    histF[slot * nn + i] = F[i];
+    histR[slot * nn + i] = R[i];
+  }
+  __syncthreads();
+  // the new residual's dot products with every slot in one pass over R
+  {
+    double d[SCF_HMAX];
+#pragma unroll
+    for (int t = 0; t < SCF_HMAX; ++t) d[t] = 0.0;
+    for (long long i = tid; i < nn; i += nt) {
+      const double r = R[i];
+#pragma unroll
+      for (int t = 0; t < SCF_HMAX; ++t)
+        if (t < h) d[t] = fma(r, histR[t * nn + i], d[t]);
+    }
+    for (int t = 0; t < h; ++t) {
+      const double x = block_sum(d[t], red);
+      if (tid == 0) {
+        G[slot * h + t] = x;
+        G[t * h + slot] = x;
+      }
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    // Gaussian elimination with partial pivoting on the padded system (in
+    // LDS: run-time indexed, it would live in scratch memory otherwise)
+    double(*M)[SCF_HMAX + 1] = gauss;
+    for (int i = 0; i < h; ++i) {
+      for (int j = 0; j < h; ++j) M[i][j] = (i < used && j < used) ? G[i * h + j] : (i == j ? 1.0 : 0.0);
+      M[i][h] = i < used ? 1.0 : 0.0;
+    }
+    for (int k = 0; k < h; ++k) {
+      int piv = k;
+      for (int i = k + 1; i < h; ++i)
+        if (fabs(M[i][k]) > fabs(M[piv][k])) piv = i;
+      if (piv != k)
+        for (int j = k; j <= h; ++j) {
+          const double x = M[k][j];
+          M[k][j] = M[piv][j];
+          M[piv][j] = x;
+        }
+      const double d = M[k][k];
+      for (int i = k + 1; i < h; ++i) {
+        const double f = M[i][k] / d;
+        for (int j = k; j <= h; ++j) M[i][j] -= f * M[k][j];
+      }
+    }
+    double sum = 0.0;
+    for (int k = h - 1; k >= 0; --k) {
+      double x = M[k][h];
+      for (int j = k + 1; j < h; ++j) x -= M[k][j] * c[j];
+      c[k] = x / M[k][k];
+    }
+    for (int k = 0; k < h; ++k) sum += c[k];
+    for (int k = 0; k < h; ++k) c[k] /= sum;
+    st[SCF_ST_SLOT] = slot;
+  }
+  __syncthreads();
+  const bool diis = st[SCF_ST_USE_DIIS] != 0.0;
+  for (long long i = tid; i < nn; i += nt) {
+    double x = F[i];
+    if (diis) {
+      x = 0.0;
+      for (int t = 0; t < used; ++t) x = fma(c[t], histF[t * nn + i], x);
+    }
+    Fu[i] = x;
+  }
+  double e = 0.0;
+  for (long long i = tid; i < nn; i += nt) e = fma(P[i], H[i], e) + 2.0 * Pa[i] * (diis ? Fu[i] : F[i]);
+  e = block_sum(e, red);
+  if (tid == 0) st[SCF_ST_ESUM] = e;
+}
+
+// Energy and convergence of the iteration, the damped density
+// Pa' = rate Pauf + (1 - rate) Pa with dP = max |Pa' - Pa|, the strike and
+// DIIS-switch rules (shelled.py:1250-1257) and the next iteration's
+// densities (Pa <- Pa', P <- 2 Pa'), unless this iteration converged: then
+// Pa keeps the density the final rebuild uses (shelled.py:1261-1285).
+__global__ __launch_bounds__(SCF_THREADS) void scf_damp_control_kernel(int n, double* __restrict__ Pa,
+                                                                       const double* __restrict__ Pauf,
+                                                                       double* __restrict__ P,
+                                                                       double* __restrict__ st) {
+  __shared__ double red[SCF_THREADS / 64];
+  __shared__ int conv;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const long long nn = (long long)n * n;
+  const double rate = st[SCF_ST_RATE];
+  double d = 0.0;
+  for (long long i = tid; i < nn; i += nt) {
+    const double x = rate * Pauf[i] + (1.0 - rate) * Pa[i];
+    d = fmax(d, fabs(x - Pa[i]));
+  }
+  const double dP = block_max(d, red);
+  if (tid == 0) {
+    const double E = 0.5 * st[SCF_ST_ESUM] + st[SCF_ST_ENN];
+    const double dE = st[SCF_ST_HAS_E] != 0.0 ? fabs(E - st[SCF_ST_E]) : __builtin_inf();
+    const bool diis = st[SCF_ST_USE_DIIS] != 0.0;
+    if ((dE > st[SCF_ST_PREV_DE] || dP > st[SCF_ST_PREV_DP]) && !diis) {
+      if (st[SCF_ST_STRIKES] == 3.0) {
+        st[SCF_ST_RATE] = rate / 1.1;
+        st[SCF_ST_STRIKES] = 0.0;
+      } else {
+        st[SCF_ST_STRIKES] += 1.0;
+      }
+    }
+    if (dE < st[SCF_ST_DIIS_TH]) st[SCF_ST_USE_DIIS] = 1.0;
+    const bool c = dE < st[SCF_ST_ETOL] && dP < st[SCF_ST_PTOL];
+    st[SCF_ST_ENEW] = E;
+    st[SCF_ST_DE] = dE;
+    st[SCF_ST_DP] = dP;
+    st[SCF_ST_CONV] = c ? 1.0 : 0.0;
+    if (!c) {
+      st[SCF_ST_E] = E;
+      st[SCF_ST_HAS_E] = 1.0;
+      st[SCF_ST_PREV_DE] = dE;
+      st[SCF_ST_PREV_DP] = dP;
+      st[SCF_ST_COUNT] += 1.0;
+    }
+    conv = c ? 1 : 0;
+  }
+  __syncthreads();
+  if (conv) return;
+  for (long long i = tid; i < nn; i += nt) {
+    const double x = rate * Pauf[i] + (1.0 - rate) * Pa[i];
+    Pa[i] = x;
+    P[i] = 2.0 * x;
+  }
+}
+
+hipError_t launch_scf_diis_stage(int n, int h, const double* F, const double* R, double* histF, double* histR,
+                                 double* G, const double* P, const double* H, const double* Pa, double* st,
+                                 double* Fu, hipStream_t s) {
+  if (h < 1 || h > SCF_HMAX) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(scf_diis_stage_kernel, dim3(1), dim3(SCF_THREADS), 0, s, n, h, F, R, histF, histR, G, P, H,
+                     Pa, st, Fu);
+  return hipGetLastError();
+}
+
+hipError_t launch_scf_damp_control(int n, double* Pa, const double* Pauf, double* P, double* st, hipStream_t s) {
+  hipLaunchKernelGGL(scf_damp_control_kernel, dim3(1), dim3(SCF_THREADS), 0, s, n, Pa, Pauf, P, st);
   return hipGetLastError();
 }
 

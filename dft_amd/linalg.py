@@ -23,10 +23,11 @@ from . import _lib
 JACOBI_NMAX = 128
 
 
-def syevj_(A: torch.Tensor, w: torch.Tensor, V: torch.Tensor, init_v: bool, stream=None) -> None:
+def syevj_(A: torch.Tensor, w: torch.Tensor, V: torch.Tensor, init_v: bool, stream=None, sweeps=None) -> None:
     """In place: A (batch, n, n) contiguous float64 is destroyed, w (batch, n)
     and V (batch, n, n) receive the eigenpairs; with init_v False, V holds
-    the starting orthonormal basis Q and A must be Q^T F Q."""
+    the starting orthonormal basis Q and A must be Q^T F Q. ``sweeps`` (an
+    int32 device tensor of batch entries) receives the Jacobi sweeps."""
     lib = _lib.load()
     n = A.shape[-1]
     batch = A.numel() // (n * n) if n else 0
@@ -34,7 +35,12 @@ def syevj_(A: torch.Tensor, w: torch.Tensor, V: torch.Tensor, init_v: bool, stre
         if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous() and t.numel() == int(torch.Size(shape).numel())):
             raise ValueError(f"{name} must be a contiguous float64 device tensor with {shape} elements")
     s = torch.cuda.current_stream(A.device) if stream is None else stream
-    _lib.check(lib.dft_syevj(n, batch, A.data_ptr(), w.data_ptr(), V.data_ptr(), int(bool(init_v)),
+    sw = None
+    if sweeps is not None:
+        if not (sweeps.is_cuda and sweeps.dtype == torch.int32 and sweeps.numel() >= batch):
+            raise ValueError("sweeps must be an int32 device tensor with batch entries")
+        sw = sweeps.data_ptr()
+    _lib.check(lib.dft_syevj(n, batch, A.data_ptr(), w.data_ptr(), V.data_ptr(), int(bool(init_v)), sw,
                              C.c_void_p(s.cuda_stream)))
 
 

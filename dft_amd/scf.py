@@ -338,3 +338,115 @@ def scf_diis(plan: Plan, Z, centers, N_a, N_b, mode="fp64", rate=0.4, diis_thres
         prev_dE, prev_dP = dE, dP
         P_a, P_b = Pa_new, Pb_new
         count += 1
+
+
+# Indices of the device SCF state (include/dft_hip.h, DFT_SCF_ST_*).
+ST_ESUM, ST_ENN, ST_E, ST_HAS_E, ST_DE, ST_DP, ST_PREV_DE, ST_PREV_DP, ST_RATE, ST_STRIKES, ST_USE_DIIS, \
+    ST_COUNT, ST_CONV, ST_ETOL, ST_PTOL, ST_DIIS_TH, ST_ENEW, ST_SLOT = range(18)
+SCF_STATE_LEN = 32
+
+
+def scf_diis_graph(plan: Plan, Z, centers, N_occ, mode="fp64", rate=0.4, diis_threshold=1e-3, history=10,
+                   e_tol=1e-8, p_tol=1e-6, max_count=500):
+    """The damped + DIIS loop of shelled.py:1103-1292 for a closed shell
+    (N_alpha = N_beta = N_occ), one iteration = one replay of a captured
+    hipGraph: the direct J/K build (dft_jk, its class launches on four
+    streams), F = H + J - K, the residual F P S - S P F (GEMMs), the DIIS
+    stage (dft_scf_diis_stage: history ring, Gram row, DIIS solve,
+    extrapolation, energy sum), X^T F X in the previous orbitals' basis,
+    the Jacobi eigensolver (dft_syevj, warm start), the aufbau density
+    (GEMMs) and damping / strikes / DIIS switch / convergence
+    (dft_scf_damp_control). The loop state stays on the device; the host
+    reads 32 doubles after each replay. The exit rebuild of
+    shelled.py:1261-1285 runs eagerly after the loop. Orders up to 128
+    (the eigensolver's limit); no checkpoint files (scf_diis writes them)."""
+    import ctypes as C
+    from . import _lib
+    from .linalg import syevj_
+    if plan.uhf:
+        raise ValueError("scf_diis_graph is closed-shell: build the plan with uhf=False")
+    n = plan.nbf
+    if n > 128:
+        raise ValueError("scf_diis_graph needs nbf <= 128 (the Jacobi eigensolver's limit); use scf_diis")
+    if not 1 <= history <= 16:
+        raise ValueError("history must be in [1, 16]")
+    lib = _lib.load()
+    dev = plan.device
+    S, H, X, E_NN = _setup(plan, Z, centers)
+    f64 = dict(dtype=torch.float64, device=dev)
+    _, Q = eigh(X.T @ H @ X)
+    C0 = X @ Q
+    Pa = (C0[:, :N_occ] @ C0[:, :N_occ].T).contiguous()
+    P = (2 * Pa).contiguous()
+    Q = Q.contiguous()
+    J, K, F, R, Fu = (torch.empty((n, n), **f64) for _ in range(5))
+    histF = torch.zeros((history, n, n), **f64)
+    histR = torch.zeros((history, n, n), **f64)
+    G = torch.zeros((history, history), **f64)
+    A = torch.empty((1, n, n), **f64)
+    V = torch.empty((1, n, n), **f64)
+    w = torch.empty((1, n), **f64)
+    st = torch.zeros(SCF_STATE_LEN, **f64)
+    st[ST_ENN], st[ST_RATE], st[ST_ETOL], st[ST_PTOL], st[ST_DIIS_TH] = E_NN, rate, e_tol, p_tol, diis_threshold
+    st[ST_PREV_DE] = st[ST_PREV_DP] = float("inf")
+    H, X, S = H.contiguous(), X.contiguous(), S.contiguous()
+
+    def body():
+        s = C.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+        plan.jk_into(P, Pa, None, J, K, None, mode)
+        torch.add(H, J, out=F)
+        F.sub_(K)
+        PS = Pa @ S
+        torch.sub(F @ PS, PS.T @ F, out=R)
+        _lib.check(lib.dft_scf_diis_stage(n, history, F.data_ptr(), R.data_ptr(), histF.data_ptr(),
+                                          histR.data_ptr(), G.data_ptr(), P.data_ptr(), H.data_ptr(),
+                                          Pa.data_ptr(), st.data_ptr(), Fu.data_ptr(), s))
+        XQ = X @ Q
+        At = XQ.T @ Fu @ XQ
+        torch.add(At, At.T, out=A[0]).mul_(0.5)
+        V[0].copy_(Q)
+        syevj_(A, w, V, False)
+        Q.copy_(V[0])
+        Cm = X @ Q
+        Pauf = Cm[:, :N_occ] @ Cm[:, :N_occ].T
+        _lib.check(lib.dft_scf_damp_control(n, Pa.data_ptr(), Pauf.contiguous().data_ptr(), P.data_ptr(),
+                                            st.data_ptr(), s))
+
+    # warm-up of the GEMM paths on other buffers, outside the capture
+    side = torch.cuda.Stream(dev)
+    side.wait_stream(torch.cuda.current_stream(dev))
+    with torch.cuda.stream(side):
+        for _ in range(2):
+            tmp = X @ (Q @ S)
+            _ = tmp.T @ tmp
+    torch.cuda.current_stream(dev).wait_stream(side)
+    torch.cuda.synchronize(dev)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        body()
+    energies, iter_times = [], []
+    converged = False
+    while True:
+        t0 = time.perf_counter()
+        graph.replay()
+        v = st.cpu()
+        iter_times.append(time.perf_counter() - t0)
+        E = float(v[ST_ENEW])
+        if not np.isfinite(E):
+            raise FloatingPointError(f"non-finite energy at iteration {int(v[ST_COUNT])}")
+        energies.append(E)
+        if v[ST_CONV] != 0:
+            converged = True
+            break
+        if int(v[ST_COUNT]) > max_count:
+            break
+    count = int(v[ST_COUNT])
+    # exit rebuild from the last density (shelled.py:1261-1285)
+    fock = _Fock(plan, mode, True)
+    Jf, Kf, _ = fock(P, Pa, Pa)
+    Ff = H + Jf - Kf
+    E_fin = float(0.5 * torch.sum(P * H + 2 * Pa * Ff)) + E_NN
+    ea, Cp = eigh(X.T @ Ff @ X, Q)
+    Cf = X @ Cp
+    return SCFResult(E_fin, converged, count, Pa.clone(), Pa.clone(), Ff, Ff, Cf, Cf, ea, ea, fock.times,
+                     iter_times, energies)

@@ -328,14 +328,70 @@ int dft_eri_prim(int64_t n_ab, const double* p_ab, const double* P_ab, const dou
  * identity first; with init_v = 0, V must hold an orthonormal basis Q and A
  * the matrix in that basis (Q^T F Q), and the eigenvectors of F are
  * returned (Q times the rotations): a warm start from the previous
- * iteration's orbitals. Two-sided cyclic Jacobi, one workgroup per matrix,
- * n <= 128 (DFT_EUNSUPPORTED above). Only enqueues work (no allocation, no
- * synchronisation), so it can be captured into a hipGraph. Device pointers.
+ * iteration's orbitals. sweeps (device, batch ints, or NULL) receives the
+ * Jacobi sweeps each matrix took. Two-sided cyclic Jacobi, one workgroup
+ * per matrix, n <= 128 (DFT_EUNSUPPORTED above). Only enqueues work (no
+ * allocation, no synchronisation), so it can be captured into a hipGraph.
+ * Device pointers.
  *
  * Replaces xp.linalg.eigh of the SCF loops (shelled.py:1210-1213,
  * main.py:796-799) and of S (shelled.py:1082, main.py:754).
  */
-int dft_syevj(int n, int batch, double* A, double* w, double* V, int init_v, void* stream);
+int dft_syevj(int n, int batch, double* A, double* w, double* V, int init_v, int32_t* sweeps, void* stream);
+
+/*
+ * One damped + DIIS SCF iteration on the device, closed shell (the loop of
+ * shelled.py:1164-1260 with P_a = P_b), split in two calls around the
+ * orbital step so that a whole iteration (dft_jk, these two, dft_syevj and
+ * GEMMs) can be captured once and replayed as a hipGraph: the loop's control
+ * state lives in the device array state[DFT_SCF_STATE_LEN] (indices
+ * DFT_SCF_ST_*), read by the host once per iteration. All matrices are
+ * device n x n row-major; hist_F, hist_R are rings of `history`
+ * (<= DFT_SCF_HMAX) matrices, G the history x history Gram matrix, all
+ * zero-initialised by the caller; state[DFT_SCF_ST_RATE, _ENN, _ETOL,
+ * _PTOL, _DIIS_TH] are set by the caller, the other entries start at 0
+ * (PREV_DE and PREV_DP at +inf). Single workgroup kernels (n <= 128).
+ *
+ * dft_scf_diis_stage: stores F and R = F Pa S - S Pa F in the ring slot
+ * count mod history, updates G's row of that slot, solves the DIIS
+ * equations (B c = 1 without a Lagrange row, c normalised to sum 1,
+ * shelled.py:1196-1203) on the used slots, writes the iteration's Fock
+ * matrix Fu (the extrapolation once DIIS is on, else F) and
+ * state[DFT_SCF_ST_ESUM] = sum P.H + 2 sum Pa.Fu.
+ * dft_scf_damp_control: from the aufbau density Pauf of Fu, the damped
+ * density rate Pauf + (1 - rate) Pa (UHF_density, shelled.py:561-574),
+ * E = ESUM / 2 + E_NN, dE, dP = max |change|, the strike / rate / DIIS
+ * switch rules (shelled.py:1250-1257) and the convergence flag; unless
+ * converged, Pa and P = 2 Pa take the damped density (converged: Pa stays
+ * the density of the reference's final rebuild, shelled.py:1261-1285).
+ *
+ * Replace the per-iteration host code of shelled.py:1164-1260 around
+ * fill_matrices (no counterpart as device code).
+ */
+#define DFT_SCF_HMAX 16
+#define DFT_SCF_STATE_LEN 32
+#define DFT_SCF_ST_ESUM 0
+#define DFT_SCF_ST_ENN 1
+#define DFT_SCF_ST_E 2
+#define DFT_SCF_ST_HAS_E 3
+#define DFT_SCF_ST_DE 4
+#define DFT_SCF_ST_DP 5
+#define DFT_SCF_ST_PREV_DE 6
+#define DFT_SCF_ST_PREV_DP 7
+#define DFT_SCF_ST_RATE 8
+#define DFT_SCF_ST_STRIKES 9
+#define DFT_SCF_ST_USE_DIIS 10
+#define DFT_SCF_ST_COUNT 11
+#define DFT_SCF_ST_CONV 12
+#define DFT_SCF_ST_ETOL 13
+#define DFT_SCF_ST_PTOL 14
+#define DFT_SCF_ST_DIIS_TH 15
+#define DFT_SCF_ST_ENEW 16
+#define DFT_SCF_ST_SLOT 17
+int dft_scf_diis_stage(int n, int history, const double* F, const double* R, double* hist_F, double* hist_R,
+                       double* G, const double* P, const double* H, const double* Pa, double* state, double* Fu,
+                       void* stream);
+int dft_scf_damp_control(int n, double* Pa, const double* Pauf, double* P, double* state, void* stream);
 
 /*
  * Boys function test hook: out[i*(mmax+1)+m] = F_m(T[i]), m = 0..mmax, with

@@ -61,6 +61,21 @@ def test_syevj_warm_start_from_nearby_eigenvectors(torch_cuda):
     _, V0 = eigh(torch.as_tensor(A, device="cuda"))
     w, V = eigh(torch.as_tensor(A2, device="cuda"), V0=V0)
     _check(A2, w.cpu().numpy(), V.cpu().numpy())
+    # sweeps: from scratch vs from the nearby eigenvectors
+    from dft_amd.linalg import syevj_
+    sw = torch.zeros(2, dtype=torch.int32, device="cuda")
+    for k, (M0, init) in enumerate(((A2, True), (None, False))):
+        Ad = torch.as_tensor(A2, device="cuda").reshape(1, n, n).clone()
+        Vd = torch.eye(n, dtype=torch.float64, device="cuda").reshape(1, n, n).clone()
+        if not init:
+            Vd[0] = V0
+            Ad[0] = V0.T @ Ad[0] @ V0
+            Ad[0] = 0.5 * (Ad[0] + Ad[0].T)
+        wd = torch.empty((1, n), dtype=torch.float64, device="cuda")
+        syevj_(Ad.contiguous(), wd, Vd, init, sweeps=sw[k:k + 1])
+    cold, warm = (int(x) for x in sw.cpu())
+    print(f"Jacobi sweeps for n = {n}: {cold} from the identity, {warm} from nearby eigenvectors", flush=True)
+    assert warm < cold <= 40
 
 
 def test_syevj_fock_matrix_of_water(torch_cuda):
@@ -77,3 +92,36 @@ def test_syevj_fock_matrix_of_water(torch_cuda):
     Fp = X.T @ F @ X
     w, V = eigh(torch.as_tensor(Fp, device="cuda"))
     _check(Fp, w.cpu().numpy(), V.cpu().numpy(), tol=1e-13)
+
+
+def test_syevj_time_per_call_is_reported(torch_cuda):
+    """Device time of one eigendecomposition at the (H2O)16 order (112),
+    from the identity and from the eigenvectors of a nearby matrix."""
+    from dft_amd.linalg import syevj_
+    torch = torch_cuda
+    n = 112
+    rng = np.random.default_rng(3)
+    M = rng.standard_normal((n, n))
+    A = torch.as_tensor(M + M.T, device="cuda")
+    w = torch.empty((1, n), dtype=torch.float64, device="cuda")
+    V = torch.empty((1, n, n), dtype=torch.float64, device="cuda")
+    sw = torch.zeros(1, dtype=torch.int32, device="cuda")
+    for init in (True, False):
+        Ad = A.reshape(1, n, n).clone()
+        if not init:
+            Q = V.clone()
+            Ad = (Q.transpose(1, 2) @ (A.reshape(1, n, n) + 1e-4) @ Q).contiguous()
+            Ad = (0.5 * (Ad + Ad.transpose(1, 2))).contiguous()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        src = Ad.clone()
+        Vs = V.clone()
+        e0.record()
+        for _ in range(5):
+            Ad.copy_(src)
+            if not init:
+                V.copy_(Vs)
+            syevj_(Ad, w, V, init, sweeps=sw)
+        e1.record()
+        torch.cuda.synchronize()
+        print(f"syevj n = {n}, {'identity' if init else 'warm'} start: {e0.elapsed_time(e1) / 5:.3f} ms per call, "
+              f"{int(sw.cpu()[0])} sweeps", flush=True)

@@ -1,0 +1,56 @@
+"""The captured SCF iteration (scf_diis_graph: one hipGraph replay per
+iteration, loop state on the device) against the eager damped + DIIS loop
+(scf_diis, itself checked against the oracle's restatement of
+shelled.py:1103-1292) and the golden energies."""
+import numpy as np
+import pytest
+
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    return torch
+
+
+def _plan(Z, xyz, tau=0.0):
+    from dft_amd.basis import build_shells, load_basis
+    from dft_amd.engine import Plan
+    return Plan(build_shells(load_basis(), Z, xyz), Z, xyz, schwarz_tau=tau)
+
+
+@pytest.mark.parametrize("name,nocc", [("water_sto3g.npz", 5), ("dimer_sto3g.npz", 10)])
+def test_graph_scf_follows_the_eager_loop(torch_cuda, name, nocc):
+    from dft_amd.scf import scf_diis, scf_diis_graph
+    g = golden(name)
+    Z, xyz = list(g["Z"]), g["xyz"]
+    plan = _plan(Z, xyz)
+    eager = scf_diis(plan, Z, xyz, nocc, nocc)
+    gr = scf_diis_graph(plan, Z, xyz, nocc)
+    assert eager.converged and gr.converged
+    n = min(len(eager.energies), len(gr.energies))
+    d = np.abs(np.asarray(eager.energies[:n]) - np.asarray(gr.energies[:n]))
+    print(f"{name}: eager {eager.iterations} / graph {gr.iterations} iterations, max|dE| first 10 "
+          f"{d[:10].max():.2e}, final dE {gr.energy - eager.energy:.2e}", flush=True)
+    assert d[:10].max() < 1e-10  # damped iterations: the same schedule
+    assert abs(gr.iterations - eager.iterations) <= 2
+    assert abs(gr.energy - eager.energy) < 1e-9
+    assert abs(gr.energy - float(g["E"])) < 1e-7
+
+
+def test_graph_scf_water16_converges_to_the_eager_energy(torch_cuda):
+    from dft_amd.geometry import water_cluster
+    from dft_amd.scf import scf_diis, scf_diis_graph
+    Z, xyz = water_cluster((4, 2, 2), 3.1, seed=0)
+    plan = _plan(Z, xyz, tau=1e-10)
+    eager = scf_diis(plan, Z, xyz, 80, 80, e_tol=1e-10, p_tol=1e-8)
+    gr = scf_diis_graph(plan, Z, xyz, 80, e_tol=1e-10, p_tol=1e-8)
+    assert eager.converged and gr.converged
+    print(f"(H2O)16: eager {eager.iterations} / graph {gr.iterations} iterations, E {gr.energy:.10f}, "
+          f"dE {gr.energy - eager.energy:.2e}, wall/iteration eager {np.median(eager.iter_times) * 1e3:.3f} ms, "
+          f"graph {np.median(gr.iter_times) * 1e3:.3f} ms", flush=True)
+    assert abs(gr.energy - eager.energy) < 1e-9

@@ -57,7 +57,7 @@ def test_unsupported_angular_momentum_is_rejected():
 def test_eigensolver_rejects_orders_above_its_limit():
     from dft_amd import _lib
     lib = _lib.load()
-    rc = lib.dft_syevj(129, 1, None, None, None, 1, None)
+    rc = lib.dft_syevj(129, 1, None, None, None, 1, None, None)
     assert rc == _lib.DFT_EUNSUPPORTED
     assert b"above 128" in lib.dft_last_error()
 
