@@ -16,6 +16,7 @@ DFT_PLAN_UHF = 1
 DFT_PLAN_NOSPLIT = 2
 DFT_PLAN_SERIAL = 4
 DFT_PLAN_STATIC_WORKSPACE = 8
+DFT_PLAN_MFMA = 16
 DFT_STATIC_SCRATCH_BYTES = 32 << 20  # include/dft_hip.h
 STATS_COLS = 8  # DFT_STATS_COLS: values per class combination in dft_plan_stats
 DFT_FP64, DFT_MIXED = 0, 1

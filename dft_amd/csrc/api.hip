@@ -147,6 +147,9 @@ struct Combo {
   // mixed mode runs the weak part in FP32 (the tuner clears this where one
   // FP64 launch over the whole list is faster than the two launches)
   bool use32 = true;
+  // heavy combinations: the FP64 part on the matrix cores (chosen by the
+  // tuner where it is faster than the column kernel)
+  bool mfma = false;
   // primitive split of the lane kernel: every quartet is evaluated by
   // splb * splk lanes, lane (cb, ck) taking bra chunk cb and ket chunk ck of
   // the primitive pairs and writing its own partial J/K blocks; chunk-major
@@ -174,6 +177,7 @@ struct dft_plan_s {
   int nsh = 0, natom = 0, nk = 1, nmol = 1;
   int device = 0;                // HIP device the plan was built on
   bool split = true;             // primitive split of small lane-kernel classes
+  bool mfma = false;             // heavy combinations' FP64 part on the matrix cores
   // class launches of a build on several streams: helps builds whose class
   // launches cannot fill the chip one at a time; large builds run faster
   // serially (set by build size, then by dft_plan_tune)
@@ -1061,6 +1065,9 @@ hipError_t launch_level(dft_plan p, const Workspace& ws, size_t ci, int lev, lon
   pc_moments(c.pcb, la, lb);
   pc_moments(c.pck, lc, ld);
   const long long ni = (long long)ncart_h(la) * ncart_h(lb) * ncart_h(lc) * ncart_h(ld);
+  if (!f32 && (p->mfma || c.mfma) && mfma_combo((int)ci))
+    return launch_eri_mfma((int)ci, c.d_q, (int)first, (int)n, c.splb, prs, p->d_rec64, ws.ibuf + c.ibuf_off, 1, ni,
+                           ni * (long long)c.q.size(), p->d_boys, sh.cap * 4, true, s);
   return launch_eri_chunk((int)ci, f32, c.d_q, (int)first, (int)n, c.splb, prs, rv, ws.ibuf + c.ibuf_off, 1, ni,
                           ni * (long long)c.q.size(), p->d_boys, sh.block, sh.cap, true, s);
 }
@@ -1419,6 +1426,7 @@ int validate_and_build(int nmol, const int32_t* mol_nsh, const int32_t* mol_nato
   p->nk = (flags & DFT_PLAN_UHF) ? 2 : 1;
   p->split = !(flags & DFT_PLAN_NOSPLIT);
   p->own_ws = !(flags & DFT_PLAN_STATIC_WORKSPACE);
+  p->mfma = (flags & DFT_PLAN_MFMA) != 0;
   if (hipGetDevice(&p->device) != hipSuccess) {
     delete p;
     return fail(DFT_EHIP, "hipGetDevice failed");
@@ -1644,6 +1652,7 @@ int dft_plan_tune(dft_plan p, const double* P, const double* Pa, const double* P
         if (sh.block <= block_max((int)ci)) cand.push_back(sh);
       float best = 1e30f;
       Shape best_sh = part ? c.shape32 : c.shape64;
+      if (part == 0) c.mfma = false;  // shapes are searched on the column kernel
       for (const Shape& sh : cand) {
         hipError_t e = launch_combo(p, p->ws, ci, part == 1, first, n, sh, P, Pa, Pb, s);  // warm-up
         std::vector<float> t;
@@ -1667,6 +1676,28 @@ int dft_plan_tune(dft_plan p, const double* P, const double* Pa, const double* P
         if (med < best) {
           best = med;
           best_sh = sh;
+        }
+      }
+      if (part == 0 && rc == DFT_OK && !p->mfma && mfma_combo((int)ci)) {
+        // the matrix-core variant of the FP64 part, kept where it is faster
+        c.mfma = true;
+        hipError_t e = launch_combo(p, p->ws, ci, false, first, n, best_sh, P, Pa, Pb, s);
+        std::vector<float> t;
+        for (int r = 0; r < reps && e == hipSuccess; ++r) {
+          (void)hipEventRecord(e0, s);
+          e = launch_combo(p, p->ws, ci, false, first, n, best_sh, P, Pa, Pb, s);
+          (void)hipEventRecord(e1, s);
+          if (e == hipSuccess) e = hipEventSynchronize(e1);
+          float ms = 0.f;
+          if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+          t.push_back(ms);
+        }
+        if (e != hipSuccess) {
+          rc = fail(DFT_EHIP, std::string("tune (mfma): ") + hipGetErrorString(e));
+        } else {
+          std::sort(t.begin(), t.end());
+          c.mfma = t[t.size() / 2] < best;
+          if (c.mfma) best = t[t.size() / 2];
         }
       }
       if (part == 0) {
@@ -1764,7 +1795,8 @@ int dft_plan_tune(dft_plan p, const double* P, const double* Pa, const double* P
   if (report)
     for (size_t ci = 0; ci < p->combos.size() && (int)(6 * ci + 5) < report_cap; ++ci) {
       const Combo& c = p->combos[ci];
-      const int v[6] = {0, c.shape64.block, c.shape64.cap, c.use32 ? 0 : 1, c.shape32.block, c.shape32.cap};
+      const int v[6] = {c.mfma ? 1 : 0, c.shape64.block, c.shape64.cap, c.use32 ? 0 : 1, c.shape32.block,
+                        c.shape32.cap};
       for (int k = 0; k < 6; ++k) report[6 * ci + k] = v[k];
     }
   return DFT_OK;
@@ -1772,28 +1804,36 @@ int dft_plan_tune(dft_plan p, const double* P, const double* Pa, const double* P
 
 int dft_plan_set_shapes(dft_plan p, const int* shapes, int n) {
   if (!p || !shapes) return fail(DFT_EINVAL, "NULL argument");
-  for (size_t ci = 0; ci < p->combos.size() && (int)(6 * ci + 5) < n; ++ci) {
-    Combo& c = p->combos[ci];
-    Shape s64, s32;
-    s64.wave = shapes[6 * ci];
-    s64.block = shapes[6 * ci + 1];
-    s64.cap = shapes[6 * ci + 2];
-    s32.wave = shapes[6 * ci + 3];
-    s32.block = shapes[6 * ci + 4];
-    s32.cap = shapes[6 * ci + 5];
-    if (s64.wave != 0) return fail(DFT_EINVAL, "the first int of the FP64 shape is reserved and must be 0");
-    if (s32.wave != 0 && s32.wave != 1) return fail(DFT_EINVAL, "the first int of the FP32 shape must be 0 or 1");
-    const bool use32 = s32.wave == 0;
-    s32.wave = 0;
-    for (const Shape* s : {&s64, &s32}) {
-      if (s->block != 64 && s->block != 128 && s->block != 256 && s->block != 512)
-        return fail(DFT_EINVAL, "block must be 64, 128, 256 or 512");
-      if (s->block > block_max((int)ci)) return fail(DFT_EINVAL, "block exceeds the class's largest workgroup");
-      if (s->cap < 1) return fail(DFT_EINVAL, "grid cap must be positive");
+  // two passes: every shape is checked before any is applied
+  for (int apply = 0; apply < 2; ++apply) {
+    for (size_t ci = 0; ci < p->combos.size() && (int)(6 * ci + 5) < n; ++ci) {
+      Combo& c = p->combos[ci];
+      Shape s64, s32;
+      s64.wave = shapes[6 * ci];
+      s64.block = shapes[6 * ci + 1];
+      s64.cap = shapes[6 * ci + 2];
+      s32.wave = shapes[6 * ci + 3];
+      s32.block = shapes[6 * ci + 4];
+      s32.cap = shapes[6 * ci + 5];
+      if (s64.wave != 0 && !(s64.wave == 1 && mfma_combo((int)ci)))
+        return fail(DFT_EINVAL, "the first int of the FP64 shape must be 0, or 1 (matrix cores) for a heavy combination");
+      const bool use_mfma = s64.wave == 1;
+      s64.wave = 0;
+      if (s32.wave != 0 && s32.wave != 1) return fail(DFT_EINVAL, "the first int of the FP32 shape must be 0 or 1");
+      const bool use32 = s32.wave == 0;
+      s32.wave = 0;
+      for (const Shape* s : {&s64, &s32}) {
+        if (s->block != 64 && s->block != 128 && s->block != 256 && s->block != 512)
+          return fail(DFT_EINVAL, "block must be 64, 128, 256 or 512");
+        if (s->block > block_max((int)ci)) return fail(DFT_EINVAL, "block exceeds the class's largest workgroup");
+        if (s->cap < 1) return fail(DFT_EINVAL, "grid cap must be positive");
+      }
+      if (!apply) continue;
+      c.shape64 = s64;
+      c.shape32 = s32;
+      c.use32 = use32;
+      c.mfma = use_mfma;
     }
-    c.shape64 = s64;
-    c.shape32 = s32;
-    c.use32 = use32;
   }
   return DFT_OK;
 }

@@ -112,6 +112,14 @@ hipError_t launch_jk_digest(int combo, const int2* quartets, const long long* sl
                             const int4* pairs, const int4* shell_dm, const double* DJ, const double* DK0,
                             const double* DK1, int nk, const double* ibuf, double* stage, hipStream_t s);
 
+// Heavy combinations on the matrix cores (FP64 MFMA products for the ket and
+// bra steps, kernels_mfma.hip): the same integrals and output layout as
+// launch_eri_chunk, one wave per (quartet, bra primitive chunk).
+bool mfma_combo(int combo);
+hipError_t launch_eri_mfma(int combo, const int2* q, int first, int n, int splb, const int4* pairs, const double* rec,
+                           double* out, long long xs, long long qs, long long ss, const double* gtab, int cap,
+                           bool cut, hipStream_t s);
+
 // raw contracted blocks (internal orientation) for one class combination:
 // out + q * NA*NB*NC*ND
 hipError_t launch_eri_raw(int combo, const int2* quartets, long long nq, const int4* pairs,

@@ -22,10 +22,11 @@ def _ncart(l: int) -> int:
     return (l + 1) * (l + 2) // 2
 
 
-def _flags(uhf: bool, split: bool, serial: bool = False, static_workspace: bool = False) -> int:
+def _flags(uhf: bool, split: bool, serial: bool = False, static_workspace: bool = False, mfma: bool = False) -> int:
     return ((_lib.DFT_PLAN_UHF if uhf else 0) | (0 if split else _lib.DFT_PLAN_NOSPLIT)
             | (_lib.DFT_PLAN_SERIAL if serial else 0)
-            | (_lib.DFT_PLAN_STATIC_WORKSPACE if static_workspace else 0))
+            | (_lib.DFT_PLAN_STATIC_WORKSPACE if static_workspace else 0)
+            | (_lib.DFT_PLAN_MFMA if mfma else 0))
 
 
 class Plan:
@@ -41,7 +42,7 @@ class Plan:
     """
 
     def __init__(self, shells, Z, centers, schwarz_tau=1e-10, fp32_tau=1e-4, uhf=False,
-                 device=None, split=True, serial=False, static_workspace=False):
+                 device=None, split=True, serial=False, static_workspace=False, mfma=False):
         self.lib = _lib.load()
         if not torch.cuda.is_available():
             raise _lib.DFTError("no HIP device visible: the integral engine runs on the GPU only")
@@ -80,7 +81,7 @@ class Plan:
                 _lib.ptr(seg_ao, C.c_int32), _lib.ptr(seg_xyz, C.c_double),
                 _lib.ptr(exps, C.c_double), _lib.ptr(coefs, C.c_double), len(Zc),
                 _lib.ptr(xyz, C.c_double), _lib.ptr(Zc, C.c_double), self.nbf,
-                float(schwarz_tau), float(fp32_tau), _flags(uhf, split, serial, static_workspace),
+                float(schwarz_tau), float(fp32_tau), _flags(uhf, split, serial, static_workspace, mfma),
                 C.byref(handle)))
         self.handle = handle
         self.schwarz_tau, self.fp32_tau = schwarz_tau, fp32_tau
@@ -333,7 +334,9 @@ class Graph:
 def tune(plan, P, Pa, Pb=None, reps=3, log=None):
     """Autotune the plan's launch shapes on these densities, one class
     combination at a time; returns the choices as an int array [ncombo, 6]
-    (FP64 {wave, block, cap}, FP32 {wave, block, cap})."""
+    (FP64 {m, block, cap}, FP32 {f, block, cap}); m = 1 where the heavy
+    d combination's FP64 part runs on the matrix cores (kernels_mfma.hip),
+    f = 1 where the combination stays FP64 in mixed mode."""
     plan.check_buffers(P, Pa, Pb, P, Pa, Pb)  # densities only (the tuner writes no J/K)
     per = plan.stats()["per_combo"]
     ncombo = len(per)

@@ -39,6 +39,9 @@ extern "C" {
                               order of a molecule the same in single and batch plans) */
 #define DFT_PLAN_SERIAL 4  /* all class launches of a Fock build on the caller's stream
                               (no concurrent launches; for per-kernel profiling) */
+#define DFT_PLAN_MFMA 16    /* heavy d class combinations (FP64 part) on the matrix cores
+                              (MFMA products for the ket and bra contractions) instead of
+                              the VALU column kernel */
 #define DFT_PLAN_STATIC_WORKSPACE 8 /* allocate no Fock-build workspace: the plan's J/K are
                               built only by dft_jk_static (dft_jk, graphs and the tuner
                               reject it) */
@@ -85,7 +88,7 @@ const char* dft_last_error(void);
  *  schwarz_tau         quartets with Q_ab Q_cd < schwarz_tau are skipped
  *  fp32_tau            quartets with Q_ab Q_cd < fp32_tau form the FP32 part
  *  flags               DFT_PLAN_UHF | DFT_PLAN_NOSPLIT | DFT_PLAN_SERIAL |
- *                      DFT_PLAN_STATIC_WORKSPACE or 0
+ *                      DFT_PLAN_STATIC_WORKSPACE | DFT_PLAN_MFMA or 0
  */
 int dft_plan_create(int nsh, const int32_t* shell_l, const int32_t* shell_nprim, const int32_t* shell_ao,
                     const double* shell_xyz, const double* prim_exp, const double* prim_coef, int natom,
@@ -134,10 +137,12 @@ int dft_graph_destroy(dft_graph graph);
  * DFT_MIXED only if FP64 part + FP32 part beat one FP64 launch over the whole
  * list by 3% (else that class runs all-FP64 in DFT_MIXED too); whole-plan
  * tuning (combo -1) keeps those choices only if whole mixed builds with them
- * beat whole mixed builds with FP32 in every class. report
+ * beat whole mixed builds with FP32 in every class. For the heavy d
+ * combinations it also times the matrix-core variant (FP64 MFMA products,
+ * kernels_mfma.hip) of the FP64 part and keeps it where it is faster. report
  * (capacity report_cap ints) receives 6 ints per combination: FP64
- * {0, block, cap}, FP32 {f, block, cap} with f = 0 (FP32 used) or 1 (FP64
- * instead). combo >= 0 tunes only that combination; -1 tunes
+ * {m, block, cap} with m = 1 where the matrix-core variant runs, FP32
+ * {f, block, cap} with f = 0 (FP32 used) or 1 (FP64 instead). combo >= 0 tunes only that combination; -1 tunes
  * all of them and then chooses between running the class launches of a
  * build serially or concurrently (timing whole FP64 builds); -2 makes only
  * that last choice. Overwrites the plan's staging (J/K are not written).
@@ -146,8 +151,9 @@ int dft_graph_destroy(dft_graph graph);
 int dft_plan_tune(dft_plan plan, const double* P, const double* Pa, const double* Pb, int reps, int combo,
                   int* report, int report_cap, void* stream);
 
-/* Set launch shapes and the FP32 choice (6 ints per combination, as
- * reported by dft_plan_tune). */
+/* Set launch shapes, the matrix-core choice and the FP32 choice (6 ints per
+ * combination, as reported by dft_plan_tune). Every entry is checked before
+ * any is applied: DFT_EINVAL leaves the plan as it was. */
 int dft_plan_set_shapes(dft_plan plan, const int* shapes, int n);
 
 /*

@@ -65,7 +65,7 @@ def test_eigensolver_rejects_orders_above_its_limit():
 def test_python_constants_match_the_header():
     from dft_amd import _lib
     text = open(os.path.join(ROOT, "include", "dft_hip.h")).read()
-    for name in ("DFT_PLAN_UHF", "DFT_PLAN_NOSPLIT", "DFT_PLAN_SERIAL", "DFT_PLAN_STATIC_WORKSPACE"):
+    for name in ("DFT_PLAN_UHF", "DFT_PLAN_NOSPLIT", "DFT_PLAN_SERIAL", "DFT_PLAN_STATIC_WORKSPACE", "DFT_PLAN_MFMA"):
         m = re.search(r"#define\s+%s\s+(\d+)" % name, text)
         assert m and int(m.group(1)) == getattr(_lib, name), name
     m = re.search(r"#define\s+DFT_STATIC_SCRATCH_BYTES\s+\((\d+)LL << (\d+)\)", text)
