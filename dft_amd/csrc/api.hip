@@ -230,8 +230,9 @@ struct dft_plan_s {
   long long* d_mol_base = nullptr;
   int* d_mol_nbf = nullptr;
   double* d_atoms = nullptr;     // x, y, z, Z
-  int2* d_shpairs = nullptr;
+  int2* d_shpairs = nullptr;     // shell pairs (later shell first), sorted by class
   int nshpairs = 0;
+  int shp_first[NPC] = {}, shp_n[NPC] = {};  // class ranges of d_shpairs
   double* d_boys = nullptr;
   // optional per-launch timing: slot 2c = combo c FP64 launch, 2c+1 = its
   // FP32 launch, then reduce, symmetrize; one begin/end event pair per slot and step
@@ -251,8 +252,9 @@ struct dft_plan_s {
   std::vector<long long> gq_n;
   std::vector<long long*> d_gslots_ab, d_gslots_cd;
   long long* d_g1slots = nullptr;
-  int4* d_g1tasks = nullptr;
-  int ng1tasks = 0;
+  int2* d_g1sp = nullptr;         // gradient shell pairs (i >= j), slot index order
+  int* d_g1pidx = nullptr;        // their indices grouped by class
+  int g1_first[NPC] = {}, g1_n[NPC] = {};
   long long* d_gbase = nullptr;
   int* d_gcount = nullptr;
   double* d_gstage = nullptr;
@@ -276,7 +278,7 @@ void free_plan(dft_plan p) {
   void* ptrs[] = {p->d_sh,    p->d_exps,  p->d_coefs,   p->d_pairs,    p->d_shell_dm,  p->d_shell_ao,
                   p->d_rec64, p->d_rec32[0], p->d_ws,      p->d_tg,       p->d_tbase,     p->d_tcount,
                   p->d_atoms, p->d_shpairs, p->d_mol_base, p->d_mol_nbf,   p->d_grec,
-                  p->d_gpairs, p->d_g1slots, p->d_gbase, p->d_gcount, p->d_gstage, p->d_recT64, p->d_g1tasks,
+                  p->d_gpairs, p->d_g1slots, p->d_gbase, p->d_gcount, p->d_gstage, p->d_recT64, p->d_g1sp, p->d_g1pidx,
                   p->d_gchunks, p->d_gcfirst, p->d_gpart,
                   p->d_pairs32[0], p->d_pairs32[1], p->d_rec32[1], p->d_recT32[0], p->d_recT32[1]};
   for (void* x : ptrs) (void)hipFree(x);
@@ -564,6 +566,19 @@ int build_plan(dft_plan p, const Input& in) {
     const auto& ms = mol_shells[m];
     for (size_t a = 0; a < ms.size(); ++a)
       for (size_t b = 0; b <= a; ++b) sp.push_back(make_int2(ms[a], ms[b]));
+  }
+  // grouped by class (shells are l-sorted, so the first shell has the larger l)
+  {
+    std::vector<int2> by;
+    for (int pc = 0; pc < NPC; ++pc) {
+      p->shp_first[pc] = (int)by.size();
+      for (const int2& x : sp) {
+        if (p->sh[x.x].l < p->sh[x.y].l) return fail(DFT_EINVAL, "internal: shell pair order");
+        if (pc_index(p->sh[x.x].l, p->sh[x.y].l) == pc) by.push_back(x);
+      }
+      p->shp_n[pc] = (int)by.size() - p->shp_first[pc];
+    }
+    sp.swap(by);
   }
   p->nshpairs = (int)sp.size();
   if ((rc = upload(&p->d_shpairs, sp))) return rc;
@@ -1235,7 +1250,7 @@ int grad_prepare(dft_plan p, const int32_t* shell_atom) {
     p->grad_atom[p->in2int[i]] = a;
   }
   for (const auto& s : p->sh)
-    if (s.l > 1) return fail(DFT_EUNSUPPORTED, "analytic gradients are built for s and p shells only");
+    if (s.l > GRAD_LMAX) return fail(DFT_EUNSUPPORTED, "analytic gradients are built for s and p shells only");
   std::vector<int4> pp, gp(p->pairs.size());
   std::vector<int> off;
   long long roff = 0;
@@ -1335,15 +1350,18 @@ int grad_prepare(dft_plan p, const int32_t* shell_atom) {
       s1[b + 2] = slot(cn);
     }
   if ((rc = upload(&p->d_g1slots, s1))) return rc;
-  // 1-e derivative tasks: per shell pair, blocks of g = 64 / (na nb) nuclei
-  std::vector<int4> tasks;
-  for (int k = 0; k < nsp; ++k) {
-    const int nc = ncart_h(p->sh[sp[k].x].l) * ncart_h(p->sh[sp[k].y].l);
-    const int g = 64 / nc;
-    for (int c0 = 0; c0 < natom; c0 += g) tasks.push_back(make_int4(k, c0, std::min(g, natom - c0), nc));
+  // 1-e derivative kernels: shell pairs grouped by class
+  {
+    std::vector<int> pidx;
+    for (int pc = 0; pc < NPC; ++pc) {
+      p->g1_first[pc] = (int)pidx.size();
+      for (int k = 0; k < nsp; ++k)
+        if (pc_index(p->sh[sp[k].x].l, p->sh[sp[k].y].l) == pc) pidx.push_back(k);
+      p->g1_n[pc] = (int)pidx.size() - p->g1_first[pc];
+    }
+    if ((rc = upload(&p->d_g1sp, sp))) return rc;
+    if ((rc = upload(&p->d_g1pidx, pidx))) return rc;
   }
-  p->ng1tasks = (int)tasks.size();
-  if ((rc = upload(&p->d_g1tasks, tasks))) return rc;
   if ((rc = upload(&p->d_gbase, base))) return rc;
   if ((rc = upload(&p->d_gcount, cnt))) return rc;
   HIP_TRY(hipMalloc(&p->d_gstage, std::max<long long>(cur, 1) * sizeof(double)));
@@ -1944,8 +1962,14 @@ int dft_eri_quartets(dft_plan p, const int32_t* quartets, int64_t nq, double* ou
 
 int dft_oneelec(dft_plan p, double* S, double* T, double* V, void* stream) {
   if (!p || !S || !T || !V) return fail(DFT_EINVAL, "NULL argument");
-  hipError_t e = launch_one_electron(p->d_shpairs, p->nshpairs, p->d_sh, p->d_exps, p->d_coefs, p->d_atoms, S, T, V,
-                                     p->d_boys, (hipStream_t)stream);
+  hipError_t e = hipSuccess;
+  for (int pc = 0; pc < NPC && e == hipSuccess; ++pc) {
+    if (p->shp_n[pc] == 0) continue;
+    int la, lb;
+    pc_moments(pc, la, lb);
+    e = launch_one_electron(la, lb, p->d_shpairs + p->shp_first[pc], p->shp_n[pc], p->d_sh, p->d_exps, p->d_coefs,
+                            p->d_atoms, S, T, V, p->d_boys, (hipStream_t)stream);
+  }
   if (e != hipSuccess) return fail(DFT_EHIP, std::string("one-electron: ") + hipGetErrorString(e));
   return DFT_OK;
 }
@@ -1986,8 +2010,14 @@ int dft_grad(dft_plan p, const int32_t* shell_atom, const double* P, const doubl
                           p->d_shell_ao, P, Pa, Pb, nbf, p->d_gstage, p->d_boys, s);
     if (e != hipSuccess) return fail(DFT_EHIP, std::string("eri_grad: ") + hipGetErrorString(e));
   }
-  hipError_t e = launch_grad1e(p->d_g1tasks, p->ng1tasks, p->d_shpairs, p->d_sh, p->d_exps, p->d_coefs, p->natom,
-                               p->d_atoms, P, W, nbf, p->d_g1slots, p->d_gstage, p->d_boys, s);
+  hipError_t e = hipSuccess;
+  for (int pc = 0; pc < NPC && e == hipSuccess; ++pc) {
+    if (p->g1_n[pc] == 0) continue;
+    int la, lb;
+    pc_moments(pc, la, lb);
+    e = launch_grad1e(la, lb, p->d_g1sp, p->d_g1pidx + p->g1_first[pc], p->g1_n[pc], p->d_sh, p->d_exps, p->d_coefs,
+                      p->natom, p->d_atoms, P, W, nbf, p->d_g1slots, p->d_gstage, p->d_boys, s);
+  }
   if (e == hipSuccess)
     e = launch_grad_reduce(p->natom, p->n_gchunk, p->d_gchunks, p->d_gcfirst, p->d_gbase, p->d_gstage, p->d_gpart,
                            grad, s);

@@ -138,10 +138,15 @@ hipError_t launch_ao_grid(int nsh, const ShellDev* shells, const double* exps, c
 hipError_t launch_symmetrize(int nmol, const long long* mol_base, const int* mol_nbf, int max_nbf,
                              const double* half, double* out, hipStream_t s);
 
-// S, T, V over shell pairs (one wave per shell pair), packed outputs
-hipError_t launch_one_electron(const int2* shell_pairs, int npairs, const ShellDev* shells,
+// S, T, V of the shell pairs sp[0..npairs) of one class (la >= lb; the
+// first shell of each pair has momentum la), one 64-lane workgroup per
+// shell pair, packed outputs (kernels_onee.hip)
+hipError_t launch_one_electron(int la, int lb, const int2* shell_pairs, int npairs, const ShellDev* shells,
                                const double* exps, const double* coefs, const double* atom_xyzq, double* S,
                                double* T, double* V, const double* boys_tab, hipStream_t s);
+
+// largest shell momentum of the analytic gradient kernels
+constexpr int GRAD_LMAX = 1;
 
 // analytic gradients (s/p, one molecule): extended pair records, derivative
 // ERIs contracted with the density pair (bra-centre derivatives; call again
@@ -153,10 +158,13 @@ hipError_t launch_eri_grad(int la, int lb, int lc, int ld, const int2* quartets,
                            const int4* gpairs, const double* grec, const int* shell_ao, const double* DJ,
                            const double* DK0, const double* DK1, int nbf, double* stage, const double* boys_tab,
                            hipStream_t s);
-hipError_t launch_grad1e(const int4* tasks /* pair, first nucleus, nuclei, components */, int ntask,
-                         const int2* shell_pairs, const ShellDev* shells, const double* exps, const double* coefs,
-                         int natom, const double* atoms, const double* P, const double* W, int nbf,
-                         const long long* slots, double* stage, const double* boys_tab, hipStream_t s);
+// 1-e derivatives of the shell pairs sp[pidx[0..npairs)] of one class
+// (la >= lb), one 64-lane workgroup per shell pair; slots: three per
+// (shell pair, nucleus) (kernels_onee.hip)
+hipError_t launch_grad1e(int la, int lb, const int2* shell_pairs, const int* pidx, int npairs, const ShellDev* shells,
+                         const double* exps, const double* coefs, int natom, const double* atoms, const double* P,
+                         const double* W, int nbf, const long long* slots, double* stage, const double* boys_tab,
+                         hipStream_t s);
 // GRAD_CHUNK: slots per stage-1 chunk of the per-atom gradient sums
 constexpr int GRAD_CHUNK = 1024;
 hipError_t launch_grad_reduce(int natom, int nchunk, const int4* chunks, const int* cfirst, const long long* base,

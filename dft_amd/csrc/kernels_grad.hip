@@ -240,217 +240,6 @@ hipError_t launch_grad_records(const int4* pp, int n, const int* rec_off, const 
   return hipGetLastError();
 }
 
-// ----------------------------------------------------- one-electron part
-__device__ void e1d_g(double a, double b, double Q, int ia, int ib, double* E) {
-  const double p = a + b;
-  double prev[12], next[12];
-  for (int t = 0; t < 12; ++t) prev[t] = 0.0;
-  prev[0] = exp(-(a * b / p) * Q * Q);
-  int order = 0;
-  for (int step = 0; step < ia + ib; ++step) {
-    const double coef = step < ia ? -(b * Q / p) : (a * Q / p);
-    const int no = order + 1;
-    for (int t = 0; t <= no; ++t) {
-      double v = 0.0;
-      if (t <= order) v += coef * prev[t];
-      if (t > 0) v += prev[t - 1] / (2.0 * p);
-      if (t + 1 <= order) v += (t + 1) * prev[t + 1];
-      next[t] = v;
-    }
-    order = no;
-    for (int t = 0; t <= order; ++t) prev[t] = next[t];
-  }
-  for (int t = 0; t < 12; ++t) E[t] = t <= order ? prev[t] : 0.0;
-}
-
-__device__ void boys_g(int L, double T, const double* __restrict__ g, double* F) {
-  if (T < BOYS_TMAX) {
-    const int k = __double2int_rn(T * BOYS_INV_DT);
-    const double d = -(T - k * BOYS_DT);
-    const double inv_fact[BOYS_TAYLOR] = {1.0, 1.0, 0.5, 1.0 / 6, 1.0 / 24, 1.0 / 120, 1.0 / 720, 1.0 / 5040};
-    double f = 0.0;
-    for (int j = BOYS_TAYLOR - 1; j >= 0; --j) f = fma(f, d, g[k * BOYS_MT + L + j] * inv_fact[j]);
-    F[L] = f;
-    const double e = exp(-T);
-    for (int m = L - 1; m >= 0; --m) F[m] = (2.0 * T * F[m + 1] + e) / (2.0 * m + 1.0);
-  } else {
-    F[0] = 0.88622692545275801365 / sqrt(T);
-    const double e = exp(-T);
-    for (int m = 0; m < L; ++m) F[m + 1] = ((2 * m + 1) * F[m] - e) / (2.0 * T);
-  }
-}
-
-__device__ void herm_g(int L, const double* F, double m2a, double X, double Y, double Z, double* R0) {
-  double lev[2][nherm(6)];
-  double s[7], pw = 1.0;
-  for (int n = 0; n <= L; ++n) { s[n] = F[n] * pw; pw *= m2a; }
-  int cur = 0;
-  lev[0][0] = s[L];
-  for (int n = L - 1; n >= 0; --n) {
-    const int S = L - n, nx = cur ^ 1;
-    lev[nx][0] = s[n];
-    for (int h = 1; h < nherm(S); ++h) {
-      const int t = h_t(h), u = h_u(h), v = h_v(h);
-      double r;
-      if (t > 0) {
-        r = X * lev[cur][hidx(t - 1, u, v)];
-        if (t > 1) r += (t - 1) * lev[cur][hidx(t - 2, u, v)];
-      } else if (u > 0) {
-        r = Y * lev[cur][hidx(t, u - 1, v)];
-        if (u > 1) r += (u - 1) * lev[cur][hidx(t, u - 2, v)];
-      } else {
-        r = Z * lev[cur][hidx(t, u, v - 1)];
-        if (v > 1) r += (v - 1) * lev[cur][hidx(t, u, v - 2)];
-      }
-      lev[nx][h] = r;
-    }
-    cur = nx;
-  }
-  for (int h = 0; h < nherm(L); ++h) R0[h] = lev[cur][h];
-}
-
-// Contribution of (shell pair pr, nucleus cn, component pair comp) to the
-// three xyz gradients: of the atom of A, of B and of nucleus cn (acc[0..8]).
-// For cn == 0 the S (with W) and T (with P) derivatives are added to the A
-// and B parts.
-__device__ void grad1e_item(int2 pr, int cn, int comp, const ShellDev* __restrict__ sh,
-                            const double* __restrict__ exps, const double* __restrict__ coefs,
-                            const double* __restrict__ atoms, const double* __restrict__ P,
-                            const double* __restrict__ W, int nbf, const double* __restrict__ gtab, double* acc) {
-  const ShellDev A = sh[pr.x], B = sh[pr.y];
-  const int nb = ncart(B.l);
-  {
-    const int ca = comp / nb, cb = comp % nb;
-    int pa[3], pb[3];
-    for (int d = 0; d < 3; ++d) {
-      pa[d] = comp_of(A.l, ca, d);
-      pb[d] = comp_of(B.l, cb, d);
-    }
-    const long long r = A.ao + ca, c = B.ao + cb;
-    // off-diagonal shell pairs stand for both (ab) and (ba)
-    const double sym = pr.x == pr.y ? 1.0 : 2.0;
-    const double Pab = sym * P[r * nbf + c], Wab = sym * W[r * nbf + c];
-    const double Ac[3] = {A.x, A.y, A.z}, Bc[3] = {B.x, B.y, B.z};
-    const double Cx = atoms[4 * cn], Cy = atoms[4 * cn + 1], Cz = atoms[4 * cn + 2], Zc = atoms[4 * cn + 3];
-    for (int i = 0; i < A.nprim; ++i)
-      for (int j = 0; j < B.nprim; ++j) {
-        const double al = exps[A.prim_off + i], be = exps[B.prim_off + j];
-        const double cc = coefs[A.prim_off + i] * coefs[B.prim_off + j];
-        const double p = al + be;
-        const double sq = sqrt(M_PI / p);
-        // E[d][k][m][t]: E_t^{pa+k-1, pb+m-2} along axis d (k: -1..+1, m: -2..+3)
-        double E[3][3][6][12];
-        for (int d = 0; d < 3; ++d)
-          for (int k = 0; k < 3; ++k)
-            for (int m = 0; m < 6; ++m) {
-              const int ia = pa[d] + k - 1, ib = pb[d] + m - 2;
-              if (ia < 0 || ib < 0)
-                for (int t = 0; t < 12; ++t) E[d][k][m][t] = 0.0;
-              else
-                e1d_g(al, be, Ac[d] - Bc[d], ia, ib, E[d][k][m]);
-            }
-        // 1-D overlap S1(k, m) and kinetic piece T1 (shelled.py:149-155) at offsets
-        auto S1 = [&](int d, int k, int m) { return E[d][k][m][0] * sq; };
-        auto T1 = [&](int d, int k) {  // k: a offset, b at its own value (m = 2)
-          const double bj = pb[d];
-          double t = -2.0 * be * (2 * bj + 1) * S1(d, k, 2) + 4.0 * be * be * S1(d, k, 4);
-          if (pb[d] >= 2) t += bj * (bj - 1) * S1(d, k, 0);
-          return t;
-        };
-        if (cn == 0) {
-          for (int d = 0; d < 3; ++d) {
-            const int o1 = (d + 1) % 3, o2 = (d + 2) % 3;
-            const double dS = 2.0 * al * S1(d, 2, 2) - (pa[d] > 0 ? pa[d] * S1(d, 0, 2) : 0.0);
-            const double dT = 2.0 * al * T1(d, 2) - (pa[d] > 0 ? pa[d] * T1(d, 0) : 0.0);
-            const double s1 = S1(o1, 1, 2), s2 = S1(o2, 1, 2);
-            const double gS = dS * s1 * s2;
-            const double gT = -0.5 * (dT * s1 * s2 + dS * (T1(o1, 1) * s2 + s1 * T1(o2, 1)));
-            const double v = cc * (Pab * gT - Wab * gS);
-            acc[d] += v;      // d/dA
-            acc[3 + d] -= v;  // d/dB = -d/dA for two-centre integrals
-          }
-        }
-        // nuclear attraction of nucleus cn: basis-centre derivatives
-        const double Pp[3] = {(al * A.x + be * B.x) / p, (al * A.y + be * B.y) / p, (al * A.z + be * B.z) / p};
-        const double X = Pp[0] - Cx, Y = Pp[1] - Cy, Z = Pp[2] - Cz;
-        const int L = A.l + B.l + 1;
-        double F[8], R0[nherm(6)];
-        boys_g(L, p * (X * X + Y * Y + Z * Z), gtab, F);
-        herm_g(L, F, -2.0 * p, X, Y, Z, R0);
-        const double pre = -2.0 * M_PI / p * Zc * cc * Pab;
-        for (int d = 0; d < 3; ++d) {
-          const int o1 = (d + 1) % 3, o2 = (d + 2) % 3;
-          double sA = 0.0, sB = 0.0;
-          for (int t = 0; t <= pa[d] + pb[d] + 1; ++t) {
-            const double dA = 2.0 * al * E[d][2][2][t] - (pa[d] > 0 ? pa[d] * E[d][0][2][t] : 0.0);
-            const double dB = 2.0 * be * E[d][1][3][t] - (pb[d] > 0 ? pb[d] * E[d][1][1][t] : 0.0);
-            for (int u = 0; u <= pa[o1] + pb[o1]; ++u)
-              for (int v = 0; v <= pa[o2] + pb[o2]; ++v) {
-                int hh[3];
-                hh[d] = t;
-                hh[o1] = u;
-                hh[o2] = v;
-                const double rv = R0[hidx(hh[0], hh[1], hh[2])] * E[o1][1][2][u] * E[o2][1][2][v];
-                sA = fma(dA, rv, sA);
-                sB = fma(dB, rv, sB);
-              }
-          }
-          acc[d] += pre * sA;
-          acc[3 + d] += pre * sB;
-          acc[6 + d] -= pre * (sA + sB);
-        }
-      }
-  }
-}
-
-// One 64-lane block per task {pair, first nucleus c0, nuclei g, components
-// nc}: lane = item * nc + comp for item < g (nucleus c0 + item), so the
-// lanes of a block cover g nuclei of one shell pair. Each (pair, nucleus)
-// sums its components in a fixed order and writes three xyz contributions
-// (atom of A, atom of B, nucleus) to its slots 3 (pair * natom + nucleus).
-__global__ __launch_bounds__(64) void grad1e_kernel(const int4* __restrict__ tasks, const int2* __restrict__ sp,
-                                                    const ShellDev* __restrict__ sh, const double* __restrict__ exps,
-                                                    const double* __restrict__ coefs, int natom,
-                                                    const double* __restrict__ atoms, const double* __restrict__ P,
-                                                    const double* __restrict__ W, int nbf,
-                                                    const long long* __restrict__ slots, double* __restrict__ stage,
-                                                    const double* __restrict__ gtab) {
-  __shared__ double red[64][9];
-  const int4 tk = tasks[blockIdx.x];
-  const int lane = threadIdx.x;
-  const int item = lane / tk.w, comp = lane - item * tk.w;
-  const int cn = tk.y + item;
-  double acc[9];
-  for (int k = 0; k < 9; ++k) acc[k] = 0.0;
-  if (item < tk.z && cn < natom)
-    grad1e_item(sp[tk.x], cn, comp, sh, exps, coefs, atoms, P, W, nbf, gtab, acc);
-  for (int k = 0; k < 9; ++k) red[lane][k] = acc[k];
-  __syncthreads();
-  if (comp == 0 && item < tk.z && cn < natom) {
-    double tot[9];
-    for (int k = 0; k < 9; ++k) {
-      double v = 0.0;
-      for (int c = 0; c < tk.w; ++c) v += red[lane + c][k];
-      tot[k] = v;
-    }
-    const long long* sl = slots + 3 * ((long long)tk.x * natom + cn);
-    for (int x = 0; x < 3; ++x) {
-      stage[sl[0] + x] = tot[x];
-      stage[sl[1] + x] = tot[3 + x];
-      stage[sl[2] + x] = tot[6 + x];
-    }
-  }
-}
-
-hipError_t launch_grad1e(const int4* tasks, int ntask, const int2* sp, const ShellDev* sh, const double* exps,
-                         const double* coefs, int natom, const double* atoms, const double* P, const double* W,
-                         int nbf, const long long* slots, double* stage, const double* gtab, hipStream_t s) {
-  if (ntask <= 0 || natom <= 0) return hipSuccess;
-  hipLaunchKernelGGL(grad1e_kernel, dim3(ntask), dim3(64), 0, s, tasks, sp, sh, exps, coefs, natom, atoms, P, W, nbf,
-                     slots, stage, gtab);
-  return hipGetLastError();
-}
-
 // Per-atom sums of the staged 3-vectors in two fixed-order stages. Stage 1:
 // one wave per chunk of at most GRAD_CHUNK slots of one atom; lanes 0..62
 // read elements lane + 63 i (component lane % 3) into four partial sums,

@@ -235,175 +235,6 @@ hipError_t launch_symmetrize(int nmol, const long long* mbase, const int* mnbf, 
   return hipGetLastError();
 }
 
-// ------------------------------------------------ run-time-L helpers
-// Boys from the global table (no LDS), for the one-time 1-e kernel.
-__device__ void boys_rt(int L, double T, const double* __restrict__ g, double* F) {
-  if (T < BOYS_TMAX) {
-    const int k = __double2int_rn(T * BOYS_INV_DT);
-    const double d = -(T - k * BOYS_DT);
-    const double inv_fact[BOYS_TAYLOR] = {1.0, 1.0, 0.5, 1.0 / 6, 1.0 / 24, 1.0 / 120, 1.0 / 720, 1.0 / 5040};
-    double f = 0.0;
-    for (int j = BOYS_TAYLOR - 1; j >= 0; --j) f = fma(f, d, g[k * BOYS_MT + L + j] * inv_fact[j]);
-    F[L] = f;
-    const double e = exp(-T);
-    for (int m = L - 1; m >= 0; --m) F[m] = (2.0 * T * F[m + 1] + e) / (2.0 * m + 1.0);
-  } else {
-    F[0] = 0.88622692545275801365 / sqrt(T);
-    const double e = exp(-T);
-    for (int m = 0; m < L; ++m) F[m + 1] = ((2 * m + 1) * F[m] - e) / (2.0 * T);
-  }
-}
-
-// R^0_tuv for t+u+v <= L (L <= 8) from F, run-time L.
-__device__ void hermite_R_rt(int L, const double* F, double m2a, double X, double Y, double Z, double* R0) {
-  double lev[2][nherm(8)];
-  double pw = 1.0, s[9];
-  for (int n = 0; n <= L; ++n) { s[n] = F[n] * pw; pw *= m2a; }
-  int cur = 0;
-  lev[cur][0] = s[L];
-  for (int n = L - 1; n >= 0; --n) {
-    const int S = L - n, nxt = cur ^ 1;
-    lev[nxt][0] = s[n];
-    for (int h = 1; h < nherm(S); ++h) {
-      const int t = h_t(h), u = h_u(h), v = h_v(h);
-      double r;
-      if (t > 0) {
-        r = X * lev[cur][hidx(t - 1, u, v)];
-        if (t > 1) r += (t - 1) * lev[cur][hidx(t - 2, u, v)];
-      } else if (u > 0) {
-        r = Y * lev[cur][hidx(t, u - 1, v)];
-        if (u > 1) r += (u - 1) * lev[cur][hidx(t, u - 2, v)];
-      } else {
-        r = Z * lev[cur][hidx(t, u, v - 1)];
-        if (v > 1) r += (v - 1) * lev[cur][hidx(t, u, v - 2)];
-      }
-      lev[nxt][h] = r;
-    }
-    cur = nxt;
-  }
-  for (int h = 0; h < nherm(L); ++h) R0[h] = lev[cur][h];
-}
-
-// 1-D E_t^{ia,ib} with E_0^{00} = exp(-mu Q^2), as calc_E (shelled.py:228-294)
-__device__ void e1d(double a, double b, double Q, int ia, int ib, double* E /* >= ia+ib+1 */) {
-  const double p = a + b;
-  double prev[12], next[12];
-  for (int t = 0; t < 12; ++t) prev[t] = 0.0;
-  prev[0] = exp(-(a * b / p) * Q * Q);
-  int order = 0;
-  for (int step = 0; step < ia + ib; ++step) {
-    const double coef = step < ia ? -(b * Q / p) : (a * Q / p);
-    const int no = order + 1;
-    for (int t = 0; t <= no; ++t) {
-      double v = 0.0;
-      if (t <= order) v += coef * prev[t];
-      if (t > 0) v += prev[t - 1] / (2.0 * p);
-      if (t + 1 <= order) v += (t + 1) * prev[t + 1];
-      next[t] = v;
-    }
-    order = no;
-    for (int t = 0; t <= order; ++t) prev[t] = next[t];
-  }
-  for (int t = 0; t <= order; ++t) E[t] = prev[t];
-}
-
-// ------------------------------------------------------- S, T, V
-// One wave per shell pair, lane = component pair. S from E_0 sqrt(pi/p) per
-// axis (equal to the binomial closed form of shelled.py:104-146), T from the
-// per-axis kinetic piece of T_raw (shelled.py:149-155) and V from the
-// nuclear_kernel sum (shelled.py:413-436). Normalisation and contraction as
-// shelled.py:976-1045; the caller's coefs already carry the radial part.
-__global__ void one_electron_kernel(const int2* __restrict__ sp, int npairs, const ShellDev* __restrict__ sh,
-                                    const double* __restrict__ exps, const double* __restrict__ coefs,
-                                    const double* __restrict__ atoms, double* __restrict__ S,
-                                    double* __restrict__ T, double* __restrict__ V,
-                                    const double* __restrict__ gtab) {
-  // lane = item * ncomp + comp: the block's lanes cover the component
-  // pairs of one shell pair and g = 64 / ncomp interleaved subsets of the
-  // nuclei (lane item takes nuclei item, item + g, ...); the V parts are
-  // summed over the subsets in a fixed order through LDS
-  __shared__ double vred[64];
-  const int pidx = blockIdx.x;
-  const int lane = threadIdx.x;
-  if (pidx >= npairs) return;
-  const int2 pr = sp[pidx];
-  const ShellDev A = sh[pr.x], B = sh[pr.y];
-  const int na = ncart(A.l), nb = ncart(B.l);
-  const int ncomp = na * nb, g = 64 / ncomp;
-  const int item = lane / ncomp, comp = lane - item * ncomp;
-  const bool active = item < g;
-  const int ca = (active ? comp : 0) / nb, cb = (active ? comp : 0) % nb;
-  int pa[3], pb[3];
-  for (int d = 0; d < 3; ++d) { pa[d] = comp_of(A.l, ca, d); pb[d] = comp_of(B.l, cb, d); }
-  const double Ac[3] = {A.x, A.y, A.z}, Bc[3] = {B.x, B.y, B.z};
-  const double dfac[5] = {1.0, 1.0, 3.0, 15.0, 105.0};
-  double nang = 1.0;
-  for (int d = 0; d < 3; ++d) nang /= sqrt(dfac[pa[d]] * dfac[pb[d]]);
-  double s_acc = 0.0, t_acc = 0.0, v_acc = 0.0;
-  for (int i = 0; i < (active ? A.nprim : 0); ++i) {
-    for (int j = 0; j < B.nprim; ++j) {
-      const double a = exps[A.prim_off + i], b = exps[B.prim_off + j];
-      const double cc = coefs[A.prim_off + i] * coefs[B.prim_off + j] * nang;
-      const double p = a + b;
-      const double sq = sqrt(M_PI / p);
-      double E[3][12], S1[3], T1[3];
-      for (int d = 0; d < 3; ++d) {
-        const double Q = Ac[d] - Bc[d];
-        double Et[12];
-        e1d(a, b, Q, pa[d], pb[d], E[d]);
-        S1[d] = E[d][0] * sq;
-        e1d(a, b, Q, pa[d], pb[d] + 2, Et);
-        double tk = -2.0 * b * (2 * pb[d] + 1) * S1[d] + 4.0 * b * b * Et[0] * sq;
-        if (pb[d] >= 2) {
-          e1d(a, b, Q, pa[d], pb[d] - 2, Et);
-          tk += pb[d] * (pb[d] - 1) * Et[0] * sq;
-        }
-        T1[d] = tk;
-      }
-      s_acc += cc * S1[0] * S1[1] * S1[2];
-      t_acc += cc * -0.5 * (T1[0] * S1[1] * S1[2] + S1[0] * T1[1] * S1[2] + S1[0] * S1[1] * T1[2]);
-      // nuclear attraction
-      const int L = A.l + B.l;
-      const double P[3] = {(a * A.x + b * B.x) / p, (a * A.y + b * B.y) / p, (a * A.z + b * B.z) / p};
-      double vsum = 0.0;
-      for (int c = A.atom_begin + item; c < A.atom_end; c += g) {
-        const double X = P[0] - atoms[4 * c], Y = P[1] - atoms[4 * c + 1], Z = P[2] - atoms[4 * c + 2];
-        double F[9], R0[nherm(8)];
-        boys_rt(L, p * (X * X + Y * Y + Z * Z), gtab, F);
-        hermite_R_rt(L, F, -2.0 * p, X, Y, Z, R0);
-        double cs = 0.0;
-        for (int t = 0; t <= pa[0] + pb[0]; ++t)
-          for (int u = 0; u <= pa[1] + pb[1]; ++u)
-            for (int v = 0; v <= pa[2] + pb[2]; ++v) cs += E[0][t] * E[1][u] * E[2][v] * R0[hidx(t, u, v)];
-        vsum += atoms[4 * c + 3] * cs;
-      }
-      v_acc += cc * vsum * (-2.0 * M_PI / p);
-    }
-  }
-  vred[lane] = v_acc;
-  __syncthreads();
-  if (!active || item != 0) return;
-  v_acc = 0.0;
-  for (int it = 0; it < g; ++it) v_acc += vred[it * ncomp + comp];
-  const long long r = A.ao + ca, c = B.ao + cb;
-  const long long nbf = A.nbf, rc = A.mbase + r * nbf + c, cr = A.mbase + c * nbf + r;
-  S[rc] = s_acc;
-  S[cr] = s_acc;
-  T[rc] = t_acc;
-  T[cr] = t_acc;
-  V[rc] = v_acc;
-  V[cr] = v_acc;
-}
-
-hipError_t launch_one_electron(const int2* sp, int npairs, const ShellDev* sh, const double* exps,
-                               const double* coefs, const double* atoms, double* S, double* T, double* V,
-                               const double* gtab, hipStream_t s) {
-  if (npairs <= 0) return hipSuccess;
-  hipLaunchKernelGGL(one_electron_kernel, dim3(npairs), dim3(64), 0, s, sp, npairs, sh, exps, coefs, atoms, S, T, V,
-                     gtab);
-  return hipGetLastError();
-}
-
 // ------------------------------------------------------- Boys hook
 template <int L>
 __global__ void boys_kernel(const double* __restrict__ T, int n, double* __restrict__ out, int fp32,

@@ -206,6 +206,43 @@ __device__ __forceinline__ void boys_eval(real T, const double* __restrict__ tab
   else boys_fp32<L>(T, tab, F);
 }
 
+// Boys F_0..F_L from the global table (F_m(T_k) at BOYS_MT values per grid
+// point): eight-term Taylor step at m = L, e^{-T}, downward recursion; the
+// asymptotic form above BOYS_TMAX (as boys_fp64, which reads the LDS copy).
+template <int L>
+__device__ __forceinline__ void boys_global(double T, const double* __restrict__ g, double (&F)[L + 1]) {
+  if (T < BOYS_TMAX) {
+    const int k = boys_point(T);
+    const double d = -(T - k * BOYS_DT);
+    const double* c = g + k * BOYS_MT + L;
+    constexpr double inv_fact[8] = {1.0, 1.0, 0.5, 1.0 / 6, 1.0 / 24, 1.0 / 120, 1.0 / 720, 1.0 / 5040};
+    double f = c[BOYS_TAYLOR - 1] * inv_fact[BOYS_TAYLOR - 1];
+#pragma unroll
+    for (int j = BOYS_TAYLOR - 2; j >= 0; --j) f = fma(f, d, c[j] * inv_fact[j]);
+    F[L] = f;
+    if constexpr (L > 0) {
+      const double e = exp(-T);
+      const double t2 = T + T;
+#pragma unroll
+      for (int m = L - 1; m >= 0; --m) F[m] = fma(t2, F[m + 1], e) * inv_odd(m);
+    }
+  } else {
+    const double rs = rsqrt_nr(T);
+    F[0] = 0.88622692545275801365 * rs;
+    if constexpr (L > 0) {
+      const double h = 0.5 * rs * rs;
+      if (L <= 1 || T >= boys_tnoexp(L)) {
+#pragma unroll
+        for (int m = 0; m < L; ++m) F[m + 1] = (2 * m + 1) * F[m] * h;
+      } else {
+        const double e = exp(-T);
+#pragma unroll
+        for (int m = 0; m < L; ++m) F[m + 1] = ((2 * m + 1) * F[m] - e) * h;
+      }
+    }
+  }
+}
+
 // Stage the rows F_{L+j}(T_k)/j!, e^{-T_k} for all k into LDS (cooperative,
 // whole block).
 template <int L>
