@@ -2071,7 +2071,9 @@ int dft_syevj(int n, int batch, double* A, double* w, double* V, int init_v, int
 int dft_scf_diis_stage(int n, int history, const double* F, const double* R, double* hist_F, double* hist_R,
                        double* G, const double* P, const double* H, const double* Pa, double* state, double* Fu,
                        void* stream) {
-  static_assert(DFT_SCF_HMAX == SCF_HMAX && DFT_SCF_ST_SLOT == SCF_ST_SLOT, "state layout");
+  static_assert(DFT_SCF_HMAX == SCF_HMAX && DFT_SCF_ST_SLOT == SCF_ST_SLOT && DFT_SCF_ST_COEF == SCF_ST_COEF &&
+                    DFT_SCF_STATE_LEN == SCF_STATE_LEN,
+                "state layout");
   if (n <= 0 || n > JAC_NMAX) return fail(DFT_EUNSUPPORTED, "dft_scf_diis_stage: order must be in [1, 128]");
   if (history < 1 || history > DFT_SCF_HMAX) return fail(DFT_EINVAL, "history must be in [1, 16]");
   if (!F || !R || !hist_F || !hist_R || !G || !P || !H || !Pa || !state || !Fu) return fail(DFT_EINVAL, "NULL argument");

@@ -196,7 +196,10 @@ enum ScfState {
   SCF_ST_ESUM = 0, SCF_ST_ENN = 1, SCF_ST_E = 2, SCF_ST_HAS_E = 3, SCF_ST_DE = 4, SCF_ST_DP = 5,
   SCF_ST_PREV_DE = 6, SCF_ST_PREV_DP = 7, SCF_ST_RATE = 8, SCF_ST_STRIKES = 9, SCF_ST_USE_DIIS = 10,
   SCF_ST_COUNT = 11, SCF_ST_CONV = 12, SCF_ST_ETOL = 13, SCF_ST_PTOL = 14, SCF_ST_DIIS_TH = 15,
-  SCF_ST_ENEW = 16, SCF_ST_SLOT = 17
+  SCF_ST_ENEW = 16, SCF_ST_SLOT = 17,
+  // DIIS coefficients (SCF_HMAX), then per-workgroup partial sums of the
+  // element-wise passes
+  SCF_ST_COEF = 32, SCF_ST_PART = 48, SCF_STATE_LEN = 1152
 };
 hipError_t launch_scf_diis_stage(int n, int h, const double* F, const double* R, double* histF, double* histR,
                                  double* G, const double* P, const double* H, const double* Pa, double* st,

@@ -7,7 +7,7 @@
 // shelled.py:1082 / main.py:754) for matrices up to JAC_NMAX. The matrix
 // lives in LDS; the eigenvectors in LDS as well when both fit, else in the
 // caller's output buffer (global memory, L2-resident at these sizes). The
-// matrix is stored as its packed upper triangle, so both fit up to n = 116.
+// matrix is stored as its packed upper triangle, so both fit up to n = 114.
 //
 // Method: Golub & Van Loan, Matrix Computations (4th ed.), section 8.5: the
 // symmetric 2x2 Schur decomposition (tau = (a_qq - a_pp) / 2 a_pq,
@@ -18,8 +18,10 @@
 // round. Rotations with |a_pq| below m 2^-52 ||A||_F are skipped (rounding
 // keeps the off-diagonal entries of a converged matrix near 2^-52 ||A|| times
 // the square root of the rotations that touched them, so a bound without the
-// factor m can be out of reach); the sweeps stop after one without any
-// rotation (at most JAC_MAX_SWEEPS). Eigenvalues
+// factor m can be out of reach); the sweeps stop when no off-diagonal entry
+// is above that bound, checked in one pass before each sweep (a sweep with
+// nothing to rotate is never run), or after a sweep without any rotation
+// (at most JAC_MAX_SWEEPS). Eigenvalues
 // are returned in ascending order, eigenvectors as the columns of V.
 #include <hip/hip_runtime.h>
 
@@ -29,11 +31,13 @@
 
 namespace dft {
 
-constexpr int JAC_THREADS = 512;
+constexpr int JAC_THREADS = 1024;
 constexpr int JAC_MAX_SWEEPS = 40;
-// LDS of one workgroup, and the matrix order up to which A and V share it
+// LDS of one workgroup, and the part of it the kernel's static arrays take
+// (rotation tables of two rounds, diagonal, reduction: 4.2 KB)
 constexpr int JAC_LDS_BYTES = 160 * 1024;
-// per-thread share of a round's blocks (h <= 64 pairs): ceil(64 * 65 / 2 / 512)
+constexpr int JAC_STATIC_LDS = 5 * 1024;
+// per-thread share of a round's blocks (h <= 64 pairs): ceil(64 * 65 / 2 / JAC_THREADS)
 constexpr int JAC_BLK_PER_THREAD = (JAC_NMAX / 2 * (JAC_NMAX / 2 + 1) / 2 + JAC_THREADS - 1) / JAC_THREADS;
 constexpr int JAC_V_GROUP = 4;
 
@@ -57,8 +61,8 @@ __global__ __launch_bounds__(JAC_THREADS) void jacobi_eigh_kernel(int n, const d
                                                                   double* __restrict__ V, long long sv,
                                                                   int init_v, int* __restrict__ sweeps) {
   extern __shared__ double lds[];
-  __shared__ double cs_c[JAC_NMAX / 2], cs_s[JAC_NMAX / 2];
-  __shared__ int cs_p[JAC_NMAX / 2], cs_q[JAC_NMAX / 2];
+  __shared__ double cs_c[2][JAC_NMAX / 2], cs_s[2][JAC_NMAX / 2];
+  __shared__ int cs_p[2][JAC_NMAX / 2], cs_q[2][JAC_NMAX / 2];
   __shared__ double red[JAC_THREADS / 64];
   __shared__ double diag[JAC_NMAX];
   __shared__ int rotated;
@@ -109,93 +113,66 @@ __global__ __launch_bounds__(JAC_THREADS) void jacobi_eigh_kernel(int n, const d
     }
     bkl[nb++] = k | (k + rem) << 8;
   }
-  const int vng = nt / h;  // row groups of the eigenvector update
-  const int vk = tid < vng * h ? tid % h : -1, vg = tid / h;
-  int sweep = 0;
-  for (; sweep < JAC_MAX_SWEEPS; ++sweep) {
-    for (int r = 0; r < m - 1; ++r) {
-      for (int k = tid; k < h; k += nt) {
-        int p = rr_pos(k, r, m), q = rr_pos(m - 1 - k, r, m);
-        if (p > q) {
-          const int t = p;
-          p = q;
-          q = t;
-        }
-        const double apq = a[tri(p, q, m)];
-        double c = 1.0, s = 0.0;
-        if (fabs(apq) > thresh) {
-          const double tau = (a[tri(q, q, m)] - a[tri(p, p, m)]) / (2.0 * apq);
-          const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-          c = 1.0 / sqrt(1.0 + t * t);
-          s = t * c;
-          rotated = 1;
-        }
-        cs_c[k] = c;
-        cs_s[k] = s;
-        cs_p[k] = p;
-        cs_q[k] = q;
+  // eigenvector update by the threads of waves 1.. (wave 0 computes the
+  // next round's rotations meanwhile): vng row groups per pair
+  const int vu = tid - 64, vng = (nt - 64) / h;
+  const int vk = (vu >= 0 && vu < vng * h) ? vu % h : -1, vg = vu >= 0 ? vu / h : 0;
+  int done = 0;  // sweeps performed
+  for (int sweep = 0; sweep < JAC_MAX_SWEEPS; ++sweep) {
+    // a sweep that would rotate no pair (every |a_pq| <= thresh) changes
+    // nothing: stop before it, from one pass over the off-diagonal entries
+    {
+      double om = 0.0;
+      for (int i = tid; i < m * m; i += nt) {
+        const int r = i / m, c = i - r * m;
+        if (r < c) om = fmax(om, fabs(a[tri(r, c, m)]));
       }
+      for (int off = 32; off > 0; off >>= 1) om = fmax(om, __shfl_xor(om, off, 64));
       __syncthreads();
-      // blocks {p_k, q_k} x {p_l, q_l} with k <= l (the others are their
-      // transposes). The entries a thread touches in a round are its own, so
-      // all of them are loaded first and stored after the arithmetic (the
-      // compiler cannot see that the stores do not alias later loads).
-      {
-        double x[JAC_BLK_PER_THREAD][4];
-        int ix[JAC_BLK_PER_THREAD][4];
-#pragma unroll
-        for (int j = 0; j < JAC_BLK_PER_THREAD; ++j) {
-          if (j < nb) {
-            const int k = bkl[j] & 255, l = bkl[j] >> 8;
-            const int pk = cs_p[k], qk = cs_q[k], pl = cs_p[l], ql = cs_q[l];
-            if (k == l) {
-              ix[j][0] = tri(pk, pk, m);
-              ix[j][1] = tri(pk, qk, m);
-              ix[j][2] = tri(qk, qk, m);
-              ix[j][3] = ix[j][1];
-            } else {
-              ix[j][0] = tri(pk, pl, m);
-              ix[j][1] = tri(pk, ql, m);
-              ix[j][2] = tri(qk, pl, m);
-              ix[j][3] = tri(qk, ql, m);
+      if ((tid & 63) == 0) red[tid >> 6] = om;
+      __syncthreads();
+      double mx = 0.0;
+      for (int k = 0; k < (nt >> 6); ++k) mx = fmax(mx, red[k]);
+      if (!(mx > thresh)) break;
+    }
+    // Round r: wave 0 computes the rotations of round r (disjoint pairs)
+    // while the other waves apply round r - 1's rotations to the
+    // eigenvector columns (double-buffered rotation tables); then every
+    // thread updates its blocks of A with round r's rotations. Pass m - 1
+    // only finishes the eigenvectors of the sweep's last round.
+    for (int r = 0; r < m; ++r) {
+      const int cur = r & 1, prev = cur ^ 1;
+      if (tid < 64) {
+        if (r < m - 1)
+          for (int k = tid; k < h; k += 64) {
+            int p = rr_pos(k, r, m), q = rr_pos(m - 1 - k, r, m);
+            if (p > q) {
+              const int t = p;
+              p = q;
+              q = t;
             }
-#pragma unroll
-            for (int e = 0; e < 4; ++e) x[j][e] = a[ix[j][e]];
-          }
-        }
-#pragma unroll
-        for (int j = 0; j < JAC_BLK_PER_THREAD; ++j) {
-          if (j < nb) {
-            const int k = bkl[j] & 255, l = bkl[j] >> 8;
-            const double sk = cs_s[k], sl = cs_s[l];
-            if (sk == 0.0 && sl == 0.0) continue;
-            const double ck = cs_c[k], cl = cs_c[l];
-            if (k == l) {
-              // the block the rotation diagonalises (t = s / c): a_pp - t a_pq, a_qq + t a_pq, 0
-              const double t = sk / ck, apq = x[j][1];
-              a[ix[j][0]] = x[j][0] - t * apq;
-              a[ix[j][2]] = x[j][2] + t * apq;
-              a[ix[j][1]] = 0.0;
-              continue;
+            const double apq = a[tri(p, q, m)];
+            double c = 1.0, s = 0.0;
+            if (fabs(apq) > thresh) {
+              const double tau = (a[tri(q, q, m)] - a[tri(p, p, m)]) / (2.0 * apq);
+              const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+              c = 1.0 / sqrt(1.0 + t * t);
+              s = t * c;
+              rotated = 1;
             }
-            // rows (J^T from the left), then columns (J from the right)
-            const double tpp = ck * x[j][0] - sk * x[j][2], tpq = ck * x[j][1] - sk * x[j][3];
-            const double tqp = sk * x[j][0] + ck * x[j][2], tqq = sk * x[j][1] + ck * x[j][3];
-            a[ix[j][0]] = cl * tpp - sl * tpq;
-            a[ix[j][1]] = sl * tpp + cl * tpq;
-            a[ix[j][2]] = cl * tqp - sl * tqq;
-            a[ix[j][3]] = sl * tqp + cl * tqq;
+            cs_c[cur][k] = c;
+            cs_s[cur][k] = s;
+            cs_p[cur][k] = p;
+            cs_q[cur][k] = q;
           }
-        }
-      }
-      // eigenvector columns p_k, q_k of rows g, g + ng, ... for this
-      // thread's fixed pair k and row group g (its rotation read once), in
-      // groups of JAC_V_GROUP rows loaded before any is stored
-      if (vk >= 0) {
-        const double sr = cs_s[vk];
-        const int p = cs_p[vk], q = cs_q[vk];
+      } else if (r > 0 && vk >= 0) {
+        // eigenvector columns p_k, q_k of rows g, g + ng, ... for this
+        // thread's fixed pair k and row group g (its rotation read once), in
+        // groups of JAC_V_GROUP rows loaded before any is stored
+        const double sr = cs_s[prev][vk];
+        const int p = cs_p[prev][vk], q = cs_q[prev][vk];
         if (sr != 0.0 && q < n) {
-          const double c = cs_c[vk];
+          const double c = cs_c[prev][vk];
           for (int i0 = vg; i0 < n; i0 += JAC_V_GROUP * vng) {
             double xp[JAC_V_GROUP], xq[JAC_V_GROUP];
 #pragma unroll
@@ -224,14 +201,69 @@ __global__ __launch_bounds__(JAC_THREADS) void jacobi_eigh_kernel(int n, const d
         }
       }
       __syncthreads();
+      if (r == m - 1) break;
+      // blocks {p_k, q_k} x {p_l, q_l} with k <= l (the others are their
+      // transposes). The entries a thread touches in a round are its own, so
+      // all of them are loaded first and stored after the arithmetic (the
+      // compiler cannot see that the stores do not alias later loads).
+      {
+        double x[JAC_BLK_PER_THREAD][4];
+        int ix[JAC_BLK_PER_THREAD][4];
+#pragma unroll
+        for (int j = 0; j < JAC_BLK_PER_THREAD; ++j) {
+          if (j < nb) {
+            const int k = bkl[j] & 255, l = bkl[j] >> 8;
+            const int pk = cs_p[cur][k], qk = cs_q[cur][k], pl = cs_p[cur][l], ql = cs_q[cur][l];
+            if (k == l) {
+              ix[j][0] = tri(pk, pk, m);
+              ix[j][1] = tri(pk, qk, m);
+              ix[j][2] = tri(qk, qk, m);
+              ix[j][3] = ix[j][1];
+            } else {
+              ix[j][0] = tri(pk, pl, m);
+              ix[j][1] = tri(pk, ql, m);
+              ix[j][2] = tri(qk, pl, m);
+              ix[j][3] = tri(qk, ql, m);
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) x[j][e] = a[ix[j][e]];
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < JAC_BLK_PER_THREAD; ++j) {
+          if (j < nb) {
+            const int k = bkl[j] & 255, l = bkl[j] >> 8;
+            const double sk = cs_s[cur][k], sl = cs_s[cur][l];
+            if (sk == 0.0 && sl == 0.0) continue;
+            const double ck = cs_c[cur][k], cl = cs_c[cur][l];
+            if (k == l) {
+              // the block the rotation diagonalises (t = s / c): a_pp - t a_pq, a_qq + t a_pq, 0
+              const double t = sk / ck, apq = x[j][1];
+              a[ix[j][0]] = x[j][0] - t * apq;
+              a[ix[j][2]] = x[j][2] + t * apq;
+              a[ix[j][1]] = 0.0;
+              continue;
+            }
+            // rows (J^T from the left), then columns (J from the right)
+            const double tpp = ck * x[j][0] - sk * x[j][2], tpq = ck * x[j][1] - sk * x[j][3];
+            const double tqp = sk * x[j][0] + ck * x[j][2], tqq = sk * x[j][1] + ck * x[j][3];
+            a[ix[j][0]] = cl * tpp - sl * tpq;
+            a[ix[j][1]] = sl * tpp + cl * tpq;
+            a[ix[j][2]] = cl * tqp - sl * tqq;
+            a[ix[j][3]] = sl * tqp + cl * tqq;
+          }
+        }
+      }
+      __syncthreads();
     }
+    ++done;
     const int any = rotated;
     __syncthreads();
     if (tid == 0) rotated = 0;
     __syncthreads();
     if (!any) break;
   }
-  if (sweeps && tid == 0) sweeps[blockIdx.x] = sweep + 1;
+  if (sweeps && tid == 0) sweeps[blockIdx.x] = done;
   // ascending order: rank of each eigenvalue (ties by index)
   for (int i = tid; i < n; i += nt) diag[i] = a[tri(i, i, m)];
   __syncthreads();
@@ -262,14 +294,13 @@ hipError_t launch_syevj(int n, int batch, double* A, long long sa, double* w, lo
   if (n > JAC_NMAX) return hipErrorInvalidValue;
   const int m = n + (n & 1);
   const size_t a_bytes = (size_t)m * (m + 1) / 2 * sizeof(double), v_bytes = (size_t)n * n * sizeof(double);
-  // the kernel's static LDS (rotation tables, diagonal, reduction) stays below 4 KB
-  const bool vl = a_bytes + v_bytes + 4096 <= (size_t)JAC_LDS_BYTES;
+  const bool vl = a_bytes + v_bytes + JAC_STATIC_LDS <= (size_t)JAC_LDS_BYTES;
   // the staging of V for the final column permutation reuses the matrix area
   const size_t dyn = vl ? a_bytes + v_bytes : std::max(a_bytes, v_bytes);
   static bool attr_set = false;
   if (!attr_set) {
     for (const void* k : {(const void*)jacobi_eigh_kernel<true>, (const void*)jacobi_eigh_kernel<false>}) {
-      hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, JAC_LDS_BYTES - 4096);
+      hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, JAC_LDS_BYTES - JAC_STATIC_LDS);
       if (e != hipSuccess) return e;
     }
     attr_set = true;
@@ -323,46 +354,79 @@ __device__ __forceinline__ double block_max(double v, double* red) {
 // slots (B c = 1 without a Lagrange row, c normalised to sum 1; unused slots
 // get c = 0), and form the Fock matrix of this iteration Fu (the
 // extrapolation once DIIS is on, F before), and E_sum = sum P.H + 2 sum Pa.Fu.
-__global__ __launch_bounds__(SCF_THREADS) void scf_diis_stage_kernel(int n, int h, const double* __restrict__ F,
-                                                                     const double* __restrict__ R,
-                                                                     double* __restrict__ histF,
-                                                                     double* __restrict__ histR,
-                                                                     double* __restrict__ G,
-                                                                     const double* __restrict__ P,
-                                                                     const double* __restrict__ H,
-                                                                     const double* __restrict__ Pa,
-                                                                     double* __restrict__ st, double* __restrict__ Fu) {
-  __shared__ double red[SCF_THREADS / 64];
-  __shared__ double c[SCF_HMAX];
-  __shared__ double gauss[SCF_HMAX][SCF_HMAX + 1];
-  const int tid = threadIdx.x, nt = blockDim.x;
+// Four launches: the element-wise passes (history store + Gram row partials,
+// extrapolation + energy partials) spread over scf_nblk(n) workgroups, each
+// writing its partial sums to its own state slots; the partials are summed
+// in block order by single-workgroup kernels (results independent of timing).
+constexpr int SCF_EW_THREADS = 256;
+constexpr int SCF_NBLK_MAX = 64;
+constexpr int SCF_PART_STRIDE = SCF_HMAX + 1;  // per block: h Gram partials, then the energy partial
+static_assert(SCF_ST_PART + SCF_NBLK_MAX * SCF_PART_STRIDE <= SCF_STATE_LEN, "SCF state too short");
+
+__host__ __device__ inline int scf_nblk(int n) {
   const long long nn = (long long)n * n;
+  const long long b = (nn + 4 * SCF_EW_THREADS - 1) / (4 * SCF_EW_THREADS);
+  return (int)(b < 1 ? 1 : (b > SCF_NBLK_MAX ? SCF_NBLK_MAX : b));
+}
+
+// fixed-order sum over the block: wave butterflies, then the waves in order
+__device__ __forceinline__ double ew_block_sum(double v, double* red) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int k = 0; k < (int)(blockDim.x >> 6); ++k) s += red[k];
+  return s;
+}
+
+// elements [b * chunk, (b + 1) * chunk) of block b
+__device__ __forceinline__ void ew_range(long long nn, long long& i0, long long& i1) {
+  const long long chunk = (nn + gridDim.x - 1) / gridDim.x;
+  i0 = (long long)blockIdx.x * chunk;
+  i1 = i0 + chunk < nn ? i0 + chunk : nn;
+}
+
+__global__ __launch_bounds__(SCF_EW_THREADS) void scf_diis_dots_kernel(int n, int h, const double* __restrict__ F,
+                                                                       const double* __restrict__ R,
+                                                                       double* __restrict__ histF,
+                                                                       double* __restrict__ histR,
+                                                                       double* __restrict__ st) {
+  __shared__ double red[SCF_EW_THREADS / 64];
+  const long long nn = (long long)n * n;
+  const int slot = (int)st[SCF_ST_COUNT] % h;
+  long long i0, i1;
+  ew_range(nn, i0, i1);
+  double d[SCF_HMAX];
+#pragma unroll
+  for (int t = 0; t < SCF_HMAX; ++t) d[t] = 0.0;
+  for (long long i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    const double r = R[i];
+    histF[slot * nn + i] = F[i];
+    histR[slot * nn + i] = r;
+#pragma unroll
+    for (int t = 0; t < SCF_HMAX; ++t)
+      if (t < h) d[t] = fma(r, t == slot ? r : histR[t * nn + i], d[t]);
+  }
+  double* part = st + SCF_ST_PART + blockIdx.x * SCF_PART_STRIDE;
+  for (int t = 0; t < h; ++t) {
+    const double x = ew_block_sum(d[t], red);
+    if (threadIdx.x == 0) part[t] = x;
+  }
+}
+
+__global__ __launch_bounds__(64) void scf_diis_solve_kernel(int h, int nblk, double* __restrict__ G,
+                                                            double* __restrict__ st) {
+  __shared__ double gauss[SCF_HMAX][SCF_HMAX + 1];
+  const int tid = threadIdx.x;
   const int count = (int)st[SCF_ST_COUNT];
   const int slot = count % h;
   const int used = min(count + 1, h);
-  for (long long i = tid; i < nn; i += nt) {
-    histF[slot * nn + i] = F[i];
-    histR[slot * nn + i] = R[i];
-  }
-  __syncthreads();
-  // the new residual's dot products with every slot in one pass over R
-  {
-    double d[SCF_HMAX];
-#pragma unroll
-    for (int t = 0; t < SCF_HMAX; ++t) d[t] = 0.0;
-    for (long long i = tid; i < nn; i += nt) {
-      const double r = R[i];
-#pragma unroll
-      for (int t = 0; t < SCF_HMAX; ++t)
-        if (t < h) d[t] = fma(r, histR[t * nn + i], d[t]);
-    }
-    for (int t = 0; t < h; ++t) {
-      const double x = block_sum(d[t], red);
-      if (tid == 0) {
-        G[slot * h + t] = x;
-        G[t * h + slot] = x;
-      }
-    }
+  if (tid < h) {
+    double x = 0.0;
+    for (int b = 0; b < nblk; ++b) x += st[SCF_ST_PART + b * SCF_PART_STRIDE + tid];
+    G[slot * h + tid] = x;
+    G[tid * h + slot] = x;
   }
   __syncthreads();
   if (tid == 0) {
@@ -389,6 +453,7 @@ __global__ __launch_bounds__(SCF_THREADS) void scf_diis_stage_kernel(int n, int 
         for (int j = k; j <= h; ++j) M[i][j] -= f * M[k][j];
       }
     }
+    double* c = st + SCF_ST_COEF;
     double sum = 0.0;
     for (int k = h - 1; k >= 0; --k) {
       double x = M[k][h];
@@ -399,20 +464,43 @@ __global__ __launch_bounds__(SCF_THREADS) void scf_diis_stage_kernel(int n, int 
     for (int k = 0; k < h; ++k) c[k] /= sum;
     st[SCF_ST_SLOT] = slot;
   }
-  __syncthreads();
+}
+
+__global__ __launch_bounds__(SCF_EW_THREADS) void scf_diis_extrap_kernel(int n, int h, const double* __restrict__ F,
+                                                                         const double* __restrict__ histF,
+                                                                         const double* __restrict__ P,
+                                                                         const double* __restrict__ H,
+                                                                         const double* __restrict__ Pa,
+                                                                         double* __restrict__ st,
+                                                                         double* __restrict__ Fu) {
+  __shared__ double red[SCF_EW_THREADS / 64];
+  __shared__ double c[SCF_HMAX];
+  const long long nn = (long long)n * n;
+  const int used = min((int)st[SCF_ST_COUNT] + 1, h);
   const bool diis = st[SCF_ST_USE_DIIS] != 0.0;
-  for (long long i = tid; i < nn; i += nt) {
+  if (threadIdx.x < SCF_HMAX) c[threadIdx.x] = threadIdx.x < h ? st[SCF_ST_COEF + threadIdx.x] : 0.0;
+  __syncthreads();
+  long long i0, i1;
+  ew_range(nn, i0, i1);
+  double e = 0.0;
+  for (long long i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
     double x = F[i];
     if (diis) {
       x = 0.0;
       for (int t = 0; t < used; ++t) x = fma(c[t], histF[t * nn + i], x);
     }
     Fu[i] = x;
+    e = fma(P[i], H[i], e) + 2.0 * Pa[i] * x;
   }
+  e = ew_block_sum(e, red);
+  if (threadIdx.x == 0) st[SCF_ST_PART + blockIdx.x * SCF_PART_STRIDE + SCF_HMAX] = e;
+}
+
+__global__ void scf_energy_sum_kernel(int nblk, double* __restrict__ st) {
+  if (threadIdx.x != 0) return;
   double e = 0.0;
-  for (long long i = tid; i < nn; i += nt) e = fma(P[i], H[i], e) + 2.0 * Pa[i] * (diis ? Fu[i] : F[i]);
-  e = block_sum(e, red);
-  if (tid == 0) st[SCF_ST_ESUM] = e;
+  for (int b = 0; b < nblk; ++b) e += st[SCF_ST_PART + b * SCF_PART_STRIDE + SCF_HMAX];
+  st[SCF_ST_ESUM] = e;
 }
 
 // Energy and convergence of the iteration, the damped density
@@ -474,9 +562,12 @@ __global__ __launch_bounds__(SCF_THREADS) void scf_damp_control_kernel(int n, do
 hipError_t launch_scf_diis_stage(int n, int h, const double* F, const double* R, double* histF, double* histR,
                                  double* G, const double* P, const double* H, const double* Pa, double* st,
                                  double* Fu, hipStream_t s) {
-  if (h < 1 || h > SCF_HMAX) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(scf_diis_stage_kernel, dim3(1), dim3(SCF_THREADS), 0, s, n, h, F, R, histF, histR, G, P, H,
-                     Pa, st, Fu);
+  if (h < 1 || h > SCF_HMAX || n < 1) return hipErrorInvalidValue;
+  const int nb = scf_nblk(n);
+  hipLaunchKernelGGL(scf_diis_dots_kernel, dim3(nb), dim3(SCF_EW_THREADS), 0, s, n, h, F, R, histF, histR, st);
+  hipLaunchKernelGGL(scf_diis_solve_kernel, dim3(1), dim3(64), 0, s, h, nb, G, st);
+  hipLaunchKernelGGL(scf_diis_extrap_kernel, dim3(nb), dim3(SCF_EW_THREADS), 0, s, n, h, F, histF, P, H, Pa, st, Fu);
+  hipLaunchKernelGGL(scf_energy_sum_kernel, dim3(1), dim3(64), 0, s, nb, st);
   return hipGetLastError();
 }
 

@@ -343,7 +343,7 @@ def scf_diis(plan: Plan, Z, centers, N_a, N_b, mode="fp64", rate=0.4, diis_thres
 # Indices of the device SCF state (include/dft_hip.h, DFT_SCF_ST_*).
 ST_ESUM, ST_ENN, ST_E, ST_HAS_E, ST_DE, ST_DP, ST_PREV_DE, ST_PREV_DP, ST_RATE, ST_STRIKES, ST_USE_DIIS, \
     ST_COUNT, ST_CONV, ST_ETOL, ST_PTOL, ST_DIIS_TH, ST_ENEW, ST_SLOT = range(18)
-SCF_STATE_LEN = 32
+SCF_STATE_LEN = 1152
 
 
 def scf_diis_graph(plan: Plan, Z, centers, N_occ, mode="fp64", rate=0.4, diis_threshold=1e-3, history=10,

@@ -356,7 +356,8 @@ int dft_syevj(int n, int batch, double* A, double* w, double* V, int init_v, int
  * (<= DFT_SCF_HMAX) matrices, G the history x history Gram matrix, all
  * zero-initialised by the caller; state[DFT_SCF_ST_RATE, _ENN, _ETOL,
  * _PTOL, _DIIS_TH] are set by the caller, the other entries start at 0
- * (PREV_DE and PREV_DP at +inf). Single workgroup kernels (n <= 128).
+ * (PREV_DE and PREV_DP at +inf); entries from DFT_SCF_ST_COEF on hold the
+ * DIIS coefficients and per-workgroup partial sums (scratch of the calls).
  *
  * dft_scf_diis_stage: stores F and R = F Pa S - S Pa F in the ring slot
  * count mod history, updates G's row of that slot, solves the DIIS
@@ -375,7 +376,7 @@ int dft_syevj(int n, int batch, double* A, double* w, double* V, int init_v, int
  * fill_matrices (no counterpart as device code).
  */
 #define DFT_SCF_HMAX 16
-#define DFT_SCF_STATE_LEN 32
+#define DFT_SCF_STATE_LEN 1152
 #define DFT_SCF_ST_ESUM 0
 #define DFT_SCF_ST_ENN 1
 #define DFT_SCF_ST_E 2
@@ -394,6 +395,7 @@ int dft_syevj(int n, int batch, double* A, double* w, double* V, int init_v, int
 #define DFT_SCF_ST_DIIS_TH 15
 #define DFT_SCF_ST_ENEW 16
 #define DFT_SCF_ST_SLOT 17
+#define DFT_SCF_ST_COEF 32
 int dft_scf_diis_stage(int n, int history, const double* F, const double* R, double* hist_F, double* hist_R,
                        double* G, const double* P, const double* H, const double* Pa, double* state, double* Fu,
                        void* stream);
