@@ -6,7 +6,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -397,7 +400,29 @@ struct Input {
   const double *XYZ, *EX, *CO, *AXYZ, *AZ;
 };
 
+// Host-side phase times of a plan build, printed to stderr when the
+// environment variable DFT_PLAN_TIMING is set.
+struct PhaseTimer {
+  bool on = std::getenv("DFT_PLAN_TIMING") != nullptr;
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now(), last = t0;
+  std::string out;
+  void mark(const char* name) {
+    if (!on) return;
+    const auto now = std::chrono::steady_clock::now();
+    char buf[96];
+    std::snprintf(buf, sizeof buf, " %s %.2f", name, std::chrono::duration<double, std::milli>(now - last).count());
+    out += buf;
+    last = now;
+  }
+  ~PhaseTimer() {
+    if (on)
+      std::fprintf(stderr, "[dft plan ms]%s total %.2f\n", out.c_str(),
+                   std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+  }
+};
+
 int build_plan(dft_plan p, const Input& in) {
+  PhaseTimer timer;
   const int nsh = p->nsh;
   // ---- molecules: packed matrix bases
   p->nmol = in.nmol;
@@ -552,6 +577,7 @@ int build_plan(dft_plan p, const Input& in) {
     if (e != hipSuccess) return fail(DFT_EHIP, std::string("pair pre-pass: ") + hipGetErrorString(e));
   }
   if ((rc = interleave_records(p, 0))) return rc;
+  timer.mark("pairs+records");
   // ---- atoms and shell pairs for the 1-e kernel
   std::vector<double> at(4 * (size_t)p->natom);
   for (int a = 0; a < p->natom; ++a) {
@@ -582,6 +608,7 @@ int build_plan(dft_plan p, const Input& in) {
   }
   p->nshpairs = (int)sp.size();
   if ((rc = upload(&p->d_shpairs, sp))) return rc;
+  timer.mark("atoms");
   // ---- Schwarz: diagonal quartets (ab|ab)
   const int npair = (int)p->pairs.size();
   p->Q.assign(npair, 0.0);
@@ -604,6 +631,7 @@ int build_plan(dft_plan p, const Input& in) {
       p->Q[ids[k]] = std::sqrt(mx);
     }
   }
+  timer.mark("schwarz");
   // ---- FP32 primitive lists. q_k = sqrt(max_x (k k|k k)_xx) of each
   // primitive pair k (its records carry c_a c_b) bounds its share of any
   // integral by q_k Q_cd (Cauchy-Schwarz), so dropping the smallest ones of a
@@ -716,6 +744,7 @@ int build_plan(dft_plan p, const Input& in) {
       p->d_recT64 = nullptr;
       if ((rc = interleave_records(p, 0))) return rc;
     }
+    timer.mark("prim-schwarz+records");
     for (int lev = 0; lev < 2; ++lev) {
       const double eps = lev ? FP32_PRIM_REL / FP32_LEVEL2 : FP32_PRIM_REL;
       std::vector<int4> pp32;
@@ -776,6 +805,7 @@ int build_plan(dft_plan p, const Input& in) {
       if ((rc = interleave_records(p, 1 + lev))) return rc;
     }
   }
+  timer.mark("fp32-records");
   // ---- screened, class-sorted quartet lists (FP64 part, then weak part),
   // molecule-major within each class combination; bra and ket always come
   // from the same molecule
@@ -878,6 +908,7 @@ int build_plan(dft_plan p, const Input& in) {
       }
       if ((rc = upload(&c.d_q, c.q))) return rc;
     }
+  timer.mark("lists");
   // ---- target-major staging: a target is an ordered shell pair (X, Y) of
   // one molecule in half matrix m (0 = J, 1.. = K per density); its blocks
   // are contiguous and ordered by (combo, quartet), so every sum has a
@@ -899,16 +930,30 @@ int build_plan(dft_plan p, const Input& in) {
     const long long ns = (long long)mol_shells[mol].size();
     return (size_t)(m * tpm + mol_toff[mol] + local[x] * ns + local[y]);
   };
+  // per pair: its molecule's first target, shells per molecule and the
+  // two shells' indices within the molecule (the quartet loops below visit
+  // every screened quartet, so the target arithmetic is kept to a few ops)
+  struct PairT {
+    long long base, ns;
+    int x, y;
+  };
+  std::vector<PairT> pt(p->pairs.size());
+  for (size_t id = 0; id < p->pairs.size(); ++id) {
+    const int A = p->pairs[id].x, B = p->pairs[id].y, mol = p->sh[A].mol;
+    pt[id] = PairT{mol_toff[mol], (long long)mol_shells[mol].size(), local[A], local[B]};
+  }
   auto quartet_targets = [&](const int2& qq, size_t* t) {
-    const int4 pb = p->pairs[qq.x], pk = p->pairs[qq.y];
-    const int A = pb.x, B = pb.y, Cc = pk.x, D = pk.y;
-    t[0] = tid(0, A, B);
-    t[1] = tid(0, Cc, D);
-    for (int k = 0; k < p->nk; ++k) {
-      t[2 + 4 * k] = tid(1 + k, A, Cc);
-      t[3 + 4 * k] = tid(1 + k, A, D);
-      t[4 + 4 * k] = tid(1 + k, B, Cc);
-      t[5 + 4 * k] = tid(1 + k, B, D);
+    const PairT& b = pt[qq.x];
+    const PairT& k = pt[qq.y];
+    const long long ns = b.ns, base = b.base;
+    t[0] = (size_t)(base + b.x * ns + b.y);
+    t[1] = (size_t)(base + k.x * ns + k.y);
+    for (int m = 0; m < p->nk; ++m) {
+      const long long o = (1 + m) * tpm + base;
+      t[2 + 4 * m] = (size_t)(o + b.x * ns + k.x);
+      t[3 + 4 * m] = (size_t)(o + b.x * ns + k.y);
+      t[4 + 4 * m] = (size_t)(o + b.y * ns + k.x);
+      t[5 + 4 * m] = (size_t)(o + b.y * ns + k.y);
     }
   };
   std::vector<int> tcount(p->ntarget, 0);
@@ -923,43 +968,62 @@ int build_plan(dft_plan p, const Input& in) {
   // Lane-kernel combinations: the lanes of a wave (WAVE_QUARTETS consecutive
   // quartets of one primitive chunk) whose block of kind k goes to the same
   // target form runs; each run is summed across its lanes in the kernel and
-  // stored once, by its last lane. visit(k, ch, i, t) is called for every
-  // run end i (t = its target) in lane order; masks receive the run starts.
+  // stored once, by its last lane. The runs depend only on the quartets, so
+  // they are found once per wave of the list (mask of run starts per block
+  // kind, run ends in (kind, lane) order) and shared by every chunk row.
   constexpr size_t NO_TARGET = ~(size_t)0;
-  auto lane_runs = [&](size_t ci, std::vector<unsigned long long>* masks, auto&& visit) {
+  struct WaveRuns {
+    std::vector<unsigned long long> mask;  // per (wave, kind)
+    std::vector<size_t> first;             // per wave: its first run end
+    std::vector<int> kind, lane;           // per run end
+    std::vector<size_t> target;
+  };
+  auto lane_runs = [&](size_t ci) {
     const Combo& c = p->combos[ci];
-    const size_t nq = c.q.size(), ns = (size_t)slot_rows(ci);
+    const size_t nq = c.q.size(), nw = nq / WAVE_QUARTETS;
+    WaveRuns r;
+    r.mask.resize(nw * nblk);
+    r.first.resize(nw + 1);
     std::vector<size_t> t(WAVE_QUARTETS * nblk);
-    for (size_t ch = 0; ch < ns; ++ch)
-      for (size_t w0 = 0; w0 < nq; w0 += WAVE_QUARTETS) {
-        for (int l = 0; l < WAVE_QUARTETS; ++l) {
-          const int2 qq = c.q[w0 + l];
-          if (qq.x < 0) {
-            for (int k = 0; k < nblk; ++k) t[l * nblk + k] = NO_TARGET;
-          } else {
-            quartet_targets(qq, tq);
-            for (int k = 0; k < nblk; ++k) t[l * nblk + k] = tq[k];
-          }
+    for (size_t w = 0; w < nw; ++w) {
+      r.first[w] = r.target.size();
+      const size_t w0 = w * WAVE_QUARTETS;
+      for (int l = 0; l < WAVE_QUARTETS; ++l) {
+        const int2 qq = c.q[w0 + l];
+        if (qq.x < 0) {
+          for (int k = 0; k < nblk; ++k) t[l * nblk + k] = NO_TARGET;
+        } else {
+          quartet_targets(qq, tq);
+          for (int k = 0; k < nblk; ++k) t[l * nblk + k] = tq[k];
         }
-        const size_t wave = (ch * nq + w0) / WAVE_QUARTETS;
-        for (int k = 0; k < nblk; ++k) {
-          unsigned long long m = 0;
-          for (int l = 0; l < WAVE_QUARTETS; ++l) {
-            const size_t tl = t[l * nblk + k];
-            if (l == 0 || tl == NO_TARGET || tl != t[(l - 1) * nblk + k]) m |= 1ull << l;
-          }
-          if (masks) (*masks)[wave * nblk + k] = m;
-          for (int l = 0; l < WAVE_QUARTETS; ++l) {
-            const bool last = l == WAVE_QUARTETS - 1 || ((m >> (l + 1)) & 1);
-            const size_t tl = t[l * nblk + k];
-            if (last && tl != NO_TARGET) visit(k, ch, w0 + l, tl);
+      }
+      for (int k = 0; k < nblk; ++k) {
+        unsigned long long m = 0;
+        for (int l = 0; l < WAVE_QUARTETS; ++l) {
+          const size_t tl = t[l * nblk + k];
+          if (l == 0 || tl == NO_TARGET || tl != t[(l - 1) * nblk + k]) m |= 1ull << l;
+        }
+        r.mask[w * nblk + k] = m;
+        for (int l = 0; l < WAVE_QUARTETS; ++l) {
+          const bool last = l == WAVE_QUARTETS - 1 || ((m >> (l + 1)) & 1);
+          const size_t tl = t[l * nblk + k];
+          if (last && tl != NO_TARGET) {
+            r.kind.push_back(k);
+            r.lane.push_back(l);
+            r.target.push_back(tl);
           }
         }
       }
+    }
+    r.first[nw] = r.target.size();
+    return r;
   };
+  std::vector<WaveRuns> runs(p->combos.size());
   for (size_t ci = 0; ci < p->combos.size(); ++ci) {
     if (lane_kernel_combo((int)ci)) {
-      lane_runs(ci, nullptr, [&](int, size_t, size_t, size_t tl) { ++tcount[tl]; });
+      runs[ci] = lane_runs(ci);
+      const int ns = slot_rows(ci);
+      for (size_t tl : runs[ci].target) tcount[tl] += ns;
       continue;
     }
     for (const int2& qq : p->combos[ci].q) {
@@ -967,6 +1031,7 @@ int build_plan(dft_plan p, const Input& in) {
       for (int k = 0; k < nblk; ++k) tcount[tq[k]] += slot_rows(ci);
     }
   }
+  timer.mark("runs");
   std::vector<long long> tbase(p->ntarget, 0), tnext(p->ntarget, 0);
   std::vector<Target> tg(p->ntarget);
   long long cur = 0;
@@ -988,6 +1053,7 @@ int build_plan(dft_plan p, const Input& in) {
           cur += (long long)tcount[t] * nr * nc;
         }
   p->stage_len = cur;
+  timer.mark("targets");
   // slot tables: block-major for the lane kernel (a wave's loads of one
   // block's slots are contiguous), quartet-major for the heavy-class digest
   // (one wave per quartet)
@@ -995,15 +1061,22 @@ int build_plan(dft_plan p, const Input& in) {
     Combo& c = p->combos[ci];
     const size_t nq = c.q.size(), ns = (size_t)slot_rows(ci);
     if (lane_kernel_combo((int)ci)) {
-      // slot of each run's last lane; -1 for the other lanes
-      std::vector<long long> slots(nq * ns * nblk, -1);
-      std::vector<unsigned long long> masks(nq * ns / WAVE_QUARTETS * nblk);
-      lane_runs(ci, &masks, [&](int k, size_t ch, size_t i, size_t tl) {
-        slots[((size_t)k * ns + ch) * nq + i] = tnext[tl];
-        tnext[tl] += (long long)tg[tl].nrow * tg[tl].ncol;
-      });
+      // slot of each run's last lane (-1 for the other lanes), wave by wave
+      // in (kind, lane) order; a run owns ns consecutive blocks of its
+      // target, one per primitive chunk (the kernel adds ch times the block
+      // size). This fixes every target's summation order.
+      WaveRuns& r = runs[ci];
+      const size_t nw = nq / WAVE_QUARTETS;
+      std::vector<long long> slots(nq * nblk, -1);
+      for (size_t w = 0; w < nw; ++w)
+        for (size_t e = r.first[w]; e < r.first[w + 1]; ++e) {
+          const size_t tl = r.target[e];
+          slots[(size_t)r.kind[e] * nq + w * WAVE_QUARTETS + r.lane[e]] = tnext[tl];
+          tnext[tl] += (long long)ns * tg[tl].nrow * tg[tl].ncol;
+        }
       if ((rc = upload(&c.d_slots, slots))) return rc;
-      if ((rc = upload(&c.d_segmask, masks))) return rc;
+      if ((rc = upload(&c.d_segmask, r.mask))) return rc;
+      runs[ci] = WaveRuns();
       continue;
     }
     std::vector<long long> slots(nq * ns * nblk);
@@ -1021,10 +1094,12 @@ int build_plan(dft_plan p, const Input& in) {
   if ((rc = upload(&p->d_tg, tg))) return rc;
   if ((rc = upload(&p->d_tbase, tbase))) return rc;
   if ((rc = upload(&p->d_tcount, tcount))) return rc;
+  timer.mark("slots");
   if (p->own_ws) {
     HIP_TRY(hipMalloc(&p->d_ws, std::max<long long>(workspace_len(p), 1) * sizeof(double)));
     p->ws = workspace_at(p, p->d_ws);
   }
+  timer.mark("workspace");
   return DFT_OK;
 }
 
@@ -1242,6 +1317,7 @@ int need_own_workspace(dft_plan p) {
 // (shell pair, nucleus) block). Slots of one atom are contiguous and ordered
 // (2-e AB, 2-e CD per combination, then 1-e), so the sum order is fixed.
 int grad_prepare(dft_plan p, const int32_t* shell_atom) {
+  PhaseTimer timer;
   const int nsh = p->nsh;
   p->grad_atom.assign(nsh, 0);
   for (int i = 0; i < nsh; ++i) {
@@ -1281,6 +1357,7 @@ int grad_prepare(dft_plan p, const int32_t* shell_atom) {
     (void)hipFree(doff);
     if (e != hipSuccess) return fail(DFT_EHIP, std::string("gradient records: ") + hipGetErrorString(e));
   }
+  timer.mark("grad-records");
   const int natom = p->natom;
   std::vector<int> cnt(natom, 0);
   auto atom_of = [&](int pid, int which) { return p->grad_atom[which == 0 ? p->pairs[pid].x : p->pairs[pid].y]; };
@@ -1341,6 +1418,7 @@ int grad_prepare(dft_plan p, const int32_t* shell_atom) {
     p->d_gslots_ab.push_back(dab);
     p->d_gslots_cd.push_back(dcd);
   }
+  timer.mark("grad-2e-slots");
   std::vector<long long> s1(3 * (size_t)nsp * natom);
   for (int k = 0; k < nsp; ++k)
     for (int cn = 0; cn < natom; ++cn) {
@@ -1378,6 +1456,7 @@ int grad_prepare(dft_plan p, const int32_t* shell_atom) {
     if ((rc = upload(&p->d_gcfirst, cfirst))) return rc;
     HIP_TRY(hipMalloc(&p->d_gpart, std::max<long long>(3LL * p->n_gchunk, 1) * sizeof(double)));
   }
+  timer.mark("grad-1e-slots");
   p->grad_ready = true;
   return DFT_OK;
 }

@@ -83,9 +83,10 @@ hipError_t launch_interleave_records(const int4* map, int nrec, const double* r6
 
 // Fused ERI + J/K digest for one class combination (lane per quartet or per
 // quartet primitive chunk); slots holds 2 + 4*nk staging offsets (doubles)
-// per quartet and chunk (-1 where the lane's block is summed into a later
-// lane's); segmask one word per (wave, block kind) with bit l set where lane
-// l starts a run of lanes with the same target; block is the workgroup size
+// per quartet (-1 where the lane's block is summed into a later lane's; the
+// primitive chunk ch of a quartet adds ch times the block size); segmask one
+// word per (wave of the list, block kind) with bit l set where lane l starts
+// a run of lanes with the same target; block is the workgroup size
 // (64 ... 512, at most lane_block_max), grid_cap the maximum number of
 // workgroups. Quartets with a negative bra pair are padding (no work, no
 // stores).

@@ -45,8 +45,10 @@ __device__ __forceinline__ void store_runs(double (&v)[NE], unsigned long long m
 // class-sorted pair order) and forms its contribution blocks
 //   Jab, Jcd, then per exchange density Kac, Kad, Kbc, Kbd;
 // the blocks of a wave's lanes that share a target are summed in the wave
-// (store_runs) and stored once at stage + slots[(k * nchunk + ch) * ntot + q]
-// (block-major, so that each slot load of a wave is one contiguous run).
+// (store_runs) and stored once at stage + slots[k * ntot + q] + ch * (block
+// size) for primitive chunk ch (block-major slot table, so that each slot
+// load of a wave is one contiguous run; a run's chunks own consecutive
+// blocks of its target).
 // With w = 2^-([a=b]+[c=d]+[ab=cd]):
 //   Jab = 2w sum_cd I P_cd        Jcd = 2w sum_ab I P_ab
 //   Kac = w sum_bd I P_bd  Kad = w sum_bc I P_bc  Kbc = w sum_ad I P_ad  Kbd = w sum_ac I P_ac
@@ -116,13 +118,15 @@ __global__ __launch_bounds__(jk_block_max(LA, LB, LC, LD)) void eri_jk_kernel(
     const int nbf = sa.y;
     const long long mb = (long long)(unsigned)sa.z | ((long long)sa.w << 32);
     const double* DJm = DJ + mb;
-    const long long sstr = (long long)ntot * (splb * splk);
-    const long long* sl = slots + (long long)ch * ntot + q;
-    // the wave's index in the combination's (chunk, quartet) lane order, the
-    // same for its 64 lanes (n and first are multiples of the wave)
+    const long long sstr = ntot;
+    const long long* sl = slots + q;
+    // slot of block kind k for this lane's chunk (the chunks of a run own
+    // consecutive blocks of ne elements)
+    auto at = [&](long long s, int ne) { return s >= 0 ? s + (long long)ch * ne : -1LL; };
+    // the wave's index in the combination's quartet list, the same for its
+    // 64 lanes (n and first are multiples of the wave, and so its chunk)
     const int lane = threadIdx.x & 63;
-    const unsigned wid =
-        (unsigned)__builtin_amdgcn_readfirstlane((int)((ch * (unsigned)ntot + (unsigned)q) / WAVE_QUARTETS));
+    const unsigned wid = (unsigned)__builtin_amdgcn_readfirstlane(q / WAVE_QUARTETS);
     const unsigned long long* msk = segmask + (size_t)wid * NBLK;
     // Coulomb
     {
@@ -151,8 +155,8 @@ __global__ __launch_bounds__(jk_block_max(LA, LB, LC, LD)) void eri_jk_kernel(
       }
 #pragma unroll
       for (int cd = 0; cd < S::NCD; ++cd) jcd[cd] *= 2.0 * w;
-      store_runs(jab, msk[0], lane, sl[0], stage);
-      store_runs(jcd, msk[1], lane, sl[sstr], stage);
+      store_runs(jab, msk[0], lane, at(sl[0], S::NAB), stage);
+      store_runs(jcd, msk[1], lane, at(sl[sstr], S::NCD), stage);
     }
     // Exchange, one density at a time. The densities are symmetric, and
     // P_bd is read as P_db (likewise P_bc, P_ad, P_ac): the lanes of a wave
@@ -209,10 +213,10 @@ __global__ __launch_bounds__(jk_block_max(LA, LB, LC, LD)) void eri_jk_kernel(
       for (int x = 0; x < NB * ND; ++x) kbd[x] *= w;
       const long long* ks = sl + (2 + 4 * k) * sstr;
       const unsigned long long* km = msk + 2 + 4 * k;
-      store_runs(kac, km[0], lane, ks[0], stage);
-      store_runs(kad, km[1], lane, ks[sstr], stage);
-      store_runs(kbc, km[2], lane, ks[2 * sstr], stage);
-      store_runs(kbd, km[3], lane, ks[3 * sstr], stage);
+      store_runs(kac, km[0], lane, at(ks[0], NA * NC), stage);
+      store_runs(kad, km[1], lane, at(ks[sstr], NA * ND), stage);
+      store_runs(kbc, km[2], lane, at(ks[2 * sstr], NB * NC), stage);
+      store_runs(kbd, km[3], lane, at(ks[3 * sstr], NB * ND), stage);
     }
   }
 }

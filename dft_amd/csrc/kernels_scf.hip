@@ -34,15 +34,20 @@ namespace dft {
 constexpr int JAC_THREADS = 1024;
 constexpr int JAC_MAX_SWEEPS = 40;
 // LDS of one workgroup, and the part of it the kernel's static arrays take
-// (rotation tables of two rounds, diagonal, reduction: 4.2 KB)
+// (rotation tables of two rounds, diagonal, reduction: 5.3 KB)
 constexpr int JAC_LDS_BYTES = 160 * 1024;
-constexpr int JAC_STATIC_LDS = 5 * 1024;
+constexpr int JAC_STATIC_LDS = 6 * 1024;
 // per-thread share of a round's blocks (h <= 64 pairs): ceil(64 * 65 / 2 / JAC_THREADS)
 constexpr int JAC_BLK_PER_THREAD = (JAC_NMAX / 2 * (JAC_NMAX / 2 + 1) / 2 + JAC_THREADS - 1) / JAC_THREADS;
 constexpr int JAC_V_GROUP = 4;
 
-// partner positions of round r of the round-robin tournament over m players
-__device__ __forceinline__ int rr_pos(int i, int r, int m) { return i == 0 ? 0 : 1 + (i - 1 + r) % (m - 1); }
+// partner positions of round r < m - 1 of the round-robin tournament over m
+// players (i < m, so i - 1 + r < 2 (m - 1) and one subtraction reduces it)
+__device__ __forceinline__ int rr_pos(int i, int r, int m) {
+  if (i == 0) return 0;
+  const int x = i - 1 + r;
+  return 1 + (x >= m - 1 ? x - (m - 1) : x);
+}
 
 // Upper triangle of the symmetric m x m matrix, packed by rows.
 __device__ __forceinline__ int tri(int r, int c, int m) {
@@ -61,7 +66,7 @@ __global__ __launch_bounds__(JAC_THREADS) void jacobi_eigh_kernel(int n, const d
                                                                   double* __restrict__ V, long long sv,
                                                                   int init_v, int* __restrict__ sweeps) {
   extern __shared__ double lds[];
-  __shared__ double cs_c[2][JAC_NMAX / 2], cs_s[2][JAC_NMAX / 2];
+  __shared__ double cs_c[2][JAC_NMAX / 2], cs_s[2][JAC_NMAX / 2], cs_t[2][JAC_NMAX / 2];
   __shared__ int cs_p[2][JAC_NMAX / 2], cs_q[2][JAC_NMAX / 2];
   __shared__ double red[JAC_THREADS / 64];
   __shared__ double diag[JAC_NMAX];
@@ -152,16 +157,19 @@ __global__ __launch_bounds__(JAC_THREADS) void jacobi_eigh_kernel(int n, const d
               q = t;
             }
             const double apq = a[tri(p, q, m)];
-            double c = 1.0, s = 0.0;
+            double c = 1.0, s = 0.0, t = 0.0;
             if (fabs(apq) > thresh) {
-              const double tau = (a[tri(q, q, m)] - a[tri(p, p, m)]) / (2.0 * apq);
-              const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-              c = 1.0 / sqrt(1.0 + t * t);
+              // t = sign(tau) / (|tau| + sqrt(1 + tau^2)) with tau = d / e,
+              // d = a_qq - a_pp, e = 2 a_pq, written as sign(d) e / (|d| + |(d, e)|)
+              const double d = a[tri(q, q, m)] - a[tri(p, p, m)], e = 2.0 * apq;
+              t = (d >= 0.0 ? e : -e) / (fabs(d) + sqrt(fma(d, d, e * e)));
+              c = 1.0 / sqrt(fma(t, t, 1.0));
               s = t * c;
               rotated = 1;
             }
             cs_c[cur][k] = c;
             cs_s[cur][k] = s;
+            cs_t[cur][k] = t;
             cs_p[cur][k] = p;
             cs_q[cur][k] = q;
           }
@@ -237,8 +245,8 @@ __global__ __launch_bounds__(JAC_THREADS) void jacobi_eigh_kernel(int n, const d
             if (sk == 0.0 && sl == 0.0) continue;
             const double ck = cs_c[cur][k], cl = cs_c[cur][l];
             if (k == l) {
-              // the block the rotation diagonalises (t = s / c): a_pp - t a_pq, a_qq + t a_pq, 0
-              const double t = sk / ck, apq = x[j][1];
+              // the block the rotation diagonalises: a_pp - t a_pq, a_qq + t a_pq, 0
+              const double t = cs_t[cur][k], apq = x[j][1];
               a[ix[j][0]] = x[j][0] - t * apq;
               a[ix[j][2]] = x[j][2] + t * apq;
               a[ix[j][1]] = 0.0;

@@ -21,7 +21,8 @@ import torch
 from . import _lib
 from .basis import build_shells, build_shells_from_entries
 from .engine import Plan
-from .scf import electron_counts, scf_diis
+from .linalg import JACOBI_NMAX
+from .scf import electron_counts, scf_diis, scf_diis_graph
 
 
 def nuclear_repulsion_gradient(Z, xyz):
@@ -74,9 +75,15 @@ def energy_and_gradient(Z, xyz, basis, charge=0, mult=1, schwarz_tau=1e-12, e_to
     N_a, N_b = electron_counts(Z, charge, mult)
     plan = Plan(shells, Z, xyz, schwarz_tau=schwarz_tau, fp32_tau=0.0, uhf=N_a != N_b)
     # undamped DIIS from the first iteration: the gradient needs a tightly
-    # converged density, which damping (the reference's schedule) slows down
-    res = scf_diis(plan, Z, xyz, N_a, N_b, e_tol=e_tol, p_tol=p_tol, max_count=300, guess=guess, rate=1.0,
-                   diis_threshold=float("inf"))
+    # converged density, which damping (the reference's schedule) slows down.
+    # Closed shells up to the eigensolver's order run the loop as one
+    # captured hipGraph per iteration (same iteration as scf_diis).
+    if N_a == N_b and plan.nbf <= JACOBI_NMAX:
+        res = scf_diis_graph(plan, Z, xyz, N_a, e_tol=e_tol, p_tol=p_tol, max_count=300, guess=guess, rate=1.0,
+                             diis_threshold=float("inf"))
+    else:
+        res = scf_diis(plan, Z, xyz, N_a, N_b, e_tol=e_tol, p_tol=p_tol, max_count=300, guess=guess, rate=1.0,
+                       diis_threshold=float("inf"))
     if not res.converged:
         raise RuntimeError("SCF did not converge; the analytic gradient needs a stationary density")
     g = electronic_gradient(plan, res.C_a, res.eps_a, N_a, res.C_b, res.eps_b, N_b).cpu().numpy()

@@ -347,7 +347,7 @@ SCF_STATE_LEN = 1152
 
 
 def scf_diis_graph(plan: Plan, Z, centers, N_occ, mode="fp64", rate=0.4, diis_threshold=1e-3, history=10,
-                   e_tol=1e-8, p_tol=1e-6, max_count=500):
+                   e_tol=1e-8, p_tol=1e-6, max_count=500, guess=None):
     """The damped + DIIS loop of shelled.py:1103-1292 for a closed shell
     (N_alpha = N_beta = N_occ), one iteration = one replay of a captured
     hipGraph: the direct J/K build (dft_jk, its class launches on four
@@ -359,7 +359,10 @@ def scf_diis_graph(plan: Plan, Z, centers, N_occ, mode="fp64", rate=0.4, diis_th
     (dft_scf_damp_control). The loop state stays on the device; the host
     reads 32 doubles after each replay. The exit rebuild of
     shelled.py:1261-1285 runs eagerly after the loop. Orders up to 128
-    (the eigensolver's limit); no checkpoint files (scf_diis writes them)."""
+    (the eigensolver's limit); no checkpoint files (scf_diis writes them).
+    ``guess`` = P_a (or (P_a, P_b) with P_a == P_b) starts from that density
+    instead of the core Hamiltonian's; the first diagonalisation then starts
+    from the core Hamiltonian's orbitals."""
     import ctypes as C
     from . import _lib
     from .linalg import syevj_
@@ -375,8 +378,14 @@ def scf_diis_graph(plan: Plan, Z, centers, N_occ, mode="fp64", rate=0.4, diis_th
     S, H, X, E_NN = _setup(plan, Z, centers)
     f64 = dict(dtype=torch.float64, device=dev)
     _, Q = eigh(X.T @ H @ X)
-    C0 = X @ Q
-    Pa = (C0[:, :N_occ] @ C0[:, :N_occ].T).contiguous()
+    if guess is not None:
+        g = guess[0] if isinstance(guess, (tuple, list)) else guess
+        Pa = torch.as_tensor(g, **f64).clone().contiguous()
+        if Pa.shape != (n, n):
+            raise ValueError(f"guess density must be {n} x {n}")
+    else:
+        C0 = X @ Q
+        Pa = (C0[:, :N_occ] @ C0[:, :N_occ].T).contiguous()
     P = (2 * Pa).contiguous()
     Q = Q.contiguous()
     J, K, F, R, Fu = (torch.empty((n, n), **f64) for _ in range(5))
