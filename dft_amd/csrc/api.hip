@@ -379,6 +379,12 @@ int raw_blocks(dft_plan p, int combo, const std::vector<int2>& q, std::vector<do
   }
   if (lane_kernel_combo(combo)) {
     e = launch_eri_raw(combo, dq, (long long)q.size(), pairs, p->d_rec64, dout, p->d_boys, nullptr);
+  } else if (generic_combo(combo)) {
+    int la, lb, lc, ld;
+    pc_moments(combo_pcb(combo), la, lb);
+    pc_moments(combo_pck(combo), lc, ld);
+    e = launch_eri_generic(la, lb, lc, ld, dq, 0, (int)q.size(), pairs, p->d_rec64, dout, 1, blk, p->d_boys, 1 << 16,
+                           false, nullptr);
   } else {
     const RecView rv = rvp ? *rvp : rec_view(p, combo, false);
     e = launch_eri_chunk(combo, false, dq, 0, (int)q.size(), 1, pairs, rv, dout, 1, blk, 0, p->d_boys, 256,
@@ -660,7 +666,7 @@ int build_plan(dft_plan p, const Input& in) {
         std::vector<int2> q(nk);
         for (int k = 0; k < nk; ++k) q[k] = make_int2(c0 + k, c0 + k);
         std::vector<double> blk;
-        if (lane_kernel_combo(ci)) {
+        if (lane_kernel_combo(ci) || generic_combo(ci)) {
           rc = raw_blocks(p, ci, q, blk, nab * nab, dpp);
         } else {
           // the column kernel reads bra records interleaved over the list
@@ -816,6 +822,8 @@ int build_plan(dft_plan p, const Input& in) {
       c.pcb = pcb;
       c.pck = pck;
       const bool lane = lane_kernel_combo(combo_index(pcb, pck));
+      // the run-time-momentum kernels (f shells) are FP64 only
+      const bool gen = generic_combo(combo_index(pcb, pck));
       // per molecule: FP64 part, first and second FP32 level
       std::vector<std::vector<int2>> part[3];
       for (auto& x : part) x.resize(in.nmol);
@@ -833,7 +841,7 @@ int build_plan(dft_plan p, const Input& in) {
             const long long np = (long long)p->pairs[ib].w * p->pairs[ik].w;
             const long long kp = (long long)p->sh[p->pairs[ib].x].nprim * p->sh[p->pairs[ib].y].nprim *
                                  p->sh[p->pairs[ik].x].nprim * p->sh[p->pairs[ik].y].nprim;
-            if (b < p->tau32) {
+            if (!gen && b < p->tau32) {
               part[b < FP32_LEVEL2 * p->tau32 ? 2 : 1][m].push_back(make_int2(ib, ik));
               c.prim64w += np;
               c.kprod32 += kp;
@@ -895,8 +903,9 @@ int build_plan(dft_plan p, const Input& in) {
         const long long ni = (long long)ncart_h(la) * ncart_h(lb) * ncart_h(lc) * ncart_h(ld);
         const long long ncd = (long long)ncart_h(lc) * ncart_h(ld);
         // column kernel: split the bra primitives into chunks where the
-        // class has too few (quartet, column) lanes to fill the chip
-        if (p->split && nq * ncd < SPLIT_LANES_HEAVY) {
+        // class has too few (quartet, column) lanes to fill the chip (the
+        // wave-per-quartet kernel of the f classes is not split)
+        if (p->split && !gen && nq * ncd < SPLIT_LANES_HEAVY) {
           int nbmax = 0;
           for (const int2& qq : c.q) nbmax = std::max(nbmax, p->pairs[qq.x].w);
           const long long want = (SPLIT_LANES_HEAVY + nq * ncd - 1) / (nq * ncd);
@@ -1155,6 +1164,11 @@ hipError_t launch_level(dft_plan p, const Workspace& ws, size_t ci, int lev, lon
   pc_moments(c.pcb, la, lb);
   pc_moments(c.pck, lc, ld);
   const long long ni = (long long)ncart_h(la) * ncart_h(lb) * ncart_h(lc) * ncart_h(ld);
+  if (generic_combo((int)ci)) {
+    if (f32) return hipErrorInvalidValue;  // f classes have no FP32 part
+    return launch_eri_generic(la, lb, lc, ld, c.d_q, (int)first, (int)n, p->d_pairs, p->d_rec64,
+                              ws.ibuf + c.ibuf_off, 1, ni, p->d_boys, sh.cap * 16, true, s);
+  }
   if (!f32 && (p->mfma || c.mfma) && mfma_combo((int)ci))
     return launch_eri_mfma((int)ci, c.d_q, (int)first, (int)n, c.splb, prs, p->d_rec64, ws.ibuf + c.ibuf_off, 1, ni,
                            ni * (long long)c.q.size(), p->d_boys, sh.cap * 4, true, s);
@@ -1187,6 +1201,9 @@ hipError_t launch_digest(dft_plan p, const Workspace& ws, size_t ci, const doubl
   pc_moments(c.pcb, la, lb);
   pc_moments(c.pck, lc, ld);
   const long long ni = (long long)ncart_h(la) * ncart_h(lb) * ncart_h(lc) * ncart_h(ld);
+  if (generic_combo((int)ci))
+    return launch_jk_generic(la, lb, lc, ld, c.d_q, c.d_slots, (int)c.q.size(), p->d_pairs, p->d_shell_dm, P, Pa, Pb,
+                             p->nk, ws.ibuf + c.ibuf_off, ws.stage, s);
   return launch_jk_digest((int)ci, c.d_q, c.d_slots, (int)c.q.size(), c.splb, ni * (long long)c.q.size(),
                           p->d_pairs, p->d_shell_dm, P, Pa, Pb, p->nk, ws.ibuf + c.ibuf_off, ws.stage, s);
 }
@@ -1736,6 +1753,7 @@ int dft_plan_tune(dft_plan p, const double* P, const double* Pa, const double* P
   int rc = DFT_OK;
   for (size_t ci = 0; ci < p->combos.size() && rc == DFT_OK; ++ci) {
     if (combo == -2 || (combo >= 0 && (int)ci != combo)) continue;
+    if (generic_combo((int)ci)) continue;  // one wave per quartet: no launch shape to choose
     Combo& c = p->combos[ci];
     const long long ntot = (long long)c.q.size();
     float t_all = 0.f, t_32 = 0.f;

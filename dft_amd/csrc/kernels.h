@@ -12,9 +12,11 @@
 
 namespace dft {
 
-// number of pair classes / class combinations for shells up to l = 2 (s, p, d)
-constexpr int LMAX = 2;
-constexpr int NPC = (LMAX + 1) * (LMAX + 2) / 2;   // pair classes: ss, ps, pp, ds, dp, dd
+// number of pair classes / class combinations for shells up to l = 3 (s, p,
+// d, f; combinations with an f shell run the run-time-momentum kernels of
+// kernels_generic.hip, the others compile-time ones)
+constexpr int LMAX = 3;
+constexpr int NPC = (LMAX + 1) * (LMAX + 2) / 2;   // pair classes: ss, ps, pp, ds, dp, dd, fs, fp, fd, ff
 constexpr int NCOMBO = NPC * (NPC + 1) / 2;        // (bra class >= ket class)
 
 // lanes of a wavefront, i.e. quartets per wave of a lane-kernel launch; the
@@ -120,6 +122,21 @@ bool mfma_combo(int combo);
 hipError_t launch_eri_mfma(int combo, const int2* q, int first, int n, int splb, const int4* pairs, const double* rec,
                            double* out, long long xs, long long qs, long long ss, const double* gtab, int cap,
                            bool cut, hipStream_t s);
+
+// Class combinations with an f shell (kernels_generic.hip): one wave per
+// quartet, momenta at run time; integrals to out[x * xs + q * qs] (q the
+// list index, quartets [first, first + n)), plain FP64 records; FP64 only.
+// cut: leave out primitive quartets below the J/K kernels' primitive cut
+// (needs the records' Schwarz factors). The J/K blocks of a whole list from
+// integrals stored with xs = 1, qs = ni, slots quartet-major.
+bool generic_combo(int combo);
+size_t generic_lds_bytes(int la, int lb, int lc, int ld);
+hipError_t launch_eri_generic(int la, int lb, int lc, int ld, const int2* quartets, int first, int n,
+                              const int4* pairs, const double* rec, double* out, long long xs, long long qs,
+                              const double* boys_tab, int grid_cap, bool cut, hipStream_t s);
+hipError_t launch_jk_generic(int la, int lb, int lc, int ld, const int2* quartets, const long long* slots, int n,
+                             const int4* pairs, const int4* shell_dm, const double* DJ, const double* DK0,
+                             const double* DK1, int nk, const double* ibuf, double* stage, hipStream_t s);
 
 // raw contracted blocks (internal orientation) for one class combination:
 // out + q * NA*NB*NC*ND

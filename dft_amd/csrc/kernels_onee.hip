@@ -369,6 +369,179 @@ __global__ __launch_bounds__(64) void grad1e_kernel(const int2* __restrict__ sp,
   }
 }
 
+// ------------------------------------- S, T, V of f-shell pairs
+// The same integrals as one_e_kernel for shell pairs with an f shell, with
+// the momenta at run time (no compile-time instantiation, whose register
+// tables would not fit): one 64-lane workgroup per shell pair walks its
+// primitive pairs in turn. Per primitive pair, three lanes build the axes'
+// Hermite tables (b raised to lb + 2 for T) in LDS, every lane evaluates
+// R_tuv(p, P - C) for its nuclei in its own LDS row and sums them with the
+// charges, the rows are summed in lane order, and each lane adds the
+// primitive pair's S, T, V to the component pairs it owns (ab = lane,
+// lane + 64).
+constexpr int OE_GLMAX = 3;                                 // largest momentum
+constexpr int OE_GL = 2 * OE_GLMAX;                         // L of the pair
+constexpr int OE_GNH = nherm(OE_GL);                        // 84
+constexpr int OE_GET = (OE_GLMAX + 1) * (OE_GLMAX + 3) * (OE_GL + 3);  // one axis table
+constexpr int OE_GNAB = ncart(OE_GLMAX) * ncart(OE_GLMAX);  // 100
+
+__global__ __launch_bounds__(64) void one_e_generic_kernel(int la, int lb, const int2* __restrict__ sp, int npairs,
+                                                           const ShellDev* __restrict__ sh,
+                                                           const double* __restrict__ exps,
+                                                           const double* __restrict__ coefs,
+                                                           const double* __restrict__ atoms, double* __restrict__ S,
+                                                           double* __restrict__ T, double* __restrict__ V,
+                                                           const double* __restrict__ gtab) {
+  __shared__ double Et[3][OE_GET];          // E[i][j][t], i <= la, j <= lb + 2, t <= i + j
+  __shared__ double Rl[64][OE_GNH + 8];     // per lane: R row, then F
+  __shared__ double Rs[OE_GNH];
+  const int lane = threadIdx.x;
+  if ((int)blockIdx.x >= npairs) return;
+  const int2 pr = sp[blockIdx.x];
+  const ShellDev A = sh[pr.x], B = sh[pr.y];
+  const int L = la + lb, nh = nherm(L), na = ncart(la), nb = ncart(lb), nab = na * nb;
+  const int JB = lb + 2, NT = la + JB + 1;  // table extents
+  auto eidx = [&](int i, int j, int t) { return (i * (JB + 1) + j) * NT + t; };
+  const double AB[3] = {A.x - B.x, A.y - B.y, A.z - B.z};
+  const double rab2 = AB[0] * AB[0] + AB[1] * AB[1] + AB[2] * AB[2];
+  double accS[2] = {0.0, 0.0}, accT[2] = {0.0, 0.0}, accV[2] = {0.0, 0.0};
+  double* R = Rl[lane];
+  double* F = Rl[lane] + OE_GNH;
+  for (int k = 0; k < A.nprim * B.nprim; ++k) {
+    const int i = k / B.nprim, j = k - i * B.nprim;
+    const double a = exps[A.prim_off + i], b = exps[B.prim_off + j];
+    const double p = a + b, ip = 1.0 / p, i2p = 0.5 * ip;
+    const double cc = coefs[A.prim_off + i] * coefs[B.prim_off + j] * exp(-a * b * ip * rab2);
+    const double P[3] = {(a * A.x + b * B.x) * ip, (a * A.y + b * B.y) * ip, (a * A.z + b * B.z) * ip};
+    __syncthreads();  // the previous primitive pair's readers are done
+    if (lane < 3) {
+      // calc_E's recursion (shelled.py:238-293) along axis `lane`
+      double* E = Et[lane];
+      const double xpa = -b * AB[lane] * ip, xpb = a * AB[lane] * ip;
+      for (int x = 0; x < (la + 1) * (JB + 1) * NT; ++x) E[x] = 0.0;
+      E[eidx(0, 0, 0)] = 1.0;
+      for (int ia = 0; ia < la; ++ia)
+        for (int t = 0; t <= ia + 1; ++t) {
+          double v = t <= ia ? xpa * E[eidx(ia, 0, t)] : 0.0;
+          if (t > 0) v = fma(i2p, E[eidx(ia, 0, t - 1)], v);
+          if (t + 1 <= ia) v = fma(double(t + 1), E[eidx(ia, 0, t + 1)], v);
+          E[eidx(ia + 1, 0, t)] = v;
+        }
+      for (int ia = 0; ia <= la; ++ia)
+        for (int jb = 0; jb < JB; ++jb)
+          for (int t = 0; t <= ia + jb + 1; ++t) {
+            double v = t <= ia + jb ? xpb * E[eidx(ia, jb, t)] : 0.0;
+            if (t > 0) v = fma(i2p, E[eidx(ia, jb, t - 1)], v);
+            if (t + 1 <= ia + jb) v = fma(double(t + 1), E[eidx(ia, jb, t + 1)], v);
+            E[eidx(ia, jb + 1, t)] = v;
+          }
+    }
+    // this lane's nuclei: R summed with the charges (in place, level by level)
+    for (int h = 0; h < nh; ++h) R[h] = 0.0;
+    for (int c = A.atom_begin + lane; c < A.atom_end; c += 64) {
+      const double X = P[0] - atoms[4 * c], Y = P[1] - atoms[4 * c + 1], Z = P[2] - atoms[4 * c + 2];
+      const double Zc = atoms[4 * c + 3];
+      const double Tb = p * (X * X + Y * Y + Z * Z);
+      // F_0..F_L as boys_global, run-time L
+      if (Tb < BOYS_TMAX) {
+        const int kk = boys_point(Tb);
+        const double d = -(Tb - kk * BOYS_DT);
+        const double inv_fact[BOYS_TAYLOR] = {1.0, 1.0, 0.5, 1.0 / 6, 1.0 / 24, 1.0 / 120, 1.0 / 720, 1.0 / 5040};
+        const double* row = gtab + kk * BOYS_MT + L;
+        double f = row[BOYS_TAYLOR - 1] * inv_fact[BOYS_TAYLOR - 1];
+        for (int jj = BOYS_TAYLOR - 2; jj >= 0; --jj) f = fma(f, d, row[jj] * inv_fact[jj]);
+        F[L] = f;
+        const double e = exp(-Tb), t2 = Tb + Tb;
+        for (int m = L - 1; m >= 0; --m) F[m] = fma(t2, F[m + 1], e) * inv_odd(m);
+      } else {
+        const double r = 1.0 / sqrt(Tb), hh = 0.5 * r * r;
+        const double e = Tb >= boys_tnoexp(L) ? 0.0 : exp(-Tb);
+        F[0] = 0.88622692545275801365 * r;
+        for (int m = 0; m < L; ++m) F[m + 1] = ((2 * m + 1) * F[m] - e) * hh;
+      }
+      // R^n: level n = L holds R_000 only; each lower level updates the
+      // entries with t+u+v = k from k = L - n down to 1 (in place: they read
+      // entries one or two lower, still of the level above), then R_000
+      double pw = 1.0;
+      for (int m = 0; m < L; ++m) pw *= -2.0 * p;
+      double Rt[OE_GNH];  // run-time indexed: private (scratch) memory
+      Rt[0] = F[L] * pw;
+      for (int n = L - 1; n >= 0; --n) {
+        for (int kk = L - n; kk >= 1; --kk)
+          for (int h = nherm(kk - 1); h < nherm(kk); ++h) {
+            const int t = h_t(h), u = h_u(h), v = h_v(h);
+            double r;
+            if (t > 0) {
+              r = X * Rt[hidx(t - 1, u, v)];
+              if (t > 1) r = fma(double(t - 1), Rt[hidx(t - 2, u, v)], r);
+            } else if (u > 0) {
+              r = Y * Rt[hidx(t, u - 1, v)];
+              if (u > 1) r = fma(double(u - 1), Rt[hidx(t, u - 2, v)], r);
+            } else {
+              r = Z * Rt[hidx(t, u, v - 1)];
+              if (v > 1) r = fma(double(v - 1), Rt[hidx(t, u, v - 2)], r);
+            }
+            Rt[h] = r;
+          }
+        pw /= -2.0 * p;
+        Rt[0] = F[n] * pw;
+      }
+      for (int h = 0; h < nh; ++h) R[h] = fma(Zc, Rt[h], R[h]);
+    }
+    __syncthreads();
+    for (int h = lane; h < nh; h += 64) {
+      double x = 0.0;
+      for (int l = 0; l < 64; ++l) x += Rl[l][h];
+      Rs[h] = x;
+    }
+    __syncthreads();
+    const double sq = sqrt(PI * ip), pre = cc * (-2.0 * PI * ip);
+    for (int u2 = 0; u2 < 2; ++u2) {
+      const int ab = lane + 64 * u2;
+      if (ab >= nab) break;
+      const int ca = ab / nb, cb = ab - ca * nb;
+      int pa[3], pb[3];
+      double n = 1.0;
+      for (int d = 0; d < 3; ++d) {
+        pa[d] = comp_of(la, ca, d);
+        pb[d] = comp_of(lb, cb, d);
+        n *= inv_sqrt_dfac(pa[d]) * inv_sqrt_dfac(pb[d]);
+      }
+      double S1[3], T1[3];
+      for (int d = 0; d < 3; ++d) {
+        const double* E = Et[d];
+        const double s0 = E[eidx(pa[d], pb[d], 0)] * sq;
+        double tk = fma(-2.0 * b * (2 * pb[d] + 1), s0, 4.0 * b * b * E[eidx(pa[d], pb[d] + 2, 0)] * sq);
+        if (pb[d] >= 2) tk = fma(double(pb[d] * (pb[d] - 1)), E[eidx(pa[d], pb[d] - 2, 0)] * sq, tk);
+        S1[d] = s0;
+        T1[d] = tk;
+      }
+      accS[u2] += cc * n * S1[0] * S1[1] * S1[2];
+      accT[u2] += cc * n * -0.5 * (T1[0] * S1[1] * S1[2] + S1[0] * T1[1] * S1[2] + S1[0] * S1[1] * T1[2]);
+      double v = 0.0;
+      for (int tt = 0; tt <= pa[0] + pb[0]; ++tt)
+        for (int uu = 0; uu <= pa[1] + pb[1]; ++uu) {
+          double w = 0.0;
+          for (int vv = 0; vv <= pa[2] + pb[2]; ++vv)
+            w = fma(Et[2][eidx(pa[2], pb[2], vv)], Rs[hidx(tt, uu, vv)], w);
+          v = fma(Et[0][eidx(pa[0], pb[0], tt)] * Et[1][eidx(pa[1], pb[1], uu)], w, v);
+        }
+      accV[u2] += pre * n * v;
+    }
+  }
+  for (int u2 = 0; u2 < 2; ++u2) {
+    const int ab = lane + 64 * u2;
+    if (ab >= nab) break;
+    const int ca = ab / nb, cb = ab - ca * nb;
+    if (pr.x == pr.y && ca < cb) continue;
+    const long long r = A.ao + ca, c = B.ao + cb;
+    const long long nbf = A.nbf, rc = A.mbase + r * nbf + c, cr = A.mbase + c * nbf + r;
+    S[rc] = S[cr] = accS[u2];
+    T[rc] = T[cr] = accT[u2];
+    V[rc] = V[cr] = accV[u2];
+  }
+}
+
 namespace {
 
 template <int LA, int LB>
@@ -395,8 +568,14 @@ hipError_t launch_one_electron(int la, int lb, const int2* sp, int npairs, const
                                const double* coefs, const double* atoms, double* S, double* T, double* V,
                                const double* gtab, hipStream_t s) {
   if (npairs <= 0) return hipSuccess;
+  if (la < lb || lb < 0 || la > OE_GLMAX) return hipErrorInvalidValue;
+  if (la > 2) {
+    hipLaunchKernelGGL(one_e_generic_kernel, dim3(npairs), dim3(64), 0, s, la, lb, sp, npairs, sh, exps, coefs, atoms,
+                       S, T, V, gtab);
+    return hipGetLastError();
+  }
   hipError_t e = hipErrorInvalidValue;
-  static_for<0, LMAX + 1>([&](auto aI) {
+  static_for<0, 3>([&](auto aI) {
     constexpr int A = decltype(aI)::value;
     static_for<0, A + 1>([&](auto bI) {
       constexpr int B = decltype(bI)::value;

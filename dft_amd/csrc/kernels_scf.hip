@@ -470,6 +470,13 @@ __global__ __launch_bounds__(64) void scf_diis_solve_kernel(int h, int nblk, dou
     }
     for (int k = 0; k < h; ++k) sum += c[k];
     for (int k = 0; k < h; ++k) c[k] /= sum;
+    // A singular Gram matrix (residuals that no longer differ, as near a
+    // tight convergence) has no DIIS solution: the iteration then uses its
+    // own Fock matrix (c = e_slot) instead of non-finite coefficients.
+    bool ok = isfinite(sum) && sum != 0.0;
+    for (int k = 0; k < h; ++k) ok = ok && isfinite(c[k]);
+    if (!ok)
+      for (int k = 0; k < h; ++k) c[k] = k == slot ? 1.0 : 0.0;
     st[SCF_ST_SLOT] = slot;
   }
 }

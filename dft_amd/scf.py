@@ -289,10 +289,16 @@ def scf_diis(plan: Plan, Z, centers, N_a, N_b, mode="fp64", rate=0.4, diis_thres
         gram = Bn
         if use_diis:
             B = gram
-            c = torch.linalg.solve(B, torch.ones(B.shape[0], dtype=B.dtype, device=B.device))
-            c = c / c.sum()
-            Fa = torch.einsum("i,iab->ab", c, torch.stack(hist["Fa"]))
-            Fb = torch.einsum("i,iab->ab", c, torch.stack(hist["Fb"]))
+            try:
+                c = torch.linalg.solve(B, torch.ones(B.shape[0], dtype=B.dtype, device=B.device))
+                c = c / c.sum()
+            except RuntimeError:  # singular: residuals that no longer differ
+                c = None
+            # without a (finite) DIIS solution the iteration keeps its own
+            # Fock matrix, as the device loop does
+            if c is not None and bool(torch.isfinite(c).all()):
+                Fa = torch.einsum("i,iab->ab", c, torch.stack(hist["Fa"]))
+                Fb = torch.einsum("i,iab->ab", c, torch.stack(hist["Fb"]))
         E_new = float(0.5 * torch.sum(P * H + P_a * Fa + P_b * Fb)) + E_NN
         dE = float("inf") if E_total is None else abs(E_new - E_total)
         _, Qa = eigh(X.T @ Fa @ X, Qa)

@@ -102,7 +102,8 @@ def test_tuner_keeps_results(torch_cuda):
     P = (2 * Pa).contiguous()
     J0, K0, _ = bp.jk(P, Pa, None, mode="fp64")
     rep = tune(bp, P, Pa, None, reps=2)
-    assert rep.shape == (21, 6) and (rep[:, 2] > 0).all()
+    # 10 pair classes (s, p, d, f) -> 55 class combinations
+    assert rep.shape == (55, 6) and (rep[:, 2] > 0).all()
     J1, K1, _ = bp.jk(P, Pa, None, mode="fp64")
     scale = max(1.0, J0.abs().max().item())
     assert (J1 - J0).abs().max().item() < 1e-12 * scale

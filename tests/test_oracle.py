@@ -221,6 +221,43 @@ def test_centre_derivative_generates_d():
     assert np.isclose(fd, 2 * a * dxx - ss, rtol=1e-7, atol=1e-12)
 
 
+def test_centre_derivative_generates_f():
+    # d/dA_x (dxx_A s_B|s_C s_D) = 2a (fxxx_A s_B|..) - 2 (px_A s_B|..), unnormalised primitives
+    a, b, c, d = 0.8, 1.1, 0.6, 1.4
+    A, B, Cc, D = np.array([0.1, 0.3, -0.2]), np.array([-0.5, 0.2, 0.4]), np.array([0.3, -0.6, 0.1]), \
+        np.array([0.7, 0.1, -0.5])
+    h = 1e-5
+
+    def raw(l, Ax):
+        s = [_prim_shell(l, a, Ax)] + [_prim_shell(0, e, r) for e, r in ((b, B), (c, Cc), (d, D))]
+        return O.eri_prim(O.pair_data(s[0], s[1]), O.pair_data(s[2], s[3]))[:, 0]
+    ex = np.array([1.0, 0, 0])
+    fd = (raw(2, A + h * ex)[0] - raw(2, A - h * ex)[0]) / (2 * h)   # component xx
+    fxxx = raw(3, A)[0]                                            # component xxx
+    px = raw(1, A)[0]                                              # component x
+    assert np.isclose(fd, 2 * a * fxxx - 2 * px, rtol=1e-7, atol=1e-12)
+
+
+def test_c_oracle_matches_numpy_oracle_f():
+    """The C restatement against the NumPy one for classes with an f shell
+    (the GPU f-shell tests use the C oracle as the checker)."""
+    import oracle.ref as R
+    from dft_amd.basis import build_shells_from_entries
+    ent = [[[[3], [1.3], [[1.0]]], [[1], [0.8], [[1.0]]]], [[[0], [0.5, 2.0], [[0.7, 0.7]]]]]
+    sh = build_shells_from_entries(ent, [[0, 0, 0], [0.4, -0.7, 0.9]])
+    seg = R.Segments(sh)
+    q = R.unique_quartets(seg.n)
+    keep = [k for k, x in enumerate(q) if sum(int(seg.l[i]) for i in x) <= 5]  # up to (fp|ps)-size classes
+    q = q[keep]
+    flat, sizes = R.eri_blocks(seg, q)
+    pairs = O.all_pairs(sh)
+    pos = 0
+    for (a, b, c, d), sz in zip(q, sizes):
+        ref = O.shell_quartet(sh, a, b, c, d, pairs).ravel()
+        assert np.allclose(flat[pos:pos + sz], ref, rtol=1e-12, atol=1e-14)
+        pos += sz
+
+
 def test_c_oracle_matches_numpy_oracle_d():
     import oracle.ref as R
     from dft_amd.basis import build_shells_from_entries
