@@ -278,8 +278,9 @@ def main():
             per_class.append({"class": "(%d%d|%d%d)" % combo_moments(c), "quartets": nq_c,
                               "model_flops": fc, "ms": ms, "tflops": fc / (ms * 1e-3) / 1e12,
                               "frac": fc / (ms * 1e-3) / FP64_PEAK})
-        return {"bound": "mfma", "pipe": "fp64 VALU: compute-bound; no MFMA instruction is issued by this "
-                                         "kernel, MI355X's FP64 VALU and FP64 MFMA peaks are both 78.6 TFLOP/s",
+        return {"bound": "fp64-valu", "pipe": "compute-bound on the FP64 vector ALUs (no MFMA instruction is "
+                                              "issued by this kernel); MI355X's FP64 VALU and FP64 MFMA peaks are "
+                                              "both 78.6 TFLOP/s",
                 "kernel": kname, "achieved": achieved, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
                 "frac": achieved * 1e12 / FP64_PEAK, "traffic": measured_traffic(kname),
                 "traffic_source": traffic_source(),

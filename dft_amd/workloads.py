@@ -256,7 +256,55 @@ def water16_grid(spacing=0.05, chunk=1 << 22, steps=5):
             "electrons": int(N_a + N_b)}
 
 
-WORKLOADS = ("water_rhf", "alkane_spd_mixed", "water_batch_scf", "alkane_gradient", "water16_grid")
+def alkane_spd_class_roofline(steps=3):
+    """C12H26 with the synthetic s/p/d basis (seed 1), Schwarz 1e-10: every
+    class combination's serial FP64 launch time (integrals + J/K digest, HIP
+    events recorded by the library on the launch stream) on two plans, the
+    heavy d classes on the column kernel and on the matrix-core variant,
+    with the algorithmic model flops of each class and the fraction of the
+    FP64 peak (78.6 TFLOP/s) of the faster variant."""
+    from .engine import profile_read, profile_start
+    from .flops import combo_flops, combo_moments
+    Z, xyz = alkane_bohr(12)
+    shells = build_shells_from_entries(synthetic_spd_basis(Z, seed=1), xyz)
+    n = shells[-1]["ao_stop"]
+    rng = np.random.default_rng(0)
+    A = rng.standard_normal((n, n)) * 0.02
+    Pa = torch.as_tensor((A + A.T) / 2, device="cuda").contiguous()
+    P = (2 * Pa).contiguous()
+    ms, st = {}, None
+    for mf in (False, True):
+        plan = Plan(shells, Z, xyz, schwarz_tau=1e-10, fp32_tau=1e-4, serial=True, mfma=mf)
+        J = torch.empty((n, n), dtype=torch.float64, device="cuda")
+        K = torch.empty_like(J)
+        for _ in range(2):
+            plan.jk_into(P, Pa, None, J, K, None, "fp64")
+        profile_start(plan, steps)
+        for _ in range(steps):
+            plan.jk_into(P, Pa, None, J, K, None, "fp64")
+        torch.cuda.synchronize()
+        t, nrec = profile_read(plan)
+        profile_start(plan, 0)
+        ms[mf] = np.asarray(t) / nrec
+        st = plan.stats()
+        del plan
+    per = st["per_combo"]
+    rows = []
+    for c in range(len(per)):
+        nq = int(per[c][0] + per[c][1])
+        if nq == 0:
+            continue
+        f = combo_flops(c, nq, int(per[c][4] + per[c][5]))
+        t0 = float(ms[False][2 * c] + ms[False][2 * c + 1])
+        t1 = float(ms[True][2 * c] + ms[True][2 * c + 1])
+        rows.append({"class": "(%d%d|%d%d)" % combo_moments(c), "quartets": nq, "model_flops": f,
+                     "ms_column": t0, "ms_mfma": t1, "frac_best": f / (min(t0, t1) * 1e-3) / 78.6e12})
+    return {"serial_build_ms_column": float(ms[False].sum()), "serial_build_ms_mfma": float(ms[True].sum()),
+            "per_class": rows}
+
+
+WORKLOADS = ("water_rhf", "alkane_spd_mixed", "water_batch_scf", "alkane_gradient", "water16_grid",
+             "alkane_spd_class_roofline")
 
 
 def main(argv=None):
