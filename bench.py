@@ -313,6 +313,8 @@ def main():
         "shell_quartets_per_s": quartets / (ms_per_step * 1e-3),
         "prim_shell_quartets_per_s": st["n_prim_quartets"] / (ms_per_step * 1e-3),
         "plan_seconds": t_plan,
+        # device workspace of one Fock build: half matrices, J/K staging slots
+        "fock_workspace_bytes": st["workspace_bytes"],
         "tune_seconds": t_tune,
         "tuned_shapes": None if shapes is None else [list(map(int, r)) for r in shapes if r.any()],
         "roofline": None,

@@ -151,7 +151,7 @@ def water_batch_scf(steps=5, warmup=2, tune_reps=2, scf_check=True):
     log(f"batched graph tuned: fp64 {ms_batch64:.2f} ms, mixed {ms_batch32:.2f} ms")
     out = {"n_molecules": len(mols), "n_quartets": st["n_quartets"], "n_quartets_fp32": st["n_quartets_fp32"],
            "n_unscreened": st["n_unscreened"], "n_prim_quartets": st["n_prim_quartets"], "plan_s": t_plan,
-           "tune_s": t_tune, "tuned_shapes": shapes.tolist(),
+           "fock_workspace_bytes": st["workspace_bytes"], "tune_s": t_tune, "tuned_shapes": shapes.tolist(),
            "ms_per_iter_plain_fp64_per_molecule": ms_plain64, "ms_per_iter_batched_fp64_eager_untuned": ms_batch64_eager,
            "ms_per_iter_batched_fp64_graph_tuned": ms_batch64, "ms_per_iter_batched_mixed_graph_tuned": ms_batch32,
            "speedup_batched_mixed_vs_plain_fp64": ms_plain64 / ms_batch32,

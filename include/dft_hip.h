@@ -74,11 +74,12 @@ const char* dft_last_error(void);
  * of calc_eri (shelled.py:634-640).
  *
  *  nsh                 number of segmented shells (one coefficient row each)
- *  shell_l[nsh]        angular momentum (0 = s, 1 = p, 2 = d)
+ *  shell_l[nsh]        angular momentum (0 = s, 1 = p, 2 = d, 3 = f)
  *  shell_nprim[nsh]    primitives per shell
  *  shell_ao[nsh]       first Cartesian AO of the shell (components follow in
- *                      the order x, y, z for p; xx, xy, yy, xz, yz, zz for d,
- *                      as combinations() of shelled.py:15-23)
+ *                      the order x, y, z for p; xx, xy, yy, xz, yz, zz for d;
+ *                      xxx, xxy, xyy, yyy, xxz, xyz, yyz, xzz, yzz, zzz for
+ *                      f, as combinations() of shelled.py:15-23)
  *  shell_xyz[3*nsh]    shell centre (bohr)
  *  prim_exp[], prim_coef[]  concatenated per shell: exponents and raw
  *                      contraction coefficients; primitive normalisation
@@ -87,6 +88,7 @@ const char* dft_last_error(void);
  *  nbf                 number of Cartesian AOs
  *  schwarz_tau         quartets with Q_ab Q_cd < schwarz_tau are skipped
  *  fp32_tau            quartets with Q_ab Q_cd < fp32_tau form the FP32 part
+ *                      (class combinations with an f shell stay FP64)
  *  flags               DFT_PLAN_UHF | DFT_PLAN_NOSPLIT | DFT_PLAN_SERIAL |
  *                      DFT_PLAN_STATIC_WORKSPACE | DFT_PLAN_MFMA or 0
  */
