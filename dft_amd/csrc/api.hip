@@ -158,6 +158,15 @@ struct Combo {
   // the primitive pairs and writing its own partial J/K blocks; chunk-major
   // slot order (slot row = (cb * splk + ck) * q.size() + quartet)
   int splb = 1, splk = 1;
+  // Lane table of a split lane-kernel combination (empty: the chunk-major
+  // lanes above): entry (quartet, first bra primitive, bra primitive count,
+  // idle), the bra chunks of a quartet in neighbouring entries of one wave
+  // (so the in-wave run sums add them) and splk ket chunks of each entry;
+  // waves end at molecule and part boundaries. lpos: table positions of
+  // the quartet positions 0, n64, n64 + n32a and q.size().
+  std::vector<int4> lanes;
+  int4* d_lanes = nullptr;
+  long long lpos[4] = {0, 0, 0, 0};
   // heavy (column-kernel) combinations: integrals of the whole list,
   // q.size() x NI x splb doubles at this offset of the workspace's integral
   // region, read by the digest kernel
@@ -277,6 +286,7 @@ void free_plan(dft_plan p) {
     (void)hipFree(c.d_q);
     (void)hipFree(c.d_slots);
     (void)hipFree(c.d_segmask);
+    (void)hipFree(c.d_lanes);
   }
   void* ptrs[] = {p->d_sh,    p->d_exps,  p->d_coefs,   p->d_pairs,    p->d_shell_dm,  p->d_shell_ao,
                   p->d_rec64, p->d_rec32[0], p->d_ws,      p->d_tg,       p->d_tbase,     p->d_tcount,
@@ -816,6 +826,57 @@ int build_plan(dft_plan p, const Input& in) {
   // molecule-major within each class combination; bra and ket always come
   // from the same molecule
   p->combos.assign(NCOMBO, Combo());
+  // lanes below which a lane-kernel combination is split (DFT_SPLIT_LANES
+  // overrides the default, for launch-shape studies)
+  long long split_lanes = SPLIT_LANES;
+  if (const char* e = std::getenv("DFT_SPLIT_LANES")) {
+    const long long v = std::atoll(e);
+    if (v > 0) split_lanes = v;
+  }
+  // Lane table of a split lane-kernel combination: ceil(n_b / bpl) entries
+  // per quartet (n_b its bra pair's FP64 primitive pairs), all in one wave;
+  // a wave is closed (filled with idle entries that repeat its last
+  // quartet, so its ket stays uniform) before a quartet that does not fit,
+  // at padding quartets (the end of a molecule's part), where the molecule
+  // changes and at the part boundaries n64 and n64 + n32a.
+  auto build_lane_table = [&](Combo& c, int bpl) {
+    std::vector<int4>& L = c.lanes;
+    L.clear();
+    auto close = [&]() {
+      if (L.empty() || L.size() % WAVE_QUARTETS == 0) return;
+      const int4 last = L.back();
+      while (L.size() % WAVE_QUARTETS) L.push_back(make_int4(last.x, 0, 0, 1));
+    };
+    const long long nq = (long long)c.q.size(), b1 = c.n64, b2 = c.n64 + c.n32a;
+    int prev_mol = -1;
+    for (long long i = 0; i < nq; ++i) {
+      if (i == b1) {
+        close();
+        c.lpos[1] = (long long)L.size();
+      }
+      if (i == b2) {
+        close();
+        c.lpos[2] = (long long)L.size();
+      }
+      const int2 qq = c.q[i];
+      if (qq.x < 0) {
+        close();
+        prev_mol = -1;
+        continue;
+      }
+      const int mol = p->sh[p->pairs[qq.x].x].mol;
+      if (mol != prev_mol) close();
+      prev_mol = mol;
+      const int nl = (std::max(1, p->pairs[qq.x].w) + bpl - 1) / bpl;
+      if ((long long)(L.size() % WAVE_QUARTETS) + nl > WAVE_QUARTETS) close();
+      for (int k = 0; k < nl; ++k) L.push_back(make_int4((int)i, k * bpl, bpl, 0));
+    }
+    close();
+    if (b1 >= nq) c.lpos[1] = (long long)L.size();
+    if (b2 >= nq) c.lpos[2] = (long long)L.size();
+    c.lpos[0] = 0;
+    c.lpos[3] = (long long)L.size();
+  };
   for (int pcb = 0; pcb < NPC; ++pcb)
     for (int pck = 0; pck <= pcb; ++pck) {
       Combo& c = p->combos[combo_index(pcb, pck)];
@@ -882,20 +943,34 @@ int build_plan(dft_plan p, const Input& in) {
       }
       if (!level2) c.n32a = (long long)c.q.size() - c.n64;
       // primitive split where the class has too few quartets to fill the
-      // chip with one lane each (~64 Ki lanes): bra chunks first, then ket
+      // chip with one lane each (split_lanes): bra chunks first, then ket.
+      // Bra chunks go to neighbouring lanes through a lane table (their
+      // blocks are summed in the wave); ket chunks stay chunk-major (one
+      // slot row each), so the wave keeps one ket primitive range.
       const long long nq = (long long)c.q.size();
-      if (p->split && nq > 0 && lane && nq < SPLIT_LANES) {
+      if (p->split && nq > 0 && lane && nq < split_lanes) {
         int nbmax = 0, nkmax = 0;
         for (const int2& qq : c.q) {
           if (qq.x < 0) continue;
           nbmax = std::max(nbmax, p->pairs[qq.x].w);
           nkmax = std::max(nkmax, p->pairs[qq.y].w);
         }
-        const long long want = (SPLIT_LANES + nq - 1) / nq;
-        c.splb = (int)std::max<long long>(1, std::min<long long>(want, nbmax));
-        c.splk = (int)std::max<long long>(1, std::min<long long>((want + c.splb - 1) / c.splb, nkmax));
+        const long long want = (split_lanes + nq - 1) / nq;
+        // bra primitives per lane, at most a wave of lanes per quartet
+        const int lpq = (int)std::max<long long>(1, std::min<long long>(want, nbmax));
+        int bpl = std::max(1, (nbmax + lpq - 1) / lpq);
+        bpl = std::max(bpl, (nbmax + WAVE_QUARTETS - 1) / WAVE_QUARTETS);
+        c.splb = std::max(1, (nbmax + bpl - 1) / bpl);
+        int splk = (int)std::max<long long>(1, std::min<long long>((want + c.splb - 1) / c.splb, nkmax));
+        if (nkmax > 0) {
+          const int szk = (nkmax + splk - 1) / splk;  // no empty ket chunk
+          splk = (nkmax + szk - 1) / szk;
+        }
+        c.splk = splk;
+        if (c.splb > 1) build_lane_table(c, bpl);
       }
-      if (nq * c.splb * c.splk > 0x7fffffffLL) return fail(DFT_EINVAL, "class combination exceeds 2^31 lanes");
+      const long long npos = c.lanes.empty() ? nq * c.splb : (long long)c.lanes.size();
+      if (npos * c.splk > 0x7fffffffLL) return fail(DFT_EINVAL, "class combination exceeds 2^31 lanes");
       if (!lane_kernel_combo(combo_index(pcb, pck)) && nq > 0) {
         int la, lb, lc, ld;
         pc_moments(pcb, la, lb);
@@ -972,14 +1047,16 @@ int build_plan(dft_plan p, const Input& in) {
   // digest sums the chunks' integrals first)
   auto slot_rows = [&](size_t ci) {
     const Combo& c = p->combos[ci];
-    return lane_kernel_combo((int)ci) ? c.splb * c.splk : 1;
+    if (!lane_kernel_combo((int)ci)) return 1;
+    return c.lanes.empty() ? c.splb * c.splk : c.splk;
   };
   // Lane-kernel combinations: the lanes of a wave (WAVE_QUARTETS consecutive
-  // quartets of one primitive chunk) whose block of kind k goes to the same
-  // target form runs; each run is summed across its lanes in the kernel and
-  // stored once, by its last lane. The runs depend only on the quartets, so
-  // they are found once per wave of the list (mask of run starts per block
-  // kind, run ends in (kind, lane) order) and shared by every chunk row.
+  // quartets of one primitive chunk, or WAVE_QUARTETS consecutive entries of
+  // the lane table) whose block of kind k goes to the same target form runs;
+  // each run is summed across its lanes in the kernel and stored once, by its
+  // last lane. The runs depend only on the quartets, so they are found once
+  // per wave of the list or table (mask of run starts per block kind, run
+  // ends in (kind, lane) order) and shared by every chunk row.
   constexpr size_t NO_TARGET = ~(size_t)0;
   struct WaveRuns {
     std::vector<unsigned long long> mask;  // per (wave, kind)
@@ -989,7 +1066,8 @@ int build_plan(dft_plan p, const Input& in) {
   };
   auto lane_runs = [&](size_t ci) {
     const Combo& c = p->combos[ci];
-    const size_t nq = c.q.size(), nw = nq / WAVE_QUARTETS;
+    const bool tab = !c.lanes.empty();
+    const size_t nq = tab ? c.lanes.size() : c.q.size(), nw = nq / WAVE_QUARTETS;
     WaveRuns r;
     r.mask.resize(nw * nblk);
     r.first.resize(nw + 1);
@@ -998,7 +1076,8 @@ int build_plan(dft_plan p, const Input& in) {
       r.first[w] = r.target.size();
       const size_t w0 = w * WAVE_QUARTETS;
       for (int l = 0; l < WAVE_QUARTETS; ++l) {
-        const int2 qq = c.q[w0 + l];
+        const int4 e = tab ? c.lanes[w0 + l] : make_int4((int)(w0 + l), 0, 0, 0);
+        const int2 qq = e.w ? make_int2(-1, 0) : c.q[e.x];
         if (qq.x < 0) {
           for (int k = 0; k < nblk; ++k) t[l * nblk + k] = NO_TARGET;
         } else {
@@ -1070,21 +1149,24 @@ int build_plan(dft_plan p, const Input& in) {
     Combo& c = p->combos[ci];
     const size_t nq = c.q.size(), ns = (size_t)slot_rows(ci);
     if (lane_kernel_combo((int)ci)) {
+      // positions: quartets, or lane-table entries
+      const size_t npos = c.lanes.empty() ? nq : c.lanes.size();
       // slot of each run's last lane (-1 for the other lanes), wave by wave
       // in (kind, lane) order; a run owns ns consecutive blocks of its
       // target, one per primitive chunk (the kernel adds ch times the block
       // size). This fixes every target's summation order.
       WaveRuns& r = runs[ci];
-      const size_t nw = nq / WAVE_QUARTETS;
-      std::vector<long long> slots(nq * nblk, -1);
+      const size_t nw = npos / WAVE_QUARTETS;
+      std::vector<long long> slots(npos * nblk, -1);
       for (size_t w = 0; w < nw; ++w)
         for (size_t e = r.first[w]; e < r.first[w + 1]; ++e) {
           const size_t tl = r.target[e];
-          slots[(size_t)r.kind[e] * nq + w * WAVE_QUARTETS + r.lane[e]] = tnext[tl];
+          slots[(size_t)r.kind[e] * npos + w * WAVE_QUARTETS + r.lane[e]] = tnext[tl];
           tnext[tl] += (long long)ns * tg[tl].nrow * tg[tl].ncol;
         }
       if ((rc = upload(&c.d_slots, slots))) return rc;
       if ((rc = upload(&c.d_segmask, r.mask))) return rc;
+      if (!c.lanes.empty() && (rc = upload(&c.d_lanes, c.lanes))) return rc;
       runs[ci] = WaveRuns();
       continue;
     }
@@ -1133,11 +1215,13 @@ RecView rec_view(dft_plan p, int ci, bool f32, int lev) {
   rv.n = 0;
   rv.ntot = 0;
   rv.splb = rv.splk = 1;
+  rv.lanes = nullptr;
   if (ci < (int)p->combos.size()) {
     const Combo& c = p->combos[ci];
-    rv.ntot = (int)c.q.size();
+    rv.ntot = (int)(c.lanes.empty() ? c.q.size() : c.lanes.size());
     rv.splb = c.splb;
     rv.splk = c.splk;
+    rv.lanes = c.d_lanes;
   }
   return rv;
 }
@@ -1155,6 +1239,20 @@ hipError_t launch_level(dft_plan p, const Workspace& ws, size_t ci, int lev, lon
   RecView rv = rec_view(p, (int)ci, f32, f32 ? lev : 0);
   rv.first = (int)first;
   rv.n = (int)n;
+  if (rv.lanes) {
+    // quartet range -> lane-table range (launches cover whole parts)
+    auto pos = [&](long long x) -> long long {
+      const long long b[4] = {0, c.n64, c.n64 + c.n32a, (long long)c.q.size()};
+      for (int k = 0; k < 4; ++k)
+        if (x == b[k]) return c.lpos[k];
+      return -1;
+    };
+    const long long lf = pos(first), le = pos(first + n);
+    if (lf < 0 || le < 0) return hipErrorInvalidValue;
+    rv.first = (int)lf;
+    rv.n = (int)(le - lf);
+    if (rv.n <= 0) return hipSuccess;
+  }
   const int4* prs = f32 ? p->d_pairs32[lev] : p->d_pairs;  // FP32 launches: that level's primitive lists
   if (lane_kernel_combo((int)ci))
     return launch_eri_jk((int)ci, f32, c.d_q, c.d_slots, c.d_segmask, prs, rv, p->d_shell_dm, P, Pa, Pb, p->nk,

@@ -75,6 +75,13 @@ struct RecView {
   // quartets [first, first + n) of the combination's list of ntot, each
   // evaluated by splb * splk lanes (primitive split)
   int first, n, ntot, splb, splk;
+  // Lane table of a split combination (nullptr: one lane per quartet and
+  // chunk as above). Entry i = (quartet, first bra primitive, bra primitive
+  // count, idle flag); the lanes of one quartet are adjacent in a wave, so
+  // the in-wave run sums add its bra chunks before anything is stored. With
+  // a table, first / n / ntot count table entries, and every entry runs
+  // splk times (ket chunk ck = chunk-major index, one slot row each).
+  const int4* lanes;
 };
 
 // Copies plain pair records (one record of rs reals per primitive pair at
@@ -88,7 +95,9 @@ hipError_t launch_interleave_records(const int4* map, int nrec, const double* r6
 // per quartet (-1 where the lane's block is summed into a later lane's; the
 // primitive chunk ch of a quartet adds ch times the block size); segmask one
 // word per (wave of the list, block kind) with bit l set where lane l starts
-// a run of lanes with the same target; block is the workgroup size
+// a run of lanes with the same target. With a lane table (rv.lanes) slots
+// and segmask are per table entry and per wave of the table, and only the
+// ket chunk adds to the slot; block is the workgroup size
 // (64 ... 512, at most lane_block_max), grid_cap the maximum number of
 // workgroups. Quartets with a negative bra pair are padding (no work, no
 // stores).

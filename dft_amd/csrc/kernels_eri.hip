@@ -48,7 +48,9 @@ __device__ __forceinline__ void store_runs(double (&v)[NE], unsigned long long m
 // (store_runs) and stored once at stage + slots[k * ntot + q] + ch * (block
 // size) for primitive chunk ch (block-major slot table, so that each slot
 // load of a wave is one contiguous run; a run's chunks own consecutive
-// blocks of its target).
+// blocks of its target). A split combination with a lane table puts the bra
+// chunks of a quartet in neighbouring lanes, so its runs also sum over bra
+// chunks; only the ket chunks keep blocks of their own.
 // With w = 2^-([a=b]+[c=d]+[ab=cd]):
 //   Jab = 2w sum_cd I P_cd        Jcd = 2w sum_ab I P_ab
 //   Kac = w sum_bd I P_bd  Kad = w sum_bc I P_bc  Kbc = w sum_ad I P_ad  Kbd = w sum_ac I P_ac
@@ -63,7 +65,7 @@ template <int LA, int LB, int LC, int LD, typename real, int NK>
 __global__ __launch_bounds__(jk_block_max(LA, LB, LC, LD)) void eri_jk_kernel(
     const int2* __restrict__ quartets, const long long* __restrict__ slots,
     const unsigned long long* __restrict__ segmask, int first, int n, int ntot, int splb, int splk,
-    const int4* __restrict__ pairs, const real* __restrict__ recb, int sb, int fb,
+    const int4* __restrict__ lanes, const int4* __restrict__ pairs, const real* __restrict__ recb, int sb, int fb,
     const real* __restrict__ reck, const int4* __restrict__ shell_dm, const double* __restrict__ DJ,
     const double* __restrict__ DK0, const double* __restrict__ DK1, double* __restrict__ stage,
     const double* __restrict__ boys_gtab) {
@@ -72,25 +74,48 @@ __global__ __launch_bounds__(jk_block_max(LA, LB, LC, LD)) void eri_jk_kernel(
   constexpr int NBLK = 2 + 4 * NK;
   __shared__ double btab[BOYS_LDS];
   boys_stage<S::L>(boys_gtab, btab);
-  // lane v: chunk ch = v / n (chunk-major, so the lanes of a wave share the
-  // chunk and mostly the ket pair), quartet first + v % n
-  const unsigned nv = (unsigned)n * (unsigned)(splb * splk);
+  // Without a lane table, lane v: chunk ch = v / n (chunk-major, so the
+  // lanes of a wave share the chunk and mostly the ket pair), quartet
+  // first + v % n. With one, lane v: ket chunk ck = v / n, table entry
+  // first + v % n (the bra chunks of a quartet are neighbouring entries).
+  const unsigned nv = (unsigned)n * (unsigned)(lanes ? splk : splb * splk);
   const unsigned stride = gridDim.x * blockDim.x;
   for (unsigned v = blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += stride) {
     const unsigned ch = v / (unsigned)n;
-    const int q = first + (int)(v - ch * (unsigned)n);
+    // idx: this lane's position in the slot table and the run masks
+    const int idx = first + (int)(v - ch * (unsigned)n);
+    int q = idx, ck = (int)ch, tb0 = 0, tbn = 0;
+    bool idle = false;
+    if (lanes) {
+      const int4 t = lanes[idx];
+      q = t.x;
+      tb0 = t.y;
+      tbn = t.z;
+      idle = t.w != 0;
+    } else {
+      ck = (int)ch % splk;
+    }
     const int2 qd = quartets[q];
-    // padding quartets (bra -1) take their ket as bra and evaluate nothing
-    const bool live = qd.x >= 0;
-    const int4 pb = pairs[live ? qd.x : qd.y];
+    // padding quartets (bra -1) take their ket as bra and evaluate nothing;
+    // idle table entries repeat a quartet of their wave and evaluate nothing
+    const bool live = qd.x >= 0 && !idle;
+    const int4 pb = pairs[qd.x >= 0 ? qd.x : qd.y];
     const int4 pk = pairs[qd.y];
     // this lane's primitive ranges [i0, i0 + nbp) of the bra pair and
-    // [j0, j0 + nkp) of the ket pair
-    const int cb = (int)ch / splk, ck = (int)ch - cb * splk;
-    const int szb = (pb.w + splb - 1) / splb, szk = (pk.w + splk - 1) / splk;
-    const int i0 = min(pb.w, cb * szb), j0 = min(pk.w, ck * szk);
-    const int nbp = live ? min(pb.w, i0 + szb) - i0 : 0, nkp = min(pk.w, j0 + szk) - j0;
-    const unsigned ob = (unsigned)((live ? qd.x : qd.y) - fb) + (unsigned)(i0 * S::RSB * sb);
+    // [j0, j0 + nkp) of the ket pair (a table entry's bra range is clipped
+    // to the pair's primitive count in the record set of this launch)
+    const int szk = (pk.w + splk - 1) / splk;
+    const int j0 = min(pk.w, ck * szk), nkp = min(pk.w, j0 + szk) - j0;
+    int i0, nbp;
+    if (lanes) {
+      i0 = min(pb.w, tb0);
+      nbp = live ? min(pb.w, tb0 + tbn) - i0 : 0;
+    } else {
+      const int cb = (int)ch / splk, szb = (pb.w + splb - 1) / splb;
+      i0 = min(pb.w, cb * szb);
+      nbp = live ? min(pb.w, i0 + szb) - i0 : 0;
+    }
+    const unsigned ob = (unsigned)((qd.x >= 0 ? qd.x : qd.y) - fb) + (unsigned)(i0 * S::RSB * sb);
     const unsigned ok = (unsigned)pk.z + (unsigned)(j0 * S::RSK);
     real I[S::NI];
     // When the whole wave shares the ket primitive range (the common case in
@@ -119,14 +144,17 @@ __global__ __launch_bounds__(jk_block_max(LA, LB, LC, LD)) void eri_jk_kernel(
     const long long mb = (long long)(unsigned)sa.z | ((long long)sa.w << 32);
     const double* DJm = DJ + mb;
     const long long sstr = ntot;
-    const long long* sl = slots + q;
+    const long long* sl = slots + idx;
     // slot of block kind k for this lane's chunk (the chunks of a run own
-    // consecutive blocks of ne elements)
-    auto at = [&](long long s, int ne) { return s >= 0 ? s + (long long)ch * ne : -1LL; };
-    // the wave's index in the combination's quartet list, the same for its
-    // 64 lanes (n and first are multiples of the wave, and so its chunk)
+    // consecutive blocks of ne elements; with a lane table the bra chunks
+    // are summed in the wave and only the ket chunk has blocks of its own)
+    const int srow = lanes ? ck : (int)ch;
+    auto at = [&](long long s, int ne) { return s >= 0 ? s + (long long)srow * ne : -1LL; };
+    // the wave's index in the combination's quartet list (or lane table),
+    // the same for its 64 lanes (n and first are multiples of the wave, and
+    // so its chunk)
     const int lane = threadIdx.x & 63;
-    const unsigned wid = (unsigned)__builtin_amdgcn_readfirstlane(q / WAVE_QUARTETS);
+    const unsigned wid = (unsigned)__builtin_amdgcn_readfirstlane(idx / WAVE_QUARTETS);
     const unsigned long long* msk = segmask + (size_t)wid * NBLK;
     // Coulomb
     {
@@ -259,17 +287,17 @@ static hipError_t jk_one(const int2* q, const long long* sl, const unsigned long
   if (rv.n <= 0) return hipSuccess;
   // whole waves only: every wave runs its shuffles with all 64 lanes
   if (rv.n % WAVE_QUARTETS || rv.first % WAVE_QUARTETS || rv.ntot % WAVE_QUARTETS) return hipErrorInvalidValue;
-  const long long nq = (long long)rv.n * rv.splb * rv.splk;
+  const long long nq = (long long)rv.n * (rv.lanes ? 1 : rv.splb) * rv.splk;
   if (blk != 64 && blk != 128 && blk != 256 && blk != 512) return hipErrorInvalidValue;
   if (blk > jk_block_max(LA, LB, LC, LD)) return hipErrorInvalidValue;
   dim3 grid(grid_for(nq, cap, blk)), block(blk);
   if (nk == 1)
     hipLaunchKernelGGL((eri_jk_kernel<LA, LB, LC, LD, real, 1>), grid, block, 0, s, q, sl, sm, rv.first, rv.n,
-                       rv.ntot, rv.splb, rv.splk, pairs, (const real*)rv.bra, rv.sb, rv.fb, (const real*)rv.ket,
+                       rv.ntot, rv.splb, rv.splk, rv.lanes, pairs, (const real*)rv.bra, rv.sb, rv.fb, (const real*)rv.ket,
                        ao, DJ, DK0, DK1, stage, bt);
   else
     hipLaunchKernelGGL((eri_jk_kernel<LA, LB, LC, LD, real, 2>), grid, block, 0, s, q, sl, sm, rv.first, rv.n,
-                       rv.ntot, rv.splb, rv.splk, pairs, (const real*)rv.bra, rv.sb, rv.fb, (const real*)rv.ket,
+                       rv.ntot, rv.splb, rv.splk, rv.lanes, pairs, (const real*)rv.bra, rv.sb, rv.fb, (const real*)rv.ket,
                        ao, DJ, DK0, DK1, stage, bt);
   return hipGetLastError();
 }

@@ -146,6 +146,39 @@ def test_primitive_split_matches_unsplit(torch_cuda, uhf):
             assert float((x - y).abs().max()) < 1e-12
 
 
+@pytest.mark.parametrize("uhf", [False, True])
+def test_lane_table_split_everywhere_matches_unsplit(torch_cuda, monkeypatch, uhf):
+    """A split threshold above every class's quartet count puts each s/p
+    class (and the light d classes) through lane tables: bra chunks in
+    neighbouring lanes, waves closed at molecule and FP64 / FP32 part
+    boundaries, ket chunks as slot rows. FP64 and mixed builds of single and
+    batch plans equal the unsplit builds to rounding (mixed: the FP32 partial
+    integrals of a chunk are rounded before the chunks are added)."""
+    torch = torch_cuda
+    from dft_amd.engine import BatchPlan, Plan
+    mols = _mols()
+    monkeypatch.setenv("DFT_SPLIT_LANES", str(1 << 26))
+    split = [Plan(sh, Z, xyz, schwarz_tau=1e-12, fp32_tau=1e-4, uhf=uhf) for sh, Z, xyz in mols]
+    bsplit = BatchPlan(mols, schwarz_tau=1e-12, fp32_tau=1e-4, uhf=uhf)
+    monkeypatch.delenv("DFT_SPLIT_LANES")
+    plain = [Plan(sh, Z, xyz, schwarz_tau=1e-12, fp32_tau=1e-4, uhf=uhf, split=False) for sh, Z, xyz in mols]
+    rng = np.random.default_rng(7)
+    Pa = [_dens(p.nbf, rng) for p in plain]
+    Pb = [_dens(p.nbf, rng) for p in plain]
+    for mode, tol in (("fp64", 1e-12), ("mixed", 1e-10)):
+        Jb, Kab, Kbb = bsplit.jk(bsplit.pack([a + b for a, b in zip(Pa, Pb)]), bsplit.pack(Pa),
+                                  bsplit.pack(Pb) if uhf else None, mode=mode)
+        for m in range(len(mols)):
+            rs = split[m].jk(Pa[m] + Pb[m], Pa[m], Pb[m] if uhf else None, mode=mode)
+            rp = plain[m].jk(Pa[m] + Pb[m], Pa[m], Pb[m] if uhf else None, mode=mode)
+            rb = (bsplit.unpack(Jb)[m], bsplit.unpack(Kab)[m], bsplit.unpack(Kbb)[m] if uhf else None)
+            for x, y, z in zip(rs, rp, rb):
+                if x is None:
+                    continue
+                assert float((x - y).abs().max()) < tol, (mode, m)
+                assert float((z - y).abs().max()) < tol, (mode, m)
+
+
 def test_serial_and_concurrent_launches_bitwise_equal(torch_cuda):
     """The class launches of a build write disjoint staging slots and the
     reduction order is fixed, so running them on one stream or spread over
