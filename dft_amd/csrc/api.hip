@@ -24,8 +24,19 @@ using namespace dft;
 
 namespace {
 
-// lanes below which a lane-kernel class combination is split over primitives
+// Primitive split of the lane-kernel combinations. Every lane of a split
+// quartet repeats the J/K digest and the run sums of its blocks, which for
+// the p classes costs about as much as a few primitive quartets, so a split
+// pays only while the class leaves the chip short of waves: tiny classes
+// (below SPLIT_LANES / 8 quartets) are split up to about SPLIT_LANES lanes,
+// classes below SPLIT_PAIR_BELOW quartets into two lanes per quartet.
+// (H2O)16 STO-3G: (pp|pp) takes 42 us with 27 lanes per quartet and 74 us
+// with 45; a pair split of (ps|ps) and (pp|ss) (126 k and 79 k quartets)
+// shortens their serial launches (63.7 -> 55.4 us, 59.1 -> 53.9 us) but
+// lengthens the concurrent build (0.321 -> 0.330 ms), where the large
+// classes already fill the chip and only the added digest work remains.
 constexpr long long SPLIT_LANES = 65536;
+constexpr long long SPLIT_PAIR_BELOW = 65536;
 // the same for the column kernel of the heavy classes (longer lanes)
 constexpr long long SPLIT_LANES_HEAVY = 262144;
 // FP32 records drop primitive pairs whose summed Schwarz bounds stay within
@@ -159,7 +170,7 @@ struct Combo {
   // slot order (slot row = (cb * splk + ck) * q.size() + quartet)
   int splb = 1, splk = 1;
   // Lane table of a split lane-kernel combination (empty: the chunk-major
-  // lanes above): entry (quartet, first bra primitive, bra primitive count,
+  // lanes above): entry (quartet, first bra primitive, bra primitive stride,
   // idle), the bra chunks of a quartet in neighbouring entries of one wave
   // (so the in-wave run sums add them) and splk ket chunks of each entry;
   // waves end at molecule and part boundaries. lpos: table positions of
@@ -826,15 +837,23 @@ int build_plan(dft_plan p, const Input& in) {
   // molecule-major within each class combination; bra and ket always come
   // from the same molecule
   p->combos.assign(NCOMBO, Combo());
-  // lanes below which a lane-kernel combination is split (DFT_SPLIT_LANES
-  // overrides the default, for launch-shape studies)
-  long long split_lanes = SPLIT_LANES;
+  // lanes of the full split of a tiny lane-kernel combination (SPLIT_LANES;
+  // DFT_SPLIT_LANES overrides it, for launch-shape studies) and the quartet
+  // count below which a combination gets two lanes per quartet
+  long long split_lanes = SPLIT_LANES, split_pair = SPLIT_PAIR_BELOW;
   if (const char* e = std::getenv("DFT_SPLIT_LANES")) {
     const long long v = std::atoll(e);
     if (v > 0) split_lanes = v;
   }
+  if (const char* e = std::getenv("DFT_SPLIT_PAIR_BELOW")) {
+    const long long v = std::atoll(e);
+    if (v >= 0) split_pair = v;
+  }
   // Lane table of a split lane-kernel combination: ceil(n_b / bpl) entries
   // per quartet (n_b its bra pair's FP64 primitive pairs), all in one wave;
+  // entry k of a quartet with ns entries takes bra primitives k, k + ns, ...
+  // (the primitives are stored in decreasing Schwarz factor, so every entry
+  // gets a like share of those above the primitive-quartet cut);
   // a wave is closed (filled with idle entries that repeat its last
   // quartet, so its ket stays uniform) before a quartet that does not fit,
   // at padding quartets (the end of a molecule's part), where the molecule
@@ -869,7 +888,7 @@ int build_plan(dft_plan p, const Input& in) {
       prev_mol = mol;
       const int nl = (std::max(1, p->pairs[qq.x].w) + bpl - 1) / bpl;
       if ((long long)(L.size() % WAVE_QUARTETS) + nl > WAVE_QUARTETS) close();
-      for (int k = 0; k < nl; ++k) L.push_back(make_int4((int)i, k * bpl, bpl, 0));
+      for (int k = 0; k < nl; ++k) L.push_back(make_int4((int)i, k, nl, 0));
     }
     close();
     if (b1 >= nq) c.lpos[1] = (long long)L.size();
@@ -943,19 +962,21 @@ int build_plan(dft_plan p, const Input& in) {
       }
       if (!level2) c.n32a = (long long)c.q.size() - c.n64;
       // primitive split where the class has too few quartets to fill the
-      // chip with one lane each (split_lanes): bra chunks first, then ket.
-      // Bra chunks go to neighbouring lanes through a lane table (their
-      // blocks are summed in the wave); ket chunks stay chunk-major (one
-      // slot row each), so the wave keeps one ket primitive range.
+      // chip with one lane each: bra chunks first, then ket. Bra chunks go
+      // to neighbouring lanes through a lane table (their blocks are summed
+      // in the wave); ket chunks stay chunk-major (one slot row each), so the
+      // wave keeps one ket primitive range.
       const long long nq = (long long)c.q.size();
-      if (p->split && nq > 0 && lane && nq < split_lanes) {
+      const long long want = nq <= 0 ? 1
+                             : nq < split_lanes / 8 ? (split_lanes + nq - 1) / nq
+                             : nq < split_pair ? 2 : 1;
+      if (p->split && lane && want > 1) {
         int nbmax = 0, nkmax = 0;
         for (const int2& qq : c.q) {
           if (qq.x < 0) continue;
           nbmax = std::max(nbmax, p->pairs[qq.x].w);
           nkmax = std::max(nkmax, p->pairs[qq.y].w);
         }
-        const long long want = (split_lanes + nq - 1) / nq;
         // bra primitives per lane, at most a wave of lanes per quartet
         const int lpq = (int)std::max<long long>(1, std::min<long long>(want, nbmax));
         int bpl = std::max(1, (nbmax + lpq - 1) / lpq);

@@ -79,12 +79,15 @@ constexpr float PRIM_QCUT32 = 1e-13f;
 // primitive pairs of a pair are stored in decreasing s, so each lane stops
 // its bra loop at the first bra primitive below the cut with the largest ket
 // factor, and the ket loop ends once no lane of the wave is above it (the
-// terms a lane leaves out are exactly its own sub-cut ones).
+// terms a lane leaves out are exactly its own sub-cut ones). The bra loop
+// visits nb primitive pairs bstride apart (a lane of a split quartet takes
+// every bstride-th one, still in decreasing s).
 template <int LA, int LB, int LC, int LD, typename real, bool CUT = false, int CD0 = 0, int NCDK = -1>
 __device__ __forceinline__ void quartet_integrals(
     const real* __restrict__ bb, int sb, unsigned ob, int nb, const real* __restrict__ bk, int sk, unsigned ok,
     int nk, const double* __restrict__ btab,
-    real (&I)[QuartetShape<LA, LB, LC, LD, real>::NAB * ncd_slice<LA, LB, LC, LD, NCDK>()], real qcut = real(0)) {
+    real (&I)[QuartetShape<LA, LB, LC, LD, real>::NAB * ncd_slice<LA, LB, LC, LD, NCDK>()], real qcut = real(0),
+    int bstride = 1) {
   using S = QuartetShape<LA, LB, LC, LD, real>;
   constexpr int NK = ncd_slice<LA, LB, LC, LD, NCDK>();
   constexpr int EXB = REC_HEAD, EYB = REC_HEAD + S::NEB, EZB = REC_HEAD + 2 * S::NEB;
@@ -93,7 +96,7 @@ __device__ __forceinline__ void quartet_integrals(
   for (int x = 0; x < S::NAB * NK; ++x) I[x] = real(0);
   const real sk0 = CUT && nk > 0 ? fld(bk, sk, 5, ok) : real(0);  // largest ket factor
   for (int i = 0; i < nb; ++i) {
-    const unsigned oi = ob + (unsigned)(i * S::RSB * sb);
+    const unsigned oi = ob + (unsigned)(i * bstride * S::RSB * sb);
     const real sab = CUT ? fld(bb, sb, 5, oi) : real(0);
     if (CUT && sab * sk0 < qcut) break;
     const real p = fld(bb, sb, 0, oi), Px = fld(bb, sb, 1, oi), Py = fld(bb, sb, 2, oi);

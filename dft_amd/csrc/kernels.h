@@ -77,7 +77,8 @@ struct RecView {
   int first, n, ntot, splb, splk;
   // Lane table of a split combination (nullptr: one lane per quartet and
   // chunk as above). Entry i = (quartet, first bra primitive, bra primitive
-  // count, idle flag); the lanes of one quartet are adjacent in a wave, so
+  // stride, idle flag): the entry takes bra primitive pairs first, first +
+  // stride, ...; the lanes of one quartet are adjacent in a wave, so
   // the in-wave run sums add its bra chunks before anything is stored. With
   // a table, first / n / ntot count table entries, and every entry runs
   // splk times (ket chunk ck = chunk-major index, one slot row each).

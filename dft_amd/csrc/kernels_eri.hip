@@ -84,13 +84,13 @@ __global__ __launch_bounds__(jk_block_max(LA, LB, LC, LD)) void eri_jk_kernel(
     const unsigned ch = v / (unsigned)n;
     // idx: this lane's position in the slot table and the run masks
     const int idx = first + (int)(v - ch * (unsigned)n);
-    int q = idx, ck = (int)ch, tb0 = 0, tbn = 0;
+    int q = idx, ck = (int)ch, tb0 = 0, tbs = 1;
     bool idle = false;
     if (lanes) {
       const int4 t = lanes[idx];
       q = t.x;
       tb0 = t.y;
-      tbn = t.z;
+      tbs = max(1, t.z);
       idle = t.w != 0;
     } else {
       ck = (int)ch % splk;
@@ -101,15 +101,17 @@ __global__ __launch_bounds__(jk_block_max(LA, LB, LC, LD)) void eri_jk_kernel(
     const bool live = qd.x >= 0 && !idle;
     const int4 pb = pairs[qd.x >= 0 ? qd.x : qd.y];
     const int4 pk = pairs[qd.y];
-    // this lane's primitive ranges [i0, i0 + nbp) of the bra pair and
-    // [j0, j0 + nkp) of the ket pair (a table entry's bra range is clipped
-    // to the pair's primitive count in the record set of this launch)
+    // this lane's primitive pairs of the bra pair (nbp of them from i0,
+    // bstride apart) and of the ket pair ([j0, j0 + nkp)); a table entry
+    // takes bra primitives tb0, tb0 + tbs, ... below the pair's count in the
+    // record set of this launch
     const int szk = (pk.w + splk - 1) / splk;
     const int j0 = min(pk.w, ck * szk), nkp = min(pk.w, j0 + szk) - j0;
-    int i0, nbp;
+    int i0, nbp, bstride = 1;
     if (lanes) {
       i0 = min(pb.w, tb0);
-      nbp = live ? min(pb.w, tb0 + tbn) - i0 : 0;
+      nbp = live ? (pb.w - i0 + tbs - 1) / tbs : 0;
+      bstride = tbs;
     } else {
       const int cb = (int)ch / splk, szb = (pb.w + splb - 1) / splb;
       i0 = min(pb.w, cb * szb);
@@ -128,9 +130,11 @@ __global__ __launch_bounds__(jk_block_max(LA, LB, LC, LD)) void eri_jk_kernel(
     // cut per pair, and the test costs a larger share of the FP32 loop)
     constexpr bool cut = std::is_same<real, double>::value;
     if (__all(ok == okf && nkp == nkf)) {
-      quartet_integrals<LA, LB, LC, LD, real, cut>(recb, sb, ob, nbp, reck, 1, okf, nkf, btab, I, PRIM_QCUT64);
+      quartet_integrals<LA, LB, LC, LD, real, cut>(recb, sb, ob, nbp, reck, 1, okf, nkf, btab, I, PRIM_QCUT64,
+                                                   bstride);
     } else {
-      quartet_integrals<LA, LB, LC, LD, real, cut>(recb, sb, ob, nbp, reck, 1, ok, nkp, btab, I, PRIM_QCUT64);
+      quartet_integrals<LA, LB, LC, LD, real, cut>(recb, sb, ob, nbp, reck, 1, ok, nkp, btab, I, PRIM_QCUT64,
+                                                   bstride);
     }
     double w = 1.0;
     if (pb.x == pb.y) w *= 0.5;
