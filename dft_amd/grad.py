@@ -22,7 +22,7 @@ from . import _lib
 from .basis import build_shells, build_shells_from_entries
 from .engine import Plan
 from .linalg import JACOBI_NMAX
-from .scf import electron_counts, scf_diis, scf_diis_graph
+from .scf import electron_counts, sad_guess, scf_diis, scf_diis_graph
 
 
 def nuclear_repulsion_gradient(Z, xyz):
@@ -68,11 +68,21 @@ def _shells(Z, xyz, basis):
 
 
 def energy_and_gradient(Z, xyz, basis, charge=0, mult=1, schwarz_tau=1e-12, e_tol=1e-11, p_tol=1e-9, guess=None):
-    """(E, dE/dR [natom, 3], SCF result) at bohr coordinates ``xyz``;
-    ``guess`` = (P_a, P_b) starts the SCF from those densities."""
+    """(E, dE/dR [natom, 3], SCF result) at bohr coordinates ``xyz``.
+    ``guess`` = (P_a, P_b) starts the SCF from those densities; None starts
+    from the superposition of atomic densities (``scf.sad_guess``, atomic
+    densities cached per element and basis), "core" from the core
+    Hamiltonian (the reference's start). The converged energy and gradient
+    do not depend on the start."""
     xyz = np.asarray(xyz, dtype=np.float64).reshape(-1, 3)
     shells = _shells(Z, xyz, basis)
     N_a, N_b = electron_counts(Z, charge, mult)
+    if isinstance(guess, str):
+        if guess != "core":
+            raise ValueError("guess must be (P_a, P_b), None or 'core'")
+        guess = None
+    elif guess is None:
+        guess = sad_guess(shells, Z, N_a, N_b)
     plan = Plan(shells, Z, xyz, schwarz_tau=schwarz_tau, fp32_tau=0.0, uhf=N_a != N_b)
     # undamped DIIS from the first iteration: the gradient needs a tightly
     # converged density, which damping (the reference's schedule) slows down.

@@ -465,3 +465,86 @@ def scf_diis_graph(plan: Plan, Z, centers, N_occ, mode="fp64", rate=0.4, diis_th
     Cf = X @ Cp
     return SCFResult(E_fin, converged, count, Pa.clone(), Pa.clone(), Ff, Ff, Cf, Cf, ea, ea, fock.times,
                      iter_times, energies)
+
+
+# ---------------------------------------------------------------------------
+# Superposition of atomic densities (SAD) as the SCF starting density
+
+_ATOM_DENSITY: dict = {}
+
+
+def _atom_shell_key(z, shells) -> tuple:
+    return (int(z),) + tuple((int(s["angular_momentum"]), np.asarray(s["exponents"], dtype=np.float64).tobytes(),
+                              np.asarray(s["coefficients"], dtype=np.float64).tobytes()) for s in shells)
+
+
+def atomic_density(z, shells, max_iter=200, tol=1e-10) -> np.ndarray:
+    """Total density of the neutral atom of charge ``z`` in the basis
+    ``shells`` (the atom's shell dicts, any centre): restricted Hartree-Fock
+    with spherically averaged fractional occupations (two electrons per
+    orbital in energy order, the last set of degenerate orbitals sharing the
+    remainder equally), damped. The integrals (S, T, V and the ERI tensor)
+    come from the library's kernels; the atomic matrices are a few functions
+    wide, so the loop itself runs in NumPy. Cached per element and basis."""
+    from .basis import build_shells_from_entries
+    key = _atom_shell_key(z, shells)
+    hit = _ATOM_DENSITY.get(key)
+    if hit is not None:
+        return hit
+    entries = [([int(s["angular_momentum"])], np.asarray(s["exponents"]), np.asarray(s["coefficients"]))
+               for s in shells]
+    ash = build_shells_from_entries([entries], np.zeros((1, 3)))
+    plan = Plan(ash, [z], np.zeros((1, 3)), schwarz_tau=0.0, fp32_tau=0.0)
+    S, T, V = (M.cpu().numpy() for M in plan.one_electron())
+    eri = plan.eri_tensor()
+    n = plan.nbf
+    H = T + V
+    w, U = np.linalg.eigh(S)
+    X = U @ np.diag(w ** -0.5) @ U.T
+    nel = float(z)
+    P = np.zeros((n, n))
+    E_old = None
+    for it in range(max_iter):
+        F = H + np.einsum("abcd,cd->ab", eri, P) - 0.5 * np.einsum("acbd,cd->ab", eri, P)
+        E = 0.5 * float(np.sum(P * (H + F)))
+        eps, Cp = np.linalg.eigh(X.T @ F @ X)
+        Cm = X @ Cp
+        occ = np.zeros(n)
+        left, i = nel, 0
+        while left > 1e-12 and i < n:
+            j = i
+            while j + 1 < n and abs(eps[j + 1] - eps[i]) <= 1e-6 * max(1.0, abs(eps[i])):
+                j += 1
+            g = j - i + 1  # degenerate set i..j
+            put = min(left, 2.0 * g)
+            occ[i:j + 1] = put / g
+            left -= put
+            i = j + 1
+        Pn = (Cm * occ) @ Cm.T
+        P = Pn if it == 0 else 0.5 * (P + Pn)
+        if E_old is not None and abs(E - E_old) < tol and np.abs(Pn - P).max() < 1e-8:
+            break
+        E_old = E
+    _ATOM_DENSITY[key] = P
+    return P
+
+
+def sad_guess(shells, Z, N_a, N_b):
+    """Alpha and beta starting densities (NumPy) from the superposition of
+    the neutral atoms' densities (``atomic_density``), scaled to N_a and N_b
+    electrons. The atoms' functions are contiguous in the shell order."""
+    n = int(shells[-1]["ao_stop"])
+    P = np.zeros((n, n))
+    by_atom: dict = {}
+    for s in shells:
+        by_atom.setdefault(int(s["atom_idx"]), []).append(s)
+    nel = 0.0
+    for a, sh in by_atom.items():
+        lo, hi = int(sh[0]["ao_start"]), int(sh[-1]["ao_stop"])
+        if sum(int(s["n_ao"]) for s in sh) != hi - lo:
+            raise ValueError(f"the functions of atom {a} are not contiguous in the shell order")
+        P[lo:hi, lo:hi] = atomic_density(Z[a], sh)
+        nel += float(Z[a])
+    if nel <= 0:
+        raise ValueError("no electrons in the superposition of atomic densities")
+    return P * (N_a / nel), P * (N_b / nel)

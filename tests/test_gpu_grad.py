@@ -124,3 +124,40 @@ def test_scf_guess_gives_same_energy_and_gradient(torch_cuda):
     E2, g2, res2 = energy_and_gradient(Z, xyz, basis, guess=(res0.P_a, res0.P_b))
     assert abs(E1 - E2) < 1e-9 and np.abs(g1 - g2).max() < 1e-7
     assert res2.iterations <= 30
+
+
+def test_atomic_densities_hold_the_atoms_electrons(torch_cuda):
+    """Spherically averaged atomic RHF densities of the SAD guess: Tr(P S)
+    equals the nuclear charge, P is symmetric, and hydrogen's single
+    contracted 1s function holds one electron (P = 1 / S_11 = 1)."""
+    from dft_amd.basis import build_shells, load_basis
+    from dft_amd.engine import Plan
+    from dft_amd.scf import atomic_density
+    basis = load_basis()
+    for z in (1, 6, 8):
+        sh = build_shells(basis, [z], np.zeros((1, 3)))
+        P = atomic_density(z, sh)
+        S = Plan(sh, [z], np.zeros((1, 3)), schwarz_tau=0.0).one_electron()[0].cpu().numpy()
+        assert abs(np.trace(P @ S) - z) < 1e-10, z
+        assert np.abs(P - P.T).max() < 1e-12
+        if z == 1:
+            assert abs(P[0, 0] * S[0, 0] - 1.0) < 1e-12
+
+
+def test_sad_and_core_guesses_give_the_same_state(torch_cuda):
+    """The superposition-of-atomic-densities start (the default of
+    energy_and_gradient) and the core-Hamiltonian start (the reference's)
+    converge to the same energy and gradient, in both the graph loop
+    (closed shell: ethane, water; there the SAD start takes no more
+    iterations) and the eager UHF loop (doublet water cation)."""
+    from dft_amd.basis import load_basis
+    from dft_amd.geometry import alkane_bohr, water
+    from dft_amd.grad import energy_and_gradient
+    basis = load_basis()
+    Z, xyz = alkane_bohr(2, noise_sigma_ang=0.05, seed=3)
+    for Zm, xm, charge, mult in ((Z, xyz, 0, 1), (*water(), 0, 1), (*water(), 1, 2)):
+        E1, g1, r1 = energy_and_gradient(Zm, xm, basis, charge, mult)
+        E2, g2, r2 = energy_and_gradient(Zm, xm, basis, charge, mult, guess="core")
+        assert abs(E1 - E2) < 1e-9 and np.abs(g1 - g2).max() < 1e-7, (charge, mult)
+        if mult == 1:
+            assert r1.iterations <= r2.iterations, (r1.iterations, r2.iterations)
