@@ -5,7 +5,9 @@ uniform random quaternion (seed 0); STO-3G, 112 basis functions, 80 shells;
 Schwarz threshold 1e-10. A few plain-SCF iterations give a realistic
 density; then ``--steps`` direct J/K builds are timed back to back on the
 device (one step = one Fock build = every screened shell quartet evaluated
-and digested into J and K). The metric value is unique (8-fold) contracted
+and digested into J and K), each one replay of the captured hipGraph of the
+build, as the device SCF iteration runs it (the same builds issued eagerly,
+with an event fork/join per build, are timed beside it). The metric value is unique (8-fold) contracted
 Cartesian AO quartets surviving screening per second, counted at shell level
 (prod n(l) per screened unique shell quartet); shell and primitive shell
 quartets/s are reported beside it; ms_per_step is the Fock-build time per
@@ -165,7 +167,7 @@ def main():
 
     import torch
     from dft_amd.basis import build_shells, load_basis
-    from dft_amd.engine import Plan, profile_read, profile_start, tune
+    from dft_amd.engine import Graph, Plan, profile_read, profile_start, tune
     from dft_amd.flops import build_flops, combo_flops
     from dft_amd.geometry import water_cluster
     from dft_amd.scf import scf_diis, scf_diis_graph, scf_plain
@@ -194,6 +196,28 @@ def main():
         for _ in range(k):
             plan.jk_into(P, P_a, None, J, K, None, mode)
 
+    graphs = {}
+
+    def run_graph(k, mode):
+        """The build as the captured SCF iteration runs it: one replay of the
+        hipGraph of the build (class launches on four streams, reduction,
+        symmetrisation; the same kernels and launch shapes as jk_into)."""
+        g = graphs.get(mode)
+        if g is None:
+            g = graphs[mode] = Graph(plan, P, P_a, None, J, K, None, mode)
+        for _ in range(k):
+            g.launch()
+
+    def time_builds(fn, mode, k):
+        fn(args.warmup, mode)
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        fn(k, mode)
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / k
+
     # launch shapes per class combination from the autotuner (untimed)
     shapes, t_tune = None, 0.0
     if not args.no_tune:
@@ -202,25 +226,29 @@ def main():
         t_tune = time.perf_counter() - t0
         log(f"tuned in {t_tune:.1f} s")
 
-    # warmup, then the timed region (no per-launch events inside it)
-    run(args.warmup, "fp64")
+    # warmup, then the timed region (no per-launch events inside it): K
+    # replays of the captured build, as the device SCF iteration runs it
+    run_graph(args.warmup, "fp64")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     e0.record()
-    run(args.steps, "fp64")
+    run_graph(args.steps, "fp64")
     e1.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_total = e0.elapsed_time(e1)
     ms_per_step = ms_total / args.steps
+    # the same builds issued eagerly (jk_into: class launches forked to the
+    # auxiliary streams and joined back by events, per build)
+    ms_eager = time_builds(run, "fp64", args.steps)
     # per-build times (outside the timed region: one event pair per build),
     # median over builds 2..K
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
     ev[0].record()
     for i in range(args.steps):
-        run(1, "fp64")
+        run_graph(1, "fp64")
         ev[i + 1].record()
     torch.cuda.synchronize()
     per_build = [ev[i].elapsed_time(ev[i + 1]) for i in range(args.steps)]
@@ -310,6 +338,8 @@ def main():
                       "rows": [list(map(int, r)) for r in per]},
         "s_per_iter_fock_fp64": ms_per_step * 1e-3,
         "s_per_iter_fock_fp64_median_2_to_K": ms_median * 1e-3,
+        "build_launch": "hipGraph replay of the captured build (the device SCF iteration's path)",
+        "s_per_iter_fock_fp64_eager": ms_eager * 1e-3,
         "shell_quartets_per_s": quartets / (ms_per_step * 1e-3),
         "prim_shell_quartets_per_s": st["n_prim_quartets"] / (ms_per_step * 1e-3),
         "plan_seconds": t_plan,
@@ -325,14 +355,9 @@ def main():
         print(json.dumps(out), flush=True)
         return
 
-    # mixed precision
-    run(args.warmup, "mixed")
-    torch.cuda.synchronize()
-    e0.record()
-    run(args.steps, "mixed")
-    e1.record()
-    torch.cuda.synchronize()
-    ms_mixed = e0.elapsed_time(e1) / args.steps
+    # mixed precision (graph replays, as the FP64 line; eager beside it)
+    ms_mixed = time_builds(run_graph, "mixed", args.steps)
+    ms_mixed_eager = time_builds(run, "mixed", args.steps)
     # converged SCF energies in both modes (damped + DIIS loop; the plain
     # Roothaan loop oscillates on this cluster), on an untuned plan of the
     # same molecule so that every FP32-eligible quartet runs in FP32 (the
@@ -348,6 +373,7 @@ def main():
 
     out.update({
         "s_per_iter_fock_mixed": ms_mixed * 1e-3,
+        "s_per_iter_fock_mixed_eager": ms_mixed_eager * 1e-3,
         "mixed_speedup": ms_per_step / ms_mixed,
         "mixed_fp32_quartets": st["n_quartets_fp32"],
         # the tuner keeps the FP32 launch of a class only where it pays
