@@ -14,6 +14,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/dft_hip.h"
@@ -129,6 +130,41 @@ int ncart_h(int l) { return (l + 1) * (l + 2) / 2; }
 
 // workspace regions start on 256-byte boundaries
 long long ws_align(long long n) { return (n + 31) / 32 * 32; }
+
+// Host threads for the plan build's loops over quartets: the hardware's,
+// at most 16 (a GPU's share of a node's cores) and at most OMP_NUM_THREADS
+// where that is set.
+unsigned plan_threads() {
+  unsigned n = std::max(1u, std::thread::hardware_concurrency());
+  n = std::min(n, 16u);
+  if (const char* e = std::getenv("OMP_NUM_THREADS")) {
+    const int v = std::atoi(e);
+    if (v > 0) n = std::min(n, (unsigned)v);
+  }
+  return n;
+}
+
+// fn(chunk, begin, end) over [0, n) in at most plan_threads() contiguous
+// chunks, one thread each (small ranges run on the calling thread).
+template <class F>
+void parallel_chunks(size_t n, size_t min_per_chunk, F&& fn) {
+  const size_t nt = std::min<size_t>(plan_threads(), std::max<size_t>(1, n / std::max<size_t>(1, min_per_chunk)));
+  if (nt <= 1) {
+    fn(0, 0, n);
+    return;
+  }
+  const size_t chunk = (n + nt - 1) / nt;
+  std::vector<std::thread> th;
+  for (size_t t = 0; t < nt; ++t) {
+    const size_t b = t * chunk, e = std::min(n, b + chunk);
+    if (b < e) th.emplace_back([&fn, t, b, e] { fn(t, b, e); });
+  }
+  for (auto& x : th) x.join();
+}
+
+size_t parallel_chunk_count(size_t n, size_t min_per_chunk) {
+  return std::min<size_t>(plan_threads(), std::max<size_t>(1, n / std::max<size_t>(1, min_per_chunk)));
+}
 
 constexpr double PRIM_CUT = 39.1439465808987;  // -ln(1e-17)
 
@@ -910,28 +946,58 @@ int build_plan(dft_plan p, const Input& in) {
       // ket-major: consecutive quartets (the lanes of a wave) share the ket
       // pair, whose primitive records the lane kernel reads in its inner loop,
       // so those loads are wave-uniform; the bra pairs of a wave are
-      // consecutive pairs of their class (coalesced in the interleaved layout)
-      for (int m = 0; m < in.nmol; ++m)
-        for (int ik : class_pairs[m][pck]) {
+      // consecutive pairs of their class (coalesced in the interleaved layout).
+      // The (molecule, ket pair) items are screened in contiguous chunks on
+      // host threads and the chunks' lists joined in item order.
+      std::vector<int2> items;  // (molecule, ket pair)
+      long long work = 0;       // bra-ket checks
+      for (int m = 0; m < in.nmol; ++m) {
+        for (int ik : class_pairs[m][pck]) items.push_back(make_int2(m, ik));
+        work += (long long)class_pairs[m][pck].size() * (long long)class_pairs[m][pcb].size();
+      }
+      // chunks of at least ~2^20 checks (thread start-up costs more below)
+      const size_t min_items =
+          std::max<size_t>(1, (size_t)((double)items.size() * (double)(1 << 20) / (double)std::max(work, 1LL)));
+      struct Chunk {
+        std::vector<int2> q[3];
+        std::vector<int> mol[3];
+        long long nun = 0, prim64 = 0, prim64w = 0, kprod64 = 0, kprod32 = 0;
+      };
+      std::vector<Chunk> chunks(parallel_chunk_count(items.size(), min_items));
+      parallel_chunks(items.size(), min_items, [&](size_t t, size_t i0, size_t i1) {
+        Chunk& ch = chunks[t];
+        for (size_t it = i0; it < i1; ++it) {
+          const int m = items[it].x, ik = items[it].y;
           for (int ib : class_pairs[m][pcb]) {
             if (pcb == pck && ib < ik) continue;
-            ++p->n_unscreened;
+            ++ch.nun;
             const double b = p->Q[ib] * p->Q[ik];
             if (b < p->tau) continue;
             const long long np = (long long)p->pairs[ib].w * p->pairs[ik].w;
             const long long kp = (long long)p->sh[p->pairs[ib].x].nprim * p->sh[p->pairs[ib].y].nprim *
                                  p->sh[p->pairs[ik].x].nprim * p->sh[p->pairs[ik].y].nprim;
-            if (!gen && b < p->tau32) {
-              part[b < FP32_LEVEL2 * p->tau32 ? 2 : 1][m].push_back(make_int2(ib, ik));
-              c.prim64w += np;
-              c.kprod32 += kp;
+            const int k = !gen && b < p->tau32 ? (b < FP32_LEVEL2 * p->tau32 ? 2 : 1) : 0;
+            ch.q[k].push_back(make_int2(ib, ik));
+            ch.mol[k].push_back(m);
+            if (k) {
+              ch.prim64w += np;
+              ch.kprod32 += kp;
             } else {
-              part[0][m].push_back(make_int2(ib, ik));
-              c.prim64 += np;
-              c.kprod64 += kp;
+              ch.prim64 += np;
+              ch.kprod64 += kp;
             }
           }
         }
+      });
+      for (const Chunk& ch : chunks) {
+        p->n_unscreened += ch.nun;
+        c.prim64 += ch.prim64;
+        c.prim64w += ch.prim64w;
+        c.kprod64 += ch.kprod64;
+        c.kprod32 += ch.kprod32;
+        for (int k = 0; k < 3; ++k)
+          for (size_t j = 0; j < ch.q[k].size(); ++j) part[k][ch.mol[k][j]].push_back(ch.q[k][j]);
+      }
       long long nreal[3] = {0, 0, 0};
       for (int k = 0; k < 3; ++k)
         for (const auto& v : part[k]) nreal[k] += (long long)v.size();
@@ -1079,66 +1145,88 @@ int build_plan(dft_plan p, const Input& in) {
   // per wave of the list or table (mask of run starts per block kind, run
   // ends in (kind, lane) order) and shared by every chunk row.
   constexpr size_t NO_TARGET = ~(size_t)0;
-  struct WaveRuns {
-    std::vector<unsigned long long> mask;  // per (wave, kind)
-    std::vector<size_t> first;             // per wave: its first run end
-    std::vector<int> kind, lane;           // per run end
-    std::vector<size_t> target;
-  };
-  auto lane_runs = [&](size_t ci) {
-    const Combo& c = p->combos[ci];
+  // targets of the 64 positions of wave w (list quartets or lane-table
+  // entries; padding and idle positions have none), t[l * nblk + k]
+  auto wave_targets = [&](const Combo& c, size_t w, size_t* t) {
     const bool tab = !c.lanes.empty();
-    const size_t nq = tab ? c.lanes.size() : c.q.size(), nw = nq / WAVE_QUARTETS;
-    WaveRuns r;
-    r.mask.resize(nw * nblk);
-    r.first.resize(nw + 1);
-    std::vector<size_t> t(WAVE_QUARTETS * nblk);
-    for (size_t w = 0; w < nw; ++w) {
-      r.first[w] = r.target.size();
-      const size_t w0 = w * WAVE_QUARTETS;
-      for (int l = 0; l < WAVE_QUARTETS; ++l) {
-        const int4 e = tab ? c.lanes[w0 + l] : make_int4((int)(w0 + l), 0, 0, 0);
-        const int2 qq = e.w ? make_int2(-1, 0) : c.q[e.x];
-        if (qq.x < 0) {
-          for (int k = 0; k < nblk; ++k) t[l * nblk + k] = NO_TARGET;
-        } else {
-          quartet_targets(qq, tq);
-          for (int k = 0; k < nblk; ++k) t[l * nblk + k] = tq[k];
-        }
+    size_t tql[10];
+    const size_t w0 = w * WAVE_QUARTETS;
+    for (int l = 0; l < WAVE_QUARTETS; ++l) {
+      const int4 e = tab ? c.lanes[w0 + l] : make_int4((int)(w0 + l), 0, 0, 0);
+      const int2 qq = e.w ? make_int2(-1, 0) : c.q[e.x];
+      if (qq.x < 0) {
+        for (int k = 0; k < nblk; ++k) t[l * nblk + k] = NO_TARGET;
+      } else {
+        quartet_targets(qq, tql);
+        for (int k = 0; k < nblk; ++k) t[l * nblk + k] = tql[k];
       }
-      for (int k = 0; k < nblk; ++k) {
-        unsigned long long m = 0;
-        for (int l = 0; l < WAVE_QUARTETS; ++l) {
-          const size_t tl = t[l * nblk + k];
-          if (l == 0 || tl == NO_TARGET || tl != t[(l - 1) * nblk + k]) m |= 1ull << l;
-        }
-        r.mask[w * nblk + k] = m;
-        for (int l = 0; l < WAVE_QUARTETS; ++l) {
-          const bool last = l == WAVE_QUARTETS - 1 || ((m >> (l + 1)) & 1);
-          const size_t tl = t[l * nblk + k];
-          if (last && tl != NO_TARGET) {
-            r.kind.push_back(k);
-            r.lane.push_back(l);
-            r.target.push_back(tl);
+    }
+  };
+  // Run-start masks of every wave and block kind (host threads over the
+  // waves), and the run ends' slot rows counted per target; the slot tables
+  // below visit the run ends again in (wave, kind, lane) order. With
+  // DFT_PLAN_TIMING the run ends per quartet of each block kind go to stderr.
+  std::vector<std::vector<unsigned long long>> masks(p->combos.size());
+  for (size_t ci = 0; ci < p->combos.size(); ++ci) {
+    const Combo& c = p->combos[ci];
+    const int ns = slot_rows(ci);
+    if (lane_kernel_combo((int)ci)) {
+      const size_t npos = c.lanes.empty() ? c.q.size() : c.lanes.size(), nw = npos / WAVE_QUARTETS;
+      std::vector<unsigned long long>& mk = masks[ci];
+      mk.assign(nw * nblk, 0);
+      std::vector<long long> ends(nblk, 0);
+      std::mutex mu;
+      const bool one = parallel_chunk_count(nw, 4096) <= 1;  // plain adds on one thread
+      parallel_chunks(nw, 4096, [&](size_t, size_t w0, size_t w1) {
+        std::vector<size_t> t(WAVE_QUARTETS * nblk);
+        std::vector<long long> le(nblk, 0);
+        for (size_t w = w0; w < w1; ++w) {
+          wave_targets(c, w, t.data());
+          for (int k = 0; k < nblk; ++k) {
+            unsigned long long m = 0;
+            for (int l = 0; l < WAVE_QUARTETS; ++l) {
+              const size_t tl = t[l * nblk + k];
+              if (l == 0 || tl == NO_TARGET || tl != t[(l - 1) * nblk + k]) m |= 1ull << l;
+            }
+            mk[w * nblk + k] = m;
+            for (int l = 0; l < WAVE_QUARTETS; ++l) {
+              const bool last = l == WAVE_QUARTETS - 1 || ((m >> (l + 1)) & 1);
+              const size_t tl = t[l * nblk + k];
+              if (last && tl != NO_TARGET) {
+                if (one) tcount[tl] += ns;
+                else __atomic_fetch_add(&tcount[tl], ns, __ATOMIC_RELAXED);
+                ++le[k];
+              }
+            }
           }
         }
+        std::lock_guard<std::mutex> g(mu);
+        for (int k = 0; k < nblk; ++k) ends[k] += le[k];
+      });
+      if (timer.on && !c.q.empty()) {
+        const long long nreal = c.r64 + c.r32a + c.r32b;
+        std::string line = "[dft plan stores/quartet] combo " + std::to_string(ci) + " (" + std::to_string(nreal) +
+                           " quartets, " + std::to_string(ns) + " rows):";
+        for (int k = 0; k < nblk; ++k) {
+          char buf[32];
+          std::snprintf(buf, sizeof buf, " %.3f", nreal ? (double)ends[k] / (double)nreal : 0.0);
+          line += buf;
+        }
+        std::fprintf(stderr, "%s\n", line.c_str());
       }
-    }
-    r.first[nw] = r.target.size();
-    return r;
-  };
-  std::vector<WaveRuns> runs(p->combos.size());
-  for (size_t ci = 0; ci < p->combos.size(); ++ci) {
-    if (lane_kernel_combo((int)ci)) {
-      runs[ci] = lane_runs(ci);
-      const int ns = slot_rows(ci);
-      for (size_t tl : runs[ci].target) tcount[tl] += ns;
       continue;
     }
-    for (const int2& qq : p->combos[ci].q) {
-      quartet_targets(qq, tq);
-      for (int k = 0; k < nblk; ++k) tcount[tq[k]] += slot_rows(ci);
-    }
+    const bool one = parallel_chunk_count(c.q.size(), 1 << 18) <= 1;
+    parallel_chunks(c.q.size(), 1 << 18, [&](size_t, size_t i0, size_t i1) {
+      size_t tql[10];
+      for (size_t i = i0; i < i1; ++i) {
+        quartet_targets(c.q[i], tql);
+        for (int k = 0; k < nblk; ++k) {
+          if (one) tcount[tql[k]] += ns;
+          else __atomic_fetch_add(&tcount[tql[k]], ns, __ATOMIC_RELAXED);
+        }
+      }
+    });
   }
   timer.mark("runs");
   std::vector<long long> tbase(p->ntarget, 0), tnext(p->ntarget, 0);
@@ -1176,19 +1264,60 @@ int build_plan(dft_plan p, const Input& in) {
       // in (kind, lane) order; a run owns ns consecutive blocks of its
       // target, one per primitive chunk (the kernel adds ch times the block
       // size). This fixes every target's summation order.
-      WaveRuns& r = runs[ci];
+      // Large lists in chunks of waves on host threads: each chunk counts its
+      // run ends per target, the counts are turned into each chunk's first
+      // slot per target (chunks in order), then each chunk assigns its slots
+      // in order: the same slots as one pass over the waves.
       const size_t nw = npos / WAVE_QUARTETS;
+      const std::vector<unsigned long long>& mk = masks[ci];
       std::vector<long long> slots(npos * nblk, -1);
-      for (size_t w = 0; w < nw; ++w)
-        for (size_t e = r.first[w]; e < r.first[w + 1]; ++e) {
-          const size_t tl = r.target[e];
-          slots[(size_t)r.kind[e] * npos + w * WAVE_QUARTETS + r.lane[e]] = tnext[tl];
-          tnext[tl] += (long long)ns * tg[tl].nrow * tg[tl].ncol;
+      auto visit = [&](size_t w0, size_t w1, auto&& at_end) {
+        std::vector<size_t> t(WAVE_QUARTETS * nblk);
+        for (size_t w = w0; w < w1; ++w) {
+          wave_targets(c, w, t.data());
+          for (int k = 0; k < nblk; ++k) {
+            const unsigned long long m = mk[w * nblk + k];
+            for (int l = 0; l < WAVE_QUARTETS; ++l) {
+              const bool last = l == WAVE_QUARTETS - 1 || ((m >> (l + 1)) & 1);
+              const size_t tl = t[l * nblk + k];
+              if (last && tl != NO_TARGET) at_end((size_t)k * npos + w * WAVE_QUARTETS + l, tl);
+            }
+          }
         }
+      };
+      const size_t nchunk = parallel_chunk_count(nw, 4096);
+      if (nchunk <= 1) {
+        visit(0, nw, [&](size_t pos, size_t tl) {
+          slots[pos] = tnext[tl];
+          tnext[tl] += (long long)ns * tg[tl].nrow * tg[tl].ncol;
+        });
+      } else {
+        std::vector<std::vector<long long>> cur(nchunk, std::vector<long long>(p->ntarget, 0));
+        parallel_chunks(nw, 4096, [&](size_t ch, size_t w0, size_t w1) {
+          visit(w0, w1, [&](size_t, size_t tl) { ++cur[ch][tl]; });
+        });
+        for (size_t tl = 0; tl < (size_t)p->ntarget; ++tl) {
+          const long long blk = (long long)ns * tg[tl].nrow * tg[tl].ncol;
+          long long off = tnext[tl];
+          for (size_t ch = 0; ch < nchunk; ++ch) {
+            const long long n = cur[ch][tl];
+            cur[ch][tl] = off;
+            off += n * blk;
+          }
+          tnext[tl] = off;
+        }
+        parallel_chunks(nw, 4096, [&](size_t ch, size_t w0, size_t w1) {
+          std::vector<long long>& nx = cur[ch];
+          visit(w0, w1, [&](size_t pos, size_t tl) {
+            slots[pos] = nx[tl];
+            nx[tl] += (long long)ns * tg[tl].nrow * tg[tl].ncol;
+          });
+        });
+      }
       if ((rc = upload(&c.d_slots, slots))) return rc;
-      if ((rc = upload(&c.d_segmask, r.mask))) return rc;
+      if ((rc = upload(&c.d_segmask, mk))) return rc;
       if (!c.lanes.empty() && (rc = upload(&c.d_lanes, c.lanes))) return rc;
-      runs[ci] = WaveRuns();
+      masks[ci] = std::vector<unsigned long long>();
       continue;
     }
     std::vector<long long> slots(nq * ns * nblk);
