@@ -220,12 +220,11 @@ struct Combo {
   long long ibuf_off = 0;
 };
 
-// Device scratch of one Fock build: the half matrices ((1 + nk) x nelem),
-// the J/K staging slots and the heavy classes' integral buffers. A plan owns
+// Device scratch of one Fock build: the J/K staging slots and the heavy
+// classes' integral buffers. A plan owns
 // one (dft_jk); dft_jk_static points the same regions into the library's
 // static scratch array.
 struct Workspace {
-  double* half = nullptr;
   double* stage = nullptr;
   double* ibuf = nullptr;
 };
@@ -286,6 +285,8 @@ struct dft_plan_s {
   Target* d_tg = nullptr;
   long long* d_tbase = nullptr;
   int* d_tcount = nullptr;
+  int2* d_tpairs = nullptr;  // (target XY, target YX) per unordered shell pair and matrix
+  int ntpair = 0;
   long long* d_mol_base = nullptr;
   int* d_mol_nbf = nullptr;
   double* d_atoms = nullptr;     // x, y, z, Z
@@ -336,7 +337,7 @@ void free_plan(dft_plan p) {
     (void)hipFree(c.d_lanes);
   }
   void* ptrs[] = {p->d_sh,    p->d_exps,  p->d_coefs,   p->d_pairs,    p->d_shell_dm,  p->d_shell_ao,
-                  p->d_rec64, p->d_rec32[0], p->d_ws,      p->d_tg,       p->d_tbase,     p->d_tcount,
+                  p->d_rec64, p->d_rec32[0], p->d_ws,      p->d_tg,       p->d_tbase,     p->d_tcount, p->d_tpairs,
                   p->d_atoms, p->d_shpairs, p->d_mol_base, p->d_mol_nbf,   p->d_grec,
                   p->d_gpairs, p->d_g1slots, p->d_gbase, p->d_gcount, p->d_gstage, p->d_recT64, p->d_g1sp, p->d_g1pidx,
                   p->d_gchunks, p->d_gcfirst, p->d_gpart,
@@ -355,16 +356,13 @@ void free_plan(dft_plan p) {
 
 long long pair_key(int i, int j) { return ((long long)i << 32) | (unsigned)j; }
 
-// Doubles of a plan's workspace: half matrices, staging, integral buffers.
-long long workspace_len(dft_plan p) {
-  return ws_align((1 + p->nk) * p->nelem) + ws_align(p->stage_len) + p->ibuf_len;
-}
+// Doubles of a plan's workspace: staging, integral buffers.
+long long workspace_len(dft_plan p) { return ws_align(p->stage_len) + p->ibuf_len; }
 
 // The workspace regions of a plan laid out from base.
 Workspace workspace_at(dft_plan p, double* base) {
   Workspace w;
-  w.half = base;
-  w.stage = base + ws_align((1 + p->nk) * p->nelem);
+  w.stage = base;
   w.ibuf = w.stage + ws_align(p->stage_len);
   return w;
 }
@@ -1244,12 +1242,26 @@ int build_plan(dft_plan p, const Input& in) {
           T.nrow = nr;
           T.ncol = nc;
           T.stride = nbf;
-          T.pad = 0;
+          T.pad = m;  // the matrix: 0 = J, 1 + s = K of density s
           tg[t] = T;
           tbase[t] = tnext[t] = cur;
           cur += (long long)tcount[t] * nr * nc;
         }
   p->stage_len = cur;
+  // unordered shell pairs {X, Y} of each matrix and molecule: the reduction
+  // writes both orientations of a pair's output from its two targets
+  {
+    std::vector<int2> tp;
+    for (int m = 0; m < nmat; ++m)
+      for (int mol = 0; mol < in.nmol; ++mol) {
+        const std::vector<int>& ms = mol_shells[mol];
+        for (size_t i = 0; i < ms.size(); ++i)
+          for (size_t j = i; j < ms.size(); ++j)
+            tp.push_back(make_int2((int)tid(m, ms[i], ms[j]), (int)tid(m, ms[j], ms[i])));
+      }
+    p->ntpair = (int)tp.size();
+    if ((rc = upload(&p->d_tpairs, tp))) return rc;
+  }
   timer.mark("targets");
   // slot tables: block-major for the lane kernel (a wave's loads of one
   // block's slots are contiguous), quartet-major for the heavy-class digest
@@ -1547,15 +1559,13 @@ int jk_enqueue(dft_plan p, const Workspace& ws, const double* P, const double* P
       HIP_TRY(hipStreamWaitEvent(s, p->ev_join[k - 1], 0));
     }
   mark(nslot - 2, 0, s);
-  hipError_t e = launch_reduce(p->ntarget, p->d_tg, p->d_tbase, p->d_tcount, ws.stage, p->stage_len, ws.half, s);
+  // the reduction writes J = Jh + Jh^T and K likewise directly (the last
+  // profile slot, once the separate symmetrisation, stays empty)
+  hipError_t e = launch_reduce_sym(p->ntpair, p->d_tpairs, p->d_tg, p->d_tbase, p->d_tcount, ws.stage, p->stage_len,
+                                   p->nelem, J, Ka, Kb, s);
   if (e != hipSuccess) return fail(DFT_EHIP, std::string("reduce: ") + hipGetErrorString(e));
   mark(nslot - 2, 1, s);
   mark(nslot - 1, 0, s);
-  double* outs[3] = {J, Ka, Kb};
-  for (int m = 0; m < 1 + p->nk; ++m) {
-    e = launch_symmetrize(p->nmol, p->d_mol_base, p->d_mol_nbf, p->max_nbf, ws.half + m * p->nelem, outs[m], s);
-    if (e != hipSuccess) return fail(DFT_EHIP, std::string("symmetrize: ") + hipGetErrorString(e));
-  }
   mark(nslot - 1, 1, s);
   return DFT_OK;
 }

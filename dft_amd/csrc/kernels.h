@@ -45,9 +45,10 @@ struct ShellDev {      // one contracted (segmented) shell, internal order
   long long mbase;           // packed-matrix offset of the molecule
 };
 
-struct Target {        // one ordered shell pair (X, Y) of a half matrix
-  long long base;      // element offset of (X row 0, Y col 0) in the half array
-  int nrow, ncol, stride, pad;
+struct Target {        // one ordered shell pair (X, Y) of a half matrix H (J = H + H^T)
+  long long base;      // pad * nelem + offset of element (X row 0, Y col 0) in its matrix
+  int nrow, ncol, stride;
+  int pad;             // the matrix: 0 = J, 1 + s = K of density s
 };
 
 // pair pre-pass: one lane per primitive pair; writes FP64 and / or FP32
@@ -153,18 +154,20 @@ hipError_t launch_jk_generic(int la, int lb, int lc, int ld, const int2* quartet
 hipError_t launch_eri_raw(int combo, const int2* quartets, long long nq, const int4* pairs,
                           const double* rec, double* out, const double* boys_tab, hipStream_t s);
 
-// deterministic target reduction: one wave per target; the blocks of target
-// t are contiguous: stage[tgt_base[t] + k * nrow * ncol + e], k < tgt_count[t]
-hipError_t launch_reduce(int ntarget, const Target* targets, const long long* tgt_base, const int* tgt_count,
-                         const double* stage, long long stage_len /* doubles */, double* half, hipStream_t s);
+// Deterministic reduction and symmetrisation: one wave per unordered
+// shell pair tpairs[i] = (target XY, target YX) of one matrix (YX = XY on
+// the diagonal); the blocks of target t are contiguous:
+// stage[tgt_base[t] + k * nrow * ncol + e], k < tgt_count[t]. Writes both
+// orientations of out = H + H^T of the target's matrix (Target.pad: 0 = J
+// to out0, 1 = K_a to out1, 2 = K_b to out2; Target.base - pad * nelem is
+// the element offset in that matrix).
+hipError_t launch_reduce_sym(int npair, const int2* tpairs, const Target* targets, const long long* tgt_base,
+                             const int* tgt_count, const double* stage, long long stage_len /* doubles */,
+                             long long nelem, double* out0, double* out1, double* out2, hipStream_t s);
 
 // AO values on a grid: out[ao * ld + g], g < npts (grid: npts x 3 FP32)
 hipError_t launch_ao_grid(int nsh, const ShellDev* shells, const double* exps, const double* coefs,
                           const float* grid, long long npts, double* out, long long ld, hipStream_t s);
-
-// M = H + H^T per molecule block (packed layout), for one half matrix
-hipError_t launch_symmetrize(int nmol, const long long* mol_base, const int* mol_nbf, int max_nbf,
-                             const double* half, double* out, hipStream_t s);
 
 // S, T, V of the shell pairs sp[0..npairs) of one class (la >= lb; the
 // first shell of each pair has momentum la), one 64-lane workgroup per

@@ -111,41 +111,31 @@ hipError_t launch_interleave_records(const int4* map, int nrec, const double* r6
 }
 
 // ------------------------------------------------- fixed-order reduction
-// One 64-lane wave per target block (an ordered shell pair of one half
-// matrix, ne = nrow * ncol <= 36 elements). The target's cnt contribution
-// blocks are contiguous in the staging buffer, so the wave streams them as
-// one contiguous array: with g = 64 / ne groups, lane l < g * ne takes
-// element l % ne of blocks l / ne, l / ne + g, ... (consecutive lanes read
-// consecutive addresses; RED_ACC partial sums per lane). The groups are then
-// combined in a fixed order
-// (a xor butterfly when ne = 1, else group 0, 1, ... by lane shuffles), so
-// J and K are bitwise reproducible run to run (the reference's add.at
-// scatter is not: shelled.py:857-859).
-constexpr int RED_ACC = 16;
-
+// The half matrices are never formed: one 64-lane wave per unordered shell
+// pair {X, Y} of one matrix (J or a K) reduces the contribution blocks of
+// its two targets, (X, Y) and (Y, X) (one target when X = Y), and writes
+// both orientations of J = Jh + Jh^T (likewise K): element (x, y) of the
+// output is S_XY[x][y] + S_YX[y][x], the sum the separate symmetrisation
+// pass formed from the half matrix (the same additions, so the same bits).
 //
-// Two register layouts of the same sums. UNIFORM makes the wave index
-// explicitly wave-uniform, so the target's base pointer sits in scalar
-// registers (64 VGPRs, 8 waves per SIMD instead of 122 VGPRs and 4): faster
-// when the staging buffer streams from HBM (64 uneven (H2O)_n clusters,
-// 7.2 GB: batch build 15.71 -> 15.48 ms), slower when it fits in the
-// 256 MB Infinity Cache ((H2O)16, 168 MB: 38 -> 49 us), where the extra
-// waves in flight cost more than they hide.
-template <bool UNIFORM>
-__global__ void reduce_kernel(int ntarget, const Target* __restrict__ tg, const long long* __restrict__ tbase,
-                              const int* __restrict__ tcount, const double* __restrict__ stage,
-                              double* __restrict__ half) {
-  const int w0 = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int wave = UNIFORM ? __builtin_amdgcn_readfirstlane(w0) : w0;
-  const int lane = threadIdx.x & 63;
-  if (wave >= ntarget) return;
-  const Target t = tg[wave];
+// A target's cnt contribution blocks (ne = nrow * ncol elements each) are
+// contiguous in the staging buffer, so the wave streams them as one
+// contiguous array: with g = 64 / ne groups, lane l < g * ne takes element
+// l % ne of blocks l / ne, l / ne + g, ... (consecutive lanes read
+// consecutive addresses; RED_ACC partial sums per lane). The groups are then
+// combined in a fixed order (a xor butterfly when ne = 1, else group 0, 1,
+// ... by lane shuffles), so J and K are bitwise reproducible run to run (the
+// reference's add.at scatter is not: shelled.py:857-859). Blocks wider than
+// a wave (f x f) sum element by element. The two targets' sums go through
+// LDS to meet their transposes.
+constexpr int RED_ACC = 16;
+constexpr int RED_NE_MAX = ncart(LMAX) * ncart(LMAX);
+
+// Sum of target t's blocks into dst[e], e < nrow * ncol (one wave).
+__device__ __forceinline__ void reduce_target(const Target& t, const double* __restrict__ src, int cnt, int lane,
+                                              double* __restrict__ dst) {
   const int ne = t.nrow * t.ncol;
-  const double* src = stage + tbase[wave];
-  const int cnt = tcount[wave];
   if (ne > 64) {
-    // blocks wider than a wave (f x f and above): lane e sums element e,
-    // e + 64, ... over the slots in slot order
     for (int e = lane; e < ne; e += 64) {
       double acc[RED_ACC];
 #pragma unroll
@@ -160,7 +150,7 @@ __global__ void reduce_kernel(int ntarget, const Target* __restrict__ tg, const 
       for (int w = 1; w < RED_ACC; w *= 2)
 #pragma unroll
         for (int j = 0; j + w < RED_ACC; j += 2 * w) acc[j] += acc[j + w];
-      half[t.base + (long long)(e / t.ncol) * t.stride + e % t.ncol] = acc[0];
+      dst[e] = acc[0];
     }
     return;
   }
@@ -190,48 +180,72 @@ __global__ void reduce_kernel(int ntarget, const Target* __restrict__ tg, const 
   if (ne == 1) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    if (lane == 0) half[t.base] = s;
+    if (lane == 0) dst[0] = s;
   } else {
     const int el = lane < ne ? lane : 0;
     double r = 0.0;
     for (int k = 0; k < g; ++k) r += __shfl(s, k * ne + el, 64);
-    if (lane < ne) half[t.base + (long long)(lane / t.ncol) * t.stride + lane % t.ncol] = r;
+    if (lane < ne) dst[lane] = r;
   }
 }
 
-hipError_t launch_reduce(int ntarget, const Target* tg, const long long* tbase, const int* tcount,
-                         const double* stage, long long stage_len, double* half, hipStream_t s) {
-  if (ntarget <= 0) return hipSuccess;
+//
+// Two register layouts of the same sums. UNIFORM makes the wave index
+// explicitly wave-uniform, so the targets' base pointers sit in scalar
+// registers (fewer VGPRs, more waves per SIMD): faster when the staging
+// buffer streams from HBM (64 uneven (H2O)_n clusters, 7.2 GB: batch build
+// 15.71 -> 15.48 ms), slower when it fits in the 256 MB Infinity Cache
+// ((H2O)16, 168 MB: 38 -> 49 us), where the extra waves in flight cost more
+// than they hide.
+template <bool UNIFORM>
+__global__ __launch_bounds__(64) void reduce_sym_kernel(int npair, const int2* __restrict__ tpairs,
+                                                        const Target* __restrict__ tg,
+                                                        const long long* __restrict__ tbase,
+                                                        const int* __restrict__ tcount,
+                                                        const double* __restrict__ stage, long long nelem,
+                                                        double* __restrict__ out0, double* __restrict__ out1,
+                                                        double* __restrict__ out2) {
+  __shared__ double s1[RED_NE_MAX], s2[RED_NE_MAX];
+  const int w0 = blockIdx.x;
+  const int wave = UNIFORM ? __builtin_amdgcn_readfirstlane(w0) : w0;
+  const int lane = threadIdx.x & 63;
+  if (wave >= npair) return;
+  const int2 pr = tpairs[wave];
+  const bool diag = pr.x == pr.y;
+  const Target t1 = tg[pr.x];
+  reduce_target(t1, stage + tbase[pr.x], tcount[pr.x], lane, s1);
+  Target t2 = t1;
+  if (!diag) {
+    t2 = tg[pr.y];
+    reduce_target(t2, stage + tbase[pr.y], tcount[pr.y], lane, s2);
+  }
+  __syncthreads();
+  // matrix index in pad; element offsets within that matrix
+  double* __restrict__ out = t1.pad == 0 ? out0 : (t1.pad == 1 ? out1 : out2);
+  const long long o1 = t1.base - (long long)t1.pad * nelem, o2 = t2.base - (long long)t2.pad * nelem;
+  const int ne = t1.nrow * t1.ncol;
+  for (int e = lane; e < ne; e += 64) {
+    const int r = e / t1.ncol, c = e - (e / t1.ncol) * t1.ncol;
+    const double v = s1[e] + (diag ? s1[c * t1.ncol + r] : s2[c * t1.nrow + r]);
+    out[o1 + (long long)r * t1.stride + c] = v;
+    if (!diag) out[o2 + (long long)c * t2.stride + r] = v;
+  }
+}
+
+hipError_t launch_reduce_sym(int npair, const int2* tpairs, const Target* tg, const long long* tbase,
+                             const int* tcount, const double* stage, long long stage_len, long long nelem,
+                             double* out0, double* out1, double* out2, hipStream_t s) {
+  if (npair <= 0) return hipSuccess;
   // one-wave workgroups: 38 vs 43 us (four waves per group) for the 168 MB
   // staging of a (H2O)16 STO-3G build
-  const int waves_per_block = 1;
-  const dim3 grid((ntarget + waves_per_block - 1) / waves_per_block), block(64 * waves_per_block);
+  const dim3 grid(npair), block(64);
   // staging larger than the 256 MB Infinity Cache streams from HBM
   if (stage_len * (long long)sizeof(double) > (256LL << 20))
-    hipLaunchKernelGGL(reduce_kernel<true>, grid, block, 0, s, ntarget, tg, tbase, tcount, stage, half);
+    hipLaunchKernelGGL(reduce_sym_kernel<true>, grid, block, 0, s, npair, tpairs, tg, tbase, tcount, stage, nelem,
+                       out0, out1, out2);
   else
-    hipLaunchKernelGGL(reduce_kernel<false>, grid, block, 0, s, ntarget, tg, tbase, tcount, stage, half);
-  return hipGetLastError();
-}
-
-// out = H + H^T per molecule block; blockIdx.y = molecule
-__global__ void symmetrize_kernel(const long long* __restrict__ mbase, const int* __restrict__ mnbf,
-                                  const double* __restrict__ h, double* __restrict__ o) {
-  const int m = blockIdx.y;
-  const int n = mnbf[m];
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (long long)n * n) return;
-  const int r = (int)(i / n), c = (int)(i % n);
-  const long long b = mbase[m];
-  o[b + i] = h[b + i] + h[b + (long long)c * n + r];
-}
-
-hipError_t launch_symmetrize(int nmol, const long long* mbase, const int* mnbf, int max_nbf, const double* half,
-                             double* out, hipStream_t s) {
-  if (nmol <= 0) return hipSuccess;
-  const long long nn = (long long)max_nbf * max_nbf;
-  hipLaunchKernelGGL(symmetrize_kernel, dim3((unsigned)((nn + 255) / 256), (unsigned)nmol), dim3(256), 0, s, mbase,
-                     mnbf, half, out);
+    hipLaunchKernelGGL(reduce_sym_kernel<false>, grid, block, 0, s, npair, tpairs, tg, tbase, tcount, stage, nelem,
+                       out0, out1, out2);
   return hipGetLastError();
 }
 

@@ -175,7 +175,7 @@ int dft_plan_set_shapes(dft_plan plan, const int* shapes, int n);
  * ncombo is the capacity of counts in combinations; the number of
  * combinations is returned in *ncombo_out. Also reports the quartets before
  * screening and, in *staging_bytes, the bytes of device workspace one Fock
- * build uses (half matrices, J/K staging slots, heavy-class integral
+ * build uses (J/K staging slots, heavy-class integral
  * buffers): what dft_jk_static needs to fit in DFT_STATIC_SCRATCH_BYTES.
  */
 #define DFT_STATS_COLS 8
@@ -214,7 +214,7 @@ int dft_jk(dft_plan plan, const double* P, const double* Pa, const double* Pb, d
            double* Kb, int mode, void* stream);
 
 /*
- * dft_jk with the Fock-build workspace (half matrices, J/K staging slots,
+ * dft_jk with the Fock-build workspace (J/K staging slots,
  * heavy-class integral buffers) in the library's static device scratch
  * instead of the plan's: allocates nothing, for callers that cannot call
  * hipMalloc (code under hipGraph capture, tight serving loops, harnesses
