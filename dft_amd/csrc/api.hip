@@ -2412,15 +2412,21 @@ int dft_eri_prim(int64_t n_ab, const double* p_ab, const double* P_ab, const dou
   return DFT_OK;
 }
 
-int dft_syevj(int n, int batch, double* A, double* w, double* V, int init_v, int32_t* sweeps, void* stream) {
+int dft_syevj_tol(int n, int batch, double* A, double* w, double* V, int init_v, int32_t* sweeps, double tol,
+                  void* stream) {
   if (n <= 0 || batch < 0) return fail(DFT_EINVAL, "n must be positive and batch >= 0");
   if (n > JAC_NMAX) return fail(DFT_EUNSUPPORTED, "dft_syevj: order " + std::to_string(n) + " above " +
                                                       std::to_string(JAC_NMAX));
+  if (!(tol >= 0.0)) return fail(DFT_EINVAL, "dft_syevj_tol: tol must be >= 0");
   if (batch > 0 && (!A || !w || !V)) return fail(DFT_EINVAL, "NULL argument");
   const long long nn = (long long)n * n;
-  hipError_t e = launch_syevj(n, batch, A, nn, w, n, V, nn, init_v ? 1 : 0, sweeps, (hipStream_t)stream);
+  hipError_t e = launch_syevj(n, batch, A, nn, w, n, V, nn, init_v ? 1 : 0, sweeps, tol, (hipStream_t)stream);
   if (e != hipSuccess) return fail(DFT_EHIP, std::string("syevj: ") + hipGetErrorString(e));
   return DFT_OK;
+}
+
+int dft_syevj(int n, int batch, double* A, double* w, double* V, int init_v, int32_t* sweeps, void* stream) {
+  return dft_syevj_tol(n, batch, A, w, V, init_v, sweeps, 0.0, stream);
 }
 
 int dft_scf_diis_stage(int n, int history, const double* F, const double* R, double* hist_F, double* hist_R,

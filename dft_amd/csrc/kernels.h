@@ -219,7 +219,7 @@ hipError_t launch_eri_prim(long long nab, const double* pab, const double* Pab, 
 // is that basis times the rotations)
 constexpr int JAC_NMAX = 128;
 hipError_t launch_syevj(int n, int batch, double* A, long long sa, double* w, long long sw, double* V,
-                        long long sv, int init_v, int* sweeps, hipStream_t s);
+                        long long sv, int init_v, int* sweeps, double tol, hipStream_t s);
 
 // device SCF iteration (state layout: the DFT_SCF_ST_* indices of dft_hip.h)
 constexpr int SCF_HMAX = 16;

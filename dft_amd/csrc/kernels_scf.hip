@@ -64,7 +64,7 @@ template <bool VL>
 __global__ __launch_bounds__(JAC_THREADS) void jacobi_eigh_kernel(int n, const double* __restrict__ A, long long sa,
                                                                   double* __restrict__ w, long long sw,
                                                                   double* __restrict__ V, long long sv,
-                                                                  int init_v, int* __restrict__ sweeps) {
+                                                                  int init_v, int* __restrict__ sweeps, double tol) {
   extern __shared__ double lds[];
   __shared__ double cs_c[2][JAC_NMAX / 2], cs_s[2][JAC_NMAX / 2], cs_t[2][JAC_NMAX / 2];
   __shared__ int cs_p[2][JAC_NMAX / 2], cs_q[2][JAC_NMAX / 2];
@@ -104,7 +104,9 @@ __global__ __launch_bounds__(JAC_THREADS) void jacobi_eigh_kernel(int n, const d
   __syncthreads();
   double fro = 0.0;
   for (int k = 0; k < (nt >> 6); ++k) fro += red[k];
-  const double thresh = m * 2.220446049250313e-16 * sqrt(fro);
+  // the rotation / convergence bound: m 2^-52 ||A||_F, or the caller's
+  // absolute tolerance on the off-diagonal entries where that is larger
+  const double thresh = fmax(m * 2.220446049250313e-16 * sqrt(fro), tol);
   // this thread's fixed share of the round's work: blocks (k, l), k <= l, of
   // the h x h block grid, and (row, pair) updates of the eigenvectors
   const int nblk = h * (h + 1) / 2;
@@ -297,7 +299,7 @@ __global__ __launch_bounds__(JAC_THREADS) void jacobi_eigh_kernel(int n, const d
 }
 
 hipError_t launch_syevj(int n, int batch, double* A, long long sa, double* w, long long sw, double* V,
-                        long long sv, int init_v, int* sweeps, hipStream_t s) {
+                        long long sv, int init_v, int* sweeps, double tol, hipStream_t s) {
   if (batch <= 0 || n <= 0) return hipSuccess;
   if (n > JAC_NMAX) return hipErrorInvalidValue;
   const int m = n + (n & 1);
@@ -315,10 +317,10 @@ hipError_t launch_syevj(int n, int batch, double* A, long long sa, double* w, lo
   }
   if (vl)
     hipLaunchKernelGGL(jacobi_eigh_kernel<true>, dim3(batch), dim3(JAC_THREADS), dyn, s, n, A, sa, w, sw, V, sv,
-                       init_v, sweeps);
+                       init_v, sweeps, tol);
   else
     hipLaunchKernelGGL(jacobi_eigh_kernel<false>, dim3(batch), dim3(JAC_THREADS), dyn, s, n, A, sa, w, sw, V, sv,
-                       init_v, sweeps);
+                       init_v, sweeps, tol);
   return hipGetLastError();
 }
 

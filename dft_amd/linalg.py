@@ -23,11 +23,13 @@ from . import _lib
 JACOBI_NMAX = 128
 
 
-def syevj_(A: torch.Tensor, w: torch.Tensor, V: torch.Tensor, init_v: bool, stream=None, sweeps=None) -> None:
+def syevj_(A: torch.Tensor, w: torch.Tensor, V: torch.Tensor, init_v: bool, stream=None, sweeps=None,
+           tol: float = 0.0) -> None:
     """In place: A (batch, n, n) contiguous float64 is destroyed, w (batch, n)
     and V (batch, n, n) receive the eigenpairs; with init_v False, V holds
     the starting orthonormal basis Q and A must be Q^T F Q. ``sweeps`` (an
-    int32 device tensor of batch entries) receives the Jacobi sweeps."""
+    int32 device tensor of batch entries) receives the Jacobi sweeps.
+    ``tol`` > 0: off-diagonal entries up to tol are left (dft_syevj_tol)."""
     lib = _lib.load()
     n = A.shape[-1]
     batch = A.numel() // (n * n) if n else 0
@@ -40,12 +42,14 @@ def syevj_(A: torch.Tensor, w: torch.Tensor, V: torch.Tensor, init_v: bool, stre
         if not (sweeps.is_cuda and sweeps.dtype == torch.int32 and sweeps.numel() >= batch):
             raise ValueError("sweeps must be an int32 device tensor with batch entries")
         sw = sweeps.data_ptr()
-    _lib.check(lib.dft_syevj(n, batch, A.data_ptr(), w.data_ptr(), V.data_ptr(), int(bool(init_v)), sw,
-                             C.c_void_p(s.cuda_stream)))
+    _lib.check(lib.dft_syevj_tol(n, batch, A.data_ptr(), w.data_ptr(), V.data_ptr(), int(bool(init_v)), sw,
+                                 float(tol), C.c_void_p(s.cuda_stream)))
 
 
-def eigh(F: torch.Tensor, V0: torch.Tensor | None = None):
-    """(w, V) with F = V diag(w) V^T for one symmetric matrix or a batch."""
+def eigh(F: torch.Tensor, V0: torch.Tensor | None = None, tol: float = 0.0):
+    """(w, V) with F = V diag(w) V^T for one symmetric matrix or a batch;
+    ``tol`` > 0 leaves off-diagonal entries up to tol (orbitals to about
+    tol / gap, for SCF loops with a coarser convergence test)."""
     n = F.shape[-1]
     if n > JACOBI_NMAX or not F.is_cuda:
         return torch.linalg.eigh(F)
@@ -63,5 +67,5 @@ def eigh(F: torch.Tensor, V0: torch.Tensor | None = None):
         V = Q.contiguous().clone()
         init = False
     w = torch.empty(A.shape[:2], dtype=A.dtype, device=A.device)
-    syevj_(A, w, V, init)
+    syevj_(A, w, V, init, tol=tol)
     return (w, V) if batched else (w[0], V[0])

@@ -195,7 +195,8 @@ def _history_array(items, history, n):
 
 
 def scf_diis(plan: Plan, Z, centers, N_a, N_b, mode="fp64", rate=0.4, diis_threshold=1e-3, history=10,
-             e_tol=1e-8, p_tol=1e-6, max_count=500, A=None, guess=None, checkpoint=None, resume=None):
+             e_tol=1e-8, p_tol=1e-6, max_count=500, A=None, guess=None, checkpoint=None, resume=None,
+             eig_tol=0.0):
     """Damped + DIIS UHF loop of shelled.py:1103-1292.
 
     Damping rate 0.4, divided by 1.1 every fourth strike (an iteration whose
@@ -204,7 +205,9 @@ def scf_diis(plan: Plan, Z, centers, N_a, N_b, mode="fp64", rate=0.4, diis_thres
     coefficients normalised to sum 1 (shelled.py:1196-1201). Unlike the
     reference it stops after ``max_count`` iterations with converged False.
     ``guess`` = (P_a, P_b) starts from those densities instead of the core
-    Hamiltonian (e.g. the converged densities of a nearby geometry). For RHF
+    Hamiltonian (e.g. the converged densities of a nearby geometry).
+    ``eig_tol`` > 0: per-iteration eigensolves to that off-diagonal size, as
+    in scf_diis_graph (0, the default, is exact). For RHF
     (N_a == N_b, P_a == P_b throughout) one diagonalisation serves both spins.
     ``checkpoint``: path written every 10th iteration in the reference's
     ``scf_checkpoint.npz`` layout (shelled.py:1219-1237); ``resume``: such a
@@ -301,8 +304,8 @@ def scf_diis(plan: Plan, Z, centers, N_a, N_b, mode="fp64", rate=0.4, diis_thres
                 Fb = torch.einsum("i,iab->ab", c, torch.stack(hist["Fb"]))
         E_new = float(0.5 * torch.sum(P * H + P_a * Fa + P_b * Fb)) + E_NN
         dE = float("inf") if E_total is None else abs(E_new - E_total)
-        _, Qa = eigh(X.T @ Fa @ X, Qa)
-        Qb = Qa if rhf else eigh(X.T @ Fb @ X, Qb)[1]
+        _, Qa = eigh(X.T @ Fa @ X, Qa, tol=eig_tol)
+        Qb = Qa if rhf else eigh(X.T @ Fb @ X, Qb, tol=eig_tol)[1]
         Ca, Cb = X @ Qa, X @ Qb
         Pa_new, Pb_new = UHF_density(Ca, Cb, N_a, N_b, P_a, P_b, rate)
         dP = float(torch.maximum((Pa_new - P_a).abs().max(), (Pb_new - P_b).abs().max()))
@@ -353,7 +356,7 @@ SCF_STATE_LEN = 1152
 
 
 def scf_diis_graph(plan: Plan, Z, centers, N_occ, mode="fp64", rate=0.4, diis_threshold=1e-3, history=10,
-                   e_tol=1e-8, p_tol=1e-6, max_count=500, guess=None):
+                   e_tol=1e-8, p_tol=1e-6, max_count=500, guess=None, eig_tol=0.0):
     """The damped + DIIS loop of shelled.py:1103-1292 for a closed shell
     (N_alpha = N_beta = N_occ), one iteration = one replay of a captured
     hipGraph: the direct J/K build (dft_jk, its class launches on four
@@ -368,7 +371,12 @@ def scf_diis_graph(plan: Plan, Z, centers, N_occ, mode="fp64", rate=0.4, diis_th
     (the eigensolver's limit); no checkpoint files (scf_diis writes them).
     ``guess`` = P_a (or (P_a, P_b) with P_a == P_b) starts from that density
     instead of the core Hamiltonian's; the first diagonalisation then starts
-    from the core Hamiltonian's orbitals."""
+    from the core Hamiltonian's orbitals. ``eig_tol`` > 0 lets the
+    per-iteration eigensolves stop at off-diagonal entries of that size
+    (dft_syevj_tol; fewer Jacobi sweeps, orbitals to about eig_tol / gap):
+    the converged state is the same, the intermediate energies no longer
+    follow the reference loop's exact eigh to rounding (0, the default,
+    keeps them exact); the exit rebuild's eigensolve is always exact."""
     import ctypes as C
     from . import _lib
     from .linalg import syevj_
@@ -420,7 +428,7 @@ def scf_diis_graph(plan: Plan, Z, centers, N_occ, mode="fp64", rate=0.4, diis_th
         At = XQ.T @ Fu @ XQ
         torch.add(At, At.T, out=A[0]).mul_(0.5)
         V[0].copy_(Q)
-        syevj_(A, w, V, False)
+        syevj_(A, w, V, False, tol=eig_tol)
         Q.copy_(V[0])
         Cm = X @ Q
         Pauf = Cm[:, :N_occ] @ Cm[:, :N_occ].T

@@ -348,6 +348,20 @@ int dft_eri_prim(int64_t n_ab, const double* p_ab, const double* P_ab, const dou
 int dft_syevj(int n, int batch, double* A, double* w, double* V, int init_v, int32_t* sweeps, void* stream);
 
 /*
+ * dft_syevj with an absolute tolerance on the off-diagonal entries: pairs
+ * with |a_pq| <= max(tol, n 2^-52 ||A||_F) are not rotated and the sweeps
+ * stop once no entry is above that bound (tol = 0: dft_syevj). For an SCF
+ * loop whose convergence test is far coarser than machine precision (its
+ * orbitals then carry errors of about tol / gap, its energies of about
+ * (tol / gap)^2 at convergence); DFT_EINVAL for tol < 0 or NaN.
+ *
+ * Replaces the same eigendecompositions as dft_syevj (shelled.py:1210-1213,
+ * main.py:796-799).
+ */
+int dft_syevj_tol(int n, int batch, double* A, double* w, double* V, int init_v, int32_t* sweeps, double tol,
+                  void* stream);
+
+/*
  * One damped + DIIS SCF iteration on the device, closed shell (the loop of
  * shelled.py:1164-1260 with P_a = P_b), split in two calls around the
  * orbital step so that a whole iteration (dft_jk, these two, dft_syevj and

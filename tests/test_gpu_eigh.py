@@ -125,3 +125,36 @@ def test_syevj_time_per_call_is_reported(torch_cuda):
         torch.cuda.synchronize()
         print(f"syevj n = {n}, {'identity' if init else 'warm'} start: {e0.elapsed_time(e1) / 5:.3f} ms per call, "
               f"{int(sw.cpu()[0])} sweeps", flush=True)
+
+
+def test_syevj_tolerance_leaves_only_small_entries(torch_cuda):
+    """dft_syevj_tol: from the eigenvectors of a nearby matrix (the SCF warm
+    start), a tolerance of 1e-9 on the off-diagonal entries takes fewer
+    sweeps than machine precision, the eigenvectors stay orthonormal to
+    rounding, and V^T A V differs from diag(w) by no more than the
+    tolerance."""
+    from dft_amd.linalg import eigh, syevj_
+    torch = torch_cuda
+    n = 112
+    rng = np.random.default_rng(5)
+    M = rng.standard_normal((n, n))
+    A = M + M.T
+    E = rng.standard_normal((n, n)) * 1e-6
+    A2 = A + E + E.T
+    _, V0 = eigh(torch.as_tensor(A, device="cuda"))
+    out = {}
+    for tol in (0.0, 1e-9):
+        Q = V0.reshape(1, n, n).contiguous().clone()
+        Ad = (Q[0].T @ torch.as_tensor(A2, device="cuda") @ Q[0])
+        Ad = (0.5 * (Ad + Ad.T)).reshape(1, n, n).contiguous()
+        w = torch.empty((1, n), dtype=torch.float64, device="cuda")
+        sw = torch.zeros(1, dtype=torch.int32, device="cuda")
+        syevj_(Ad, w, Q, False, sweeps=sw, tol=tol)
+        out[tol] = (w[0].cpu().numpy(), Q[0].cpu().numpy(), int(sw.cpu()[0]))
+    w, V, s1 = out[1e-9]
+    s0 = out[0.0][2]
+    assert s1 < s0, (s1, s0)
+    assert np.abs(V.T @ V - np.eye(n)).max() < 1e-12
+    D = V.T @ A2 @ V
+    assert np.abs(D - np.diag(w)).max() < 1e-9 + 1e-12 * np.abs(A2).max() * n
+    assert np.abs(w - out[0.0][0]).max() < 1e-12 * np.abs(A2).max() * n

@@ -60,6 +60,11 @@ def test_eigensolver_rejects_orders_above_its_limit():
     rc = lib.dft_syevj(129, 1, None, None, None, 1, None, None)
     assert rc == _lib.DFT_EUNSUPPORTED
     assert b"above 128" in lib.dft_last_error()
+    # a negative or NaN off-diagonal tolerance is an argument error
+    for tol in (-1e-9, float("nan")):
+        rc = lib.dft_syevj_tol(16, 1, None, None, None, 1, None, tol, None)
+        assert rc == _lib.DFT_EINVAL
+        assert b"tol" in lib.dft_last_error()
 
 
 def test_python_constants_match_the_header():
