@@ -25,19 +25,21 @@ using namespace dft;
 
 namespace {
 
-// Primitive split of the lane-kernel combinations. Every lane of a split
-// quartet repeats the J/K digest and the run sums of its blocks, which for
-// the p classes costs about as much as a few primitive quartets, so a split
-// pays only while the class leaves the chip short of waves: tiny classes
-// (below SPLIT_LANES / 8 quartets) are split up to about SPLIT_LANES lanes,
-// classes below SPLIT_PAIR_BELOW quartets into two lanes per quartet.
-// (H2O)16 STO-3G: (pp|pp) takes 42 us with 27 lanes per quartet and 74 us
-// with 45; a pair split of (ps|ps) and (pp|ss) (126 k and 79 k quartets)
-// shortens their serial launches (63.7 -> 55.4 us, 59.1 -> 53.9 us) but
-// lengthens the concurrent build (0.321 -> 0.330 ms), where the large
-// classes already fill the chip and only the added digest work remains.
+// Primitive split of the lane-kernel combinations: a class with fewer than
+// SPLIT_LANES quartets gets about SPLIT_LANES lanes, but every lane of a
+// split quartet repeats the J/K digest and the run sums of its blocks, which
+// for a (pp|pp) quartet cost about as much as three primitive quartets, so
+// no split leaves a lane fewer than SPLIT_MIN_PRIM primitive quartets
+// (K_a K_b K_c K_d, class average). (H2O)16 STO-3G (pp|pp), 81 primitive
+// quartets per quartet: 27 lanes per quartet (3 each) 42 us, 45 lanes 74 us;
+// C12H26 s/p/d (pp|pp), 12.8 k quartets of up to 1296: two lanes per quartet
+// 1.26 ms. A pair split of (ps|ps) and (pp|ss) on (H2O)16 (126 k and 79 k
+// quartets) shortens their serial launches (63.7 -> 55.4 us, 59.1 -> 53.9
+// us) but lengthens the concurrent build (0.321 -> 0.330 ms), where the
+// large classes already fill the chip and only the added digest work
+// remains, so classes at or above SPLIT_LANES quartets stay unsplit.
 constexpr long long SPLIT_LANES = 65536;
-constexpr long long SPLIT_PAIR_BELOW = 65536;
+constexpr long long SPLIT_MIN_PRIM = 3;
 // the same for the column kernel of the heavy classes (longer lanes)
 constexpr long long SPLIT_LANES_HEAVY = 262144;
 // FP32 records drop primitive pairs whose summed Schwarz bounds stay within
@@ -871,17 +873,12 @@ int build_plan(dft_plan p, const Input& in) {
   // molecule-major within each class combination; bra and ket always come
   // from the same molecule
   p->combos.assign(NCOMBO, Combo());
-  // lanes of the full split of a tiny lane-kernel combination (SPLIT_LANES;
-  // DFT_SPLIT_LANES overrides it, for launch-shape studies) and the quartet
-  // count below which a combination gets two lanes per quartet
-  long long split_lanes = SPLIT_LANES, split_pair = SPLIT_PAIR_BELOW;
+  // lanes a split lane-kernel combination aims at (SPLIT_LANES;
+  // DFT_SPLIT_LANES overrides it, for launch-shape studies)
+  long long split_lanes = SPLIT_LANES;
   if (const char* e = std::getenv("DFT_SPLIT_LANES")) {
     const long long v = std::atoll(e);
     if (v > 0) split_lanes = v;
-  }
-  if (const char* e = std::getenv("DFT_SPLIT_PAIR_BELOW")) {
-    const long long v = std::atoll(e);
-    if (v >= 0) split_pair = v;
   }
   // Lane table of a split lane-kernel combination: ceil(n_b / bpl) entries
   // per quartet (n_b its bra pair's FP64 primitive pairs), all in one wave;
@@ -1031,9 +1028,10 @@ int build_plan(dft_plan p, const Input& in) {
       // in the wave); ket chunks stay chunk-major (one slot row each), so the
       // wave keeps one ket primitive range.
       const long long nq = (long long)c.q.size();
-      const long long want = nq <= 0 ? 1
-                             : nq < split_lanes / 8 ? (split_lanes + nq - 1) / nq
-                             : nq < split_pair ? 2 : 1;
+      const long long nreal_all = c.r64 + c.r32a + c.r32b;
+      long long want = nq > 0 && nq < split_lanes ? (split_lanes + nq - 1) / nq : 1;
+      if (nreal_all > 0)  // at least SPLIT_MIN_PRIM primitive quartets per lane
+        want = std::min(want, std::max(1LL, (c.kprod64 + c.kprod32) / (nreal_all * SPLIT_MIN_PRIM)));
       if (p->split && lane && want > 1) {
         int nbmax = 0, nkmax = 0;
         for (const int2& qq : c.q) {
