@@ -427,7 +427,6 @@ __global__ __launch_bounds__(SCF_EW_THREADS) void scf_diis_dots_kernel(int n, in
 
 __global__ __launch_bounds__(64) void scf_diis_solve_kernel(int h, int nblk, double* __restrict__ G,
                                                             double* __restrict__ st) {
-  __shared__ double gauss[SCF_HMAX][SCF_HMAX + 1];
   const int tid = threadIdx.x;
   const int count = (int)st[SCF_ST_COUNT];
   const int slot = count % h;
@@ -439,46 +438,90 @@ __global__ __launch_bounds__(64) void scf_diis_solve_kernel(int h, int nblk, dou
     G[tid * h + slot] = x;
   }
   __syncthreads();
-  if (tid == 0) {
-    // Gaussian elimination with partial pivoting on the padded system (in
-    // LDS: run-time indexed, it would live in scratch memory otherwise)
-    double(*M)[SCF_HMAX + 1] = gauss;
-    for (int i = 0; i < h; ++i) {
-      for (int j = 0; j < h; ++j) M[i][j] = (i < used && j < used) ? G[i * h + j] : (i == j ? 1.0 : 0.0);
-      M[i][h] = i < used ? 1.0 : 0.0;
-    }
-    for (int k = 0; k < h; ++k) {
-      int piv = k;
-      for (int i = k + 1; i < h; ++i)
-        if (fabs(M[i][k]) > fabs(M[piv][k])) piv = i;
-      if (piv != k)
-        for (int j = k; j <= h; ++j) {
-          const double x = M[k][j];
-          M[k][j] = M[piv][j];
-          M[piv][j] = x;
-        }
-      const double d = M[k][k];
-      for (int i = k + 1; i < h; ++i) {
-        const double f = M[i][k] / d;
-        for (int j = k; j <= h; ++j) M[i][j] -= f * M[k][j];
+  // Gaussian elimination with partial pivoting on the padded system (rows
+  // beyond `used` are identity rows with a zero right-hand side), one row
+  // per lane in registers: the pivot search is a wave reduction (largest
+  // |M[i][k]|, the lowest row on ties), a row swap and the pivot row's
+  // broadcast are lane shuffles, and every lane updates its own row with
+  // the same operations, in the same order, as a sequential elimination.
+  const int lane = tid;
+  double row[SCF_HMAX + 1];
+#pragma unroll
+  for (int j = 0; j <= SCF_HMAX; ++j) {
+    double x = 0.0;
+    if (lane < h && j < h) x = (lane < used && j < used) ? G[lane * h + j] : (lane == j ? 1.0 : 0.0);
+    if (lane < h && j == h) x = lane < used ? 1.0 : 0.0;
+    row[j] = x;
+  }
+  for (int k = 0; k < h; ++k) {
+    // pivot: largest |M[i][k]| over rows i >= k (first such row)
+    double mag = (lane >= k && lane < h) ? fabs(row[0]) : -1.0;
+#pragma unroll
+    for (int j = 1; j <= SCF_HMAX; ++j)
+      if (j == k) mag = (lane >= k && lane < h) ? fabs(row[j]) : -1.0;
+    int piv = lane;
+    for (int off = 32; off > 0; off >>= 1) {
+      const double om = __shfl_xor(mag, off, 64);
+      const int op = __shfl_xor(piv, off, 64);
+      if (om > mag || (om == mag && op < piv)) {
+        mag = om;
+        piv = op;
       }
     }
-    double* c = st + SCF_ST_COEF;
-    double sum = 0.0;
-    for (int k = h - 1; k >= 0; --k) {
-      double x = M[k][h];
-      for (int j = k + 1; j < h; ++j) x -= M[k][j] * c[j];
-      c[k] = x / M[k][k];
+    // rows k and piv trade places
+    const int src = lane == k ? piv : (lane == piv ? k : lane);
+#pragma unroll
+    for (int j = 0; j <= SCF_HMAX; ++j) row[j] = __shfl(row[j], src, 64);
+    // pivot row broadcast, then rows below it eliminate column k
+    double pr[SCF_HMAX + 1];
+#pragma unroll
+    for (int j = 0; j <= SCF_HMAX; ++j) pr[j] = __shfl(row[j], k, 64);
+    double d = pr[0], mk = row[0];
+#pragma unroll
+    for (int j = 1; j <= SCF_HMAX; ++j)
+      if (j == k) {
+        d = pr[j];
+        mk = row[j];
+      }
+    if (lane > k && lane < h) {
+      const double f = mk / d;
+#pragma unroll
+      for (int j = 0; j <= SCF_HMAX; ++j)
+        if (j >= k && j <= h) row[j] -= f * pr[j];
     }
+  }
+  // back substitution: c[k] from row k (lane k), broadcast to every lane
+  double c[SCF_HMAX];
+#pragma unroll
+  for (int j = 0; j < SCF_HMAX; ++j) c[j] = 0.0;
+  for (int k = h - 1; k >= 0; --k) {
+    double x = 0.0, diag = 1.0;
+#pragma unroll
+    for (int j = 0; j <= SCF_HMAX; ++j)
+      if (j == h) x = row[j];
+#pragma unroll
+    for (int j = 0; j < SCF_HMAX; ++j)
+      if (j > k && j < h) x -= row[j] * c[j];
+#pragma unroll
+    for (int j = 0; j < SCF_HMAX; ++j)
+      if (j == k) diag = row[j];
+    const double ck = __shfl(x / diag, k, 64);
+#pragma unroll
+    for (int j = 0; j < SCF_HMAX; ++j)
+      if (j == k) c[j] = ck;
+  }
+  if (lane == 0) {
+    double* co = st + SCF_ST_COEF;
+    double sum = 0.0;
     for (int k = 0; k < h; ++k) sum += c[k];
-    for (int k = 0; k < h; ++k) c[k] /= sum;
+    for (int k = 0; k < h; ++k) co[k] = c[k] / sum;
     // A singular Gram matrix (residuals that no longer differ, as near a
     // tight convergence) has no DIIS solution: the iteration then uses its
     // own Fock matrix (c = e_slot) instead of non-finite coefficients.
     bool ok = isfinite(sum) && sum != 0.0;
-    for (int k = 0; k < h; ++k) ok = ok && isfinite(c[k]);
+    for (int k = 0; k < h; ++k) ok = ok && isfinite(co[k]);
     if (!ok)
-      for (int k = 0; k < h; ++k) c[k] = k == slot ? 1.0 : 0.0;
+      for (int k = 0; k < h; ++k) co[k] = k == slot ? 1.0 : 0.0;
     st[SCF_ST_SLOT] = slot;
   }
 }
