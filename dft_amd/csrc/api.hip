@@ -2412,13 +2412,28 @@ int dft_eri_prim(int64_t n_ab, const double* p_ab, const double* P_ab, const dou
 
 int dft_syevj_tol(int n, int batch, double* A, double* w, double* V, int init_v, int32_t* sweeps, double tol,
                   void* stream) {
+  return dft_syevj_ws(n, batch, A, w, V, init_v, sweeps, tol, nullptr, 0, stream);
+}
+
+int64_t dft_syevj_ws_bytes(int n) {
+  if (n <= 0 || n > JAC_NMAX) return 0;
+  return (int64_t)syevj_log_bytes(n);
+}
+
+int dft_syevj_ws(int n, int batch, double* A, double* w, double* V, int init_v, int32_t* sweeps, double tol,
+                 void* work, int64_t work_bytes, void* stream) {
   if (n <= 0 || batch < 0) return fail(DFT_EINVAL, "n must be positive and batch >= 0");
   if (n > JAC_NMAX) return fail(DFT_EUNSUPPORTED, "dft_syevj: order " + std::to_string(n) + " above " +
                                                       std::to_string(JAC_NMAX));
   if (!(tol >= 0.0)) return fail(DFT_EINVAL, "dft_syevj_tol: tol must be >= 0");
   if (batch > 0 && (!A || !w || !V)) return fail(DFT_EINVAL, "NULL argument");
+  const long long per = syevj_log_bytes(n);
+  if (work && work_bytes < per * (long long)batch)
+    return fail(DFT_EINVAL, "dft_syevj_ws: work_bytes " + std::to_string(work_bytes) + " below batch x " +
+                                std::to_string(per));
   const long long nn = (long long)n * n;
-  hipError_t e = launch_syevj(n, batch, A, nn, w, n, V, nn, init_v ? 1 : 0, sweeps, tol, (hipStream_t)stream);
+  hipError_t e = launch_syevj(n, batch, A, nn, w, n, V, nn, init_v ? 1 : 0, sweeps, tol, (char*)work,
+                              work ? per : 0, (hipStream_t)stream);
   if (e != hipSuccess) return fail(DFT_EHIP, std::string("syevj: ") + hipGetErrorString(e));
   return DFT_OK;
 }

@@ -216,10 +216,15 @@ hipError_t launch_eri_prim(long long nab, const double* pab, const double* Pab, 
 // matrix, order n <= JAC_NMAX): A (batch x n x n, row-major, destroyed), w
 // (batch x n, ascending), V (batch x n x n, eigenvectors as columns; with
 // init_v = 0 it holds an orthonormal starting basis on entry and the result
-// is that basis times the rotations)
+// is that basis times the rotations). With a rotation log (jlog, at least
+// syevj_log_bytes(n) bytes per matrix, jlog_stride apart) the matrix
+// workgroup keeps no eigenvectors: it logs its rotations and a second kernel
+// applies them to the rows of V in parallel.
 constexpr int JAC_NMAX = 128;
+long long syevj_log_bytes(int n);
 hipError_t launch_syevj(int n, int batch, double* A, long long sa, double* w, long long sw, double* V,
-                        long long sv, int init_v, int* sweeps, double tol, hipStream_t s);
+                        long long sv, int init_v, int* sweeps, double tol, char* jlog, long long jlog_stride,
+                        hipStream_t s);
 
 // device SCF iteration (state layout: the DFT_SCF_ST_* indices of dft_hip.h)
 constexpr int SCF_HMAX = 16;

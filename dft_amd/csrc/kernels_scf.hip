@@ -49,22 +49,39 @@ __device__ __forceinline__ int rr_pos(int i, int r, int m) {
   return 1 + (x >= m - 1 ? x - (m - 1) : x);
 }
 
-// Upper triangle of the symmetric m x m matrix, packed by rows.
+// The pair of round r at table position k, lower index first.
+__device__ __forceinline__ int2 rr_pair(int k, int r, int m) {
+  const int p = rr_pos(k, r, m), q = rr_pos(m - 1 - k, r, m);
+  return p < q ? make_int2(p, q) : make_int2(q, p);
+}
+
+// Upper triangle of the symmetric m x m matrix, packed by rows (24-bit
+// products: indices and orders stay below 2^12; at lo = 0 the second
+// product is 0 whatever lo - 1 wraps to).
 __device__ __forceinline__ int tri(int r, int c, int m) {
-  if (r > c) {
-    const int t = r;
-    r = c;
-    c = t;
-  }
-  return r * m - (r * (r - 1)) / 2 + (c - r);
+  const unsigned lo = (unsigned)min(r, c), hi = (unsigned)max(r, c);
+  return (int)(__umul24(lo, (unsigned)m) - (__umul24(lo, lo - 1u) >> 1) + (hi - lo));
+}
+
+// Rotation log of one matrix (LOG kernels): an int header (the sweeps done,
+// then the rank of each column in ascending eigenvalue order), then (c, s)
+// of every table position k of every round, JAC_MAX_SWEEPS sweeps of m - 1
+// rounds of m / 2 positions; the pair of a position is rr_pair(k, r, m).
+constexpr long long JAC_LOG_HEAD = 1024;  // bytes, >= 4 (1 + JAC_NMAX)
+long long syevj_log_bytes(int n) {
+  const int m = n + (n & 1);
+  return JAC_LOG_HEAD + (long long)JAC_MAX_SWEEPS * (m - 1) * (m / 2) * (long long)sizeof(double2);
 }
 
 // VL: eigenvectors in LDS (with the packed matrix) or in the output buffer.
-template <bool VL>
+// LOG: no eigenvectors here; the rotations go to the log (jlog, jlog_stride
+// bytes per matrix) and jacobi_apply_kernel applies them to the rows of V.
+template <bool VL, bool LOG>
 __global__ __launch_bounds__(JAC_THREADS) void jacobi_eigh_kernel(int n, const double* __restrict__ A, long long sa,
                                                                   double* __restrict__ w, long long sw,
                                                                   double* __restrict__ V, long long sv,
-                                                                  int init_v, int* __restrict__ sweeps, double tol) {
+                                                                  int init_v, int* __restrict__ sweeps, double tol,
+                                                                  char* __restrict__ jlog, long long jlog_stride) {
   extern __shared__ double lds[];
   __shared__ double cs_c[2][JAC_NMAX / 2], cs_s[2][JAC_NMAX / 2], cs_t[2][JAC_NMAX / 2];
   __shared__ int cs_p[2][JAC_NMAX / 2], cs_q[2][JAC_NMAX / 2];
@@ -79,16 +96,19 @@ __global__ __launch_bounds__(JAC_THREADS) void jacobi_eigh_kernel(int n, const d
   const double* Ab = A + (size_t)blockIdx.x * sa;
   double* __restrict__ Vb = V + (size_t)blockIdx.x * sv;
   double* vl = lds + ntri;  // n x n (VL only)
+  char* lg = LOG ? jlog + (size_t)blockIdx.x * jlog_stride : nullptr;
+  double2* lcs = LOG ? (double2*)(lg + JAC_LOG_HEAD) : nullptr;
   for (int i = tid; i < m * m; i += nt) {
     const int r = i / m, c = i - r * m;
     if (r <= c) a[tri(r, c, m)] = (r < n && c < n) ? Ab[(size_t)r * n + c] : 0.0;
   }
-  for (int i = tid; i < n * n; i += nt) {
-    const int r = i / n, c = i - r * n;
-    const double x = init_v ? (r == c ? 1.0 : 0.0) : Vb[i];
-    if (VL) vl[i] = x;
-    else Vb[i] = x;
-  }
+  if (!LOG)
+    for (int i = tid; i < n * n; i += nt) {
+      const int r = i / n, c = i - r * n;
+      const double x = init_v ? (r == c ? 1.0 : 0.0) : Vb[i];
+      if (VL) vl[i] = x;
+      else Vb[i] = x;
+    }
   if (tid == 0) rotated = 0;
   __syncthreads();
   // ||A||_F by a fixed-order reduction: per-thread partial, per-wave
@@ -152,12 +172,8 @@ __global__ __launch_bounds__(JAC_THREADS) void jacobi_eigh_kernel(int n, const d
       if (tid < 64) {
         if (r < m - 1)
           for (int k = tid; k < h; k += 64) {
-            int p = rr_pos(k, r, m), q = rr_pos(m - 1 - k, r, m);
-            if (p > q) {
-              const int t = p;
-              p = q;
-              q = t;
-            }
+            const int2 pq = rr_pair(k, r, m);
+            const int p = pq.x, q = pq.y;
             const double apq = a[tri(p, q, m)];
             double c = 1.0, s = 0.0, t = 0.0;
             if (fabs(apq) > thresh) {
@@ -174,8 +190,9 @@ __global__ __launch_bounds__(JAC_THREADS) void jacobi_eigh_kernel(int n, const d
             cs_t[cur][k] = t;
             cs_p[cur][k] = p;
             cs_q[cur][k] = q;
+            if (LOG) lcs[(size_t)(sweep * (m - 1) + r) * h + k] = make_double2(c, s);
           }
-      } else if (r > 0 && vk >= 0) {
+      } else if (!LOG && r > 0 && vk >= 0) {
         // eigenvector columns p_k, q_k of rows g, g + ng, ... for this
         // thread's fixed pair k and row group g (its rotation read once), in
         // groups of JAC_V_GROUP rows loaded before any is stored
@@ -223,7 +240,10 @@ __global__ __launch_bounds__(JAC_THREADS) void jacobi_eigh_kernel(int n, const d
         for (int j = 0; j < JAC_BLK_PER_THREAD; ++j) {
           if (j < nb) {
             const int k = bkl[j] & 255, l = bkl[j] >> 8;
-            const int pk = cs_p[cur][k], qk = cs_q[cur][k], pl = cs_p[cur][l], ql = cs_q[cur][l];
+            // the pairs of this round, computed here rather than read from
+            // the rotation table
+            const int2 pqk = rr_pair(k, r, m), pql = rr_pair(l, r, m);
+            const int pk = pqk.x, qk = pqk.y, pl = pql.x, ql = pql.y;
             if (k == l) {
               ix[j][0] = tri(pk, pk, m);
               ix[j][1] = tri(pk, qk, m);
@@ -274,10 +294,24 @@ __global__ __launch_bounds__(JAC_THREADS) void jacobi_eigh_kernel(int n, const d
     if (!any) break;
   }
   if (sweeps && tid == 0) sweeps[blockIdx.x] = done;
+  if (LOG && tid == 0) ((int*)lg)[0] = done;
   // ascending order: rank of each eigenvalue (ties by index)
   for (int i = tid; i < n; i += nt) diag[i] = a[tri(i, i, m)];
   __syncthreads();
   double* wb = w + (size_t)blockIdx.x * sw;
+  if (LOG) {
+    for (int i = tid; i < n; i += nt) {
+      const double di = diag[i];
+      int rank = 0;
+      for (int j = 0; j < n; ++j) {
+        const double dj = diag[j];
+        rank += (dj < di || (dj == di && j < i)) ? 1 : 0;
+      }
+      wb[rank] = di;
+      ((int*)lg)[1 + i] = rank;
+    }
+    return;
+  }
   const double* v = vl;
   if (!VL) {
     // stage V through LDS (the packed matrix is no longer needed) to permute its columns
@@ -298,29 +332,104 @@ __global__ __launch_bounds__(JAC_THREADS) void jacobi_eigh_kernel(int n, const d
   }
 }
 
+// The eigenvectors from a rotation log: one wave per row of V (the row in
+// LDS), the logged rounds in the order they were made, each pair's rotation
+// with the same arithmetic as the in-workgroup update (so the same V bit for
+// bit); then the row is written with its columns in ascending eigenvalue
+// order. Rows are independent, so the waves need no barrier; the pairs of a
+// round are disjoint, and a wave's LDS accesses execute in program order.
+constexpr int JAC_APPLY_ROWS = 4;    // rows (waves) per workgroup
+constexpr int JAC_APPLY_AHEAD = 8;   // rounds of (c, s) loaded ahead
+
+__global__ __launch_bounds__(64 * JAC_APPLY_ROWS) void jacobi_apply_kernel(int n, const char* __restrict__ jlog,
+                                                                           long long jlog_stride,
+                                                                           double* __restrict__ V, long long sv,
+                                                                           int init_v) {
+  __shared__ double rows[JAC_APPLY_ROWS][JAC_NMAX];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i = blockIdx.x * JAC_APPLY_ROWS + wv;
+  if (i >= n) return;  // whole waves; nothing below waits for other waves
+  const char* lg = jlog + (size_t)blockIdx.y * jlog_stride;
+  const int* head = (const int*)lg;
+  const double2* lcs = (const double2*)(lg + JAC_LOG_HEAD);
+  double* __restrict__ Vr = V + (size_t)blockIdx.y * sv + (size_t)i * n;
+  double* x = rows[wv];
+  for (int j = lane; j < n; j += 64) x[j] = init_v ? (i == j ? 1.0 : 0.0) : Vr[j];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const int m = n + (n & 1), h = m / 2;
+  const int nround = head[0] * (m - 1);
+  const bool act = lane < h;
+  double2 g[JAC_APPLY_AHEAD], gn[JAC_APPLY_AHEAD];
+#pragma unroll
+  for (int u = 0; u < JAC_APPLY_AHEAD; ++u)
+    g[u] = (act && u < nround) ? lcs[(size_t)u * h + lane] : make_double2(1.0, 0.0);
+  int r = 0;  // round within the sweep of round t
+  for (int t0 = 0; t0 < nround; t0 += JAC_APPLY_AHEAD) {
+#pragma unroll
+    for (int u = 0; u < JAC_APPLY_AHEAD; ++u) {
+      const int t = t0 + JAC_APPLY_AHEAD + u;
+      gn[u] = (act && t < nround) ? lcs[(size_t)t * h + lane] : make_double2(1.0, 0.0);
+    }
+#pragma unroll
+    for (int u = 0; u < JAC_APPLY_AHEAD; ++u) {
+      if (t0 + u < nround) {
+        if (act && g[u].y != 0.0) {
+          const int2 pq = rr_pair(lane, r, m);
+          if (pq.y < n) {
+            const double c = g[u].x, sr = g[u].y;
+            const double xp = x[pq.x], xq = x[pq.y];
+            x[pq.x] = c * xp - sr * xq;
+            x[pq.y] = sr * xp + c * xq;
+          }
+        }
+        r = r + 1 == m - 1 ? 0 : r + 1;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < JAC_APPLY_AHEAD; ++u) g[u] = gn[u];
+  }
+  for (int j = lane; j < n; j += 64) Vr[head[1 + j]] = x[j];
+}
+
 hipError_t launch_syevj(int n, int batch, double* A, long long sa, double* w, long long sw, double* V,
-                        long long sv, int init_v, int* sweeps, double tol, hipStream_t s) {
+                        long long sv, int init_v, int* sweeps, double tol, char* jlog, long long jlog_stride,
+                        hipStream_t s) {
   if (batch <= 0 || n <= 0) return hipSuccess;
   if (n > JAC_NMAX) return hipErrorInvalidValue;
+  if (jlog && jlog_stride < syevj_log_bytes(n)) return hipErrorInvalidValue;
   const int m = n + (n & 1);
   const size_t a_bytes = (size_t)m * (m + 1) / 2 * sizeof(double), v_bytes = (size_t)n * n * sizeof(double);
   const bool vl = a_bytes + v_bytes + JAC_STATIC_LDS <= (size_t)JAC_LDS_BYTES;
   // the staging of V for the final column permutation reuses the matrix area
-  const size_t dyn = vl ? a_bytes + v_bytes : std::max(a_bytes, v_bytes);
+  const size_t dyn = jlog ? a_bytes : vl ? a_bytes + v_bytes : std::max(a_bytes, v_bytes);
   static bool attr_set = false;
   if (!attr_set) {
-    for (const void* k : {(const void*)jacobi_eigh_kernel<true>, (const void*)jacobi_eigh_kernel<false>}) {
+    for (const void* k : {(const void*)jacobi_eigh_kernel<true, false>, (const void*)jacobi_eigh_kernel<false, false>,
+                          (const void*)jacobi_eigh_kernel<false, true>}) {
       hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, JAC_LDS_BYTES - JAC_STATIC_LDS);
       if (e != hipSuccess) return e;
     }
     attr_set = true;
   }
-  if (vl)
-    hipLaunchKernelGGL(jacobi_eigh_kernel<true>, dim3(batch), dim3(JAC_THREADS), dyn, s, n, A, sa, w, sw, V, sv,
-                       init_v, sweeps, tol);
-  else
-    hipLaunchKernelGGL(jacobi_eigh_kernel<false>, dim3(batch), dim3(JAC_THREADS), dyn, s, n, A, sa, w, sw, V, sv,
-                       init_v, sweeps, tol);
+  if (jlog) {
+    hipLaunchKernelGGL((jacobi_eigh_kernel<false, true>), dim3(batch), dim3(JAC_THREADS), dyn, s, n, A, sa, w, sw, V,
+                       sv, init_v, sweeps, tol, jlog, jlog_stride);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(jacobi_apply_kernel, dim3((n + JAC_APPLY_ROWS - 1) / JAC_APPLY_ROWS, batch),
+                       dim3(64 * JAC_APPLY_ROWS), 0, s, n, (const char*)jlog, jlog_stride, V, sv, init_v);
+  } else if (vl) {
+    hipLaunchKernelGGL((jacobi_eigh_kernel<true, false>), dim3(batch), dim3(JAC_THREADS), dyn, s, n, A, sa, w, sw, V,
+                       sv, init_v, sweeps, tol, (char*)nullptr, 0LL);
+  } else {
+    hipLaunchKernelGGL((jacobi_eigh_kernel<false, false>), dim3(batch), dim3(JAC_THREADS), dyn, s, n, A, sa, w, sw,
+                       V, sv, init_v, sweeps, tol, (char*)nullptr, 0LL);
+  }
   return hipGetLastError();
 }
 

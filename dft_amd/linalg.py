@@ -23,13 +23,21 @@ from . import _lib
 JACOBI_NMAX = 128
 
 
+# batches whose rotation logs fit this many bytes take the two-kernel path
+# (dft_syevj_ws: the eigenvectors updated outside the matrix workgroup)
+SYEVJ_LOG_MAX_BYTES = 64 << 20
+
+
 def syevj_(A: torch.Tensor, w: torch.Tensor, V: torch.Tensor, init_v: bool, stream=None, sweeps=None,
-           tol: float = 0.0) -> None:
+           tol: float = 0.0, log=None) -> None:
     """In place: A (batch, n, n) contiguous float64 is destroyed, w (batch, n)
     and V (batch, n, n) receive the eigenpairs; with init_v False, V holds
     the starting orthonormal basis Q and A must be Q^T F Q. ``sweeps`` (an
     int32 device tensor of batch entries) receives the Jacobi sweeps.
-    ``tol`` > 0: off-diagonal entries up to tol are left (dft_syevj_tol)."""
+    ``tol`` > 0: off-diagonal entries up to tol are left (dft_syevj_tol).
+    ``log``: True / False forces the rotation-log path (dft_syevj_ws, a
+    device workspace of batch x dft_syevj_ws_bytes(n) bytes) on or off; None
+    takes it when that workspace is at most SYEVJ_LOG_MAX_BYTES."""
     lib = _lib.load()
     n = A.shape[-1]
     batch = A.numel() // (n * n) if n else 0
@@ -42,8 +50,18 @@ def syevj_(A: torch.Tensor, w: torch.Tensor, V: torch.Tensor, init_v: bool, stre
         if not (sweeps.is_cuda and sweeps.dtype == torch.int32 and sweeps.numel() >= batch):
             raise ValueError("sweeps must be an int32 device tensor with batch entries")
         sw = sweeps.data_ptr()
-    _lib.check(lib.dft_syevj_tol(n, batch, A.data_ptr(), w.data_ptr(), V.data_ptr(), int(bool(init_v)), sw,
-                                 float(tol), C.c_void_p(s.cuda_stream)))
+    per = int(lib.dft_syevj_ws_bytes(n))
+    if log is None:
+        log = 0 < batch * per <= SYEVJ_LOG_MAX_BYTES
+    work, nbytes = None, 0
+    if log and batch > 0:
+        nbytes = batch * per
+        buf = torch.empty(nbytes, dtype=torch.uint8, device=A.device)
+        if stream is not None:
+            buf.record_stream(s)  # freed here, still read by the queued kernels
+        work = buf.data_ptr()
+    _lib.check(lib.dft_syevj_ws(n, batch, A.data_ptr(), w.data_ptr(), V.data_ptr(), int(bool(init_v)), sw,
+                                float(tol), work, nbytes, C.c_void_p(s.cuda_stream)))
 
 
 def eigh(F: torch.Tensor, V0: torch.Tensor | None = None, tol: float = 0.0):

@@ -362,6 +362,31 @@ int dft_syevj_tol(int n, int batch, double* A, double* w, double* V, int init_v,
                   void* stream);
 
 /*
+ * Bytes of device workspace dft_syevj_ws takes per matrix of order n (0 for
+ * n outside 1..128): a rotation log of 40 sweeps of (n + n mod 2 - 1)
+ * rounds of (n + n mod 2) / 2 rotations, 16 bytes each, plus a 1 KiB header;
+ * 4.0 MB at n = 112.
+ */
+int64_t dft_syevj_ws_bytes(int n);
+
+/*
+ * dft_syevj_tol with a device workspace (work, work_bytes >= batch x
+ * dft_syevj_ws_bytes(n), or NULL for dft_syevj_tol itself). The matrix
+ * workgroup then keeps no eigenvectors: it records every round's rotations
+ * in the workspace, and a second kernel applies them to the rows of V, one
+ * wave per row (the same rotations in the same order and arithmetic). For
+ * small batches (the SCF step), where the eigenvector update shares the
+ * matrix workgroup's LDS bandwidth. Same results and error behaviour as
+ * dft_syevj_tol; DFT_EINVAL for a workspace below the size. Still only
+ * enqueues work (two kernels), so it can be captured into a hipGraph.
+ *
+ * Replaces the same eigendecompositions as dft_syevj (shelled.py:1210-1213,
+ * main.py:796-799).
+ */
+int dft_syevj_ws(int n, int batch, double* A, double* w, double* V, int init_v, int32_t* sweeps, double tol,
+                 void* work, int64_t work_bytes, void* stream);
+
+/*
  * One damped + DIIS SCF iteration on the device, closed shell (the loop of
  * shelled.py:1164-1260 with P_a = P_b), split in two calls around the
  * orbital step so that a whole iteration (dft_jk, these two, dft_syevj and

@@ -158,3 +158,44 @@ def test_syevj_tolerance_leaves_only_small_entries(torch_cuda):
     D = V.T @ A2 @ V
     assert np.abs(D - np.diag(w)).max() < 1e-9 + 1e-12 * np.abs(A2).max() * n
     assert np.abs(w - out[0.0][0]).max() < 1e-12 * np.abs(A2).max() * n
+
+
+@pytest.mark.parametrize("n", [1, 7, 58, 97, 112, 128])
+def test_rotation_log_path_equals_the_workgroup_path(torch_cuda, n):
+    """dft_syevj_ws (rotations logged, eigenvectors updated by a second
+    kernel one row per wave) against dft_syevj_tol (eigenvectors updated in
+    the matrix workgroup, in LDS up to n = 114 and in global memory above):
+    the same rotations in the same order and arithmetic, so the same
+    eigenvalues, sweeps and eigenvectors bit for bit; cold starts and warm
+    starts from a nearby matrix's eigenvectors, two matrices per batch."""
+    from dft_amd.linalg import syevj_
+    torch = torch_cuda
+    rng = np.random.default_rng(100 + n)
+    M = rng.standard_normal((2, n, n))
+    A = M + M.transpose(0, 2, 1)
+    Ad = torch.as_tensor(A, device="cuda")
+    res = {}
+    for log in (False, True):
+        A1 = Ad.clone()
+        w = torch.empty((2, n), dtype=torch.float64, device="cuda")
+        V = torch.empty_like(A1)
+        sw = torch.zeros(2, dtype=torch.int32, device="cuda")
+        syevj_(A1, w, V, True, sweeps=sw, log=log)
+        # warm start: the eigenvectors of A for A + a small symmetric change
+        E = torch.as_tensor(rng.standard_normal((2, n, n)) * 1e-4 if not log else res[False][4], device="cuda")
+        A2 = Ad + E + E.transpose(1, 2)
+        Q = V.clone()
+        B = Q.transpose(1, 2) @ A2 @ Q
+        B = (0.5 * (B + B.transpose(1, 2))).contiguous()
+        w2 = torch.empty_like(w)
+        sw2 = torch.zeros_like(sw)
+        syevj_(B, w2, Q, False, sweeps=sw2, log=log)
+        res[log] = (w, V, sw, (w2, Q, sw2), E.cpu().numpy())
+        torch.cuda.synchronize()
+    a, b = res[False], res[True]
+    assert torch.equal(a[2], b[2]) and torch.equal(a[3][2], b[3][2])
+    assert torch.equal(a[0], b[0]) and torch.equal(a[3][0], b[3][0])
+    assert torch.equal(a[1], b[1]), (a[1] - b[1]).abs().max().item()
+    assert torch.equal(a[3][1], b[3][1]), (a[3][1] - b[3][1]).abs().max().item()
+    for k in range(2):
+        _check(A[k], b[0][k].cpu().numpy(), b[1][k].cpu().numpy())

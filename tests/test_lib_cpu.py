@@ -67,6 +67,28 @@ def test_eigensolver_rejects_orders_above_its_limit():
         assert b"tol" in lib.dft_last_error()
 
 
+def test_eigensolver_workspace_size_and_check():
+    """dft_syevj_ws_bytes: a 1 KiB header and 40 sweeps of (m - 1) rounds of
+    m / 2 rotations of 16 bytes (m = n rounded up to even); dft_syevj_ws
+    rejects a workspace below batch times that before it touches the device
+    (placeholder pointers: nothing is read)."""
+    from dft_amd import _lib
+    lib = _lib.load()
+    for n in (1, 2, 7, 112, 128):
+        m = n + (n & 1)
+        assert lib.dft_syevj_ws_bytes(n) == 1024 + 40 * (m - 1) * (m // 2) * 16
+    assert lib.dft_syevj_ws_bytes(112) == 3_979_264
+    for n in (0, 129, -3):
+        assert lib.dft_syevj_ws_bytes(n) == 0
+    fake = ctypes.c_void_p(0x1000)
+    need = 3 * lib.dft_syevj_ws_bytes(40)
+    rc = lib.dft_syevj_ws(40, 3, fake, fake, fake, 1, None, 0.0, fake, need - 1, None)
+    assert rc == _lib.DFT_EINVAL
+    assert b"work_bytes" in lib.dft_last_error()
+    rc = lib.dft_syevj_ws(129, 1, fake, fake, fake, 1, None, 0.0, fake, need, None)
+    assert rc == _lib.DFT_EUNSUPPORTED
+
+
 def test_python_constants_match_the_header():
     from dft_amd import _lib
     text = open(os.path.join(ROOT, "include", "dft_hip.h")).read()
