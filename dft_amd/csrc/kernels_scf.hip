@@ -190,7 +190,12 @@ __global__ __launch_bounds__(JAC_THREADS) void jacobi_eigh_kernel(int n, const d
             cs_t[cur][k] = t;
             cs_p[cur][k] = p;
             cs_q[cur][k] = q;
-            if (LOG) lcs[(size_t)(sweep * (m - 1) + r) * h + k] = make_double2(c, s);
+            // logged for the order of the tournament slots (k, m - 1 - k),
+            // which jacobi_apply_kernel keeps in one lane: s, or -s where
+            // slot k holds the larger index
+            if (LOG)
+              lcs[(size_t)(sweep * (m - 1) + r) * h + k] =
+                  make_double2(c, rr_pos(k, r, m) < rr_pos(m - 1 - k, r, m) ? s : -s);
           }
       } else if (!LOG && r > 0 && vk >= 0) {
         // eigenvector columns p_k, q_k of rows g, g + ng, ... for this
@@ -214,7 +219,10 @@ __global__ __launch_bounds__(JAC_THREADS) void jacobi_eigh_kernel(int n, const d
             for (int u = 0; u < JAC_V_GROUP; ++u) {
               const int i = i0 + u * vng;
               if (i < n) {
-                const double np_ = c * xp[u] - sr * xq[u], nq_ = sr * xp[u] + c * xq[u];
+                // explicit fused forms, shared with jacobi_apply_kernel (c
+                // multiplies the entry itself in both, so swapping p and q
+                // is the same arithmetic with -s)
+                const double np_ = fma(c, xp[u], -(sr * xq[u])), nq_ = fma(c, xq[u], sr * xp[u]);
                 if (VL) {
                   vl[i * n + p] = np_;
                   vl[i * n + q] = nq_;
@@ -332,68 +340,86 @@ __global__ __launch_bounds__(JAC_THREADS) void jacobi_eigh_kernel(int n, const d
   }
 }
 
-// The eigenvectors from a rotation log: one wave per row of V (the row in
-// LDS), the logged rounds in the order they were made, each pair's rotation
-// with the same arithmetic as the in-workgroup update (so the same V bit for
-// bit); then the row is written with its columns in ascending eigenvalue
-// order. Rows are independent, so the waves need no barrier; the pairs of a
-// round are disjoint, and a wave's LDS accesses execute in program order.
+// A double moved one lane along the wave by DPP (wave_shl1 = 0x130: lane k
+// takes lane k + 1's value; wave_shr1 = 0x138: lane k takes lane k - 1's;
+// a lane without a source gets 0).
+template <int CTRL>
+__device__ __forceinline__ double wave_shift(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// The eigenvectors from a rotation log: one wave per row of V, the row in
+// registers. Lane k holds the entries at tournament slots k and m - 1 - k,
+// which form pair k of every round; slot 0 keeps index 0 and the indices at
+// slots 1 .. m - 1 move one slot down per round (rr_pos(i, r + 1) =
+// rr_pos(i + 1, r), cyclically), so between rounds the first entries shift
+// one lane down and the second entries one lane up (DPP), with the two ends
+// of the ring turning round inside lanes h - 1 and 0. The log holds s signed
+// for the slot order, so a round is the in-workgroup update's explicit fused
+// forms with no selects (the same V bit for bit, up to the sign of a zero
+// where a pair is skipped). A pair with the padding index n of an odd order
+// is never rotated (its entries are 0), and no lane below h reads a lane at
+// or above h. After the logged whole sweeps every slot holds its own index
+// again, and the row is written with its columns in ascending eigenvalue
+// order.
 constexpr int JAC_APPLY_ROWS = 4;    // rows (waves) per workgroup
 constexpr int JAC_APPLY_AHEAD = 8;   // rounds of (c, s) loaded ahead
+
+__device__ __forceinline__ void jac_apply_round(double& xa, double& xb, double2 g, int lane, int h) {
+  const double na = fma(g.x, xa, -(g.y * xb)), nb = fma(g.x, xb, g.y * xa);
+  const double df = wave_shift<0x130>(na), us = wave_shift<0x138>(nb);
+  // h = 1 (orders 1 and 2): a ring of one slot, nothing moves
+  xa = lane == 0 ? na : (lane == h - 1 ? nb : df);
+  xb = lane == 0 ? (h > 1 ? df : nb) : us;
+}
 
 __global__ __launch_bounds__(64 * JAC_APPLY_ROWS) void jacobi_apply_kernel(int n, const char* __restrict__ jlog,
                                                                            long long jlog_stride,
                                                                            double* __restrict__ V, long long sv,
                                                                            int init_v) {
-  __shared__ double rows[JAC_APPLY_ROWS][JAC_NMAX];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int i = blockIdx.x * JAC_APPLY_ROWS + wv;
-  if (i >= n) return;  // whole waves; nothing below waits for other waves
+  if (i >= n) return;  // whole waves: the shifts below run with all 64 lanes
   const char* lg = jlog + (size_t)blockIdx.y * jlog_stride;
   const int* head = (const int*)lg;
   const double2* lcs = (const double2*)(lg + JAC_LOG_HEAD);
   double* __restrict__ Vr = V + (size_t)blockIdx.y * sv + (size_t)i * n;
-  double* x = rows[wv];
-  for (int j = lane; j < n; j += 64) x[j] = init_v ? (i == j ? 1.0 : 0.0) : Vr[j];
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   const int m = n + (n & 1), h = m / 2;
-  const int nround = head[0] * (m - 1);
   const bool act = lane < h;
+  // round 0: slot k holds index k (k < h <= n); index m - 1 = n of an odd
+  // order is the decoupled padding entry, 0 throughout
+  const int ib = m - 1 - lane;
+  double xa = 0.0, xb = 0.0;
+  if (act) {
+    xa = init_v ? (i == lane ? 1.0 : 0.0) : Vr[lane];
+    if (ib < n) xb = init_v ? (i == ib ? 1.0 : 0.0) : Vr[ib];
+  }
+  const int nround = head[0] * (m - 1);
+  // loads stay inside the log (clamped round and position; what a load past
+  // the logged rounds or of an idle lane returns is never used)
+  const int last = JAC_MAX_SWEEPS * (m - 1) - 1, kk = min(lane, h - 1);
+  auto load = [&](int t) { return lcs[(size_t)min(t, last) * h + kk]; };
   double2 g[JAC_APPLY_AHEAD], gn[JAC_APPLY_AHEAD];
 #pragma unroll
-  for (int u = 0; u < JAC_APPLY_AHEAD; ++u)
-    g[u] = (act && u < nround) ? lcs[(size_t)u * h + lane] : make_double2(1.0, 0.0);
-  int r = 0;  // round within the sweep of round t
-  for (int t0 = 0; t0 < nround; t0 += JAC_APPLY_AHEAD) {
+  for (int u = 0; u < JAC_APPLY_AHEAD; ++u) g[u] = load(u);
+  const int nfull = nround - nround % JAC_APPLY_AHEAD;
+  int t0 = 0;
+  for (; t0 < nfull; t0 += JAC_APPLY_AHEAD) {
 #pragma unroll
-    for (int u = 0; u < JAC_APPLY_AHEAD; ++u) {
-      const int t = t0 + JAC_APPLY_AHEAD + u;
-      gn[u] = (act && t < nround) ? lcs[(size_t)t * h + lane] : make_double2(1.0, 0.0);
-    }
+    for (int u = 0; u < JAC_APPLY_AHEAD; ++u) gn[u] = load(t0 + JAC_APPLY_AHEAD + u);
 #pragma unroll
-    for (int u = 0; u < JAC_APPLY_AHEAD; ++u) {
-      if (t0 + u < nround) {
-        if (act && g[u].y != 0.0) {
-          const int2 pq = rr_pair(lane, r, m);
-          if (pq.y < n) {
-            const double c = g[u].x, sr = g[u].y;
-            const double xp = x[pq.x], xq = x[pq.y];
-            x[pq.x] = c * xp - sr * xq;
-            x[pq.y] = sr * xp + c * xq;
-          }
-        }
-        r = r + 1 == m - 1 ? 0 : r + 1;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      }
-    }
+    for (int u = 0; u < JAC_APPLY_AHEAD; ++u) jac_apply_round(xa, xb, g[u], lane, h);
 #pragma unroll
     for (int u = 0; u < JAC_APPLY_AHEAD; ++u) g[u] = gn[u];
   }
-  for (int j = lane; j < n; j += 64) Vr[head[1 + j]] = x[j];
+  for (int t = t0; t < nround; ++t) jac_apply_round(xa, xb, load(t), lane, h);
+  if (act) {
+    Vr[head[1 + lane]] = xa;
+    if (ib < n) Vr[head[1 + ib]] = xb;
+  }
 }
 
 hipError_t launch_syevj(int n, int batch, double* A, long long sa, double* w, long long sw, double* V,
