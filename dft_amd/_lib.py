@@ -52,6 +52,10 @@ SIGNATURES = {
     "dft_syevj": (C.c_int, [C.c_int, C.c_int, _vp, _vp, _vp, C.c_int, _vp, _vp]),
     "dft_syevj_tol": (C.c_int, [C.c_int, C.c_int, _vp, _vp, _vp, C.c_int, _vp, C.c_double, _vp]),
     "dft_syevj_ws_bytes": (C.c_int64, [C.c_int]),
+    "dft_scf_gemm": (C.c_int, [C.c_int, C.c_int, C.c_int, _vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int, _vp,
+                               C.c_int, _vp, _vp, _vp]),
+    "dft_scf_fock_residual": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "dft_scf_orbital_matrix": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
     "dft_syevj_ws": (C.c_int, [C.c_int, C.c_int, _vp, _vp, _vp, C.c_int, _vp, C.c_double, _vp, C.c_int64, _vp]),
     "dft_scf_diis_stage": (C.c_int, [C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "dft_scf_damp_control": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp, _vp]),

@@ -2464,6 +2464,37 @@ int dft_scf_damp_control(int n, double* Pa, const double* Pauf, double* P, doubl
   return DFT_OK;
 }
 
+int dft_scf_gemm(int m, int ncol, int k, const double* A, int lda, int trans_a, const double* B, int ldb,
+                 int trans_b, double* C, int ldc, const double* copy_src, double* copy_dst, void* stream) {
+  if (m < 1 || ncol < 1 || k < 1 || m > JAC_NMAX || ncol > JAC_NMAX || k > JAC_NMAX)
+    return fail(DFT_EUNSUPPORTED, "dft_scf_gemm: m, ncol, k must be in [1, 128]");
+  if (!A || !B || !C || (copy_src == nullptr) != (copy_dst == nullptr)) return fail(DFT_EINVAL, "NULL argument");
+  if (lda < (trans_a ? m : k) || ldb < (trans_b ? k : ncol) || ldc < ncol)
+    return fail(DFT_EINVAL, "dft_scf_gemm: leading dimension below the row length");
+  hipError_t e = launch_scf_gemm(m, ncol, k, A, lda, trans_a ? 1 : 0, B, ldb, trans_b ? 1 : 0, C, ldc, copy_src,
+                                 copy_dst, (hipStream_t)stream);
+  if (e != hipSuccess) return fail(DFT_EHIP, std::string("scf gemm: ") + hipGetErrorString(e));
+  return DFT_OK;
+}
+
+int dft_scf_fock_residual(int n, const double* H, const double* J, const double* K, const double* PS, double* F,
+                          double* R, void* stream) {
+  if (n < 1 || n > JAC_NMAX) return fail(DFT_EUNSUPPORTED, "dft_scf_fock_residual: order must be in [1, 128]");
+  if (!H || !J || !K || !PS || !F || !R) return fail(DFT_EINVAL, "NULL argument");
+  hipError_t e = launch_scf_fock_residual(n, H, J, K, PS, F, R, (hipStream_t)stream);
+  if (e != hipSuccess) return fail(DFT_EHIP, std::string("scf fock/residual: ") + hipGetErrorString(e));
+  return DFT_OK;
+}
+
+int dft_scf_orbital_matrix(int n, const double* XQ, const double* T, double* A, const double* Q, double* V,
+                           void* stream) {
+  if (n < 1 || n > JAC_NMAX) return fail(DFT_EUNSUPPORTED, "dft_scf_orbital_matrix: order must be in [1, 128]");
+  if (!XQ || !T || !A || !Q || !V) return fail(DFT_EINVAL, "NULL argument");
+  hipError_t e = launch_scf_orbital_matrix(n, XQ, T, A, Q, V, (hipStream_t)stream);
+  if (e != hipSuccess) return fail(DFT_EHIP, std::string("scf orbital matrix: ") + hipGetErrorString(e));
+  return DFT_OK;
+}
+
 int dft_boys(const double* T, int n, int mmax, double* out, int fp32, void* stream) {
   if (n < 0 || (n > 0 && (!T || !out))) return fail(DFT_EINVAL, "bad arguments");
   if (mmax < 0 || mmax > 16) return fail(DFT_EUNSUPPORTED, "mmax must be in [0, 16]");

@@ -241,6 +241,13 @@ hipError_t launch_scf_diis_stage(int n, int h, const double* F, const double* R,
                                  double* G, const double* P, const double* H, const double* Pa, double* st,
                                  double* Fu, hipStream_t s);
 hipError_t launch_scf_damp_control(int n, double* Pa, const double* Pauf, double* P, double* st, hipStream_t s);
+// dense products of the iteration (orders <= JAC_NMAX, 16 x 16 tiles)
+hipError_t launch_scf_gemm(int m, int nc, int kd, const double* A, int lda, int ta, const double* B, int ldb, int tb,
+                           double* C, int ldc, const double* cp_src, double* cp_dst, hipStream_t s);
+hipError_t launch_scf_fock_residual(int n, const double* H, const double* J, const double* K, const double* PS,
+                                    double* F, double* R, hipStream_t s);
+hipError_t launch_scf_orbital_matrix(int n, const double* XQ, const double* T, double* A, const double* Q, double* V,
+                                     hipStream_t s);
 
 // F_m(T) test hook
 hipError_t launch_boys(const double* T, int n, int mmax, double* out, int fp32,

@@ -771,4 +771,156 @@ hipError_t launch_scf_damp_control(int n, double* Pa, const double* Pauf, double
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Dense products of the SCF iteration (orders up to JAC_NMAX): one 256-thread
+// workgroup per 16 x 16 output tile ((n / 16)^2 workgroups: 49 at n = 112, a
+// library GEMM's 128 x 64 tiles would use 2), the operand panels staged in
+// LDS as [k][16] columns, each output one thread's FMA chain over k in order.
+
+constexpr int SG_T = 16;
+constexpr int SG_THREADS = SG_T * SG_T;
+typedef double SgPanel[JAC_NMAX][SG_T + 1];
+
+// P[k][c] = M(k, idx0 + c) (col = true: column idx0 + c of row-major M) or
+// M(idx0 + c, k) (col = false: row idx0 + c), k < kd; entries past nlim
+// read as 0. Consecutive threads walk the contiguous index.
+__device__ __forceinline__ void sg_panel(SgPanel& P, const double* __restrict__ M, int ld, bool col, int idx0,
+                                         int kd, int nlim) {
+  for (int e = threadIdx.x; e < kd * SG_T; e += SG_THREADS) {
+    int k, c;
+    if (col) {
+      k = e / SG_T;
+      c = e - k * SG_T;
+    } else {
+      c = e / kd;
+      k = e - c * kd;
+    }
+    const int x = idx0 + c;
+    P[k][c] = x < nlim ? (col ? M[(size_t)k * ld + x] : M[(size_t)x * ld + k]) : 0.0;
+  }
+}
+
+// The same with M = (H + J) - K formed on the fly (the Fock matrix, added
+// in that order).
+__device__ __forceinline__ void sg_panel_fock(SgPanel& P, const double* __restrict__ H, const double* __restrict__ J,
+                                              const double* __restrict__ K, int n, bool col, int idx0) {
+  for (int e = threadIdx.x; e < n * SG_T; e += SG_THREADS) {
+    int k, c;
+    if (col) {
+      k = e / SG_T;
+      c = e - k * SG_T;
+    } else {
+      c = e / n;
+      k = e - c * n;
+    }
+    const int x = idx0 + c;
+    double v = 0.0;
+    if (x < n) {
+      const size_t a = col ? (size_t)k * n + x : (size_t)x * n + k;
+      v = (H[a] + J[a]) - K[a];
+    }
+    P[k][c] = v;
+  }
+}
+
+// sum_k A[k][r] B[k][c] in k order
+__device__ __forceinline__ double sg_dot(const SgPanel& A, const SgPanel& B, int r, int c, int kd) {
+  double s = 0.0;
+  for (int k = 0; k < kd; ++k) s = fma(A[k][r], B[k][c], s);
+  return s;
+}
+
+// C (m x nc, ldc) = op(A) op(B) over kd; op(A) = A (m x kd, lda) or A^T
+// (A kd x m); op(B) = B (kd x nc, ldb) or B^T (B nc x kd). With cp_src, the
+// workgroup also copies its tile of cp_src to cp_dst (same shape and ldc).
+__global__ __launch_bounds__(SG_THREADS) void scf_gemm_kernel(int m, int nc, int kd, const double* __restrict__ A,
+                                                              int lda, int ta, const double* __restrict__ B,
+                                                              int ldb, int tb, double* __restrict__ C, int ldc,
+                                                              const double* __restrict__ cp_src,
+                                                              double* __restrict__ cp_dst) {
+  __shared__ SgPanel pa, pb;
+  const int i0 = blockIdx.y * SG_T, j0 = blockIdx.x * SG_T;
+  sg_panel(pa, A, lda, ta != 0, i0, kd, m);  // pa[k][r] = op(A)(i0 + r, k)
+  sg_panel(pb, B, ldb, tb == 0, j0, kd, nc);  // pb[k][c] = op(B)(k, j0 + c)
+  __syncthreads();
+  const int r = threadIdx.x / SG_T, c = threadIdx.x - r * SG_T;
+  const int i = i0 + r, j = j0 + c;
+  if (i < m && j < nc) {
+    C[(size_t)i * ldc + j] = sg_dot(pa, pb, r, c, kd);
+    if (cp_src) cp_dst[(size_t)i * ldc + j] = cp_src[(size_t)i * ldc + j];
+  }
+}
+
+// F = (H + J) - K and R = F PS - PS^T F (the DIIS commutator residual with
+// PS = P_a S), one tile of each per workgroup.
+__global__ __launch_bounds__(SG_THREADS) void scf_fock_residual_kernel(int n, const double* __restrict__ H,
+                                                                       const double* __restrict__ J,
+                                                                       const double* __restrict__ K,
+                                                                       const double* __restrict__ PS,
+                                                                       double* __restrict__ F,
+                                                                       double* __restrict__ R) {
+  __shared__ SgPanel fr, psj, psi, fc;
+  const int i0 = blockIdx.y * SG_T, j0 = blockIdx.x * SG_T;
+  sg_panel_fock(fr, H, J, K, n, false, i0);  // fr[k][r] = F(i0 + r, k)
+  sg_panel(psj, PS, n, true, j0, n, n);      // psj[k][c] = PS(k, j0 + c)
+  sg_panel(psi, PS, n, true, i0, n, n);      // psi[k][r] = PS(k, i0 + r) = PS^T(i0 + r, k)
+  sg_panel_fock(fc, H, J, K, n, true, j0);   // fc[k][c] = F(k, j0 + c)
+  __syncthreads();
+  const int r = threadIdx.x / SG_T, c = threadIdx.x - r * SG_T;
+  const int i = i0 + r, j = j0 + c;
+  if (i < n && j < n) {
+    const double a = sg_dot(fr, psj, r, c, n), b = sg_dot(psi, fc, r, c, n);
+    R[(size_t)i * n + j] = a - b;
+    F[(size_t)i * n + j] = fc[i][c];
+  }
+}
+
+// A = (At + At^T) / 2 with At = XQ^T T (T = F XQ: the Fock matrix in the
+// previous orbitals' basis, symmetrised), and V = Q (the eigensolver's warm
+// start), one tile of each per workgroup.
+__global__ __launch_bounds__(SG_THREADS) void scf_orbital_matrix_kernel(int n, const double* __restrict__ XQ,
+                                                                        const double* __restrict__ T,
+                                                                        double* __restrict__ A,
+                                                                        const double* __restrict__ Q,
+                                                                        double* __restrict__ V) {
+  __shared__ SgPanel xi, tj, xj, ti;
+  const int i0 = blockIdx.y * SG_T, j0 = blockIdx.x * SG_T;
+  sg_panel(xi, XQ, n, true, i0, n, n);  // XQ(k, i0 + r)
+  sg_panel(tj, T, n, true, j0, n, n);   // T(k, j0 + c)
+  sg_panel(xj, XQ, n, true, j0, n, n);  // XQ(k, j0 + c)
+  sg_panel(ti, T, n, true, i0, n, n);   // T(k, i0 + r)
+  __syncthreads();
+  const int r = threadIdx.x / SG_T, c = threadIdx.x - r * SG_T;
+  const int i = i0 + r, j = j0 + c;
+  if (i < n && j < n) {
+    const double aij = sg_dot(xi, tj, r, c, n), aji = sg_dot(ti, xj, r, c, n);
+    A[(size_t)i * n + j] = (aij + aji) * 0.5;
+    V[(size_t)i * n + j] = Q[(size_t)i * n + j];
+  }
+}
+
+static dim3 sg_grid(int m, int nc) { return dim3((nc + SG_T - 1) / SG_T, (m + SG_T - 1) / SG_T); }
+
+hipError_t launch_scf_gemm(int m, int nc, int kd, const double* A, int lda, int ta, const double* B, int ldb, int tb,
+                           double* C, int ldc, const double* cp_src, double* cp_dst, hipStream_t s) {
+  if (m < 1 || nc < 1 || kd < 1 || m > JAC_NMAX || nc > JAC_NMAX || kd > JAC_NMAX) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(scf_gemm_kernel, sg_grid(m, nc), dim3(SG_THREADS), 0, s, m, nc, kd, A, lda, ta, B, ldb, tb, C,
+                     ldc, cp_src, cp_dst);
+  return hipGetLastError();
+}
+
+hipError_t launch_scf_fock_residual(int n, const double* H, const double* J, const double* K, const double* PS,
+                                    double* F, double* R, hipStream_t s) {
+  if (n < 1 || n > JAC_NMAX) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(scf_fock_residual_kernel, sg_grid(n, n), dim3(SG_THREADS), 0, s, n, H, J, K, PS, F, R);
+  return hipGetLastError();
+}
+
+hipError_t launch_scf_orbital_matrix(int n, const double* XQ, const double* T, double* A, const double* Q, double* V,
+                                     hipStream_t s) {
+  if (n < 1 || n > JAC_NMAX) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(scf_orbital_matrix_kernel, sg_grid(n, n), dim3(SG_THREADS), 0, s, n, XQ, T, A, Q, V);
+  return hipGetLastError();
+}
+
 }  // namespace dft

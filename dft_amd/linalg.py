@@ -15,6 +15,7 @@ main.py:754,796-799).
 from __future__ import annotations
 
 import ctypes as C
+import ctypes as _ctypes  # for functions whose output matrix is named C
 
 import torch
 
@@ -87,3 +88,60 @@ def eigh(F: torch.Tensor, V0: torch.Tensor | None = None, tol: float = 0.0):
     w = torch.empty(A.shape[:2], dtype=A.dtype, device=A.device)
     syevj_(A, w, V, init, tol=tol)
     return (w, V) if batched else (w[0], V[0])
+
+
+def _check_f64(name, t, rows, cols):
+    if not (t.is_cuda and t.dtype == torch.float64 and t.dim() == 2 and t.shape[0] >= rows and t.shape[1] >= cols
+            and t.stride(1) == 1):
+        raise ValueError(f"{name} must be a float64 device matrix of at least {rows} x {cols} with unit column stride")
+
+
+def scf_gemm_(A: torch.Tensor, B: torch.Tensor, C: torch.Tensor, trans_a: bool = False, trans_b: bool = False,
+              k: int | None = None, copy=None, stream=None) -> None:
+    """C = op(A) op(B) in place for orders up to 128 (dft_scf_gemm: one 16 x
+    16 tile per workgroup); op(A) = A or A^T, op(B) = B or B^T, the inner
+    dimension k defaults to all of it. ``copy`` = (src, dst): also dst = src
+    (C's shape) in the same launch."""
+    lib = _lib.load()
+    m, nc = C.shape
+    kd = k if k is not None else (A.shape[0] if trans_a else A.shape[1])
+    _check_f64("C", C, m, nc)
+    _check_f64("A", A, kd if trans_a else m, m if trans_a else kd)
+    _check_f64("B", B, nc if trans_b else kd, kd if trans_b else nc)
+    src = dst = None
+    if copy is not None:
+        src, dst = copy
+        _check_f64("copy source", src, m, nc)
+        _check_f64("copy target", dst, m, nc)
+        if src.stride(0) != C.stride(0) or dst.stride(0) != C.stride(0):
+            raise ValueError("copy source and target must have C's row stride")
+        src, dst = src.data_ptr(), dst.data_ptr()
+    s = torch.cuda.current_stream(C.device) if stream is None else stream
+    _lib.check(lib.dft_scf_gemm(m, nc, kd, A.data_ptr(), A.stride(0), int(trans_a), B.data_ptr(), B.stride(0),
+                                int(trans_b), C.data_ptr(), C.stride(0), src, dst, _ctypes.c_void_p(s.cuda_stream)))
+
+
+def scf_fock_residual_(H, J, K, PS, F, R, stream=None) -> None:
+    """F = (H + J) - K and R = F PS - PS^T F (dft_scf_fock_residual), n x n
+    contiguous, n <= 128."""
+    lib = _lib.load()
+    n = H.shape[0]
+    for name, t in (("H", H), ("J", J), ("K", K), ("PS", PS), ("F", F), ("R", R)):
+        if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous() and t.shape == (n, n)):
+            raise ValueError(f"{name} must be a contiguous float64 device {n} x {n} matrix")
+    s = torch.cuda.current_stream(H.device) if stream is None else stream
+    _lib.check(lib.dft_scf_fock_residual(n, H.data_ptr(), J.data_ptr(), K.data_ptr(), PS.data_ptr(), F.data_ptr(),
+                                         R.data_ptr(), _ctypes.c_void_p(s.cuda_stream)))
+
+
+def scf_orbital_matrix_(XQ, T, A, Q, V, stream=None) -> None:
+    """A = (XQ^T T + (XQ^T T)^T) / 2 and V = Q (dft_scf_orbital_matrix), n x n
+    contiguous, n <= 128."""
+    lib = _lib.load()
+    n = XQ.shape[0]
+    for name, t in (("XQ", XQ), ("T", T), ("A", A), ("Q", Q), ("V", V)):
+        if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous() and t.shape == (n, n)):
+            raise ValueError(f"{name} must be a contiguous float64 device {n} x {n} matrix")
+    s = torch.cuda.current_stream(XQ.device) if stream is None else stream
+    _lib.check(lib.dft_scf_orbital_matrix(n, XQ.data_ptr(), T.data_ptr(), A.data_ptr(), Q.data_ptr(), V.data_ptr(),
+                                          _ctypes.c_void_p(s.cuda_stream)))

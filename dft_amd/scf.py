@@ -382,7 +382,7 @@ def scf_diis_graph(plan: Plan, Z, centers, N_occ, mode="fp64", rate=0.4, diis_th
     eig_tol = 1e-9: 193 iterations instead of 55)."""
     import ctypes as C
     from . import _lib
-    from .linalg import syevj_
+    from .linalg import scf_fock_residual_, scf_gemm_, scf_orbital_matrix_, syevj_
     if plan.uhf:
         raise ValueError("scf_diis_graph is closed-shell: build the plan with uhf=False")
     n = plan.nbf
@@ -405,7 +405,7 @@ def scf_diis_graph(plan: Plan, Z, centers, N_occ, mode="fp64", rate=0.4, diis_th
         Pa = (C0[:, :N_occ] @ C0[:, :N_occ].T).contiguous()
     P = (2 * Pa).contiguous()
     Q = Q.contiguous()
-    J, K, F, R, Fu = (torch.empty((n, n), **f64) for _ in range(5))
+    J, K, F, R, Fu, PS, XQ, T, Cm, Pauf = (torch.empty((n, n), **f64) for _ in range(10))
     histF = torch.zeros((history, n, n), **f64)
     histR = torch.zeros((history, n, n), **f64)
     G = torch.zeros((history, history), **f64)
@@ -420,23 +420,21 @@ def scf_diis_graph(plan: Plan, Z, centers, N_occ, mode="fp64", rate=0.4, diis_th
     def body():
         s = C.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
         plan.jk_into(P, Pa, None, J, K, None, mode)
-        torch.add(H, J, out=F)
-        F.sub_(K)
-        PS = Pa @ S
-        torch.sub(F @ PS, PS.T @ F, out=R)
+        # F = H + J - K, R = F Pa S - S Pa F (16 x 16-tile products)
+        scf_gemm_(Pa, S, PS)
+        scf_fock_residual_(H, J, K, PS, F, R)
         _lib.check(lib.dft_scf_diis_stage(n, history, F.data_ptr(), R.data_ptr(), histF.data_ptr(),
                                           histR.data_ptr(), G.data_ptr(), P.data_ptr(), H.data_ptr(),
                                           Pa.data_ptr(), st.data_ptr(), Fu.data_ptr(), s))
-        XQ = X @ Q
-        At = XQ.T @ Fu @ XQ
-        torch.add(At, At.T, out=A[0]).mul_(0.5)
-        V[0].copy_(Q)
+        # A = (X Q)^T Fu (X Q), symmetrised; V = Q
+        scf_gemm_(X, Q, XQ)
+        scf_gemm_(Fu, XQ, T)
+        scf_orbital_matrix_(XQ, T, A[0], Q, V[0])
         syevj_(A, w, V, False, tol=eig_tol)
-        Q.copy_(V[0])
-        Cm = X @ Q
-        Pauf = Cm[:, :N_occ] @ Cm[:, :N_occ].T
-        _lib.check(lib.dft_scf_damp_control(n, Pa.data_ptr(), Pauf.contiguous().data_ptr(), P.data_ptr(),
-                                            st.data_ptr(), s))
+        # orbitals X V (and Q = V), aufbau density C_occ C_occ^T
+        scf_gemm_(X, V[0], Cm, copy=(V[0], Q))
+        scf_gemm_(Cm, Cm, Pauf, trans_b=True, k=N_occ)
+        _lib.check(lib.dft_scf_damp_control(n, Pa.data_ptr(), Pauf.data_ptr(), P.data_ptr(), st.data_ptr(), s))
 
     # warm-up of the GEMM paths on other buffers, outside the capture
     side = torch.cuda.Stream(dev)

@@ -443,6 +443,35 @@ int dft_scf_diis_stage(int n, int history, const double* F, const double* R, dou
 int dft_scf_damp_control(int n, double* Pa, const double* Pauf, double* P, double* state, void* stream);
 
 /*
+ * The dense products around those two calls, for orders up to 128 (one
+ * 16 x 16 output tile per workgroup, each output one FMA chain over k in
+ * order; DFT_EUNSUPPORTED above 128). Device pointers, row-major; they only
+ * enqueue work, so a whole iteration can be captured.
+ *
+ * dft_scf_gemm: C (m x ncol, ldc) = op(A) op(B) over k, with op(A) = A
+ * (m x k, lda) or A^T (trans_a: A is k x m), op(B) = B (k x ncol, ldb) or
+ * B^T (trans_b: B is ncol x k). With copy_src (and copy_dst) non-NULL, also
+ * copies the m x ncol block copy_src to copy_dst (leading dimension ldc).
+ * Replaces the matrix products of the SCF step (P_a S, X Q, F X Q, the
+ * orbitals X C and the aufbau density C_occ C_occ^T; shelled.py:1191-1193,
+ * 1210-1217).
+ *
+ * dft_scf_fock_residual: F = (H + J) - K and the DIIS residual
+ * R = F PS - PS^T F for PS = P_a S (shelled.py:1179-1180, 1191-1193), each
+ * n x n.
+ *
+ * dft_scf_orbital_matrix: A = (At + At^T) / 2 for At = XQ^T T (T = F XQ:
+ * the Fock matrix in the previous orbitals' basis, shelled.py:1210-1211) and
+ * V = Q (the eigensolver's warm start), each n x n.
+ */
+int dft_scf_gemm(int m, int ncol, int k, const double* A, int lda, int trans_a, const double* B, int ldb,
+                 int trans_b, double* C, int ldc, const double* copy_src, double* copy_dst, void* stream);
+int dft_scf_fock_residual(int n, const double* H, const double* J, const double* K, const double* PS, double* F,
+                          double* R, void* stream);
+int dft_scf_orbital_matrix(int n, const double* XQ, const double* T, double* A, const double* Q, double* V,
+                           void* stream);
+
+/*
  * Boys function test hook: out[i*(mmax+1)+m] = F_m(T[i]), m = 0..mmax, with
  * the tabulated evaluator the ERI kernels use. T and out are device arrays.
  * fp32 selects the FP32 recursion (FP64 seed). Replaces boys/taylor

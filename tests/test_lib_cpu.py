@@ -67,6 +67,23 @@ def test_eigensolver_rejects_orders_above_its_limit():
         assert b"tol" in lib.dft_last_error()
 
 
+def test_scf_dense_products_reject_bad_shapes():
+    """dft_scf_gemm / _fock_residual / _orbital_matrix check orders (1..128)
+    and leading dimensions before touching the device (placeholder
+    pointers: nothing is read)."""
+    from dft_amd import _lib
+    lib = _lib.load()
+    f = ctypes.c_void_p(0x1000)
+    assert lib.dft_scf_gemm(129, 8, 8, f, 8, 0, f, 8, 0, f, 8, None, None, None) == _lib.DFT_EUNSUPPORTED
+    assert lib.dft_scf_gemm(8, 8, 0, f, 8, 0, f, 8, 0, f, 8, None, None, None) == _lib.DFT_EUNSUPPORTED
+    assert lib.dft_scf_gemm(8, 8, 8, f, 7, 0, f, 8, 0, f, 8, None, None, None) == _lib.DFT_EINVAL
+    assert b"leading dimension" in lib.dft_last_error()
+    assert lib.dft_scf_gemm(8, 8, 8, f, 8, 0, f, 8, 0, f, 8, f, None, None) == _lib.DFT_EINVAL
+    assert lib.dft_scf_fock_residual(0, f, f, f, f, f, f, None) == _lib.DFT_EUNSUPPORTED
+    assert lib.dft_scf_orbital_matrix(129, f, f, f, f, f, None) == _lib.DFT_EUNSUPPORTED
+    assert lib.dft_scf_orbital_matrix(8, f, None, f, f, f, None) == _lib.DFT_EINVAL
+
+
 def test_eigensolver_workspace_size_and_check():
     """dft_syevj_ws_bytes: a 1 KiB header and 40 sweeps of (m - 1) rounds of
     m / 2 rotations of 16 bytes (m = n rounded up to even); dft_syevj_ws
