@@ -311,8 +311,26 @@ def alkane_spd_class_roofline(steps=3):
             "per_class": rows}
 
 
+def water16_scf_graph(eig_tol=0.0):
+    """(H2O)16 / STO-3G closed-shell damped + DIIS SCF with one captured
+    hipGraph replay per iteration (scf_diis_graph: Fock build, tile products,
+    DIIS stage, Jacobi eigensolver with its rotation log, damping/control),
+    from the core guess; the same loop with eig_tol = 1e-9 beside it."""
+    from .geometry import water_cluster
+    from .scf import scf_diis_graph
+    Z, xyz = water_cluster((4, 2, 2), 3.1, seed=0)
+    plan = Plan(build_shells(load_basis(), Z, xyz), Z, xyz, schwarz_tau=1e-10)
+    nocc = int(sum(Z)) // 2
+    out = {}
+    for tol, key in ((eig_tol, "exact"), (1e-9, "eig_tol_1e-9")):
+        r = scf_diis_graph(plan, Z, xyz, nocc, eig_tol=tol)
+        out[key] = {"energy_Eh": r.energy, "converged": r.converged, "iterations": r.iterations,
+                    "ms_per_iteration_median": float(np.median(r.iter_times[1:]) * 1e3)}
+    return out
+
+
 WORKLOADS = ("water_rhf", "alkane_spd_mixed", "water_batch_scf", "alkane_gradient", "water16_grid",
-             "alkane_spd_class_roofline")
+             "alkane_spd_class_roofline", "water16_scf_graph")
 
 
 def main(argv=None):
