@@ -560,6 +560,23 @@ __global__ __launch_bounds__(SCF_EW_THREADS) void scf_diis_dots_kernel(int n, in
   }
 }
 
+// sum_b p[b * SCF_PART_STRIDE] over b < nblk, in b order, with the loads
+// issued 16 at a time ahead of the adds (a loop that adds each term as it
+// arrives waits one memory latency per term: 64 terms, about 20 us)
+__device__ __forceinline__ double scf_part_sum(const double* __restrict__ p, int nblk) {
+  constexpr int B = 16;
+  double s = 0.0;
+  for (int b0 = 0; b0 < nblk; b0 += B) {
+    double v[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) v[u] = b0 + u < nblk ? p[(size_t)(b0 + u) * SCF_PART_STRIDE] : 0.0;
+#pragma unroll
+    for (int u = 0; u < B; ++u)
+      if (b0 + u < nblk) s += v[u];
+  }
+  return s;
+}
+
 __global__ __launch_bounds__(64) void scf_diis_solve_kernel(int h, int nblk, double* __restrict__ G,
                                                             double* __restrict__ st) {
   const int tid = threadIdx.x;
@@ -567,8 +584,7 @@ __global__ __launch_bounds__(64) void scf_diis_solve_kernel(int h, int nblk, dou
   const int slot = count % h;
   const int used = min(count + 1, h);
   if (tid < h) {
-    double x = 0.0;
-    for (int b = 0; b < nblk; ++b) x += st[SCF_ST_PART + b * SCF_PART_STRIDE + tid];
+    const double x = scf_part_sum(st + SCF_ST_PART + tid, nblk);
     G[slot * h + tid] = x;
     G[tid * h + slot] = x;
   }
@@ -693,9 +709,7 @@ __global__ __launch_bounds__(SCF_EW_THREADS) void scf_diis_extrap_kernel(int n, 
 
 __global__ void scf_energy_sum_kernel(int nblk, double* __restrict__ st) {
   if (threadIdx.x != 0) return;
-  double e = 0.0;
-  for (int b = 0; b < nblk; ++b) e += st[SCF_ST_PART + b * SCF_PART_STRIDE + SCF_HMAX];
-  st[SCF_ST_ESUM] = e;
+  st[SCF_ST_ESUM] = scf_part_sum(st + SCF_ST_PART + SCF_HMAX, nblk);
 }
 
 // Energy and convergence of the iteration, the damped density
