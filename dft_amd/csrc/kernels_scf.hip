@@ -537,7 +537,6 @@ __global__ __launch_bounds__(SCF_EW_THREADS) void scf_diis_dots_kernel(int n, in
                                                                        double* __restrict__ histF,
                                                                        double* __restrict__ histR,
                                                                        double* __restrict__ st) {
-  __shared__ double red[SCF_EW_THREADS / 64];
   const long long nn = (long long)n * n;
   const int slot = (int)st[SCF_ST_COUNT] % h;
   long long i0, i1;
@@ -553,10 +552,23 @@ __global__ __launch_bounds__(SCF_EW_THREADS) void scf_diis_dots_kernel(int n, in
     for (int t = 0; t < SCF_HMAX; ++t)
       if (t < h) d[t] = fma(r, t == slot ? r : histR[t * nn + i], d[t]);
   }
+  // the block's h dot products together: each wave's butterflies, one
+  // barrier, then the waves in order (the additions of ew_block_sum, which
+  // would take two barriers per product)
+  __shared__ double redv[SCF_EW_THREADS / 64][SCF_HMAX];
+#pragma unroll
+  for (int t = 0; t < SCF_HMAX; ++t)
+    if (t < h) {
+      double v = d[t];
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+      if ((threadIdx.x & 63) == 0) redv[threadIdx.x >> 6][t] = v;
+    }
+  __syncthreads();
   double* part = st + SCF_ST_PART + blockIdx.x * SCF_PART_STRIDE;
-  for (int t = 0; t < h; ++t) {
-    const double x = ew_block_sum(d[t], red);
-    if (threadIdx.x == 0) part[t] = x;
+  if ((int)threadIdx.x < h) {
+    double x = 0.0;
+    for (int k = 0; k < SCF_EW_THREADS / 64; ++k) x += redv[k][threadIdx.x];
+    part[threadIdx.x] = x;
   }
 }
 
