@@ -735,6 +735,10 @@ __global__ __launch_bounds__(SCF_THREADS) void scf_damp_control_kernel(int n, do
                                                                        double* __restrict__ st) {
   __shared__ double red[SCF_THREADS / 64];
   __shared__ int conv;
+  // an iteration after the converged one (a host loop that queues the next
+  // iteration before it reads this one's state) leaves the loop state and
+  // the densities as they are
+  if (st[SCF_ST_CONV] != 0.0) return;
   const int tid = threadIdx.x, nt = blockDim.x;
   const long long nn = (long long)n * n;
   const double rate = st[SCF_ST_RATE];

@@ -450,13 +450,29 @@ def scf_diis_graph(plan: Plan, Z, centers, N_occ, mode="fp64", rate=0.4, diis_th
         body()
     energies, iter_times = [], []
     converged = False
+    # replay k + 1 is queued before iteration k's state is read (pinned
+    # double buffer), so the device does not wait for the host between
+    # iterations; a replay after convergence changes neither the densities
+    # nor the loop state (dft_scf_damp_control)
+    pin = [torch.empty(st.shape, dtype=st.dtype).pin_memory() for _ in range(2)]
+    ev = [torch.cuda.Event(), torch.cuda.Event()]
+    graph.replay()
+    pin[0].copy_(st, non_blocking=True)
+    ev[0].record()
+    k = 0
+    t0 = time.perf_counter()
     while True:
-        t0 = time.perf_counter()
         graph.replay()
-        v = st.cpu()
-        iter_times.append(time.perf_counter() - t0)
+        pin[(k + 1) % 2].copy_(st, non_blocking=True)
+        ev[(k + 1) % 2].record()
+        ev[k % 2].synchronize()
+        v = pin[k % 2].numpy().copy()
+        t1 = time.perf_counter()
+        iter_times.append(t1 - t0)
+        t0 = t1
         E = float(v[ST_ENEW])
         if not np.isfinite(E):
+            torch.cuda.current_stream(dev).synchronize()
             raise FloatingPointError(f"non-finite energy at iteration {int(v[ST_COUNT])}")
         energies.append(E)
         if v[ST_CONV] != 0:
@@ -464,6 +480,15 @@ def scf_diis_graph(plan: Plan, Z, centers, N_occ, mode="fp64", rate=0.4, diis_th
             break
         if int(v[ST_COUNT]) > max_count:
             break
+        k += 1
+    torch.cuda.current_stream(dev).synchronize()
+    if not converged:
+        # the iteration queued after the last one read ran in full: report it
+        w = pin[(k + 1) % 2].numpy().copy()
+        if int(w[ST_COUNT]) > int(v[ST_COUNT]):
+            v = w
+            energies.append(float(v[ST_ENEW]))
+            converged = bool(v[ST_CONV] != 0)
     count = int(v[ST_COUNT])
     # exit rebuild from the last density (shelled.py:1261-1285)
     fock = _Fock(plan, mode, True)

@@ -117,3 +117,21 @@ def test_scf_dense_kernels_match_torch(torch_cuda):
         At = XQ.T @ T
         assert close(Am, 0.5 * (At + At.T)), n
         assert torch.equal(Am, Am.T) and torch.equal(V, Q)
+
+
+def test_graph_scf_stopped_early_reports_the_iterations_it_ran(torch_cuda):
+    """The loop queues the next replay before it reads the current state, so
+    a run stopped by max_count has run one replay past the last one read: it
+    reports that iteration too (energies, count), and its energies are the
+    first ones of the converged run."""
+    from dft_amd.scf import scf_diis_graph
+    g = golden("water_sto3g.npz")
+    Z, xyz = list(g["Z"]), g["xyz"]
+    plan = _plan(Z, xyz)
+    full = scf_diis_graph(plan, Z, xyz, 5)
+    part = scf_diis_graph(plan, Z, xyz, 5, max_count=4)
+    assert full.converged and not part.converged
+    assert part.iterations == len(part.energies), (part.iterations, len(part.energies))
+    assert 5 <= len(part.energies) < len(full.energies)
+    k = len(part.energies)
+    assert np.abs(np.asarray(part.energies) - np.asarray(full.energies[:k])).max() < 1e-12
