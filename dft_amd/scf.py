@@ -362,11 +362,12 @@ def scf_diis_graph(plan: Plan, Z, centers, N_occ, mode="fp64", rate=0.4, diis_th
     hipGraph: the direct J/K build (dft_jk, its class launches on four
     streams), F = H + J - K, the residual F P S - S P F (GEMMs), the DIIS
     stage (dft_scf_diis_stage: history ring, Gram row, DIIS solve,
-    extrapolation, energy sum), X^T F X in the previous orbitals' basis,
-    the Jacobi eigensolver (dft_syevj, warm start), the aufbau density
-    (GEMMs) and damping / strikes / DIIS switch / convergence
-    (dft_scf_damp_control). The loop state stays on the device; the host
-    reads 32 doubles after each replay. The exit rebuild of
+    extrapolation, energy sum), X^T F X in the previous orbitals' basis
+    (dft_scf_gemm, dft_scf_fock_residual, dft_scf_orbital_matrix), the
+    Jacobi eigensolver (dft_syevj_ws, warm start), the aufbau density and
+    damping / strikes / DIIS switch / convergence (dft_scf_damp_control).
+    The loop state stays on the device; the host reads it from a pinned
+    copy while the next replay runs. The exit rebuild of
     shelled.py:1261-1285 runs eagerly after the loop. Orders up to 128
     (the eigensolver's limit); no checkpoint files (scf_diis writes them).
     ``guess`` = P_a (or (P_a, P_b) with P_a == P_b) starts from that density
